@@ -1,0 +1,232 @@
+#!/usr/bin/env python3
+"""Headline benchmark: the FP64 symmetry-sector matvec of the N = 32 J1-J2 ring (J2 = 1/2, periodic,
+Sz = 0, k = 0, reflection-even; dim 9,398,520, real) on one MI355X, plus the time to E0 of its
+300-step fully reorthogonalised Lanczos.
+
+A step is one apply of the sector operator (the merged reduced CSR built on the device) to one
+vector already resident in HBM. value = algorithmic bytes per apply / apply time, with
+    B(1) = D*8 + nnz*(4 + w_id) + (nnz*8 + D*8)
+(row descriptor, column + value id, gathered x and written y; w_id the reference's id width).
+The apply time is measured with HIP events on the stream the SpMV runs on, over exactly --steps
+applies after --warmup untimed ones.
+
+    python bench.py [--gpus 1] [--steps 200] [--warmup 20] [--no-lanczos] [--no-cpu]
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+N = 32
+J2 = 0.5
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(m, D, w_id, target_s=12.0):
+    """The oracle's real-block SpMV (the reference's RealCsrView::spmv restated, one thread) over a
+    bounded sample: the first rows of this same sector's CSR (downloaded), applied to a full-length
+    vector, repeated until about target_s seconds. Reported in the metric's unit over the sample's
+    algorithmic bytes."""
+    import numpy as np
+
+    from oracle import pyoracle as po
+    from qed_spin_amd import _lib
+
+    rows = min(D, 1_000_000)
+    rp = np.zeros(rows + 1, np.uint64)
+    _lib.call("qedh_matvec_export_csr_rows", m.h, 0, rows, _lib.ptr(rp, C.c_uint64), None, None)
+    nnz = int(rp[-1] - rp[0])
+    col = np.zeros(nnz, np.uint32)
+    val = np.zeros(2 * nnz, np.float64)
+    _lib.call("qedh_matvec_export_csr_rows", m.h, 0, rows, _lib.ptr(rp, C.c_uint64), _lib.ptr(col, C.c_uint32),
+              _lib.ptr(val, C.c_double))
+    re = np.ascontiguousarray(val[0::2])
+    rp0 = np.ascontiguousarray(rp - rp[0])
+    x = np.random.default_rng(3).normal(size=D)
+    y = np.zeros(rows)
+    L = po.lib()
+    L.orc_csr_matvec_real.restype = None
+    L.orc_csr_matvec_real.argtypes = [C.c_int64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32),
+                                      C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    args = (rows, rp0.ctypes.data_as(C.POINTER(C.c_uint64)), col.ctypes.data_as(C.POINTER(C.c_uint32)),
+            re.ctypes.data_as(C.POINTER(C.c_double)), x.ctypes.data_as(C.POINTER(C.c_double)),
+            y.ctypes.data_as(C.POINTER(C.c_double)))
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        L.orc_csr_matvec_real(*args)
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= target_s:
+            break
+    per = el / reps
+    B = rows * 8 + nnz * (4 + w_id) + (nnz * 8 + rows * 8)
+    return {
+        "value": round(B / per / 1e9, 3),
+        "unit": "GB/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"rows 0..{rows} of the same sector CSR ({nnz} entries), {reps} applies, {per * 1e3:.1f} ms each",
+        "cpu": _cpu_model(),
+    }
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--lanczos-steps", type=int, default=300)
+    ap.add_argument("--no-lanczos", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--lanes", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU: every box here has one; run --gpus 1"}))
+        sys.exit(2)
+
+    import numpy as np
+
+    import qed_spin_amd as qs
+    from qed_spin_amd import _lib
+    from qed_spin_amd import symmetry as sym
+
+    if _lib.device_count() < 1:
+        raise SystemExit("no HIP device visible")
+
+    b = qs.HamiltonianBuilder(N)
+    b.heisenberg([(i, (i + 1) % N) for i in range(N)], 1.0)
+    b.heisenberg([(i, (i + 2) % N) for i in range(N)], J2)
+    op = b.to_operator()
+
+    t0 = time.perf_counter()
+    sector = qs.Sector(sym.dihedral_group(N, 0, +1), N // 2)
+    t_sector = time.perf_counter() - t0
+    log(f"sector dim={sector.dim} orbits={sector.n_orbits} sublattice_L={sector.sublattice_block} "
+        f"build {t_sector:.2f}s")
+    t1 = time.perf_counter()
+    m = qs.SectorMatvec(op, sector, lane=qs.LANE_CSR)
+    t_csr = time.perf_counter() - t1
+    if args.lanes:
+        m.set_lanes(args.lanes)
+    info = m.info
+    D, nnz = sector.dim, int(info["nnz"])
+    w_id = int(info["id_width_ref"])
+    s = 8
+    B1 = D * 8 + nnz * (4 + w_id) + (nnz * s + D * s)
+    log(f"csr nnz={nnz} ({nnz / D:.2f}/row) dict={info['dict_size']} bytes/value={info['value_bytes']} "
+        f"w_id(ref)={w_id} lanes={info['lanes']} build {t_csr:.2f}s; B(1)={B1 / 1e9:.3f} GB")
+
+    avg = C.c_double()
+    _lib.call("qedh_matvec_benchmark", m.h, 1, 0, 0, int(args.warmup), int(args.steps), C.byref(avg))
+    t_apply = float(avg.value)
+    gbs = B1 / t_apply / 1e9
+    log(f"apply {t_apply * 1e3:.3f} ms -> {gbs:.1f} GB/s algorithmic ({100 * gbs / HBM_PEAK_GBS:.1f}% of peak)")
+
+    # multi-vector applies (the batched regime), for the record
+    multi = {}
+    for k in (2, 4, 8):
+        _lib.call("qedh_matvec_benchmark", m.h, k, 1, 0, 3, 10, C.byref(avg))
+        Bk = D * 8 + nnz * (4 + w_id) + k * (nnz * s + D * s)
+        multi[str(k)] = {"ms": avg.value * 1e3, "GBps": Bk / avg.value / 1e9}
+        log(f"block k={k}: {avg.value * 1e3:.3f} ms -> {Bk / avg.value / 1e9:.1f} GB/s")
+
+    lanczos = None
+    if not args.no_lanczos:
+        rng = np.random.default_rng(1)
+        v0 = rng.normal(size=D)
+        v0 /= np.linalg.norm(v0)
+        t2 = time.perf_counter()
+        a, bb, tm = m.lanczos(v0, args.lanczos_steps, reorth=1)
+        t_lz = time.perf_counter() - t2
+        from qed_spin_amd.krylov import tridiag_lowest
+        e0 = tridiag_lowest(a, bb)
+        lanczos = {
+            "steps": int(len(a)),
+            "reorth": "FullCGS2",
+            "E0": e0,
+            "E0_exact": -3.0 * N / 8.0,
+            "E0_rel_err": abs(e0 + 3.0 * N / 8.0) / (3.0 * N / 8.0),
+            "seconds": t_lz,
+            "apply_s": tm["apply_s"],
+            "reorth_s": tm["reorth_s"],
+            "vector_s": tm["vector_s"],
+            "time_to_E0_s": t_sector + t_csr + t_lz,
+        }
+        log(f"lanczos {len(a)} steps: E0={e0:.12f} in {t_lz:.2f}s (apply {tm['apply_s']:.2f}s, "
+            f"reorth {tm['reorth_s']:.2f}s); time to E0 {t_sector + t_csr + t_lz:.2f}s")
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(m, D, w_id)
+        log(f"cpu baseline: {cpu}")
+
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "spmv_n32_traffic.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    out = {
+        "metric": "FP64 sector matvec GB/s (% HBM peak)",
+        "value": round(gbs, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": int(args.steps),
+        "warmup": int(args.warmup),
+        "ms_per_step": round(t_apply * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (the J1-J2 ring Hamiltonian itself; random unit input vector)",
+        "config": {
+            "workload": "J1-J2 Heisenberg ring N=32, J1=1, J2=0.5, periodic, Sz=0, k=0, reflection-even",
+            "dim": D,
+            "nnz": nnz,
+            "bytes_per_apply": B1,
+            "value_bytes": int(info["value_bytes"]),
+            "dict_size": int(info["dict_size"]),
+            "lanes_per_row": int(info["lanes"]),
+            "sector_build_s": round(t_sector, 3),
+            "csr_build_s": round(t_csr, 3),
+            "parallelism": "1 GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(gbs, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+        },
+        "multi_vector": multi,
+        "lanczos_to_E0": lanczos,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,156 @@
+/*
+ * qedh.h -- the C-ABI of the MI355X exact-diagonalisation backend (libqedspin.so).
+ *
+ * Plain pointers and sizes, opaque handles, int status codes: 0 ok, 1 invalid request,
+ * 2 resource limit (out of device memory / over a budget), 3 no device, 4 unsupported on the
+ * device, 5 runtime error, 6 convergence failure; qedh_last_error() holds the message of the
+ * last failure on the calling thread. The reference's error classes map one to one
+ * (include/ed/core/errors.h:24-50: InvalidRequest, ResourceLimit, DeviceUnavailable,
+ * DeviceUnsupported, ConvergenceError).
+ *
+ * Vectors passed to the apply entry points are DEVICE pointers (qedh_malloc or any HIP
+ * allocation); a sector operator's vectors are double when qedh_matvec_info reports it real,
+ * else interleaved complex double (std::complex<double> / hipDoubleComplex layout).
+ *
+ * Each entry point names the reference interface it replaces (file:line in the reference
+ * checkout).
+ */
+#ifndef QEDH_H
+#define QEDH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct qedh_operator_s* qedh_operator;
+typedef struct qedh_program_s* qedh_program;
+typedef struct qedh_sector_s* qedh_sector;
+typedef struct qedh_matvec_s* qedh_matvec;
+
+/* ---- runtime ------------------------------------------------------------------------------ */
+/* The last failure's message on this thread (empty when none). */
+const char* qedh_last_error(void);
+/* Visible HIP devices; 0 without a device (replaces ed::cuda_device_count / has_cuda_build,
+ * python/qed/_verbs/_device.py:18-29). */
+int qedh_device_count(int* count);
+/* Device memory for callers without their own allocator (replaces CudaBackend::allocate /
+ * deallocate / copy_from_host / copy_to_host, include/ed/gpu/cuda_backend.cuh:104-230). */
+int qedh_malloc(void** ptr, size_t bytes);
+int qedh_free(void* ptr);
+int qedh_memcpy_htod(void* dst, const void* src, size_t bytes);
+int qedh_memcpy_dtoh(void* dst, const void* src, size_t bytes);
+int qedh_memset(void* dst, int value, size_t bytes);
+int qedh_synchronize(void);
+
+/* ---- operators: canonical spin-1/2 terms (src/ops/algebra.cpp, include/ed/ops/algebra.h) ----- */
+int qedh_operator_create(int n_sites, qedh_operator* out);
+/* coeff * ops[0](sites[0]) ops[1](sites[1]) ..., the last factor acting first; ops from
+ * "+-zxyudI" (MaskedOperator::product, src/ops/algebra.cpp:67-100). */
+int qedh_operator_add_product(qedh_operator op, const char* ops, const int* sites, double re, double im);
+/* s_H (sum |c|), number of canonical terms, realness (Operator::norm_bound, isReal). */
+int qedh_operator_info(qedh_operator op, int64_t* n_terms, double* norm_bound, int* is_real);
+int qedh_operator_destroy(qedh_operator op);
+
+/* ---- programs (src/ops/program.cpp:34-141, include/ed/ops/operator.h:132-138) ---------------- */
+/* The flip-grouped program of op (adjoint != 0: of op^dagger, the row program of a sector walk). */
+int qedh_program_compile(qedh_operator op, int adjoint, qedh_program* out);
+/* The lambda-projected program of op (adjoint: of op^dagger) with ket sector src and bra sector
+ * tgt of one group (compile_program, src/ops/program.cpp:61-130). */
+int qedh_program_compile_projected(qedh_operator op, int adjoint, qedh_sector src, qedh_sector tgt,
+                                   qedh_program* out);
+int qedh_program_counts(qedh_program p, int64_t* n_groups, int64_t* n_vsub, int64_t* n_terms);
+int qedh_program_export(qedh_program p, uint64_t* group_flip, int32_t* group_setbits, uint32_t* group_vbegin,
+                        uint64_t* vsub_val, uint32_t* vsub_tbegin, uint64_t* term_sign, double* term_coeff);
+int qedh_program_destroy(qedh_program p);
+
+/* ---- sectors: enumerated on the device (include/ed/basis/orbit_table.h:250-286,
+ * src/engine/group_sector.cpp:52-76 + 159-205, include/ed/basis/rep_sector.h:134-408) ----------- */
+/* perms: group_size x n_sites row-major (bit i of g s = bit perms[g n + i] of s); flips: per
+ * element XOR mask or NULL; chars: group_size complex characters (re, im); n_up < 0: the full
+ * 2^N space. sublattice_mode: -1 the size rule (relaxed != 0: 16 distinct permutations, else 64),
+ * 0 never, 1 whenever a block system exists (ED_SYM_SUBLATTICE,
+ * include/ed/basis/sublattice_code.h:187-230). rank_budget_bytes caps the dense rank table. */
+int qedh_sector_create(int n_sites, int n_up, int group_size, const int* perms, const uint64_t* flips,
+                       const double* chars, int sublattice_mode, int relaxed, uint64_t rank_budget_bytes,
+                       qedh_sector* out);
+/* dim, orbits before the character filter, sublattice block size (0: plain order), build seconds. */
+int qedh_sector_info(qedh_sector s, int64_t* dim, int64_t* n_orbits, int* sublattice_block, double* build_seconds);
+int qedh_sector_export(qedh_sector s, uint64_t* reps, double* inv_norms);
+/* key_bit[n_sites] of the sublattice order; returns the block size through *block (0: none). */
+int qedh_sector_key_bits(qedh_sector s, int* key_bit, int* block);
+/* index_and_projection of n host states on the device (rep_symmetry_basis_policy.h:221-241). */
+int qedh_sector_index_and_projection(qedh_sector s, const uint64_t* states, int64_t n, int64_t* index,
+                                     double* proj);
+int qedh_sector_destroy(qedh_sector s);
+
+/* ---- sector operators (RepSectorMatVec / CrossSectorMatVec device lanes,
+ * src/engine/internal.h:163-609; make_sector_matvec_gpu_rep, make_cross_matvec_gpu_rep,
+ * build_sector_csr_gpu, csr_matvec_gpu(_multi), include/ed/gpu/rep_matvec.h:43-101) ----------- */
+enum { QEDH_LANE_AUTO = 0, QEDH_LANE_GATHER = 1, QEDH_LANE_CSR = 2 };
+typedef struct {
+    int64_t rows, cols;
+    int is_real;          /* vectors are double */
+    int lane;             /* QEDH_LANE_GATHER or QEDH_LANE_CSR */
+    int64_t nnz;          /* CSR entries (merged), 0 on the gather lane */
+    int value_bytes;      /* 1 / 2 (dictionary ids) or 8 / 16 (full values) */
+    int id_width_ref;     /* the reference's rule: 1, 2 or 16 */
+    int64_t dict_size;
+    uint64_t csr_bytes;
+    double build_seconds;
+    int lanes;            /* SpMV lanes per row */
+    double norm_bound;
+} qedh_matvec_info_t;
+/* cols == NULL or == rows: one sector (the diagonal shortcut applies). prog: the row program
+ * (the walk of O^dagger; for two sectors the projected program of O^dagger with rows as ket). */
+int qedh_matvec_create(qedh_sector rows, qedh_sector cols, qedh_program prog, double norm_bound, int lane,
+                       uint64_t csr_budget_bytes, qedh_matvec* out);
+int qedh_matvec_info(qedh_matvec m, qedh_matvec_info_t* info);
+/* out = O in (bind_cuda, include/ed/matvec/linear_operator.h:51-54); complex_vectors != 0 applies a
+ * real operator to complex vectors. */
+int qedh_matvec_apply(qedh_matvec m, const void* in, void* out, int complex_vectors);
+/* outs[i] = O ins[i] (bind_cuda_multi; k-vector applies equal single applies bit for bit). */
+int qedh_matvec_apply_multi(qedh_matvec m, const void* const* ins, void* const* outs, int k, int complex_vectors);
+/* Block-interleaved X[row * nv + i], nv in {1, 2, 4, 8}; CSR lane. */
+int qedh_matvec_apply_block(qedh_matvec m, int nv, const void* X, void* Y, int complex_vectors);
+int qedh_matvec_set_lanes(qedh_matvec m, int lanes);
+/* The merged CSR (download_csr): row_ptr[rows+1], col[nnz], val[2 nnz] (re, im). */
+int qedh_matvec_export_csr(qedh_matvec m, uint64_t* row_ptr, uint32_t* col, double* val);
+/* Rows [r0, r1) of it: row_ptr[r1 - r0 + 1] (absolute offsets), then col / val of those rows
+ * (pass NULL col / val to read the offsets first). */
+int qedh_matvec_export_csr_rows(qedh_matvec m, int64_t r0, int64_t r1, uint64_t* row_ptr, uint32_t* col,
+                                double* val);
+int qedh_matvec_destroy(qedh_matvec m);
+/* Average seconds of one apply over `iters` timed applies (after `warmup`) of k vectors at once
+ * (k = 1: the single apply; 2..8 multi-vector; block != 0: block-interleaved), measured with HIP
+ * events on the stream the kernels run on. Vectors are allocated and filled inside. */
+int qedh_matvec_benchmark(qedh_matvec m, int k, int block, int complex_vectors, int warmup, int iters,
+                          double* avg_seconds);
+
+/* ---- Krylov drivers (include/ed/krylov/lanczos.h:266-576, src/engine/block_solve.cpp:433-655,
+ * include/ed/thermal/ftlm.h:277-480, include/ed/dynamics/cf.h:103-161) -------------------------- */
+/* One Lanczos run from a host start vector (the operator's scalar type unless complex_vectors):
+ * reorth 0 None, 1 FullCGS2, 2 LocalDGKS3. alpha[max_iter], beta[max_iter + 1]; *iters steps
+ * done; timings[4] = apply s, reorth s, other vector s, total s (may be NULL). */
+int qedh_lanczos(qedh_matvec m, const void* v0_host, int complex_vectors, int max_iter, int reorth,
+                 double breakdown_tol, double* alpha, double* beta, int* iters, double* timings);
+/* The lowest k levels of the block (solve_block_lowest): the k = 1 scan or Krylov-Schur.
+ * values[k]; *n_values found; *converged; *applies. */
+int qedh_block_lowest(qedh_matvec m, int k, int64_t max_iter, double* values, int* n_values, int* converged,
+                      int64_t* applies);
+/* FTLM of the block: samples from splitmix64(seed + phi (s + 1)); lnZ/E/V per beta; *e_min. */
+int qedh_ftlm(qedh_matvec m, int samples, int krylov, uint64_t seed, int n_beta, const double* betas,
+              double breakdown_tol, double* lnZ, double* E, double* V, double* e_min);
+/* The continued fraction from a host vector phi (complex): S[n_omega]; has_shift selects
+ * energy_shift, else the lowest Ritz value; out_shift, out_norm_sq, out_change may be NULL. */
+int qedh_cf(qedh_matvec m, const double* phi_host, int krylov, int n_omega, const double* omega, double eta,
+            int has_shift, double energy_shift, double breakdown_tol, double* S, double* out_shift,
+            double* out_norm_sq, double* out_change);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QEDH_H */

@@ -1,0 +1,181 @@
+// GPU-resident sector enumeration: the orbit scan of a fixed-Sz (or full) subspace into ascending
+// representatives with closed-form norms, and the dense rank -> index table.
+//
+// The subspace is cut into 64-state words in rank (= ascending state) order. Pass 1 tests every
+// state for being its orbit's representative and keeps those whose character sum over the
+// stabiliser survives (a bit per state); an exclusive scan of the words' popcounts places them;
+// pass 2 re-walks the words with a set bit and writes reps and 1/norm in order. The result is the
+// reference's sector bit for bit: same representatives, same order, same norms
+// (orbit_table.h:147-239, group_sector.cpp:159-205).
+#include <hipcub/hipcub.hpp>
+
+#include "dev_common.h"
+#include "launch.h"
+
+namespace qsa::dev {
+
+namespace {
+
+__device__ __forceinline__ u64 binom_at(const SectorView& S, int n, int k) {
+    if (k < 0 || n < 0 || k > n) return 0;
+    return S.binom[n * 65 + k];
+}
+
+/// The state of colex rank r with n_up set bits (combinadic.h:86-96); the state itself for the
+/// full space.
+__device__ u64 state_of_rank(const SectorView& S, u64 rank) {
+    if (S.n_up < 0) return rank;
+    u64 s = 0;
+    for (int i = S.n_up - 1; i >= 0; --i) {
+        int p = i;
+        while (p + 1 < S.n_sites && binom_at(S, p + 1, i + 1) <= rank) ++p;
+        s |= 1ULL << p;
+        rank -= binom_at(S, p, i + 1);
+    }
+    return s;
+}
+
+__device__ __forceinline__ u64 next_state(const SectorView& S, u64 v) {
+    if (S.n_up < 0) return v + 1;
+    const u64 c = v & (~v + 1ULL);
+    const u64 r = v + c;
+    return (((r ^ v) >> 2) / c) | r;
+}
+
+/// Representative test (orbit_table.h:147-171): no image smaller, in the plain or key order.
+template <bool SLC> __device__ bool is_rep(const SectorView& S, u64 s) {
+    if constexpr (!SLC) {
+        for (int g = 0; g < S.G; ++g)
+            if (apply_element(S, g, s) < s) return false;
+        return true;
+    } else {
+        const u64 k = key_of(S, s);
+        u32 least = 0xFFFFFFFFu;
+        for (int j = 0; j < S.slc_m; ++j) {
+            const u32 l = S.lead[(static_cast<u32>(j) << S.slc_L) + pattern_of(S, k, j)];
+            least = l < least ? l : least;
+        }
+        if (least < pattern_of(S, k, 0)) return false;
+        bool rep = true;
+        for_each_candidate(S, k, [&](int g) {
+            if (rep && key_of(S, apply_element(S, g, s)) < k) rep = false;
+        });
+        return rep;
+    }
+}
+
+/// norm^2 = |sum_{h in Stab(s), ascending} chi(h)|^2 / |Stab|, 1 when |Stab| = 1 (orbit_table.h:93-99).
+__device__ double norm_sq(const SectorView& S, u64 s) {
+    double sr = 0.0, si = 0.0;
+    int cnt = 0;
+    for (int g = 0; g < S.G; ++g)
+        if (apply_element(S, g, s) == s) {
+            sr += S.chi_re[g];
+            si += S.chi_im[g];
+            ++cnt;
+        }
+    if (cnt == 1) return 1.0;
+    const double a = sr * sr;
+    const double b = si * si;
+    return (a + b) / static_cast<double>(cnt);
+}
+
+template <bool SLC>
+__global__ __launch_bounds__(256) void k_orbit_flags(SectorView S, u64 total, u64 n_words, u64* kept,
+                                                     unsigned long long* orbits) {
+    const u64 w = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x;
+    unsigned long long my_orbits = 0;
+    if (w < n_words) {
+        const u64 r0 = w * 64;
+        const u64 r1 = (r0 + 64 < total) ? r0 + 64 : total;
+        u64 s = state_of_rank(S, r0);
+        u64 f = 0;
+        for (u64 r = r0; r < r1; ++r) {
+            if (is_rep<SLC>(S, s)) {
+                ++my_orbits;
+                if (norm_sq(S, s) > 1e-12) f |= 1ULL << (r - r0);
+            }
+            if (r + 1 < r1) s = next_state(S, s);
+        }
+        kept[w] = f;
+    }
+    // integer sum: any order gives the same count
+    for (int off = 32; off > 0; off >>= 1) my_orbits += __shfl_down(my_orbits, off, 64);
+    if ((threadIdx.x & 63) == 0 && my_orbits) atomicAdd(orbits, my_orbits);
+}
+
+__global__ void k_popcount_words(const u64* kept, u64 n_words, u64* counts) {
+    const u64 w = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (w < n_words) counts[w] = static_cast<u64>(__popcll(kept[w]));
+}
+
+__global__ __launch_bounds__(256) void k_orbit_emit(SectorView S, u64 total, u64 n_words, const u64* kept,
+                                                    const u64* offsets, u64* reps, double* inv_norms) {
+    const u64 w = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (w >= n_words) return;
+    const u64 f = kept[w];
+    if (!f) return;
+    u64 off = offsets[w];
+    const u64 r0 = w * 64;
+    const u64 r1 = (r0 + 64 < total) ? r0 + 64 : total;
+    u64 s = state_of_rank(S, r0);
+    for (u64 r = r0; r < r1; ++r) {
+        if ((f >> (r - r0)) & 1ULL) {
+            reps[off] = s;
+            inv_norms[off] = 1.0 / sqrt(norm_sq(S, s));
+            ++off;
+        }
+        if (r + 1 < r1) s = next_state(S, s);
+    }
+}
+
+__global__ void k_rank_fill(SectorView S, i32* table) {
+    const i64 i = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= S.dim) return;
+    const u64 s = S.reps[i];
+    const i64 r = S.n_up >= 0 ? rank_of(S, s) : static_cast<i64>(s);
+    table[r] = static_cast<i32>(i);
+}
+
+inline unsigned blocks_for(u64 n, unsigned t) { return static_cast<unsigned>((n + t - 1) / t); }
+
+}  // namespace
+
+// The kept-representative bitmap and the orbit count of the subspace (before the character filter).
+void orbit_flags(const SectorView& S, u64 total, u64* kept, unsigned long long* orbits, hipStream_t st) {
+    const u64 n_words = (total + 63) / 64;
+    if (S.slc_L > 0)
+        hipLaunchKernelGGL(k_orbit_flags<true>, dim3(blocks_for(n_words, 256)), dim3(256), 0, st, S, total, n_words,
+                           kept, orbits);
+    else
+        hipLaunchKernelGGL(k_orbit_flags<false>, dim3(blocks_for(n_words, 256)), dim3(256), 0, st, S, total,
+                           n_words, kept, orbits);
+}
+
+std::size_t scan_words_temp_bytes(u64 n_words) {
+    std::size_t bytes = 0;
+    hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, static_cast<u64*>(nullptr), static_cast<u64*>(nullptr),
+                                     static_cast<int>(n_words));
+    return bytes;
+}
+
+// offsets[w] = number of kept states in words before w; returns via *total_kept (device) the sum.
+void scan_words(const u64* kept, u64 n_words, u64* counts, u64* offsets, void* temp, std::size_t temp_bytes,
+                hipStream_t st) {
+    hipLaunchKernelGGL(k_popcount_words, dim3(blocks_for(n_words, 256)), dim3(256), 0, st, kept, n_words, counts);
+    hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, counts, offsets, static_cast<int>(n_words), st);
+}
+
+void orbit_emit(const SectorView& S, u64 total, const u64* kept, const u64* offsets, u64* reps, double* inv_norms,
+                hipStream_t st) {
+    const u64 n_words = (total + 63) / 64;
+    hipLaunchKernelGGL(k_orbit_emit, dim3(blocks_for(n_words, 256)), dim3(256), 0, st, S, total, n_words, kept,
+                       offsets, reps, inv_norms);
+}
+
+void rank_fill(const SectorView& S, i32* table, hipStream_t st) {
+    if (S.dim == 0) return;
+    hipLaunchKernelGGL(k_rank_fill, dim3(blocks_for(static_cast<u64>(S.dim), 256)), dim3(256), 0, st, S, table);
+}
+
+}  // namespace qsa::dev

@@ -1,0 +1,391 @@
+// BLAS-1 of the Krylov drivers on device vectors of double or complex<double>, with fused forms
+// (axpy + dot, axpy + norm, dot_many / axpy_many for Gram-Schmidt). Every reduction is
+// deterministic: a fixed grid for a given n, a fixed xor-butterfly inside the wavefront, fixed
+// slots across wavefronts and blocks, and a final single-block sum in block order. The result stays
+// in device memory; the host reads it when it needs it.
+#include "dev_common.h"
+#include "launch.h"
+
+namespace qsa::dev {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kMaxBlocks = 1024;
+constexpr int kWaves = kThreads / 64;
+
+inline unsigned grid_for(std::size_t n) {
+    const std::size_t b = (n + kThreads * 4 - 1) / (kThreads * 4);
+    return static_cast<unsigned>(b < 1 ? 1 : (b > kMaxBlocks ? kMaxBlocks : b));
+}
+
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
+}
+
+/// Block sum of (a, b): every thread returns the block total.
+__device__ __forceinline__ void block_sum2(double& a, double& b) {
+    __shared__ double sa[kWaves], sb[kWaves];
+    a = wave_sum(a);
+    b = wave_sum(b);
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) {
+        sa[w] = a;
+        sb[w] = b;
+    }
+    __syncthreads();
+    a = 0.0;
+    b = 0.0;
+    for (int i = 0; i < kWaves; ++i) {
+        a += sa[i];
+        b += sb[i];
+    }
+}
+
+template <bool C> struct V;
+template <> struct V<false> {
+    using T = double;
+    static __device__ __forceinline__ Zd load(const void* p, std::size_t i) { return Zd{static_cast<const double*>(p)[i], 0.0}; }
+    static __device__ __forceinline__ void store(void* p, std::size_t i, Zd z) { static_cast<double*>(p)[i] = z.re; }
+};
+template <> struct V<true> {
+    using T = Zd;
+    static __device__ __forceinline__ Zd load(const void* p, std::size_t i) { return static_cast<const Zd*>(p)[i]; }
+    static __device__ __forceinline__ void store(void* p, std::size_t i, Zd z) { static_cast<Zd*>(p)[i] = z; }
+};
+
+__device__ __forceinline__ Zd cmul(Zd a, Zd b) { return Zd{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+__device__ __forceinline__ Zd cadd(Zd a, Zd b) { return Zd{a.re + b.re, a.im + b.im}; }
+/// conj(a) b
+__device__ __forceinline__ Zd cdot(Zd a, Zd b) { return Zd{a.re * b.re + a.im * b.im, a.re * b.im - a.im * b.re}; }
+
+template <bool C> __global__ void k_fill_zero(void* x, std::size_t n) {
+    for (std::size_t i = blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<std::size_t>(gridDim.x) * blockDim.x)
+        V<C>::store(x, i, Zd{0.0, 0.0});
+}
+
+template <bool C> __global__ void k_axpby(Zd a, const void* x, Zd b, void* y, std::size_t n, int mode) {
+    // mode 0: y += a x; 1: x *= a (x is y); 2: y = a x + b y
+    for (std::size_t i = blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+        if (mode == 1) {
+            V<C>::store(y, i, C ? cmul(a, V<C>::load(y, i)) : Zd{a.re * V<C>::load(y, i).re, 0.0});
+        } else if (mode == 0) {
+            const Zd xv = V<C>::load(x, i), yv = V<C>::load(y, i);
+            V<C>::store(y, i, C ? cadd(yv, cmul(a, xv)) : Zd{yv.re + a.re * xv.re, 0.0});
+        } else {
+            const Zd xv = V<C>::load(x, i), yv = V<C>::load(y, i);
+            V<C>::store(y, i, C ? cadd(cmul(a, xv), cmul(b, yv)) : Zd{a.re * xv.re + b.re * yv.re, 0.0});
+        }
+    }
+}
+
+// Reduction kinds: 0 dot z^H y; 1 sum |y|^2; both optionally after y += a x (fuse != 0).
+template <bool C, int KIND>
+__global__ __launch_bounds__(kThreads) void k_reduce(Zd a, const void* x, void* y, const void* z, std::size_t n,
+                                                     int fuse, double* partial) {
+    double s0 = 0.0, s1 = 0.0;
+    for (std::size_t i = blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+        Zd yv = V<C>::load(y, i);
+        if (fuse) {
+            const Zd xv = V<C>::load(x, i);
+            yv = C ? cadd(yv, cmul(a, xv)) : Zd{yv.re + a.re * xv.re, 0.0};
+            V<C>::store(y, i, yv);
+        }
+        if constexpr (KIND == 0) {
+            const Zd zv = V<C>::load(z, i);
+            if constexpr (C) {
+                const Zd d = cdot(zv, yv);
+                s0 += d.re;
+                s1 += d.im;
+            } else {
+                s0 += zv.re * yv.re;
+            }
+        } else {
+            s0 += yv.re * yv.re;
+            if constexpr (C) s0 += yv.im * yv.im;
+        }
+    }
+    block_sum2(s0, s1);
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = s0;
+        partial[2 * blockIdx.x + 1] = s1;
+    }
+}
+
+// One block: sum the partials in block order. out_kind 0: Zd (dot), 1: sqrt (norm), 2: raw sum.
+__global__ __launch_bounds__(64) void k_finish(const double* partial, int nb, int out_kind, void* out) {
+    if (threadIdx.x != 0) return;
+    double a = 0.0, b = 0.0;
+    for (int i = 0; i < nb; ++i) {
+        a += partial[2 * i];
+        b += partial[2 * i + 1];
+    }
+    if (out_kind == 0) *static_cast<Zd*>(out) = Zd{a, b};
+    else if (out_kind == 1) *static_cast<double*>(out) = sqrt(a);
+    else *static_cast<double*>(out) = a;
+}
+
+// dot_many: the block's rows of w staged in LDS, then for every column j the block's partial of
+// V_j^H w -- wave sums into fixed LDS slots, summed across wavefronts in order.
+constexpr int kManyCols = 256;     // columns per launch
+constexpr int kManyRows = 2048;    // rows per block tile
+
+template <bool C>
+__global__ __launch_bounds__(kThreads) void k_dot_many(const void* Vb, std::size_t ld, int m, const void* w,
+                                                       std::size_t n, std::size_t rows_per_block, double* partial,
+                                                       int want_norm) {
+    __shared__ Zd sw[kManyRows];
+    __shared__ double sre[kWaves][kManyCols + 1], sim[kWaves][kManyCols + 1];
+    const std::size_t r0 = blockIdx.x * rows_per_block;
+    const std::size_t r1 = (r0 + rows_per_block < n) ? r0 + rows_per_block : n;
+    const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+    double acc_re[1] = {0.0};
+    for (int j = threadIdx.x; j < (m + 1) * kWaves; j += blockDim.x) {
+        sre[j / (m + 1)][j % (m + 1)] = 0.0;
+        sim[j / (m + 1)][j % (m + 1)] = 0.0;
+    }
+    __syncthreads();
+    for (std::size_t t0 = r0; t0 < r1; t0 += kManyRows) {
+        const std::size_t t1 = (t0 + kManyRows < r1) ? t0 + kManyRows : r1;
+        const int cnt = static_cast<int>(t1 - t0);
+        for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
+            const Zd v = V<C>::load(w, t0 + i);
+            sw[i] = v;
+            if (want_norm) acc_re[0] += v.re * v.re + (C ? v.im * v.im : 0.0);
+        }
+        __syncthreads();
+        for (int j = 0; j < m; ++j) {
+            double pr = 0.0, pi = 0.0;
+            for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
+                const Zd vj = V<C>::load(Vb, j * ld + t0 + i);
+                if constexpr (C) {
+                    const Zd d = cdot(vj, sw[i]);
+                    pr += d.re;
+                    pi += d.im;
+                } else {
+                    pr += vj.re * sw[i].re;
+                }
+            }
+            pr = wave_sum(pr);
+            if constexpr (C) pi = wave_sum(pi);
+            if (ln == 0) {
+                sre[wv][j] += pr;
+                if constexpr (C) sim[wv][j] += pi;
+            }
+        }
+        __syncthreads();
+    }
+    // the norm of w in the last slot
+    if (want_norm) {
+        double nr = wave_sum(acc_re[0]);
+        if (ln == 0) sre[wv][m] = nr;
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j <= m; j += blockDim.x) {
+        double a = 0.0, b = 0.0;
+        for (int q = 0; q < kWaves; ++q) {
+            a += sre[q][j];
+            b += sim[q][j];
+        }
+        partial[(static_cast<std::size_t>(blockIdx.x) * (m + 1) + j) * 2] = a;
+        partial[(static_cast<std::size_t>(blockIdx.x) * (m + 1) + j) * 2 + 1] = b;
+    }
+}
+
+__global__ void k_dot_many_finish(const double* partial, int nb, int m, Zd* coeffs, double* wnorm2) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j > m) return;
+    double a = 0.0, b = 0.0;
+    for (int q = 0; q < nb; ++q) {
+        a += partial[(static_cast<std::size_t>(q) * (m + 1) + j) * 2];
+        b += partial[(static_cast<std::size_t>(q) * (m + 1) + j) * 2 + 1];
+    }
+    if (j < m) coeffs[j] = Zd{a, b};
+    else if (wnorm2) *wnorm2 = a;
+}
+
+// w += sum_j c_j V_j over the block's rows; the partial of ||w||^2 after the update.
+template <bool C>
+__global__ __launch_bounds__(kThreads) void k_axpy_many(const Zd* coeffs, const void* Vb, std::size_t ld, int m,
+                                                        void* w, std::size_t n, double* partial, int want_norm,
+                                                        double sign) {
+    __shared__ Zd sc[kManyCols];
+    for (int j = threadIdx.x; j < m; j += blockDim.x) sc[j] = Zd{sign * coeffs[j].re, sign * coeffs[j].im};
+    __syncthreads();
+    double s0 = 0.0, s1 = 0.0;
+    for (std::size_t i = blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+        Zd acc = V<C>::load(w, i);
+        for (int j = 0; j < m; ++j) {
+            const Zd vj = V<C>::load(Vb, j * ld + i);
+            acc = C ? cadd(acc, cmul(sc[j], vj)) : Zd{acc.re + sc[j].re * vj.re, 0.0};
+        }
+        V<C>::store(w, i, acc);
+        if (want_norm) s0 += acc.re * acc.re + (C ? acc.im * acc.im : 0.0);
+    }
+    if (want_norm) {
+        block_sum2(s0, s1);
+        if (threadIdx.x == 0) {
+            partial[2 * blockIdx.x] = s0;
+            partial[2 * blockIdx.x + 1] = 0.0;
+        }
+    }
+}
+
+}  // namespace
+
+std::size_t blas_scratch_bytes(std::size_t max_cols) {
+    const std::size_t red = 2 * kMaxBlocks * sizeof(double);
+    const std::size_t many = static_cast<std::size_t>(kMaxBlocks) * (kManyCols + 1) * 2 * sizeof(double);
+    (void)max_cols;
+    return red > many ? red : many;
+}
+
+void fill_zero(bool c, void* x, std::size_t n, hipStream_t st) {
+    if (!n) return;
+    const unsigned g = grid_for(n);
+    if (c) hipLaunchKernelGGL(k_fill_zero<true>, dim3(g), dim3(kThreads), 0, st, x, n);
+    else hipLaunchKernelGGL(k_fill_zero<false>, dim3(g), dim3(kThreads), 0, st, x, n);
+}
+
+static void launch_axpby(bool c, Zd a, const void* x, Zd b, void* y, std::size_t n, int mode, hipStream_t st) {
+    if (!n) return;
+    const unsigned g = grid_for(n);
+    if (c) hipLaunchKernelGGL(k_axpby<true>, dim3(g), dim3(kThreads), 0, st, a, x, b, y, n, mode);
+    else hipLaunchKernelGGL(k_axpby<false>, dim3(g), dim3(kThreads), 0, st, a, x, b, y, n, mode);
+}
+
+void axpy(bool c, Zd a, const void* x, void* y, std::size_t n, hipStream_t st) { launch_axpby(c, a, x, Zd{0, 0}, y, n, 0, st); }
+void scale(bool c, Zd a, void* x, std::size_t n, hipStream_t st) { launch_axpby(c, a, nullptr, Zd{0, 0}, x, n, 1, st); }
+void axpby(bool c, Zd a, const void* x, Zd b, void* y, std::size_t n, hipStream_t st) {
+    launch_axpby(c, a, x, b, y, n, 2, st);
+}
+
+static void reduce(bool c, int kind, Zd a, const void* x, void* y, const void* z, std::size_t n, int fuse,
+                   void* scratch, int out_kind, void* out, hipStream_t st) {
+    const unsigned g = grid_for(n);
+    double* part = static_cast<double*>(scratch);
+    if (c) {
+        if (kind == 0) hipLaunchKernelGGL((k_reduce<true, 0>), dim3(g), dim3(kThreads), 0, st, a, x, y, z, n, fuse, part);
+        else hipLaunchKernelGGL((k_reduce<true, 1>), dim3(g), dim3(kThreads), 0, st, a, x, y, z, n, fuse, part);
+    } else {
+        if (kind == 0) hipLaunchKernelGGL((k_reduce<false, 0>), dim3(g), dim3(kThreads), 0, st, a, x, y, z, n, fuse, part);
+        else hipLaunchKernelGGL((k_reduce<false, 1>), dim3(g), dim3(kThreads), 0, st, a, x, y, z, n, fuse, part);
+    }
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, st, part, static_cast<int>(g), out_kind, out);
+}
+
+void dot(bool c, const void* x, const void* y, std::size_t n, void* scratch, Zd* result, hipStream_t st) {
+    reduce(c, 0, Zd{0, 0}, nullptr, const_cast<void*>(y), x, n, 0, scratch, 0, result, st);
+}
+void nrm2(bool c, const void* x, std::size_t n, void* scratch, double* result, hipStream_t st) {
+    reduce(c, 1, Zd{0, 0}, nullptr, const_cast<void*>(x), nullptr, n, 0, scratch, 1, result, st);
+}
+void axpy_dot(bool c, Zd a, const void* x, void* y, const void* z, std::size_t n, void* scratch, Zd* result,
+              hipStream_t st) {
+    reduce(c, 0, a, x, y, z, n, 1, scratch, 0, result, st);
+}
+void axpy_nrm2(bool c, Zd a, const void* x, void* y, std::size_t n, void* scratch, double* result, hipStream_t st) {
+    reduce(c, 1, a, x, y, nullptr, n, 1, scratch, 1, result, st);
+}
+
+void dot_many(bool c, const void* V, std::size_t ld, std::size_t m, const void* w, std::size_t n, void* scratch,
+              Zd* coeffs, double* wnorm2, hipStream_t st) {
+    double* part = static_cast<double*>(scratch);
+    const std::size_t elem = c ? sizeof(Zd) : sizeof(double);
+    std::size_t done = 0;
+    bool first = true;
+    do {
+        const int mc = static_cast<int>((m - done) < static_cast<std::size_t>(kManyCols) ? (m - done) : kManyCols);
+        std::size_t nb = (n + kManyRows - 1) / kManyRows;
+        if (nb > static_cast<std::size_t>(kMaxBlocks)) nb = kMaxBlocks;
+        if (nb < 1) nb = 1;
+        const std::size_t rpb = (n + nb - 1) / nb;
+        const void* Vc = static_cast<const char*>(V) + done * ld * elem;
+        const int want = (first && wnorm2) ? 1 : 0;
+        if (c)
+            hipLaunchKernelGGL(k_dot_many<true>, dim3(static_cast<unsigned>(nb)), dim3(kThreads), 0, st, Vc, ld, mc, w,
+                               n, rpb, part, want);
+        else
+            hipLaunchKernelGGL(k_dot_many<false>, dim3(static_cast<unsigned>(nb)), dim3(kThreads), 0, st, Vc, ld, mc,
+                               w, n, rpb, part, want);
+        hipLaunchKernelGGL(k_dot_many_finish, dim3((mc + 1 + 63) / 64), dim3(64), 0, st, part, static_cast<int>(nb),
+                           mc, coeffs + done, want ? wnorm2 : nullptr);
+        done += static_cast<std::size_t>(mc);
+        first = false;
+    } while (done < m);
+}
+
+void axpy_many(bool c, const Zd* coeffs, const void* V, std::size_t ld, std::size_t m, void* w, std::size_t n,
+               void* scratch, double* wnorm2, hipStream_t st, double sign) {
+    double* part = static_cast<double*>(scratch);
+    const std::size_t elem = c ? sizeof(Zd) : sizeof(double);
+    const unsigned g = grid_for(n);
+    std::size_t done = 0;
+    do {
+        const int mc = static_cast<int>((m - done) < static_cast<std::size_t>(kManyCols) ? (m - done) : kManyCols);
+        const bool last = done + static_cast<std::size_t>(mc) >= m;
+        const int want = (last && wnorm2) ? 1 : 0;
+        const void* Vc = static_cast<const char*>(V) + done * ld * elem;
+        if (c)
+            hipLaunchKernelGGL(k_axpy_many<true>, dim3(g), dim3(kThreads), 0, st, coeffs + done, Vc, ld, mc, w, n, part,
+                               want, sign);
+        else
+            hipLaunchKernelGGL(k_axpy_many<false>, dim3(g), dim3(kThreads), 0, st, coeffs + done, Vc, ld, mc, w, n,
+                               part, want, sign);
+        if (want) hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, st, part, static_cast<int>(g), 2, wnorm2);
+        done += static_cast<std::size_t>(mc);
+    } while (done < m);
+}
+
+}  // namespace qsa::dev
+
+namespace qsa::dev {
+
+namespace {
+constexpr int kRotCols = 16;
+template <bool C>
+__global__ __launch_bounds__(256) void k_block_rotate(const void* Vb, std::size_t ld, int mm, const double* S, int c0,
+                                                      int nc, void* out, std::size_t n) {
+    const std::size_t i = blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x;
+    if (i >= n) return;
+    Zd acc[kRotCols];
+#pragma unroll
+    for (int c = 0; c < kRotCols; ++c) acc[c] = Zd{0.0, 0.0};
+    for (int r = 0; r < mm; ++r) {
+        const Zd v = V<C>::load(Vb, r * ld + i);
+#pragma unroll
+        for (int c = 0; c < kRotCols; ++c)
+            if (c < nc) {
+                const double s = S[static_cast<std::size_t>(c0 + c) * mm + r];
+                acc[c].re += v.re * s;
+                if constexpr (C) acc[c].im += v.im * s;
+            }
+    }
+#pragma unroll
+    for (int c = 0; c < kRotCols; ++c)
+        if (c < nc) V<C>::store(out, static_cast<std::size_t>(c0 + c) * n + i, acc[c]);
+}
+}  // namespace
+
+void block_rotate(bool c, const void* V, std::size_t ld, std::size_t mm, const double* S, std::size_t cols, void* out,
+                  std::size_t n, hipStream_t st) {
+    const dim3 grid(static_cast<unsigned>((n + 255) / 256)), block(256);
+    for (std::size_t c0 = 0; c0 < cols; c0 += kRotCols) {
+        const int nc = static_cast<int>(std::min<std::size_t>(kRotCols, cols - c0));
+        if (c)
+            hipLaunchKernelGGL(k_block_rotate<true>, grid, block, 0, st, V, ld, static_cast<int>(mm), S,
+                               static_cast<int>(c0), nc, out, n);
+        else
+            hipLaunchKernelGGL(k_block_rotate<false>, grid, block, 0, st, V, ld, static_cast<int>(mm), S,
+                               static_cast<int>(c0), nc, out, n);
+    }
+}
+
+}  // namespace qsa::dev

@@ -1,0 +1,238 @@
+// Device-side view of a symmetry sector and of a compiled program, and the per-state operations
+// every kernel shares: the group action (byte lookup tables), the sublattice key order, the
+// representative search with its character phase (index_and_projection), the reverse lookup and
+// the row walk's connection. Compiled for gfx950 only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace qsa::dev {
+
+using u64 = std::uint64_t;
+using i64 = std::int64_t;
+using u32 = std::uint32_t;
+using i32 = std::int32_t;
+using u16 = std::uint16_t;
+using u8 = std::uint8_t;
+
+struct Zd {
+    double re, im;
+};
+
+/// A sector as the kernels read it (all pointers device memory). The representative of an orbit
+/// is its least member (in key order when slc_L > 0); reps ascend by state value.
+struct SectorView {
+    int n_sites = 0, n_up = -1, G = 0, bpw = 0;
+    const u64* lut = nullptr;        // G x bpw x 256: image of byte b = v under element g
+    const u64* flips = nullptr;      // G
+    const double* chi_re = nullptr;  // G
+    const double* chi_im = nullptr;  // G
+    int slc_L = 0, slc_m = 0;
+    const u64* to_key = nullptr;     // bpw x 256
+    const u16* lead = nullptr;       // m x 2^L
+    const u32* cand_off = nullptr;   // m x 2^L + 1
+    const u16* cand = nullptr;
+    const u64* reps = nullptr;       // dim, ascending
+    const double* inv_norms = nullptr;
+    i64 dim = 0;
+    const i32* rank_table = nullptr; // C(N, n_up) entries (state-indexed 2^N when n_up < 0), or null
+    const u64* binom = nullptr;      // 65 x 65 Pascal table, binom[n * 65 + k] = C(n, k)
+};
+
+/// A compiled program (program.h layout) in device memory.
+struct ProgramView {
+    u32 n_groups = 0;
+    const u64* group_flip = nullptr;
+    const i32* group_setbits = nullptr;
+    const u32* group_vbegin = nullptr;
+    const u64* vsub_val = nullptr;
+    const u32* vsub_tbegin = nullptr;
+    const u64* term_sign = nullptr;
+    const double* coeff_re = nullptr;
+    const double* coeff_im = nullptr;
+};
+
+__device__ __forceinline__ int popc(u64 x) { return __popcll(x); }
+
+/// Element g on state s: OR of the byte tables, then the XOR flip (compiled_group.h:73-78).
+__device__ __forceinline__ u64 apply_element(const SectorView& S, int g, u64 s) {
+    const u64* l = S.lut + static_cast<size_t>(g) * S.bpw * 256;
+    u64 o = 0;
+    for (int b = 0; b < S.bpw; ++b) o |= l[(b << 8) + static_cast<int>((s >> (8 * b)) & 0xFFu)];
+    return o ^ S.flips[g];
+}
+
+__device__ __forceinline__ u64 key_of(const SectorView& S, u64 s) {
+    u64 k = 0;
+    for (int b = 0; b < S.bpw; ++b) k |= S.to_key[(b << 8) + static_cast<int>((s >> (8 * b)) & 0xFFu)];
+    return k;
+}
+
+__device__ __forceinline__ u32 pattern_of(const SectorView& S, u64 k, int j) {
+    return static_cast<u32>(k >> (S.n_sites - (j + 1) * S.slc_L)) & ((1u << S.slc_L) - 1u);
+}
+
+/// fn(g) for the elements that can map the state (key k) to its representative, ascending: those
+/// whose image has the least leading block (sublattice_code.h:115-147).
+template <class Fn> __device__ __forceinline__ void for_each_candidate(const SectorView& S, u64 k, Fn&& fn) {
+    u32 entry[8];
+    u32 best = 0xFFFFFFFFu;
+    for (int j = 0; j < S.slc_m; ++j) {
+        entry[j] = (static_cast<u32>(j) << S.slc_L) + pattern_of(S, k, j);
+        const u32 l = S.lead[entry[j]];
+        best = l < best ? l : best;
+    }
+    u32 pos[8], end[8];
+    int n = 0;
+    for (int j = 0; j < S.slc_m; ++j)
+        if (S.lead[entry[j]] == best) {
+            pos[n] = S.cand_off[entry[j]];
+            end[n] = S.cand_off[entry[j] + 1];
+            ++n;
+        }
+    if (n == 1) {
+        for (u32 p = pos[0]; p < end[0]; ++p) fn(static_cast<int>(S.cand[p]));
+        return;
+    }
+    for (;;) {
+        int pick = -1;
+        u32 g = 0xFFFFFFFFu;
+        for (int i = 0; i < n; ++i)
+            if (pos[i] < end[i] && S.cand[pos[i]] < g) {
+                g = S.cand[pos[i]];
+                pick = i;
+            }
+        if (pick < 0) return;
+        ++pos[pick];
+        fn(static_cast<int>(g));
+    }
+}
+
+/// Colex combinadic rank of a state with n_up set bits: the j-th set bit at position b adds C(b, j).
+__device__ __forceinline__ i64 rank_of(const SectorView& S, u64 s) {
+    i64 r = 0;
+    int j = 0;
+    while (s) {
+        const int b = __ffsll(static_cast<long long>(s)) - 1;
+        ++j;
+        r += static_cast<i64>(S.binom[b * 65 + j]);
+        s &= s - 1;
+    }
+    return r;
+}
+
+/// Index of a representative, or -1: the rank table when present, else a binary search of reps.
+__device__ __forceinline__ i64 index_of_rep(const SectorView& S, u64 rb) {
+    if (S.rank_table) {
+        const i64 r = S.n_up >= 0 ? rank_of(S, rb) : static_cast<i64>(rb);
+        return static_cast<i64>(S.rank_table[r]);
+    }
+    i64 lo = 0, hi = S.dim;
+    while (lo < hi) {
+        const i64 mid = lo + (hi - lo) / 2;
+        if (S.reps[mid] < rb) lo = mid + 1;
+        else hi = mid;
+    }
+    return (lo < S.dim && S.reps[lo] == rb) ? lo : -1;
+}
+
+/// The orbit index of t and its projection conj(sum_{g: g t = rep} chi(g)) / norm, or -1
+/// (rep_symmetry_basis_policy.h:221-241): a running minimum over the images in ascending element
+/// order; the character sum restarts at a new minimum (0.0 + re, 0.0 - im) and grows on ties.
+template <bool SLC>
+__device__ __forceinline__ i64 index_and_projection(const SectorView& S, u64 t, double& pr, double& pi) {
+    if (S.n_up >= 0 && popc(t) != S.n_up) return -1;
+    u64 rb = ~0ULL, best = ~0ULL;
+    double ar = 0.0, ai = 0.0;
+    if constexpr (!SLC) {
+        for (int g = 0; g < S.G; ++g) {
+            const u64 img = apply_element(S, g, t);
+            if (img < best) {
+                best = img;
+                rb = img;
+                ar = 0.0 + S.chi_re[g];
+                ai = 0.0 - S.chi_im[g];
+            } else if (img == best) {
+                ar += S.chi_re[g];
+                ai -= S.chi_im[g];
+            }
+        }
+    } else {
+        const u64 k = key_of(S, t);
+        for_each_candidate(S, k, [&](int g) {
+            const u64 img = apply_element(S, g, t);
+            const u64 ik = key_of(S, img);
+            if (ik < best) {
+                best = ik;
+                rb = img;
+                ar = 0.0 + S.chi_re[g];
+                ai = 0.0 - S.chi_im[g];
+            } else if (ik == best) {
+                ar += S.chi_re[g];
+                ai -= S.chi_im[g];
+            }
+        });
+    }
+    if (S.n_up >= 0 && popc(rb) != S.n_up) return -1;   // the group preserves popcount; never index past the table
+    const i64 idx = index_of_rep(S, rb);
+    if (idx < 0) return -1;
+    const double w = S.inv_norms[idx];
+    pr = ar * w;
+    pi = ai * w;
+    return idx;
+}
+
+/// Group g of the program on state s (row_walk.h:28-53): false when it does not act; else its
+/// target t = s ^ F and h = sum of +-coeff of the subgroup whose value is s & F, in key order.
+__device__ __forceinline__ bool connection(const ProgramView& P, u64 s, u32 g, u64& t, double& hr, double& hi) {
+    const u64 F = P.group_flip[g];
+    const u64 v = s & F;
+    const i32 sb = P.group_setbits[g];
+    if (sb >= 0 && popc(v) != sb) return false;
+    u32 lo = P.group_vbegin[g], hi_ = P.group_vbegin[g + 1];
+    const u32 stop = hi_;
+    while (lo < hi_) {
+        const u32 mid = lo + (hi_ - lo) / 2;
+        if (P.vsub_val[mid] < v) lo = mid + 1;
+        else hi_ = mid;
+    }
+    if (lo == stop || P.vsub_val[lo] != v) return false;
+    hr = 0.0;
+    hi = 0.0;
+    for (u32 k = P.vsub_tbegin[lo]; k < P.vsub_tbegin[lo + 1]; ++k) {
+        if (popc(s & P.term_sign[k]) & 1) {
+            hr += -P.coeff_re[k];
+            hi += -P.coeff_im[k];
+        } else {
+            hr += P.coeff_re[k];
+            hi += P.coeff_im[k];
+        }
+    }
+    t = s ^ F;
+    return true;
+}
+
+/// Matrix entry of row r (state s, 1/norm w) for a connection (t, h) into the column sector:
+/// diagonal conj(h) when the sectors are one and t == s, else w conj(h proj(t)) at index(t)
+/// (sector_rows.h:41-56). Returns the column or -1. Written without fused multiply-adds so the
+/// values carry the reference's bits.
+template <bool SLC>
+__device__ __forceinline__ i64 entry_of(const SectorView& C, bool same, i64 r, u64 s, double w, u64 t, double hr,
+                                        double hi, double& vr, double& vi) {
+    if (same && t == s) {
+        vr = hr;
+        vi = -hi;
+        return r;
+    }
+    double pr, pi;
+    const i64 c = index_and_projection<SLC>(C, t, pr, pi);
+    if (c < 0) return -1;
+    const double xr = hr * pr - hi * pi;
+    const double xi = hr * pi + hi * pr;
+    vr = w * xr;
+    vi = w * (-xi);
+    return c;
+}
+
+}  // namespace qsa::dev

@@ -1,0 +1,113 @@
+// Host-callable launchers of the device kernels (basis.hip, walk.hip, spmv.hip, blas.hip). All
+// take an explicit stream and never synchronise; the caller owns every buffer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+#include "dev_common.h"
+
+namespace qsa::dev {
+
+// ---- basis.hip -------------------------------------------------------------------------------
+void orbit_flags(const SectorView& S, u64 total, u64* kept, unsigned long long* orbits, hipStream_t st);
+std::size_t scan_words_temp_bytes(u64 n_words);
+void scan_words(const u64* kept, u64 n_words, u64* counts, u64* offsets, void* temp, std::size_t temp_bytes,
+                hipStream_t st);
+void orbit_emit(const SectorView& S, u64 total, const u64* kept, const u64* offsets, u64* reps, double* inv_norms,
+                hipStream_t st);
+void rank_fill(const SectorView& S, i32* table, hipStream_t st);
+
+// ---- walk.hip --------------------------------------------------------------------------------
+/// Up to this many vectors per multi-vector launch.
+constexpr int kMaxVec = 8;
+struct VecSet {
+    const void* in[kMaxVec];
+    void* out[kMaxVec];
+    int k = 0;
+};
+
+/// out_v = O in_v for every vector of the set, row by row on the fly (no CSR): rows of sector R,
+/// columns of sector C (same: one sector). real: vectors of double (the operator is real), else
+/// interleaved complex.
+void walk_gather(const SectorView& R, const SectorView& C, bool same, const ProgramView& P, bool real,
+                 const VecSet& v, hipStream_t st);
+
+/// Global value dictionary of a CSR build (open addressing on a 64-bit fingerprint, payload bits).
+struct DictTable {
+    static constexpr int kSlots = 1 << 17;
+    u64* fp = nullptr;
+    u64* re = nullptr;
+    u64* im = nullptr;
+    u16* slot_id = nullptr;
+    unsigned* count = nullptr;     // distinct values inserted
+    unsigned* flags = nullptr;     // bit 0: overflow, bit 1: fingerprint collision
+};
+
+/// Pass 1 of the CSR build: merged row lengths into row_len, distinct values into the dictionary.
+/// Returns false (nothing launched) when the program has more groups than a row can hold.
+bool csr_count(const SectorView& R, const SectorView& C, bool same, const ProgramView& P, u32* row_len,
+               const DictTable& D, bool use_dict, hipStream_t st);
+/// Pass 2: columns and value ids (width 1 or 2) or full values (width 0: complex Zd, or double
+/// when real) at row_ptr.
+void csr_fill(const SectorView& R, const SectorView& C, bool same, const ProgramView& P, const u64* row_ptr,
+              u32* col, void* vals, int width, bool real, const DictTable& D, hipStream_t st);
+std::size_t row_scan_temp_bytes(i64 rows);
+void row_scan(const u32* row_len, i64 rows, u64* row_ptr, void* temp, std::size_t temp_bytes, hipStream_t st);
+
+// ---- spmv.hip --------------------------------------------------------------------------------
+struct CsrView {
+    i64 rows = 0;
+    const u64* row_ptr = nullptr;
+    const u32* col = nullptr;
+    const void* vals = nullptr;    // u8 / u16 ids, or full values
+    int width = 0;                 // 1: u8 ids, 2: u16 ids, 0: full values
+    const double* dict_re = nullptr;
+    const double* dict_im = nullptr;
+    int lanes = 16;                // lanes per row: 4, 8, 16, 32 or 64
+    bool values_real = false;      // the stored values are real (full values stored as double)
+};
+/// out_v = A in_v (real: double vectors; else complex vectors, with real or complex values).
+void csr_spmv(const CsrView& A, bool real, const VecSet& v, hipStream_t st);
+/// Block-interleaved multi-vector SpMV: X, Y are rows x nv, entry (row, vector) at row * nv + vector.
+void csr_spmv_block(const CsrView& A, bool real, int nv, const void* X, void* Y, hipStream_t st);
+
+// ---- blas.hip (vectors of T = double or Zd; n entries) -------------------------------------------
+/// Partial-sum scratch a reduction needs (device), sized by blas_scratch_bytes.
+std::size_t blas_scratch_bytes(std::size_t max_cols);
+void fill_zero(bool cplx, void* x, std::size_t n, hipStream_t st);
+void axpy(bool cplx, Zd a, const void* x, void* y, std::size_t n, hipStream_t st);
+void scale(bool cplx, Zd a, void* x, std::size_t n, hipStream_t st);
+void axpby(bool cplx, Zd a, const void* x, Zd b, void* y, std::size_t n, hipStream_t st);
+/// result (device, one Zd) = x^H y.
+void dot(bool cplx, const void* x, const void* y, std::size_t n, void* scratch, Zd* result, hipStream_t st);
+/// result (device, one double) = ||x||.
+void nrm2(bool cplx, const void* x, std::size_t n, void* scratch, double* result, hipStream_t st);
+/// y += a x; result = z^H y.
+void axpy_dot(bool cplx, Zd a, const void* x, void* y, const void* z, std::size_t n, void* scratch, Zd* result,
+              hipStream_t st);
+/// y += a x; result = ||y||.
+void axpy_nrm2(bool cplx, Zd a, const void* x, void* y, std::size_t n, void* scratch, double* result,
+               hipStream_t st);
+/// coeffs[j] = V_j^H w for the m columns of the block V (column j at V + j ld); also ||w||^2 into
+/// *wnorm2 when non-null.
+void dot_many(bool cplx, const void* V, std::size_t ld, std::size_t m, const void* w, std::size_t n, void* scratch,
+              Zd* coeffs, double* wnorm2, hipStream_t st);
+/// w += sign sum_j coeffs[j] V_j (coeffs in device memory); ||w||^2 after the update into *wnorm2
+/// when non-null.
+void axpy_many(bool cplx, const Zd* coeffs, const void* V, std::size_t ld, std::size_t m, void* w, std::size_t n,
+               void* scratch, double* wnorm2, hipStream_t st, double sign = 1.0);
+
+}  // namespace qsa::dev
+
+namespace qsa::dev {
+/// out[:, c] = sum_r V[:, r] S[r + c mm] for c < cols (S real, column-major, device memory).
+void block_rotate(bool cplx, const void* V, std::size_t ld, std::size_t mm, const double* S, std::size_t cols,
+                  void* out, std::size_t n, hipStream_t st);
+}  // namespace qsa::dev
+
+namespace qsa::dev {
+/// index_and_projection of n states (device arrays): idx[i], proj[2 i], proj[2 i + 1].
+void index_proj_batch(const SectorView& S, const u64* states, i64 n, i64* idx, double* proj, hipStream_t st);
+}  // namespace qsa::dev

@@ -1,0 +1,218 @@
+// Reduced-CSR SpMV for the symmetry-sector operator, real or complex, one to eight vectors.
+//
+// L lanes of a wavefront share a row: lane l takes entries e0 + l, e0 + l + L, ... (coalesced
+// column / id streams), and the L partial sums meet through a fixed xor-butterfly, so the result
+// is deterministic and a k-vector apply equals k single applies bit for bit (the contract of the
+// reference's csr_spmv, src/gpu/rep_matvec.cu:1003-1036). Values come from a dictionary by u8 or
+// u16 id (the reference's reduced_csr.h layout) or are stored in full.
+#include "dev_common.h"
+#include "launch.h"
+
+namespace qsa::dev {
+
+namespace {
+
+template <int NV> struct Ptrs {
+    const void* in[NV];
+    void* out[NV];
+};
+
+template <int WIDTH, bool VREAL>
+__device__ __forceinline__ void value_at(const CsrView& A, u64 e, double& vr, double& vi) {
+    if constexpr (WIDTH == 1) {
+        const u32 id = static_cast<const u8*>(A.vals)[e];
+        vr = A.dict_re[id];
+        if constexpr (!VREAL) vi = A.dict_im[id];
+    } else if constexpr (WIDTH == 2) {
+        const u32 id = static_cast<const u16*>(A.vals)[e];
+        vr = A.dict_re[id];
+        if constexpr (!VREAL) vi = A.dict_im[id];
+    } else {
+        if constexpr (VREAL) {
+            vr = static_cast<const double*>(A.vals)[e];
+        } else {
+            const Zd z = static_cast<const Zd*>(A.vals)[e];
+            vr = z.re;
+            vi = z.im;
+        }
+    }
+}
+
+template <int WIDTH, bool VREAL, bool REAL, int L, int NV>
+__global__ __launch_bounds__(256) void k_spmv(CsrView A, Ptrs<NV> v) {
+    const i64 gid = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const i64 row = gid / L;
+    const int lane = static_cast<int>(gid % L);
+    double ar[NV], ai[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) ar[k] = ai[k] = 0.0;
+    if (row < A.rows) {
+        const u64 e0 = A.row_ptr[row], e1 = A.row_ptr[row + 1];
+        for (u64 e = e0 + lane; e < e1; e += L) {
+            const u32 c = A.col[e];
+            double vr, vi = 0.0;
+            value_at<WIDTH, VREAL>(A, e, vr, vi);
+#pragma unroll
+            for (int k = 0; k < NV; ++k) {
+                if constexpr (REAL) {
+                    ar[k] += vr * static_cast<const double*>(v.in[k])[c];
+                } else {
+                    const Zd x = static_cast<const Zd*>(v.in[k])[c];
+                    ar[k] += vr * x.re - vi * x.im;
+                    ai[k] += vr * x.im + vi * x.re;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int off = L / 2; off > 0; off >>= 1) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            ar[k] += __shfl_xor(ar[k], off, L);
+            if constexpr (!REAL) ai[k] += __shfl_xor(ai[k], off, L);
+        }
+    }
+    if (row < A.rows && lane == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            if constexpr (REAL) static_cast<double*>(v.out[k])[row] = ar[k];
+            else static_cast<Zd*>(v.out[k])[row] = Zd{ar[k], ai[k]};
+        }
+    }
+}
+
+// Block-interleaved vectors: X[row * NV + k].
+template <int WIDTH, bool VREAL, bool REAL, int L, int NV>
+__global__ __launch_bounds__(256) void k_spmv_block(CsrView A, const void* X, void* Y) {
+    const i64 gid = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const i64 row = gid / L;
+    const int lane = static_cast<int>(gid % L);
+    double ar[NV], ai[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) ar[k] = ai[k] = 0.0;
+    if (row < A.rows) {
+        const u64 e0 = A.row_ptr[row], e1 = A.row_ptr[row + 1];
+        for (u64 e = e0 + lane; e < e1; e += L) {
+            const u64 c = A.col[e];
+            double vr, vi = 0.0;
+            value_at<WIDTH, VREAL>(A, e, vr, vi);
+#pragma unroll
+            for (int k = 0; k < NV; ++k) {
+                if constexpr (REAL) {
+                    ar[k] += vr * static_cast<const double*>(X)[c * NV + k];
+                } else {
+                    const Zd x = static_cast<const Zd*>(X)[c * NV + k];
+                    ar[k] += vr * x.re - vi * x.im;
+                    ai[k] += vr * x.im + vi * x.re;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int off = L / 2; off > 0; off >>= 1) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            ar[k] += __shfl_xor(ar[k], off, L);
+            if constexpr (!REAL) ai[k] += __shfl_xor(ai[k], off, L);
+        }
+    }
+    if (row < A.rows && lane == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            if constexpr (REAL) static_cast<double*>(Y)[row * NV + k] = ar[k];
+            else static_cast<Zd*>(Y)[row * NV + k] = Zd{ar[k], ai[k]};
+        }
+    }
+}
+
+template <int WIDTH, bool VREAL, bool REAL, int L> void launch_set(const CsrView& A, const VecSet& v, hipStream_t st) {
+    const u64 threads = static_cast<u64>(A.rows) * L;
+    const dim3 grid(static_cast<unsigned>((threads + 255) / 256)), block(256);
+    int done = 0;
+    while (done < v.k) {
+        const int rest = v.k - done;
+        const int nv = rest >= 8 ? 8 : rest >= 4 ? 4 : rest >= 2 ? 2 : 1;
+        auto go = [&](auto tag) {
+            constexpr int NV = decltype(tag)::value;
+            Ptrs<NV> p;
+            for (int k = 0; k < NV; ++k) {
+                p.in[k] = v.in[done + k];
+                p.out[k] = v.out[done + k];
+            }
+            hipLaunchKernelGGL((k_spmv<WIDTH, VREAL, REAL, L, NV>), grid, block, 0, st, A, p);
+        };
+        if (nv == 8) go(std::integral_constant<int, 8>{});
+        else if (nv == 4) go(std::integral_constant<int, 4>{});
+        else if (nv == 2) go(std::integral_constant<int, 2>{});
+        else go(std::integral_constant<int, 1>{});
+        done += nv;
+    }
+}
+
+template <int WIDTH, bool VREAL, bool REAL> void pick_lanes(const CsrView& A, const VecSet& v, hipStream_t st) {
+    switch (A.lanes) {
+    case 4: launch_set<WIDTH, VREAL, REAL, 4>(A, v, st); break;
+    case 8: launch_set<WIDTH, VREAL, REAL, 8>(A, v, st); break;
+    case 32: launch_set<WIDTH, VREAL, REAL, 32>(A, v, st); break;
+    case 64: launch_set<WIDTH, VREAL, REAL, 64>(A, v, st); break;
+    default: launch_set<WIDTH, VREAL, REAL, 16>(A, v, st); break;
+    }
+}
+
+template <int WIDTH, bool VREAL, bool REAL, int L>
+void launch_block(const CsrView& A, int nv, const void* X, void* Y, hipStream_t st) {
+    const u64 threads = static_cast<u64>(A.rows) * L;
+    const dim3 grid(static_cast<unsigned>((threads + 255) / 256)), block(256);
+    switch (nv) {
+    case 1: hipLaunchKernelGGL((k_spmv_block<WIDTH, VREAL, REAL, L, 1>), grid, block, 0, st, A, X, Y); break;
+    case 2: hipLaunchKernelGGL((k_spmv_block<WIDTH, VREAL, REAL, L, 2>), grid, block, 0, st, A, X, Y); break;
+    case 4: hipLaunchKernelGGL((k_spmv_block<WIDTH, VREAL, REAL, L, 4>), grid, block, 0, st, A, X, Y); break;
+    case 8: hipLaunchKernelGGL((k_spmv_block<WIDTH, VREAL, REAL, L, 8>), grid, block, 0, st, A, X, Y); break;
+    default: break;
+    }
+}
+
+template <int WIDTH, bool VREAL, bool REAL> void pick_lanes_block(const CsrView& A, int nv, const void* X, void* Y, hipStream_t st) {
+    switch (A.lanes) {
+    case 4: launch_block<WIDTH, VREAL, REAL, 4>(A, nv, X, Y, st); break;
+    case 8: launch_block<WIDTH, VREAL, REAL, 8>(A, nv, X, Y, st); break;
+    case 32: launch_block<WIDTH, VREAL, REAL, 32>(A, nv, X, Y, st); break;
+    case 64: launch_block<WIDTH, VREAL, REAL, 64>(A, nv, X, Y, st); break;
+    default: launch_block<WIDTH, VREAL, REAL, 16>(A, nv, X, Y, st); break;
+    }
+}
+
+template <bool BLOCK>
+void dispatch(const CsrView& A, bool vreal, bool real, const VecSet* v, int nv, const void* X, void* Y, hipStream_t st) {
+    // real vectors need real values; complex vectors take either
+    auto go = [&](auto w, auto vr, auto xr) {
+        constexpr int W = decltype(w)::value;
+        constexpr bool VR = decltype(vr)::value, XR = decltype(xr)::value;
+        if constexpr (BLOCK) pick_lanes_block<W, VR, XR>(A, nv, X, Y, st);
+        else pick_lanes<W, VR, XR>(A, *v, st);
+    };
+    using T = std::true_type;
+    using F = std::false_type;
+    auto by_width = [&](auto vr, auto xr) {
+        if (A.width == 1) go(std::integral_constant<int, 1>{}, vr, xr);
+        else if (A.width == 2) go(std::integral_constant<int, 2>{}, vr, xr);
+        else go(std::integral_constant<int, 0>{}, vr, xr);
+    };
+    if (real) by_width(T{}, T{});
+    else if (vreal) by_width(T{}, F{});
+    else by_width(F{}, F{});
+}
+
+}  // namespace
+
+void csr_spmv(const CsrView& A, bool real, const VecSet& v, hipStream_t st) {
+    if (A.rows == 0 || v.k == 0) return;
+    dispatch<false>(A, A.values_real, real, &v, 0, nullptr, nullptr, st);
+}
+
+void csr_spmv_block(const CsrView& A, bool real, int nv, const void* X, void* Y, hipStream_t st) {
+    if (A.rows == 0) return;
+    dispatch<true>(A, A.values_real, real, nullptr, nv, X, Y, st);
+}
+
+}  // namespace qsa::dev

@@ -1,0 +1,340 @@
+// Term application in a symmetry sector: the on-the-fly row walk (gather lane) and the two passes
+// of the merged reduced-CSR build.
+//
+// A row r of O between sector R (rows) and sector C (columns) comes from the walk of the program
+// of O^dagger from R's representative s_r: each acting group gives its target t and h, the target
+// is reduced to its orbit representative with the character phase (index_and_projection), and the
+// entry inv_norm[r] conj(h proj) lands at the target's index (sector_rows.h:41-56). The gather lane
+// sums entries in emission order; the CSR build merges each row first -- columns sorted (stable),
+// equal columns summed in emission order, exact zeros dropped (sector_rows.h:147-158) -- so its
+// nnz and values are the reference's merged CSR entry for entry.
+#include <hipcub/hipcub.hpp>
+
+#include "dev_common.h"
+#include "launch.h"
+
+namespace qsa::dev {
+
+namespace {
+
+constexpr int kMaxRow = 128;     // entries of one row before merging (= acting groups)
+constexpr int kLdsSlots = 512;   // per-block value dictionary
+
+inline unsigned blocks_for(u64 n, unsigned t) { return static_cast<unsigned>((n + t - 1) / t); }
+
+template <int NV> struct Ptrs {
+    const void* in[NV];
+    void* out[NV];
+};
+
+template <bool SLC, bool REAL, int NV>
+__global__ __launch_bounds__(256) void k_gather(SectorView R, SectorView C, int same, ProgramView P, Ptrs<NV> v) {
+    const i64 r = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= R.dim) return;
+    const u64 s = R.reps[r];
+    const double w = R.inv_norms[r];
+    double ar[NV], ai[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) ar[k] = ai[k] = 0.0;
+    for (u32 g = 0; g < P.n_groups; ++g) {
+        u64 t;
+        double hr, hi;
+        if (!connection(P, s, g, t, hr, hi)) continue;
+        double vr, vi;
+        const i64 c = entry_of<SLC>(C, same != 0, r, s, w, t, hr, hi, vr, vi);
+        if (c < 0) continue;
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            if constexpr (REAL) {
+                ar[k] += vr * static_cast<const double*>(v.in[k])[c];
+            } else {
+                const Zd x = static_cast<const Zd*>(v.in[k])[c];
+                ar[k] += vr * x.re - vi * x.im;
+                ai[k] += vr * x.im + vi * x.re;
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        if constexpr (REAL) static_cast<double*>(v.out[k])[r] = ar[k];
+        else static_cast<Zd*>(v.out[k])[r] = Zd{ar[k], ai[k]};
+    }
+}
+
+template <bool SLC, bool REAL>
+void launch_gather(const SectorView& R, const SectorView& C, bool same, const ProgramView& P, const VecSet& v,
+                   hipStream_t st) {
+    const dim3 grid(blocks_for(static_cast<u64>(R.dim), 256)), block(256);
+    int done = 0;
+    while (done < v.k) {
+        const int rest = v.k - done;
+        const int nv = rest >= 8 ? 8 : rest >= 4 ? 4 : rest >= 2 ? 2 : 1;
+        auto go = [&](auto tag) {
+            constexpr int NV = decltype(tag)::value;
+            Ptrs<NV> p;
+            for (int k = 0; k < NV; ++k) {
+                p.in[k] = v.in[done + k];
+                p.out[k] = v.out[done + k];
+            }
+            hipLaunchKernelGGL((k_gather<SLC, REAL, NV>), grid, block, 0, st, R, C, same ? 1 : 0, P, p);
+        };
+        if (nv == 8) go(std::integral_constant<int, 8>{});
+        else if (nv == 4) go(std::integral_constant<int, 4>{});
+        else if (nv == 2) go(std::integral_constant<int, 2>{});
+        else go(std::integral_constant<int, 1>{});
+        done += nv;
+    }
+}
+
+/// The merged entries of row r: columns ascending, equal columns summed in emission order, exact
+/// zeros dropped. Returns the count.
+template <bool SLC>
+__device__ int merged_row(const SectorView& R, const SectorView& C, bool same, const ProgramView& P, i64 r, u32* cols,
+                          double* vre, double* vim) {
+    const u64 s = R.reps[r];
+    const double w = R.inv_norms[r];
+    int n = 0;
+    for (u32 g = 0; g < P.n_groups; ++g) {
+        u64 t;
+        double hr, hi;
+        if (!connection(P, s, g, t, hr, hi)) continue;
+        double vr, vi;
+        const i64 c = entry_of<SLC>(C, same, r, s, w, t, hr, hi, vr, vi);
+        if (c < 0) continue;
+        // stable insertion: after every entry with a column <= c
+        int j = n;
+        while (j > 0 && cols[j - 1] > static_cast<u32>(c)) {
+            cols[j] = cols[j - 1];
+            vre[j] = vre[j - 1];
+            vim[j] = vim[j - 1];
+            --j;
+        }
+        cols[j] = static_cast<u32>(c);
+        vre[j] = vr;
+        vim[j] = vi;
+        ++n;
+    }
+    int o = 0;
+    for (int i = 0; i < n;) {
+        double sr = vre[i], si = vim[i];
+        int k = i + 1;
+        for (; k < n && cols[k] == cols[i]; ++k) {
+            sr += vre[k];
+            si += vim[k];
+        }
+        if (!(sr == 0.0 && si == 0.0)) {
+            cols[o] = cols[i];
+            vre[o] = sr;
+            vim[o] = si;
+            ++o;
+        }
+        i = k;
+    }
+    return o;
+}
+
+__device__ __forceinline__ u64 bits_of(double x) { return static_cast<u64>(__double_as_longlong(x)); }
+
+__device__ __forceinline__ u64 fingerprint(u64 re, u64 im) {
+    u64 z = re * 0x9E3779B97F4A7C15ULL ^ (im + 0x632BE59BD9B4E019ULL + (re << 7));
+    z ^= z >> 31;
+    z *= 0xBF58476D1CE4E5B9ULL;
+    z ^= z >> 29;
+    z *= 0x94D049BB133111EBULL;
+    z ^= z >> 32;
+    return z | 1ULL;
+}
+
+template <bool SLC>
+__global__ __launch_bounds__(256) void k_csr_count(SectorView R, SectorView C, int same, ProgramView P, u32* row_len,
+                                                   DictTable D, int use_dict) {
+    __shared__ unsigned long long s_fp[kLdsSlots];
+    __shared__ u64 s_re[kLdsSlots], s_im[kLdsSlots];
+    __shared__ int s_full;
+    for (int i = threadIdx.x; i < kLdsSlots; i += blockDim.x) s_fp[i] = 0ULL;
+    if (threadIdx.x == 0) s_full = 0;
+    __syncthreads();
+    const i64 r = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r < R.dim) {
+        u32 cols[kMaxRow];
+        double vre[kMaxRow], vim[kMaxRow];
+        const int n = merged_row<SLC>(R, C, same != 0, P, r, cols, vre, vim);
+        row_len[r] = static_cast<u32>(n);
+        if (use_dict) {
+            for (int e = 0; e < n; ++e) {
+                const u64 re = bits_of(vre[e]), im = bits_of(vim[e]);
+                const u64 fp = fingerprint(re, im);
+                const int h = static_cast<int>((fp >> 17) & (kLdsSlots - 1));
+                bool placed = false;
+                for (int p = 0; p < kLdsSlots && !placed; ++p) {
+                    const int i = (h + p) & (kLdsSlots - 1);
+                    const unsigned long long old = atomicCAS(&s_fp[i], 0ULL, static_cast<unsigned long long>(fp));
+                    if (old == 0ULL) {
+                        s_re[i] = re;
+                        s_im[i] = im;
+                        placed = true;
+                    } else if (old == fp) {
+                        placed = true;
+                    }
+                }
+                if (!placed) s_full = 1;
+            }
+        }
+    }
+    __syncthreads();
+    if (!use_dict) return;
+    if (s_full) {
+        if (threadIdx.x == 0) atomicOr(D.flags, 1u);
+        return;
+    }
+    for (int i = threadIdx.x; i < kLdsSlots; i += blockDim.x) {
+        const u64 fp = s_fp[i];
+        if (!fp) continue;
+        const int h = static_cast<int>((fp >> 17) & (DictTable::kSlots - 1));
+        bool placed = false;
+        for (int p = 0; p < 4096 && !placed; ++p) {
+            const int j = (h + p) & (DictTable::kSlots - 1);
+            const unsigned long long old =
+                atomicCAS(reinterpret_cast<unsigned long long*>(&D.fp[j]), 0ULL, static_cast<unsigned long long>(fp));
+            if (old == 0ULL) {
+                D.re[j] = s_re[i];
+                D.im[j] = s_im[i];
+                atomicAdd(D.count, 1u);
+                placed = true;
+            } else if (old == fp) {
+                placed = true;
+            }
+        }
+        if (!placed) atomicOr(D.flags, 1u);
+    }
+}
+
+template <bool SLC, int WIDTH, bool REAL>
+__global__ __launch_bounds__(256) void k_csr_fill(SectorView R, SectorView C, int same, ProgramView P,
+                                                  const u64* row_ptr, u32* col, void* vals, DictTable D) {
+    const i64 r = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= R.dim) return;
+    u32 cols[kMaxRow];
+    double vre[kMaxRow], vim[kMaxRow];
+    const int n = merged_row<SLC>(R, C, same != 0, P, r, cols, vre, vim);
+    const u64 e0 = row_ptr[r];
+    for (int e = 0; e < n; ++e) {
+        col[e0 + e] = cols[e];
+        if constexpr (WIDTH == 0) {
+            if constexpr (REAL) static_cast<double*>(vals)[e0 + e] = vre[e];
+            else static_cast<Zd*>(vals)[e0 + e] = Zd{vre[e], vim[e]};
+        } else {
+            const u64 re = bits_of(vre[e]), im = bits_of(vim[e]);
+            const u64 fp = fingerprint(re, im);
+            const int h = static_cast<int>((fp >> 17) & (DictTable::kSlots - 1));
+            int slot = -1;
+            for (int p = 0; p < 4096; ++p) {
+                const int j = (h + p) & (DictTable::kSlots - 1);
+                const u64 f = D.fp[j];
+                if (f == fp) {
+                    slot = j;
+                    break;
+                }
+                if (f == 0) break;
+            }
+            u32 id = 0;
+            if (slot < 0 || D.re[slot] != re || D.im[slot] != im) atomicOr(D.flags, 2u);
+            else id = D.slot_id[slot];
+            if constexpr (WIDTH == 1) static_cast<u8*>(vals)[e0 + e] = static_cast<u8>(id);
+            else static_cast<u16*>(vals)[e0 + e] = static_cast<u16>(id);
+        }
+    }
+}
+
+}  // namespace
+
+void walk_gather(const SectorView& R, const SectorView& C, bool same, const ProgramView& P, bool real,
+                 const VecSet& v, hipStream_t st) {
+    if (R.dim == 0 || v.k == 0) return;
+    const bool slc = C.slc_L > 0;
+    if (slc && real) launch_gather<true, true>(R, C, same, P, v, st);
+    else if (slc) launch_gather<true, false>(R, C, same, P, v, st);
+    else if (real) launch_gather<false, true>(R, C, same, P, v, st);
+    else launch_gather<false, false>(R, C, same, P, v, st);
+}
+
+bool csr_count(const SectorView& R, const SectorView& C, bool same, const ProgramView& P, u32* row_len,
+               const DictTable& D, bool use_dict, hipStream_t st) {
+    if (P.n_groups > static_cast<u32>(kMaxRow)) return false;
+    if (R.dim == 0) return true;
+    const dim3 grid(blocks_for(static_cast<u64>(R.dim), 256)), block(256);
+    if (C.slc_L > 0)
+        hipLaunchKernelGGL(k_csr_count<true>, grid, block, 0, st, R, C, same ? 1 : 0, P, row_len, D, use_dict ? 1 : 0);
+    else
+        hipLaunchKernelGGL(k_csr_count<false>, grid, block, 0, st, R, C, same ? 1 : 0, P, row_len, D,
+                           use_dict ? 1 : 0);
+    return true;
+}
+
+void csr_fill(const SectorView& R, const SectorView& C, bool same, const ProgramView& P, const u64* row_ptr,
+              u32* col, void* vals, int width, bool real, const DictTable& D, hipStream_t st) {
+    if (R.dim == 0) return;
+    const dim3 grid(blocks_for(static_cast<u64>(R.dim), 256)), block(256);
+    const int sm = same ? 1 : 0;
+    const bool slc = C.slc_L > 0;
+#define QSA_FILL(SL, W, RE) \
+    hipLaunchKernelGGL((k_csr_fill<SL, W, RE>), grid, block, 0, st, R, C, sm, P, row_ptr, col, vals, D)
+    if (slc) {
+        if (width == 1) QSA_FILL(true, 1, false);
+        else if (width == 2) QSA_FILL(true, 2, false);
+        else if (real) QSA_FILL(true, 0, true);
+        else QSA_FILL(true, 0, false);
+    } else {
+        if (width == 1) QSA_FILL(false, 1, false);
+        else if (width == 2) QSA_FILL(false, 2, false);
+        else if (real) QSA_FILL(false, 0, true);
+        else QSA_FILL(false, 0, false);
+    }
+#undef QSA_FILL
+}
+
+namespace {
+struct ToU64 {
+    __host__ __device__ u64 operator()(u32 x) const { return static_cast<u64>(x); }
+};
+}  // namespace
+
+std::size_t row_scan_temp_bytes(i64 rows) {
+    std::size_t bytes = 0;
+    hipcub::TransformInputIterator<u64, ToU64, const u32*> it(nullptr, ToU64{});
+    hipcub::DeviceScan::InclusiveSum(nullptr, bytes, it, static_cast<u64*>(nullptr), static_cast<int>(rows));
+    return bytes;
+}
+
+// row_ptr[0] = 0, row_ptr[r + 1] = sum of row_len[0..r].
+void row_scan(const u32* row_len, i64 rows, u64* row_ptr, void* temp, std::size_t temp_bytes, hipStream_t st) {
+    hipMemsetAsync(row_ptr, 0, sizeof(u64), st);
+    if (rows == 0) return;
+    hipcub::TransformInputIterator<u64, ToU64, const u32*> it(row_len, ToU64{});
+    hipcub::DeviceScan::InclusiveSum(temp, temp_bytes, it, row_ptr + 1, static_cast<int>(rows), st);
+}
+
+}  // namespace qsa::dev
+
+namespace qsa::dev {
+
+namespace {
+template <bool SLC>
+__global__ void k_index_proj(SectorView S, const u64* states, i64 n, i64* idx, double* proj) {
+    const i64 i = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double pr = 0.0, pi = 0.0;
+    idx[i] = index_and_projection<SLC>(S, states[i], pr, pi);
+    proj[2 * i] = pr;
+    proj[2 * i + 1] = pi;
+}
+}  // namespace
+
+void index_proj_batch(const SectorView& S, const u64* states, i64 n, i64* idx, double* proj, hipStream_t st) {
+    if (n <= 0) return;
+    const dim3 grid(static_cast<unsigned>((n + 255) / 256)), block(256);
+    if (S.slc_L > 0) hipLaunchKernelGGL(k_index_proj<true>, grid, block, 0, st, S, states, n, idx, proj);
+    else hipLaunchKernelGGL(k_index_proj<false>, grid, block, 0, st, S, states, n, idx, proj);
+}
+
+}  // namespace qsa::dev

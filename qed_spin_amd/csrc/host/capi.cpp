@@ -1,0 +1,460 @@
+// The C-ABI (include/qedh.h): opaque handles over the host engine, exceptions to status codes.
+#include "../../../include/qedh.h"
+
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "algebra.h"
+#include "krylov.h"
+#include "sector.h"
+#include "sector_operator.h"
+#include "spectral.h"
+
+struct qedh_operator_s {
+    qsa::SpinOperator op;
+};
+struct qedh_program_s {
+    qsa::Program prog;
+    std::shared_ptr<qsa::DeviceProgram> dev;   // uploaded on first use
+};
+struct qedh_sector_s {
+    std::shared_ptr<qsa::DeviceSector> sector;
+};
+struct qedh_matvec_s {
+    std::shared_ptr<qsa::SectorOperator> op;
+};
+
+namespace {
+
+thread_local std::string g_error;
+
+template <class F> int guarded(F&& f) {
+    try {
+        g_error.clear();
+        f();
+        return qsa::kOk;
+    } catch (const qsa::Error& e) {
+        g_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc& e) {
+        g_error = std::string("host allocation failed: ") + e.what();
+        return qsa::kResourceLimit;
+    } catch (const std::invalid_argument& e) {
+        g_error = e.what();
+        return qsa::kInvalid;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return qsa::kRuntime;
+    } catch (...) {
+        g_error = "unknown error";
+        return qsa::kRuntime;
+    }
+}
+
+void need(const void* p, const char* what) {
+    if (!p) throw qsa::Error(qsa::kInvalid, std::string(what) + " is null");
+}
+
+std::shared_ptr<qsa::DeviceProgram> device_program(qedh_program p) {
+    if (!p->dev) p->dev = std::make_shared<qsa::DeviceProgram>(p->prog);
+    return p->dev;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* qedh_last_error(void) { return g_error.c_str(); }
+
+int qedh_device_count(int* count) {
+    return guarded([&] {
+        need(count, "count");
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess) {
+            (void)hipGetLastError();
+            n = 0;
+        }
+        *count = n;
+    });
+}
+
+int qedh_malloc(void** ptr, size_t bytes) {
+    return guarded([&] {
+        need(ptr, "ptr");
+        qsa::require_device();
+        *ptr = nullptr;
+        if (bytes) QSA_HIP(hipMalloc(ptr, bytes));
+    });
+}
+int qedh_free(void* ptr) {
+    return guarded([&] {
+        if (ptr) QSA_HIP(hipFree(ptr));
+    });
+}
+int qedh_memcpy_htod(void* dst, const void* src, size_t bytes) {
+    return guarded([&] {
+        if (bytes) QSA_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    });
+}
+int qedh_memcpy_dtoh(void* dst, const void* src, size_t bytes) {
+    return guarded([&] {
+        if (bytes) QSA_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    });
+}
+int qedh_memset(void* dst, int value, size_t bytes) {
+    return guarded([&] {
+        if (bytes) QSA_HIP(hipMemset(dst, value, bytes));
+    });
+}
+int qedh_synchronize(void) {
+    return guarded([&] { QSA_HIP(hipDeviceSynchronize()); });
+}
+
+int qedh_operator_create(int n_sites, qedh_operator* out) {
+    return guarded([&] {
+        need(out, "out");
+        *out = new qedh_operator_s{qsa::SpinOperator(n_sites)};
+    });
+}
+int qedh_operator_add_product(qedh_operator op, const char* ops, const int* sites, double re, double im) {
+    return guarded([&] {
+        need(op, "operator");
+        need(ops, "ops");
+        const std::string s(ops);
+        std::vector<int> st(sites, sites + s.size());
+        op->op.add(qsa::SpinOperator::product(op->op.n_sites(), s, st, qsa::cplx(re, im)));
+    });
+}
+int qedh_operator_info(qedh_operator op, int64_t* n_terms, double* norm_bound, int* is_real) {
+    return guarded([&] {
+        need(op, "operator");
+        if (n_terms) *n_terms = static_cast<int64_t>(op->op.size());
+        if (norm_bound) *norm_bound = op->op.l1_norm();
+        if (is_real) *is_real = op->op.is_real() ? 1 : 0;
+    });
+}
+int qedh_operator_destroy(qedh_operator op) {
+    return guarded([&] { delete op; });
+}
+
+int qedh_program_compile(qedh_operator op, int adjoint, qedh_program* out) {
+    return guarded([&] {
+        need(op, "operator");
+        need(out, "out");
+        *out = new qedh_program_s{qsa::compile_operator(adjoint ? op->op.dagger() : op->op), nullptr};
+    });
+}
+int qedh_program_compile_projected(qedh_operator op, int adjoint, qedh_sector src, qedh_sector tgt,
+                                   qedh_program* out) {
+    return guarded([&] {
+        need(op, "operator");
+        need(src, "src");
+        need(tgt, "tgt");
+        need(out, "out");
+        const auto& a = src->sector->spec();
+        const auto& b = tgt->sector->spec();
+        if (a.perms != b.perms || a.flips != b.flips) throw qsa::Error(qsa::kInvalid, "program: sectors of different groups");
+        const qsa::SpinOperator O = adjoint ? op->op.dagger() : op->op;
+        *out = new qedh_program_s{
+            qsa::compile_projected(O, a.G, a.perms.data(), a.flips.data(), a.chars.data(), b.chars.data(), a.n_up, b.n_up),
+            nullptr};
+    });
+}
+int qedh_program_counts(qedh_program p, int64_t* n_groups, int64_t* n_vsub, int64_t* n_terms) {
+    return guarded([&] {
+        need(p, "program");
+        if (n_groups) *n_groups = static_cast<int64_t>(p->prog.n_groups());
+        if (n_vsub) *n_vsub = static_cast<int64_t>(p->prog.vsub_val.size());
+        if (n_terms) *n_terms = static_cast<int64_t>(p->prog.n_terms());
+    });
+}
+int qedh_program_export(qedh_program p, uint64_t* group_flip, int32_t* group_setbits, uint32_t* group_vbegin,
+                        uint64_t* vsub_val, uint32_t* vsub_tbegin, uint64_t* term_sign, double* term_coeff) {
+    return guarded([&] {
+        need(p, "program");
+        const auto& P = p->prog;
+        auto cp = [](auto* dst, const auto& v) {
+            if (dst && !v.empty()) std::memcpy(dst, v.data(), v.size() * sizeof(v[0]));
+        };
+        cp(group_flip, P.group_flip);
+        cp(group_setbits, P.group_setbits);
+        cp(group_vbegin, P.group_vbegin);
+        cp(vsub_val, P.vsub_val);
+        cp(vsub_tbegin, P.vsub_tbegin);
+        cp(term_sign, P.term_sign);
+        if (term_coeff)
+            for (std::size_t k = 0; k < P.term_coeff.size(); ++k) {
+                term_coeff[2 * k] = P.term_coeff[k].real();
+                term_coeff[2 * k + 1] = P.term_coeff[k].imag();
+            }
+    });
+}
+int qedh_program_destroy(qedh_program p) {
+    return guarded([&] { delete p; });
+}
+
+int qedh_sector_create(int n_sites, int n_up, int group_size, const int* perms, const uint64_t* flips,
+                       const double* chars, int sublattice_mode, int relaxed, uint64_t rank_budget_bytes,
+                       qedh_sector* out) {
+    return guarded([&] {
+        need(perms, "perms");
+        need(chars, "chars");
+        need(out, "out");
+        if (n_sites < 1 || n_sites > 63 || group_size < 1) throw qsa::Error(qsa::kInvalid, "sector: bad sizes");
+        qsa::GroupSpec g;
+        g.n_sites = n_sites;
+        g.n_up = n_up;
+        g.G = group_size;
+        g.perms.assign(perms, perms + static_cast<std::size_t>(group_size) * n_sites);
+        if (flips) g.flips.assign(flips, flips + group_size);
+        else g.flips.assign(group_size, 0ULL);
+        g.chars.resize(group_size);
+        for (int i = 0; i < group_size; ++i) g.chars[i] = qsa::cplx(chars[2 * i], chars[2 * i + 1]);
+        g.slc = qsa::build_sublattice(g.perms, g.flips, group_size, n_sites, sublattice_mode, relaxed != 0);
+        *out = new qedh_sector_s{std::make_shared<qsa::DeviceSector>(std::move(g), rank_budget_bytes)};
+    });
+}
+int qedh_sector_info(qedh_sector s, int64_t* dim, int64_t* n_orbits, int* sublattice_block, double* build_seconds) {
+    return guarded([&] {
+        need(s, "sector");
+        if (dim) *dim = s->sector->dim();
+        if (n_orbits) *n_orbits = s->sector->stats().orbits;
+        if (sublattice_block) *sublattice_block = s->sector->spec().slc.L;
+        if (build_seconds) *build_seconds = s->sector->stats().seconds;
+    });
+}
+int qedh_sector_export(qedh_sector s, uint64_t* reps, double* inv_norms) {
+    return guarded([&] {
+        need(s, "sector");
+        if (reps) {
+            const auto r = s->sector->reps();
+            std::memcpy(reps, r.data(), r.size() * 8);
+        }
+        if (inv_norms) {
+            const auto v = s->sector->inv_norms();
+            std::memcpy(inv_norms, v.data(), v.size() * 8);
+        }
+    });
+}
+int qedh_sector_key_bits(qedh_sector s, int* key_bit, int* block) {
+    return guarded([&] {
+        need(s, "sector");
+        const auto& slc = s->sector->spec().slc;
+        if (block) *block = slc.L;
+        if (key_bit && slc.engaged()) std::memcpy(key_bit, slc.key_bit.data(), slc.key_bit.size() * sizeof(int));
+    });
+}
+int qedh_sector_destroy(qedh_sector s) {
+    return guarded([&] { delete s; });
+}
+
+int qedh_matvec_create(qedh_sector rows, qedh_sector cols, qedh_program prog, double norm_bound, int lane,
+                       uint64_t csr_budget_bytes, qedh_matvec* out) {
+    return guarded([&] {
+        need(rows, "rows");
+        need(prog, "program");
+        need(out, "out");
+        auto c = (cols && cols != rows) ? cols->sector : rows->sector;
+        *out = new qedh_matvec_s{std::make_shared<qsa::SectorOperator>(rows->sector, c, device_program(prog), norm_bound,
+                                                                      lane, csr_budget_bytes)};
+    });
+}
+int qedh_matvec_info(qedh_matvec m, qedh_matvec_info_t* info) {
+    return guarded([&] {
+        need(m, "matvec");
+        need(info, "info");
+        const auto& o = *m->op;
+        const auto& c = o.csr_info();
+        info->rows = o.dim();
+        info->cols = o.col_dim();
+        info->is_real = o.is_real() ? 1 : 0;
+        info->lane = o.lane();
+        info->nnz = o.lane() == qsa::kLaneCsr ? c.nnz : 0;
+        info->value_bytes = c.width;
+        info->id_width_ref = static_cast<int>(c.id_width_ref);
+        info->dict_size = c.dict;
+        info->csr_bytes = c.bytes;
+        info->build_seconds = c.build_seconds;
+        info->lanes = c.lanes;
+        info->norm_bound = o.norm_bound();
+    });
+}
+int qedh_matvec_apply(qedh_matvec m, const void* in, void* out, int complex_vectors) {
+    return guarded([&] {
+        need(m, "matvec");
+        m->op->apply(in, out, complex_vectors != 0);
+    });
+}
+int qedh_matvec_apply_multi(qedh_matvec m, const void* const* ins, void* const* outs, int k, int complex_vectors) {
+    return guarded([&] {
+        need(m, "matvec");
+        m->op->apply_multi(ins, outs, k, complex_vectors != 0);
+    });
+}
+int qedh_matvec_apply_block(qedh_matvec m, int nv, const void* X, void* Y, int complex_vectors) {
+    return guarded([&] {
+        need(m, "matvec");
+        m->op->apply_block(nv, X, Y, complex_vectors != 0);
+    });
+}
+int qedh_matvec_set_lanes(qedh_matvec m, int lanes) {
+    return guarded([&] {
+        need(m, "matvec");
+        if (lanes != 4 && lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64)
+            throw qsa::Error(qsa::kInvalid, "lanes must be 4, 8, 16, 32 or 64");
+        m->op->set_lanes(lanes);
+    });
+}
+int qedh_matvec_export_csr(qedh_matvec m, uint64_t* row_ptr, uint32_t* col, double* val) {
+    return guarded([&] {
+        need(m, "matvec");
+        std::vector<std::uint64_t> rp;
+        std::vector<std::uint32_t> c;
+        std::vector<double> v;
+        m->op->csr_download(rp, c, v);
+        if (row_ptr) std::memcpy(row_ptr, rp.data(), rp.size() * 8);
+        if (col && !c.empty()) std::memcpy(col, c.data(), c.size() * 4);
+        if (val && !v.empty()) std::memcpy(val, v.data(), v.size() * 8);
+    });
+}
+int qedh_matvec_destroy(qedh_matvec m) {
+    return guarded([&] { delete m; });
+}
+
+int qedh_lanczos(qedh_matvec m, const void* v0_host, int complex_vectors, int max_iter, int reorth,
+                 double breakdown_tol, double* alpha, double* beta, int* iters, double* timings) {
+    return guarded([&] {
+        need(m, "matvec");
+        need(v0_host, "v0");
+        const auto& op = *m->op;
+        const std::size_t n = static_cast<std::size_t>(op.dim());
+        if (op.dim() != op.col_dim()) throw qsa::Error(qsa::kInvalid, "lanczos: the operator is not square");
+        auto run = [&](auto tag) {
+            using S = decltype(tag);
+            qsa::HipBackend<S> be;
+            auto v0 = be.make_zero_vector(n);
+            be.copy_from_host(static_cast<const S*>(v0_host), v0.get(), n);
+            qsa::LanczosOptions<S> o;
+            o.max_iter = static_cast<std::size_t>(max_iter);
+            o.reorth = static_cast<qsa::Reorth>(reorth);
+            o.keep_basis = o.reorth == qsa::Reorth::FullCGS2;
+            if (breakdown_tol > 0.0) o.breakdown_tol = breakdown_tol;
+            const bool cv = qsa::HipBackend<S>::kComplex;
+            auto r = qsa::lanczos_run(be, [&](const S* in, S* out) { op.apply(in, out, cv); }, n, v0.get(), o);
+            for (std::size_t j = 0; j < r.alpha.size(); ++j) alpha[j] = r.alpha[j];
+            for (std::size_t j = 0; j < r.beta.size(); ++j) beta[j] = r.beta[j];
+            *iters = static_cast<int>(r.iters);
+            if (timings) {
+                timings[0] = r.timings.apply_s;
+                timings[1] = r.timings.reorth_s;
+                timings[2] = r.timings.vector_s;
+                timings[3] = r.timings.total_s;
+            }
+        };
+        if (op.is_real() && !complex_vectors) run(double{});
+        else run(qsa::cplx{});
+    });
+}
+
+int qedh_block_lowest(qedh_matvec m, int k, int64_t max_iter, double* values, int* n_values, int* converged,
+                      int64_t* applies) {
+    return guarded([&] {
+        need(m, "matvec");
+        const auto& op = *m->op;
+        const std::size_t n = static_cast<std::size_t>(op.dim());
+        if (n == 0) {
+            *n_values = 0;
+            *converged = 1;
+            return;
+        }
+        const std::size_t kk = std::max<std::size_t>(1, std::min<std::size_t>(static_cast<std::size_t>(k), n));
+        auto run = [&](auto tag) {
+            using S = decltype(tag);
+            qsa::HipBackend<S> be;
+            const bool cv = qsa::HipBackend<S>::kComplex;
+            auto H = [&](const S* in, S* out) { op.apply(in, out, cv); };
+            qsa::BlockSolution sol;
+            const std::uint64_t budget = max_iter > 0 ? static_cast<std::uint64_t>(max_iter) : 0;
+            if (kk > 1) sol = qsa::krylov_schur_lane(be, H, n, kk, op.norm_bound(), budget);
+            else sol = qsa::lowest_scan(be, H, n, budget > 0 ? budget : std::max<std::uint64_t>(400, 0));
+            *n_values = static_cast<int>(sol.values.size());
+            for (std::size_t i = 0; i < sol.values.size(); ++i) values[i] = sol.values[i];
+            *converged = sol.converged ? 1 : 0;
+            if (applies) *applies = static_cast<int64_t>(sol.applies);
+        };
+        if (op.is_real()) run(double{});
+        else run(qsa::cplx{});
+    });
+}
+
+int qedh_ftlm(qedh_matvec m, int samples, int krylov, uint64_t seed, int n_beta, const double* betas,
+              double breakdown_tol, double* lnZ, double* E, double* V, double* e_min) {
+    return guarded([&] {
+        need(m, "matvec");
+        const auto& op = *m->op;
+        const std::size_t n = static_cast<std::size_t>(op.dim());
+        if (samples < 1 || krylov < 2 || n_beta < 1) throw qsa::Error(qsa::kInvalid, "ftlm: bad sizes");
+        qsa::HipBackend<qsa::cplx> be;
+        qsa::FtlmOptions o;
+        o.samples = static_cast<std::size_t>(samples);
+        o.krylov = static_cast<std::size_t>(krylov);
+        o.seed = seed;
+        o.betas.assign(betas, betas + n_beta);
+        o.breakdown_tol = breakdown_tol;
+        const auto c = qsa::ftlm_block(be, [&](const qsa::cplx* in, qsa::cplx* out) { op.apply(in, out, true); }, n, o);
+        for (int t = 0; t < n_beta; ++t) {
+            lnZ[t] = c.lnZ[t];
+            E[t] = c.E[t];
+            V[t] = c.V[t];
+        }
+        if (e_min) *e_min = c.e_min;
+    });
+}
+
+int qedh_cf(qedh_matvec m, const double* phi_host, int krylov, int n_omega, const double* omega, double eta,
+            int has_shift, double energy_shift, double breakdown_tol, double* S, double* out_shift, double* out_norm_sq,
+            double* out_change) {
+    return guarded([&] {
+        need(m, "matvec");
+        need(phi_host, "phi");
+        const auto& op = *m->op;
+        const std::size_t n = static_cast<std::size_t>(op.dim());
+        qsa::HipBackend<qsa::cplx> be;
+        auto phi = be.make_zero_vector(n);
+        be.copy_from_host(reinterpret_cast<const qsa::cplx*>(phi_host), phi.get(), n);
+        std::vector<double> w(omega, omega + n_omega);
+        const auto r = qsa::cf_from_vector(
+            be, [&](const qsa::cplx* in, qsa::cplx* out) { op.apply(in, out, true); }, n, phi.get(), w, eta,
+            static_cast<std::size_t>(krylov), has_shift != 0, energy_shift, breakdown_tol > 0.0 ? breakdown_tol : 1e-12);
+        for (int i = 0; i < n_omega; ++i) S[i] = r.S[i];
+        if (out_shift) *out_shift = r.energy_shift;
+        if (out_norm_sq) *out_norm_sq = r.norm_sq;
+        if (out_change) *out_change = r.convergence_change;
+    });
+}
+
+}  // extern "C"
+
+extern "C" int qedh_sector_index_and_projection(qedh_sector s, const uint64_t* states, int64_t n, int64_t* index,
+                                                double* proj) {
+    return guarded([&] {
+        need(s, "sector");
+        if (n <= 0) return;
+        auto st = qsa::upload(states, static_cast<std::size_t>(n));
+        qsa::DeviceBuffer di(static_cast<std::size_t>(n) * 8), dp(static_cast<std::size_t>(n) * 16);
+        qsa::dev::index_proj_batch(s->sector->view(), st.get<std::uint64_t>(), n, di.get<qsa::dev::i64>(),
+                                   dp.get<double>(), qsa::engine_stream());
+        QSA_HIP(hipGetLastError());
+        QSA_HIP(hipMemcpy(index, di.get(), static_cast<std::size_t>(n) * 8, hipMemcpyDeviceToHost));
+        QSA_HIP(hipMemcpy(proj, dp.get(), static_cast<std::size_t>(n) * 16, hipMemcpyDeviceToHost));
+    });
+}
+
+namespace qsa {
+void capi_set_error(const std::string& m) { g_error = m; }
+int capi_status(const std::exception& e) {
+    g_error = e.what();
+    return kRuntime;
+}
+}  // namespace qsa

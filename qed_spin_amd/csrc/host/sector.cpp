@@ -1,0 +1,194 @@
+#include "sector.h"
+
+#include <chrono>
+
+#include "../device/launch.h"
+#include "algebra.h"
+
+namespace qsa {
+
+namespace {
+std::uint64_t binom_host(int n, int k) {
+    if (k < 0 || n < 0 || k > n) return 0;
+    std::uint64_t r = 1;
+    for (int i = 1; i <= k; ++i) r = r * static_cast<std::uint64_t>(n - k + i) / static_cast<std::uint64_t>(i);
+    return r;
+}
+}  // namespace
+
+void require_device() {
+    int n = 0;
+    const hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) {
+        (void)hipGetLastError();
+        throw Error(kDeviceUnavailable, "no HIP device is visible");
+    }
+}
+
+DeviceSector::DeviceSector(GroupSpec spec, std::uint64_t rank_budget_bytes) : spec_(std::move(spec)) {
+    require_device();
+    const auto t0 = std::chrono::steady_clock::now();
+    const int N = spec_.n_sites, G = spec_.G;
+    if (N < 1 || N > 63) throw Error(kInvalid, "sector: n_sites must be in [1, 63]");
+    if (G < 1) throw Error(kInvalid, "sector: the group needs at least the identity");
+    if (static_cast<int>(spec_.perms.size()) != G * N || static_cast<int>(spec_.chars.size()) != G)
+        throw Error(kInvalid, "sector: permutations / characters do not match the group size");
+    if (spec_.flips.empty()) spec_.flips.assign(G, 0ULL);
+    if (spec_.n_up > N) throw Error(kInvalid, "sector: n_up > n_sites");
+    if (spec_.n_up < 0 && N > 40) throw Error(kDeviceUnsupported, "sector: a full-space sector above 40 sites");
+    for (int g = 0; g < G; ++g) {
+        std::uint64_t seen = 0;
+        for (int i = 0; i < N; ++i) {
+            const int p = spec_.perms[static_cast<std::size_t>(g) * N + i];
+            if (p < 0 || p >= N) throw Error(kInvalid, "sector: permutation entry out of range");
+            seen |= 1ULL << p;
+        }
+        if (seen != ((1ULL << N) - 1ULL)) throw Error(kInvalid, "sector: an element is not a permutation");
+        if (spec_.n_up >= 0 && spec_.flips[g] != 0 && 2 * spec_.n_up != N)
+            throw Error(kInvalid, "sector: a flip element needs n_up = N/2");
+        if ((spec_.flips[g] & ~((1ULL << N) - 1ULL)) != 0) throw Error(kInvalid, "sector: flip mask beyond n_sites");
+        if (spec_.n_up >= 0 && spec_.flips[g] != 0 && spec_.flips[g] != ((1ULL << N) - 1ULL))
+            throw Error(kInvalid, "sector: a fixed-Sz sector takes only the global spin flip");
+    }
+
+    const auto lut = spec_.byte_lut();
+    lut_ = upload(lut.data(), lut.size());
+    flips_ = upload(spec_.flips.data(), spec_.flips.size());
+    std::vector<double> cr(G), ci(G);
+    for (int g = 0; g < G; ++g) {
+        cr[g] = spec_.chars[g].real();
+        ci[g] = spec_.chars[g].imag();
+    }
+    chi_re_ = upload(cr.data(), cr.size());
+    chi_im_ = upload(ci.data(), ci.size());
+    std::vector<std::uint64_t> pascal(65 * 65, 0);
+    for (int n = 0; n <= 64; ++n)
+        for (int k = 0; k <= n; ++k) pascal[n * 65 + k] = binom_host(n, k);
+    binom_ = upload(pascal.data(), pascal.size());
+    if (spec_.slc.engaged()) {
+        to_key_ = upload(spec_.slc.to_key.data(), spec_.slc.to_key.size());
+        lead_ = upload(spec_.slc.lead.data(), spec_.slc.lead.size());
+        cand_off_ = upload(spec_.slc.cand_off.data(), spec_.slc.cand_off.size());
+        cand_ = upload(spec_.slc.cand.data(), spec_.slc.cand.size());
+    }
+
+    const std::uint64_t total = spec_.n_up >= 0 ? binom_host(N, spec_.n_up) : (1ULL << N);
+    stats_.subspace_dim = total;
+    const std::uint64_t n_words = (total + 63) / 64;
+    if (n_words > 0x7FFFFFFFULL) throw Error(kDeviceUnsupported, "sector: subspace beyond 2^37 states");
+    const hipStream_t st = engine_stream();
+    dev::SectorView v = view();
+    DeviceBuffer kept(n_words * 8), counts(n_words * 8), offsets(n_words * 8), orbits(8);
+    QSA_HIP(hipMemsetAsync(orbits.get(), 0, 8, st));
+    dev::orbit_flags(v, total, kept.get<std::uint64_t>(), orbits.get<unsigned long long>(), st);
+    QSA_HIP(hipGetLastError());
+    const std::size_t tb = dev::scan_words_temp_bytes(n_words);
+    DeviceBuffer temp(tb ? tb : 8);
+    dev::scan_words(kept.get<std::uint64_t>(), n_words, counts.get<std::uint64_t>(), offsets.get<std::uint64_t>(),
+                    temp.get(), tb, st);
+    QSA_HIP(hipGetLastError());
+    std::uint64_t last[2] = {0, 0};
+    unsigned long long n_orbits = 0;
+    QSA_HIP(hipMemcpyAsync(&last[0], offsets.get<std::uint64_t>() + (n_words - 1), 8, hipMemcpyDeviceToHost, st));
+    QSA_HIP(hipMemcpyAsync(&last[1], counts.get<std::uint64_t>() + (n_words - 1), 8, hipMemcpyDeviceToHost, st));
+    QSA_HIP(hipMemcpyAsync(&n_orbits, orbits.get(), 8, hipMemcpyDeviceToHost, st));
+    QSA_HIP(hipStreamSynchronize(st));
+    dim_ = static_cast<std::int64_t>(last[0] + last[1]);
+    stats_.orbits = static_cast<std::int64_t>(n_orbits);
+    if (dim_ > 0x7FFFFFFFLL) throw Error(kDeviceUnsupported, "sector: more than 2^31 representatives");
+    reps_.reset(static_cast<std::size_t>(dim_) * 8);
+    inv_norms_.reset(static_cast<std::size_t>(dim_) * 8);
+    if (dim_ > 0) {
+        dev::orbit_emit(v, total, kept.get<std::uint64_t>(), offsets.get<std::uint64_t>(), reps_.get<std::uint64_t>(),
+                        inv_norms_.get<double>(), st);
+        QSA_HIP(hipGetLastError());
+    }
+    const std::uint64_t table_bytes = total * 4;
+    if (dim_ > 0 && table_bytes <= rank_budget_bytes) {
+        rank_table_.reset(table_bytes);
+        QSA_HIP(hipMemsetAsync(rank_table_.get(), 0xFF, table_bytes, st));
+        dev::SectorView v2 = view();
+        v2.rank_table = nullptr;
+        dev::rank_fill(v2, rank_table_.get<dev::i32>(), st);
+        QSA_HIP(hipGetLastError());
+        stats_.rank_table = true;
+    }
+    QSA_HIP(hipStreamSynchronize(st));
+    stats_.device_bytes = lut_.bytes() + reps_.bytes() + inv_norms_.bytes() + rank_table_.bytes() + lead_.bytes() +
+                          cand_off_.bytes() + cand_.bytes();
+    stats_.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+dev::SectorView DeviceSector::view() const {
+    dev::SectorView v;
+    v.n_sites = spec_.n_sites;
+    v.n_up = spec_.n_up;
+    v.G = spec_.G;
+    v.bpw = spec_.bpw();
+    v.lut = lut_.get<std::uint64_t>();
+    v.flips = flips_.get<std::uint64_t>();
+    v.chi_re = chi_re_.get<double>();
+    v.chi_im = chi_im_.get<double>();
+    if (spec_.slc.engaged()) {
+        v.slc_L = spec_.slc.L;
+        v.slc_m = spec_.slc.m;
+        v.to_key = to_key_.get<std::uint64_t>();
+        v.lead = lead_.get<std::uint16_t>();
+        v.cand_off = cand_off_.get<std::uint32_t>();
+        v.cand = cand_.get<std::uint16_t>();
+    }
+    v.reps = reps_.get<std::uint64_t>();
+    v.inv_norms = inv_norms_.get<double>();
+    v.dim = dim_;
+    v.rank_table = rank_table_ ? rank_table_.get<dev::i32>() : nullptr;
+    v.binom = binom_.get<std::uint64_t>();
+    return v;
+}
+
+std::vector<std::uint64_t> DeviceSector::reps() const {
+    std::vector<std::uint64_t> h(static_cast<std::size_t>(dim_));
+    if (dim_) QSA_HIP(hipMemcpy(h.data(), reps_.get(), h.size() * 8, hipMemcpyDeviceToHost));
+    return h;
+}
+
+std::vector<double> DeviceSector::inv_norms() const {
+    std::vector<double> h(static_cast<std::size_t>(dim_));
+    if (dim_) QSA_HIP(hipMemcpy(h.data(), inv_norms_.get(), h.size() * 8, hipMemcpyDeviceToHost));
+    return h;
+}
+
+DeviceProgram::DeviceProgram(const Program& p) {
+    n_groups = static_cast<std::uint32_t>(p.n_groups());
+    n_terms = p.n_terms();
+    flip = upload(p.group_flip.data(), p.group_flip.size());
+    setbits = upload(p.group_setbits.data(), p.group_setbits.size());
+    vbegin = upload(p.group_vbegin.data(), p.group_vbegin.size());
+    vval = upload(p.vsub_val.data(), p.vsub_val.size());
+    vtbegin = upload(p.vsub_tbegin.data(), p.vsub_tbegin.size());
+    sign = upload(p.term_sign.data(), p.term_sign.size());
+    std::vector<double> re(n_terms), im(n_terms);
+    real = true;
+    for (std::size_t k = 0; k < n_terms; ++k) {
+        re[k] = p.term_coeff[k].real();
+        im[k] = p.term_coeff[k].imag();
+        if (im[k] != 0.0) real = false;
+    }
+    cre = upload(re.data(), re.size());
+    cim = upload(im.data(), im.size());
+}
+
+dev::ProgramView DeviceProgram::view() const {
+    dev::ProgramView v;
+    v.n_groups = n_groups;
+    v.group_flip = flip.get<std::uint64_t>();
+    v.group_setbits = setbits.get<dev::i32>();
+    v.group_vbegin = vbegin.get<std::uint32_t>();
+    v.vsub_val = vval.get<std::uint64_t>();
+    v.vsub_tbegin = vtbegin.get<std::uint32_t>();
+    v.term_sign = sign.get<std::uint64_t>();
+    v.coeff_re = cre.get<double>();
+    v.coeff_im = cim.get<double>();
+    return v;
+}
+
+}  // namespace qsa

@@ -1,0 +1,56 @@
+// A symmetry sector resident in HBM (the reference's RepSectorData + its device mirror,
+// include/ed/basis/rep_sector.h:134-408, src/gpu/rep_matvec.cu:253-383), built on the device.
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "../device/dev_common.h"
+#include "group.h"
+#include "hip_util.h"
+
+namespace qsa {
+
+struct SectorBuildStats {
+    double seconds = 0.0;
+    std::int64_t orbits = 0;           ///< orbits of the subspace before the character filter
+    std::uint64_t subspace_dim = 0;
+    std::uint64_t device_bytes = 0;
+    bool rank_table = false;
+};
+
+class DeviceSector {
+public:
+    /// Enumerate the sector of `spec` on the device. `rank_budget_bytes` caps the dense rank ->
+    /// index table (C(N, n_up) x 4 bytes); above it lookups binary-search the representatives.
+    DeviceSector(GroupSpec spec, std::uint64_t rank_budget_bytes);
+
+    const GroupSpec& spec() const { return spec_; }
+    std::int64_t dim() const { return dim_; }
+    const SectorBuildStats& stats() const { return stats_; }
+    dev::SectorView view() const;
+    bool real_characters() const { return spec_.real_characters(); }
+
+    /// Host copies (tests, vector expansion).
+    std::vector<std::uint64_t> reps() const;
+    std::vector<double> inv_norms() const;
+
+private:
+    GroupSpec spec_;
+    std::int64_t dim_ = 0;
+    SectorBuildStats stats_;
+    DeviceBuffer lut_, flips_, chi_re_, chi_im_, to_key_, lead_, cand_off_, cand_, binom_;
+    DeviceBuffer reps_, inv_norms_, rank_table_;
+};
+
+/// A compiled program in device memory.
+struct DeviceProgram {
+    explicit DeviceProgram(const struct Program& p);
+    dev::ProgramView view() const;
+    std::uint32_t n_groups = 0;
+    std::size_t n_terms = 0;
+    bool real = true;
+    DeviceBuffer flip, setbits, vbegin, vval, vtbegin, sign, cre, cim;
+};
+
+}  // namespace qsa

@@ -1,0 +1,209 @@
+#include "sector_operator.h"
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+
+namespace qsa {
+
+SectorOperator::SectorOperator(std::shared_ptr<const DeviceSector> rows, std::shared_ptr<const DeviceSector> cols,
+                               std::shared_ptr<const DeviceProgram> prog, double norm_bound, int lane,
+                               std::uint64_t csr_budget_bytes)
+    : rows_(std::move(rows)), cols_(cols ? std::move(cols) : rows_), prog_(std::move(prog)), norm_bound_(norm_bound) {
+    if (!rows_ || !prog_) throw Error(kInvalid, "SectorOperator: missing sector or program");
+    const auto& a = rows_->spec();
+    const auto& b = cols_->spec();
+    if (a.n_sites != b.n_sites || a.G != b.G || a.perms != b.perms || a.flips != b.flips)
+        throw Error(kInvalid, "SectorOperator: the two sectors come from different groups");
+    if (a.slc.engaged() != b.slc.engaged() || a.slc.key_bit != b.slc.key_bit)
+        throw Error(kInvalid, "SectorOperator: the two sectors use different representative orders");
+    real_ = prog_->real && rows_->real_characters() && cols_->real_characters();
+    lane_ = kLaneGather;
+    if (lane != kLaneGather && dim() > 0) build_csr(csr_budget_bytes);
+    if (lane == kLaneCsr && lane_ != kLaneCsr)
+        throw Error(kResourceLimit, "SectorOperator: the CSR lane was requested but does not fit its budget");
+}
+
+void SectorOperator::build_csr(std::uint64_t budget) {
+    const auto t0 = std::chrono::steady_clock::now();
+    const hipStream_t st = engine_stream();
+    const std::int64_t rows = dim();
+    const dev::SectorView R = rows_->view(), C = cols_->view();
+    const dev::ProgramView P = prog_->view();
+    DeviceBuffer row_len(static_cast<std::size_t>(rows) * 4);
+    DeviceBuffer dfp(dev::DictTable::kSlots * 8), dre(dev::DictTable::kSlots * 8), dim_(dev::DictTable::kSlots * 8);
+    DeviceBuffer dslot(dev::DictTable::kSlots * 2), dcnt(8);
+    dev::DictTable D;
+    D.fp = dfp.get<std::uint64_t>();
+    D.re = dre.get<std::uint64_t>();
+    D.im = dim_.get<std::uint64_t>();
+    D.slot_id = dslot.get<std::uint16_t>();
+    D.count = dcnt.get<unsigned>();
+    D.flags = dcnt.get<unsigned>() + 1;
+    QSA_HIP(hipMemsetAsync(dfp.get(), 0, dfp.bytes(), st));
+    QSA_HIP(hipMemsetAsync(dcnt.get(), 0, 8, st));
+    if (!dev::csr_count(R, C, same(), P, row_len.get<std::uint32_t>(), D, true, st)) return;   // too many groups
+    QSA_HIP(hipGetLastError());
+    row_ptr_.reset(static_cast<std::size_t>(rows + 1) * 8);
+    const std::size_t tb = dev::row_scan_temp_bytes(rows);
+    DeviceBuffer temp(tb ? tb : 8);
+    dev::row_scan(row_len.get<std::uint32_t>(), rows, row_ptr_.get<std::uint64_t>(), temp.get(), tb, st);
+    QSA_HIP(hipGetLastError());
+    std::uint64_t nnz = 0;
+    unsigned cnt[2] = {0, 0};
+    QSA_HIP(hipMemcpyAsync(&nnz, row_ptr_.get<std::uint64_t>() + rows, 8, hipMemcpyDeviceToHost, st));
+    QSA_HIP(hipMemcpyAsync(cnt, dcnt.get(), 8, hipMemcpyDeviceToHost, st));
+    QSA_HIP(hipStreamSynchronize(st));
+    const bool dict_ok = (cnt[1] & 1u) == 0 && cnt[0] <= 65536 && cnt[0] > 0;
+    // The reference's choice (sector_rows.h:365-369): ids into a dictionary unless the full values
+    // are the smaller form; u8 ids up to 256 values.
+    int width = 0;
+    if (dict_ok) {
+        const bool narrow = cnt[0] <= 256;
+        const bool full = static_cast<std::uint64_t>(cnt[0]) * 16 + nnz * (narrow ? 1 : 2) >= nnz * 16;
+        width = full ? 0 : (narrow ? 1 : 2);
+        csr_.id_width_ref = full ? 16 : (narrow ? 1 : 2);
+    } else {
+        csr_.id_width_ref = 16;
+    }
+    const std::size_t vbytes = width == 0 ? (real_ ? 8 : 16) : static_cast<std::size_t>(width);
+    const std::uint64_t bytes = static_cast<std::uint64_t>(rows + 1) * 8 + nnz * 4 + nnz * vbytes;
+    if (nnz > 0xFFFFFFFFFFULL || bytes > budget) {
+        row_ptr_.release();
+        return;
+    }
+    col_.reset(nnz * 4);
+    vals_.reset(nnz * vbytes);
+    if (width > 0) {
+        std::vector<std::uint64_t> fp(dev::DictTable::kSlots), re(dev::DictTable::kSlots), im(dev::DictTable::kSlots);
+        QSA_HIP(hipMemcpy(fp.data(), dfp.get(), dfp.bytes(), hipMemcpyDeviceToHost));
+        QSA_HIP(hipMemcpy(re.data(), dre.get(), dre.bytes(), hipMemcpyDeviceToHost));
+        QSA_HIP(hipMemcpy(im.data(), dim_.get(), dim_.bytes(), hipMemcpyDeviceToHost));
+        std::vector<std::uint16_t> slot_id(dev::DictTable::kSlots, 0);
+        std::vector<double> dr, di;
+        for (int s = 0; s < dev::DictTable::kSlots; ++s) {
+            if (!fp[s]) continue;
+            slot_id[s] = static_cast<std::uint16_t>(dr.size());
+            double a, b;
+            std::memcpy(&a, &re[s], 8);
+            std::memcpy(&b, &im[s], 8);
+            dr.push_back(a);
+            di.push_back(b);
+        }
+        QSA_HIP(hipMemcpy(dslot.get(), slot_id.data(), slot_id.size() * 2, hipMemcpyHostToDevice));
+        dict_re_ = upload(dr.data(), dr.size());
+        dict_im_ = upload(di.data(), di.size());
+        csr_.dict = static_cast<std::int64_t>(dr.size());
+    }
+    dev::csr_fill(R, C, same(), P, row_ptr_.get<std::uint64_t>(), col_.get<std::uint32_t>(), vals_.get(), width,
+                  real_, D, st);
+    QSA_HIP(hipGetLastError());
+    QSA_HIP(hipMemcpyAsync(cnt, dcnt.get(), 8, hipMemcpyDeviceToHost, st));
+    QSA_HIP(hipStreamSynchronize(st));
+    if (width > 0 && (cnt[1] & 2u)) {   // a fingerprint collision: store the values in full
+        width = 0;
+        vals_.reset(nnz * (real_ ? 8 : 16));
+        dict_re_.release();
+        dict_im_.release();
+        csr_.dict = 0;
+        dev::csr_fill(R, C, same(), P, row_ptr_.get<std::uint64_t>(), col_.get<std::uint32_t>(), vals_.get(), 0,
+                      real_, D, st);
+        QSA_HIP(hipStreamSynchronize(st));
+    }
+    csr_.nnz = static_cast<std::int64_t>(nnz);
+    csr_.width = width == 0 ? (real_ ? 8 : 16) : width;
+    csr_.bytes = row_ptr_.bytes() + col_.bytes() + vals_.bytes() + dict_re_.bytes() + dict_im_.bytes();
+    const double mean = rows ? static_cast<double>(nnz) / static_cast<double>(rows) : 0.0;
+    int L = 4;
+    while (L < 64 && L < mean / 2.0) L *= 2;
+    csr_.lanes = L;
+    lane_ = kLaneCsr;
+    csr_.build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+dev::CsrView SectorOperator::csr_view() const {
+    dev::CsrView A;
+    A.rows = dim();
+    A.row_ptr = row_ptr_.get<std::uint64_t>();
+    A.col = col_.get<std::uint32_t>();
+    A.vals = vals_.get();
+    A.width = (csr_.width == 1 || csr_.width == 2) ? csr_.width : 0;
+    A.dict_re = dict_re_.get<double>();
+    A.dict_im = dict_im_.get<double>();
+    A.lanes = csr_.lanes;
+    A.values_real = real_;
+    return A;
+}
+
+void SectorOperator::apply(const void* in, void* out, bool complex_vectors) const {
+    const void* ins[1] = {in};
+    void* outs[1] = {out};
+    apply_multi(ins, outs, 1, complex_vectors);
+}
+
+void SectorOperator::apply_multi(const void* const* ins, void* const* outs, int k, bool complex_vectors) const {
+    const bool real_vectors = real_ && !complex_vectors;
+    const hipStream_t st = engine_stream();
+    for (int done = 0; done < k; done += dev::kMaxVec) {
+        dev::VecSet v;
+        v.k = std::min(dev::kMaxVec, k - done);
+        for (int i = 0; i < v.k; ++i) {
+            v.in[i] = ins[done + i];
+            v.out[i] = outs[done + i];
+        }
+        if (lane_ == kLaneCsr) dev::csr_spmv(csr_view(), real_vectors, v, st);
+        else dev::walk_gather(rows_->view(), cols_->view(), same(), prog_->view(), real_vectors, v, st);
+    }
+    QSA_HIP(hipGetLastError());
+}
+
+void SectorOperator::apply_block(int nv, const void* X, void* Y, bool complex_vectors) const {
+    if (lane_ != kLaneCsr) throw Error(kDeviceUnsupported, "apply_block needs the CSR lane");
+    if (nv != 1 && nv != 2 && nv != 4 && nv != 8) throw Error(kInvalid, "apply_block: nv must be 1, 2, 4 or 8");
+    dev::csr_spmv_block(csr_view(), real_ && !complex_vectors, nv, X, Y, engine_stream());
+    QSA_HIP(hipGetLastError());
+}
+
+void SectorOperator::csr_download(std::vector<std::uint64_t>& row_ptr, std::vector<std::uint32_t>& col,
+                                  std::vector<double>& val) const {
+    csr_download_rows(0, dim(), row_ptr, col, val);
+}
+
+void SectorOperator::csr_download_rows(std::int64_t r0, std::int64_t r1, std::vector<std::uint64_t>& row_ptr,
+                                       std::vector<std::uint32_t>& col, std::vector<double>& val) const {
+    if (lane_ != kLaneCsr) throw Error(kInvalid, "csr_download: no CSR built");
+    if (r0 < 0 || r1 < r0 || r1 > dim()) throw Error(kInvalid, "csr_download: row range out of bounds");
+    row_ptr.resize(static_cast<std::size_t>(r1 - r0 + 1));
+    QSA_HIP(hipMemcpy(row_ptr.data(), row_ptr_.get<std::uint64_t>() + r0, row_ptr.size() * 8, hipMemcpyDeviceToHost));
+    const std::size_t e0 = static_cast<std::size_t>(row_ptr.front());
+    const std::size_t nnz = static_cast<std::size_t>(row_ptr.back() - row_ptr.front());
+    col.resize(nnz);
+    if (nnz) QSA_HIP(hipMemcpy(col.data(), col_.get<std::uint32_t>() + e0, nnz * 4, hipMemcpyDeviceToHost));
+    val.assign(2 * nnz, 0.0);
+    if (csr_.width == 1 || csr_.width == 2) {
+        std::vector<double> dr(static_cast<std::size_t>(csr_.dict)), di(static_cast<std::size_t>(csr_.dict));
+        QSA_HIP(hipMemcpy(dr.data(), dict_re_.get(), dr.size() * 8, hipMemcpyDeviceToHost));
+        QSA_HIP(hipMemcpy(di.data(), dict_im_.get(), di.size() * 8, hipMemcpyDeviceToHost));
+        std::vector<std::uint16_t> ids(nnz);
+        if (csr_.width == 1) {
+            std::vector<std::uint8_t> b(nnz);
+            if (nnz) QSA_HIP(hipMemcpy(b.data(), vals_.get<std::uint8_t>() + e0, nnz, hipMemcpyDeviceToHost));
+            for (std::size_t e = 0; e < nnz; ++e) ids[e] = b[e];
+        } else if (nnz) {
+            QSA_HIP(hipMemcpy(ids.data(), vals_.get<std::uint16_t>() + e0, nnz * 2, hipMemcpyDeviceToHost));
+        }
+        for (std::size_t e = 0; e < nnz; ++e) {
+            val[2 * e] = dr[ids[e]];
+            val[2 * e + 1] = di[ids[e]];
+        }
+    } else if (real_) {
+        std::vector<double> v(nnz);
+        if (nnz) QSA_HIP(hipMemcpy(v.data(), vals_.get<double>() + e0, nnz * 8, hipMemcpyDeviceToHost));
+        for (std::size_t e = 0; e < nnz; ++e) val[2 * e] = v[e];
+    } else if (nnz) {
+        QSA_HIP(hipMemcpy(val.data(), vals_.get<double>() + 2 * e0, nnz * 16, hipMemcpyDeviceToHost));
+    }
+}
+
+}  // namespace qsa

@@ -1,0 +1,75 @@
+// The operator of one symmetry sector (or between two sectors of one group) on device vectors: the
+// reference's RepSectorMatVec / CrossSectorMatVec device lanes (src/engine/internal.h:163-609,
+// src/gpu/rep_matvec.cu) rebuilt on HIP. Two lanes:
+//   gather -- the row walk on the fly, every apply (no storage beyond the sector);
+//   csr    -- the merged reduced CSR built once on the device (u8 / u16 value ids into a
+//             dictionary, or full values), then a streaming SpMV per apply.
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "../device/launch.h"
+#include "algebra.h"
+#include "sector.h"
+
+namespace qsa {
+
+enum Lane : int { kLaneAuto = 0, kLaneGather = 1, kLaneCsr = 2 };
+
+struct CsrInfo {
+    std::int64_t nnz = 0;
+    int width = 0;             ///< stored value bytes: 1 (u8 id), 2 (u16 id), or 8 / 16 (full values)
+    std::int64_t dict = 0;     ///< distinct values (0: no dictionary)
+    std::uint64_t bytes = 0;   ///< row_ptr + col + values/ids + dictionary
+    double build_seconds = 0.0;
+    int lanes = 16;
+    std::int64_t id_width_ref = 0;   ///< the value width the reference's rule gives (1, 2, 16)
+};
+
+class SectorOperator {
+public:
+    SectorOperator(std::shared_ptr<const DeviceSector> rows, std::shared_ptr<const DeviceSector> cols,
+                   std::shared_ptr<const DeviceProgram> prog, double norm_bound, int lane,
+                   std::uint64_t csr_budget_bytes);
+
+    std::int64_t dim() const { return rows_->dim(); }
+    std::int64_t col_dim() const { return cols_->dim(); }
+    bool same() const { return rows_ == cols_; }
+    bool is_real() const { return real_; }
+    int lane() const { return lane_; }
+    double norm_bound() const { return norm_bound_; }
+    const CsrInfo& csr_info() const { return csr_; }
+    const DeviceSector& rows() const { return *rows_; }
+    const DeviceSector& cols() const { return *cols_; }
+
+    /// out = O in on device vectors (double when is_real() and not complex_vectors, else complex),
+    /// on the engine stream.
+    void apply(const void* in, void* out, bool complex_vectors = false) const;
+    /// outs[i] = O ins[i], i < k (k-vector applies equal k single applies bit for bit).
+    void apply_multi(const void* const* ins, void* const* outs, int k, bool complex_vectors = false) const;
+    /// Block-interleaved vectors: X[row * nv + i] (nv in 1, 2, 4, 8); CSR lane only.
+    void apply_block(int nv, const void* X, void* Y, bool complex_vectors = false) const;
+    /// The CSR on the host (tests): row_ptr, col, values (re, im interleaved).
+    void csr_download(std::vector<std::uint64_t>& row_ptr, std::vector<std::uint32_t>& col,
+                      std::vector<double>& val) const;
+    /// Rows [r0, r1) of the CSR on the host: row_ptr (absolute, r1 - r0 + 1), col, values.
+    void csr_download_rows(std::int64_t r0, std::int64_t r1, std::vector<std::uint64_t>& row_ptr,
+                           std::vector<std::uint32_t>& col, std::vector<double>& val) const;
+    /// Pick the lanes per row of the SpMV (4..64).
+    void set_lanes(int lanes) { csr_.lanes = lanes; }
+
+private:
+    void build_csr(std::uint64_t budget);
+    dev::CsrView csr_view() const;
+
+    std::shared_ptr<const DeviceSector> rows_, cols_;
+    std::shared_ptr<const DeviceProgram> prog_;
+    double norm_bound_;
+    bool real_;
+    int lane_ = kLaneGather;
+    CsrInfo csr_;
+    DeviceBuffer row_ptr_, col_, vals_, dict_re_, dict_im_;
+};
+
+}  // namespace qsa

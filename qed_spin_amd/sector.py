@@ -1,0 +1,257 @@
+"""Symmetry sectors resident in HBM and the operators on them (the reference's RepSectorData +
+RepSectorMatVec / CrossSectorMatVec device lanes, behind include/qedh.h)."""
+
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from .errors import InvalidRequest
+from .operator import Operator, Program
+
+LANE_AUTO, LANE_GATHER, LANE_CSR = 0, 1, 2
+DEFAULT_RANK_BUDGET = 16 << 30   # dense rank table up to 16 GiB of HBM
+DEFAULT_CSR_BUDGET = 64 << 30
+
+
+class DeviceArray:
+    """A device allocation holding a 1-D array (double or complex128)."""
+
+    def __init__(self, n: int, dtype=np.complex128):
+        self.n = int(n)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = self.n * self.dtype.itemsize
+        p = C.c_void_p()
+        _lib.call("qedh_malloc", C.byref(p), C.c_size_t(self.nbytes))
+        self.ptr = p
+
+    @classmethod
+    def from_host(cls, a) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(a.size, a.dtype)
+        if d.nbytes:
+            _lib.call("qedh_memcpy_htod", d.ptr, a.ctypes.data_as(C.c_void_p), C.c_size_t(d.nbytes))
+        return d
+
+    def to_host(self) -> np.ndarray:
+        out = np.empty(self.n, self.dtype)
+        if self.nbytes:
+            _lib.call("qedh_memcpy_dtoh", out.ctypes.data_as(C.c_void_p), self.ptr, C.c_size_t(self.nbytes))
+        return out
+
+    def zero(self) -> None:
+        if self.nbytes:
+            _lib.call("qedh_memset", self.ptr, 0, C.c_size_t(self.nbytes))
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and _lib._lib is not None:
+            _lib.lib().qedh_free(self.ptr)
+            self.ptr = None
+
+
+@dataclass
+class GroupSpec:
+    """|G| site permutations (bit i of g s = bit perms[g][i] of s), XOR flips, characters."""
+
+    perms: np.ndarray            # (G, N) int32
+    chars: np.ndarray            # (G,) complex128
+    flips: Optional[np.ndarray] = None   # (G,) uint64
+
+    @property
+    def G(self) -> int:
+        return int(self.perms.shape[0])
+
+    @property
+    def N(self) -> int:
+        return int(self.perms.shape[1])
+
+
+class Sector:
+    """One symmetry sector, enumerated on the device: ascending representatives, 1/norms and the
+    reverse lookup (n_up < 0: the full 2^N space)."""
+
+    def __init__(self, group: GroupSpec, n_up: int, sublattice: int = -1, relaxed: bool = True,
+                 rank_budget: int = DEFAULT_RANK_BUDGET):
+        perms = np.ascontiguousarray(group.perms, dtype=np.int32)
+        chars = np.ascontiguousarray(np.asarray(group.chars, np.complex128).view(np.float64))
+        flips = None if group.flips is None else np.ascontiguousarray(group.flips, dtype=np.uint64)
+        if perms.ndim != 2 or chars.size != 2 * perms.shape[0]:
+            raise InvalidRequest("sector: perms must be (G, N) and chars (G,)")
+        self.group = group
+        self.n_up = int(n_up)
+        h = C.c_void_p()
+        _lib.call(
+            "qedh_sector_create",
+            int(perms.shape[1]),
+            int(n_up),
+            int(perms.shape[0]),
+            perms.ctypes.data_as(C.POINTER(C.c_int)),
+            None if flips is None else flips.ctypes.data_as(C.POINTER(C.c_uint64)),
+            chars.ctypes.data_as(C.POINTER(C.c_double)),
+            int(sublattice),
+            int(bool(relaxed)),
+            C.c_uint64(int(rank_budget)),
+            C.byref(h),
+        )
+        self.h = h
+        d, o, b, s = C.c_int64(), C.c_int64(), C.c_int(), C.c_double()
+        _lib.call("qedh_sector_info", self.h, C.byref(d), C.byref(o), C.byref(b), C.byref(s))
+        self.dim = int(d.value)
+        self.n_orbits = int(o.value)
+        self.sublattice_block = int(b.value)
+        self.build_seconds = float(s.value)
+
+    @property
+    def real(self) -> bool:
+        return bool(np.all(np.asarray(self.group.chars).imag == 0.0))
+
+    def export(self):
+        reps = np.zeros(self.dim, np.uint64)
+        inv = np.zeros(self.dim, np.float64)
+        _lib.call("qedh_sector_export", self.h, _lib.ptr(reps, C.c_uint64), _lib.ptr(inv, C.c_double))
+        return reps, inv
+
+    def key_bits(self):
+        kb = np.zeros(self.group.N, np.int32)
+        L = C.c_int()
+        _lib.call("qedh_sector_key_bits", self.h, kb.ctypes.data_as(C.POINTER(C.c_int)), C.byref(L))
+        return (kb if L.value else None), int(L.value)
+
+    def index_and_projection(self, states):
+        st = np.ascontiguousarray(states, dtype=np.uint64)
+        idx = np.zeros(st.size, np.int64)
+        proj = np.zeros(2 * st.size, np.float64)
+        _lib.call("qedh_sector_index_and_projection", self.h, _lib.ptr(st, C.c_uint64), C.c_int64(st.size),
+                  _lib.ptr(idx, C.c_int64), _lib.ptr(proj, C.c_double))
+        return idx, proj.view(np.complex128)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib._lib is not None:
+            _lib.lib().qedh_sector_destroy(self.h)
+            self.h = None
+
+
+class SectorMatvec:
+    """O restricted to a sector (or from sector `cols` to sector `rows`, one group) on device vectors.
+
+    The program is the walk of O^dagger (for two sectors, lambda-projected with rows as ket)."""
+
+    def __init__(self, op: Operator, rows: Sector, cols: Optional[Sector] = None, lane: int = LANE_AUTO,
+                 csr_budget: int = DEFAULT_CSR_BUDGET):
+        self.op = op
+        self.rows = rows
+        self.cols = cols if cols is not None else rows
+        if self.cols is rows:
+            self.program = Program(op, adjoint=True)
+        else:
+            self.program = Program(op, adjoint=True, src=rows, tgt=self.cols)
+        h = C.c_void_p()
+        _lib.call("qedh_matvec_create", rows.h, None if self.cols is rows else self.cols.h, self.program.h,
+                  C.c_double(op.norm_bound()), int(lane), C.c_uint64(int(csr_budget)), C.byref(h))
+        self.h = h
+        self.info = self._info()
+
+    def _info(self) -> dict:
+        i = _lib.MatvecInfo()
+        _lib.call("qedh_matvec_info", self.h, C.byref(i))
+        return {k: getattr(i, k) for k, _ in _lib.MatvecInfo._fields_}
+
+    @property
+    def dim(self) -> int:
+        return int(self.info["rows"])
+
+    @property
+    def is_real(self) -> bool:
+        return bool(self.info["is_real"])
+
+    @property
+    def lane(self) -> str:
+        return "device-csr" if self.info["lane"] == LANE_CSR else "device-gather"
+
+    def dtype(self, complex_vectors=False):
+        return np.float64 if (self.is_real and not complex_vectors) else np.complex128
+
+    def apply(self, x: DeviceArray, y: DeviceArray, complex_vectors: bool = False) -> None:
+        _lib.call("qedh_matvec_apply", self.h, x.ptr, y.ptr, int(bool(complex_vectors)))
+
+    def apply_multi(self, xs: Sequence[DeviceArray], ys: Sequence[DeviceArray], complex_vectors=False) -> None:
+        k = len(xs)
+        ins = (C.c_void_p * k)(*[x.ptr.value for x in xs])
+        outs = (C.c_void_p * k)(*[y.ptr.value for y in ys])
+        _lib.call("qedh_matvec_apply_multi", self.h, ins, outs, k, int(bool(complex_vectors)))
+
+    def apply_block(self, nv: int, X: DeviceArray, Y: DeviceArray, complex_vectors=False) -> None:
+        _lib.call("qedh_matvec_apply_block", self.h, int(nv), X.ptr, Y.ptr, int(bool(complex_vectors)))
+
+    def set_lanes(self, lanes: int) -> None:
+        _lib.call("qedh_matvec_set_lanes", self.h, int(lanes))
+        self.info = self._info()
+
+    def apply_host(self, x: np.ndarray, complex_vectors: bool = False) -> np.ndarray:
+        """y = O x for a host vector (staged through HBM)."""
+        dt = self.dtype(complex_vectors)
+        X = DeviceArray.from_host(np.asarray(x, dt))
+        Y = DeviceArray(self.dim, dt)
+        self.apply(X, Y, complex_vectors)
+        return Y.to_host()
+
+    def export_csr(self):
+        nnz = int(self.info["nnz"])
+        row_ptr = np.zeros(self.dim + 1, np.uint64)
+        col = np.zeros(nnz, np.uint32)
+        val = np.zeros(2 * nnz, np.float64)
+        _lib.call("qedh_matvec_export_csr", self.h, _lib.ptr(row_ptr, C.c_uint64), _lib.ptr(col, C.c_uint32),
+                  _lib.ptr(val, C.c_double))
+        return row_ptr, col, val.view(np.complex128)
+
+    def lanczos(self, v0: np.ndarray, max_iter: int, reorth: int = 1, breakdown_tol: float = 0.0,
+                complex_vectors: bool = False):
+        dt = self.dtype(complex_vectors)
+        v0 = np.ascontiguousarray(v0, dtype=dt)
+        alpha = np.zeros(max_iter, np.float64)
+        beta = np.zeros(max_iter + 1, np.float64)
+        iters = C.c_int()
+        t = np.zeros(4, np.float64)
+        _lib.call("qedh_lanczos", self.h, v0.ctypes.data_as(C.c_void_p), int(bool(complex_vectors)), int(max_iter),
+                  int(reorth), C.c_double(breakdown_tol), _lib.ptr(alpha, C.c_double), _lib.ptr(beta, C.c_double),
+                  C.byref(iters), _lib.ptr(t, C.c_double))
+        m = int(iters.value)
+        return alpha[:m], beta[: m + 1], {"apply_s": t[0], "reorth_s": t[1], "vector_s": t[2], "total_s": t[3]}
+
+    def lowest(self, k: int = 1, max_iter: int = 0):
+        vals = np.zeros(max(1, k), np.float64)
+        n, conv, applies = C.c_int(), C.c_int(), C.c_int64()
+        _lib.call("qedh_block_lowest", self.h, int(k), C.c_int64(max_iter), _lib.ptr(vals, C.c_double),
+                  C.byref(n), C.byref(conv), C.byref(applies))
+        return vals[: n.value], bool(conv.value), int(applies.value)
+
+    def ftlm(self, samples: int, krylov: int, seed: int, betas, breakdown_tol: float = 0.0):
+        betas = np.ascontiguousarray(betas, np.float64)
+        nT = betas.size
+        lnZ, E, V = np.zeros(nT), np.zeros(nT), np.zeros(nT)
+        emin = C.c_double()
+        _lib.call("qedh_ftlm", self.h, int(samples), int(krylov), C.c_uint64(int(seed)), int(nT),
+                  _lib.ptr(betas, C.c_double), C.c_double(breakdown_tol), _lib.ptr(lnZ, C.c_double),
+                  _lib.ptr(E, C.c_double), _lib.ptr(V, C.c_double), C.byref(emin))
+        return lnZ, E, V, float(emin.value)
+
+    def continued_fraction(self, phi: np.ndarray, krylov: int, omega, eta: float, energy_shift=None,
+                           breakdown_tol: float = 0.0):
+        phi = np.ascontiguousarray(phi, np.complex128)
+        omega = np.ascontiguousarray(omega, np.float64)
+        S = np.zeros(omega.size)
+        sh, nsq, ch = C.c_double(), C.c_double(), C.c_double()
+        _lib.call("qedh_cf", self.h, phi.view(np.float64).ctypes.data_as(C.POINTER(C.c_double)), int(krylov),
+                  int(omega.size), _lib.ptr(omega, C.c_double), C.c_double(eta), int(energy_shift is not None),
+                  C.c_double(0.0 if energy_shift is None else energy_shift), C.c_double(breakdown_tol),
+                  _lib.ptr(S, C.c_double), C.byref(sh), C.byref(nsq), C.byref(ch))
+        return S, {"energy_shift": sh.value, "norm_sq": nsq.value, "convergence_change": ch.value}
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib._lib is not None:
+            _lib.lib().qedh_matvec_destroy(self.h)
+            self.h = None

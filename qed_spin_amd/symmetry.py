@@ -1,0 +1,120 @@
+"""Group data of the sectors the verbs solve: translations (and reflections) of a ring, their
+characters, and the (Sz, k) walk.
+
+Conventions follow the reference: an element acts as bit i of g s = bit perm[i] of s; the translation
+generator is T[i] = (i + 1) mod N (tests/python/support/models.py:97); the abelian characters are
+chi_m(T^a) = e^{2 pi i m a / N}, exact at quarter turns (src/basis/irreps.cpp:161-175 root_of_unity,
+:181-242 abelian_characters with the group in the order T^0, T^1, ...).
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+from typing import List, Optional
+
+import numpy as np
+
+from .sector import GroupSpec
+
+
+def root_of_unity(p: int, L: int) -> complex:
+    """e^{2 pi i p / L}, exact at the quarter turns; cos/sin of (2 pi p) / L otherwise."""
+    p %= L
+    if (4 * p) % L == 0:
+        return [1 + 0j, 1j, -1 + 0j, -1j][(4 * p) // L]
+    th = 6.283185307179586476925286766559 * float(p) / float(L)
+    return complex(math.cos(th), math.sin(th))
+
+
+def translations(N: int) -> np.ndarray:
+    """T^a, a = 0..N-1: T^a[i] = (i + a) mod N."""
+    return np.array([[(i + a) % N for i in range(N)] for a in range(N)], dtype=np.int32)
+
+
+def reflections(N: int) -> np.ndarray:
+    """T^a R with R[i] = -i mod N (R first): i -> (a - i) mod N."""
+    return np.array([[(a - i) % N for i in range(N)] for a in range(N)], dtype=np.int32)
+
+
+def momentum_chars(N: int, m: int) -> np.ndarray:
+    return np.array([root_of_unity(m * a, N) for a in range(N)], dtype=np.complex128)
+
+
+def chain_group(N: int, m: int) -> GroupSpec:
+    """The momentum-m sector of the N translations."""
+    return GroupSpec(perms=translations(N), chars=momentum_chars(N, m))
+
+
+def dihedral_group(N: int, m: int, parity: int) -> GroupSpec:
+    """Translations and reflections at a reflection-invariant momentum (m = 0 or N/2): characters
+    chi(T^a) = e^{2 pi i m a/N}, chi(T^a R) = parity chi(T^a)."""
+    if (2 * m) % N != 0:
+        raise ValueError("a reflection-parity sector needs m = 0 or N/2")
+    ck = momentum_chars(N, m)
+    perms = np.concatenate([translations(N), reflections(N)])
+    chars = np.concatenate([ck, parity * ck])
+    return GroupSpec(perms=perms, chars=chars)
+
+
+def trivial_group(N: int) -> GroupSpec:
+    return GroupSpec(perms=np.arange(N, dtype=np.int32)[None, :], chars=np.ones(1, np.complex128))
+
+
+@dataclass
+class Block:
+    """One symmetry block of the walk: its group, magnetisation and labels."""
+
+    group: GroupSpec
+    n_up: int
+    momentum: Optional[int] = None
+    parity: Optional[int] = None
+
+    @property
+    def label(self) -> str:
+        parts = [f"n_up={self.n_up}"]
+        if self.momentum is not None:
+            parts.append(f"k={self.momentum}")
+        if self.parity is not None:
+            parts.append(f"R={self.parity:+d}")
+        return ",".join(parts)
+
+
+@dataclass
+class Symmetry:
+    """A chain's symmetry request: translations (momentum sectors), Sz, reflection parity.
+
+    sz: an n_up, a list of them, or "all" (every n_up) / "off" (the full space, no U(1));
+    momenta: "all" or a list of m; reflection: None, +1 or -1 (only with momenta 0 or N/2).
+    translations=False: no spatial symmetry (one block per n_up)."""
+
+    sz: object = "all"
+    momenta: object = "all"
+    translations: bool = True
+    reflection: Optional[int] = None
+
+    @classmethod
+    def none(cls) -> "Symmetry":
+        return cls(sz="off", momenta=None, translations=False)
+
+    def blocks(self, N: int) -> List[Block]:
+        if self.sz == "off":
+            nups = [-1]
+        elif self.sz == "all":
+            nups = list(range(N + 1))
+        elif isinstance(self.sz, (list, tuple)):
+            nups = [int(x) for x in self.sz]
+        else:
+            nups = [int(self.sz)]
+        out = []
+        for n_up in nups:
+            if not self.translations:
+                out.append(Block(trivial_group(N), n_up))
+                continue
+            ms = range(N) if self.momenta in ("all", None) else [int(m) % N for m in self.momenta]
+            for m in ms:
+                if self.reflection is not None:
+                    out.append(Block(dihedral_group(N, m, int(self.reflection)), n_up, m, int(self.reflection)))
+                else:
+                    out.append(Block(chain_group(N, m), n_up, m))
+        return out

@@ -1,0 +1,211 @@
+"""GPU parity of the HIP path against the oracle and the reference's fixtures.
+
+Bit-exact: representatives, 1/norms, key order, index_and_projection, the merged CSR (columns and
+values). Matvec within 1e-13 relative norm (both lanes, single / multi / block vectors; multi equals
+single bit for bit). Eigenvalues within 1e-10 of dense diagonalisation. Every case prints its name
+before it touches the device."""
+
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import fixture_names, fixture_products, load_fixture
+from oracle import pyoracle as po
+
+import qed_spin_amd as qs
+from qed_spin_amd import symmetry as sym
+
+pytestmark = pytest.mark.gpu
+
+NAMES = fixture_names()
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def fixture_sector(fx):
+    g = qs.GroupSpec(perms=fx["perms"], chars=fx["chars"], flips=fx["flips"])
+    return qs.Sector(g, int(fx["n_up"]), sublattice=int(fx["sublattice_mode"]))
+
+
+def fixture_operator(fx):
+    op = qs.Operator(int(fx["N"]))
+    for ops, sites, c in fixture_products(fx):
+        op.add_product(ops, sites, c)
+    return op
+
+
+def bits(a):
+    return np.ascontiguousarray(a).view(np.uint64)
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_sector_bit_exact(name):
+    say("sector", name)
+    fx = load_fixture(name)
+    s = fixture_sector(fx)
+    reps, inv = s.export()
+    assert s.n_orbits == int(fx["n_orbits"][0])
+    assert np.array_equal(reps, fx["reps"])
+    assert np.array_equal(bits(inv), bits(fx["inv_norms"]))
+    kb, L = s.key_bits()
+    if "key_bit" in fx:
+        assert L > 0 and np.array_equal(kb, fx["key_bit"])
+    else:
+        assert L == 0
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_index_and_projection_bit_exact(name):
+    say("index_and_projection", name)
+    fx = load_fixture(name)
+    s = fixture_sector(fx)
+    idx, proj = s.index_and_projection(fx["probes"])
+    assert np.array_equal(idx, fx["probe_index"])
+    ref = fx["probe_proj"].view(np.complex128)
+    hit = idx >= 0
+    assert np.array_equal(bits(proj[hit].view(np.float64)), bits(ref[hit].view(np.float64)))
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_csr_bit_exact(name):
+    say("csr", name)
+    fx = load_fixture(name)
+    s = fixture_sector(fx)
+    m = qs.SectorMatvec(fixture_operator(fx), s, lane=qs.LANE_CSR)
+    row_ptr, col, val = m.export_csr()
+    assert np.array_equal(row_ptr, fx["csr_row_ptr"])
+    assert np.array_equal(col, fx["csr_col"])
+    ref = fx["csr_val"].view(np.complex128)
+    if m.is_real:   # a real operator keeps the real parts (its imaginary parts are zeros)
+        assert np.array_equal(bits(val.real), bits(ref.real))
+        assert np.all(ref.imag == 0.0)
+    else:
+        assert np.array_equal(bits(val.view(np.float64)), bits(ref.view(np.float64)))
+    assert m.info["nnz"] == int(fx["csr_meta"][0])
+    if m.info["value_bytes"] in (1, 2):
+        assert m.info["dict_size"] == int(fx["csr_meta"][1])
+    assert m.info["id_width_ref"] == int(fx["csr_meta"][2])
+
+
+@pytest.mark.parametrize("lane", [qs.LANE_GATHER, qs.LANE_CSR])
+@pytest.mark.parametrize("name", NAMES)
+def test_matvec_matches_reference(name, lane):
+    say("matvec", name, lane)
+    fx = load_fixture(name)
+    s = fixture_sector(fx)
+    m = qs.SectorMatvec(fixture_operator(fx), s, lane=lane)
+    x = fx["gather_in"].view(np.complex128)
+    ref = fx["gather_out"].view(np.complex128)
+    y = m.apply_host(x, complex_vectors=True)
+    assert np.linalg.norm(y - ref) <= 1e-13 * np.linalg.norm(ref)
+    if m.is_real:   # real vectors: the real and imaginary parts separately
+        yr = m.apply_host(x.real) + 1j * m.apply_host(x.imag)
+        assert np.linalg.norm(yr - ref) <= 1e-13 * np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize("lane", [qs.LANE_GATHER, qs.LANE_CSR])
+def test_multi_equals_single_bitwise(lane):
+    say("multi", lane)
+    fx = load_fixture("j1j2_n16_k3_slc")
+    s = fixture_sector(fx)
+    m = qs.SectorMatvec(fixture_operator(fx), s, lane=lane)
+    rng = np.random.default_rng(7)
+    xs = [rng.normal(size=s.dim) + 1j * rng.normal(size=s.dim) for _ in range(11)]
+    X = [qs.DeviceArray.from_host(x) for x in xs]
+    Y = [qs.DeviceArray(s.dim) for _ in xs]
+    m.apply_multi(X, Y)
+    for x, y in zip(xs, Y):
+        single = m.apply_host(x)
+        assert np.array_equal(bits(y.to_host().view(np.float64)), bits(single.view(np.float64)))
+    if lane == qs.LANE_CSR:
+        for nv in (1, 2, 4, 8):
+            B = np.stack(xs[:nv], axis=1)   # row-major rows x nv
+            XB = qs.DeviceArray.from_host(B.ravel())
+            YB = qs.DeviceArray(s.dim * nv)
+            m.apply_block(nv, XB, YB)
+            got = YB.to_host().reshape(s.dim, nv)
+            for i in range(nv):
+                assert np.array_equal(bits(got[:, i].copy().view(np.float64)),
+                                      bits(m.apply_host(xs[i]).view(np.float64)))
+
+
+def dense_of(m):
+    row_ptr, col, val = m.export_csr()
+    A = np.zeros((m.dim, m.dim), np.complex128)
+    for r in range(m.dim):
+        A[r, col[row_ptr[r]:row_ptr[r + 1]]] += val[row_ptr[r]:row_ptr[r + 1]]
+    return A
+
+
+@pytest.mark.parametrize("name", ["j1j2_n16_k3_plain", "j1j2_n16_d_k0_slc", "chiral_n10_k2"])
+def test_lanczos_and_lowest_levels_against_dense(name):
+    say("lanczos", name)
+    fx = load_fixture(name)
+    s = fixture_sector(fx)
+    m = qs.SectorMatvec(fixture_operator(fx), s, lane=qs.LANE_CSR)
+    w = np.linalg.eigvalsh(dense_of(m))
+    rng = np.random.default_rng(2)
+    v0 = rng.normal(size=s.dim) if m.is_real else rng.normal(size=s.dim) + 1j * rng.normal(size=s.dim)
+    for reorth in (0, 1, 2):
+        a, b, _ = m.lanczos(v0, min(150, s.dim), reorth)
+        ritz = po.tridiag_eig(a, b)[0]
+        assert abs(ritz[0] - w[0]) <= 1e-10 * max(1.0, abs(w[0]))
+    vals, conv, _ = m.lowest(1)
+    assert conv and abs(vals[0] - w[0]) <= 1e-10 * max(1.0, abs(w[0]))
+    vals, conv, _ = m.lowest(4)
+    assert conv
+    assert np.allclose(vals, w[:4], rtol=1e-10, atol=1e-10)
+
+
+def j1j2_operator(N, J2):
+    b = qs.HamiltonianBuilder(N)
+    b.heisenberg([(i, (i + 1) % N) for i in range(N)], 1.0)
+    b.heisenberg([(i, (i + 2) % N) for i in range(N)], J2)
+    return b.to_operator()
+
+
+def test_heisenberg_n20_k0_lowest4_against_oracle():
+    """The N = 20 ring's k = 0 sector (dim 9252): four lowest levels against the oracle's own sector
+    and a dense diagonalisation of its merged CSR."""
+    say("heisenberg n20 k0")
+    N = 20
+    op = j1j2_operator(N, 0.0)
+    g = sym.chain_group(N, 0)
+    s = qs.Sector(g, N // 2)
+    assert s.dim == 9252
+    o = po.OracleSector(N, N // 2, g.perms, g.chars)
+    reps, inv = s.export()
+    assert np.array_equal(reps, o.reps) and np.array_equal(bits(inv), bits(o.inv_norms))
+    m = qs.SectorMatvec(op, s, lane=qs.LANE_CSR)
+    row_ptr, col, val = po.sector_csr(qs.Program(op).export(), o)
+    A = np.zeros((o.dim, o.dim))
+    for r in range(o.dim):
+        A[r, col[row_ptr[r]:row_ptr[r + 1]]] += val[row_ptr[r]:row_ptr[r + 1]].real
+    w = np.linalg.eigvalsh(A)
+    vals, conv, _ = m.lowest(4)
+    assert conv and np.allclose(vals, w[:4], rtol=1e-10, atol=0)
+
+
+def test_majumdar_ghosh_n32_full_size():
+    """The N = 32 J1-J2 ring at J2 = 1/2, k = 0, reflection-even: dim 9,398,520 (Burnside), the
+    Hermitian form <x, H y> = <H x, y>, and E0 = -3N/8 = -12 exactly (the Majumdar-Ghosh point)."""
+    say("majumdar-ghosh n32")
+    N = 32
+    op = j1j2_operator(N, 0.5)
+    s = qs.Sector(sym.dihedral_group(N, 0, +1), N // 2)
+    assert s.dim == 9398520
+    assert s.sublattice_block == 16
+    m = qs.SectorMatvec(op, s, lane=qs.LANE_CSR)
+    assert m.is_real
+    rng = np.random.default_rng(4)
+    x = rng.normal(size=s.dim)
+    y = rng.normal(size=s.dim)
+    hx, hy = m.apply_host(x), m.apply_host(y)
+    assert abs(x @ hy - hx @ y) <= 1e-12 * abs(x @ hy)
+    vals, conv, _ = m.lowest(1)
+    assert conv and abs(vals[0] + 12.0) <= 1e-10 * 12.0

@@ -1,0 +1,211 @@
+"""Pins of the oracle (CPU): against the reference's own hot path (fixtures written by oracle/_ref),
+brute force, the Burnside count and closed forms. A plausible slip -- a dropped character, a wrong
+sign in the projection, a transposed permutation, a column merge in the wrong order -- fails here."""
+
+import math
+
+import numpy as np
+import pytest
+
+from conftest import fixture_names, fixture_program, fixture_products, load_fixture
+from oracle import pyoracle as po
+
+NAMES = fixture_names()
+
+
+def oracle_sector(fx, key_bit=None):
+    kb = fx["key_bit"] if "key_bit" in fx else key_bit
+    return po.OracleSector(int(fx["N"]), int(fx["n_up"]), fx["perms"], fx["chars"], fx["flips"], kb)
+
+
+def test_fixtures_present():
+    assert len(NAMES) >= 10
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_sector_bit_exact(name):
+    fx = load_fixture(name)
+    s = oracle_sector(fx)
+    assert s.n_orbits == int(fx["n_orbits"][0])
+    assert s.dim == len(fx["reps"])
+    assert np.array_equal(s.reps, fx["reps"])
+    assert np.array_equal(s.inv_norms.view(np.uint64), fx["inv_norms"].view(np.uint64))
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_key_order_bit_exact(name):
+    fx = load_fixture(name)
+    mode = int(fx["sublattice_mode"])
+    kb, L = po.sublattice_key_bits(fx["perms"], fx["flips"], int(fx["N"]), mode)
+    if "key_bit" in fx:
+        assert L > 0 and np.array_equal(kb, fx["key_bit"])
+    else:
+        assert L == 0
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_index_and_projection_bit_exact(name):
+    fx = load_fixture(name)
+    s = oracle_sector(fx)
+    proj = fx["probe_proj"].view(np.complex128)
+    for t, k_ref, p_ref in zip(fx["probes"], fx["probe_index"], proj):
+        k, p = s.index_and_projection(int(t))
+        assert k == k_ref
+        if k >= 0:
+            assert np.float64(p.real).view(np.uint64) == np.float64(p_ref.real).view(np.uint64)
+            assert np.float64(p.imag).view(np.uint64) == np.float64(p_ref.imag).view(np.uint64)
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_merged_csr_bit_exact(name):
+    fx = load_fixture(name)
+    s = oracle_sector(fx)
+    row_ptr, col, val = po.sector_csr(fixture_program(fx), s)
+    assert np.array_equal(row_ptr.astype(np.uint64), fx["csr_row_ptr"])
+    assert np.array_equal(col.astype(np.uint32), fx["csr_col"])
+    ref = fx["csr_val"].view(np.complex128)
+    assert np.array_equal(val.view(np.float64).view(np.uint64), ref.view(np.float64).view(np.uint64))
+    assert int(fx["csr_meta"][0]) == len(col)
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_matvec_matches_reference_walk(name):
+    fx = load_fixture(name)
+    s = oracle_sector(fx)
+    row_ptr, col, val = po.sector_csr(fixture_program(fx), s)
+    x = fx["gather_in"].view(np.complex128)
+    y = po.csr_matvec(row_ptr, col, val, x)
+    ref = fx["gather_out"].view(np.complex128)
+    assert np.linalg.norm(y - ref) <= 1e-13 * np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize("name", ["j1j2_n12_k3", "j1j2_n12_d_k0_odd", "heis_n12_flip_k6", "chiral_n10_k2",
+                                  "xxz_n10_nup4_k1", "heis_n8_full_k1"])
+def test_sector_block_equals_projected_dense(name):
+    """E^dag H E of the dense matrix (terms only, no lane) equals the sector CSR."""
+    fx = load_fixture(name)
+    N, n_up = int(fx["N"]), int(fx["n_up"])
+    s = oracle_sector(fx)
+    states = [x for x in range(1 << N) if n_up < 0 or bin(x).count("1") == n_up]
+    H = po.dense_matrix(fixture_products(fx), N, states)
+    E = po.sector_basis_matrix(s, states)
+    assert np.allclose(E.conj().T @ E, np.eye(s.dim), atol=1e-12)
+    Hs = E.conj().T @ H @ E
+    row_ptr, col, val = po.sector_csr(fixture_program(fx), s)
+    A = np.zeros((s.dim, s.dim), np.complex128)
+    for r in range(s.dim):
+        for e in range(row_ptr[r], row_ptr[r + 1]):
+            A[r, col[e]] += val[e]
+    scale = np.abs(H).max()
+    assert np.abs(A - Hs).max() <= 1e-12 * scale
+
+
+def test_burnside_tiles_the_sz_sector():
+    """Momentum sectors of the translations partition C(N, n_up) (the sum rule)."""
+    N, n_up = 12, 6
+    perms = np.array([[(i + a) % N for i in range(N)] for a in range(N)], np.int32)
+    total = 0
+    for m in range(N):
+        chars = np.exp(2j * np.pi * m * np.arange(N) / N)
+        total += po.OracleSector(N, n_up, perms, chars).dim
+    assert total == math.comb(N, n_up)
+
+
+def test_rank_unrank_roundtrip():
+    L = po.lib()
+    N, k = 16, 7
+    s = L.orc_unrank_state(0, N, k)
+    for r in range(math.comb(N, k)):
+        assert L.orc_rank_state(s, N, k) == r
+        assert L.orc_unrank_state(r, N, k) == s
+        if r + 1 < math.comb(N, k):
+            s = L.orc_next_bit_permutation(s)
+
+
+def test_key_order_planted_error_is_caught():
+    """A transposed key order (blocks swapped) changes the representatives of a fixture: the fixture
+    comparison is sensitive to the order it pins."""
+    fx = load_fixture("j1j2_n16_k3_slc")
+    kb = fx["key_bit"].copy()
+    N = int(fx["N"])
+    swapped = (kb + N // 2) % N
+    s = po.OracleSector(N, int(fx["n_up"]), fx["perms"], fx["chars"], fx["flips"], swapped)
+    assert s.dim == len(fx["reps"]) and not np.array_equal(s.reps, fx["reps"])
+
+
+def dense_csr(row_ptr, col, val, n):
+    A = np.zeros((n, n), np.complex128)
+    for r in range(n):
+        for e in range(row_ptr[r], row_ptr[r + 1]):
+            A[r, col[e]] += val[e]
+    return A
+
+
+@pytest.mark.parametrize("reorth", [0, 1, 2])
+def test_lanczos_against_dense(reorth):
+    fx = load_fixture("j1j2_n16_k3_plain")
+    s = oracle_sector(fx)
+    row_ptr, col, val = po.sector_csr(fixture_program(fx), s)
+    A = dense_csr(row_ptr, col, val, s.dim)
+    w = np.linalg.eigvalsh(A)
+    rng = np.random.default_rng(1)
+    v0 = rng.normal(size=s.dim) + 1j * rng.normal(size=s.dim)
+    alpha, beta = po.lanczos(row_ptr, col, val, v0, 120, reorth)
+    ritz, _ = po.tridiag_eig(alpha, beta)
+    assert abs(ritz[0] - w[0]) <= 1e-10 * abs(w[0])
+    if reorth == 1:   # a full basis: T = V^H A V, extremes converge on both ends
+        assert abs(ritz[-1] - w[-1]) <= 1e-10 * abs(w[-1])
+
+
+def test_tridiag_eig_against_numpy():
+    rng = np.random.default_rng(3)
+    a = rng.normal(size=40)
+    b = np.concatenate([[0.0], rng.normal(size=39) ** 2 + 0.1])
+    T = np.diag(a) + np.diag(b[1:], 1) + np.diag(b[1:], -1)
+    w, V = po.tridiag_eig(a, b)
+    assert np.allclose(w, np.linalg.eigvalsh(T), atol=1e-12)
+    assert np.allclose(T @ V, V * w, atol=1e-10)
+
+
+def test_ftlm_full_trace_is_exact():
+    """With start vectors spanning the space (the unit vectors, one Lanczos each run to the full
+    dimension) FTLM's moments sum to the exact partition function of a small block."""
+    fx = load_fixture("xxz_n10_nup4_k1")
+    s = oracle_sector(fx)
+    row_ptr, col, val = po.sector_csr(fixture_program(fx), s)
+    A = dense_csr(row_ptr, col, val, s.dim)
+    ev = np.linalg.eigvalsh(A)
+    betas = np.array([0.3, 1.0, 2.5])
+    moments = []
+    for i in range(s.dim):
+        v0 = np.zeros(s.dim, np.complex128)
+        v0[i] = 1.0
+        alpha, beta = po.lanczos(row_ptr, col, val, v0, s.dim, 1, 1e-13)
+        ritz, V = po.tridiag_eig(alpha, beta)
+        moments.append(po.ftlm_moments(ritz, V[0] ** 2, betas))
+    # R samples average to Z / R; scaled by n = dim inside combine, so use n = R = dim
+    lnZ, E, Vv, eref = po.ftlm_combine(np.array(moments), s.dim, betas)
+    Z = np.array([np.exp(-b * ev).sum() for b in betas])
+    Em = np.array([(ev * np.exp(-b * ev)).sum() for b in betas]) / Z
+    assert np.allclose(lnZ, np.log(Z), rtol=1e-10, atol=1e-10)
+    assert np.allclose(E, Em, rtol=1e-10, atol=1e-10)
+
+
+def test_continued_fraction_two_level_closed_form():
+    """A 2x2 tridiagonal: G(z) = (z - a1) / ((z - a0)(z - a1) - b1^2)."""
+    a = np.array([0.3, 1.7])
+    b = np.array([0.0, 0.8])
+    w = np.linspace(-2, 4, 31)
+    eta = 0.07
+    S = po.continued_fraction(a, b, w, eta, 2.5)
+    z = w + 1j * eta
+    G = 2.5 * (z - a[1]) / ((z - a[0]) * (z - a[1]) - b[1] ** 2)
+    assert np.allclose(S, -G.imag / np.pi, rtol=1e-13, atol=1e-15)
+
+
+def test_seeded_gaussian_is_unit_and_reproducible():
+    a = po.gaussian_vector(11, 0, 1000)
+    b = po.gaussian_vector(11, 0, 1000)
+    c = po.gaussian_vector(11, 1, 1000)
+    assert np.array_equal(a, b) and not np.array_equal(a, c)
+    assert abs(np.linalg.norm(a) - 1.0) < 1e-14
