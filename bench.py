@@ -101,6 +101,8 @@ def main():
     ap.add_argument("--no-lanczos", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--lanes", type=int, default=0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first copy 1 GiB device to device (the known-size kernel the HBM counters are scaled by)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU: every box here has one; run --gpus 1"}))
@@ -114,6 +116,13 @@ def main():
 
     if _lib.device_count() < 1:
         raise SystemExit("no HIP device visible")
+    if args.calibrate_copy:
+        src, dst = qs.DeviceArray(1 << 27, np.float64), qs.DeviceArray(1 << 27, np.float64)
+        src.zero()
+        _lib.call("qedh_memcpy_dtod", dst.ptr, src.ptr, C.c_size_t(1 << 30))
+        _lib.call("qedh_synchronize")
+        del src, dst
+        log("calibration copy: 1 GiB device to device")
 
     b = qs.HamiltonianBuilder(N)
     b.heisenberg([(i, (i + 1) % N) for i in range(N)], 1.0)

@@ -42,6 +42,7 @@ int qedh_malloc(void** ptr, size_t bytes);
 int qedh_free(void* ptr);
 int qedh_memcpy_htod(void* dst, const void* src, size_t bytes);
 int qedh_memcpy_dtoh(void* dst, const void* src, size_t bytes);
+int qedh_memcpy_dtod(void* dst, const void* src, size_t bytes);
 int qedh_memset(void* dst, int value, size_t bytes);
 int qedh_synchronize(void);
 
@@ -116,6 +117,8 @@ int qedh_matvec_apply_multi(qedh_matvec m, const void* const* ins, void* const* 
 /* Block-interleaved X[row * nv + i], nv in {1, 2, 4, 8}; CSR lane. */
 int qedh_matvec_apply_block(qedh_matvec m, int nv, const void* X, void* Y, int complex_vectors);
 int qedh_matvec_set_lanes(qedh_matvec m, int lanes);
+/* SpMV tuning: lanes per row (0 keeps the current) and non-temporal streaming of the CSR arrays. */
+int qedh_matvec_set_tuning(qedh_matvec m, int lanes, int nontemporal);
 /* The merged CSR (download_csr): row_ptr[rows+1], col[nnz], val[2 nnz] (re, im). */
 int qedh_matvec_export_csr(qedh_matvec m, uint64_t* row_ptr, uint32_t* col, double* val);
 /* Rows [r0, r1) of it: row_ptr[r1 - r0 + 1] (absolute offsets), then col / val of those rows

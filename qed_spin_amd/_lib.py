@@ -50,6 +50,7 @@ _SIGNATURES = {
     "qedh_free": [C.c_void_p],
     "qedh_memcpy_htod": [C.c_void_p, C.c_void_p, C.c_size_t],
     "qedh_memcpy_dtoh": [C.c_void_p, C.c_void_p, C.c_size_t],
+    "qedh_memcpy_dtod": [C.c_void_p, C.c_void_p, C.c_size_t],
     "qedh_memset": [C.c_void_p, C.c_int, C.c_size_t],
     "qedh_synchronize": [],
     "qedh_operator_create": [C.c_int, C.POINTER(C.c_void_p)],
@@ -75,6 +76,7 @@ _SIGNATURES = {
     "qedh_matvec_apply_multi": [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_int, C.c_int],
     "qedh_matvec_apply_block": [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int],
     "qedh_matvec_set_lanes": [C.c_void_p, C.c_int],
+    "qedh_matvec_set_tuning": [C.c_void_p, C.c_int, C.c_int],
     "qedh_matvec_export_csr": [C.c_void_p, c_u64_p, c_u32_p, c_dbl_p],
     "qedh_matvec_export_csr_rows": [C.c_void_p, C.c_int64, C.c_int64, c_u64_p, c_u32_p, c_dbl_p],
     "qedh_matvec_destroy": [C.c_void_p],

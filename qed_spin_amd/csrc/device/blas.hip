@@ -1,8 +1,8 @@
 // BLAS-1 of the Krylov drivers on device vectors of double or complex<double>, with fused forms
 // (axpy + dot, axpy + norm, dot_many / axpy_many for Gram-Schmidt). Every reduction is
 // deterministic: a fixed grid for a given n, a fixed xor-butterfly inside the wavefront, fixed
-// slots across wavefronts and blocks, and a final single-block sum in block order. The result stays
-// in device memory; the host reads it when it needs it.
+// slots across wavefronts and blocks, and a final tree over the block partials in a fixed order.
+// The result stays in device memory; the host reads it when it needs it.
 #include "dev_common.h"
 #include "launch.h"
 
@@ -25,7 +25,7 @@ __device__ __forceinline__ double wave_sum(double x) {
     return x;
 }
 
-/// Block sum of (a, b): every thread returns the block total.
+/// Block sum of (a, b): every thread returns the block total (wave sums, then the waves in order).
 __device__ __forceinline__ void block_sum2(double& a, double& b) {
     __shared__ double sa[kWaves], sb[kWaves];
     a = wave_sum(a);
@@ -47,12 +47,10 @@ __device__ __forceinline__ void block_sum2(double& a, double& b) {
 
 template <bool C> struct V;
 template <> struct V<false> {
-    using T = double;
     static __device__ __forceinline__ Zd load(const void* p, std::size_t i) { return Zd{static_cast<const double*>(p)[i], 0.0}; }
     static __device__ __forceinline__ void store(void* p, std::size_t i, Zd z) { static_cast<double*>(p)[i] = z.re; }
 };
 template <> struct V<true> {
-    using T = Zd;
     static __device__ __forceinline__ Zd load(const void* p, std::size_t i) { return static_cast<const Zd*>(p)[i]; }
     static __device__ __forceinline__ void store(void* p, std::size_t i, Zd z) { static_cast<Zd*>(p)[i] = z; }
 };
@@ -69,7 +67,7 @@ template <bool C> __global__ void k_fill_zero(void* x, std::size_t n) {
 }
 
 template <bool C> __global__ void k_axpby(Zd a, const void* x, Zd b, void* y, std::size_t n, int mode) {
-    // mode 0: y += a x; 1: x *= a (x is y); 2: y = a x + b y
+    // mode 0: y += a x; 1: y *= a; 2: y = a x + b y
     for (std::size_t i = blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x; i < n;
          i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
         if (mode == 1) {
@@ -118,52 +116,59 @@ __global__ __launch_bounds__(kThreads) void k_reduce(Zd a, const void* x, void* 
     }
 }
 
-// One block: sum the partials in block order. out_kind 0: Zd (dot), 1: sqrt (norm), 2: raw sum.
-__global__ __launch_bounds__(64) void k_finish(const double* partial, int nb, int out_kind, void* out) {
-    if (threadIdx.x != 0) return;
+/// Sum of nb partial pairs (stride 2) by one block: thread t takes t, t + 256, ... in order, then
+/// the block sum. out_kind 0: Zd, 1: sqrt of the first part (a norm), 2: the first part.
+__device__ void finish_pairs(const double* partial, int nb, int out_kind, void* out) {
     double a = 0.0, b = 0.0;
-    for (int i = 0; i < nb; ++i) {
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
         a += partial[2 * i];
         b += partial[2 * i + 1];
     }
-    if (out_kind == 0) *static_cast<Zd*>(out) = Zd{a, b};
-    else if (out_kind == 1) *static_cast<double*>(out) = sqrt(a);
-    else *static_cast<double*>(out) = a;
+    block_sum2(a, b);
+    if (threadIdx.x == 0) {
+        if (out_kind == 0) *static_cast<Zd*>(out) = Zd{a, b};
+        else if (out_kind == 1) *static_cast<double*>(out) = sqrt(a);
+        else *static_cast<double*>(out) = a;
+    }
 }
 
-// dot_many: the block's rows of w staged in LDS, then for every column j the block's partial of
-// V_j^H w -- wave sums into fixed LDS slots, summed across wavefronts in order.
-constexpr int kManyCols = 256;     // columns per launch
-constexpr int kManyRows = 2048;    // rows per block tile
+__global__ __launch_bounds__(kThreads) void k_finish(const double* partial, int nb, int out_kind, void* out) {
+    finish_pairs(partial, nb, out_kind, out);
+}
+
+// dot_many: block b owns a row range; the rows come in tiles of kTile staged in LDS (the w tile),
+// wavefront v owns the columns j = v, v + 4, ...: its lanes read 64 consecutive rows of column j
+// per load (coalesced), sum, and the wave total accumulates into the column's LDS slot (one owner:
+// no race). Partials are written column-major [j][b] so the finish reads them contiguously.
+constexpr int kManyCols = 256;   // columns per launch
+constexpr int kTile = 1024;      // rows per LDS tile
 
 template <bool C>
 __global__ __launch_bounds__(kThreads) void k_dot_many(const void* Vb, std::size_t ld, int m, const void* w,
                                                        std::size_t n, std::size_t rows_per_block, double* partial,
-                                                       int want_norm) {
-    __shared__ Zd sw[kManyRows];
-    __shared__ double sre[kWaves][kManyCols + 1], sim[kWaves][kManyCols + 1];
+                                                       int nb, int want_norm) {
+    __shared__ Zd sw[kTile];
+    __shared__ Zd acc[kManyCols + 1];
     const std::size_t r0 = blockIdx.x * rows_per_block;
     const std::size_t r1 = (r0 + rows_per_block < n) ? r0 + rows_per_block : n;
     const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
-    double acc_re[1] = {0.0};
-    for (int j = threadIdx.x; j < (m + 1) * kWaves; j += blockDim.x) {
-        sre[j / (m + 1)][j % (m + 1)] = 0.0;
-        sim[j / (m + 1)][j % (m + 1)] = 0.0;
-    }
+    for (int j = threadIdx.x; j <= m; j += blockDim.x) acc[j] = Zd{0.0, 0.0};
+    double nrm = 0.0;
     __syncthreads();
-    for (std::size_t t0 = r0; t0 < r1; t0 += kManyRows) {
-        const std::size_t t1 = (t0 + kManyRows < r1) ? t0 + kManyRows : r1;
-        const int cnt = static_cast<int>(t1 - t0);
+    for (std::size_t t0 = r0; t0 < r1; t0 += kTile) {
+        const int cnt = static_cast<int>((t0 + kTile < r1) ? kTile : (r1 - t0));
         for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
             const Zd v = V<C>::load(w, t0 + i);
             sw[i] = v;
-            if (want_norm) acc_re[0] += v.re * v.re + (C ? v.im * v.im : 0.0);
+            if (want_norm) nrm += v.re * v.re + (C ? v.im * v.im : 0.0);
         }
         __syncthreads();
-        for (int j = 0; j < m; ++j) {
+        for (int j = wv; j < m; j += kWaves) {
+            const std::size_t base = static_cast<std::size_t>(j) * ld + t0;
             double pr = 0.0, pi = 0.0;
-            for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
-                const Zd vj = V<C>::load(Vb, j * ld + t0 + i);
+#pragma unroll 4
+            for (int i = ln; i < cnt; i += 64) {
+                const Zd vj = V<C>::load(Vb, base + i);
                 if constexpr (C) {
                     const Zd d = cdot(vj, sw[i]);
                     pr += d.re;
@@ -174,43 +179,31 @@ __global__ __launch_bounds__(kThreads) void k_dot_many(const void* Vb, std::size
             }
             pr = wave_sum(pr);
             if constexpr (C) pi = wave_sum(pi);
-            if (ln == 0) {
-                sre[wv][j] += pr;
-                if constexpr (C) sim[wv][j] += pi;
-            }
+            if (ln == 0) acc[j] = Zd{acc[j].re + pr, acc[j].im + pi};
         }
         __syncthreads();
     }
-    // the norm of w in the last slot
     if (want_norm) {
-        double nr = wave_sum(acc_re[0]);
-        if (ln == 0) sre[wv][m] = nr;
+        double dummy = 0.0;
+        block_sum2(nrm, dummy);
+        if (threadIdx.x == 0) acc[m] = Zd{nrm, 0.0};
     }
     __syncthreads();
     for (int j = threadIdx.x; j <= m; j += blockDim.x) {
-        double a = 0.0, b = 0.0;
-        for (int q = 0; q < kWaves; ++q) {
-            a += sre[q][j];
-            b += sim[q][j];
-        }
-        partial[(static_cast<std::size_t>(blockIdx.x) * (m + 1) + j) * 2] = a;
-        partial[(static_cast<std::size_t>(blockIdx.x) * (m + 1) + j) * 2 + 1] = b;
+        partial[(static_cast<std::size_t>(j) * nb + blockIdx.x) * 2] = acc[j].re;
+        partial[(static_cast<std::size_t>(j) * nb + blockIdx.x) * 2 + 1] = acc[j].im;
     }
 }
 
-__global__ void k_dot_many_finish(const double* partial, int nb, int m, Zd* coeffs, double* wnorm2) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j > m) return;
-    double a = 0.0, b = 0.0;
-    for (int q = 0; q < nb; ++q) {
-        a += partial[(static_cast<std::size_t>(q) * (m + 1) + j) * 2];
-        b += partial[(static_cast<std::size_t>(q) * (m + 1) + j) * 2 + 1];
-    }
-    if (j < m) coeffs[j] = Zd{a, b};
-    else if (wnorm2) *wnorm2 = a;
+// One block per column: the sum of its nb block partials (contiguous) in a fixed order.
+__global__ __launch_bounds__(kThreads) void k_dot_many_finish(const double* partial, int nb, int m, Zd* coeffs,
+                                                              double* wnorm2) {
+    const int j = blockIdx.x;
+    if (j < m) finish_pairs(partial + static_cast<std::size_t>(j) * nb * 2, nb, 0, coeffs + j);
+    else if (wnorm2) finish_pairs(partial + static_cast<std::size_t>(j) * nb * 2, nb, 2, wnorm2);
 }
 
-// w += sum_j c_j V_j over the block's rows; the partial of ||w||^2 after the update.
+// w += sign sum_j c_j V_j over grid-strided rows, four columns per step; the partial of ||w||^2 after.
 template <bool C>
 __global__ __launch_bounds__(kThreads) void k_axpy_many(const Zd* coeffs, const void* Vb, std::size_t ld, int m,
                                                         void* w, std::size_t n, double* partial, int want_norm,
@@ -222,8 +215,26 @@ __global__ __launch_bounds__(kThreads) void k_axpy_many(const Zd* coeffs, const 
     for (std::size_t i = blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x; i < n;
          i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
         Zd acc = V<C>::load(w, i);
-        for (int j = 0; j < m; ++j) {
-            const Zd vj = V<C>::load(Vb, j * ld + i);
+        int j = 0;
+        for (; j + 4 <= m; j += 4) {
+            const Zd v0 = V<C>::load(Vb, static_cast<std::size_t>(j) * ld + i);
+            const Zd v1 = V<C>::load(Vb, static_cast<std::size_t>(j + 1) * ld + i);
+            const Zd v2 = V<C>::load(Vb, static_cast<std::size_t>(j + 2) * ld + i);
+            const Zd v3 = V<C>::load(Vb, static_cast<std::size_t>(j + 3) * ld + i);
+            if constexpr (C) {
+                acc = cadd(acc, cmul(sc[j], v0));
+                acc = cadd(acc, cmul(sc[j + 1], v1));
+                acc = cadd(acc, cmul(sc[j + 2], v2));
+                acc = cadd(acc, cmul(sc[j + 3], v3));
+            } else {
+                acc.re += sc[j].re * v0.re;
+                acc.re += sc[j + 1].re * v1.re;
+                acc.re += sc[j + 2].re * v2.re;
+                acc.re += sc[j + 3].re * v3.re;
+            }
+        }
+        for (; j < m; ++j) {
+            const Zd vj = V<C>::load(Vb, static_cast<std::size_t>(j) * ld + i);
             acc = C ? cadd(acc, cmul(sc[j], vj)) : Zd{acc.re + sc[j].re * vj.re, 0.0};
         }
         V<C>::store(w, i, acc);
@@ -238,12 +249,36 @@ __global__ __launch_bounds__(kThreads) void k_axpy_many(const Zd* coeffs, const 
     }
 }
 
+constexpr int kRotCols = 16;
+template <bool C>
+__global__ __launch_bounds__(256) void k_block_rotate(const void* Vb, std::size_t ld, int mm, const double* S, int c0,
+                                                      int nc, void* out, std::size_t n) {
+    const std::size_t i = blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x;
+    if (i >= n) return;
+    Zd acc[kRotCols];
+#pragma unroll
+    for (int c = 0; c < kRotCols; ++c) acc[c] = Zd{0.0, 0.0};
+    for (int r = 0; r < mm; ++r) {
+        const Zd v = V<C>::load(Vb, r * ld + i);
+#pragma unroll
+        for (int c = 0; c < kRotCols; ++c)
+            if (c < nc) {
+                const double s = S[static_cast<std::size_t>(c0 + c) * mm + r];
+                acc[c].re += v.re * s;
+                if constexpr (C) acc[c].im += v.im * s;
+            }
+    }
+#pragma unroll
+    for (int c = 0; c < kRotCols; ++c)
+        if (c < nc) V<C>::store(out, static_cast<std::size_t>(c0 + c) * n + i, acc[c]);
+}
+
 }  // namespace
 
 std::size_t blas_scratch_bytes(std::size_t max_cols) {
+    (void)max_cols;
     const std::size_t red = 2 * kMaxBlocks * sizeof(double);
     const std::size_t many = static_cast<std::size_t>(kMaxBlocks) * (kManyCols + 1) * 2 * sizeof(double);
-    (void)max_cols;
     return red > many ? red : many;
 }
 
@@ -278,7 +313,7 @@ static void reduce(bool c, int kind, Zd a, const void* x, void* y, const void* z
         if (kind == 0) hipLaunchKernelGGL((k_reduce<false, 0>), dim3(g), dim3(kThreads), 0, st, a, x, y, z, n, fuse, part);
         else hipLaunchKernelGGL((k_reduce<false, 1>), dim3(g), dim3(kThreads), 0, st, a, x, y, z, n, fuse, part);
     }
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, st, part, static_cast<int>(g), out_kind, out);
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(kThreads), 0, st, part, static_cast<int>(g), out_kind, out);
 }
 
 void dot(bool c, const void* x, const void* y, std::size_t n, void* scratch, Zd* result, hipStream_t st) {
@@ -301,22 +336,22 @@ void dot_many(bool c, const void* V, std::size_t ld, std::size_t m, const void* 
     const std::size_t elem = c ? sizeof(Zd) : sizeof(double);
     std::size_t done = 0;
     bool first = true;
+    std::size_t nb = (n + kTile - 1) / kTile;
+    if (nb > static_cast<std::size_t>(kMaxBlocks)) nb = kMaxBlocks;
+    if (nb < 1) nb = 1;
+    const std::size_t rpb = (n + nb - 1) / nb;
     do {
         const int mc = static_cast<int>((m - done) < static_cast<std::size_t>(kManyCols) ? (m - done) : kManyCols);
-        std::size_t nb = (n + kManyRows - 1) / kManyRows;
-        if (nb > static_cast<std::size_t>(kMaxBlocks)) nb = kMaxBlocks;
-        if (nb < 1) nb = 1;
-        const std::size_t rpb = (n + nb - 1) / nb;
         const void* Vc = static_cast<const char*>(V) + done * ld * elem;
         const int want = (first && wnorm2) ? 1 : 0;
         if (c)
             hipLaunchKernelGGL(k_dot_many<true>, dim3(static_cast<unsigned>(nb)), dim3(kThreads), 0, st, Vc, ld, mc, w,
-                               n, rpb, part, want);
+                               n, rpb, part, static_cast<int>(nb), want);
         else
             hipLaunchKernelGGL(k_dot_many<false>, dim3(static_cast<unsigned>(nb)), dim3(kThreads), 0, st, Vc, ld, mc,
-                               w, n, rpb, part, want);
-        hipLaunchKernelGGL(k_dot_many_finish, dim3((mc + 1 + 63) / 64), dim3(64), 0, st, part, static_cast<int>(nb),
-                           mc, coeffs + done, want ? wnorm2 : nullptr);
+                               w, n, rpb, part, static_cast<int>(nb), want);
+        hipLaunchKernelGGL(k_dot_many_finish, dim3(static_cast<unsigned>(mc + want)), dim3(kThreads), 0, st, part,
+                           static_cast<int>(nb), mc, coeffs + done, want ? wnorm2 : nullptr);
         done += static_cast<std::size_t>(mc);
         first = false;
     } while (done < m);
@@ -339,51 +374,21 @@ void axpy_many(bool c, const Zd* coeffs, const void* V, std::size_t ld, std::siz
         else
             hipLaunchKernelGGL(k_axpy_many<false>, dim3(g), dim3(kThreads), 0, st, coeffs + done, Vc, ld, mc, w, n,
                                part, want, sign);
-        if (want) hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, st, part, static_cast<int>(g), 2, wnorm2);
+        if (want) hipLaunchKernelGGL(k_finish, dim3(1), dim3(kThreads), 0, st, part, static_cast<int>(g), 2, wnorm2);
         done += static_cast<std::size_t>(mc);
     } while (done < m);
 }
 
-}  // namespace qsa::dev
-
-namespace qsa::dev {
-
-namespace {
-constexpr int kRotCols = 16;
-template <bool C>
-__global__ __launch_bounds__(256) void k_block_rotate(const void* Vb, std::size_t ld, int mm, const double* S, int c0,
-                                                      int nc, void* out, std::size_t n) {
-    const std::size_t i = blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x;
-    if (i >= n) return;
-    Zd acc[kRotCols];
-#pragma unroll
-    for (int c = 0; c < kRotCols; ++c) acc[c] = Zd{0.0, 0.0};
-    for (int r = 0; r < mm; ++r) {
-        const Zd v = V<C>::load(Vb, r * ld + i);
-#pragma unroll
-        for (int c = 0; c < kRotCols; ++c)
-            if (c < nc) {
-                const double s = S[static_cast<std::size_t>(c0 + c) * mm + r];
-                acc[c].re += v.re * s;
-                if constexpr (C) acc[c].im += v.im * s;
-            }
-    }
-#pragma unroll
-    for (int c = 0; c < kRotCols; ++c)
-        if (c < nc) V<C>::store(out, static_cast<std::size_t>(c0 + c) * n + i, acc[c]);
-}
-}  // namespace
-
-void block_rotate(bool c, const void* V, std::size_t ld, std::size_t mm, const double* S, std::size_t cols, void* out,
+void block_rotate(bool c, const void* Vm, std::size_t ld, std::size_t mm, const double* S, std::size_t cols, void* out,
                   std::size_t n, hipStream_t st) {
     const dim3 grid(static_cast<unsigned>((n + 255) / 256)), block(256);
     for (std::size_t c0 = 0; c0 < cols; c0 += kRotCols) {
         const int nc = static_cast<int>(std::min<std::size_t>(kRotCols, cols - c0));
         if (c)
-            hipLaunchKernelGGL(k_block_rotate<true>, grid, block, 0, st, V, ld, static_cast<int>(mm), S,
+            hipLaunchKernelGGL(k_block_rotate<true>, grid, block, 0, st, Vm, ld, static_cast<int>(mm), S,
                                static_cast<int>(c0), nc, out, n);
         else
-            hipLaunchKernelGGL(k_block_rotate<false>, grid, block, 0, st, V, ld, static_cast<int>(mm), S,
+            hipLaunchKernelGGL(k_block_rotate<false>, grid, block, 0, st, Vm, ld, static_cast<int>(mm), S,
                                static_cast<int>(c0), nc, out, n);
     }
 }

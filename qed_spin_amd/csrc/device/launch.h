@@ -67,6 +67,7 @@ struct CsrView {
     const double* dict_im = nullptr;
     int lanes = 16;                // lanes per row: 4, 8, 16, 32 or 64
     bool values_real = false;      // the stored values are real (full values stored as double)
+    int nt = 0;                    // stream the column / id arrays with non-temporal loads
 };
 /// out_v = A in_v (real: double vectors; else complex vectors, with real or complex values).
 void csr_spmv(const CsrView& A, bool real, const VecSet& v, hipStream_t st);

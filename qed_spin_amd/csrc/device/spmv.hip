@@ -20,11 +20,11 @@ template <int NV> struct Ptrs {
 template <int WIDTH, bool VREAL>
 __device__ __forceinline__ void value_at(const CsrView& A, u64 e, double& vr, double& vi) {
     if constexpr (WIDTH == 1) {
-        const u32 id = static_cast<const u8*>(A.vals)[e];
+        const u32 id = A.nt ? __builtin_nontemporal_load(static_cast<const u8*>(A.vals) + e) : static_cast<const u8*>(A.vals)[e];
         vr = A.dict_re[id];
         if constexpr (!VREAL) vi = A.dict_im[id];
     } else if constexpr (WIDTH == 2) {
-        const u32 id = static_cast<const u16*>(A.vals)[e];
+        const u32 id = A.nt ? __builtin_nontemporal_load(static_cast<const u16*>(A.vals) + e) : static_cast<const u16*>(A.vals)[e];
         vr = A.dict_re[id];
         if constexpr (!VREAL) vi = A.dict_im[id];
     } else {
@@ -49,7 +49,7 @@ __global__ __launch_bounds__(256) void k_spmv(CsrView A, Ptrs<NV> v) {
     if (row < A.rows) {
         const u64 e0 = A.row_ptr[row], e1 = A.row_ptr[row + 1];
         for (u64 e = e0 + lane; e < e1; e += L) {
-            const u32 c = A.col[e];
+            const u32 c = A.nt ? __builtin_nontemporal_load(A.col + e) : A.col[e];
             double vr, vi = 0.0;
             value_at<WIDTH, VREAL>(A, e, vr, vi);
 #pragma unroll
@@ -93,7 +93,7 @@ __global__ __launch_bounds__(256) void k_spmv_block(CsrView A, const void* X, vo
     if (row < A.rows) {
         const u64 e0 = A.row_ptr[row], e1 = A.row_ptr[row + 1];
         for (u64 e = e0 + lane; e < e1; e += L) {
-            const u64 c = A.col[e];
+            const u64 c = A.nt ? __builtin_nontemporal_load(A.col + e) : A.col[e];
             double vr, vi = 0.0;
             value_at<WIDTH, VREAL>(A, e, vr, vi);
 #pragma unroll

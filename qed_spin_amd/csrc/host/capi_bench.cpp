@@ -88,3 +88,31 @@ extern "C" int qedh_matvec_export_csr_rows(qedh_matvec m, int64_t r0, int64_t r1
         return qsa::capi_status(e);
     }
 }
+
+extern "C" int qedh_matvec_set_tuning(qedh_matvec m, int lanes, int nontemporal) {
+    try {
+        if (!m) throw qsa::Error(qsa::kInvalid, "matvec is null");
+        if (lanes != 0 && lanes != 4 && lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64)
+            throw qsa::Error(qsa::kInvalid, "lanes must be 0 (keep), 4, 8, 16, 32 or 64");
+        if (lanes) m->op->set_lanes(lanes);
+        m->op->set_nontemporal(nontemporal != 0);
+        return 0;
+    } catch (const qsa::Error& e) {
+        qsa::capi_set_error(e.what());
+        return e.code;
+    } catch (const std::exception& e) {
+        return qsa::capi_status(e);
+    }
+}
+
+extern "C" int qedh_memcpy_dtod(void* dst, const void* src, size_t bytes) {
+    try {
+        if (bytes) QSA_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+        return 0;
+    } catch (const qsa::Error& e) {
+        qsa::capi_set_error(e.what());
+        return e.code;
+    } catch (const std::exception& e) {
+        return qsa::capi_status(e);
+    }
+}

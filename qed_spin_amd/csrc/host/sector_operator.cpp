@@ -133,6 +133,7 @@ dev::CsrView SectorOperator::csr_view() const {
     A.dict_im = dict_im_.get<double>();
     A.lanes = csr_.lanes;
     A.values_real = real_;
+    A.nt = nt_ ? 1 : 0;
     return A;
 }
 

@@ -58,6 +58,9 @@ public:
                            std::vector<std::uint32_t>& col, std::vector<double>& val) const;
     /// Pick the lanes per row of the SpMV (4..64).
     void set_lanes(int lanes) { csr_.lanes = lanes; }
+    /// Stream the CSR's column / id arrays with non-temporal loads (they are read once per apply).
+    void set_nontemporal(bool on) { nt_ = on; }
+    bool nontemporal() const { return nt_; }
 
 private:
     void build_csr(std::uint64_t budget);
@@ -68,6 +71,7 @@ private:
     double norm_bound_;
     bool real_;
     int lane_ = kLaneGather;
+    bool nt_ = false;
     CsrInfo csr_;
     DeviceBuffer row_ptr_, col_, vals_, dict_re_, dict_im_;
 };
