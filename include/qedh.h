@@ -152,6 +152,37 @@ int qedh_cf(qedh_matvec m, const double* phi_host, int krylov, int n_omega, cons
             int has_shift, double energy_shift, double breakdown_tol, double* S, double* out_shift,
             double* out_norm_sq, double* out_change);
 
+/* ---- Batched Krylov engine: many blocks x R vectors per block in one packed multi-vector
+ * matvec per step, the step captured as a hipGraph (replaces the per-block, lockstep-of-8 sample
+ * loop of include/ed/thermal/ftlm.h:277-480 driven by src/engine/thermal.cpp:161-344, 471-535) ---- */
+/* LocalDGKS3 Lanczos (K = 1, no basis) of R vectors in each of n_blocks square CSR-lane blocks.
+ * start: the complex start vectors in batch layout, block b vector s at entry (off_b + i) R + s,
+ * off_b = the summed dimensions of the blocks before b (normalised inside). breakdown_tol[b] per
+ * block. Outputs alpha[krylov][n_blocks][R], beta[krylov][n_blocks][R] (beta of step j is
+ * beta_{j+1}), steps[n_blocks][R]. timings[3] = pack s, steps s, total s (may be NULL). */
+int qedh_batch_lanczos(const qedh_matvec* blocks, int n_blocks, int R, int krylov, const double* start,
+                       const double* breakdown_tol, int use_graph, double* alpha, double* beta, int* steps,
+                       double* timings);
+/* FTLM of n_blocks blocks at once: block b samples s = 0..samples-1 from
+ * splitmix64(block_seeds[b] + phi (s + 1)) as qedh_ftlm does, breakdown_tol[b] per block (NULL:
+ * 64 eps times the block's norm bound, kBreakdownRel of include/ed/core/numerics.h:23).
+ * lnZ/E/V[n_blocks][n_beta], e_min[n_blocks]. stats[8] (may be NULL) = pack s, seed s, steps s,
+ * post s, total s, block-vector applies, packed rows, packed nnz. */
+int qedh_ftlm_batched(const qedh_matvec* blocks, int n_blocks, int samples, int krylov, const uint64_t* block_seeds,
+                      const double* breakdown_tol, int n_beta, const double* betas, int use_graph, double* lnZ,
+                      double* E, double* V, double* e_min, double* stats);
+/* Exact thermodynamics of small blocks (the reference's host dense lane for blocks of at most
+ * dense_max_dim states, src/engine/thermal.cpp:197-200, curves of include/ed/thermal/curves.h:38-65):
+ * each block's CSR expanded to a dense Hermitian matrix on the host and its whole spectrum solved.
+ * lnZ/E/V[n_blocks][n_beta], e_min[n_blocks]; blocks of at most 8192 states. */
+int qedh_exact_thermal(const qedh_matvec* blocks, int n_blocks, int n_beta, const double* betas, double* lnZ,
+                       double* E, double* V, double* e_min);
+/* The system's curves from its blocks' (src/engine/thermal.cpp:597-640): weights[n_blocks]
+ * (multiplicities), inputs [n_blocks][n_beta]; outputs [n_beta]: lnZ, E, C = beta^2 Var, S, F. */
+int qedh_thermal_combine(int n_blocks, int n_beta, const double* betas, const double* weights, const double* lnZ,
+                         const double* E, const double* V, double* out_lnZ, double* out_E, double* out_C,
+                         double* out_S, double* out_F);
+
 #ifdef __cplusplus
 }
 #endif

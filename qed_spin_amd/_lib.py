@@ -90,6 +90,20 @@ _SIGNATURES = {
         C.c_void_p, c_dbl_p, C.c_int, C.c_int, c_dbl_p, C.c_double, C.c_int, C.c_double, C.c_double, c_dbl_p,
         c_dbl_p, c_dbl_p, c_dbl_p,
     ],
+    "qedh_batch_lanczos": [
+        C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_int, c_dbl_p, c_dbl_p, C.c_int, c_dbl_p, c_dbl_p, c_int_p,
+        c_dbl_p,
+    ],
+    "qedh_ftlm_batched": [
+        C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_int, c_u64_p, c_dbl_p, C.c_int, c_dbl_p, C.c_int, c_dbl_p,
+        c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p,
+    ],
+    "qedh_exact_thermal": [
+        C.POINTER(C.c_void_p), C.c_int, C.c_int, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p,
+    ],
+    "qedh_thermal_combine": [
+        C.c_int, C.c_int, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p,
+    ],
 }
 
 EXPORTED = sorted(list(_SIGNATURES) + ["qedh_last_error"])

@@ -112,3 +112,38 @@ namespace qsa::dev {
 /// index_and_projection of n states (device arrays): idx[i], proj[2 i], proj[2 i + 1].
 void index_proj_batch(const SectorView& S, const u64* states, i64 n, i64* idx, double* proj, hipStream_t st);
 }  // namespace qsa::dev
+
+// ---- batch.hip: many blocks x R vectors per Lanczos step ----------------------------------------
+namespace qsa::dev {
+/// Rows [row_begin, row_end) of one block (at most 256 rows).
+struct Tile {
+    u64 row_begin, row_end;
+    u32 block, pad;
+};
+struct BatchView {
+    u32 n_blocks = 0, n_tiles = 0;
+    int R = 0;                          // vectors per block (<= 64), X[row R + s]
+    const Tile* tiles = nullptr;        // n_tiles, blocks contiguous
+    const u32* tile_begin = nullptr;    // n_blocks + 1
+    const u64* row_ptr = nullptr;       // packed block-diagonal CSR
+    const u32* col = nullptr;
+    const Zd* val = nullptr;
+    int* step = nullptr;                // the current step (device; advanced by batch_advance)
+    int* active = nullptr;              // n_blocks x R: the pair is still running
+    int* steps = nullptr;               // n_blocks x R: steps recorded
+    const int* cap = nullptr;           // n_blocks: Krylov steps of the block (min(M, dim))
+};
+constexpr int kBatchTileRows = 256;
+constexpr int kBatchMaxR = 64;
+void batch_spmv(const BatchView& B, const Zd* X, Zd* Y, hipStream_t st);
+void batch_vec(const BatchView& B, int mode, const Zd* Vprev, const Zd* Vcur, Zd* W, const double* coef,
+               const Zd* zcoef, double threshold, double* partial, hipStream_t st);
+void batch_finish(const BatchView& B, int kind, int step, const double* partial, double* alpha_hist,
+                  double* beta_hist, Zd* ov, double* cur_alpha, double* cur_beta, double* inv_beta,
+                  const double* breakdown_tol, hipStream_t st);
+void batch_normalize(const BatchView& B, Zd* V, double* partial, double* inv, hipStream_t st);
+void batch_fill_zero(Zd* x, u64 n, hipStream_t st);
+void pack_csr(const CsrView& A, u64 row_off, u64 col_off, u64 nnz_off, u64* row_ptr, u32* col, Zd* val,
+              hipStream_t st);
+void batch_advance(const BatchView& B, hipStream_t st);
+}  // namespace qsa::dev

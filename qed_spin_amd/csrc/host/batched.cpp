@@ -1,0 +1,328 @@
+#include "batched.h"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <thread>
+
+#include "tridiag.h"
+
+namespace qsa {
+
+namespace {
+
+using clock_t_ = std::chrono::steady_clock;
+double secs(clock_t_::time_point a, clock_t_::time_point b) { return std::chrono::duration<double>(b - a).count(); }
+
+/// Host worker threads for the per-pair host work (seeds, tridiagonals): at most 16.
+unsigned host_threads() {
+    const unsigned hc = std::max(1u, std::thread::hardware_concurrency());
+    return std::min(16u, hc);
+}
+
+/// fn(i) for i < n on the host worker threads (dynamic schedule); the first exception is rethrown.
+template <class Fn> void parallel_for(std::size_t n, Fn&& fn) {
+    const unsigned T = static_cast<unsigned>(std::min<std::size_t>(host_threads(), n));
+    if (T <= 1) {
+        for (std::size_t i = 0; i < n; ++i) fn(i);
+        return;
+    }
+    std::atomic<std::size_t> next{0};
+    std::exception_ptr failure;
+    std::atomic<bool> failed{false};
+    std::vector<std::thread> pool;
+    for (unsigned t = 0; t < T; ++t)
+        pool.emplace_back([&] {
+            for (;;) {
+                const std::size_t i = next.fetch_add(1);
+                if (i >= n || failed.load()) return;
+                try {
+                    fn(i);
+                } catch (...) {
+                    if (!failed.exchange(true)) failure = std::current_exception();
+                    return;
+                }
+            }
+        });
+    for (auto& th : pool) th.join();
+    if (failure) std::rethrow_exception(failure);
+}
+
+template <class T> void put(DeviceBuffer& b, const std::vector<T>& h, hipStream_t st) {
+    b.reset(h.size() * sizeof(T));
+    if (!h.empty()) QSA_HIP(hipMemcpyAsync(b.get(), h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st));
+    QSA_HIP(hipStreamSynchronize(st));
+}
+
+/// A stream of its own at the default priority (graph capture needs a stream other than the null one).
+struct OwnStream {
+    hipStream_t s = nullptr;
+    OwnStream() { QSA_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)); }
+    ~OwnStream() {
+        if (s) (void)hipStreamDestroy(s);
+    }
+};
+
+}  // namespace
+
+void parallel_blocks(std::size_t n, const std::function<void(std::size_t)>& fn) { parallel_for(n, fn); }
+
+BatchedLanczos::BatchedLanczos(std::vector<std::shared_ptr<const SectorOperator>> blocks, int R)
+    : ops_(std::move(blocks)), R_(R) {
+    if (ops_.empty()) throw Error(kInvalid, "batched lanczos: no blocks");
+    if (R_ < 1 || R_ > dev::kBatchMaxR)
+        throw Error(kInvalid, "batched lanczos: vectors per block must be 1.." + std::to_string(dev::kBatchMaxR));
+    row_off_.assign(1, 0);
+    std::vector<std::uint64_t> nnz_off(1, 0);
+    for (const auto& op : ops_) {
+        if (!op) throw Error(kInvalid, "batched lanczos: null block");
+        if (!op->same()) throw Error(kInvalid, "batched lanczos: a block must map its sector to itself");
+        if (op->lane() != kLaneCsr)
+            throw Error(kDeviceUnsupported, "batched lanczos: every block needs its device CSR (csr lane)");
+        if (op->dim() <= 0) throw Error(kInvalid, "batched lanczos: empty block");
+        row_off_.push_back(row_off_.back() + static_cast<std::uint64_t>(op->dim()));
+        nnz_off.push_back(nnz_off.back() + static_cast<std::uint64_t>(op->csr_info().nnz));
+    }
+    rows_ = row_off_.back();
+    nnz_ = nnz_off.back();
+    if (rows_ >= (std::uint64_t{1} << 32)) throw Error(kInvalid, "batched lanczos: more than 2^32 rows in one batch");
+    // The operators' CSRs were built on the null stream; the packing reads them.
+    QSA_HIP(hipStreamSynchronize(nullptr));
+    const hipStream_t st = nullptr;
+    row_ptr_.reset((rows_ + 1) * 8);
+    col_.reset(std::max<std::uint64_t>(nnz_, 1) * 4);
+    val_.reset(std::max<std::uint64_t>(nnz_, 1) * sizeof(dev::Zd));
+    for (std::size_t b = 0; b < ops_.size(); ++b)
+        dev::pack_csr(ops_[b]->device_csr(), row_off_[b], row_off_[b], nnz_off[b], row_ptr_.get<std::uint64_t>(),
+                      col_.get<std::uint32_t>(), val_.get<dev::Zd>(), st);
+    QSA_HIP(hipGetLastError());
+    // Tiles: every block cut into runs of at most kBatchTileRows rows, blocks in order.
+    tile_begin_.assign(1, 0);
+    for (std::size_t b = 0; b < ops_.size(); ++b) {
+        for (std::uint64_t r = row_off_[b]; r < row_off_[b + 1]; r += dev::kBatchTileRows) {
+            dev::Tile t{};
+            t.row_begin = r;
+            t.row_end = std::min<std::uint64_t>(r + dev::kBatchTileRows, row_off_[b + 1]);
+            t.block = static_cast<std::uint32_t>(b);
+            tiles_.push_back(t);
+        }
+        tile_begin_.push_back(static_cast<std::uint32_t>(tiles_.size()));
+    }
+    put(tiles_d_, tiles_, st);
+    put(tile_begin_d_, tile_begin_, st);
+    const std::size_t vec_bytes = rows_ * static_cast<std::size_t>(R_) * sizeof(dev::Zd);
+    V0_.reset(vec_bytes);
+    V1_.reset(vec_bytes);
+    W_.reset(vec_bytes);
+    partial_.reset(tiles_.size() * 64 * 2 * sizeof(double));
+    const std::size_t pairs = ops_.size() * static_cast<std::size_t>(R_);
+    ov_.reset(pairs * sizeof(dev::Zd));
+    cur_a_.reset(pairs * 8);
+    cur_b_.reset(pairs * 8);
+    inv_b_.reset(pairs * 8);
+    active_d_.reset(pairs * sizeof(int));
+    steps_d_.reset(pairs * sizeof(int));
+    step_d_.reset(sizeof(int));
+    cap_d_.reset(ops_.size() * sizeof(int));
+    tol_d_.reset(ops_.size() * sizeof(double));
+    dev::batch_fill_zero(V1_.get<dev::Zd>(), rows_ * R_, st);
+    dev::batch_fill_zero(W_.get<dev::Zd>(), rows_ * R_, st);
+    QSA_HIP(hipStreamSynchronize(st));
+}
+
+dev::BatchView BatchedLanczos::view() const {
+    dev::BatchView B;
+    B.n_blocks = static_cast<std::uint32_t>(ops_.size());
+    B.n_tiles = static_cast<std::uint32_t>(tiles_.size());
+    B.R = R_;
+    B.tiles = tiles_d_.get<dev::Tile>();
+    B.tile_begin = tile_begin_d_.get<std::uint32_t>();
+    B.row_ptr = row_ptr_.get<std::uint64_t>();
+    B.col = col_.get<std::uint32_t>();
+    B.val = val_.get<dev::Zd>();
+    B.step = step_d_.get<int>();
+    B.active = active_d_.get<int>();
+    B.steps = steps_d_.get<int>();
+    B.cap = cap_d_.get<int>();
+    return B;
+}
+
+void BatchedLanczos::set_start(const cplx* host) {
+    const hipStream_t st = nullptr;
+    QSA_HIP(hipMemcpyAsync(V0_.get(), host, rows_ * R_ * sizeof(dev::Zd), hipMemcpyHostToDevice, st));
+    dev::BatchView B = view();
+    B.active = nullptr;
+    dev::batch_normalize(B, V0_.get<dev::Zd>(), partial_.get<double>(), inv_b_.get<double>(), st);
+    QSA_HIP(hipGetLastError());
+    QSA_HIP(hipStreamSynchronize(st));
+}
+
+void BatchedLanczos::apply(const cplx* X, cplx* Y) const {
+    dev::BatchView B = view();
+    B.active = nullptr;
+    dev::batch_spmv(B, reinterpret_cast<const dev::Zd*>(X), reinterpret_cast<dev::Zd*>(Y), nullptr);
+    QSA_HIP(hipGetLastError());
+}
+
+void BatchedLanczos::run(int M, const std::vector<double>& breakdown_tol, bool use_graph, std::vector<double>& alpha,
+                         std::vector<double>& beta, std::vector<int>& steps, BatchedTimings& t) {
+    if (M < 1) throw Error(kInvalid, "batched lanczos: at least one step");
+    if (breakdown_tol.size() != ops_.size()) throw Error(kInvalid, "batched lanczos: one breakdown tolerance per block");
+    const std::size_t nb = ops_.size(), pairs = nb * static_cast<std::size_t>(R_);
+    const std::size_t hist = static_cast<std::size_t>(M) * pairs;
+    OwnStream own;
+    const hipStream_t st = own.s;
+    QSA_HIP(hipStreamSynchronize(nullptr));
+    std::vector<int> cap(nb), ones(pairs, 1), zeros(pairs, 0);
+    for (std::size_t b = 0; b < nb; ++b) cap[b] = static_cast<int>(std::min<std::uint64_t>(M, dim(static_cast<int>(b))));
+    put(cap_d_, cap, st);
+    put(tol_d_, breakdown_tol, st);
+    put(active_d_, ones, st);
+    put(steps_d_, zeros, st);
+    put(step_d_, std::vector<int>{0}, st);
+    alpha_h_.reset(hist * 8);
+    beta_h_.reset(hist * 8);
+    dev::fill_zero(false, alpha_h_.get(), hist, st);
+    dev::fill_zero(false, beta_h_.get(), hist, st);
+    dev::fill_zero(false, cur_b_.get(), pairs, st);
+    QSA_HIP(hipStreamSynchronize(st));
+
+    const dev::BatchView B = view();
+    dev::Zd* V[3] = {V1_.get<dev::Zd>(), V0_.get<dev::Zd>(), W_.get<dev::Zd>()};
+    double* partial = partial_.get<double>();
+    double* ah = alpha_h_.get<double>();
+    double* bh = beta_h_.get<double>();
+    dev::Zd* ov = ov_.get<dev::Zd>();
+    double* ca = cur_a_.get<double>();
+    double* cb = cur_b_.get<double>();
+    double* ib = inv_b_.get<double>();
+    const double* tol = tol_d_.get<double>();
+    const double thr = 1.49011611938476562e-08;   // sqrt(eps): LocalDGKS3's overlap threshold
+    auto step = [&](int phase) {
+        dev::Zd* p = V[phase % 3];
+        dev::Zd* c = V[(phase + 1) % 3];
+        dev::Zd* w = V[(phase + 2) % 3];
+        dev::batch_spmv(B, c, w, st);
+        dev::batch_vec(B, 0, p, c, w, cb, nullptr, 0.0, partial, st);
+        dev::batch_finish(B, 0, -1, partial, ah, bh, ov, ca, cb, ib, tol, st);
+        dev::batch_vec(B, 1, nullptr, c, w, ca, nullptr, 0.0, partial, st);
+        dev::batch_finish(B, 1, -1, partial, ah, bh, ov, ca, cb, ib, tol, st);
+        dev::batch_vec(B, 2, nullptr, c, w, nullptr, ov, thr, partial, st);
+        dev::batch_finish(B, 2, -1, partial, ah, bh, ov, ca, cb, ib, tol, st);
+        dev::batch_vec(B, 3, nullptr, nullptr, w, ib, nullptr, 0.0, partial, st);
+        dev::batch_advance(B, st);
+    };
+    const int Mrun = *std::max_element(cap.begin(), cap.end());
+    hipEvent_t e0, e1;
+    QSA_HIP(hipEventCreate(&e0));
+    QSA_HIP(hipEventCreate(&e1));
+    hipGraphExec_t exec[3] = {nullptr, nullptr, nullptr};
+    hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
+    auto cleanup = [&] {
+        for (int i = 0; i < 3; ++i) {
+            if (exec[i]) (void)hipGraphExecDestroy(exec[i]);
+            if (graph[i]) (void)hipGraphDestroy(graph[i]);
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    };
+    try {
+        if (use_graph) {
+            for (int ph = 0; ph < 3 && ph < Mrun; ++ph) {
+                QSA_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                step(ph);
+                QSA_HIP(hipStreamEndCapture(st, &graph[ph]));
+                QSA_HIP(hipGraphInstantiate(&exec[ph], graph[ph], nullptr, nullptr, 0));
+            }
+        }
+        QSA_HIP(hipEventRecord(e0, st));
+        for (int j = 0; j < Mrun; ++j) {
+            if (use_graph) QSA_HIP(hipGraphLaunch(exec[j % 3], st));
+            else step(j % 3);
+        }
+        QSA_HIP(hipEventRecord(e1, st));
+        QSA_HIP(hipGetLastError());
+        QSA_HIP(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        QSA_HIP(hipEventElapsedTime(&ms, e0, e1));
+        t.steps_s += ms * 1e-3;
+        t.steps += Mrun;
+        t.graph = use_graph;
+    } catch (...) {
+        (void)hipStreamSynchronize(st);
+        cleanup();
+        throw;
+    }
+    cleanup();
+    alpha.resize(hist);
+    beta.resize(hist);
+    steps.resize(pairs);
+    QSA_HIP(hipMemcpy(alpha.data(), ah, hist * 8, hipMemcpyDeviceToHost));
+    QSA_HIP(hipMemcpy(beta.data(), bh, hist * 8, hipMemcpyDeviceToHost));
+    QSA_HIP(hipMemcpy(steps.data(), steps_d_.get(), pairs * sizeof(int), hipMemcpyDeviceToHost));
+    for (int s : steps) t.applies += static_cast<std::uint64_t>(s);
+    t.rows = rows_;
+    t.nnz = nnz_;
+}
+
+std::vector<Curves> batched_ftlm(const std::vector<std::shared_ptr<const SectorOperator>>& blocks,
+                                 const BatchedFtlmOptions& o, BatchedTimings& t) {
+    const auto t_all = clock_t_::now();
+    const std::size_t nb = blocks.size();
+    if (nb == 0) return {};
+    if (o.samples < 1 || o.krylov < 1 || o.betas.empty()) throw Error(kInvalid, "ftlm: bad sizes");
+    if (o.block_seeds.size() != nb || o.breakdown_tol.size() != nb)
+        throw Error(kInvalid, "ftlm: one seed and one breakdown tolerance per block");
+    const auto t_pack = clock_t_::now();
+    const int R = std::min(o.samples, dev::kBatchMaxR);
+    BatchedLanczos L(blocks, R);
+    t.pack_s += secs(t_pack, clock_t_::now());
+    std::vector<std::vector<SampleMoments>> mom(nb);
+    std::vector<double> alpha, beta;
+    std::vector<int> steps;
+    std::vector<cplx> host(L.rows() * static_cast<std::size_t>(R));
+    for (int s0 = 0; s0 < o.samples; s0 += R) {
+        const int Rc = std::min(R, o.samples - s0);
+        // Samples of this round: pair (block, s) starts from sample s0 + s of the block's stream; the
+        // unused columns of a short last round hold copies of its first sample (never read back).
+        const auto t_seed = clock_t_::now();
+        parallel_for(nb * static_cast<std::size_t>(R), [&](std::size_t i) {
+            const std::size_t b = i / R;
+            const int s = static_cast<int>(i % R);
+            const int sample = s0 + (s < Rc ? s : 0);
+            std::mt19937 eng = sample_engine(o.block_seeds[b], static_cast<std::uint64_t>(sample));
+            const std::uint64_t n = L.dim(static_cast<int>(b)), off = L.row_offset(static_cast<int>(b));
+            const std::vector<cplx> v = gaussian_vector(n, eng);
+            for (std::uint64_t r = 0; r < n; ++r) host[(off + r) * R + s] = v[r];
+        });
+        L.set_start(host.data());
+        t.seed_s += secs(t_seed, clock_t_::now());
+        L.run(o.krylov, o.breakdown_tol, o.use_graph, alpha, beta, steps, t);
+        const auto t_post = clock_t_::now();
+        const std::size_t pairs = nb * static_cast<std::size_t>(R);
+        std::vector<SampleMoments> round(nb * static_cast<std::size_t>(Rc));
+        parallel_for(nb * static_cast<std::size_t>(Rc), [&](std::size_t i) {
+            const std::size_t b = i / Rc, s = i % Rc;
+            const std::size_t pr = b * R + s;
+            const int m = steps[pr];
+            std::vector<double> a(static_cast<std::size_t>(m)), bn(static_cast<std::size_t>(m));
+            for (int j = 0; j < m; ++j) {
+                a[j] = alpha[static_cast<std::size_t>(j) * pairs + pr];
+                bn[j] = beta[static_cast<std::size_t>(j) * pairs + pr];
+            }
+            std::vector<double> values, weights;
+            tridiag_weights(a.data(), bn.data(), static_cast<std::size_t>(m), values, weights);
+            round[i] = sample_moments(values, weights, o.betas);
+        });
+        for (std::size_t b = 0; b < nb; ++b)
+            for (int s = 0; s < Rc; ++s) mom[b].push_back(std::move(round[b * Rc + s]));
+        t.post_s += secs(t_post, clock_t_::now());
+    }
+    std::vector<Curves> out(nb);
+    for (std::size_t b = 0; b < nb; ++b) out[b] = combine_samples(mom[b], L.dim(static_cast<int>(b)), o.betas);
+    t.total_s += secs(t_all, clock_t_::now());
+    return out;
+}
+
+}  // namespace qsa

@@ -1,0 +1,74 @@
+// Batched Lanczos over many blocks and R vectors per block (device/batch.hip), and the sampled
+// thermodynamics built on it: FTLM of every block of a symmetry walk with all blocks and all
+// samples advancing through one packed multi-vector matvec per step (the reference runs them block
+// by block, samples in lockstep groups of at most 8: include/ed/thermal/ftlm.h:277-480,
+// src/engine/thermal.cpp:161-344).
+#pragma once
+
+#include <functional>
+#include <memory>
+#include <vector>
+
+#include "sector_operator.h"
+#include "spectral.h"
+
+namespace qsa {
+
+struct BatchedTimings {
+    double pack_s = 0.0, seed_s = 0.0, steps_s = 0.0, post_s = 0.0, total_s = 0.0;
+    std::uint64_t applies = 0;      ///< block-vector applies (sum over steps of the live pairs)
+    std::uint64_t rows = 0, nnz = 0;
+    int steps = 0;
+    bool graph = false;
+};
+
+/// The packed engine: blocks (square sector operators on the CSR lane) x R complex vectors.
+class BatchedLanczos {
+public:
+    BatchedLanczos(std::vector<std::shared_ptr<const SectorOperator>> blocks, int R);
+    int n_blocks() const { return static_cast<int>(ops_.size()); }
+    int R() const { return R_; }
+    std::uint64_t rows() const { return rows_; }
+    std::uint64_t row_offset(int b) const { return row_off_[b]; }
+    std::uint64_t dim(int b) const { return row_off_[b + 1] - row_off_[b]; }
+
+    /// Start vectors from the host, block b sample s at host[(row_offset(b) + i) R + s] (the batch
+    /// layout); normalised on the device.
+    void set_start(const cplx* host_interleaved);
+    /// M Lanczos steps (LocalDGKS3 with K = 1, no basis) with breakdown_tol[b] per block; the
+    /// step is captured as a hipGraph when use_graph. Returns alpha[M][B][R], beta[M][B][R]
+    /// (beta of step j is beta_{j+1}) and steps[B][R].
+    void run(int M, const std::vector<double>& breakdown_tol, bool use_graph, std::vector<double>& alpha,
+             std::vector<double>& beta, std::vector<int>& steps, BatchedTimings& t);
+    /// Y = H X on batch vectors (device pointers, rows() x R complex each).
+    void apply(const cplx* X, cplx* Y) const;
+
+private:
+    dev::BatchView view() const;
+    std::vector<std::shared_ptr<const SectorOperator>> ops_;
+    int R_;
+    std::uint64_t rows_ = 0, nnz_ = 0;
+    std::vector<std::uint64_t> row_off_;
+    std::vector<dev::Tile> tiles_;
+    std::vector<std::uint32_t> tile_begin_;
+    DeviceBuffer row_ptr_, col_, val_, tiles_d_, tile_begin_d_, cap_d_, active_d_, steps_d_, step_d_;
+    DeviceBuffer V0_, V1_, W_, partial_, alpha_h_, beta_h_, ov_, cur_a_, cur_b_, inv_b_, tol_d_;
+    friend struct BatchedAccess;
+};
+
+struct BatchedFtlmOptions {
+    int samples = 40, krylov = 100;
+    std::vector<double> betas;
+    std::vector<std::uint64_t> block_seeds;   ///< the base seed of every block
+    std::vector<double> breakdown_tol;        ///< per block (the reference: 64 eps s_H)
+    bool use_graph = true;
+};
+
+/// fn(i) for i < n on up to 16 host threads (dynamic schedule); the first exception is rethrown.
+void parallel_blocks(std::size_t n, const std::function<void(std::size_t)>& fn);
+
+/// FTLM curves per block (lnZ, E, V per beta, the block's lowest Ritz value in e_min).
+std::vector<Curves> batched_ftlm(const std::vector<std::shared_ptr<const SectorOperator>>& blocks,
+                                 const BatchedFtlmOptions& o, BatchedTimings& t);
+
+}  // namespace qsa

@@ -1,0 +1,196 @@
+// qedh_batch_lanczos, qedh_ftlm_batched: the batched Krylov engine behind the C-ABI.
+#include "../../../include/qedh.h"
+
+#include <chrono>
+#include <limits>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "batched.h"
+#include "tridiag.h"
+
+struct qedh_matvec_s {
+    std::shared_ptr<qsa::SectorOperator> op;
+};
+
+namespace qsa {
+int capi_status(const std::exception& e);
+void capi_set_error(const std::string& m);
+}  // namespace qsa
+
+namespace {
+
+template <class F> int guarded_batch(F&& f) {
+    try {
+        f();
+        return 0;
+    } catch (const qsa::Error& e) {
+        qsa::capi_set_error(e.what());
+        return e.code;
+    } catch (const std::bad_alloc& e) {
+        qsa::capi_set_error(std::string("host allocation failed: ") + e.what());
+        return qsa::kResourceLimit;
+    } catch (const std::exception& e) {
+        return qsa::capi_status(e);
+    }
+}
+
+std::vector<std::shared_ptr<const qsa::SectorOperator>> gather_blocks(const qedh_matvec* blocks, int n) {
+    if (!blocks || n < 1) throw qsa::Error(qsa::kInvalid, "batched: no blocks");
+    std::vector<std::shared_ptr<const qsa::SectorOperator>> out;
+    out.reserve(static_cast<std::size_t>(n));
+    for (int b = 0; b < n; ++b) {
+        if (!blocks[b]) throw qsa::Error(qsa::kInvalid, "batched: block " + std::to_string(b) + " is null");
+        out.push_back(blocks[b]->op);
+    }
+    return out;
+}
+
+}  // namespace
+
+extern "C" int qedh_batch_lanczos(const qedh_matvec* blocks, int n_blocks, int R, int krylov, const double* start,
+                                  const double* breakdown_tol, int use_graph, double* alpha, double* beta, int* steps,
+                                  double* timings) {
+    return guarded_batch([&] {
+        if (!start || !breakdown_tol || !alpha || !beta || !steps)
+            throw qsa::Error(qsa::kInvalid, "qedh_batch_lanczos: null argument");
+        const auto ops = gather_blocks(blocks, n_blocks);
+        qsa::BatchedTimings t;
+        const auto t0 = std::chrono::steady_clock::now();
+        qsa::BatchedLanczos L(ops, R);
+        t.pack_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        L.set_start(reinterpret_cast<const qsa::cplx*>(start));
+        std::vector<double> a, b;
+        std::vector<int> st;
+        L.run(krylov, std::vector<double>(breakdown_tol, breakdown_tol + n_blocks), use_graph != 0, a, b, st, t);
+        std::copy(a.begin(), a.end(), alpha);
+        std::copy(b.begin(), b.end(), beta);
+        std::copy(st.begin(), st.end(), steps);
+        if (timings) {
+            timings[0] = t.pack_s;
+            timings[1] = t.steps_s;
+            timings[2] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        }
+    });
+}
+
+extern "C" int qedh_ftlm_batched(const qedh_matvec* blocks, int n_blocks, int samples, int krylov,
+                                 const uint64_t* block_seeds, const double* breakdown_tol, int n_beta,
+                                 const double* betas, int use_graph, double* lnZ, double* E, double* V, double* e_min,
+                                 double* stats) {
+    return guarded_batch([&] {
+        if (!block_seeds || !betas || !lnZ || !E || !V || n_beta < 1)
+            throw qsa::Error(qsa::kInvalid, "qedh_ftlm_batched: null argument or no temperature");
+        const auto ops = gather_blocks(blocks, n_blocks);
+        qsa::BatchedFtlmOptions o;
+        o.samples = samples;
+        o.krylov = krylov;
+        o.betas.assign(betas, betas + n_beta);
+        o.block_seeds.assign(block_seeds, block_seeds + n_blocks);
+        o.use_graph = use_graph != 0;
+        for (int b = 0; b < n_blocks; ++b) {
+            double tol = 0.0;
+            if (breakdown_tol) {
+                tol = breakdown_tol[b];
+            } else {
+                const double s = ops[b]->norm_bound();
+                tol = 64.0 * std::numeric_limits<double>::epsilon() * (s > 0.0 ? s : 1.0);
+            }
+            o.breakdown_tol.push_back(tol);
+        }
+        qsa::BatchedTimings t;
+        const auto c = qsa::batched_ftlm(ops, o, t);
+        for (int b = 0; b < n_blocks; ++b) {
+            for (int i = 0; i < n_beta; ++i) {
+                lnZ[b * n_beta + i] = c[b].lnZ[i];
+                E[b * n_beta + i] = c[b].E[i];
+                V[b * n_beta + i] = c[b].V[i];
+            }
+            if (e_min) e_min[b] = c[b].e_min;
+        }
+        if (stats) {
+            stats[0] = t.pack_s;
+            stats[1] = t.seed_s;
+            stats[2] = t.steps_s;
+            stats[3] = t.post_s;
+            stats[4] = t.total_s;
+            stats[5] = static_cast<double>(t.applies);
+            stats[6] = static_cast<double>(t.rows);
+            stats[7] = static_cast<double>(t.nnz);
+        }
+    });
+}
+
+/* Exact thermodynamics of small blocks: each block's CSR is expanded to a dense Hermitian matrix on
+ * the host and its whole spectrum solved (the reference's host dense lane for blocks at most
+ * dense_max_dim, src/engine/thermal.cpp:197-200), blocks in parallel on the host threads. */
+extern "C" int qedh_exact_thermal(const qedh_matvec* blocks, int n_blocks, int n_beta, const double* betas,
+                                  double* lnZ, double* E, double* V, double* e_min) {
+    return guarded_batch([&] {
+        if (!betas || !lnZ || !E || !V || n_beta < 1)
+            throw qsa::Error(qsa::kInvalid, "qedh_exact_thermal: null argument or no temperature");
+        const auto ops = gather_blocks(blocks, n_blocks);
+        for (const auto& op : ops) {
+            if (!op->same()) throw qsa::Error(qsa::kInvalid, "qedh_exact_thermal: a block must be square");
+            if (op->lane() != qsa::kLaneCsr)
+                throw qsa::Error(qsa::kDeviceUnsupported, "qedh_exact_thermal: a block needs its CSR (csr lane)");
+            if (op->dim() > 8192) throw qsa::Error(qsa::kInvalid, "qedh_exact_thermal: block above 8192 states");
+        }
+        // Download on this thread (device calls stay on one thread), solve on the pool.
+        std::vector<std::vector<std::complex<double>>> dense(ops.size());
+        for (std::size_t b = 0; b < ops.size(); ++b) {
+            std::vector<std::uint64_t> rp;
+            std::vector<std::uint32_t> col;
+            std::vector<double> val;
+            ops[b]->csr_download(rp, col, val);
+            const std::size_t n = static_cast<std::size_t>(ops[b]->dim());
+            dense[b].assign(n * n, std::complex<double>(0.0, 0.0));
+            for (std::size_t r = 0; r < n; ++r)
+                for (std::uint64_t e = rp[r]; e < rp[r + 1]; ++e)
+                    dense[b][col[e] * n + r] += std::complex<double>(val[2 * e], val[2 * e + 1]);
+        }
+        const std::vector<double> bt(betas, betas + n_beta);
+        std::vector<qsa::Curves> out(ops.size());
+        qsa::parallel_blocks(ops.size(), [&](std::size_t b) {
+            const std::size_t n = static_cast<std::size_t>(ops[b]->dim());
+            const auto ev = qsa::hermitian_eigvals_householder(std::move(dense[b]), n);
+            out[b] = qsa::exact_curves(ev, bt);
+        });
+        for (int b = 0; b < n_blocks; ++b) {
+            for (int i = 0; i < n_beta; ++i) {
+                lnZ[b * n_beta + i] = out[b].lnZ[i];
+                E[b * n_beta + i] = out[b].E[i];
+                V[b * n_beta + i] = out[b].V[i];
+            }
+            if (e_min) e_min[b] = out[b].e_min;
+        }
+    });
+}
+
+/* The system's curves from its blocks' (log-sum-exp over weighted blocks, law of total variance):
+ * src/engine/thermal.cpp:597-640. Inputs [n_blocks][n_beta]; outputs [n_beta]. */
+extern "C" int qedh_thermal_combine(int n_blocks, int n_beta, const double* betas, const double* weights,
+                                    const double* lnZ, const double* E, const double* V, double* out_lnZ,
+                                    double* out_E, double* out_C, double* out_S, double* out_F) {
+    return guarded_batch([&] {
+        if (n_blocks < 1 || n_beta < 1 || !betas || !weights || !lnZ || !E || !V || !out_lnZ || !out_E || !out_C ||
+            !out_S || !out_F)
+            throw qsa::Error(qsa::kInvalid, "qedh_thermal_combine: bad arguments");
+        std::vector<qsa::Curves> blocks(static_cast<std::size_t>(n_blocks));
+        for (int b = 0; b < n_blocks; ++b) {
+            blocks[b].lnZ.assign(lnZ + b * n_beta, lnZ + (b + 1) * n_beta);
+            blocks[b].E.assign(E + b * n_beta, E + (b + 1) * n_beta);
+            blocks[b].V.assign(V + b * n_beta, V + (b + 1) * n_beta);
+        }
+        const auto s = qsa::combine_blocks(blocks, std::vector<double>(weights, weights + n_blocks),
+                                           std::vector<double>(betas, betas + n_beta));
+        for (int i = 0; i < n_beta; ++i) {
+            out_lnZ[i] = s.lnZ[i];
+            out_E[i] = s.E[i];
+            out_C[i] = s.C[i];
+            out_S[i] = s.S[i];
+            out_F[i] = s.F[i];
+        }
+    });
+}

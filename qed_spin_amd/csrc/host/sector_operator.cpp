@@ -115,14 +115,64 @@ void SectorOperator::build_csr(std::uint64_t budget) {
     csr_.width = width == 0 ? (real_ ? 8 : 16) : width;
     csr_.bytes = row_ptr_.bytes() + col_.bytes() + vals_.bytes() + dict_re_.bytes() + dict_im_.bytes();
     const double mean = rows ? static_cast<double>(nnz) / static_cast<double>(rows) : 0.0;
+    // About four entries per lane (measured on the N = 32 ring). Block applies use the same width:
+    // the lanes set the summation order of a row, so a block apply then equals single applies bit
+    // for bit.
     int L = 4;
-    while (L < 64 && L < mean / 2.0) L *= 2;
+    while (L < 64 && L < mean / 4.0) L *= 2;
     csr_.lanes = L;
+    lanes_block_ = L;
     lane_ = kLaneCsr;
+    if (rows >= (std::int64_t{1} << 18)) autotune();
     csr_.build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
-dev::CsrView SectorOperator::csr_view() const {
+void SectorOperator::autotune() {
+    if (lane_ != kLaneCsr || dim() == 0) return;
+    const hipStream_t st = engine_stream();
+    const bool cv = !real_;
+    const std::size_t elem = cv ? 16 : 8;
+    const std::size_t rows = static_cast<std::size_t>(dim()), cols = static_cast<std::size_t>(col_dim());
+    DeviceBuffer x(cols * elem), y(rows * elem);
+    QSA_HIP(hipMemsetAsync(x.get(), 0, x.bytes(), st));
+    hipEvent_t a, b;
+    QSA_HIP(hipEventCreate(&a));
+    QSA_HIP(hipEventCreate(&b));
+    auto time_it = [&](auto&& launch) {
+        launch();
+        QSA_HIP(hipEventRecord(a, st));
+        for (int i = 0; i < 3; ++i) launch();
+        QSA_HIP(hipEventRecord(b, st));
+        QSA_HIP(hipEventSynchronize(b));
+        float ms = 0.0f;
+        QSA_HIP(hipEventElapsedTime(&ms, a, b));
+        return ms;
+    };
+    float best1 = 1e30f;
+    int l1 = csr_.lanes;
+    for (int L : {4, 8, 16, 32}) {
+        csr_.lanes = L;
+        lanes_block_ = L;
+        const float t1 = time_it([&] {
+            dev::VecSet v;
+            v.k = 1;
+            v.in[0] = x.get();
+            v.out[0] = y.get();
+            dev::csr_spmv(csr_view(), real_, v, st);
+        });
+        if (t1 < best1) {
+            best1 = t1;
+            l1 = L;
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    QSA_HIP(hipGetLastError());
+    csr_.lanes = l1;
+    lanes_block_ = l1;
+}
+
+dev::CsrView SectorOperator::csr_view(bool block) const {
     dev::CsrView A;
     A.rows = dim();
     A.row_ptr = row_ptr_.get<std::uint64_t>();
@@ -131,7 +181,7 @@ dev::CsrView SectorOperator::csr_view() const {
     A.width = (csr_.width == 1 || csr_.width == 2) ? csr_.width : 0;
     A.dict_re = dict_re_.get<double>();
     A.dict_im = dict_im_.get<double>();
-    A.lanes = csr_.lanes;
+    A.lanes = block ? lanes_block_ : csr_.lanes;
     A.values_real = real_;
     A.nt = nt_ ? 1 : 0;
     return A;
@@ -162,7 +212,7 @@ void SectorOperator::apply_multi(const void* const* ins, void* const* outs, int 
 void SectorOperator::apply_block(int nv, const void* X, void* Y, bool complex_vectors) const {
     if (lane_ != kLaneCsr) throw Error(kDeviceUnsupported, "apply_block needs the CSR lane");
     if (nv != 1 && nv != 2 && nv != 4 && nv != 8) throw Error(kInvalid, "apply_block: nv must be 1, 2, 4 or 8");
-    dev::csr_spmv_block(csr_view(), real_ && !complex_vectors, nv, X, Y, engine_stream());
+    dev::csr_spmv_block(csr_view(true), real_ && !complex_vectors, nv, X, Y, engine_stream());
     QSA_HIP(hipGetLastError());
 }
 

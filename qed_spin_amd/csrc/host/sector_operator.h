@@ -57,14 +57,24 @@ public:
     void csr_download_rows(std::int64_t r0, std::int64_t r1, std::vector<std::uint64_t>& row_ptr,
                            std::vector<std::uint32_t>& col, std::vector<double>& val) const;
     /// Pick the lanes per row of the SpMV (4..64).
-    void set_lanes(int lanes) { csr_.lanes = lanes; }
+    void set_lanes(int lanes) {
+        csr_.lanes = lanes;
+        lanes_block_ = lanes;
+    }
+    /// Time the single-vector SpMV at every lanes-per-row width on this matrix and keep the fastest
+    /// (block applies share it); a few milliseconds on a large sector.
+    void autotune();
+    int lanes_block() const { return lanes_block_; }
+    /// The CSR as the kernels read it (CSR lane only).
+    dev::CsrView device_csr() const { return csr_view(false); }
     /// Stream the CSR's column / id arrays with non-temporal loads (they are read once per apply).
     void set_nontemporal(bool on) { nt_ = on; }
     bool nontemporal() const { return nt_; }
 
 private:
     void build_csr(std::uint64_t budget);
-    dev::CsrView csr_view() const;
+    dev::CsrView csr_view(bool block = false) const;
+    int lanes_block_ = 16;
 
     std::shared_ptr<const DeviceSector> rows_, cols_;
     std::shared_ptr<const DeviceProgram> prog_;

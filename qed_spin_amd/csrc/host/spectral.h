@@ -5,6 +5,7 @@
 // the reference's vectors.
 #pragma once
 
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <limits>
@@ -133,6 +134,73 @@ inline Curves combine_samples(const std::vector<SampleMoments>& S, std::uint64_t
     return c;
 }
 
+/// The exact thermodynamics of a block from its whole spectrum (include/ed/thermal/curves.h:38-65):
+/// Boltzmann weights about the lowest level, E from the first moment, V by a second pass about E.
+inline Curves exact_curves(const std::vector<double>& ev, const std::vector<double>& betas) {
+    Curves c;
+    const double e0 = *std::min_element(ev.begin(), ev.end());
+    c.e_min = e0;
+    for (double bt : betas) {
+        double z = 0.0, e = 0.0;
+        for (double x : ev) {
+            const double w = std::exp(-bt * (x - e0));
+            z += w;
+            e += w * (x - e0);
+        }
+        c.lnZ.push_back(std::log(z) - bt * e0);
+        c.E.push_back(e0 + e / z);
+        double v = 0.0;
+        for (double x : ev) {
+            const double dx = x - e0 - e / z;
+            v += std::exp(-bt * (x - e0)) * dx * dx;
+        }
+        c.V.push_back(v / z);
+    }
+    return c;
+}
+
+/// The whole system from its blocks (src/engine/thermal.cpp:597-640): block weights w_j
+/// (multiplicities), p_j = w_j Z_j / max, lnZ by log-sum-exp, E the weighted mean, C = beta^2 times
+/// the law-of-total-variance V, S = lnZ + beta E, F = -lnZ / beta.
+struct SystemCurves {
+    std::vector<double> lnZ, E, C, S, F;
+};
+inline SystemCurves combine_blocks(const std::vector<Curves>& blocks, const std::vector<double>& weights,
+                                   const std::vector<double>& betas) {
+    SystemCurves out;
+    const std::size_t nT = betas.size(), nb = blocks.size();
+    out.lnZ.resize(nT);
+    out.E.resize(nT);
+    out.C.resize(nT);
+    out.S.resize(nT);
+    out.F.resize(nT);
+    std::vector<double> p(nb);
+    for (std::size_t i = 0; i < nT; ++i) {
+        double mx = -std::numeric_limits<double>::infinity();
+        for (std::size_t j = 0; j < nb; ++j) mx = std::max(mx, std::log(weights[j]) + blocks[j].lnZ[i]);
+        double z = 0.0, e = 0.0;
+        for (std::size_t j = 0; j < nb; ++j) {
+            p[j] = std::exp(std::log(weights[j]) + blocks[j].lnZ[i] - mx);
+            z += p[j];
+            e += p[j] * blocks[j].E[i];
+        }
+        e /= z;
+        double var = 0.0;
+        for (std::size_t j = 0; j < nb; ++j) {
+            const double d = blocks[j].E[i] - e;
+            var += p[j] * (blocks[j].V[i] + d * d);
+        }
+        var /= z;
+        const double bt = betas[i];
+        out.lnZ[i] = mx + std::log(z);
+        out.E[i] = e;
+        out.C[i] = bt * bt * var;
+        out.S[i] = out.lnZ[i] + bt * e;
+        out.F[i] = -out.lnZ[i] / bt;
+    }
+    return out;
+}
+
 struct FtlmOptions {
     std::size_t samples = 40, krylov = 100;
     double breakdown_tol = 0.0;
@@ -165,8 +233,9 @@ Curves ftlm_block(const HipBackend<cplx>& be, Matvec&& H, std::size_t n, const F
             timings->total_s += r.timings.total_s;
         }
         if (r.alpha.empty()) continue;
-        const TridiagEig t = tridiag_eig(r.alpha, r.beta, r.alpha.size(), true);
-        mom.push_back(sample_moments(t.values, t.weights(), o.betas));
+        std::vector<double> values, weights;
+        tridiag_weights(r.alpha.data(), r.beta.data() + 1, r.alpha.size(), values, weights);
+        mom.push_back(sample_moments(values, weights, o.betas));
     }
     if (mom.empty()) throw Error(kConvergence, "ftlm: every sample failed");
     return combine_samples(mom, n, o.betas);

@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cmath>
 #include <limits>
+#include <complex>
 #include <numeric>
 #include <string>
 
@@ -19,8 +20,10 @@ std::vector<double> TridiagEig::weights() const {
 namespace {
 
 // Implicit QL with Wilkinson shifts on the tridiagonal (d, e), e[i] coupling i and i+1, rotations
-// accumulated into the column-major n x n matrix z (when non-null).
-void ql_implicit(std::vector<double>& d, std::vector<double>& e, std::size_t n, double* z) {
+// accumulated into the first `rows` rows of the column-major matrix z (leading dimension rows; when
+// non-null). Each row is rotated independently, so rows = 1 gives the first row of the full
+// eigenvector matrix bit for bit.
+void ql_implicit(std::vector<double>& d, std::vector<double>& e, std::size_t n, double* z, std::size_t rows) {
     if (n == 0) return;
     e.resize(n, 0.0);
     e[n - 1] = 0.0;
@@ -59,9 +62,9 @@ void ql_implicit(std::vector<double>& d, std::vector<double>& e, std::size_t n, 
                 d[ii + 1] = g + p;
                 g = c * r - b;
                 if (z) {
-                    double* zi = z + ii * n;
-                    double* zj = z + (ii + 1) * n;
-                    for (std::size_t k = 0; k < n; ++k) {
+                    double* zi = z + ii * rows;
+                    double* zj = z + (ii + 1) * rows;
+                    for (std::size_t k = 0; k < rows; ++k) {
                         const double fk = zj[k];
                         zj[k] = s * zi[k] + c * fk;
                         zi[k] = c * zi[k] - s * fk;
@@ -109,8 +112,33 @@ TridiagEig tridiag_eig(const std::vector<double>& alpha, const std::vector<doubl
         z.assign(m * m, 0.0);
         for (std::size_t i = 0; i < m; ++i) z[i * m + i] = 1.0;
     }
-    ql_implicit(d, e, m, vectors ? z.data() : nullptr);
+    ql_implicit(d, e, m, vectors ? z.data() : nullptr, m);
     return sorted(d, vectors ? &z : nullptr, m);
+}
+
+void tridiag_weights(const double* alpha, const double* beta_next, std::size_t m, std::vector<double>& values,
+                     std::vector<double>& weights) {
+    values.clear();
+    weights.clear();
+    if (m == 0) return;
+    std::vector<double> d(alpha, alpha + m);
+    std::vector<double> e(m, 0.0);
+    for (std::size_t i = 0; i + 1 < m; ++i) e[i] = beta_next[i];
+    for (std::size_t i = 0; i < m; ++i)
+        if (!std::isfinite(d[i]) || !std::isfinite(e[i]))
+            throw Error(kConvergence, "tridiagonal eigensolve: non-finite entry at step " + std::to_string(i));
+    std::vector<double> z0(m, 0.0);
+    z0[0] = 1.0;
+    ql_implicit(d, e, m, z0.data(), 1);
+    std::vector<std::size_t> order(m);
+    std::iota(order.begin(), order.end(), std::size_t{0});
+    std::stable_sort(order.begin(), order.end(), [&d](std::size_t a, std::size_t b) { return d[a] < d[b]; });
+    values.resize(m);
+    weights.resize(m);
+    for (std::size_t j = 0; j < m; ++j) {
+        values[j] = d[order[j]];
+        weights[j] = z0[order[j]] * z0[order[j]];
+    }
 }
 
 TridiagEig symmetric_eig(std::vector<double> a, std::size_t n) {
@@ -156,8 +184,61 @@ TridiagEig symmetric_eig(std::vector<double> a, std::size_t n) {
     std::vector<double> d(n), e(n, 0.0);
     for (std::size_t i = 0; i < n; ++i) d[i] = A(i, i);
     for (std::size_t i = 0; i + 1 < n; ++i) e[i] = A(i + 1, i);
-    ql_implicit(d, e, n, q.data());
+    ql_implicit(d, e, n, q.data(), n);
     return sorted(d, &q, n);
+}
+
+std::vector<double> hermitian_eigvals_householder(std::vector<std::complex<double>> a, std::size_t n) {
+    using cd = std::complex<double>;
+    if (n == 0) return {};
+    for (const cd& x : a)
+        if (!std::isfinite(x.real()) || !std::isfinite(x.imag()))
+            throw Error(kConvergence, "dense eigensolve: non-finite entry");
+    // Column-major A; column k below the diagonal is mapped onto a multiple of e_1 by the Hermitian
+    // reflector H = I - t u u^H, t = 2 / u^H u, u = x + e^{i arg x_0} ||x|| e_1; the trailing block
+    // becomes H A22 H = A22 - u w^H - w u^H with p = t A22 u, K = t u^H p / 2, w = p - K u.
+    auto A = [&](std::size_t r, std::size_t c) -> cd& { return a[c * n + r]; };
+    std::vector<double> d(n), e(n, 0.0);
+    std::vector<cd> u(n), p(n);
+    for (std::size_t k = 0; k + 1 < n; ++k) {
+        const std::size_t r0 = k + 1;
+        double xn2 = 0.0;
+        for (std::size_t i = r0; i < n; ++i) xn2 += std::norm(A(i, k));
+        const double xn = std::sqrt(xn2);
+        e[k] = xn;   // |the subdiagonal entry| after the reflection
+        if (k + 2 >= n || xn == 0.0) continue;
+        const cd x0 = A(r0, k);
+        const double ax0 = std::abs(x0);
+        const cd ph = ax0 > 0.0 ? x0 / ax0 : cd(1.0, 0.0);
+        for (std::size_t i = r0; i < n; ++i) u[i] = A(i, k);
+        u[r0] += ph * xn;
+        double un2 = 0.0;
+        for (std::size_t i = r0; i < n; ++i) un2 += std::norm(u[i]);
+        if (un2 == 0.0) continue;
+        const double t = 2.0 / un2;
+        for (std::size_t i = r0; i < n; ++i) p[i] = 0.0;
+        for (std::size_t c = r0; c < n; ++c) {
+            const cd uc = u[c];
+            const cd* col = &a[c * n];
+            for (std::size_t r = r0; r < n; ++r) p[r] += col[r] * uc;
+        }
+        cd up(0.0, 0.0);
+        for (std::size_t i = r0; i < n; ++i) {
+            p[i] *= t;
+            up += std::conj(u[i]) * p[i];
+        }
+        const double K = 0.5 * t * up.real();
+        for (std::size_t i = r0; i < n; ++i) p[i] -= K * u[i];
+        for (std::size_t c = r0; c < n; ++c) {
+            const cd wc = std::conj(p[c]), uc = std::conj(u[c]);
+            cd* col = &a[c * n];
+            for (std::size_t r = r0; r < n; ++r) col[r] -= u[r] * wc + p[r] * uc;
+        }
+    }
+    for (std::size_t k = 0; k < n; ++k) d[k] = A(k, k).real();
+    std::vector<double> beta(n + 1, 0.0);
+    for (std::size_t k = 0; k + 1 < n; ++k) beta[k + 1] = e[k];
+    return tridiag_eig(d, beta, n, false).values;
 }
 
 std::vector<double> hermitian_eigvals(const std::vector<double>& a, std::size_t n) {

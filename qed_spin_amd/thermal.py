@@ -1,0 +1,207 @@
+"""Thermodynamics of a spin Hamiltonian over its symmetry blocks (the reference's qed.thermal /
+ed::sectors::thermal, src/engine/thermal.cpp:345-650).
+
+Every non-empty block of the (Sz, k) walk gets its own seed from the reference's per-block stream
+(seed <- seed * 6364136223846793005 + 1442695040888963407, in walk order, thermal.cpp:471). Blocks of
+at most ``dense_max_dim`` states are solved exactly (their whole spectrum, thermal.cpp:197-200);
+all other blocks run FTLM together in the batched engine: every block and every random vector of
+the walk advance through one packed multi-vector matvec per Lanczos step, the step replayed as a
+hipGraph (qedh_ftlm_batched). The blocks' curves are combined by log-sum-exp with the law of total
+variance (thermal.cpp:597-640).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import time
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from .errors import DeviceUnavailable, DeviceUnsupported, InvalidRequest
+from .operator import Operator
+from .sector import DEFAULT_CSR_BUDGET, LANE_CSR, Sector, SectorMatvec
+from .symmetry import Symmetry
+
+LCG_MUL = 6364136223846793005
+LCG_ADD = 1442695040888963407
+_U64 = (1 << 64) - 1
+KBREAKDOWN_REL = 64.0 * np.finfo(np.float64).eps
+
+
+def next_block_seed(seed: int) -> int:
+    """The next block's seed of the thermal walk (thermal.cpp:471)."""
+    return (int(seed) * LCG_MUL + LCG_ADD) & _U64
+
+
+def resolve_device(device: str, verb: str) -> None:
+    """The reference's device= keyword (python/qed/_verbs/_device.py:18-29): "gpu" and "auto" run
+    on the HIP device and raise DeviceUnavailable without one; "cpu" names the reference's host path,
+    which this backend does not provide."""
+    if device not in ("cpu", "gpu", "auto"):
+        raise InvalidRequest(f"{verb}: device must be 'cpu', 'gpu' or 'auto', got {device!r}")
+    if device == "cpu":
+        raise DeviceUnsupported(f"{verb}: this backend runs on the HIP device; device='cpu' is the host path")
+    if _lib.device_count() < 1:
+        raise DeviceUnavailable(f"{verb}: no HIP device is visible")
+
+
+def _handles(mats: Sequence[SectorMatvec]):
+    return (C.c_void_p * len(mats))(*[m.h.value for m in mats])
+
+
+def batch_lanczos(mats: Sequence[SectorMatvec], R: int, krylov: int, start: np.ndarray,
+                  breakdown_tol: Optional[Sequence[float]] = None, use_graph: bool = True):
+    """LocalDGKS3 Lanczos of R vectors per block, all blocks at once. start: complex (rows, R) with
+    the blocks' rows stacked in order. Returns alpha, beta (krylov, blocks, R), steps (blocks, R)."""
+    nb = len(mats)
+    rows = sum(m.dim for m in mats)
+    st = np.ascontiguousarray(np.asarray(start, np.complex128).reshape(rows, R))
+    tol = np.ascontiguousarray(
+        [KBREAKDOWN_REL * (m.op.norm_bound() or 1.0) for m in mats] if breakdown_tol is None else breakdown_tol,
+        np.float64)
+    alpha = np.zeros((krylov, nb, R))
+    beta = np.zeros((krylov, nb, R))
+    steps = np.zeros((nb, R), np.int32)
+    t = np.zeros(3)
+    _lib.call("qedh_batch_lanczos", _handles(mats), nb, int(R), int(krylov),
+              st.view(np.float64).ctypes.data_as(C.POINTER(C.c_double)), _lib.ptr(tol, C.c_double),
+              int(bool(use_graph)), _lib.ptr(alpha, C.c_double), _lib.ptr(beta, C.c_double),
+              _lib.ptr(steps, C.c_int), _lib.ptr(t, C.c_double))
+    return alpha, beta, steps, {"pack_s": t[0], "steps_s": t[1], "total_s": t[2]}
+
+
+def ftlm_batched(mats: Sequence[SectorMatvec], samples: int, krylov: int, seeds: Sequence[int], betas,
+                 breakdown_tol: Optional[Sequence[float]] = None, use_graph: bool = True):
+    """FTLM of every block at once; block b draws its samples from seeds[b]. Returns lnZ, E, V
+    (blocks, temperatures), e_min (blocks) and the engine's timings."""
+    nb = len(mats)
+    betas = np.ascontiguousarray(betas, np.float64)
+    nT = betas.size
+    sd = np.ascontiguousarray([int(s) & _U64 for s in seeds], np.uint64)
+    tol = None if breakdown_tol is None else np.ascontiguousarray(breakdown_tol, np.float64)
+    lnZ, E, V = np.zeros((nb, nT)), np.zeros((nb, nT)), np.zeros((nb, nT))
+    emin = np.zeros(nb)
+    stats = np.zeros(8)
+    _lib.call("qedh_ftlm_batched", _handles(mats), nb, int(samples), int(krylov), _lib.ptr(sd, C.c_uint64),
+              _lib.ptr(tol, C.c_double), int(nT), _lib.ptr(betas, C.c_double), int(bool(use_graph)),
+              _lib.ptr(lnZ, C.c_double), _lib.ptr(E, C.c_double), _lib.ptr(V, C.c_double),
+              _lib.ptr(emin, C.c_double), _lib.ptr(stats, C.c_double))
+    keys = ["pack_s", "seed_s", "steps_s", "post_s", "total_s", "applies", "rows", "nnz"]
+    return lnZ, E, V, emin, dict(zip(keys, stats.tolist()))
+
+
+def exact_thermal(mats: Sequence[SectorMatvec], betas):
+    """Exact curves of small blocks (whole spectra of the dense blocks)."""
+    nb = len(mats)
+    betas = np.ascontiguousarray(betas, np.float64)
+    nT = betas.size
+    lnZ, E, V = np.zeros((nb, nT)), np.zeros((nb, nT)), np.zeros((nb, nT))
+    emin = np.zeros(nb)
+    _lib.call("qedh_exact_thermal", _handles(mats), nb, int(nT), _lib.ptr(betas, C.c_double),
+              _lib.ptr(lnZ, C.c_double), _lib.ptr(E, C.c_double), _lib.ptr(V, C.c_double), _lib.ptr(emin, C.c_double))
+    return lnZ, E, V, emin
+
+
+def combine(betas, weights, lnZ, E, V) -> Dict[str, np.ndarray]:
+    betas = np.ascontiguousarray(betas, np.float64)
+    w = np.ascontiguousarray(weights, np.float64)
+    lnZ, E, V = (np.ascontiguousarray(a, np.float64) for a in (lnZ, E, V))
+    nb, nT = lnZ.shape
+    out = {k: np.zeros(nT) for k in ("lnZ", "E", "C", "S", "F")}
+    _lib.call("qedh_thermal_combine", int(nb), int(nT), _lib.ptr(betas, C.c_double), _lib.ptr(w, C.c_double),
+              _lib.ptr(lnZ, C.c_double), _lib.ptr(E, C.c_double), _lib.ptr(V, C.c_double),
+              *[_lib.ptr(out[k], C.c_double) for k in ("lnZ", "E", "C", "S", "F")])
+    return out
+
+
+@dataclass
+class ThermalResult:
+    """The reference's ThermalCurves (include/ed/sectors/thermal.h): per temperature ln Z, E, C, S,
+    F; e0 the lowest level any block resolved; per-block records and timings."""
+
+    T: np.ndarray
+    lnZ: np.ndarray
+    E: np.ndarray
+    C: np.ndarray
+    S: np.ndarray
+    F: np.ndarray
+    e0: float
+    total_dim: int
+    blocks: List[dict] = field(default_factory=list)
+    timings: Dict[str, float] = field(default_factory=dict)
+    engine: Dict[str, float] = field(default_factory=dict)
+
+
+def walk_blocks(H: Operator, sym: Symmetry, csr_budget: int = DEFAULT_CSR_BUDGET):
+    """The non-empty blocks of the walk in the reference's order (n_up ascending, then momentum),
+    each with its sector operator on the device CSR lane."""
+    out = []
+    for blk in sym.blocks(H.num_sites):
+        sec = Sector(blk.group, blk.n_up)
+        if sec.dim == 0:
+            continue
+        m = SectorMatvec(H, sec, lane=LANE_CSR, csr_budget=csr_budget)
+        out.append((blk, sec, m))
+    return out
+
+
+def thermal(H: Operator, T, *, sym: Optional[Symmetry] = None, method: str = "ftlm", samples: int = 40,
+            krylov: int = 100, seed: int = 0, device: str = "gpu", dense_max_dim: Optional[int] = None,
+            use_graph: bool = True, csr_budget: int = DEFAULT_CSR_BUDGET) -> ThermalResult:
+    """FTLM thermodynamics of H at temperatures T over the blocks of `sym` (default: every (Sz, k)
+    block of a ring). seed 0 draws one. dense_max_dim (default 512, the reference's ThermalSpec
+    default): blocks up to this dimension are solved exactly; 0 samples every block."""
+    resolve_device(device, "thermal")
+    if method != "ftlm":
+        raise DeviceUnsupported(f"thermal: method {method!r} is not on the device path (ftlm only)")
+    T = np.asarray(T, np.float64).ravel()
+    if T.size == 0 or np.any(~(T > 0)):
+        raise InvalidRequest("thermal: temperatures must be positive")
+    if samples < 1 or krylov < 1:
+        raise InvalidRequest("thermal: samples and krylov must be >= 1")
+    dmax = 512 if dense_max_dim is None else int(dense_max_dim)
+    if dmax < 0:
+        raise InvalidRequest(f"dense_max_dim must be >= 0 or None, got {dense_max_dim}")
+    betas = 1.0 / T
+    sym = sym if sym is not None else Symmetry()
+    s = int(seed) if seed else int.from_bytes(os.urandom(8), "little")
+    t0 = time.perf_counter()
+    blocks = walk_blocks(H, sym, csr_budget)
+    t_build = time.perf_counter() - t0
+    if not blocks:
+        raise InvalidRequest("thermal: the symmetry selection holds no state")
+    recs = []
+    for blk, sec, m in blocks:
+        s = next_block_seed(s)
+        recs.append({"label": blk.label, "dim": sec.dim, "seed": s, "exact": sec.dim <= dmax, "weight": 1.0})
+    nb, nT = len(blocks), betas.size
+    lnZ, E, V, emin = np.zeros((nb, nT)), np.zeros((nb, nT)), np.zeros((nb, nT)), np.zeros(nb)
+    ex = [i for i, r in enumerate(recs) if r["exact"]]
+    sm = [i for i, r in enumerate(recs) if not r["exact"]]
+    t1 = time.perf_counter()
+    if ex:
+        a = exact_thermal([blocks[i][2] for i in ex], betas)
+        for arr, src in zip((lnZ, E, V, emin), a):
+            arr[ex] = src
+    t_exact = time.perf_counter() - t1
+    engine = {}
+    t2 = time.perf_counter()
+    if sm:
+        mats = [blocks[i][2] for i in sm]
+        a = ftlm_batched(mats, samples, krylov, [recs[i]["seed"] for i in sm], betas, use_graph=use_graph)
+        for arr, src in zip((lnZ, E, V, emin), a[:4]):
+            arr[sm] = src
+        engine = a[4]
+    t_ftlm = time.perf_counter() - t2
+    c = combine(betas, [r["weight"] for r in recs], lnZ, E, V)
+    for i, r in enumerate(recs):
+        r.update(lnZ=lnZ[i].copy(), E=E[i].copy(), V=V[i].copy(), e_min=float(emin[i]))
+    return ThermalResult(
+        T=T, lnZ=c["lnZ"], E=c["E"], C=c["C"], S=c["S"], F=c["F"], e0=float(np.min(emin)),
+        total_dim=int(sum(r["dim"] for r in recs)), blocks=recs,
+        timings={"build_s": t_build, "exact_s": t_exact, "ftlm_s": t_ftlm, "total_s": time.perf_counter() - t0},
+        engine=engine)

@@ -1,0 +1,259 @@
+"""GPU parity of the batched Krylov engine and the thermal verb.
+
+- batch_lanczos (all blocks x R vectors in one packed multi-vector matvec per step) against the
+  one-vector LocalDGKS3 Lanczos of each (block, vector) pair: equal steps, alpha and beta of the
+  first 12 steps within 1e-12 (later steps of a recurrence without full reorthogonalisation
+  amplify rounding differences of the reduction order exponentially), the extreme Ritz values of
+  the whole run within 1e-9;
+  graph replay equals eager launches bit for bit, and two graph runs agree bit for bit;
+- ftlm_batched against the per-block FTLM driver at equal seeds (1e-8, the FTLM tolerance), also
+  with more samples than one batch holds;
+- exact_thermal against numpy's dense eigensolve of the exported block;
+- thermal() against the reference's own recorded curves of j1j2_chain12 (FTLM of the 4096-state
+  block at seed 11; exact curves over every (Sz, k) block), and a mixed walk (exact small blocks,
+  sampled large ones) against the oracle block by block.
+
+Every case prints its name before it touches the device."""
+
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import FIXTURES
+from oracle import pyoracle as po
+
+import qed_spin_amd as qs
+from qed_spin_amd import symmetry as sym
+from qed_spin_amd import thermal as th
+
+pytestmark = pytest.mark.gpu
+
+EPS64 = 64 * np.finfo(np.float64).eps
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def ring(N, d=1):
+    return [(i, (i + d) % N) for i in range(N)]
+
+
+def j1j2(N, J2=0.5):
+    b = qs.HamiltonianBuilder(N)
+    b.heisenberg(ring(N), 1.0)
+    b.heisenberg(ring(N, 2), J2)
+    return b.to_operator()
+
+
+def xxz(N, delta=0.5):
+    return qs.HamiltonianBuilder(N).xxz(ring(N), 1.0, delta).to_operator()
+
+
+def chiral_j1j2(N, chi=0.2):
+    b = qs.HamiltonianBuilder(N)
+    b.heisenberg(ring(N), 1.0)
+    b.heisenberg(ring(N, 2), 0.3)
+    b.chirality([(i, (i + 1) % N, (i + 2) % N) for i in range(N)], chi)
+    return b.to_operator()
+
+
+def blocks(H, N, pairs):
+    out = []
+    for n_up, k in pairs:
+        s = qs.Sector(sym.chain_group(N, k), n_up)
+        out.append(qs.SectorMatvec(H, s, lane=qs.LANE_CSR))
+    return out
+
+
+def records():
+    with open(os.path.join(FIXTURES, "reference_records.json")) as f:
+        return json.load(f)
+
+
+def test_batch_lanczos_matches_one_vector_lanczos():
+    say("batch_lanczos vs lanczos LocalDGKS3")
+    N, M, R = 12, 30, 4
+    H = chiral_j1j2(N)
+    mats = blocks(H, N, [(6, 0), (6, 1), (5, 2), (2, 1), (6, 5)])
+    dims = [m.dim for m in mats]
+    assert min(dims) < M < max(dims)
+    rows = sum(dims)
+    rng = np.random.default_rng(3)
+    start = rng.normal(size=(rows, R)) + 1j * rng.normal(size=(rows, R))
+    tol = [EPS64 * H.norm_bound()] * len(mats)
+    alpha, beta, steps, _ = th.batch_lanczos(mats, R, M, start, tol)
+    off = 0
+    for b, m in enumerate(mats):
+        for s in range(R):
+            a1, b1, _ = m.lanczos(start[off:off + m.dim, s], M, reorth=2, breakdown_tol=tol[b], complex_vectors=True)
+            n = len(a1)
+            assert steps[b, s] == n, (b, s, steps[b, s], n)
+            scale = max(1.0, np.abs(a1).max())
+            h = min(n, 12)
+            assert np.abs(alpha[:h, b, s] - a1[:h]).max() <= 1e-12 * scale, (b, s)
+            assert np.abs(beta[:h, b, s] - b1[1:h + 1]).max() <= 1e-12 * scale, (b, s)
+            r0 = po.tridiag_eig(alpha[:n, b, s], np.concatenate([[0.0], beta[:n, b, s]]))[0]
+            r1 = po.tridiag_eig(a1, b1[: n + 1])[0]
+            assert abs(r0[0] - r1[0]) <= 1e-9 * scale and abs(r0[-1] - r1[-1]) <= 1e-9 * scale, (b, s)
+        off += m.dim
+
+
+def test_batch_lanczos_graph_equals_eager_bitwise():
+    say("batch_lanczos graph vs eager")
+    N, M, R = 12, 25, 3
+    H = xxz(N)
+    mats = blocks(H, N, [(6, 0), (5, 3), (4, 7)])
+    rows = sum(m.dim for m in mats)
+    rng = np.random.default_rng(5)
+    start = rng.normal(size=(rows, R)) + 1j * rng.normal(size=(rows, R))
+    tol = [EPS64 * H.norm_bound()] * len(mats)
+    ag, bg, sg, _ = th.batch_lanczos(mats, R, M, start, tol, use_graph=True)
+    ae, be_, se, _ = th.batch_lanczos(mats, R, M, start, tol, use_graph=False)
+    ag2, bg2, sg2, _ = th.batch_lanczos(mats, R, M, start, tol, use_graph=True)
+    for x, y in ((ag, ae), (bg, be_), (ag, ag2), (bg, bg2)):
+        assert np.array_equal(x.view(np.uint64), y.view(np.uint64))
+    assert np.array_equal(sg, se) and np.array_equal(sg, sg2)
+
+
+@pytest.mark.parametrize("samples", [6, 70])
+def test_ftlm_batched_matches_per_block_driver(samples):
+    say("ftlm_batched vs ftlm, samples", samples)
+    N = 10
+    H = chiral_j1j2(N)
+    mats = blocks(H, N, [(5, 0), (5, 3), (4, 1), (4, 6)])
+    betas = 1.0 / np.linspace(0.2, 3.0, 7)
+    krylov = 18
+    seeds = [th.next_block_seed(7 + i) for i in range(len(mats))]
+    tol = [EPS64 * H.norm_bound()] * len(mats)
+    lnZ, E, V, emin, stats = th.ftlm_batched(mats, samples, krylov, seeds, betas, breakdown_tol=tol)
+    assert stats["rows"] == sum(m.dim for m in mats)
+    for b, m in enumerate(mats):
+        l1, e1, v1, em1 = m.ftlm(samples, krylov, seeds[b], betas, breakdown_tol=tol[b])
+        assert np.allclose(lnZ[b], l1, rtol=1e-8, atol=1e-8)
+        assert np.allclose(E[b], e1, rtol=1e-8, atol=1e-8)
+        assert np.allclose(V[b], v1, rtol=1e-8, atol=1e-8)
+        assert abs(emin[b] - em1) <= 1e-8 * max(1.0, abs(em1))
+
+
+def test_exact_thermal_matches_dense_eigensolve():
+    say("exact_thermal vs numpy eigvalsh")
+    N = 12
+    H = chiral_j1j2(N)
+    mats = blocks(H, N, [(6, 0), (6, 5), (3, 2), (1, 0)])
+    betas = 1.0 / np.linspace(0.25, 3.0, 6)
+    lnZ, E, V, emin = th.exact_thermal(mats, betas)
+    for b, m in enumerate(mats):
+        row_ptr, col, val = m.export_csr()
+        A = np.zeros((m.dim, m.dim), np.complex128)
+        for r in range(m.dim):
+            for e in range(int(row_ptr[r]), int(row_ptr[r + 1])):
+                A[r, col[e]] += val[e]
+        assert np.allclose(A, A.conj().T, atol=1e-13)
+        ev = np.linalg.eigvalsh(A)
+        e0 = ev.min()
+        assert abs(emin[b] - e0) <= 1e-10 * max(1.0, abs(e0))
+        for i, bt in enumerate(betas):
+            w = np.exp(-bt * (ev - e0))
+            Z = w.sum()
+            Em = (w * ev).sum() / Z
+            Vm = (w * (ev - Em) ** 2).sum() / Z
+            assert abs(lnZ[b, i] - (np.log(Z) - bt * e0)) <= 1e-10 * max(1.0, abs(lnZ[b, i]))
+            assert abs(E[b, i] - Em) <= 1e-10 * max(1.0, abs(Em))
+            assert abs(V[b, i] - Vm) <= 1e-10 * max(1.0, abs(Vm))
+
+
+def test_thermal_reproduces_reference_ftlm_record():
+    say("thermal ftlm none vs reference record")
+    rec = records()
+    r = rec["records"]["j1j2_chain12/api/thermal/ftlm/none"]["values"]
+    res = th.thermal(j1j2(12), np.array(r["T"]), sym=qs.Symmetry.none(), samples=8, krylov=40, seed=11)
+    assert len(res.blocks) == 1 and res.blocks[0]["dim"] == 4096 and not res.blocks[0]["exact"]
+    for key, got in (("E", res.E), ("C", res.C), ("S", res.S)):
+        err = np.abs(got - np.array(r[key])).max()
+        print(f"  {key}: max |diff| {err:.3e}")
+        assert np.allclose(got, r[key], rtol=1e-8, atol=1e-8), key
+
+
+def test_thermal_reproduces_reference_exact_record():
+    say("thermal exact over (Sz, k) blocks vs reference record")
+    rec = records()
+    r = rec["records"]["j1j2_chain12/api/thermal/exact/auto"]["values"]
+    res = th.thermal(j1j2(12), np.array(r["T"]), sym=qs.Symmetry(), samples=4, krylov=20, seed=3)
+    assert all(b["exact"] for b in res.blocks) and res.total_dim == 4096
+    for key, got in (("E", res.E), ("C", res.C), ("S", res.S)):
+        assert np.allclose(got, r[key], rtol=1e-10, atol=1e-10), key
+
+
+def _oracle_block(H_products, N, n_up, k):
+    """The block matrix from the dense oracle: E^dagger H E on the brute-force sector."""
+    s = po.OracleSector(N, n_up, sym.translations(N), sym.momentum_chars(N, k))
+    states = np.array([x for x in range(1 << N) if bin(x).count("1") == n_up], np.uint64)
+    Hd = po.dense_matrix(H_products, N, states)
+    Em = po.sector_basis_matrix(s, states)
+    return s, Em.conj().T @ Hd @ Em
+
+
+def test_thermal_mixed_walk_matches_oracle():
+    say("thermal mixed exact/sampled walk vs oracle")
+    N, dmax, samples, krylov, seed = 10, 12, 5, 15, 7
+    H = xxz(N)
+    T = np.linspace(0.3, 3.0, 5)
+    betas = 1.0 / T
+    res = th.thermal(H, T, sym=qs.Symmetry(), samples=samples, krylov=krylov, seed=seed, dense_max_dim=dmax)
+    assert res.total_dim == 1 << N
+    assert any(b["exact"] for b in res.blocks) and any(not b["exact"] for b in res.blocks)
+    # the oracle, block by block in the same walk order with the same seed stream
+    seed_b = seed
+    lnZ, E, V = [], [], []
+    for n_up in range(N + 1):
+        for k in range(N):
+            s, A = _oracle_block(H.products, N, n_up, k)
+            if s.dim == 0:
+                continue
+            seed_b = th.next_block_seed(seed_b)
+            if s.dim <= dmax:
+                ev = np.linalg.eigvalsh(A)
+                e0 = ev.min()
+                w = np.exp(-np.outer(betas, ev - e0))
+                Z = w.sum(1)
+                Em = (w * ev).sum(1) / Z
+                lnZ.append(np.log(Z) - betas * e0)
+                E.append(Em)
+                V.append((w * (ev[None, :] - Em[:, None]) ** 2).sum(1) / Z)
+                continue
+            rp = np.zeros(s.dim + 1, np.int64)
+            cols, vals = [], []
+            for r_ in range(s.dim):
+                nz = np.nonzero(A[r_])[0]
+                cols += list(nz)
+                vals += list(A[r_, nz])
+                rp[r_ + 1] = len(cols)
+            cols, vals = np.array(cols, np.int64), np.array(vals, np.complex128)
+            mom = []
+            for q in range(samples):
+                v0 = po.gaussian_vector(seed_b, q, s.dim)
+                a, b = po.lanczos(rp, cols, vals, v0, krylov, 2, EPS64 * H.norm_bound())
+                ritz, Vv = po.tridiag_eig(a, b)
+                mom.append(po.ftlm_moments(ritz, Vv[0] ** 2, betas))
+            l_, e_, v_, _ = po.ftlm_combine(np.array(mom), s.dim, betas)
+            lnZ.append(l_)
+            E.append(e_)
+            V.append(v_)
+    lnZ, E, V = np.array(lnZ), np.array(E), np.array(V)
+    assert lnZ.shape[0] == len(res.blocks)
+    for i, b in enumerate(res.blocks):
+        assert np.allclose(b["lnZ"], lnZ[i], rtol=1e-8, atol=1e-8), b["label"]
+        assert np.allclose(b["E"], E[i], rtol=1e-8, atol=1e-8), b["label"]
+    mx = lnZ.max(0)
+    p = np.exp(lnZ - mx)
+    z = p.sum(0)
+    Et = (p * E).sum(0) / z
+    var = (p * (V + (E - Et) ** 2)).sum(0) / z
+    assert np.allclose(res.lnZ, mx + np.log(z), rtol=1e-8, atol=1e-8)
+    assert np.allclose(res.E, Et, rtol=1e-8, atol=1e-8)
+    assert np.allclose(res.C, betas ** 2 * var, rtol=1e-8, atol=1e-8)

@@ -3,11 +3,12 @@ brute force, the Burnside count and closed forms. A plausible slip -- a dropped 
 sign in the projection, a transposed permutation, a column merge in the wrong order -- fails here."""
 
 import math
+import os
 
 import numpy as np
 import pytest
 
-from conftest import fixture_names, fixture_program, fixture_products, load_fixture
+from conftest import FIXTURES, fixture_names, fixture_program, fixture_products, load_fixture
 from oracle import pyoracle as po
 
 NAMES = fixture_names()
@@ -209,3 +210,63 @@ def test_seeded_gaussian_is_unit_and_reproducible():
     c = po.gaussian_vector(11, 1, 1000)
     assert np.array_equal(a, b) and not np.array_equal(a, c)
     assert abs(np.linalg.norm(a) - 1.0) < 1e-14
+
+
+def _reference_records():
+    import json
+
+    with open(os.path.join(FIXTURES, "reference_records.json")) as f:
+        return json.load(f)
+
+
+def _full_space_j1j2_chain12():
+    """The J1-J2 ring N = 12, J2 = 0.5 on the whole 2^12 space (trivial group): the reference's
+    thermal 'none' block, states in ascending order."""
+    fx = load_fixture("j1j2_n12_k0")   # the same Hamiltonian's row program
+    s = po.OracleSector(12, -1, np.arange(12, dtype=np.int32)[None, :], [1.0])
+    row_ptr, col, val = po.sector_csr(fixture_program(fx), s)
+    return s, row_ptr, col, val
+
+
+def test_ftlm_reproduces_reference_golden_record():
+    """The oracle's FTLM (sample seeds, LocalDGKS3 Lanczos, Ritz weights, moments, Jensen combine)
+    against the reference's own recorded FTLM of j1j2_chain12 with no symmetry (one 4096-state
+    block, 8 samples, krylov 40, seed 11: the block's seed is the walk's first LCG step)."""
+    rec = _reference_records()
+    r = rec["records"]["j1j2_chain12/api/thermal/ftlm/none"]["values"]
+    T = np.array(r["T"])
+    betas = 1.0 / T
+    s, row_ptr, col, val = _full_space_j1j2_chain12()
+    assert s.dim == 4096
+    seed = (11 * 6364136223846793005 + 1442695040888963407) % (1 << 64)
+    moments = []
+    for k in range(8):
+        v0 = po.gaussian_vector(seed, k, s.dim)
+        alpha, beta = po.lanczos(row_ptr, col, val, v0, 40, 2, 64 * np.finfo(float).eps * 36.0)
+        ritz, V = po.tridiag_eig(alpha, beta)
+        moments.append(po.ftlm_moments(ritz, V[0] ** 2, betas))
+    lnZ, E, Vv, _ = po.ftlm_combine(np.array(moments), s.dim, betas)
+    C = betas ** 2 * Vv
+    S = lnZ + betas * E
+    assert np.allclose(E, r["E"], rtol=1e-10, atol=1e-10)
+    assert np.allclose(C, r["C"], rtol=1e-10, atol=1e-10)
+    assert np.allclose(S, r["S"], rtol=1e-10, atol=1e-10)
+
+
+def test_exact_thermal_reproduces_reference_golden_record():
+    """Exact thermodynamics of the whole spectrum (dense eigvalsh of the oracle's matrix) against
+    the reference's recorded exact curves of j1j2_chain12."""
+    rec = _reference_records()
+    r = rec["records"]["j1j2_chain12/api/thermal/exact/none"]["values"]
+    betas = 1.0 / np.array(r["T"])
+    s, row_ptr, col, val = _full_space_j1j2_chain12()
+    ev = np.linalg.eigvalsh(dense_csr(row_ptr, col, val, s.dim))
+    e0 = ev.min()
+    for i, b in enumerate(betas):
+        w = np.exp(-b * (ev - e0))
+        Z = w.sum()
+        E = (w * ev).sum() / Z
+        V = (w * (ev - E) ** 2).sum() / Z
+        assert abs(E - r["E"][i]) < 1e-10 * max(1.0, abs(E))
+        assert abs(b * b * V - r["C"][i]) < 1e-10 * max(1.0, abs(r["C"][i]))
+        assert abs(np.log(Z) - b * e0 + b * E - r["S"][i]) < 1e-10 * max(1.0, abs(r["S"][i]))
