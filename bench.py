@@ -191,11 +191,15 @@ def main():
         cpu = cpu_baseline(m, D, w_id)
         log(f"cpu baseline: {cpu}")
 
+    # HBM bytes per launch of this SpMV from the committed PMC summary (FETCH_SIZE and WRITE_SIZE,
+    # each scaled by the same counter of a 1 GiB copy), valid for the lanes-per-row it was taken at
     traffic = None
     tf = os.path.join(ROOT, "profiles", "spmv_n32_traffic.json")
     if os.path.exists(tf):
         with open(tf) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            rec = json.load(f)
+        if int(rec.get("lanes_per_row", -1)) == int(info["lanes"]):
+            traffic = rec.get("hbm_bytes_per_launch")
 
     out = {
         "metric": "FP64 sector matvec GB/s (% HBM peak)",

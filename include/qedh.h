@@ -65,6 +65,13 @@ int qedh_program_compile_projected(qedh_operator op, int adjoint, qedh_sector sr
 int qedh_program_counts(qedh_program p, int64_t* n_groups, int64_t* n_vsub, int64_t* n_terms);
 int qedh_program_export(qedh_program p, uint64_t* group_flip, int32_t* group_setbits, uint32_t* group_vbegin,
                         uint64_t* vsub_val, uint32_t* vsub_tbegin, uint64_t* term_sign, double* term_coeff);
+/* A program from the reference's MaskedProgram arrays as they are (group_vbegin[n_groups + 1],
+ * vsub_tbegin[n_vsub + 1], term_coeff[2 n_terms] re/im): the adapter's way in for a program the
+ * reference already compiled (include/ed/ops/program.h:66-95). Offsets are checked. */
+int qedh_program_import(int64_t n_groups, const uint64_t* group_flip, const int32_t* group_setbits,
+                        const uint32_t* group_vbegin, int64_t n_vsub, const uint64_t* vsub_val,
+                        const uint32_t* vsub_tbegin, int64_t n_terms, const uint64_t* term_sign,
+                        const double* term_coeff, qedh_program* out);
 int qedh_program_destroy(qedh_program p);
 
 /* ---- sectors: enumerated on the device (include/ed/basis/orbit_table.h:250-286,

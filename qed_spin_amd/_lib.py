@@ -61,6 +61,10 @@ _SIGNATURES = {
     "qedh_program_compile_projected": [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)],
     "qedh_program_counts": [C.c_void_p, c_i64_p, c_i64_p, c_i64_p],
     "qedh_program_export": [C.c_void_p, c_u64_p, c_i32_p, c_u32_p, c_u64_p, c_u32_p, c_u64_p, c_dbl_p],
+    "qedh_program_import": [
+        C.c_int64, c_u64_p, c_i32_p, c_u32_p, C.c_int64, c_u64_p, c_u32_p, C.c_int64, c_u64_p, c_dbl_p,
+        C.POINTER(C.c_void_p),
+    ],
     "qedh_program_destroy": [C.c_void_p],
     "qedh_sector_create": [
         C.c_int, C.c_int, C.c_int, c_int_p, c_u64_p, c_dbl_p, C.c_int, C.c_int, C.c_uint64, C.POINTER(C.c_void_p)

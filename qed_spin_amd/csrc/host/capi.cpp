@@ -190,6 +190,45 @@ int qedh_program_export(qedh_program p, uint64_t* group_flip, int32_t* group_set
             }
     });
 }
+int qedh_program_import(int64_t n_groups, const uint64_t* group_flip, const int32_t* group_setbits,
+                        const uint32_t* group_vbegin, int64_t n_vsub, const uint64_t* vsub_val,
+                        const uint32_t* vsub_tbegin, int64_t n_terms, const uint64_t* term_sign,
+                        const double* term_coeff, qedh_program* out) {
+    return guarded([&] {
+        need(out, "out");
+        if (n_groups < 0 || n_vsub < 0 || n_terms < 0) throw qsa::Error(qsa::kInvalid, "program: negative count");
+        if (n_groups > 0) {
+            need(group_flip, "group_flip");
+            need(group_setbits, "group_setbits");
+        }
+        need(group_vbegin, "group_vbegin");
+        need(vsub_tbegin, "vsub_tbegin");
+        if (n_vsub > 0) need(vsub_val, "vsub_val");
+        if (n_terms > 0) {
+            need(term_sign, "term_sign");
+            need(term_coeff, "term_coeff");
+        }
+        qsa::Program P;
+        P.group_flip.assign(group_flip, group_flip + n_groups);
+        P.group_setbits.assign(group_setbits, group_setbits + n_groups);
+        P.group_vbegin.assign(group_vbegin, group_vbegin + n_groups + 1);
+        P.vsub_val.assign(vsub_val, vsub_val + n_vsub);
+        P.vsub_tbegin.assign(vsub_tbegin, vsub_tbegin + n_vsub + 1);
+        P.term_sign.assign(term_sign, term_sign + n_terms);
+        for (int64_t k = 0; k < n_terms; ++k) P.term_coeff.emplace_back(term_coeff[2 * k], term_coeff[2 * k + 1]);
+        // The offsets must be monotone and end at the counts the kernels index with.
+        auto monotone = [](const std::vector<std::uint32_t>& v, std::uint64_t end) {
+            if (v.empty() || v.front() != 0 || v.back() != end) return false;
+            for (std::size_t i = 1; i < v.size(); ++i)
+                if (v[i] < v[i - 1]) return false;
+            return true;
+        };
+        if (!monotone(P.group_vbegin, static_cast<std::uint64_t>(n_vsub)) ||
+            !monotone(P.vsub_tbegin, static_cast<std::uint64_t>(n_terms)))
+            throw qsa::Error(qsa::kInvalid, "program: offsets are not monotone or do not end at the counts");
+        *out = new qedh_program_s{std::move(P), nullptr};
+    });
+}
 int qedh_program_destroy(qedh_program p) {
     return guarded([&] { delete p; });
 }

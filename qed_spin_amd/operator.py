@@ -109,6 +109,31 @@ class Program:
             _lib.call("qedh_program_compile_projected", eop.h, int(adjoint), src.h, tgt.h, C.byref(h))
         self.h = h
 
+    @classmethod
+    def from_arrays(cls, arrays: dict) -> "Program":
+        """A program from the reference's MaskedProgram arrays (group_flip, group_setbits,
+        group_vbegin, vsub_val, vsub_tbegin, term_sign, term_coeff as re/im pairs)."""
+        a = {
+            "group_flip": np.ascontiguousarray(arrays["group_flip"], np.uint64),
+            "group_setbits": np.ascontiguousarray(arrays["group_setbits"], np.int32),
+            "group_vbegin": np.ascontiguousarray(arrays["group_vbegin"], np.uint32),
+            "vsub_val": np.ascontiguousarray(arrays["vsub_val"], np.uint64),
+            "vsub_tbegin": np.ascontiguousarray(arrays["vsub_tbegin"], np.uint32),
+            "term_sign": np.ascontiguousarray(arrays["term_sign"], np.uint64),
+            "term_coeff": np.ascontiguousarray(np.asarray(arrays["term_coeff"]).view(np.float64)
+                                               if np.iscomplexobj(arrays["term_coeff"]) else arrays["term_coeff"],
+                                               np.float64),
+        }
+        self = cls.__new__(cls)
+        h = C.c_void_p()
+        P = _lib.ptr
+        _lib.call("qedh_program_import", C.c_int64(a["group_flip"].size), P(a["group_flip"], C.c_uint64),
+                  P(a["group_setbits"], C.c_int32), P(a["group_vbegin"], C.c_uint32), C.c_int64(a["vsub_val"].size),
+                  P(a["vsub_val"], C.c_uint64), P(a["vsub_tbegin"], C.c_uint32), C.c_int64(a["term_sign"].size),
+                  P(a["term_sign"], C.c_uint64), P(a["term_coeff"], C.c_double), C.byref(h))
+        self.h = h
+        return self
+
     def counts(self):
         g, v, t = C.c_int64(), C.c_int64(), C.c_int64()
         _lib.call("qedh_program_counts", self.h, C.byref(g), C.byref(v), C.byref(t))

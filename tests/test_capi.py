@@ -10,7 +10,7 @@ import pytest
 
 from conftest import ROOT, fixture_names, fixture_products, load_fixture
 
-from qed_spin_amd import _lib
+from qed_spin_amd import _lib, errors
 from qed_spin_amd.operator import Operator, Program
 
 HEADER = os.path.join(ROOT, "include", "qedh.h")
@@ -63,3 +63,21 @@ def test_norm_bound_and_realness():
     assert abs(op.norm_bound() - 1.25) < 1e-15
     op.add_product("xyz", (0, 1, 2), 0.3)
     assert not op.is_real()
+
+
+@pytest.mark.parametrize("name", ["j1j2_n12_k3", "chiral_n10_k2"])
+def test_program_import_round_trip(name):
+    """A program handed over as the reference's MaskedProgram arrays comes back unchanged."""
+    fx = load_fixture(name)
+    arrays = {k[len("program_"):]: fx[k] for k in fx if k.startswith("program_")}
+    P = Program.from_arrays(arrays).export()
+    for k, v in arrays.items():
+        assert np.array_equal(np.asarray(P[k]).view(np.uint8), np.ascontiguousarray(v).view(np.uint8)), k
+
+
+def test_program_import_rejects_bad_offsets():
+    fx = load_fixture("j1j2_n12_k3")
+    arrays = {k[len("program_"):]: fx[k].copy() for k in fx if k.startswith("program_")}
+    arrays["group_vbegin"][-1] += 1
+    with pytest.raises(errors.InvalidRequest):
+        Program.from_arrays(arrays)
