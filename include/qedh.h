@@ -124,8 +124,11 @@ int qedh_matvec_apply_multi(qedh_matvec m, const void* const* ins, void* const* 
 /* Block-interleaved X[row * nv + i], nv in {1, 2, 4, 8}; CSR lane. */
 int qedh_matvec_apply_block(qedh_matvec m, int nv, const void* X, void* Y, int complex_vectors);
 int qedh_matvec_set_lanes(qedh_matvec m, int lanes);
-/* SpMV tuning: lanes per row (0 keeps the current) and non-temporal streaming of the CSR arrays. */
-int qedh_matvec_set_tuning(qedh_matvec m, int lanes, int nontemporal);
+/* SpMV tuning (no counterpart in the reference): lanes per row (0 keeps the current) and flags:
+ * non-temporal streaming of the CSR arrays, one contiguous row range per XCD, the single-vector
+ * gathers issued four entries at a time (every flag leaves the results bit for bit unchanged). */
+enum { QEDH_TUNE_NONTEMPORAL = 1, QEDH_TUNE_XCD_ROWS = 2, QEDH_TUNE_UNROLL = 4 };
+int qedh_matvec_set_tuning(qedh_matvec m, int lanes, int flags);
 /* The merged CSR (download_csr): row_ptr[rows+1], col[nnz], val[2 nnz] (re, im). */
 int qedh_matvec_export_csr(qedh_matvec m, uint64_t* row_ptr, uint32_t* col, double* val);
 /* Rows [r0, r1) of it: row_ptr[r1 - r0 + 1] (absolute offsets), then col / val of those rows
@@ -150,6 +153,11 @@ int qedh_lanczos(qedh_matvec m, const void* v0_host, int complex_vectors, int ma
  * values[k]; *n_values found; *converged; *applies. */
 int qedh_block_lowest(qedh_matvec m, int k, int64_t max_iter, double* values, int* n_values, int* converged,
                       int64_t* applies);
+/* The lowest k eigenpairs of the block (Krylov-Schur with locking at 1e-10 s_H, the vectors of
+ * solve_block_eigenpairs, src/engine/block_solve.cpp:600-655): values[k] ascending, vectors: a
+ * DEVICE buffer of k x dim complex entries (vector i at vectors + i dim), normalised. */
+int qedh_block_eigenpairs(qedh_matvec m, int k, int64_t max_iter, double* values, void* vectors, int* n_values,
+                          int* converged);
 /* FTLM of the block: samples from splitmix64(seed + phi (s + 1)); lnZ/E/V per beta; *e_min. */
 int qedh_ftlm(qedh_matvec m, int samples, int krylov, uint64_t seed, int n_beta, const double* betas,
               double breakdown_tol, double* lnZ, double* E, double* V, double* e_min);
@@ -162,14 +170,15 @@ int qedh_cf(qedh_matvec m, const double* phi_host, int krylov, int n_omega, cons
 /* ---- Batched Krylov engine: many blocks x R vectors per block in one packed multi-vector
  * matvec per step, the step captured as a hipGraph (replaces the per-block, lockstep-of-8 sample
  * loop of include/ed/thermal/ftlm.h:277-480 driven by src/engine/thermal.cpp:161-344, 471-535) ---- */
-/* LocalDGKS3 Lanczos (K = 1, no basis) of R vectors in each of n_blocks square CSR-lane blocks.
+/* Lanczos without a kept basis of R vectors in each of n_blocks square CSR-lane blocks: reorth 2
+ * LocalDGKS3 with K = 1 (FTLM), 0 none (the continued fraction).
  * start: the complex start vectors in batch layout, block b vector s at entry (off_b + i) R + s,
  * off_b = the summed dimensions of the blocks before b (normalised inside). breakdown_tol[b] per
  * block. Outputs alpha[krylov][n_blocks][R], beta[krylov][n_blocks][R] (beta of step j is
  * beta_{j+1}), steps[n_blocks][R]. timings[3] = pack s, steps s, total s (may be NULL). */
 int qedh_batch_lanczos(const qedh_matvec* blocks, int n_blocks, int R, int krylov, const double* start,
-                       const double* breakdown_tol, int use_graph, double* alpha, double* beta, int* steps,
-                       double* timings);
+                       const double* breakdown_tol, int reorth, int use_graph, double* alpha, double* beta,
+                       int* steps, double* timings);
 /* FTLM of n_blocks blocks at once: block b samples s = 0..samples-1 from
  * splitmix64(block_seeds[b] + phi (s + 1)) as qedh_ftlm does, breakdown_tol[b] per block (NULL:
  * 64 eps times the block's norm bound, kBreakdownRel of include/ed/core/numerics.h:23).
@@ -178,6 +187,16 @@ int qedh_batch_lanczos(const qedh_matvec* blocks, int n_blocks, int R, int krylo
 int qedh_ftlm_batched(const qedh_matvec* blocks, int n_blocks, int samples, int krylov, const uint64_t* block_seeds,
                       const double* breakdown_tol, int n_beta, const double* betas, int use_graph, double* lnZ,
                       double* E, double* V, double* e_min, double* stats);
+/* Continued fractions of many (block, vector) pairs at once (cf_spectral_from_vector,
+ * include/ed/dynamics/cf.h:103-161, for every probe image the dynamics verb forms,
+ * src/engine/dynamics.cpp:528-560): starts[b R + s] a DEVICE complex vector of block b's dimension
+ * (NULL: no pair); Lanczos without reorthogonalisation, krylov steps, breakdown_tol (<= 0: 1e-12);
+ * shift = energy_shift when has_shift, else each pair's lowest Ritz value. Outputs
+ * S[n_blocks R][n_omega] (weighted by ||phi||^2), norm_sq, shifts, steps (each may be NULL but S),
+ * stats[7] = pack s, steps s, host s, total s, applies, rows, nnz (may be NULL). */
+int qedh_cf_batched(const qedh_matvec* blocks, int n_blocks, int R, const void* const* starts, int krylov,
+                    int has_shift, double energy_shift, double breakdown_tol, int n_omega, const double* omega,
+                    double eta, int use_graph, double* S, double* norm_sq, double* shifts, int* steps, double* stats);
 /* Exact thermodynamics of small blocks (the reference's host dense lane for blocks of at most
  * dense_max_dim states, src/engine/thermal.cpp:197-200, curves of include/ed/thermal/curves.h:38-65):
  * each block's CSR expanded to a dense Hermitian matrix on the host and its whole spectrum solved.

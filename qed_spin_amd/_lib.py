@@ -87,6 +87,7 @@ _SIGNATURES = {
     "qedh_matvec_benchmark": [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, c_dbl_p],
     "qedh_lanczos": [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, c_dbl_p, c_dbl_p, c_int_p, c_dbl_p],
     "qedh_block_lowest": [C.c_void_p, C.c_int, C.c_int64, c_dbl_p, c_int_p, c_int_p, c_i64_p],
+    "qedh_block_eigenpairs": [C.c_void_p, C.c_int, C.c_int64, c_dbl_p, C.c_void_p, c_int_p, c_int_p],
     "qedh_ftlm": [
         C.c_void_p, C.c_int, C.c_int, C.c_uint64, C.c_int, c_dbl_p, C.c_double, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p
     ],
@@ -95,12 +96,16 @@ _SIGNATURES = {
         c_dbl_p, c_dbl_p, c_dbl_p,
     ],
     "qedh_batch_lanczos": [
-        C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_int, c_dbl_p, c_dbl_p, C.c_int, c_dbl_p, c_dbl_p, c_int_p,
-        c_dbl_p,
+        C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_int, c_dbl_p, c_dbl_p, C.c_int, C.c_int, c_dbl_p, c_dbl_p,
+        c_int_p, c_dbl_p,
     ],
     "qedh_ftlm_batched": [
         C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_int, c_u64_p, c_dbl_p, C.c_int, c_dbl_p, C.c_int, c_dbl_p,
         c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p,
+    ],
+    "qedh_cf_batched": [
+        C.POINTER(C.c_void_p), C.c_int, C.c_int, C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_double, C.c_double,
+        C.c_int, c_dbl_p, C.c_double, C.c_int, c_dbl_p, c_dbl_p, c_dbl_p, c_int_p, c_dbl_p,
     ],
     "qedh_exact_thermal": [
         C.POINTER(C.c_void_p), C.c_int, C.c_int, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p,

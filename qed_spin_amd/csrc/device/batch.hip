@@ -3,13 +3,23 @@
 //
 // Layout: the blocks' CSRs are packed into one block-diagonal CSR (rows and columns offset by the
 // block's first row, values in full); a batch vector holds R entries per row, X[row R + s], so the
-// R samples of a row are one contiguous line and a wavefront maps lane s to sample s. Rows are cut
-// into tiles of at most kTileRows rows of one block; every kernel runs one workgroup per tile and one
-// wavefront per row, so a per-(block, sample) reduction is a lane-wise sum over the tile's rows, the
-// four wavefronts' sums in order, and then the tiles of the block in order (deterministic, and
-// independent of which other blocks share the batch). The Lanczos scalars alpha, beta and the
-// local-reorthogonalisation overlap live on the device; a step needs no host synchronisation, so
-// it is captured once as a hipGraph and replayed.
+// R vectors of a row are one contiguous line. Rows are cut into tiles of at most kBatchTileRows rows
+// of one block, one workgroup of 256 threads per tile.
+//
+// Vector kernels: thread t of a tile is (slot t / R, vector t % R); the 256 / R slots take the
+// tile's rows in turn, so consecutive threads touch consecutive entries whatever R is. A
+// per-(block, vector) reduction is each thread's sum over its rows, then the slots' sums in slot
+// order (LDS), then the tiles of the block in order: deterministic, and independent of which other
+// blocks share the batch.
+//
+// SpMV: for R >= 8 one thread per (row, vector), the R threads of a row reading the same column and
+// value and gathering one contiguous line of R entries; for R < 8, 16 threads per row (R rounded up
+// to a power of two, 16 / that many lanes per (row, vector)) splitting the row's entries, summed
+// by a fixed xor butterfly.
+//
+// The Lanczos scalars alpha, beta, the local-reorthogonalisation overlap, the active flags and the
+// step counter live on the device; a step needs no host synchronisation, so it is captured once
+// as a hipGraph and replayed.
 #include "dev_common.h"
 #include "launch.h"
 
@@ -17,72 +27,110 @@ namespace qsa::dev {
 
 namespace {
 
-constexpr int kWavesPerTile = 4;
+constexpr int kThreads = 256;
 
 __device__ __forceinline__ Zd cmul(Zd a, Zd b) { return Zd{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
 __device__ __forceinline__ Zd cdot(Zd a, Zd b) { return Zd{a.re * b.re + a.im * b.im, a.re * b.im - a.im * b.re}; }
 
-// Y[r, s] = sum_e A[r, e] X[col[e], s]: one wavefront per row, lane s per sample, entries in order
-// (pairs no longer running are skipped when B.active is set).
-__global__ __launch_bounds__(256) void k_batch_spmv(BatchView B, const Zd* X, Zd* Y) {
+__device__ __forceinline__ bool pair_live(const BatchView& B, u32 block, int s) {
+    return !B.active || B.active[static_cast<u64>(block) * B.R + s] != 0;
+}
+
+// One thread per (row, vector), R >= 8.
+__global__ __launch_bounds__(kThreads) void k_batch_spmv_wide(BatchView B, const Zd* X, Zd* Y) {
     const Tile t = B.tiles[blockIdx.x];
-    const int wv = threadIdx.x >> 6, s = threadIdx.x & 63;
-    const int R = B.R;
-    const bool live = s < R && (!B.active || B.active[t.block * R + s] != 0);
-    for (u64 r = t.row_begin + wv; r < t.row_end; r += kWavesPerTile) {
+    const int R = B.R, slots = kThreads / R;
+    const int slot = threadIdx.x / R, s = threadIdx.x % R;
+    if (slot >= slots || !pair_live(B, t.block, s)) return;
+    for (u64 r = t.row_begin + slot; r < t.row_end; r += slots) {
         const u64 e0 = B.row_ptr[r], e1 = B.row_ptr[r + 1];
         double ar = 0.0, ai = 0.0;
-        if (live) {
-            for (u64 e = e0; e < e1; ++e) {
-                const u64 c = B.col[e];  // u32 column
+        for (u64 e = e0; e < e1; ++e) {
+            const u64 c = B.col[e];
+            const Zd v = B.val[e];
+            const Zd x = X[c * R + s];
+            ar += v.re * x.re - v.im * x.im;
+            ai += v.re * x.im + v.im * x.re;
+        }
+        Y[r * R + s] = Zd{ar, ai};
+    }
+}
+
+// 16 threads per row for R < 8: RP = R rounded up to a power of two, LR = 16 / RP lanes per
+// (row, vector); lane l of a vector takes entries e0 + l, e0 + l + LR, ...
+template <int RP>
+__global__ __launch_bounds__(kThreads) void k_batch_spmv_narrow(BatchView B, const Zd* X, Zd* Y) {
+    constexpr int LR = 16 / RP;
+    constexpr int slots = kThreads / 16;
+    const Tile t = B.tiles[blockIdx.x];
+    const int R = B.R;
+    const int slot = threadIdx.x / 16, w = threadIdx.x % 16;
+    const int s = w / LR, lane = w % LR;
+    const bool live = s < R && pair_live(B, t.block, s);
+    // every thread of a row group runs the same rounds (the butterfly needs all of them)
+    for (u64 base = t.row_begin; base < t.row_end; base += slots) {
+        const u64 r = base + slot;
+        double ar = 0.0, ai = 0.0;
+        if (live && r < t.row_end) {
+            const u64 e0 = B.row_ptr[r], e1 = B.row_ptr[r + 1];
+            for (u64 e = e0 + lane; e < e1; e += LR) {
+                const u64 c = B.col[e];
                 const Zd v = B.val[e];
                 const Zd x = X[c * R + s];
                 ar += v.re * x.re - v.im * x.im;
                 ai += v.re * x.im + v.im * x.re;
             }
-            Y[r * R + s] = Zd{ar, ai};
         }
+#pragma unroll
+        for (int off = LR / 2; off > 0; off >>= 1) {
+            ar += __shfl_xor(ar, off, LR);
+            ai += __shfl_xor(ai, off, LR);
+        }
+        if (live && r < t.row_end && lane == 0) Y[r * R + s] = Zd{ar, ai};
     }
 }
 
-// Lane-wise sums of the tile's rows into partial[tile][s] (the four wavefronts in order).
-__device__ __forceinline__ void tile_partial(double pr, double pi, double* partial) {
-    __shared__ double sr[kWavesPerTile][64], si[kWavesPerTile][64];
-    const int wv = threadIdx.x >> 6, s = threadIdx.x & 63;
-    sr[wv][s] = pr;
-    si[wv][s] = pi;
+// The slots' sums of each vector in slot order into partial[tile][s] (re, im). Every thread of the
+// workgroup calls it.
+__device__ __forceinline__ void tile_partial(const BatchView& B, double pr, double pi, double* partial) {
+    __shared__ double sr[kThreads], si[kThreads];
+    const int R = B.R, slots = kThreads / R;
+    sr[threadIdx.x] = pr;
+    si[threadIdx.x] = pi;
     __syncthreads();
-    if (wv == 0) {
+    if (static_cast<int>(threadIdx.x) < R) {
         double a = 0.0, b = 0.0;
-        for (int q = 0; q < kWavesPerTile; ++q) {
-            a += sr[q][s];
-            b += si[q][s];
+        for (int q = 0; q < slots; ++q) {
+            a += sr[q * R + threadIdx.x];
+            b += si[q * R + threadIdx.x];
         }
-        partial[(static_cast<u64>(blockIdx.x) * 64 + s) * 2] = a;
-        partial[(static_cast<u64>(blockIdx.x) * 64 + s) * 2 + 1] = b;
+        partial[(static_cast<u64>(blockIdx.x) * 64 + threadIdx.x) * 2] = a;
+        partial[(static_cast<u64>(blockIdx.x) * 64 + threadIdx.x) * 2 + 1] = b;
     }
 }
 
-// mode 0: W -= beta V_prev (beta of this step; skipped at step 0), then partial of <V_cur, W>;
-// mode 1: W -= alpha V_cur, then partial of <V_cur, W>;
-// mode 2: W -= ov V_cur when |ov| > threshold, then partial of ||W||^2;
+// mode 0: W -= beta V_prev (skipped at step 0), then partial of <V_cur, W> (alpha);
+// mode 1: W -= alpha V_cur, then partial of <V_cur, W> (the local overlap);
+// mode 4: W -= alpha V_cur, then partial of ||W||^2 (no reorthogonalisation);
+// mode 2: W -= ov V_cur when |ov| > threshold (from step 1 on), then partial of ||W||^2;
 // mode 3: W *= 1 / beta_next (the next basis vector).
-// Inactive (block, sample) pairs are left alone.
-__global__ __launch_bounds__(256) void k_batch_vec(BatchView B, int mode, const Zd* Vprev, const Zd* Vcur, Zd* W,
-                                                   const double* coef, const Zd* zcoef, double threshold,
-                                                   double* partial) {
+// Pairs no longer running are left alone.
+__global__ __launch_bounds__(kThreads) void k_batch_vec(BatchView B, int mode, const Zd* Vprev, const Zd* Vcur, Zd* W,
+                                                        const double* coef, const Zd* zcoef, double threshold,
+                                                        double* partial) {
     const Tile t = B.tiles[blockIdx.x];
-    const int wv = threadIdx.x >> 6, s = threadIdx.x & 63;
-    const int R = B.R;
-    const bool live = s < R && B.active[t.block * R + s] != 0;
+    const int R = B.R, slots = kThreads / R;
+    const int slot = threadIdx.x / R, s = threadIdx.x % R;
+    const bool live = slot < slots && pair_live(B, t.block, s);
     const u64 bs = static_cast<u64>(t.block) * R + s;
     double pr = 0.0, pi = 0.0;
     if (live) {
-        for (u64 r = t.row_begin + wv; r < t.row_end; r += kWavesPerTile) {
+        const int step = *B.step;
+        for (u64 r = t.row_begin + slot; r < t.row_end; r += slots) {
             const u64 i = r * R + s;
             Zd w = W[i];
             if (mode == 0) {
-                if (*B.step > 0) {
+                if (step > 0) {
                     const double b = coef[bs];
                     const Zd p = Vprev[i];
                     w = Zd{w.re - b * p.re, w.im - b * p.im};
@@ -91,17 +139,21 @@ __global__ __launch_bounds__(256) void k_batch_vec(BatchView B, int mode, const 
                 const Zd d = cdot(Vcur[i], w);
                 pr += d.re;
                 pi += d.im;
-            } else if (mode == 1) {
+            } else if (mode == 1 || mode == 4) {
                 const double a = coef[bs];
                 const Zd v = Vcur[i];
                 w = Zd{w.re - a * v.re, w.im - a * v.im};
                 W[i] = w;
-                const Zd d = cdot(v, w);
-                pr += d.re;
-                pi += d.im;
+                if (mode == 1) {
+                    const Zd d = cdot(v, w);
+                    pr += d.re;
+                    pi += d.im;
+                } else {
+                    pr += w.re * w.re + w.im * w.im;
+                }
             } else if (mode == 2) {
                 const Zd ov = zcoef[bs];
-                if (*B.step > 0 && sqrt(ov.re * ov.re + ov.im * ov.im) > threshold) {
+                if (step > 0 && sqrt(ov.re * ov.re + ov.im * ov.im) > threshold) {
                     const Zd m = cmul(ov, Vcur[i]);
                     w = Zd{w.re - m.re, w.im - m.im};
                     W[i] = w;
@@ -113,21 +165,21 @@ __global__ __launch_bounds__(256) void k_batch_vec(BatchView B, int mode, const 
             }
         }
     }
-    if (mode != 3) tile_partial(pr, pi, partial);
+    if (mode != 3) tile_partial(B, pr, pi, partial);
 }
 
-// Per block, lane s: the block's tiles in order. kind 0: alpha (real part) -> alpha_hist[step];
-// kind 1: the overlap (complex) -> ov; kind 2: beta = sqrt -> beta_hist[step + 1], 1 / beta ->
-// inv_beta, breakdown (beta < tol) or the block's Krylov cap ends the pair's run.
-__global__ __launch_bounds__(64) void k_batch_finish(BatchView B, int kind, int step, const double* partial,
-                                                     double* alpha_hist, double* beta_hist, Zd* ov, double* cur_alpha,
-                                                     double* cur_beta, double* inv_beta, const double* breakdown_tol) {
+// Per block, thread s: the block's tiles in order. kind 0: alpha (real part) -> alpha_hist[step];
+// kind 1: the overlap (complex) -> ov; kind 2: beta = sqrt -> beta_hist[step], 1 / beta ->
+// inv_beta; breakdown (beta < tol) or the block's Krylov cap ends the pair's run.
+__global__ __launch_bounds__(64) void k_batch_finish(BatchView B, int kind, const double* partial, double* alpha_hist,
+                                                     double* beta_hist, Zd* ov, double* cur_alpha, double* cur_beta,
+                                                     double* inv_beta, const double* breakdown_tol) {
     const int b = blockIdx.x, s = threadIdx.x;
     const int R = B.R;
     if (s >= R) return;
     const u64 bs = static_cast<u64>(b) * R + s;
     if (!B.active[bs]) return;
-    if (step < 0) step = *B.step;   // captured graphs read the step from the device
+    const int step = *B.step;
     double a = 0.0, c = 0.0;
     for (u32 t = B.tile_begin[b]; t < B.tile_begin[b + 1]; ++t) {
         a += partial[(static_cast<u64>(t) * 64 + s) * 2];
@@ -153,36 +205,47 @@ __global__ __launch_bounds__(64) void k_batch_finish(BatchView B, int kind, int 
     }
 }
 
-// 1 / ||V0|| per pair, then V0 scaled: the normalisation of the start vectors.
-__global__ __launch_bounds__(256) void k_batch_norm_partial(BatchView B, const Zd* V, double* partial) {
+// ||V||^2 per pair, then V scaled by 1 / ||V||: the normalisation of the start vectors.
+__global__ __launch_bounds__(kThreads) void k_batch_norm_partial(BatchView B, const Zd* V, double* partial) {
     const Tile t = B.tiles[blockIdx.x];
-    const int wv = threadIdx.x >> 6, s = threadIdx.x & 63;
+    const int R = B.R, slots = kThreads / R;
+    const int slot = threadIdx.x / R, s = threadIdx.x % R;
     double p = 0.0;
-    if (s < B.R)
-        for (u64 r = t.row_begin + wv; r < t.row_end; r += kWavesPerTile) {
-            const Zd v = V[r * B.R + s];
+    if (slot < slots)
+        for (u64 r = t.row_begin + slot; r < t.row_end; r += slots) {
+            const Zd v = V[r * R + s];
             p += v.re * v.re + v.im * v.im;
         }
-    tile_partial(p, 0.0, partial);
+    tile_partial(B, p, 0.0, partial);
 }
 
-__global__ __launch_bounds__(64) void k_batch_norm_finish(BatchView B, const double* partial, double* inv) {
+__global__ __launch_bounds__(64) void k_batch_norm_finish(BatchView B, const double* partial, double* inv,
+                                                          double* norm2) {
     const int b = blockIdx.x, s = threadIdx.x;
     if (s >= B.R) return;
     double a = 0.0;
     for (u32 t = B.tile_begin[b]; t < B.tile_begin[b + 1]; ++t) a += partial[(static_cast<u64>(t) * 64 + s) * 2];
-    inv[static_cast<u64>(b) * B.R + s] = a > 0.0 ? 1.0 / sqrt(a) : 0.0;
+    const u64 bs = static_cast<u64>(b) * B.R + s;
+    inv[bs] = a > 0.0 ? 1.0 / sqrt(a) : 0.0;
+    if (norm2) norm2[bs] = a;
 }
 
-__global__ __launch_bounds__(256) void k_batch_scale(BatchView B, Zd* V, const double* inv) {
+__global__ __launch_bounds__(kThreads) void k_batch_scale(BatchView B, Zd* V, const double* inv) {
     const Tile t = B.tiles[blockIdx.x];
-    const int wv = threadIdx.x >> 6, s = threadIdx.x & 63;
-    if (s >= B.R) return;
-    const double f = inv[static_cast<u64>(t.block) * B.R + s];
-    for (u64 r = t.row_begin + wv; r < t.row_end; r += kWavesPerTile) {
-        const Zd v = V[r * B.R + s];
-        V[r * B.R + s] = Zd{v.re * f, v.im * f};
+    const int R = B.R, slots = kThreads / R;
+    const int slot = threadIdx.x / R, s = threadIdx.x % R;
+    if (slot >= slots) return;
+    const double f = inv[static_cast<u64>(t.block) * R + s];
+    for (u64 r = t.row_begin + slot; r < t.row_end; r += slots) {
+        const Zd v = V[r * R + s];
+        V[r * R + s] = Zd{v.re * f, v.im * f};
     }
+}
+
+// V[(row0 + i) R + s] = src[i], i < n: one vector placed into the batch layout.
+__global__ void k_batch_place(Zd* V, int R, u64 row0, int s, const Zd* src, u64 n) {
+    for (u64 i = blockIdx.x * static_cast<u64>(blockDim.x) + threadIdx.x; i < n; i += static_cast<u64>(gridDim.x) * blockDim.x)
+        V[(row0 + i) * R + s] = src[i];
 }
 
 __global__ void k_batch_advance(BatchView B) {
@@ -219,32 +282,47 @@ __global__ void k_pack_csr(CsrView A, u64 row_off, u64 col_off, u64 nnz_off, u64
     }
 }
 
+unsigned grid_for(u64 n) {
+    u64 g = (n + 255) / 256;
+    if (g > 65536) g = 65536;
+    return static_cast<unsigned>(g ? g : 1);
+}
+
 }  // namespace
 
 void batch_spmv(const BatchView& B, const Zd* X, Zd* Y, hipStream_t st) {
     if (!B.n_tiles) return;
-    hipLaunchKernelGGL(k_batch_spmv, dim3(B.n_tiles), dim3(256), 0, st, B, X, Y);
+    const dim3 grid(B.n_tiles), block(kThreads);
+    if (B.R >= 8) hipLaunchKernelGGL(k_batch_spmv_wide, grid, block, 0, st, B, X, Y);
+    else if (B.R == 1) hipLaunchKernelGGL(k_batch_spmv_narrow<1>, grid, block, 0, st, B, X, Y);
+    else if (B.R == 2) hipLaunchKernelGGL(k_batch_spmv_narrow<2>, grid, block, 0, st, B, X, Y);
+    else if (B.R <= 4) hipLaunchKernelGGL(k_batch_spmv_narrow<4>, grid, block, 0, st, B, X, Y);
+    else hipLaunchKernelGGL(k_batch_spmv_narrow<8>, grid, block, 0, st, B, X, Y);
 }
 
 void batch_vec(const BatchView& B, int mode, const Zd* Vprev, const Zd* Vcur, Zd* W, const double* coef,
                const Zd* zcoef, double threshold, double* partial, hipStream_t st) {
     if (!B.n_tiles) return;
-    hipLaunchKernelGGL(k_batch_vec, dim3(B.n_tiles), dim3(256), 0, st, B, mode, Vprev, Vcur, W, coef, zcoef, threshold,
-                       partial);
+    hipLaunchKernelGGL(k_batch_vec, dim3(B.n_tiles), dim3(kThreads), 0, st, B, mode, Vprev, Vcur, W, coef, zcoef,
+                       threshold, partial);
 }
 
-void batch_finish(const BatchView& B, int kind, int step, const double* partial, double* alpha_hist,
-                  double* beta_hist, Zd* ov, double* cur_alpha, double* cur_beta, double* inv_beta,
-                  const double* breakdown_tol, hipStream_t st) {
-    hipLaunchKernelGGL(k_batch_finish, dim3(B.n_blocks), dim3(64), 0, st, B, kind, step, partial, alpha_hist,
-                       beta_hist, ov, cur_alpha, cur_beta, inv_beta, breakdown_tol);
+void batch_finish(const BatchView& B, int kind, const double* partial, double* alpha_hist, double* beta_hist, Zd* ov,
+                  double* cur_alpha, double* cur_beta, double* inv_beta, const double* breakdown_tol, hipStream_t st) {
+    hipLaunchKernelGGL(k_batch_finish, dim3(B.n_blocks), dim3(64), 0, st, B, kind, partial, alpha_hist, beta_hist, ov,
+                       cur_alpha, cur_beta, inv_beta, breakdown_tol);
 }
 
-void batch_normalize(const BatchView& B, Zd* V, double* partial, double* inv, hipStream_t st) {
+void batch_normalize(const BatchView& B, Zd* V, double* partial, double* inv, double* norm2, hipStream_t st) {
     if (!B.n_tiles) return;
-    hipLaunchKernelGGL(k_batch_norm_partial, dim3(B.n_tiles), dim3(256), 0, st, B, V, partial);
-    hipLaunchKernelGGL(k_batch_norm_finish, dim3(B.n_blocks), dim3(64), 0, st, B, partial, inv);
-    hipLaunchKernelGGL(k_batch_scale, dim3(B.n_tiles), dim3(256), 0, st, B, V, inv);
+    hipLaunchKernelGGL(k_batch_norm_partial, dim3(B.n_tiles), dim3(kThreads), 0, st, B, V, partial);
+    hipLaunchKernelGGL(k_batch_norm_finish, dim3(B.n_blocks), dim3(64), 0, st, B, partial, inv, norm2);
+    hipLaunchKernelGGL(k_batch_scale, dim3(B.n_tiles), dim3(kThreads), 0, st, B, V, inv);
+}
+
+void batch_place(Zd* V, int R, u64 row0, int s, const Zd* src, u64 n, hipStream_t st) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_batch_place, dim3(grid_for(n)), dim3(256), 0, st, V, R, row0, s, src, n);
 }
 
 void batch_advance(const BatchView& B, hipStream_t st) {
@@ -253,9 +331,7 @@ void batch_advance(const BatchView& B, hipStream_t st) {
 
 void batch_fill_zero(Zd* x, u64 n, hipStream_t st) {
     if (!n) return;
-    u64 g = (n + 255) / 256;
-    if (g > 65536) g = 65536;
-    hipLaunchKernelGGL(k_fill_zd, dim3(static_cast<unsigned>(g)), dim3(256), 0, st, x, n);
+    hipLaunchKernelGGL(k_fill_zd, dim3(grid_for(n)), dim3(256), 0, st, x, n);
 }
 
 void pack_csr(const CsrView& A, u64 row_off, u64 col_off, u64 nnz_off, u64* row_ptr, u32* col, Zd* val,

@@ -68,6 +68,8 @@ struct CsrView {
     int lanes = 16;                // lanes per row: 4, 8, 16, 32 or 64
     bool values_real = false;      // the stored values are real (full values stored as double)
     int nt = 0;                    // stream the column / id arrays with non-temporal loads
+    int xcd = 0;                   // contiguous row ranges per XCD (grid padded to a multiple of 8)
+    int unroll = 1;                // single-vector SpMV: gathers issued 4 entries at a time (same sums)
 };
 /// out_v = A in_v (real: double vectors; else complex vectors, with real or complex values).
 void csr_spmv(const CsrView& A, bool real, const VecSet& v, hipStream_t st);
@@ -136,12 +138,18 @@ struct BatchView {
 constexpr int kBatchTileRows = 256;
 constexpr int kBatchMaxR = 64;
 void batch_spmv(const BatchView& B, const Zd* X, Zd* Y, hipStream_t st);
+/// mode 0: W -= beta V_prev, <V_cur, W>; 1: W -= alpha V_cur, <V_cur, W>; 4: W -= alpha V_cur,
+/// ||W||^2; 2: local reorthogonalisation, ||W||^2; 3: W *= 1 / beta. Partials per tile and vector.
 void batch_vec(const BatchView& B, int mode, const Zd* Vprev, const Zd* Vcur, Zd* W, const double* coef,
                const Zd* zcoef, double threshold, double* partial, hipStream_t st);
-void batch_finish(const BatchView& B, int kind, int step, const double* partial, double* alpha_hist,
-                  double* beta_hist, Zd* ov, double* cur_alpha, double* cur_beta, double* inv_beta,
-                  const double* breakdown_tol, hipStream_t st);
-void batch_normalize(const BatchView& B, Zd* V, double* partial, double* inv, hipStream_t st);
+/// kind 0: alpha; 1: the overlap; 2: beta (ends breaking-down or capped pairs). The step is read
+/// from B.step on the device.
+void batch_finish(const BatchView& B, int kind, const double* partial, double* alpha_hist, double* beta_hist, Zd* ov,
+                  double* cur_alpha, double* cur_beta, double* inv_beta, const double* breakdown_tol, hipStream_t st);
+/// V scaled to unit norm per (block, vector); inv = 1 / norm, norm2 (may be null) = norm^2.
+void batch_normalize(const BatchView& B, Zd* V, double* partial, double* inv, double* norm2, hipStream_t st);
+/// V[(row0 + i) R + s] = src[i] for i < n.
+void batch_place(Zd* V, int R, u64 row0, int s, const Zd* src, u64 n, hipStream_t st);
 void batch_fill_zero(Zd* x, u64 n, hipStream_t st);
 void pack_csr(const CsrView& A, u64 row_off, u64 col_off, u64 nnz_off, u64* row_ptr, u32* col, Zd* val,
               hipStream_t st);

@@ -38,30 +38,73 @@ __device__ __forceinline__ void value_at(const CsrView& A, u64 e, double& vr, do
     }
 }
 
-template <int WIDTH, bool VREAL, bool REAL, int L, int NV>
+// The workgroup's place in the row order. With A.xcd the grid is a multiple of 8 and the
+// workgroups the dispatcher sends to one XCD (blockIdx = x mod 8, round robin) take one contiguous
+// eighth of the rows, so each XCD's L2 sees the columns of neighbouring rows together.
+__device__ __forceinline__ i64 row_block(const CsrView& A) {
+    if (!A.xcd) return blockIdx.x;
+    const u32 per = gridDim.x >> 3;
+    return static_cast<i64>(blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+}
+
+inline unsigned grid_blocks(const CsrView& A, u64 threads) {
+    u64 b = (threads + 255) / 256;
+    if (A.xcd) b = (b + 7) / 8 * 8;
+    return static_cast<unsigned>(b);
+}
+
+template <int WIDTH, bool VREAL, bool REAL, int L, int NV, int U>
 __global__ __launch_bounds__(256) void k_spmv(CsrView A, Ptrs<NV> v) {
-    const i64 gid = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const i64 gid = row_block(A) * blockDim.x + threadIdx.x;
     const i64 row = gid / L;
     const int lane = static_cast<int>(gid % L);
     double ar[NV], ai[NV];
 #pragma unroll
     for (int k = 0; k < NV; ++k) ar[k] = ai[k] = 0.0;
+    auto accumulate = [&](u32 c, double vr, double vi) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            if constexpr (REAL) {
+                ar[k] += vr * static_cast<const double*>(v.in[k])[c];
+            } else {
+                const Zd x = static_cast<const Zd*>(v.in[k])[c];
+                ar[k] += vr * x.re - vi * x.im;
+                ai[k] += vr * x.im + vi * x.re;
+            }
+        }
+    };
     if (row < A.rows) {
         const u64 e0 = A.row_ptr[row], e1 = A.row_ptr[row + 1];
-        for (u64 e = e0 + lane; e < e1; e += L) {
+        u64 e = e0 + lane;
+        if constexpr (U > 1) {
+            // U of the lane's entries per round: their columns and values first, then the U gathers
+            // issued together; the sums are taken in entry order, as in the one-at-a-time loop
+            for (; e + static_cast<u64>(U - 1) * L < e1; e += static_cast<u64>(U) * L) {
+                u32 c[U];
+                double vr[U], vi[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    c[u] = A.nt ? __builtin_nontemporal_load(A.col + e + u * L) : A.col[e + u * L];
+                    vi[u] = 0.0;
+                    value_at<WIDTH, VREAL>(A, e + u * L, vr[u], vi[u]);
+                }
+                if constexpr (REAL && NV == 1) {
+                    double x[U];
+#pragma unroll
+                    for (int u = 0; u < U; ++u) x[u] = static_cast<const double*>(v.in[0])[c[u]];
+#pragma unroll
+                    for (int u = 0; u < U; ++u) ar[0] += vr[u] * x[u];
+                } else {
+#pragma unroll
+                    for (int u = 0; u < U; ++u) accumulate(c[u], vr[u], vi[u]);
+                }
+            }
+        }
+        for (; e < e1; e += L) {
             const u32 c = A.nt ? __builtin_nontemporal_load(A.col + e) : A.col[e];
             double vr, vi = 0.0;
             value_at<WIDTH, VREAL>(A, e, vr, vi);
-#pragma unroll
-            for (int k = 0; k < NV; ++k) {
-                if constexpr (REAL) {
-                    ar[k] += vr * static_cast<const double*>(v.in[k])[c];
-                } else {
-                    const Zd x = static_cast<const Zd*>(v.in[k])[c];
-                    ar[k] += vr * x.re - vi * x.im;
-                    ai[k] += vr * x.im + vi * x.re;
-                }
-            }
+            accumulate(c, vr, vi);
         }
     }
 #pragma unroll
@@ -84,7 +127,7 @@ __global__ __launch_bounds__(256) void k_spmv(CsrView A, Ptrs<NV> v) {
 // Block-interleaved vectors: X[row * NV + k].
 template <int WIDTH, bool VREAL, bool REAL, int L, int NV>
 __global__ __launch_bounds__(256) void k_spmv_block(CsrView A, const void* X, void* Y) {
-    const i64 gid = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const i64 gid = row_block(A) * blockDim.x + threadIdx.x;
     const i64 row = gid / L;
     const int lane = static_cast<int>(gid % L);
     double ar[NV], ai[NV];
@@ -127,7 +170,7 @@ __global__ __launch_bounds__(256) void k_spmv_block(CsrView A, const void* X, vo
 
 template <int WIDTH, bool VREAL, bool REAL, int L> void launch_set(const CsrView& A, const VecSet& v, hipStream_t st) {
     const u64 threads = static_cast<u64>(A.rows) * L;
-    const dim3 grid(static_cast<unsigned>((threads + 255) / 256)), block(256);
+    const dim3 grid(grid_blocks(A, threads)), block(256);
     int done = 0;
     while (done < v.k) {
         const int rest = v.k - done;
@@ -139,7 +182,8 @@ template <int WIDTH, bool VREAL, bool REAL, int L> void launch_set(const CsrView
                 p.in[k] = v.in[done + k];
                 p.out[k] = v.out[done + k];
             }
-            hipLaunchKernelGGL((k_spmv<WIDTH, VREAL, REAL, L, NV>), grid, block, 0, st, A, p);
+            if (NV == 1 && A.unroll > 1) hipLaunchKernelGGL((k_spmv<WIDTH, VREAL, REAL, L, NV, 4>), grid, block, 0, st, A, p);
+            else hipLaunchKernelGGL((k_spmv<WIDTH, VREAL, REAL, L, NV, 1>), grid, block, 0, st, A, p);
         };
         if (nv == 8) go(std::integral_constant<int, 8>{});
         else if (nv == 4) go(std::integral_constant<int, 4>{});
@@ -162,7 +206,7 @@ template <int WIDTH, bool VREAL, bool REAL> void pick_lanes(const CsrView& A, co
 template <int WIDTH, bool VREAL, bool REAL, int L>
 void launch_block(const CsrView& A, int nv, const void* X, void* Y, hipStream_t st) {
     const u64 threads = static_cast<u64>(A.rows) * L;
-    const dim3 grid(static_cast<unsigned>((threads + 255) / 256)), block(256);
+    const dim3 grid(grid_blocks(A, threads)), block(256);
     switch (nv) {
     case 1: hipLaunchKernelGGL((k_spmv_block<WIDTH, VREAL, REAL, L, 1>), grid, block, 0, st, A, X, Y); break;
     case 2: hipLaunchKernelGGL((k_spmv_block<WIDTH, VREAL, REAL, L, 2>), grid, block, 0, st, A, X, Y); break;

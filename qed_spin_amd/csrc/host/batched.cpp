@@ -149,13 +149,36 @@ dev::BatchView BatchedLanczos::view() const {
 }
 
 void BatchedLanczos::set_start(const cplx* host) {
+    QSA_HIP(hipMemcpy(V0_.get(), host, rows_ * R_ * sizeof(dev::Zd), hipMemcpyHostToDevice));
+    normalize(nullptr);
+}
+
+void BatchedLanczos::clear_start() {
+    dev::batch_fill_zero(V0_.get<dev::Zd>(), rows_ * R_, nullptr);
+    QSA_HIP(hipGetLastError());
+}
+
+void BatchedLanczos::place(int b, int s, const cplx* device_vector) {
+    if (b < 0 || b >= n_blocks() || s < 0 || s >= R_) throw Error(kInvalid, "batched lanczos: no such pair");
+    dev::batch_place(V0_.get<dev::Zd>(), R_, row_off_[b], s, reinterpret_cast<const dev::Zd*>(device_vector), dim(b),
+                     nullptr);
+    QSA_HIP(hipGetLastError());
+}
+
+void BatchedLanczos::normalize(std::vector<double>* norm2) {
     const hipStream_t st = nullptr;
-    QSA_HIP(hipMemcpyAsync(V0_.get(), host, rows_ * R_ * sizeof(dev::Zd), hipMemcpyHostToDevice, st));
     dev::BatchView B = view();
     B.active = nullptr;
-    dev::batch_normalize(B, V0_.get<dev::Zd>(), partial_.get<double>(), inv_b_.get<double>(), st);
+    const std::size_t pairs = ops_.size() * static_cast<std::size_t>(R_);
+    DeviceBuffer n2(norm2 ? pairs * 8 : 0);
+    dev::batch_normalize(B, V0_.get<dev::Zd>(), partial_.get<double>(), inv_b_.get<double>(),
+                         norm2 ? n2.get<double>() : nullptr, st);
     QSA_HIP(hipGetLastError());
     QSA_HIP(hipStreamSynchronize(st));
+    if (norm2) {
+        norm2->resize(pairs);
+        QSA_HIP(hipMemcpy(norm2->data(), n2.get(), pairs * 8, hipMemcpyDeviceToHost));
+    }
 }
 
 void BatchedLanczos::apply(const cplx* X, cplx* Y) const {
@@ -165,8 +188,9 @@ void BatchedLanczos::apply(const cplx* X, cplx* Y) const {
     QSA_HIP(hipGetLastError());
 }
 
-void BatchedLanczos::run(int M, const std::vector<double>& breakdown_tol, bool use_graph, std::vector<double>& alpha,
-                         std::vector<double>& beta, std::vector<int>& steps, BatchedTimings& t) {
+void BatchedLanczos::run(int M, const std::vector<double>& breakdown_tol, bool local, bool use_graph,
+                         std::vector<double>& alpha, std::vector<double>& beta, std::vector<int>& steps,
+                         BatchedTimings& t, const std::vector<int>* live) {
     if (M < 1) throw Error(kInvalid, "batched lanczos: at least one step");
     if (breakdown_tol.size() != ops_.size()) throw Error(kInvalid, "batched lanczos: one breakdown tolerance per block");
     const std::size_t nb = ops_.size(), pairs = nb * static_cast<std::size_t>(R_);
@@ -175,6 +199,10 @@ void BatchedLanczos::run(int M, const std::vector<double>& breakdown_tol, bool u
     const hipStream_t st = own.s;
     QSA_HIP(hipStreamSynchronize(nullptr));
     std::vector<int> cap(nb), ones(pairs, 1), zeros(pairs, 0);
+    if (live) {
+        if (live->size() != pairs) throw Error(kInvalid, "batched lanczos: one live flag per pair");
+        ones = *live;
+    }
     for (std::size_t b = 0; b < nb; ++b) cap[b] = static_cast<int>(std::min<std::uint64_t>(M, dim(static_cast<int>(b))));
     put(cap_d_, cap, st);
     put(tol_d_, breakdown_tol, st);
@@ -205,11 +233,15 @@ void BatchedLanczos::run(int M, const std::vector<double>& breakdown_tol, bool u
         dev::Zd* w = V[(phase + 2) % 3];
         dev::batch_spmv(B, c, w, st);
         dev::batch_vec(B, 0, p, c, w, cb, nullptr, 0.0, partial, st);
-        dev::batch_finish(B, 0, -1, partial, ah, bh, ov, ca, cb, ib, tol, st);
-        dev::batch_vec(B, 1, nullptr, c, w, ca, nullptr, 0.0, partial, st);
-        dev::batch_finish(B, 1, -1, partial, ah, bh, ov, ca, cb, ib, tol, st);
-        dev::batch_vec(B, 2, nullptr, c, w, nullptr, ov, thr, partial, st);
-        dev::batch_finish(B, 2, -1, partial, ah, bh, ov, ca, cb, ib, tol, st);
+        dev::batch_finish(B, 0, partial, ah, bh, ov, ca, cb, ib, tol, st);
+        if (local) {
+            dev::batch_vec(B, 1, nullptr, c, w, ca, nullptr, 0.0, partial, st);
+            dev::batch_finish(B, 1, partial, ah, bh, ov, ca, cb, ib, tol, st);
+            dev::batch_vec(B, 2, nullptr, c, w, nullptr, ov, thr, partial, st);
+        } else {
+            dev::batch_vec(B, 4, nullptr, c, w, ca, nullptr, 0.0, partial, st);
+        }
+        dev::batch_finish(B, 2, partial, ah, bh, ov, ca, cb, ib, tol, st);
         dev::batch_vec(B, 3, nullptr, nullptr, w, ib, nullptr, 0.0, partial, st);
         dev::batch_advance(B, st);
     };
@@ -298,7 +330,7 @@ std::vector<Curves> batched_ftlm(const std::vector<std::shared_ptr<const SectorO
         });
         L.set_start(host.data());
         t.seed_s += secs(t_seed, clock_t_::now());
-        L.run(o.krylov, o.breakdown_tol, o.use_graph, alpha, beta, steps, t);
+        L.run(o.krylov, o.breakdown_tol, /*local=*/true, o.use_graph, alpha, beta, steps, t);
         const auto t_post = clock_t_::now();
         const std::size_t pairs = nb * static_cast<std::size_t>(R);
         std::vector<SampleMoments> round(nb * static_cast<std::size_t>(Rc));

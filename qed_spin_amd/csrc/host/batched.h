@@ -35,11 +35,18 @@ public:
     /// Start vectors from the host, block b sample s at host[(row_offset(b) + i) R + s] (the batch
     /// layout); normalised on the device.
     void set_start(const cplx* host_interleaved);
-    /// M Lanczos steps (LocalDGKS3 with K = 1, no basis) with breakdown_tol[b] per block; the
-    /// step is captured as a hipGraph when use_graph. Returns alpha[M][B][R], beta[M][B][R]
-    /// (beta of step j is beta_{j+1}) and steps[B][R].
-    void run(int M, const std::vector<double>& breakdown_tol, bool use_graph, std::vector<double>& alpha,
-             std::vector<double>& beta, std::vector<int>& steps, BatchedTimings& t);
+    /// Start vectors from the device: clear_start(), then place() each vector (dim(b) entries of
+    /// a device vector), then normalize(); norm2[b R + s] gets each vector's squared norm.
+    void clear_start();
+    void place(int b, int s, const cplx* device_vector);
+    void normalize(std::vector<double>* norm2);
+    /// M Lanczos steps without a kept basis, with breakdown_tol[b] per block: local = LocalDGKS3
+    /// with K = 1 (FTLM), else no reorthogonalisation (the continued fraction). The step is captured
+    /// as a hipGraph when use_graph. live (optional, B x R): the pairs to run. Returns
+    /// alpha[M][B][R], beta[M][B][R] (beta of step j is beta_{j+1}) and steps[B][R].
+    void run(int M, const std::vector<double>& breakdown_tol, bool local, bool use_graph, std::vector<double>& alpha,
+             std::vector<double>& beta, std::vector<int>& steps, BatchedTimings& t,
+             const std::vector<int>* live = nullptr);
     /// Y = H X on batch vectors (device pointers, rows() x R complex each).
     void apply(const cplx* X, cplx* Y) const;
 

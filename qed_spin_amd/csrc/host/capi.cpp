@@ -427,6 +427,29 @@ int qedh_block_lowest(qedh_matvec m, int k, int64_t max_iter, double* values, in
     });
 }
 
+int qedh_block_eigenpairs(qedh_matvec m, int k, int64_t max_iter, double* values, void* vectors, int* n_values,
+                          int* converged) {
+    return guarded([&] {
+        need(m, "matvec");
+        need(values, "values");
+        need(vectors, "vectors");
+        const auto& op = *m->op;
+        if (!op.same()) throw qsa::Error(qsa::kInvalid, "eigenpairs: the operator maps two sectors");
+        const std::size_t n = static_cast<std::size_t>(op.dim());
+        if (k < 1) throw qsa::Error(qsa::kInvalid, "eigenpairs: k must be >= 1");
+        const std::size_t kk = std::min<std::size_t>(static_cast<std::size_t>(k), n);
+        qsa::HipBackend<qsa::cplx> be;
+        auto H = [&](const qsa::cplx* in, qsa::cplx* out) { op.apply(in, out, true); };
+        const qsa::BlockSolution sol =
+            qsa::krylov_schur_lane(be, H, n, kk, op.norm_bound(), max_iter > 0 ? static_cast<std::uint64_t>(max_iter) : 0,
+                                   static_cast<qsa::cplx*>(vectors));
+        be.synchronize();
+        *n_values = static_cast<int>(sol.values.size());
+        for (std::size_t i = 0; i < sol.values.size(); ++i) values[i] = sol.values[i];
+        if (converged) *converged = sol.converged ? 1 : 0;
+    });
+}
+
 int qedh_ftlm(qedh_matvec m, int samples, int krylov, uint64_t seed, int n_beta, const double* betas,
               double breakdown_tol, double* lnZ, double* E, double* V, double* e_min) {
     return guarded([&] {

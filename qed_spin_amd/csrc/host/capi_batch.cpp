@@ -50,8 +50,8 @@ std::vector<std::shared_ptr<const qsa::SectorOperator>> gather_blocks(const qedh
 }  // namespace
 
 extern "C" int qedh_batch_lanczos(const qedh_matvec* blocks, int n_blocks, int R, int krylov, const double* start,
-                                  const double* breakdown_tol, int use_graph, double* alpha, double* beta, int* steps,
-                                  double* timings) {
+                                  const double* breakdown_tol, int reorth, int use_graph, double* alpha, double* beta,
+                                  int* steps, double* timings) {
     return guarded_batch([&] {
         if (!start || !breakdown_tol || !alpha || !beta || !steps)
             throw qsa::Error(qsa::kInvalid, "qedh_batch_lanczos: null argument");
@@ -63,7 +63,8 @@ extern "C" int qedh_batch_lanczos(const qedh_matvec* blocks, int n_blocks, int R
         L.set_start(reinterpret_cast<const qsa::cplx*>(start));
         std::vector<double> a, b;
         std::vector<int> st;
-        L.run(krylov, std::vector<double>(breakdown_tol, breakdown_tol + n_blocks), use_graph != 0, a, b, st, t);
+        L.run(krylov, std::vector<double>(breakdown_tol, breakdown_tol + n_blocks), reorth != 0, use_graph != 0, a, b, st,
+              t);
         std::copy(a.begin(), a.end(), alpha);
         std::copy(b.begin(), b.end(), beta);
         std::copy(st.begin(), st.end(), steps);
@@ -191,6 +192,74 @@ extern "C" int qedh_thermal_combine(int n_blocks, int n_beta, const double* beta
             out_C[i] = s.C[i];
             out_S[i] = s.S[i];
             out_F[i] = s.F[i];
+        }
+    });
+}
+
+/* Continued fractions of many (block, vector) pairs at once (cf_spectral_from_vector,
+ * include/ed/dynamics/cf.h:103-161, for every probe image the dynamics verb forms,
+ * src/engine/dynamics.cpp:528-560): starts[b R + s] is a DEVICE vector of block b's dimension (or
+ * NULL: no pair); each is normalised, run through Lanczos without reorthogonalisation in the
+ * batched engine, and its spectral function S = -Im G / pi, G the continued fraction of alpha -
+ * shift, beta, weight ||phi||^2, evaluated on the host. */
+extern "C" int qedh_cf_batched(const qedh_matvec* blocks, int n_blocks, int R, const void* const* starts, int krylov,
+                               int has_shift, double energy_shift, double breakdown_tol, int n_omega,
+                               const double* omega, double eta, int use_graph, double* S, double* norm_sq,
+                               double* shifts, int* steps_out, double* stats) {
+    return guarded_batch([&] {
+        if (!starts || !omega || !S || n_omega < 1 || krylov < 1)
+            throw qsa::Error(qsa::kInvalid, "qedh_cf_batched: bad arguments");
+        const auto ops = gather_blocks(blocks, n_blocks);
+        const auto t0 = std::chrono::steady_clock::now();
+        qsa::BatchedTimings t;
+        qsa::BatchedLanczos L(ops, R);
+        L.clear_start();
+        const std::size_t pairs = static_cast<std::size_t>(n_blocks) * static_cast<std::size_t>(R);
+        for (int b = 0; b < n_blocks; ++b)
+            for (int s = 0; s < R; ++s)
+                if (starts[b * R + s]) L.place(b, s, static_cast<const qsa::cplx*>(starts[b * R + s]));
+        std::vector<double> n2;
+        L.normalize(&n2);
+        std::vector<int> live(pairs, 0);
+        for (std::size_t i = 0; i < pairs; ++i) live[i] = (starts[i] && std::sqrt(n2[i]) >= 1e-14) ? 1 : 0;
+        t.pack_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        std::vector<double> alpha, beta;
+        std::vector<int> steps;
+        const double tol = breakdown_tol > 0.0 ? breakdown_tol : 1e-12;
+        L.run(krylov, std::vector<double>(static_cast<std::size_t>(n_blocks), tol), /*local=*/false, use_graph != 0,
+              alpha, beta, steps, t, &live);
+        const auto t1 = std::chrono::steady_clock::now();
+        const std::vector<double> w(omega, omega + n_omega);
+        qsa::parallel_blocks(pairs, [&](std::size_t i) {
+            double* out = S + i * static_cast<std::size_t>(n_omega);
+            if (norm_sq) norm_sq[i] = live[i] ? n2[i] : 0.0;
+            if (steps_out) steps_out[i] = live[i] ? steps[i] : 0;
+            if (!live[i]) {
+                std::fill(out, out + n_omega, 0.0);
+                if (shifts) shifts[i] = 0.0;
+                return;
+            }
+            const int m = steps[i];
+            std::vector<double> a(static_cast<std::size_t>(m)), bfull(static_cast<std::size_t>(m) + 1, 0.0);
+            for (int j = 0; j < m; ++j) {
+                a[j] = alpha[static_cast<std::size_t>(j) * pairs + i];
+                bfull[j + 1] = beta[static_cast<std::size_t>(j) * pairs + i];
+            }
+            double E = energy_shift;
+            if (!has_shift) E = qsa::tridiag_eig(a, bfull, a.size(), false).values.front();
+            for (auto& x : a) x -= E;
+            if (shifts) shifts[i] = E;
+            const auto Sv = qsa::continued_fraction(a, bfull, w, eta, n2[i]);
+            std::copy(Sv.begin(), Sv.end(), out);
+        });
+        if (stats) {
+            stats[0] = t.pack_s;
+            stats[1] = t.steps_s;
+            stats[2] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
+            stats[3] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            stats[4] = static_cast<double>(t.applies);
+            stats[5] = static_cast<double>(t.rows);
+            stats[6] = static_cast<double>(t.nnz);
         }
     });
 }

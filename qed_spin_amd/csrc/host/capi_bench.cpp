@@ -89,13 +89,15 @@ extern "C" int qedh_matvec_export_csr_rows(qedh_matvec m, int64_t r0, int64_t r1
     }
 }
 
-extern "C" int qedh_matvec_set_tuning(qedh_matvec m, int lanes, int nontemporal) {
+extern "C" int qedh_matvec_set_tuning(qedh_matvec m, int lanes, int flags) {
     try {
         if (!m) throw qsa::Error(qsa::kInvalid, "matvec is null");
         if (lanes != 0 && lanes != 4 && lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64)
             throw qsa::Error(qsa::kInvalid, "lanes must be 0 (keep), 4, 8, 16, 32 or 64");
         if (lanes) m->op->set_lanes(lanes);
-        m->op->set_nontemporal(nontemporal != 0);
+        m->op->set_nontemporal((flags & QEDH_TUNE_NONTEMPORAL) != 0);
+        m->op->set_xcd_rows((flags & QEDH_TUNE_XCD_ROWS) != 0);
+        m->op->set_unroll((flags & QEDH_TUNE_UNROLL) != 0);
         return 0;
     } catch (const qsa::Error& e) {
         qsa::capi_set_error(e.what());

@@ -151,7 +151,7 @@ struct KsResult {
 /// Thick-restart Krylov-Schur with locking, fresh deflated starts and the degeneracy probe.
 template <class Scalar, class Matvec>
 KsResult krylov_schur(const HipBackend<Scalar>& be, Matvec&& matvec, std::size_t n, const Scalar* seed,
-                      const KsOptions& opts) {
+                      const KsOptions& opts, Scalar* vectors_out = nullptr, std::size_t vectors_cap = 0) {
     KsResult R;
     if (n == 0) {
         R.converged = R.exhausted = true;
@@ -330,8 +330,15 @@ KsResult krylov_schur(const HipBackend<Scalar>& be, Matvec&& matvec, std::size_t
             }
         }
     }
-    R.eigenvalues = found_vals;
-    std::sort(R.eigenvalues.begin(), R.eigenvalues.end());
+    // ascending, the locked vectors in the same order
+    std::vector<std::size_t> order(found_vals.size());
+    for (std::size_t i = 0; i < order.size(); ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](std::size_t a, std::size_t b) { return found_vals[a] < found_vals[b]; });
+    R.eigenvalues.resize(order.size());
+    for (std::size_t i = 0; i < order.size(); ++i) R.eigenvalues[i] = found_vals[order[i]];
+    if (vectors_out)
+        for (std::size_t i = 0; i < order.size() && i < vectors_cap; ++i)
+            be.copy(Fown.get() + order[i] * n, vectors_out + i * n, n);
     R.converged = converged;
     R.exhausted = exhausted;
     return R;
@@ -342,7 +349,7 @@ KsResult krylov_schur(const HipBackend<Scalar>& be, Matvec&& matvec, std::size_t
 /// tolerance kLockRel * s_H, breakdown kBreakdownRel * s_H.
 template <class Scalar, class Matvec>
 BlockSolution krylov_schur_lane(const HipBackend<Scalar>& be, Matvec&& H, std::size_t n, std::size_t k,
-                                double norm_bound, std::uint64_t max_iter) {
+                                double norm_bound, std::uint64_t max_iter, Scalar* vectors_out = nullptr) {
     const std::uint64_t budget = max_iter > 0 ? max_iter : std::max<std::uint64_t>(200u * k, 2000u);
     const std::size_t per_cycle =
         std::min<std::size_t>(krylov_subspace_dim(k, 2 * k + 60, n, 0), static_cast<std::size_t>(std::max<std::uint64_t>(budget, k + 1)));
@@ -360,7 +367,7 @@ BlockSolution krylov_schur_lane(const HipBackend<Scalar>& be, Matvec&& H, std::s
     const auto h = gaussian_start<Scalar>(n, kKsStartSeed);
     auto v0 = be.make_zero_vector(n);
     be.copy_from_host(h.data(), v0.get(), n);
-    const KsResult r = krylov_schur(be, H, n, v0.get(), o);
+    const KsResult r = krylov_schur(be, H, n, v0.get(), o, vectors_out, k);
     BlockSolution sol;
     for (std::size_t i = 0; i < r.eigenvalues.size() && sol.values.size() < k; ++i) sol.values.push_back(r.eigenvalues[i]);
     sol.whole = r.exhausted;

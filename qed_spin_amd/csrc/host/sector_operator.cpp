@@ -184,6 +184,8 @@ dev::CsrView SectorOperator::csr_view(bool block) const {
     A.lanes = block ? lanes_block_ : csr_.lanes;
     A.values_real = real_;
     A.nt = nt_ ? 1 : 0;
+    A.xcd = xcd_ ? 1 : 0;
+    A.unroll = unroll_ ? 4 : 1;
     return A;
 }
 

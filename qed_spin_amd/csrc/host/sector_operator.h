@@ -70,6 +70,11 @@ public:
     /// Stream the CSR's column / id arrays with non-temporal loads (they are read once per apply).
     void set_nontemporal(bool on) { nt_ = on; }
     bool nontemporal() const { return nt_; }
+    /// Give each XCD one contiguous range of rows (workgroup order remapped).
+    void set_xcd_rows(bool on) { xcd_ = on; }
+    /// Issue the single-vector SpMV's gathers four entries at a time (the sums are unchanged).
+    void set_unroll(bool on) { unroll_ = on; }
+    bool xcd_rows() const { return xcd_; }
 
 private:
     void build_csr(std::uint64_t budget);
@@ -82,6 +87,8 @@ private:
     bool real_;
     int lane_ = kLaneGather;
     bool nt_ = false;
+    bool xcd_ = false;
+    bool unroll_ = false;
     CsrInfo csr_;
     DeviceBuffer row_ptr_, col_, vals_, dict_re_, dict_im_;
 };

@@ -1,0 +1,209 @@
+"""Dynamical correlation functions S_O(omega) at T = 0 (the reference's qed.dynamics /
+ed::sectors::dynamics, src/engine/dynamics.cpp:282-690).
+
+The ground manifold is every level within degeneracy_tol * s_H of the lowest level E0 over the
+blocks of the walk (include/ed/sectors/dynamics.h:17-42). For every ground state |g> (block b) and
+probe O, every block t the probe reaches gets phi = O|g> through the cross-sector operator
+(CrossSectorMatVec, src/engine/internal.h:552-609, on the device), and each phi one continued
+fraction on H of block t with the shift E0 and weight ||phi||^2 (cf.h:103-161). All (state, probe,
+target) continued fractions run together in the batched engine (qedh_cf_batched): the targets'
+CSRs packed into one block-diagonal matrix, the probe images as its right-hand sides, one
+multi-vector matvec per Lanczos step replayed as a hipGraph. S_O is the sum over targets, averaged
+over the ground manifold (dynamics.cpp:680-684).
+"""
+
+from __future__ import annotations
+
+import cmath
+import ctypes as C
+import math
+import time
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from .errors import DeviceUnsupported, InvalidRequest
+from .operator import Operator, Program
+from .sector import DEFAULT_CSR_BUDGET, LANE_GATHER, Sector, SectorMatvec
+from .symmetry import Symmetry
+from .thermal import KBREAKDOWN_REL, _handles, resolve_device, walk_blocks
+
+
+def sz_q(N: int, q: float, sign: int = -1) -> Operator:
+    """S^z(q) = N^-1/2 sum_j e^{sign i q j} S^z_j."""
+    op = Operator(N)
+    for j in range(N):
+        op.add_product("z", (j,), cmath.exp(sign * 1j * q * j) / math.sqrt(N))
+    return op
+
+
+def _delta_up(O: Operator) -> int:
+    """The change of up spins every product of O makes (one value, or the request is refused)."""
+    ds = set()
+    for ops, _, _ in O.products:
+        d = 0
+        for o in ops:
+            d += {"+": 1, "-": -1}.get(o, 0)
+            if o in "xy":
+                raise DeviceUnsupported("dynamics: a probe must change Sz by one definite amount (use S+, S-, Sz)")
+        ds.add(d)
+    if len(ds) > 1:
+        raise DeviceUnsupported("dynamics: a probe must change Sz by one definite amount")
+    return ds.pop() if ds else 0
+
+
+class _DevView:
+    """A device pointer into a larger allocation (no ownership)."""
+
+    def __init__(self, base: int, offset_bytes: int):
+        self.ptr = C.c_void_p(base + offset_bytes)
+
+
+@dataclass
+class DynamicsResult:
+    omega: np.ndarray
+    S: List[np.ndarray]            # one spectral function per probe
+    e0: float
+    ground_manifold: int
+    target_sectors: int
+    timings: Dict[str, float] = field(default_factory=dict)
+    engine: Dict[str, float] = field(default_factory=dict)
+
+
+def ground_manifold(blocks, window: float, k_max: int = 16):
+    """(E0, [(block index, device buffer, vector index, energy)]): every level within `window` of the
+    lowest level of the walk. Each block's lowest level comes from the k = 1 scan; the blocks within
+    reach of the lowest then give their eigenpairs (Krylov-Schur, k doubled until a level outside the
+    window shows or k_max), and E0 is the lowest eigenvalue among them."""
+    lows = []
+    for blk, sec, m in blocks:
+        vals, _, _ = m.lowest(1)
+        lows.append(vals[0] if len(vals) else math.inf)
+    e_scan = min(lows)
+    slack = window + 1e-9 * max(1.0, abs(e_scan))
+    found = []
+    for i, (blk, sec, m) in enumerate(blocks):
+        if lows[i] > e_scan + slack:
+            continue
+        k = 2
+        while True:
+            k = min(k, sec.dim)
+            buf = _lib_alloc(k * sec.dim * 16)
+            vals = np.zeros(k)
+            n, conv = C.c_int(), C.c_int()
+            _lib.call("qedh_block_eigenpairs", m.h, int(k), C.c_int64(0), _lib.ptr(vals, C.c_double), buf.ptr,
+                      C.byref(n), C.byref(conv))
+            got = vals[: n.value]
+            if len(got) == 0:
+                raise RuntimeError(f"dynamics: no eigenpair of block {blk.label}")
+            if got.max() > got.min() + slack or k >= sec.dim or k >= k_max:
+                break
+            k *= 2
+        found.append((i, buf, got))
+    e0 = min(float(g.min()) for _, _, g in found)
+    states = []
+    for i, buf, got in found:
+        for j, v in enumerate(got):
+            if v <= e0 + window:
+                states.append((i, buf, j, float(v)))
+    return e0, states
+
+
+class _lib_alloc:
+    def __init__(self, nbytes: int):
+        p = C.c_void_p()
+        _lib.call("qedh_malloc", C.byref(p), C.c_size_t(max(int(nbytes), 16)))
+        self.ptr = p
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and _lib._lib is not None:
+            _lib.lib().qedh_free(self.ptr)
+            self.ptr = None
+
+
+def dynamics(H: Operator, probes, omega, *, eta: float = 0.1, T=None, sym: Optional[Symmetry] = None,
+             krylov: int = 200, device: str = "gpu", degeneracy_tol: float = 1e-8, use_graph: bool = True,
+             csr_budget: int = DEFAULT_CSR_BUDGET, blocks=None) -> DynamicsResult:
+    """S_O(omega) = sum_n |<n|O|g>|^2 L_eta(omega - E_n + E0), averaged over the ground manifold,
+    for every probe O (an Operator or a list of them). blocks: the walk's (block, sector, matvec)
+    triples when the caller already built them (thermal.walk_blocks)."""
+    resolve_device(device, "dynamics")
+    if T is not None and len(np.atleast_1d(T)) > 0:
+        raise DeviceUnsupported("dynamics: T > 0 (FTLM dynamics) is not on the device path")
+    probes = [probes] if isinstance(probes, Operator) else list(probes)
+    if not probes:
+        raise InvalidRequest("dynamics: no probe")
+    omega = np.ascontiguousarray(omega, np.float64)
+    if omega.size == 0 or not eta > 0:
+        raise InvalidRequest("dynamics: an omega grid and eta > 0 are required")
+    sym = sym if sym is not None else Symmetry()
+    t0 = time.perf_counter()
+    if blocks is None:
+        blocks = walk_blocks(H, sym, csr_budget)
+    if not blocks:
+        raise InvalidRequest("dynamics: the symmetry selection holds no state")
+    t_walk = time.perf_counter() - t0
+    sH = H.norm_bound() or 1.0
+    window = degeneracy_tol * sH
+    t1 = time.perf_counter()
+    e0, states = ground_manifold(blocks, window)
+    t_gm = time.perf_counter() - t1
+    # probe images: (state, probe) -> every block it reaches
+    t2 = time.perf_counter()
+    deltas = [_delta_up(O) for O in probes]
+    images = []          # (target block index, probe index, device buffer of phi)
+    keep = []            # the cross operators and buffers stay alive until the batched run ends
+    for (b, buf, j, _) in states:
+        blk_b, sec_b, _ = blocks[b]
+        g = _DevView(buf.ptr.value, j * sec_b.dim * 16)
+        for p, O in enumerate(probes):
+            for t, (blk_t, sec_t, _) in enumerate(blocks):
+                if sec_b.n_up >= 0 and sec_t.n_up != sec_b.n_up + deltas[p]:
+                    continue
+                # The probe is not invariant under the group, so even within the source's own block it
+                # needs the lambda-projected program of two sectors: the target then gets a handle of
+                # its own (the same representatives, enumerated again).
+                tgt = sec_t if t != b else Sector(blk_t.group, blk_t.n_up)
+                X = SectorMatvec(O, tgt, sec_b, lane=LANE_GATHER)
+                if X.program.counts()[2] == 0:
+                    continue
+                phi = _lib_alloc(sec_t.dim * 16)
+                X.apply(g, phi, complex_vectors=True)
+                images.append((t, p, phi))
+                keep.append(X)
+    t_probe = time.perf_counter() - t2
+    S = [np.zeros(omega.size) for _ in probes]
+    engine = {}
+    targets = sorted({t for t, _, _ in images})
+    if images:
+        slot = {t: [] for t in targets}
+        for t, p, phi in images:
+            slot[t].append((p, phi))
+        R = max(len(v) for v in slot.values())
+        if R > 64:
+            raise InvalidRequest("dynamics: more than 64 probe images reach one block")
+        mats = [blocks[t][2] for t in targets]
+        starts = (C.c_void_p * (len(targets) * R))()
+        owner = []
+        for ti, t in enumerate(targets):
+            for s, (p, phi) in enumerate(slot[t]):
+                starts[ti * R + s] = phi.ptr.value
+                owner.append((ti * R + s, p))
+        out = np.zeros((len(targets) * R, omega.size))
+        nsq = np.zeros(len(targets) * R)
+        stats = np.zeros(7)
+        _lib.call("qedh_cf_batched", _handles(mats), len(targets), int(R), starts, int(max(krylov, 2)), 1,
+                  C.c_double(e0), C.c_double(KBREAKDOWN_REL * sH), int(omega.size), _lib.ptr(omega, C.c_double),
+                  C.c_double(eta), int(bool(use_graph)), _lib.ptr(out, C.c_double), _lib.ptr(nsq, C.c_double),
+                  None, None, _lib.ptr(stats, C.c_double))
+        for i, p in owner:
+            S[p] += out[i]
+        engine = dict(zip(["pack_s", "steps_s", "host_s", "total_s", "applies", "rows", "nnz"], stats.tolist()))
+    for p in range(len(probes)):
+        S[p] /= float(len(states))
+    return DynamicsResult(
+        omega=omega, S=S, e0=e0, ground_manifold=len(states), target_sectors=len(targets),
+        timings={"walk_s": t_walk, "ground_s": t_gm, "probe_s": t_probe, "total_s": time.perf_counter() - t0},
+        engine=engine)

@@ -54,9 +54,10 @@ def _handles(mats: Sequence[SectorMatvec]):
 
 
 def batch_lanczos(mats: Sequence[SectorMatvec], R: int, krylov: int, start: np.ndarray,
-                  breakdown_tol: Optional[Sequence[float]] = None, use_graph: bool = True):
-    """LocalDGKS3 Lanczos of R vectors per block, all blocks at once. start: complex (rows, R) with
-    the blocks' rows stacked in order. Returns alpha, beta (krylov, blocks, R), steps (blocks, R)."""
+                  breakdown_tol: Optional[Sequence[float]] = None, use_graph: bool = True, reorth: int = 2):
+    """Lanczos of R vectors per block, all blocks at once (reorth 2: LocalDGKS3, 0: none). start:
+    complex (rows, R) with the blocks' rows stacked in order. Returns alpha, beta (krylov, blocks, R),
+    steps (blocks, R)."""
     nb = len(mats)
     rows = sum(m.dim for m in mats)
     st = np.ascontiguousarray(np.asarray(start, np.complex128).reshape(rows, R))
@@ -69,7 +70,7 @@ def batch_lanczos(mats: Sequence[SectorMatvec], R: int, krylov: int, start: np.n
     t = np.zeros(3)
     _lib.call("qedh_batch_lanczos", _handles(mats), nb, int(R), int(krylov),
               st.view(np.float64).ctypes.data_as(C.POINTER(C.c_double)), _lib.ptr(tol, C.c_double),
-              int(bool(use_graph)), _lib.ptr(alpha, C.c_double), _lib.ptr(beta, C.c_double),
+              int(reorth), int(bool(use_graph)), _lib.ptr(alpha, C.c_double), _lib.ptr(beta, C.c_double),
               _lib.ptr(steps, C.c_int), _lib.ptr(t, C.c_double))
     return alpha, beta, steps, {"pack_s": t[0], "steps_s": t[1], "total_s": t[2]}
 

@@ -257,3 +257,87 @@ def test_thermal_mixed_walk_matches_oracle():
     assert np.allclose(res.lnZ, mx + np.log(z), rtol=1e-8, atol=1e-8)
     assert np.allclose(res.E, Et, rtol=1e-8, atol=1e-8)
     assert np.allclose(res.C, betas ** 2 * var, rtol=1e-8, atol=1e-8)
+
+
+def test_cross_sector_matvec_matches_dense_oracle():
+    """O from sector s to sector t (both lanes) against E_t^dagger O E_s of the dense oracle, for
+    S^z(q) (same n_up) and S^+(q) (one more up spin); N^-1/2 sum_j e^{-i q j} O_j takes momentum
+    index k to k - m (q = 2 pi m / N)."""
+    from qed_spin_amd import dynamics as dy
+
+    say("cross-sector matvec vs dense")
+    N = 10
+    states = np.arange(1 << N, dtype=np.uint64)
+    splus = qs.Operator(N)
+    for j in range(N):
+        splus.add_product("+", (j,), np.exp(-2j * np.pi * 3 * j / N) / np.sqrt(N))
+    cases = [(dy.sz_q(N, 2 * np.pi * 2 / N), 5, 1, 5, 9), (dy.sz_q(N, np.pi), 4, 0, 4, 5), (splus, 4, 2, 5, 9)]
+    rng = np.random.default_rng(9)
+    for O, nu_s, k_s, nu_t, k_t in cases:
+        ss = qs.Sector(sym.chain_group(N, k_s), nu_s)
+        st = qs.Sector(sym.chain_group(N, k_t), nu_t)
+        Es = po.sector_basis_matrix(po.OracleSector(N, nu_s, sym.translations(N), sym.momentum_chars(N, k_s)), states)
+        Et = po.sector_basis_matrix(po.OracleSector(N, nu_t, sym.translations(N), sym.momentum_chars(N, k_t)), states)
+        A = Et.conj().T @ po.dense_matrix(O.products, N, states) @ Es
+        x = rng.normal(size=ss.dim) + 1j * rng.normal(size=ss.dim)
+        want = A @ x
+        assert np.linalg.norm(want) > 1e-3
+        for lane in (qs.LANE_GATHER, qs.LANE_CSR):
+            X = qs.SectorMatvec(O, st, ss, lane=lane)
+            got = X.apply_host(x, complex_vectors=True)
+            assert np.linalg.norm(got - want) <= 1e-13 * np.linalg.norm(want), (nu_s, k_s, nu_t, k_t, lane)
+
+
+def _lehmann(Hd, O, omega, eta, window=1e-8):
+    """The T = 0 spectral function of a dense H (the oracle): ground manifold average of
+    sum_n |<n|O|g>|^2 eta / pi / ((omega - E_n + E0)^2 + eta^2)."""
+    ev, U = np.linalg.eigh(Hd)
+    e0 = ev[0]
+    g = [i for i in range(len(ev)) if ev[i] <= e0 + window]
+    S = np.zeros(len(omega))
+    for i in g:
+        amp = U.conj().T @ (O @ U[:, i])
+        w = np.abs(amp) ** 2
+        S += (w[None, :] * (eta / np.pi) / ((omega[:, None] - (ev[None, :] - e0)) ** 2 + eta ** 2)).sum(1)
+    return S / len(g), e0, len(g)
+
+
+def test_dynamics_T0_matches_dense_lehmann_sum():
+    """A complex (chiral) ring, every (Sz, k) block, S^z(q) probes: the batched continued fractions
+    against the Lehmann sum of the dense oracle (krylov above every target dimension)."""
+    from qed_spin_amd import dynamics as dy
+
+    say("dynamics T=0 vs dense Lehmann sum")
+    N = 10
+    H = chiral_j1j2(N, 0.25)
+    omega = np.linspace(-0.5, 4.0, 61)
+    eta = 0.1
+    probes = [dy.sz_q(N, 2 * np.pi * m / N) for m in (0, 2, 5)]
+    r = dy.dynamics(H, probes, omega, eta=eta, sym=qs.Symmetry(sz=[N // 2]), krylov=120)
+    states = np.array([x for x in range(1 << N) if bin(x).count("1") == N // 2], np.uint64)
+    Hd = po.dense_matrix(H.products, N, states)
+    for p, O in enumerate(probes):
+        Od = po.dense_matrix(O.products, N, states)
+        want, e0, ng = _lehmann(Hd, Od, omega, eta)
+        assert abs(r.e0 - e0) < 1e-10 and r.ground_manifold == ng
+        assert np.abs(r.S[p] - want).max() <= 1e-8 * max(1.0, np.abs(want).max()), p
+
+
+def test_dynamics_reproduces_reference_T0_records():
+    from qed_spin_amd import dynamics as dy
+
+    say("dynamics T=0 vs reference records")
+    rec = records()
+    dd = rec["dynamics"]
+    omega = np.array(dd["omega"])
+    N = 12
+    O = qs.Operator(N)
+    for i in range(N):
+        O.add_product("z", (i,), np.exp(1j * np.pi * i) / np.sqrt(N))
+    H = j1j2(N)
+    for label, symm in (("none", qs.Symmetry.none()), ("auto", qs.Symmetry())):
+        want = np.array(rec["records"][f"j1j2_chain12/api/dynamics/T=0/{label}/SzQ"]["values"]["S"])
+        r = dy.dynamics(H, O, omega, eta=dd["eta"], sym=symm, krylov=dd["krylov_T0"])
+        err = np.abs(r.S[0] - want).max()
+        print(f"  {label}: ground manifold {r.ground_manifold}, E0 {r.e0:.12f}, max |diff| {err:.3e}")
+        assert np.allclose(r.S[0], want, rtol=1e-8, atol=1e-8), label
