@@ -209,6 +209,42 @@ int qedh_thermal_combine(int n_blocks, int n_beta, const double* betas, const do
                          const double* E, const double* V, double* out_lnZ, double* out_E, double* out_C,
                          double* out_S, double* out_F);
 
+/* ---- Serving (no counterpart in the reference, which has no serving layer): a stream of solves
+ * on spin chains, admitted against an HBM budget and run concurrently on slots with a HIP stream
+ * each, sectors kept resident between requests ------------------------------------------------ */
+/* One request: the periodic chain of n_sites spins with J1 on (i, i+1), J2 on (i, i+2) and
+ * chi S_i . (S_{i+1} x S_{i+2}), in the sector n_up = n_sites / 2, momentum 0. kind 0: the k lowest
+ * levels; 1: FTLM (samples, depth, seed) at the options' temperatures; 2: S^z(q) for q the lattice
+ * momentum nearest pi from the ground state (continued fraction of depth krylov). */
+typedef struct {
+    int kind;
+    int n_sites;
+    double j1, j2, chi;
+    double arrival_s;
+    int k, samples, depth, krylov;
+    uint64_t seed;
+} qedh_request_t;
+typedef struct {
+    int slots;          /* requests in flight */
+    int batched;        /* FTLM samples through one multi-vector apply (else one after another) */
+    int cache_sectors;  /* keep sectors resident between requests */
+    int realtime;       /* honour arrival times (else all requests are there at 0) */
+    uint64_t hbm_budget, cache_budget;   /* 0: defaults (200 GiB, 48 GiB) */
+    int n_beta;
+    const double* betas;
+    int n_omega;
+    const double* omega;
+    double eta;
+} qedh_serve_options_t;
+/* Run the stream. values[n][stride]: eigs levels / thermal lnZ, E, V per beta / dynamics S(omega);
+ * e0[n]; timing[n][4] = arrival, start, finish, latency (s); status[n] (0 ok); dims[n];
+ * stats[12] = wall s, solves/s, p50, p99, mean, max latency, completed, failed, peak reserved bytes,
+ * peak in flight, sector builds, sector cache hits. Every output but values may be NULL. */
+int qedh_serve(const qedh_request_t* reqs, int n, const qedh_serve_options_t* opt, int stride, double* values,
+               double* e0, double* timing, int* status, int64_t* dims, double* stats);
+/* The HBM bytes a request reserves at admission. */
+int qedh_request_footprint(const qedh_request_t* req, int batched, uint64_t* bytes);
+
 #ifdef __cplusplus
 }
 #endif

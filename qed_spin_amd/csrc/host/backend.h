@@ -25,20 +25,14 @@ public:
     HipBackend() : scratch_(dev::blas_scratch_bytes(256)), result_(64 * 16) {}
 
     struct Deleter {
-        void operator()(Scalar* p) const noexcept {
-            if (p) (void)hipFree(p);
-        }
+        void operator()(Scalar* p) const noexcept { device_free(p); }
     };
     using UniqueVec = std::unique_ptr<Scalar, Deleter>;
 
     Scalar* allocate(std::size_t n) const {
         void* p = nullptr;
         if (n == 0) return nullptr;
-        const hipError_t e = hipMalloc(&p, n * sizeof(Scalar));
-        if (e != hipSuccess) {
-            (void)hipGetLastError();
-            throw Error(e == hipErrorOutOfMemory ? kResourceLimit : kRuntime, "HipBackend: device allocation failed");
-        }
+        device_alloc(&p, n * sizeof(Scalar));
         return static_cast<Scalar*>(p);
     }
     UniqueVec make_zero_vector(std::size_t n) const {

@@ -87,9 +87,7 @@ BatchedLanczos::BatchedLanczos(std::vector<std::shared_ptr<const SectorOperator>
     rows_ = row_off_.back();
     nnz_ = nnz_off.back();
     if (rows_ >= (std::uint64_t{1} << 32)) throw Error(kInvalid, "batched lanczos: more than 2^32 rows in one batch");
-    // The operators' CSRs were built on the null stream; the packing reads them.
-    QSA_HIP(hipStreamSynchronize(nullptr));
-    const hipStream_t st = nullptr;
+    const hipStream_t st = engine_stream();
     row_ptr_.reset((rows_ + 1) * 8);
     col_.reset(std::max<std::uint64_t>(nnz_, 1) * 4);
     val_.reset(std::max<std::uint64_t>(nnz_, 1) * sizeof(dev::Zd));
@@ -149,24 +147,24 @@ dev::BatchView BatchedLanczos::view() const {
 }
 
 void BatchedLanczos::set_start(const cplx* host) {
-    QSA_HIP(hipMemcpy(V0_.get(), host, rows_ * R_ * sizeof(dev::Zd), hipMemcpyHostToDevice));
+    copy_to_device(V0_.get(), host, rows_ * R_ * sizeof(dev::Zd));
     normalize(nullptr);
 }
 
 void BatchedLanczos::clear_start() {
-    dev::batch_fill_zero(V0_.get<dev::Zd>(), rows_ * R_, nullptr);
+    dev::batch_fill_zero(V0_.get<dev::Zd>(), rows_ * R_, engine_stream());
     QSA_HIP(hipGetLastError());
 }
 
 void BatchedLanczos::place(int b, int s, const cplx* device_vector) {
     if (b < 0 || b >= n_blocks() || s < 0 || s >= R_) throw Error(kInvalid, "batched lanczos: no such pair");
     dev::batch_place(V0_.get<dev::Zd>(), R_, row_off_[b], s, reinterpret_cast<const dev::Zd*>(device_vector), dim(b),
-                     nullptr);
+                     engine_stream());
     QSA_HIP(hipGetLastError());
 }
 
 void BatchedLanczos::normalize(std::vector<double>* norm2) {
-    const hipStream_t st = nullptr;
+    const hipStream_t st = engine_stream();
     dev::BatchView B = view();
     B.active = nullptr;
     const std::size_t pairs = ops_.size() * static_cast<std::size_t>(R_);
@@ -177,14 +175,14 @@ void BatchedLanczos::normalize(std::vector<double>* norm2) {
     QSA_HIP(hipStreamSynchronize(st));
     if (norm2) {
         norm2->resize(pairs);
-        QSA_HIP(hipMemcpy(norm2->data(), n2.get(), pairs * 8, hipMemcpyDeviceToHost));
+        copy_to_host(norm2->data(), n2.get(), pairs * 8);
     }
 }
 
 void BatchedLanczos::apply(const cplx* X, cplx* Y) const {
     dev::BatchView B = view();
     B.active = nullptr;
-    dev::batch_spmv(B, reinterpret_cast<const dev::Zd*>(X), reinterpret_cast<dev::Zd*>(Y), nullptr);
+    dev::batch_spmv(B, reinterpret_cast<const dev::Zd*>(X), reinterpret_cast<dev::Zd*>(Y), engine_stream());
     QSA_HIP(hipGetLastError());
 }
 
@@ -195,9 +193,16 @@ void BatchedLanczos::run(int M, const std::vector<double>& breakdown_tol, bool l
     if (breakdown_tol.size() != ops_.size()) throw Error(kInvalid, "batched lanczos: one breakdown tolerance per block");
     const std::size_t nb = ops_.size(), pairs = nb * static_cast<std::size_t>(R_);
     const std::size_t hist = static_cast<std::size_t>(M) * pairs;
-    OwnStream own;
-    const hipStream_t st = own.s;
-    QSA_HIP(hipStreamSynchronize(nullptr));
+    // Graph capture needs a stream other than the null one: a stream of its own when the engine
+    // runs on the null stream (everything before it on the null stream is finished first).
+    const hipStream_t outer = engine_stream();
+    std::unique_ptr<OwnStream> own;
+    if (!outer) {
+        QSA_HIP(hipStreamSynchronize(nullptr));
+        own = std::make_unique<OwnStream>();
+    }
+    const hipStream_t st = outer ? outer : own->s;
+    StreamScope scope(st);
     std::vector<int> cap(nb), ones(pairs, 1), zeros(pairs, 0);
     if (live) {
         if (live->size() != pairs) throw Error(kInvalid, "batched lanczos: one live flag per pair");
@@ -290,9 +295,9 @@ void BatchedLanczos::run(int M, const std::vector<double>& breakdown_tol, bool l
     alpha.resize(hist);
     beta.resize(hist);
     steps.resize(pairs);
-    QSA_HIP(hipMemcpy(alpha.data(), ah, hist * 8, hipMemcpyDeviceToHost));
-    QSA_HIP(hipMemcpy(beta.data(), bh, hist * 8, hipMemcpyDeviceToHost));
-    QSA_HIP(hipMemcpy(steps.data(), steps_d_.get(), pairs * sizeof(int), hipMemcpyDeviceToHost));
+    copy_to_host(alpha.data(), ah, hist * 8);
+    copy_to_host(beta.data(), bh, hist * 8);
+    copy_to_host(steps.data(), steps_d_.get(), pairs * sizeof(int));
     for (int s : steps) t.applies += static_cast<std::uint64_t>(s);
     t.rows = rows_;
     t.nnz = nnz_;

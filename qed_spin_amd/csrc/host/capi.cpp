@@ -508,8 +508,8 @@ extern "C" int qedh_sector_index_and_projection(qedh_sector s, const uint64_t* s
         qsa::dev::index_proj_batch(s->sector->view(), st.get<std::uint64_t>(), n, di.get<qsa::dev::i64>(),
                                    dp.get<double>(), qsa::engine_stream());
         QSA_HIP(hipGetLastError());
-        QSA_HIP(hipMemcpy(index, di.get(), static_cast<std::size_t>(n) * 8, hipMemcpyDeviceToHost));
-        QSA_HIP(hipMemcpy(proj, dp.get(), static_cast<std::size_t>(n) * 16, hipMemcpyDeviceToHost));
+        qsa::copy_to_host(index, di.get(), static_cast<std::size_t>(n) * 8);
+        qsa::copy_to_host(proj, dp.get(), static_cast<std::size_t>(n) * 16);
     });
 }
 

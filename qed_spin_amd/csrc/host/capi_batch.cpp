@@ -1,13 +1,16 @@
 // qedh_batch_lanczos, qedh_ftlm_batched: the batched Krylov engine behind the C-ABI.
 #include "../../../include/qedh.h"
 
+#include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <limits>
 #include <memory>
 #include <string>
 #include <vector>
 
 #include "batched.h"
+#include "serve.h"
 #include "tridiag.h"
 
 struct qedh_matvec_s {
@@ -261,5 +264,77 @@ extern "C" int qedh_cf_batched(const qedh_matvec* blocks, int n_blocks, int R, c
             stats[5] = static_cast<double>(t.rows);
             stats[6] = static_cast<double>(t.nnz);
         }
+    });
+}
+
+extern "C" int qedh_serve(const qedh_request_t* reqs, int n, const qedh_serve_options_t* opt, int stride,
+                          double* values, double* e0, double* timing, int* status, int64_t* dims, double* stats) {
+    return guarded_batch([&] {
+        if (!reqs || n < 1 || !opt || stride < 1 || !values)
+            throw qsa::Error(qsa::kInvalid, "qedh_serve: bad arguments");
+        std::vector<qsa::Request> rs(static_cast<std::size_t>(n));
+        for (int i = 0; i < n; ++i) {
+            const auto& q = reqs[i];
+            auto& r = rs[i];
+            r.kind = q.kind;
+            r.n_sites = q.n_sites;
+            r.j1 = q.j1;
+            r.j2 = q.j2;
+            r.chi = q.chi;
+            r.arrival_s = q.arrival_s;
+            r.k = q.k;
+            r.samples = q.samples;
+            r.depth = q.depth;
+            r.krylov = q.krylov;
+            r.seed = q.seed;
+        }
+        qsa::ServeOptions o;
+        o.slots = opt->slots;
+        o.batched = opt->batched != 0;
+        o.cache_sectors = opt->cache_sectors != 0;
+        o.realtime = opt->realtime != 0;
+        if (opt->hbm_budget) o.hbm_budget = opt->hbm_budget;
+        if (opt->cache_budget) o.cache_budget = opt->cache_budget;
+        if (opt->n_beta > 0 && opt->betas) o.betas.assign(opt->betas, opt->betas + opt->n_beta);
+        if (opt->n_omega > 0 && opt->omega) o.omega.assign(opt->omega, opt->omega + opt->n_omega);
+        o.eta = opt->eta;
+        std::vector<qsa::RequestResult> out;
+        const qsa::ServeStats st = qsa::serve(rs, o, out);
+        for (int i = 0; i < n; ++i) {
+            const auto& r = out[i];
+            double* v = values + static_cast<std::size_t>(i) * stride;
+            std::fill(v, v + stride, 0.0);
+            for (std::size_t j = 0; j < r.values.size() && j < static_cast<std::size_t>(stride); ++j) v[j] = r.values[j];
+            if (e0) e0[i] = r.e0;
+            if (timing) {
+                timing[4 * i] = rs[i].arrival_s;
+                timing[4 * i + 1] = r.start_s;
+                timing[4 * i + 2] = r.finish_s;
+                timing[4 * i + 3] = r.latency_s;
+            }
+            if (status) status[i] = r.status;
+            if (dims) dims[i] = r.dim;
+        }
+        if (stats) {
+            const double s[12] = {st.wall_s, st.solves_per_s, st.p50_s, st.p99_s, st.mean_s, st.max_s,
+                                  static_cast<double>(st.completed), static_cast<double>(st.failed),
+                                  static_cast<double>(st.peak_reserved), static_cast<double>(st.peak_in_flight),
+                                  static_cast<double>(st.sector_builds), static_cast<double>(st.sector_hits)};
+            std::copy(s, s + 12, stats);
+        }
+    });
+}
+
+extern "C" int qedh_request_footprint(const qedh_request_t* req, int batched, uint64_t* bytes) {
+    return guarded_batch([&] {
+        if (!req || !bytes) throw qsa::Error(qsa::kInvalid, "qedh_request_footprint: null argument");
+        qsa::Request r;
+        r.kind = req->kind;
+        r.n_sites = req->n_sites;
+        r.samples = req->samples;
+        r.depth = req->depth;
+        qsa::ServeOptions o;
+        o.batched = batched != 0;
+        *bytes = qsa::request_footprint(r, o);
     });
 }

@@ -39,6 +39,47 @@ inline void hip_check(hipError_t e, const char* what) {
 
 #define QSA_HIP(x) ::qsa::hip_check((x), #x)
 
+namespace detail {
+inline hipStream_t& tls_stream() {
+    thread_local hipStream_t s = nullptr;
+    return s;
+}
+}  // namespace detail
+
+/// The stream every engine launch, copy and allocation of the calling thread goes on: the null
+/// stream unless a StreamScope routes the thread elsewhere (the serving loop gives each of its
+/// slots a stream of its own, so requests in flight overlap on the device).
+inline hipStream_t engine_stream() { return detail::tls_stream(); }
+
+class StreamScope {
+public:
+    explicit StreamScope(hipStream_t s) : prev_(detail::tls_stream()) { detail::tls_stream() = s; }
+    ~StreamScope() { detail::tls_stream() = prev_; }
+    StreamScope(const StreamScope&) = delete;
+    StreamScope& operator=(const StreamScope&) = delete;
+
+private:
+    hipStream_t prev_;
+};
+
+/// Stream-ordered device memory on the engine stream (the device's default pool, kept by the
+/// process instead of returned at every synchronisation). A buffer is freed in the order of the
+/// freeing thread's stream: work of another stream that still reads it must have been waited for.
+void device_alloc(void** p, std::size_t bytes);
+void device_free(void* p) noexcept;
+
+/// Synchronous copies through the engine stream.
+inline void copy_to_host(void* dst, const void* src, std::size_t bytes) {
+    if (!bytes) return;
+    QSA_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, engine_stream()));
+    QSA_HIP(hipStreamSynchronize(engine_stream()));
+}
+inline void copy_to_device(void* dst, const void* src, std::size_t bytes) {
+    if (!bytes) return;
+    QSA_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, engine_stream()));
+    QSA_HIP(hipStreamSynchronize(engine_stream()));
+}
+
 /// One device allocation, freed with the object.
 class DeviceBuffer {
 public:
@@ -59,17 +100,11 @@ public:
     void reset(std::size_t bytes) {
         release();
         if (bytes == 0) return;
-        const hipError_t e = hipMalloc(&p_, bytes);
-        if (e != hipSuccess) {
-            p_ = nullptr;
-            (void)hipGetLastError();
-            throw Error(e == hipErrorOutOfMemory ? kResourceLimit : kRuntime,
-                        "device allocation of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
-        }
+        device_alloc(&p_, bytes);
         n_ = bytes;
     }
     void release() noexcept {
-        if (p_) (void)hipFree(p_);
+        if (p_) device_free(p_);
         p_ = nullptr;
         n_ = 0;
     }
@@ -85,13 +120,9 @@ private:
 /// Upload a host array into a fresh device buffer (synchronous).
 template <class T> DeviceBuffer upload(const T* host, std::size_t n) {
     DeviceBuffer b(n * sizeof(T));
-    if (n) QSA_HIP(hipMemcpy(b.get(), host, n * sizeof(T), hipMemcpyHostToDevice));
+    copy_to_device(b.get(), host, n * sizeof(T));
     return b;
 }
-
-/// The stream every engine launch goes on (the null stream of the current device: launches from
-/// one host thread are ordered, as the reference's legacy default stream orders them).
-inline hipStream_t engine_stream() { return nullptr; }
 
 /// Fails with DeviceUnavailable when no HIP device is visible.
 void require_device();

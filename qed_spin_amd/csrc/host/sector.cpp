@@ -1,5 +1,7 @@
 #include "sector.h"
 
+#include <mutex>
+
 #include <chrono>
 
 #include "../device/launch.h"
@@ -15,6 +17,33 @@ std::uint64_t binom_host(int n, int k) {
     return r;
 }
 }  // namespace
+
+void device_alloc(void** p, std::size_t bytes) {
+    *p = nullptr;
+    int dev = 0;
+    QSA_HIP(hipGetDevice(&dev));
+    static std::once_flag once[64];
+    if (dev >= 0 && dev < 64)
+        std::call_once(once[dev], [dev] {
+            hipMemPool_t pool = nullptr;
+            if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess && pool) {
+                std::uint64_t keep = ~std::uint64_t{0};
+                (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+            }
+            (void)hipGetLastError();
+        });
+    const hipError_t e = hipMallocAsync(p, bytes, engine_stream());
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        throw Error(e == hipErrorOutOfMemory ? kResourceLimit : kRuntime,
+                    "device allocation of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
+    }
+}
+
+void device_free(void* p) noexcept {
+    if (p) (void)hipFreeAsync(p, engine_stream());
+}
 
 void require_device() {
     int n = 0;
@@ -147,13 +176,13 @@ dev::SectorView DeviceSector::view() const {
 
 std::vector<std::uint64_t> DeviceSector::reps() const {
     std::vector<std::uint64_t> h(static_cast<std::size_t>(dim_));
-    if (dim_) QSA_HIP(hipMemcpy(h.data(), reps_.get(), h.size() * 8, hipMemcpyDeviceToHost));
+    if (dim_) copy_to_host(h.data(), reps_.get(), h.size() * 8);
     return h;
 }
 
 std::vector<double> DeviceSector::inv_norms() const {
     std::vector<double> h(static_cast<std::size_t>(dim_));
-    if (dim_) QSA_HIP(hipMemcpy(h.data(), inv_norms_.get(), h.size() * 8, hipMemcpyDeviceToHost));
+    if (dim_) copy_to_host(h.data(), inv_norms_.get(), h.size() * 8);
     return h;
 }
 

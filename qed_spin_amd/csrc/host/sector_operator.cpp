@@ -9,8 +9,9 @@ namespace qsa {
 
 SectorOperator::SectorOperator(std::shared_ptr<const DeviceSector> rows, std::shared_ptr<const DeviceSector> cols,
                                std::shared_ptr<const DeviceProgram> prog, double norm_bound, int lane,
-                               std::uint64_t csr_budget_bytes)
-    : rows_(std::move(rows)), cols_(cols ? std::move(cols) : rows_), prog_(std::move(prog)), norm_bound_(norm_bound) {
+                               std::uint64_t csr_budget_bytes, bool tune)
+    : rows_(std::move(rows)), cols_(cols ? std::move(cols) : rows_), prog_(std::move(prog)), norm_bound_(norm_bound),
+      tune_(tune) {
     if (!rows_ || !prog_) throw Error(kInvalid, "SectorOperator: missing sector or program");
     const auto& a = rows_->spec();
     const auto& b = cols_->spec();
@@ -77,9 +78,9 @@ void SectorOperator::build_csr(std::uint64_t budget) {
     vals_.reset(nnz * vbytes);
     if (width > 0) {
         std::vector<std::uint64_t> fp(dev::DictTable::kSlots), re(dev::DictTable::kSlots), im(dev::DictTable::kSlots);
-        QSA_HIP(hipMemcpy(fp.data(), dfp.get(), dfp.bytes(), hipMemcpyDeviceToHost));
-        QSA_HIP(hipMemcpy(re.data(), dre.get(), dre.bytes(), hipMemcpyDeviceToHost));
-        QSA_HIP(hipMemcpy(im.data(), dim_.get(), dim_.bytes(), hipMemcpyDeviceToHost));
+        copy_to_host(fp.data(), dfp.get(), dfp.bytes());
+        copy_to_host(re.data(), dre.get(), dre.bytes());
+        copy_to_host(im.data(), dim_.get(), dim_.bytes());
         std::vector<std::uint16_t> slot_id(dev::DictTable::kSlots, 0);
         std::vector<double> dr, di;
         for (int s = 0; s < dev::DictTable::kSlots; ++s) {
@@ -91,7 +92,7 @@ void SectorOperator::build_csr(std::uint64_t budget) {
             dr.push_back(a);
             di.push_back(b);
         }
-        QSA_HIP(hipMemcpy(dslot.get(), slot_id.data(), slot_id.size() * 2, hipMemcpyHostToDevice));
+        copy_to_device(dslot.get(), slot_id.data(), slot_id.size() * 2);
         dict_re_ = upload(dr.data(), dr.size());
         dict_im_ = upload(di.data(), di.size());
         csr_.dict = static_cast<std::int64_t>(dr.size());
@@ -123,7 +124,7 @@ void SectorOperator::build_csr(std::uint64_t budget) {
     csr_.lanes = L;
     lanes_block_ = L;
     lane_ = kLaneCsr;
-    if (rows >= (std::int64_t{1} << 18)) autotune();
+    if (tune_ && rows >= (std::int64_t{1} << 18)) autotune();
     csr_.build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
@@ -228,23 +229,23 @@ void SectorOperator::csr_download_rows(std::int64_t r0, std::int64_t r1, std::ve
     if (lane_ != kLaneCsr) throw Error(kInvalid, "csr_download: no CSR built");
     if (r0 < 0 || r1 < r0 || r1 > dim()) throw Error(kInvalid, "csr_download: row range out of bounds");
     row_ptr.resize(static_cast<std::size_t>(r1 - r0 + 1));
-    QSA_HIP(hipMemcpy(row_ptr.data(), row_ptr_.get<std::uint64_t>() + r0, row_ptr.size() * 8, hipMemcpyDeviceToHost));
+    copy_to_host(row_ptr.data(), row_ptr_.get<std::uint64_t>() + r0, row_ptr.size() * 8);
     const std::size_t e0 = static_cast<std::size_t>(row_ptr.front());
     const std::size_t nnz = static_cast<std::size_t>(row_ptr.back() - row_ptr.front());
     col.resize(nnz);
-    if (nnz) QSA_HIP(hipMemcpy(col.data(), col_.get<std::uint32_t>() + e0, nnz * 4, hipMemcpyDeviceToHost));
+    if (nnz) copy_to_host(col.data(), col_.get<std::uint32_t>() + e0, nnz * 4);
     val.assign(2 * nnz, 0.0);
     if (csr_.width == 1 || csr_.width == 2) {
         std::vector<double> dr(static_cast<std::size_t>(csr_.dict)), di(static_cast<std::size_t>(csr_.dict));
-        QSA_HIP(hipMemcpy(dr.data(), dict_re_.get(), dr.size() * 8, hipMemcpyDeviceToHost));
-        QSA_HIP(hipMemcpy(di.data(), dict_im_.get(), di.size() * 8, hipMemcpyDeviceToHost));
+        copy_to_host(dr.data(), dict_re_.get(), dr.size() * 8);
+        copy_to_host(di.data(), dict_im_.get(), di.size() * 8);
         std::vector<std::uint16_t> ids(nnz);
         if (csr_.width == 1) {
             std::vector<std::uint8_t> b(nnz);
-            if (nnz) QSA_HIP(hipMemcpy(b.data(), vals_.get<std::uint8_t>() + e0, nnz, hipMemcpyDeviceToHost));
+            if (nnz) copy_to_host(b.data(), vals_.get<std::uint8_t>() + e0, nnz);
             for (std::size_t e = 0; e < nnz; ++e) ids[e] = b[e];
         } else if (nnz) {
-            QSA_HIP(hipMemcpy(ids.data(), vals_.get<std::uint16_t>() + e0, nnz * 2, hipMemcpyDeviceToHost));
+            copy_to_host(ids.data(), vals_.get<std::uint16_t>() + e0, nnz * 2);
         }
         for (std::size_t e = 0; e < nnz; ++e) {
             val[2 * e] = dr[ids[e]];
@@ -252,10 +253,10 @@ void SectorOperator::csr_download_rows(std::int64_t r0, std::int64_t r1, std::ve
         }
     } else if (real_) {
         std::vector<double> v(nnz);
-        if (nnz) QSA_HIP(hipMemcpy(v.data(), vals_.get<double>() + e0, nnz * 8, hipMemcpyDeviceToHost));
+        if (nnz) copy_to_host(v.data(), vals_.get<double>() + e0, nnz * 8);
         for (std::size_t e = 0; e < nnz; ++e) val[2 * e] = v[e];
     } else if (nnz) {
-        QSA_HIP(hipMemcpy(val.data(), vals_.get<double>() + 2 * e0, nnz * 16, hipMemcpyDeviceToHost));
+        copy_to_host(val.data(), vals_.get<double>() + 2 * e0, nnz * 16);
     }
 }
 

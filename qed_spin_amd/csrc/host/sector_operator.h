@@ -29,9 +29,12 @@ struct CsrInfo {
 
 class SectorOperator {
 public:
+    /// tune: time the SpMV widths on a large CSR and keep the fastest (else the width follows the
+    /// mean row length only, so the summation order -- and the last bits of every apply -- do not
+    /// depend on timing).
     SectorOperator(std::shared_ptr<const DeviceSector> rows, std::shared_ptr<const DeviceSector> cols,
                    std::shared_ptr<const DeviceProgram> prog, double norm_bound, int lane,
-                   std::uint64_t csr_budget_bytes);
+                   std::uint64_t csr_budget_bytes, bool tune = true);
 
     std::int64_t dim() const { return rows_->dim(); }
     std::int64_t col_dim() const { return cols_->dim(); }
@@ -88,6 +91,7 @@ private:
     int lane_ = kLaneGather;
     bool nt_ = false;
     bool xcd_ = false;
+    bool tune_ = true;
     bool unroll_ = false;
     CsrInfo csr_;
     DeviceBuffer row_ptr_, col_, vals_, dict_re_, dict_im_;

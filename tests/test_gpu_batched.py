@@ -341,3 +341,50 @@ def test_dynamics_reproduces_reference_T0_records():
         err = np.abs(r.S[0] - want).max()
         print(f"  {label}: ground manifold {r.ground_manifold}, E0 {r.e0:.12f}, max |diff| {err:.3e}")
         assert np.allclose(r.S[0], want, rtol=1e-8, atol=1e-8), label
+
+
+def _small_stream(n=18, seed=5):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        out.append(dict(kind=i % 3, n_sites=int(rng.integers(8, 15)), j1=float(rng.uniform(0.5, 1.5)),
+                        j2=float(rng.uniform(0.0, 0.6)), chi=float(rng.uniform(-0.3, 0.3)), arrival_s=0.01 * i,
+                        k=3, samples=6, depth=20, krylov=40, seed=100 + i))
+    return out
+
+
+def test_serve_results_same_alone_and_in_the_mix():
+    """Every request's result served in a mix of concurrent requests equals the same request
+    served alone (one slot) bit for bit, and the plain path (no batching, no sector cache) within
+    1e-8; eigs levels against the dense oracle."""
+    from qed_spin_amd import serve as sv
+
+    say("serve: mix vs alone vs plain")
+    reqs = _small_stream()
+    T = np.linspace(0.3, 3.0, 5)
+    omega = np.linspace(0.0, 4.0, 50)
+    mix = sv.serve(reqs, slots=6, T=T, omega=omega, eta=0.1, realtime=True)
+    alone = sv.serve(reqs, slots=1, T=T, omega=omega, eta=0.1, realtime=False)
+    plain = sv.serve(reqs, slots=1, batched=False, cache_sectors=False, T=T, omega=omega, eta=0.1, realtime=False)
+    print("  mix stats", mix.stats)
+    assert mix.stats["completed"] == len(reqs) and mix.stats["peak_in_flight"] > 1
+    for i, r in enumerate(reqs):
+        assert mix.status[i] == 0 and alone.status[i] == 0 and plain.status[i] == 0, i
+        assert np.array_equal(mix.values[i].view(np.uint64), alone.values[i].view(np.uint64)), i
+        assert np.allclose(mix.values[i], plain.values[i], rtol=1e-8, atol=1e-8), i
+    # eigs against the dense oracle of the sector
+    for i, r in enumerate(reqs):
+        if r["kind"] != 0:
+            continue
+        N = r["n_sites"]
+        b = qs.HamiltonianBuilder(N)
+        b.heisenberg(ring(N), r["j1"])
+        b.heisenberg(ring(N, 2), r["j2"])
+        b.chirality([(j, (j + 1) % N, (j + 2) % N) for j in range(N)], r["chi"])
+        H = b.to_operator()
+        s = po.OracleSector(N, N // 2, sym.translations(N), sym.momentum_chars(N, 0))
+        states = np.array([x for x in range(1 << N) if bin(x).count("1") == N // 2], np.uint64)
+        E = po.sector_basis_matrix(s, states)
+        ev = np.linalg.eigvalsh(E.conj().T @ po.dense_matrix(H.products, N, states) @ E)
+        k = len(mix.values[i])
+        assert np.allclose(mix.values[i], ev[:k], rtol=1e-10, atol=1e-10), i
