@@ -82,6 +82,99 @@ def cpu_baseline(m, D, w_id, target_s=12.0):
     }
 
 
+def run_ftlm(args):
+    """FTLM thermodynamics of the XXZ ring N=24 (Jxy = 1, Delta = 0.5) over every (Sz, k) block:
+    40 samples per block (seed 7), Lanczos depth 100, 40 temperatures linspace(0.1, 4); blocks of at
+    most 512 states exactly, all others through the batched engine in one packed multi-vector matvec
+    per step."""
+    import numpy as np
+
+    import qed_spin_amd as qs
+    from qed_spin_amd import thermal as th
+
+    n = args.ftlm_sites
+    H = qs.HamiltonianBuilder(n).xxz([(i, (i + 1) % n) for i in range(n)], 1.0, 0.5).to_operator()
+    T = np.linspace(0.1, 4.0, 40)
+    t0 = time.perf_counter()
+    r = th.thermal(H, T, sym=qs.Symmetry(), samples=40, krylov=100, seed=7)
+    wall = time.perf_counter() - t0
+    e = r.engine
+    R = 40
+    rows, nnz = e.get("rows", 0.0), e.get("nnz", 0.0)
+    step_bytes = rows * 8 + nnz * (4 + 16) + R * (nnz * 16 + rows * 16)
+    gbs = step_bytes * 100 / e["steps_s"] / 1e9 if e.get("steps_s") else None
+    out = {
+        "workload": f"FTLM XXZ ring N={n}, Delta=0.5, every (Sz, k) block, 40 samples, depth 100, 40 T",
+        "blocks": len(r.blocks),
+        "sampled_blocks": sum(1 for b in r.blocks if not b["exact"]),
+        "exact_blocks": sum(1 for b in r.blocks if b["exact"]),
+        "total_dim": r.total_dim,
+        "wall_s": round(wall, 3),
+        "timings": {k: round(v, 3) for k, v in r.timings.items()},
+        "engine": {k: round(v, 4) for k, v in e.items()},
+        "batched_step_GBps_algorithmic": round(gbs, 1) if gbs else None,
+        "E_T0.1": float(r.E[0]),
+        "C_max": float(r.C.max()),
+    }
+    log(f"ftlm: {out}")
+    return out
+
+
+def run_dynamics(args):
+    """T = 0 S^z(q) of the Heisenberg ring N=28 for all 28 q: the ground state of (Sz=0, k=0), the
+    probe images in the 28 momentum sectors, 28 continued fractions of depth 200 through the batched
+    engine (omega = linspace(0, 4, 400), eta = 0.05)."""
+    import numpy as np
+
+    import qed_spin_amd as qs
+    from qed_spin_amd import dynamics as dy
+    from qed_spin_amd import thermal as th
+
+    n = args.dyn_sites
+    H = qs.HamiltonianBuilder(n).heisenberg([(i, (i + 1) % n) for i in range(n)], 1.0).to_operator()
+    omega = np.linspace(0.0, 4.0, 400)
+    t0 = time.perf_counter()
+    blocks = th.walk_blocks(H, qs.Symmetry(sz=[n // 2]))
+    t_build = time.perf_counter() - t0
+    k0 = [i for i, (b, s, m) in enumerate(blocks) if b.momentum == 0]
+    probes = [dy.sz_q(n, 2 * np.pi * m / n) for m in range(n)]
+    r = dy.dynamics(H, probes, omega, eta=0.05, krylov=200, blocks=blocks, ground_blocks=k0)
+    wall = time.perf_counter() - t0
+    out = {
+        "workload": f"T=0 S^z(q) of the Heisenberg ring N={n}, all {n} q, krylov 200, 400 omega, eta 0.05",
+        "e0": r.e0,
+        "ground_manifold": r.ground_manifold,
+        "target_sectors": r.target_sectors,
+        "wall_s": round(wall, 3),
+        "build_s": round(t_build, 3),
+        "timings": {k: round(v, 3) for k, v in r.timings.items()},
+        "engine": {k: round(v, 4) for k, v in r.engine.items()},
+        "S_pi_max": float(max(r.S[n // 2])),
+    }
+    log(f"dynamics: {out}")
+    return out
+
+
+def run_serve(args):
+    """The mixed request stream served (slots, HBM admission, resident sectors, batched FTLM)
+    against the plain path (one request after another, no batching, no cache)."""
+    from qed_spin_amd import serve as sv
+
+    reqs = sv.mixed_stream(n=args.serve_requests, rate=args.serve_rate, seed=3, n_max=args.serve_nmax)
+    served = sv.serve(reqs, slots=args.serve_slots, realtime=True)
+    out = {"workload": f"{len(reqs)} mixed requests (eigs/thermal/dynamics 50/30/20), N in 16..{args.serve_nmax}, "
+                       f"Poisson {args.serve_rate}/s, seed 3",
+           "served": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in served.stats.items()}}
+    if not args.no_plain:
+        plain = sv.serve(reqs, slots=1, batched=False, cache_sectors=False, realtime=False)
+        sat = sv.serve(reqs, slots=args.serve_slots, realtime=False)
+        out["plain"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in plain.stats.items()}
+        out["served_saturated"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in sat.stats.items()}
+        out["speedup_solves_per_s"] = round(sat.stats["solves_per_s"] / plain.stats["solves_per_s"], 3)
+    log(f"serve: {out}")
+    return out
+
+
 def _cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -101,6 +194,15 @@ def main():
     ap.add_argument("--no-lanczos", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--lanes", type=int, default=0)
+    ap.add_argument("--workload", default="matvec",
+                    help="matvec (the headline), or a comma list adding ftlm, dynamics, serve")
+    ap.add_argument("--ftlm-sites", type=int, default=24)
+    ap.add_argument("--dyn-sites", type=int, default=28)
+    ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--serve-nmax", type=int, default=30)
+    ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 1 GiB device to device (the known-size kernel the HBM counters are scaled by)")
     args = ap.parse_args()
@@ -238,6 +340,16 @@ def main():
         "lanczos_to_E0": lanczos,
         "cpu_baseline": cpu,
     }
+    extra = [w.strip() for w in args.workload.split(",") if w.strip() and w.strip() != "matvec"]
+    for w in extra:
+        if w == "ftlm":
+            out["ftlm"] = run_ftlm(args)
+        elif w == "dynamics":
+            out["dynamics"] = run_dynamics(args)
+        elif w == "serve":
+            out["serve"] = run_serve(args)
+        else:
+            raise SystemExit(f"unknown workload {w!r}")
     print(json.dumps(out))
 
 

@@ -72,13 +72,16 @@ class DynamicsResult:
     engine: Dict[str, float] = field(default_factory=dict)
 
 
-def ground_manifold(blocks, window: float, k_max: int = 16):
+def ground_manifold(blocks, window: float, k_max: int = 16, only=None):
     """(E0, [(block index, device buffer, vector index, energy)]): every level within `window` of the
     lowest level of the walk. Each block's lowest level comes from the k = 1 scan; the blocks within
     reach of the lowest then give their eigenpairs (Krylov-Schur, k doubled until a level outside the
     window shows or k_max), and E0 is the lowest eigenvalue among them."""
     lows = []
-    for blk, sec, m in blocks:
+    for i, (blk, sec, m) in enumerate(blocks):
+        if only is not None and i not in only:
+            lows.append(math.inf)
+            continue
         vals, _, _ = m.lowest(1)
         lows.append(vals[0] if len(vals) else math.inf)
     e_scan = min(lows)
@@ -125,10 +128,11 @@ class _lib_alloc:
 
 def dynamics(H: Operator, probes, omega, *, eta: float = 0.1, T=None, sym: Optional[Symmetry] = None,
              krylov: int = 200, device: str = "gpu", degeneracy_tol: float = 1e-8, use_graph: bool = True,
-             csr_budget: int = DEFAULT_CSR_BUDGET, blocks=None) -> DynamicsResult:
+             csr_budget: int = DEFAULT_CSR_BUDGET, blocks=None, ground_blocks=None) -> DynamicsResult:
     """S_O(omega) = sum_n |<n|O|g>|^2 L_eta(omega - E_n + E0), averaged over the ground manifold,
     for every probe O (an Operator or a list of them). blocks: the walk's (block, sector, matvec)
-    triples when the caller already built them (thermal.walk_blocks)."""
+    triples when the caller already built them (thermal.walk_blocks). ground_blocks: indices of the
+    blocks known to hold the ground manifold (default: every block is searched)."""
     resolve_device(device, "dynamics")
     if T is not None and len(np.atleast_1d(T)) > 0:
         raise DeviceUnsupported("dynamics: T > 0 (FTLM dynamics) is not on the device path")
@@ -148,7 +152,7 @@ def dynamics(H: Operator, probes, omega, *, eta: float = 0.1, T=None, sym: Optio
     sH = H.norm_bound() or 1.0
     window = degeneracy_tol * sH
     t1 = time.perf_counter()
-    e0, states = ground_manifold(blocks, window)
+    e0, states = ground_manifold(blocks, window, only=None if ground_blocks is None else set(ground_blocks))
     t_gm = time.perf_counter() - t1
     # probe images: (state, probe) -> every block it reaches
     t2 = time.perf_counter()
