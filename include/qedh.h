@@ -126,8 +126,16 @@ int qedh_matvec_apply_block(qedh_matvec m, int nv, const void* X, void* Y, int c
 int qedh_matvec_set_lanes(qedh_matvec m, int lanes);
 /* SpMV tuning (no counterpart in the reference): lanes per row (0 keeps the current) and flags:
  * non-temporal streaming of the CSR arrays, one contiguous row range per XCD, the single-vector
- * gathers issued four entries at a time (every flag leaves the results bit for bit unchanged). */
-enum { QEDH_TUNE_NONTEMPORAL = 1, QEDH_TUNE_XCD_ROWS = 2, QEDH_TUNE_UNROLL = 4 };
+ * gathers issued four (or eight) entries at a time, u8-id dictionaries read from global memory
+ * instead of LDS (every flag leaves the results bit for bit unchanged). The default is XCD rows,
+ * unroll 4, LDS dictionary; flags replace it. */
+enum {
+    QEDH_TUNE_NONTEMPORAL = 1,
+    QEDH_TUNE_XCD_ROWS = 2,
+    QEDH_TUNE_UNROLL = 4,
+    QEDH_TUNE_UNROLL8 = 8,
+    QEDH_TUNE_GLOBAL_DICT = 16
+};
 int qedh_matvec_set_tuning(qedh_matvec m, int lanes, int flags);
 /* The merged CSR (download_csr): row_ptr[rows+1], col[nnz], val[2 nnz] (re, im). */
 int qedh_matvec_export_csr(qedh_matvec m, uint64_t* row_ptr, uint32_t* col, double* val);

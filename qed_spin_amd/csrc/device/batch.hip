@@ -9,13 +9,13 @@
 // Vector kernels: thread t of a tile is (slot t / R, vector t % R); the 256 / R slots take the
 // tile's rows in turn, so consecutive threads touch consecutive entries whatever R is. A
 // per-(block, vector) reduction is each thread's sum over its rows, then the slots' sums in slot
-// order (LDS), then the tiles of the block in order: deterministic, and independent of which other
-// blocks share the batch.
+// order (LDS), then the block's tile sums by a fixed tree: deterministic, and independent of which
+// other blocks share the batch.
 //
 // SpMV: for R >= 8 one thread per (row, vector), the R threads of a row reading the same column and
-// value and gathering one contiguous line of R entries; for R < 8, 16 threads per row (R rounded up
-// to a power of two, 16 / that many lanes per (row, vector)) splitting the row's entries, summed
-// by a fixed xor butterfly.
+// value and gathering one contiguous line of R entries; for 2 <= R < 8, 16 threads per row (R
+// rounded up to a power of two, 16 / that many lanes per (row, vector)) splitting the row's entries,
+// summed by a fixed xor butterfly; R = 1 is the sector SpMV of spmv.hip on the packed CSR.
 //
 // The Lanczos scalars alpha, beta, the local-reorthogonalisation overlap, the active flags and the
 // step counter live on the device; a step needs no host synchronisation, so it is captured once
@@ -168,23 +168,46 @@ __global__ __launch_bounds__(kThreads) void k_batch_vec(BatchView B, int mode, c
     if (mode != 3) tile_partial(B, pr, pi, partial);
 }
 
-// Per block, thread s: the block's tiles in order. kind 0: alpha (real part) -> alpha_hist[step];
-// kind 1: the overlap (complex) -> ov; kind 2: beta = sqrt -> beta_hist[step], 1 / beta ->
-// inv_beta; breakdown (beta < tol) or the block's Krylov cap ends the pair's run.
-__global__ __launch_bounds__(64) void k_batch_finish(BatchView B, int kind, const double* partial, double* alpha_hist,
-                                                     double* beta_hist, Zd* ov, double* cur_alpha, double* cur_beta,
-                                                     double* inv_beta, const double* breakdown_tol) {
-    const int b = blockIdx.x, s = threadIdx.x;
-    const int R = B.R;
-    if (s >= R) return;
-    const u64 bs = static_cast<u64>(b) * R + s;
-    if (!B.active[bs]) return;
-    const int step = *B.step;
-    double a = 0.0, c = 0.0;
-    for (u32 t = B.tile_begin[b]; t < B.tile_begin[b + 1]; ++t) {
-        a += partial[(static_cast<u64>(t) * 64 + s) * 2];
-        c += partial[(static_cast<u64>(t) * 64 + s) * 2 + 1];
+// The sum over block b's tiles of partial[tile][s] (re, im), one workgroup of 256 threads per
+// (block, vector): thread t takes tiles t, t + 256, ... in order, then a fixed pairwise tree in LDS.
+// Thread 0 returns the sums. Every thread of the workgroup calls it.
+__device__ __forceinline__ void block_sum(const BatchView& B, int b, int s, const double* partial, double& a,
+                                          double& c) {
+    __shared__ double ra[kThreads], rc[kThreads];
+    double x = 0.0, y = 0.0;
+    for (u32 t = B.tile_begin[b] + threadIdx.x; t < B.tile_begin[b + 1]; t += kThreads) {
+        x += partial[(static_cast<u64>(t) * 64 + s) * 2];
+        y += partial[(static_cast<u64>(t) * 64 + s) * 2 + 1];
     }
+    ra[threadIdx.x] = x;
+    rc[threadIdx.x] = y;
+    __syncthreads();
+    for (int off = kThreads / 2; off > 0; off >>= 1) {
+        if (static_cast<int>(threadIdx.x) < off) {
+            ra[threadIdx.x] += ra[threadIdx.x + off];
+            rc[threadIdx.x] += rc[threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    a = ra[0];
+    c = rc[0];
+}
+
+// One workgroup per (block, vector). kind 0: alpha (real part) -> alpha_hist[step]; kind 1: the
+// overlap (complex) -> ov; kind 2: beta = sqrt -> beta_hist[step], 1 / beta -> inv_beta; breakdown
+// (beta < tol) or the block's Krylov cap ends the pair's run.
+__global__ __launch_bounds__(kThreads) void k_batch_finish(BatchView B, int kind, const double* partial,
+                                                           double* alpha_hist, double* beta_hist, Zd* ov,
+                                                           double* cur_alpha, double* cur_beta, double* inv_beta,
+                                                           const double* breakdown_tol) {
+    const int R = B.R;
+    const u64 bs = blockIdx.x;
+    const int b = static_cast<int>(bs / R), s = static_cast<int>(bs % R);
+    if (!B.active[bs]) return;   // the whole workgroup
+    double a, c;
+    block_sum(B, b, s, partial, a, c);
+    if (threadIdx.x != 0) return;
+    const int step = *B.step;
     const u64 hist = static_cast<u64>(step) * B.n_blocks * R + bs;
     if (kind == 0) {
         alpha_hist[hist] = a;
@@ -219,13 +242,13 @@ __global__ __launch_bounds__(kThreads) void k_batch_norm_partial(BatchView B, co
     tile_partial(B, p, 0.0, partial);
 }
 
-__global__ __launch_bounds__(64) void k_batch_norm_finish(BatchView B, const double* partial, double* inv,
-                                                          double* norm2) {
-    const int b = blockIdx.x, s = threadIdx.x;
-    if (s >= B.R) return;
-    double a = 0.0;
-    for (u32 t = B.tile_begin[b]; t < B.tile_begin[b + 1]; ++t) a += partial[(static_cast<u64>(t) * 64 + s) * 2];
-    const u64 bs = static_cast<u64>(b) * B.R + s;
+__global__ __launch_bounds__(kThreads) void k_batch_norm_finish(BatchView B, const double* partial, double* inv,
+                                                                double* norm2) {
+    const u64 bs = blockIdx.x;
+    const int b = static_cast<int>(bs / B.R), s = static_cast<int>(bs % B.R);
+    double a, c;
+    block_sum(B, b, s, partial, a, c);
+    if (threadIdx.x != 0) return;
     inv[bs] = a > 0.0 ? 1.0 / sqrt(a) : 0.0;
     if (norm2) norm2[bs] = a;
 }
@@ -292,6 +315,27 @@ unsigned grid_for(u64 n) {
 
 void batch_spmv(const BatchView& B, const Zd* X, Zd* Y, hipStream_t st) {
     if (!B.n_tiles) return;
+    if (B.R == 1) {
+        // one vector per block: the packed matrix is an ordinary CSR, applied by the sector SpMV
+        // (8 lanes per row, gathers four entries at a time, a contiguous row range per XCD); pairs
+        // that finished still compute their rows, which nothing reads
+        CsrView A;
+        A.rows = static_cast<i64>(B.rows);
+        A.row_ptr = B.row_ptr;
+        A.col = B.col;
+        A.vals = B.val;
+        A.width = 0;
+        A.values_real = false;
+        A.lanes = 8;
+        A.xcd = 1;
+        A.unroll = 4;
+        VecSet v;
+        v.k = 1;
+        v.in[0] = X;
+        v.out[0] = Y;
+        csr_spmv(A, false, v, st);
+        return;
+    }
     const dim3 grid(B.n_tiles), block(kThreads);
     if (B.R >= 8) hipLaunchKernelGGL(k_batch_spmv_wide, grid, block, 0, st, B, X, Y);
     else if (B.R == 1) hipLaunchKernelGGL(k_batch_spmv_narrow<1>, grid, block, 0, st, B, X, Y);
@@ -309,14 +353,15 @@ void batch_vec(const BatchView& B, int mode, const Zd* Vprev, const Zd* Vcur, Zd
 
 void batch_finish(const BatchView& B, int kind, const double* partial, double* alpha_hist, double* beta_hist, Zd* ov,
                   double* cur_alpha, double* cur_beta, double* inv_beta, const double* breakdown_tol, hipStream_t st) {
-    hipLaunchKernelGGL(k_batch_finish, dim3(B.n_blocks), dim3(64), 0, st, B, kind, partial, alpha_hist, beta_hist, ov,
-                       cur_alpha, cur_beta, inv_beta, breakdown_tol);
+    hipLaunchKernelGGL(k_batch_finish, dim3(B.n_blocks * static_cast<u32>(B.R)), dim3(kThreads), 0, st, B, kind, partial,
+                       alpha_hist, beta_hist, ov, cur_alpha, cur_beta, inv_beta, breakdown_tol);
 }
 
 void batch_normalize(const BatchView& B, Zd* V, double* partial, double* inv, double* norm2, hipStream_t st) {
     if (!B.n_tiles) return;
     hipLaunchKernelGGL(k_batch_norm_partial, dim3(B.n_tiles), dim3(kThreads), 0, st, B, V, partial);
-    hipLaunchKernelGGL(k_batch_norm_finish, dim3(B.n_blocks), dim3(64), 0, st, B, partial, inv, norm2);
+    hipLaunchKernelGGL(k_batch_norm_finish, dim3(B.n_blocks * static_cast<u32>(B.R)), dim3(kThreads), 0, st, B, partial,
+                       inv, norm2);
     hipLaunchKernelGGL(k_batch_scale, dim3(B.n_tiles), dim3(kThreads), 0, st, B, V, inv);
 }
 

@@ -69,7 +69,9 @@ struct CsrView {
     bool values_real = false;      // the stored values are real (full values stored as double)
     int nt = 0;                    // stream the column / id arrays with non-temporal loads
     int xcd = 0;                   // contiguous row ranges per XCD (grid padded to a multiple of 8)
-    int unroll = 1;                // single-vector SpMV: gathers issued 4 entries at a time (same sums)
+    int unroll = 1;                // single-vector SpMV: gathers issued 4 or 8 entries at a time (same sums)
+    int dict_n = 0;                // dictionary entries (u8 / u16 ids)
+    int lds_dict = 1;              // u8 ids: stage the dictionary in LDS
 };
 /// out_v = A in_v (real: double vectors; else complex vectors, with real or complex values).
 void csr_spmv(const CsrView& A, bool real, const VecSet& v, hipStream_t st);
@@ -123,6 +125,7 @@ struct Tile {
     u32 block, pad;
 };
 struct BatchView {
+    u64 rows = 0;                       // packed rows
     u32 n_blocks = 0, n_tiles = 0;
     int R = 0;                          // vectors per block (<= 64), X[row R + s]
     const Tile* tiles = nullptr;        // n_tiles, blocks contiguous

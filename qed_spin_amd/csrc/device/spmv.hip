@@ -17,12 +17,34 @@ template <int NV> struct Ptrs {
     void* out[NV];
 };
 
+// The dictionary an entry's value id points into: u8 ids read it from LDS (staged at kernel start),
+// u16 ids from global memory.
+struct Dict {
+    const double* re;
+    const double* im;
+};
+
 template <int WIDTH, bool VREAL>
-__device__ __forceinline__ void value_at(const CsrView& A, u64 e, double& vr, double& vi) {
+__device__ __forceinline__ Dict stage_dict(const CsrView& A, double* sre, double* sim) {
+    if constexpr (WIDTH == 1) {
+        if (A.lds_dict) {
+            for (int i = threadIdx.x; i < A.dict_n; i += blockDim.x) {
+                sre[i] = A.dict_re[i];
+                if constexpr (!VREAL) sim[i] = A.dict_im[i];
+            }
+            __syncthreads();
+            return Dict{sre, sim};
+        }
+    }
+    return Dict{A.dict_re, A.dict_im};
+}
+
+template <int WIDTH, bool VREAL>
+__device__ __forceinline__ void value_at(const CsrView& A, const Dict& d, u64 e, double& vr, double& vi) {
     if constexpr (WIDTH == 1) {
         const u32 id = A.nt ? __builtin_nontemporal_load(static_cast<const u8*>(A.vals) + e) : static_cast<const u8*>(A.vals)[e];
-        vr = A.dict_re[id];
-        if constexpr (!VREAL) vi = A.dict_im[id];
+        vr = d.re[id];
+        if constexpr (!VREAL) vi = d.im[id];
     } else if constexpr (WIDTH == 2) {
         const u32 id = A.nt ? __builtin_nontemporal_load(static_cast<const u16*>(A.vals) + e) : static_cast<const u16*>(A.vals)[e];
         vr = A.dict_re[id];
@@ -55,6 +77,8 @@ inline unsigned grid_blocks(const CsrView& A, u64 threads) {
 
 template <int WIDTH, bool VREAL, bool REAL, int L, int NV, int U>
 __global__ __launch_bounds__(256) void k_spmv(CsrView A, Ptrs<NV> v) {
+    __shared__ double sre[WIDTH == 1 ? 256 : 1], sim[WIDTH == 1 && !VREAL ? 256 : 1];
+    const Dict d = stage_dict<WIDTH, VREAL>(A, sre, sim);
     const i64 gid = row_block(A) * blockDim.x + threadIdx.x;
     const i64 row = gid / L;
     const int lane = static_cast<int>(gid % L);
@@ -86,7 +110,7 @@ __global__ __launch_bounds__(256) void k_spmv(CsrView A, Ptrs<NV> v) {
                 for (int u = 0; u < U; ++u) {
                     c[u] = A.nt ? __builtin_nontemporal_load(A.col + e + u * L) : A.col[e + u * L];
                     vi[u] = 0.0;
-                    value_at<WIDTH, VREAL>(A, e + u * L, vr[u], vi[u]);
+                    value_at<WIDTH, VREAL>(A, d, e + u * L, vr[u], vi[u]);
                 }
                 if constexpr (REAL && NV == 1) {
                     double x[U];
@@ -103,7 +127,7 @@ __global__ __launch_bounds__(256) void k_spmv(CsrView A, Ptrs<NV> v) {
         for (; e < e1; e += L) {
             const u32 c = A.nt ? __builtin_nontemporal_load(A.col + e) : A.col[e];
             double vr, vi = 0.0;
-            value_at<WIDTH, VREAL>(A, e, vr, vi);
+            value_at<WIDTH, VREAL>(A, d, e, vr, vi);
             accumulate(c, vr, vi);
         }
     }
@@ -127,6 +151,8 @@ __global__ __launch_bounds__(256) void k_spmv(CsrView A, Ptrs<NV> v) {
 // Block-interleaved vectors: X[row * NV + k].
 template <int WIDTH, bool VREAL, bool REAL, int L, int NV>
 __global__ __launch_bounds__(256) void k_spmv_block(CsrView A, const void* X, void* Y) {
+    __shared__ double sre[WIDTH == 1 ? 256 : 1], sim[WIDTH == 1 && !VREAL ? 256 : 1];
+    const Dict d = stage_dict<WIDTH, VREAL>(A, sre, sim);
     const i64 gid = row_block(A) * blockDim.x + threadIdx.x;
     const i64 row = gid / L;
     const int lane = static_cast<int>(gid % L);
@@ -138,7 +164,7 @@ __global__ __launch_bounds__(256) void k_spmv_block(CsrView A, const void* X, vo
         for (u64 e = e0 + lane; e < e1; e += L) {
             const u64 c = A.nt ? __builtin_nontemporal_load(A.col + e) : A.col[e];
             double vr, vi = 0.0;
-            value_at<WIDTH, VREAL>(A, e, vr, vi);
+            value_at<WIDTH, VREAL>(A, d, e, vr, vi);
 #pragma unroll
             for (int k = 0; k < NV; ++k) {
                 if constexpr (REAL) {
@@ -182,7 +208,8 @@ template <int WIDTH, bool VREAL, bool REAL, int L> void launch_set(const CsrView
                 p.in[k] = v.in[done + k];
                 p.out[k] = v.out[done + k];
             }
-            if (NV == 1 && A.unroll > 1) hipLaunchKernelGGL((k_spmv<WIDTH, VREAL, REAL, L, NV, 4>), grid, block, 0, st, A, p);
+            if (NV == 1 && A.unroll >= 8) hipLaunchKernelGGL((k_spmv<WIDTH, VREAL, REAL, L, NV, 8>), grid, block, 0, st, A, p);
+            else if (NV == 1 && A.unroll > 1) hipLaunchKernelGGL((k_spmv<WIDTH, VREAL, REAL, L, NV, 4>), grid, block, 0, st, A, p);
             else hipLaunchKernelGGL((k_spmv<WIDTH, VREAL, REAL, L, NV, 1>), grid, block, 0, st, A, p);
         };
         if (nv == 8) go(std::integral_constant<int, 8>{});

@@ -151,7 +151,7 @@ void SectorOperator::autotune() {
     };
     float best1 = 1e30f;
     int l1 = csr_.lanes;
-    for (int L : {4, 8, 16, 32}) {
+    for (int L : {4, 8, 16}) {
         csr_.lanes = L;
         lanes_block_ = L;
         const float t1 = time_it([&] {
@@ -185,8 +185,11 @@ dev::CsrView SectorOperator::csr_view(bool block) const {
     A.lanes = block ? lanes_block_ : csr_.lanes;
     A.values_real = real_;
     A.nt = nt_ ? 1 : 0;
-    A.xcd = xcd_ ? 1 : 0;
-    A.unroll = unroll_ ? 4 : 1;
+    // (block applies keep the dispatcher's workgroup order: their gathers read whole lines already)
+    A.xcd = (xcd_ && !block) ? 1 : 0;
+    A.unroll = unroll_;
+    A.dict_n = static_cast<int>(csr_.dict);
+    A.lds_dict = global_dict_ ? 0 : 1;
     return A;
 }
 

@@ -75,8 +75,10 @@ public:
     bool nontemporal() const { return nt_; }
     /// Give each XCD one contiguous range of rows (workgroup order remapped).
     void set_xcd_rows(bool on) { xcd_ = on; }
-    /// Issue the single-vector SpMV's gathers four entries at a time (the sums are unchanged).
-    void set_unroll(bool on) { unroll_ = on; }
+    /// Issue the single-vector SpMV's gathers 4 or 8 entries at a time, or 1 (the sums are unchanged).
+    void set_unroll(int u) { unroll_ = u; }
+    /// Read u8-id dictionaries from global memory instead of staging them in LDS.
+    void set_global_dict(bool on) { global_dict_ = on; }
     bool xcd_rows() const { return xcd_; }
 
 private:
@@ -90,9 +92,10 @@ private:
     bool real_;
     int lane_ = kLaneGather;
     bool nt_ = false;
-    bool xcd_ = false;
+    bool xcd_ = true;      // single-vector applies: one contiguous row range per XCD
     bool tune_ = true;
-    bool unroll_ = false;
+    int unroll_ = 4;       // single-vector applies: gathers issued this many entries at a time
+    bool global_dict_ = false;
     CsrInfo csr_;
     DeviceBuffer row_ptr_, col_, vals_, dict_re_, dict_im_;
 };
