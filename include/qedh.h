@@ -205,6 +205,17 @@ int qedh_ftlm_batched(const qedh_matvec* blocks, int n_blocks, int samples, int 
 int qedh_cf_batched(const qedh_matvec* blocks, int n_blocks, int R, const void* const* starts, int krylov,
                     int has_shift, double energy_shift, double breakdown_tol, int n_omega, const double* omega,
                     double eta, int use_graph, double* S, double* norm_sq, double* shifts, int* steps, double* stats);
+/* Finite-temperature dynamics of one source sector (ftlm_dynamics_kernel,
+ * include/ed/dynamics/ftlm_dynamics.h:110-397, for an autocorrelation A = B): samples from
+ * splitmix64(seed + phi (s + 1)) (seed: the source's block seed), Lanczos with kept bases on the
+ * source and on every target (full CGS2 up to max(4 krylov, 2^16) states, LocalDGKS3 above),
+ * probes[t] from the source to target t (its H: target_h[t]). Outputs, un-normalised (times the
+ * source dimension): Z[n_t], S[n_targets][n_t][n_omega] complex (re, im); *e_min the samples'
+ * common reference; *samples_done. */
+int qedh_ftlm_dynamics(qedh_matvec source_h, int n_targets, const qedh_matvec* target_h, const qedh_matvec* probes,
+                       int samples, int krylov, uint64_t seed, double breakdown_tol, int n_t,
+                       const double* temperatures, int n_omega, const double* omega, double eta, double* Z,
+                       double* S, double* e_min, int* samples_done);
 /* Exact thermodynamics of small blocks (the reference's host dense lane for blocks of at most
  * dense_max_dim states, src/engine/thermal.cpp:197-200, curves of include/ed/thermal/curves.h:38-65):
  * each block's CSR expanded to a dense Hermitian matrix on the host and its whole spectrum solved.

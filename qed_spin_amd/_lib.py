@@ -109,6 +109,10 @@ _SIGNATURES = {
     ],
     "qedh_serve": [C.c_void_p, C.c_int, C.c_void_p, C.c_int, c_dbl_p, c_dbl_p, c_dbl_p, c_int_p, c_i64_p, c_dbl_p],
     "qedh_request_footprint": [C.c_void_p, C.c_int, c_u64_p],
+    "qedh_ftlm_dynamics": [
+        C.c_void_p, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_uint64, C.c_double,
+        C.c_int, c_dbl_p, C.c_int, c_dbl_p, C.c_double, c_dbl_p, c_dbl_p, c_dbl_p, c_int_p,
+    ],
     "qedh_exact_thermal": [
         C.POINTER(C.c_void_p), C.c_int, C.c_int, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p,
     ],

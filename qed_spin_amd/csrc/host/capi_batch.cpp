@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "batched.h"
+#include "ftlm_dynamics.h"
 #include "serve.h"
 #include "tridiag.h"
 
@@ -336,5 +337,45 @@ extern "C" int qedh_request_footprint(const qedh_request_t* req, int batched, ui
         qsa::ServeOptions o;
         o.batched = batched != 0;
         *bytes = qsa::request_footprint(r, o);
+    });
+}
+
+extern "C" int qedh_ftlm_dynamics(qedh_matvec source_h, int n_targets, const qedh_matvec* target_h,
+                                  const qedh_matvec* probes, int samples, int krylov, uint64_t seed,
+                                  double breakdown_tol, int n_t, const double* temperatures, int n_omega,
+                                  const double* omega, double eta, double* Z, double* S, double* e_min,
+                                  int* samples_done) {
+    return guarded_batch([&] {
+        if (!source_h || n_targets < 0 || (n_targets > 0 && (!target_h || !probes)) || samples < 1 || krylov < 1 ||
+            n_t < 1 || !temperatures || n_omega < 1 || !omega || !Z || (n_targets > 0 && !S))
+            throw qsa::Error(qsa::kInvalid, "qedh_ftlm_dynamics: bad arguments");
+        const auto& Hs = *source_h->op;
+        if (!Hs.same()) throw qsa::Error(qsa::kInvalid, "qedh_ftlm_dynamics: the source operator must be square");
+        std::vector<qsa::FtlmDynTarget> tg;
+        for (int t = 0; t < n_targets; ++t) {
+            if (!target_h[t] || !probes[t]) throw qsa::Error(qsa::kInvalid, "qedh_ftlm_dynamics: null target");
+            const auto& Ht = *target_h[t]->op;
+            const auto& A = *probes[t]->op;
+            if (A.col_dim() != Hs.dim() || A.dim() != Ht.dim() || !Ht.same())
+                throw qsa::Error(qsa::kInvalid, "qedh_ftlm_dynamics: a probe does not map the source to its target");
+            tg.push_back({&Ht, &A});
+        }
+        qsa::FtlmDynOptions o;
+        o.krylov = static_cast<std::size_t>(krylov);
+        o.samples = static_cast<std::size_t>(samples);
+        o.breakdown_tol = breakdown_tol;
+        o.eta = eta;
+        o.seed = seed;
+        o.temperatures.assign(temperatures, temperatures + n_t);
+        o.omega.assign(omega, omega + n_omega);
+        const qsa::FtlmDynResult r = qsa::ftlm_dynamics_source(Hs, tg, o);
+        for (int i = 0; i < n_t; ++i) Z[i] = r.Z[i];
+        for (int t = 0; t < n_targets; ++t)
+            for (std::size_t q = 0; q < r.S[t].size(); ++q) {
+                S[(static_cast<std::size_t>(t) * r.S[t].size() + q) * 2] = r.S[t][q].real();
+                S[(static_cast<std::size_t>(t) * r.S[t].size() + q) * 2 + 1] = r.S[t][q].imag();
+            }
+        if (e_min) *e_min = r.E_min;
+        if (samples_done) *samples_done = static_cast<int>(r.samples_done);
     });
 }

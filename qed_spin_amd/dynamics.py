@@ -126,16 +126,94 @@ class _lib_alloc:
             self.ptr = None
 
 
+def block_seed(base: int, key: int) -> int:
+    """splitmix64 of base + phi (key + 1): a source sector's seed (include/ed/thermal/sample_seed.h:48-53)."""
+    m = (1 << 64) - 1
+    z = (int(base) + 0x9E3779B97F4A7C15 * (int(key) + 1)) & m
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & m
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & m
+    return z ^ (z >> 31)
+
+
+def _reach(blocks, b, O, delta):
+    """(target index, probe operator on the device) for every block the probe reaches from block b."""
+    blk_b, sec_b, _ = blocks[b]
+    out = []
+    for t, (blk_t, sec_t, _) in enumerate(blocks):
+        if sec_b.n_up >= 0 and sec_t.n_up != sec_b.n_up + delta:
+            continue
+        tgt = sec_t if t != b else Sector(blk_t.group, blk_t.n_up)
+        X = SectorMatvec(O, tgt, sec_b)
+        if X.program.counts()[2] == 0:
+            continue
+        out.append((t, X))
+    return out
+
+
+def _dynamics_finite_T(H, probes, omega, T, eta, blocks, krylov, samples, seed, sources):
+    """FTLM over every source sector (dynamics.cpp:693-1257): per source its samples' Z and S from
+    qedh_ftlm_dynamics, then S_O(omega, T) = sum_src w_src S_src / sum_src w_src Z_src with
+    w_src = e^{-beta (emin_src - ref)}."""
+    T = np.ascontiguousarray(np.atleast_1d(T), np.float64)
+    nT, nW, P = T.size, omega.size, len(probes)
+    deltas = [_delta_up(O) for O in probes]
+    s0 = int(seed) if seed else int.from_bytes(__import__("os").urandom(8), "little")
+    n_irr = {}
+    for blk, _, _ in blocks:
+        n_irr[blk.n_up] = max(n_irr.get(blk.n_up, 0), (blk.momentum or 0) + 1)
+    tol = KBREAKDOWN_REL * (H.norm_bound() or 1.0)
+    per_src = []
+    for b in (range(len(blocks)) if sources is None else sources):
+        blk, sec, m = blocks[b]
+        irr = blk.group.G if blk.momentum is not None else 1
+        key = ((sec.n_up + 1) * 3 + 0) * irr + (blk.momentum or 0)
+        reach = []
+        for p, O in enumerate(probes):
+            for t, X in _reach(blocks, b, O, deltas[p]):
+                reach.append((p, t, X))
+        nt = len(reach)
+        th = (C.c_void_p * max(nt, 1))(*[blocks[t][2].h.value for _, t, _ in reach] or [None])
+        ph = (C.c_void_p * max(nt, 1))(*[X.h.value for _, _, X in reach] or [None])
+        Z = np.zeros(nT)
+        S = np.zeros((max(nt, 1), nT, nW, 2))
+        emin, done = C.c_double(), C.c_int()
+        _lib.call("qedh_ftlm_dynamics", m.h, nt, th, ph, int(samples), int(krylov), C.c_uint64(block_seed(s0, key)),
+                  C.c_double(tol), nT, _lib.ptr(T, C.c_double), nW, _lib.ptr(omega, C.c_double), C.c_double(eta),
+                  _lib.ptr(Z, C.c_double), _lib.ptr(S, C.c_double), C.byref(emin), C.byref(done))
+        Sp = np.zeros((P, nT, nW), np.complex128)
+        for q, (p, _, _) in enumerate(reach):
+            Sp[p] += S[q, :, :, 0] + 1j * S[q, :, :, 1]
+        per_src.append((emin.value, Z, Sp))
+    ref = min(e for e, _, _ in per_src)
+    out = []
+    for p in range(P):
+        Sres = np.zeros((nT, nW), np.complex128)
+        for it in range(nT):
+            beta = 1.0 / T[it]
+            Zp = 0.0
+            for e, Z, Sp in per_src:
+                w = math.exp(-beta * (e - ref))
+                Zp += w * Z[it]
+                Sres[it] += w * Sp[p, it]
+            Sres[it] /= Zp
+        out.append(Sres)
+    return out, ref
+
+
 def dynamics(H: Operator, probes, omega, *, eta: float = 0.1, T=None, sym: Optional[Symmetry] = None,
              krylov: int = 200, device: str = "gpu", degeneracy_tol: float = 1e-8, use_graph: bool = True,
-             csr_budget: int = DEFAULT_CSR_BUDGET, blocks=None, ground_blocks=None) -> DynamicsResult:
+             csr_budget: int = DEFAULT_CSR_BUDGET, blocks=None, ground_blocks=None, samples: int = 30,
+             seed: int = 0, sources=None) -> DynamicsResult:
     """S_O(omega) = sum_n |<n|O|g>|^2 L_eta(omega - E_n + E0), averaged over the ground manifold,
     for every probe O (an Operator or a list of them). blocks: the walk's (block, sector, matvec)
     triples when the caller already built them (thermal.walk_blocks). ground_blocks: indices of the
-    blocks known to hold the ground manifold (default: every block is searched)."""
+    blocks known to hold the ground manifold (default: every block is searched).
+
+    T (a list of temperatures > 0): finite-temperature FTLM dynamics instead, `samples` random
+    vectors per source sector from the seed (0 draws one); S[p] is then (len(T), len(omega)),
+    complex (the autocorrelation's imaginary part is roundoff). sources: the source blocks
+    (default: every block of the walk)."""
     resolve_device(device, "dynamics")
-    if T is not None and len(np.atleast_1d(T)) > 0:
-        raise DeviceUnsupported("dynamics: T > 0 (FTLM dynamics) is not on the device path")
     probes = [probes] if isinstance(probes, Operator) else list(probes)
     if not probes:
         raise InvalidRequest("dynamics: no probe")
@@ -149,6 +227,12 @@ def dynamics(H: Operator, probes, omega, *, eta: float = 0.1, T=None, sym: Optio
     if not blocks:
         raise InvalidRequest("dynamics: the symmetry selection holds no state")
     t_walk = time.perf_counter() - t0
+    if T is not None and len(np.atleast_1d(T)) > 0:
+        if np.any(~(np.atleast_1d(T) > 0)):
+            raise InvalidRequest("dynamics: temperatures must be positive")
+        S, ref = _dynamics_finite_T(H, probes, omega, T, eta, blocks, krylov, samples, seed, sources)
+        return DynamicsResult(omega=omega, S=S, e0=ref, ground_manifold=0, target_sectors=0,
+                              timings={"walk_s": t_walk, "total_s": time.perf_counter() - t0})
     sH = H.norm_bound() or 1.0
     window = degeneracy_tol * sH
     t1 = time.perf_counter()

@@ -388,3 +388,123 @@ def test_serve_results_same_alone_and_in_the_mix():
         ev = np.linalg.eigvalsh(E.conj().T @ po.dense_matrix(H.products, N, states) @ E)
         k = len(mix.values[i])
         assert np.allclose(mix.values[i], ev[:k], rtol=1e-10, atol=1e-10), i
+
+
+def test_dynamics_finite_T_reproduces_reference_record():
+    """FTLM dynamics at T = 1 of j1j2_chain12 with no symmetry (one 4096-state source, 6 samples,
+    krylov 40, seed 11: the source's seed is block_seed(11, 0)) against the reference's record."""
+    from qed_spin_amd import dynamics as dy
+
+    say("dynamics T=1 vs reference record")
+    rec = records()
+    dd = rec["dynamics"]
+    omega = np.array(dd["omega"])
+    N = 12
+    O = qs.Operator(N)
+    for i in range(N):
+        O.add_product("z", (i,), np.exp(1j * np.pi * i) / np.sqrt(N))
+    want = np.array(rec["records"]["j1j2_chain12/api/dynamics/T=1/none/SzQ"]["values"]["S"])
+    r = dy.dynamics(j1j2(N), O, omega, eta=dd["eta"], T=[1.0], sym=qs.Symmetry.none(), krylov=dd["krylov_T"],
+                    samples=dd["samples"], seed=dd["seed"])
+    got = r.S[0][0]
+    err = np.abs(got.real - want).max()
+    print(f"  max |diff| {err:.3e}, max |imag| {np.abs(got.imag).max():.3e}")
+    assert np.allclose(got.real, want, rtol=1e-8, atol=1e-8)
+
+
+def test_dynamics_finite_T_symmetric_walk_matches_oracle():
+    """The same estimator over every momentum sector of Sz = 0 (N = 10, S^z(q) probes) against the
+    oracle's restatement of the kernel on dense block matrices with the same seeds."""
+    from qed_spin_amd import dynamics as dy
+
+    say("dynamics T>0 walk vs oracle")
+    N, samples, krylov, seed, eta = 10, 3, 12, 5, 0.2
+    H = chiral_j1j2(N, 0.2)
+    omega = np.linspace(-1.0, 4.0, 21)
+    Ts = np.array([0.7, 1.5])
+    probes = [dy.sz_q(N, 2 * np.pi * 3 / N)]
+    r = dy.dynamics(H, probes, omega, eta=eta, T=Ts, sym=qs.Symmetry(sz=[N // 2]), krylov=krylov, samples=samples,
+                    seed=seed)
+    states = np.array([x for x in range(1 << N) if bin(x).count("1") == N // 2], np.uint64)
+    Hd = po.dense_matrix(H.products, N, states)
+    Od = po.dense_matrix(probes[0].products, N, states)
+    Es = [po.sector_basis_matrix(po.OracleSector(N, N // 2, sym.translations(N), sym.momentum_chars(N, k)), states)
+          for k in range(N)]
+    Hb = [E.conj().T @ Hd @ E for E in Es]
+    tol = EPS64 * H.norm_bound()
+    per = []
+    for k in range(N):
+        n = Es[k].shape[1]
+        key = ((N // 2 + 1) * 3) * N + k
+        bseed = dy.block_seed(seed, key)
+        targets = []
+        for t in range(N):
+            A = Es[t].conj().T @ Od @ Es[k]
+            if np.abs(A).max() > 1e-12:
+                targets.append((t, A))
+        Zs, Ss, smins = [], [], []
+        for s in range(samples):
+            v0 = po.gaussian_vector(bseed, s, n)
+            a, b, V = _lanczos_basis(Hb[k], v0, min(n, krylov), tol)
+            ev, U = np.linalg.eigh(np.diag(a) + np.diag(b[1:len(a)], 1) + np.diag(b[1:len(a)], -1))
+            c = U[0]
+            smin = ev.min()
+            wt = np.exp(-np.outer(1.0 / Ts, ev - smin))
+            Z = (wt * c ** 2).sum(1)
+            S = np.zeros((len(Ts), omega.size), np.complex128)
+            for t, A in targets:
+                phi = A @ v0
+                nphi = np.linalg.norm(phi)
+                if nphi < 1e-14:
+                    continue
+                a2, b2, V2 = _lanczos_basis(Hb[t], phi / nphi, min(Hb[t].shape[0], krylov), tol)
+                lam, US = np.linalg.eigh(np.diag(a2) + np.diag(b2[1:len(a2)], 1) + np.diag(b2[1:len(a2)], -1))
+                W = (A @ V).conj().T @ V2                  # W[a, b] = <A v_a | w_b>
+                Tm = U.T @ W                                 # [i, b]
+                obar = Tm @ US                               # [i, j]
+                w_ij = c[:, None] * obar * (nphi * US[0][None, :])
+                E_ij = lam[None, :] - ev[:, None]
+                L = (eta / np.pi) / ((omega[None, None, :] - E_ij[:, :, None]) ** 2 + eta ** 2)
+                si = (w_ij[:, :, None] * L).sum(1)           # [i, w]
+                S += wt @ si
+            Zs.append(Z)
+            Ss.append(S)
+            smins.append(smin)
+        E_min = min(smins)
+        Zt = sum(np.exp(-(sm - E_min) / Ts) * Z for sm, Z in zip(smins, Zs)) * n / samples
+        St = sum(np.exp(-(sm - E_min) / Ts)[:, None] * S for sm, S in zip(smins, Ss)) * n / samples
+        per.append((E_min, Zt, St))
+    ref = min(e for e, _, _ in per)
+    want = np.zeros((len(Ts), omega.size), np.complex128)
+    for it, Tv in enumerate(Ts):
+        Zp = sum(np.exp(-(e - ref) / Tv) * Z[it] for e, Z, _ in per)
+        want[it] = sum(np.exp(-(e - ref) / Tv) * S[it] for e, _, S in per) / Zp
+    got = r.S[0]
+    print(f"  max |diff| {np.abs(got - want).max():.3e}")
+    assert np.allclose(got, want, rtol=1e-8, atol=1e-8)
+
+
+def _lanczos_basis(A, v0, m, tol):
+    """Lanczos with full classical Gram-Schmidt twice (the dense restatement): alpha, beta (beta[0]
+    = 0), the basis as columns."""
+    n = A.shape[0]
+    V = np.zeros((n, m), np.complex128)
+    alpha, beta = [], [0.0]
+    v = v0 / np.linalg.norm(v0)
+    V[:, 0] = v
+    for j in range(m):
+        w = A @ V[:, j]
+        if j > 0:
+            w = w - beta[j] * V[:, j - 1]
+        a = np.vdot(V[:, j], w).real
+        alpha.append(a)
+        w = w - a * V[:, j]
+        for _ in range(2):
+            w = w - V[:, : j + 1] @ (V[:, : j + 1].conj().T @ w)
+        b = np.linalg.norm(w)
+        beta.append(b)
+        if b < tol or j + 1 >= m:
+            break
+        V[:, j + 1] = w / b
+    k = len(alpha)
+    return np.array(alpha), np.array(beta), V[:, :k]
