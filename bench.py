@@ -156,22 +156,36 @@ def run_dynamics(args):
 
 
 def run_serve(args):
-    """The mixed request stream served (slots, HBM admission, resident sectors, batched FTLM)
-    against the plain path (one request after another, no batching, no cache)."""
+    """The mixed request stream: served at saturation (every request there at 0: solves/s), served
+    with Poisson arrivals (latency p50/p99), and the plain path (one request after another, no
+    batching, no sector cache) on the same stream's first requests against the served path on the
+    same prefix."""
     from qed_spin_amd import serve as sv
 
+    def brief(st):
+        keys = ("wall_s", "solves_per_s", "p50_s", "p99_s", "mean_s", "completed", "failed", "peak_in_flight",
+                "peak_reserved", "sector_builds", "sector_hits")
+        return {k: (round(st[k], 4) if isinstance(st[k], float) else st[k]) for k in keys}
+
     reqs = sv.mixed_stream(n=args.serve_requests, rate=args.serve_rate, seed=3, n_max=args.serve_nmax)
-    served = sv.serve(reqs, slots=args.serve_slots, realtime=True)
-    out = {"workload": f"{len(reqs)} mixed requests (eigs/thermal/dynamics 50/30/20), N in 16..{args.serve_nmax}, "
-                       f"Poisson {args.serve_rate}/s, seed 3",
-           "served": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in served.stats.items()}}
-    if not args.no_plain:
-        plain = sv.serve(reqs, slots=1, batched=False, cache_sectors=False, realtime=False)
-        sat = sv.serve(reqs, slots=args.serve_slots, realtime=False)
-        out["plain"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in plain.stats.items()}
-        out["served_saturated"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in sat.stats.items()}
-        out["speedup_solves_per_s"] = round(sat.stats["solves_per_s"] / plain.stats["solves_per_s"], 3)
-    log(f"serve: {out}")
+    out = {"workload": f"{len(reqs)} mixed requests (eigs k=4 / thermal FTLM 20x80 / dynamics S^z(pi) 150, "
+                       f"50/30/20), N in 16..{args.serve_nmax}, seed 3",
+           "slots": args.serve_slots}
+    sat = sv.serve(reqs, slots=args.serve_slots, realtime=False)
+    out["served_saturated"] = brief(sat.stats)
+    log(f"serve saturated: {out['served_saturated']}")
+    if args.serve_realtime_requests > 0:
+        rt = sv.serve(reqs[: args.serve_realtime_requests], slots=args.serve_slots, realtime=True)
+        out["served_poisson"] = dict(brief(rt.stats), rate_per_s=args.serve_rate, requests=args.serve_realtime_requests)
+        log(f"serve poisson: {out['served_poisson']}")
+    if args.serve_plain_requests > 0:
+        m = args.serve_plain_requests
+        plain = sv.serve(reqs[:m], slots=1, batched=False, cache_sectors=False, realtime=False)
+        same = sv.serve(reqs[:m], slots=args.serve_slots, realtime=False)
+        out["plain_prefix"] = dict(brief(plain.stats), requests=m)
+        out["served_prefix"] = dict(brief(same.stats), requests=m)
+        out["speedup_solves_per_s"] = round(same.stats["solves_per_s"] / plain.stats["solves_per_s"], 3)
+        log(f"serve plain vs served on {m}: {out['plain_prefix']} / {out['served_prefix']}")
     return out
 
 
@@ -199,10 +213,11 @@ def main():
     ap.add_argument("--ftlm-sites", type=int, default=24)
     ap.add_argument("--dyn-sites", type=int, default=28)
     ap.add_argument("--serve-requests", type=int, default=2000)
-    ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-rate", type=float, default=5.0)
     ap.add_argument("--serve-slots", type=int, default=8)
     ap.add_argument("--serve-nmax", type=int, default=30)
-    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--serve-plain-requests", type=int, default=200)
+    ap.add_argument("--serve-realtime-requests", type=int, default=0)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 1 GiB device to device (the known-size kernel the HBM counters are scaled by)")
     args = ap.parse_args()
