@@ -11,6 +11,11 @@ The apply time is measured with HIP events on the stream the SpMV runs on, over 
 applies after --warmup untimed ones.
 
     python bench.py [--gpus 1] [--steps 200] [--warmup 20] [--no-lanczos] [--no-cpu]
+                    [--workload matvec,serve,ftlm,dynamics] [--serve-requests 2000 --serve-plain-requests 200]
+
+By default the line also carries a sample of the served stream ("serve": the first 200 requests of
+the mixed stream served at saturation, and the plain one-request-at-a-time path against the served
+path on its first 40).
 """
 
 from __future__ import annotations
@@ -171,17 +176,17 @@ def run_serve(args):
     out = {"workload": f"{len(reqs)} mixed requests (eigs k=4 / thermal FTLM 20x80 / dynamics S^z(pi) 150, "
                        f"50/30/20), N in 16..{args.serve_nmax}, seed 3",
            "slots": args.serve_slots}
-    sat = sv.serve(reqs, slots=args.serve_slots, realtime=False)
+    sat = sv.serve(reqs, slots=args.serve_slots, realtime=False, progress_s=20.0)
     out["served_saturated"] = brief(sat.stats)
     log(f"serve saturated: {out['served_saturated']}")
     if args.serve_realtime_requests > 0:
-        rt = sv.serve(reqs[: args.serve_realtime_requests], slots=args.serve_slots, realtime=True)
+        rt = sv.serve(reqs[: args.serve_realtime_requests], slots=args.serve_slots, realtime=True, progress_s=20.0)
         out["served_poisson"] = dict(brief(rt.stats), rate_per_s=args.serve_rate, requests=args.serve_realtime_requests)
         log(f"serve poisson: {out['served_poisson']}")
     if args.serve_plain_requests > 0:
         m = args.serve_plain_requests
-        plain = sv.serve(reqs[:m], slots=1, batched=False, cache_sectors=False, realtime=False)
-        same = sv.serve(reqs[:m], slots=args.serve_slots, realtime=False)
+        plain = sv.serve(reqs[:m], slots=1, batched=False, cache_sectors=False, realtime=False, progress_s=20.0)
+        same = sv.serve(reqs[:m], slots=args.serve_slots, realtime=False, progress_s=20.0)
         out["plain_prefix"] = dict(brief(plain.stats), requests=m)
         out["served_prefix"] = dict(brief(same.stats), requests=m)
         out["speedup_solves_per_s"] = round(same.stats["solves_per_s"] / plain.stats["solves_per_s"], 3)
@@ -208,15 +213,15 @@ def main():
     ap.add_argument("--no-lanczos", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--lanes", type=int, default=0)
-    ap.add_argument("--workload", default="matvec",
-                    help="matvec (the headline), or a comma list adding ftlm, dynamics, serve")
+    ap.add_argument("--workload", default="matvec,serve",
+                    help="matvec (the headline) and a comma list of ftlm, dynamics, serve")
     ap.add_argument("--ftlm-sites", type=int, default=24)
     ap.add_argument("--dyn-sites", type=int, default=28)
-    ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-requests", type=int, default=200)
     ap.add_argument("--serve-rate", type=float, default=5.0)
     ap.add_argument("--serve-slots", type=int, default=8)
     ap.add_argument("--serve-nmax", type=int, default=30)
-    ap.add_argument("--serve-plain-requests", type=int, default=200)
+    ap.add_argument("--serve-plain-requests", type=int, default=40)
     ap.add_argument("--serve-realtime-requests", type=int, default=0)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 1 GiB device to device (the known-size kernel the HBM counters are scaled by)")

@@ -126,15 +126,19 @@ int qedh_matvec_apply_block(qedh_matvec m, int nv, const void* X, void* Y, int c
 int qedh_matvec_set_lanes(qedh_matvec m, int lanes);
 /* SpMV tuning (no counterpart in the reference): lanes per row (0 keeps the current) and flags:
  * non-temporal streaming of the CSR arrays, one contiguous row range per XCD, the single-vector
- * gathers issued four (or eight) entries at a time, u8-id dictionaries read from global memory
- * instead of LDS (every flag leaves the results bit for bit unchanged). The default is XCD rows,
- * unroll 4, LDS dictionary; flags replace it. */
+ * gathers issued two, four or eight entries at a time, u8-id dictionaries read from global memory
+ * instead of LDS, 256 or 1024 threads per workgroup instead of 512 (every flag leaves the results
+ * bit for bit unchanged). The default is XCD rows, unroll 4 (or what the autotuner found), LDS
+ * dictionary, 512 threads; flags replace it. */
 enum {
     QEDH_TUNE_NONTEMPORAL = 1,
     QEDH_TUNE_XCD_ROWS = 2,
     QEDH_TUNE_UNROLL = 4,
     QEDH_TUNE_UNROLL8 = 8,
-    QEDH_TUNE_GLOBAL_DICT = 16
+    QEDH_TUNE_GLOBAL_DICT = 16,
+    QEDH_TUNE_WG256 = 32,
+    QEDH_TUNE_WG1024 = 64,
+    QEDH_TUNE_UNROLL2 = 128
 };
 int qedh_matvec_set_tuning(qedh_matvec m, int lanes, int flags);
 /* The merged CSR (download_csr): row_ptr[rows+1], col[nnz], val[2 nnz] (re, im). */
@@ -254,6 +258,7 @@ typedef struct {
     int n_omega;
     const double* omega;
     double eta;
+    double progress_s;  /* > 0: a progress line on stderr at most this often */
 } qedh_serve_options_t;
 /* Run the stream. values[n][stride]: eigs levels / thermal lnZ, E, V per beta / dynamics S(omega);
  * e0[n]; timing[n][4] = arrival, start, finish, latency (s); status[n] (0 ok); dims[n];

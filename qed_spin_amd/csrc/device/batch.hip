@@ -109,6 +109,44 @@ __device__ __forceinline__ void tile_partial(const BatchView& B, double pr, doub
     }
 }
 
+// The SpMV of R >= 8 with the first recurrence pass in its epilogue: W = H V_cur - beta V_prev
+// (beta of this step; skipped at step 0) and the partial of <V_cur, W>, the same arithmetic as
+// batch_spmv followed by batch_vec mode 0, without writing and re-reading W in between.
+__global__ __launch_bounds__(kThreads) void k_batch_spmv_wide_fused(BatchView B, const Zd* Vprev, const Zd* Vcur,
+                                                                    Zd* W, const double* beta, double* partial) {
+    const Tile t = B.tiles[blockIdx.x];
+    const int R = B.R, slots = kThreads / R;
+    const int slot = threadIdx.x / R, s = threadIdx.x % R;
+    const bool live = slot < slots && pair_live(B, t.block, s);
+    double pr = 0.0, pi = 0.0;
+    if (live) {
+        const int step = *B.step;
+        const double b = beta[static_cast<u64>(t.block) * R + s];
+        for (u64 r = t.row_begin + slot; r < t.row_end; r += slots) {
+            const u64 e0 = B.row_ptr[r], e1 = B.row_ptr[r + 1];
+            double ar = 0.0, ai = 0.0;
+            for (u64 e = e0; e < e1; ++e) {
+                const u64 c = B.col[e];
+                const Zd v = B.val[e];
+                const Zd x = Vcur[c * R + s];
+                ar += v.re * x.re - v.im * x.im;
+                ai += v.re * x.im + v.im * x.re;
+            }
+            const u64 i = r * R + s;
+            Zd w{ar, ai};
+            if (step > 0) {
+                const Zd p = Vprev[i];
+                w = Zd{w.re - b * p.re, w.im - b * p.im};
+            }
+            W[i] = w;
+            const Zd d = cdot(Vcur[i], w);
+            pr += d.re;
+            pi += d.im;
+        }
+    }
+    tile_partial(B, pr, pi, partial);
+}
+
 // mode 0: W -= beta V_prev (skipped at step 0), then partial of <V_cur, W> (alpha);
 // mode 1: W -= alpha V_cur, then partial of <V_cur, W> (the local overlap);
 // mode 4: W -= alpha V_cur, then partial of ||W||^2 (no reorthogonalisation);
@@ -342,6 +380,15 @@ void batch_spmv(const BatchView& B, const Zd* X, Zd* Y, hipStream_t st) {
     else if (B.R == 2) hipLaunchKernelGGL(k_batch_spmv_narrow<2>, grid, block, 0, st, B, X, Y);
     else if (B.R <= 4) hipLaunchKernelGGL(k_batch_spmv_narrow<4>, grid, block, 0, st, B, X, Y);
     else hipLaunchKernelGGL(k_batch_spmv_narrow<8>, grid, block, 0, st, B, X, Y);
+}
+
+bool batch_spmv_fused(const BatchView& B, const Zd* Vprev, const Zd* Vcur, Zd* W, const double* beta, double* partial,
+                      hipStream_t st) {
+    if (B.R < 8) return false;
+    if (!B.n_tiles) return true;
+    hipLaunchKernelGGL(k_batch_spmv_wide_fused, dim3(B.n_tiles), dim3(kThreads), 0, st, B, Vprev, Vcur, W, beta,
+                       partial);
+    return true;
 }
 
 void batch_vec(const BatchView& B, int mode, const Zd* Vprev, const Zd* Vcur, Zd* W, const double* coef,

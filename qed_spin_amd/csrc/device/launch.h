@@ -72,6 +72,7 @@ struct CsrView {
     int unroll = 1;                // single-vector SpMV: gathers issued 4 or 8 entries at a time (same sums)
     int dict_n = 0;                // dictionary entries (u8 / u16 ids)
     int lds_dict = 1;              // u8 ids: stage the dictionary in LDS
+    int wg = 256;                  // single-vector SpMV: threads per workgroup (256, 512, 1024)
 };
 /// out_v = A in_v (real: double vectors; else complex vectors, with real or complex values).
 void csr_spmv(const CsrView& A, bool real, const VecSet& v, hipStream_t st);
@@ -141,6 +142,10 @@ struct BatchView {
 constexpr int kBatchTileRows = 256;
 constexpr int kBatchMaxR = 64;
 void batch_spmv(const BatchView& B, const Zd* X, Zd* Y, hipStream_t st);
+/// W = H V_cur - beta V_prev with the partials of <V_cur, W> (batch_spmv + batch_vec mode 0 in one
+/// kernel, the same arithmetic); false (nothing launched) when R < 8.
+bool batch_spmv_fused(const BatchView& B, const Zd* Vprev, const Zd* Vcur, Zd* W, const double* beta, double* partial,
+                      hipStream_t st);
 /// mode 0: W -= beta V_prev, <V_cur, W>; 1: W -= alpha V_cur, <V_cur, W>; 4: W -= alpha V_cur,
 /// ||W||^2; 2: local reorthogonalisation, ||W||^2; 3: W *= 1 / beta. Partials per tile and vector.
 void batch_vec(const BatchView& B, int mode, const Zd* Vprev, const Zd* Vcur, Zd* W, const double* coef,

@@ -69,14 +69,17 @@ __device__ __forceinline__ i64 row_block(const CsrView& A) {
     return static_cast<i64>(blockIdx.x & 7u) * per + (blockIdx.x >> 3);
 }
 
+inline unsigned block_threads(const CsrView& A) { return (A.wg == 512 || A.wg == 1024) ? A.wg : 256; }
+
 inline unsigned grid_blocks(const CsrView& A, u64 threads) {
-    u64 b = (threads + 255) / 256;
+    const u64 t = block_threads(A);
+    u64 b = (threads + t - 1) / t;
     if (A.xcd) b = (b + 7) / 8 * 8;
     return static_cast<unsigned>(b);
 }
 
 template <int WIDTH, bool VREAL, bool REAL, int L, int NV, int U>
-__global__ __launch_bounds__(256) void k_spmv(CsrView A, Ptrs<NV> v) {
+__global__ __launch_bounds__(1024) void k_spmv(CsrView A, Ptrs<NV> v) {
     __shared__ double sre[WIDTH == 1 ? 256 : 1], sim[WIDTH == 1 && !VREAL ? 256 : 1];
     const Dict d = stage_dict<WIDTH, VREAL>(A, sre, sim);
     const i64 gid = row_block(A) * blockDim.x + threadIdx.x;
@@ -196,7 +199,7 @@ __global__ __launch_bounds__(256) void k_spmv_block(CsrView A, const void* X, vo
 
 template <int WIDTH, bool VREAL, bool REAL, int L> void launch_set(const CsrView& A, const VecSet& v, hipStream_t st) {
     const u64 threads = static_cast<u64>(A.rows) * L;
-    const dim3 grid(grid_blocks(A, threads)), block(256);
+    const dim3 grid(grid_blocks(A, threads)), block(block_threads(A));
     int done = 0;
     while (done < v.k) {
         const int rest = v.k - done;
@@ -209,7 +212,8 @@ template <int WIDTH, bool VREAL, bool REAL, int L> void launch_set(const CsrView
                 p.out[k] = v.out[done + k];
             }
             if (NV == 1 && A.unroll >= 8) hipLaunchKernelGGL((k_spmv<WIDTH, VREAL, REAL, L, NV, 8>), grid, block, 0, st, A, p);
-            else if (NV == 1 && A.unroll > 1) hipLaunchKernelGGL((k_spmv<WIDTH, VREAL, REAL, L, NV, 4>), grid, block, 0, st, A, p);
+            else if (NV == 1 && A.unroll >= 4) hipLaunchKernelGGL((k_spmv<WIDTH, VREAL, REAL, L, NV, 4>), grid, block, 0, st, A, p);
+            else if (NV == 1 && A.unroll > 1) hipLaunchKernelGGL((k_spmv<WIDTH, VREAL, REAL, L, NV, 2>), grid, block, 0, st, A, p);
             else hipLaunchKernelGGL((k_spmv<WIDTH, VREAL, REAL, L, NV, 1>), grid, block, 0, st, A, p);
         };
         if (nv == 8) go(std::integral_constant<int, 8>{});
@@ -232,8 +236,10 @@ template <int WIDTH, bool VREAL, bool REAL> void pick_lanes(const CsrView& A, co
 
 template <int WIDTH, bool VREAL, bool REAL, int L>
 void launch_block(const CsrView& A, int nv, const void* X, void* Y, hipStream_t st) {
+    CsrView B = A;
+    B.wg = 256;
     const u64 threads = static_cast<u64>(A.rows) * L;
-    const dim3 grid(grid_blocks(A, threads)), block(256);
+    const dim3 grid(grid_blocks(B, threads)), block(256);
     switch (nv) {
     case 1: hipLaunchKernelGGL((k_spmv_block<WIDTH, VREAL, REAL, L, 1>), grid, block, 0, st, A, X, Y); break;
     case 2: hipLaunchKernelGGL((k_spmv_block<WIDTH, VREAL, REAL, L, 2>), grid, block, 0, st, A, X, Y); break;

@@ -261,8 +261,10 @@ void BatchedLanczos::run(int M, const std::vector<double>& breakdown_tol, bool l
         dev::Zd* p = V[phase % 3];
         dev::Zd* c = V[(phase + 1) % 3];
         dev::Zd* w = V[(phase + 2) % 3];
-        dev::batch_spmv(B, c, w, st);
-        dev::batch_vec(B, 0, p, c, w, cb, nullptr, 0.0, partial, st);
+        if (!dev::batch_spmv_fused(B, p, c, w, cb, partial, st)) {
+            dev::batch_spmv(B, c, w, st);
+            dev::batch_vec(B, 0, p, c, w, cb, nullptr, 0.0, partial, st);
+        }
         dev::batch_finish(B, 0, partial, ah, bh, ov, ca, cb, ib, tol, st);
         if (local) {
             dev::batch_vec(B, 1, nullptr, c, w, ca, nullptr, 0.0, partial, st);

@@ -299,6 +299,7 @@ extern "C" int qedh_serve(const qedh_request_t* reqs, int n, const qedh_serve_op
         if (opt->n_beta > 0 && opt->betas) o.betas.assign(opt->betas, opt->betas + opt->n_beta);
         if (opt->n_omega > 0 && opt->omega) o.omega.assign(opt->omega, opt->omega + opt->n_omega);
         o.eta = opt->eta;
+        o.progress_s = opt->progress_s;
         std::vector<qsa::RequestResult> out;
         const qsa::ServeStats st = qsa::serve(rs, o, out);
         for (int i = 0; i < n; ++i) {

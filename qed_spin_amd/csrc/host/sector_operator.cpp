@@ -150,10 +150,12 @@ void SectorOperator::autotune() {
         return ms;
     };
     float best1 = 1e30f;
-    int l1 = csr_.lanes;
-    for (int L : {4, 8, 16}) {
+    int l1 = csr_.lanes, u1 = unroll_;
+    for (int L : {4, 8, 16})
+    for (int U : {2, 4}) {
         csr_.lanes = L;
         lanes_block_ = L;
+        unroll_ = U;
         const float t1 = time_it([&] {
             dev::VecSet v;
             v.k = 1;
@@ -164,6 +166,7 @@ void SectorOperator::autotune() {
         if (t1 < best1) {
             best1 = t1;
             l1 = L;
+            u1 = U;
         }
     }
     (void)hipEventDestroy(a);
@@ -171,6 +174,7 @@ void SectorOperator::autotune() {
     QSA_HIP(hipGetLastError());
     csr_.lanes = l1;
     lanes_block_ = l1;
+    unroll_ = u1;
 }
 
 dev::CsrView SectorOperator::csr_view(bool block) const {
@@ -190,6 +194,7 @@ dev::CsrView SectorOperator::csr_view(bool block) const {
     A.unroll = unroll_;
     A.dict_n = static_cast<int>(csr_.dict);
     A.lds_dict = global_dict_ ? 0 : 1;
+    A.wg = wg_;
     return A;
 }
 

@@ -64,8 +64,9 @@ public:
         csr_.lanes = lanes;
         lanes_block_ = lanes;
     }
-    /// Time the single-vector SpMV at every lanes-per-row width on this matrix and keep the fastest
-    /// (block applies share it); a few milliseconds on a large sector.
+    /// Time the single-vector SpMV at every lanes-per-row width (4, 8, 16) and gather depth (2, 4) on
+    /// this matrix and keep the fastest (block applies share the width); a few milliseconds on a
+    /// large sector.
     void autotune();
     int lanes_block() const { return lanes_block_; }
     /// The CSR as the kernels read it (CSR lane only).
@@ -79,6 +80,8 @@ public:
     void set_unroll(int u) { unroll_ = u; }
     /// Read u8-id dictionaries from global memory instead of staging them in LDS.
     void set_global_dict(bool on) { global_dict_ = on; }
+    /// Threads per workgroup of the single-vector SpMV (256, 512 or 1024).
+    void set_workgroup(int wg) { wg_ = wg; }
     bool xcd_rows() const { return xcd_; }
 
 private:
@@ -96,6 +99,7 @@ private:
     bool tune_ = true;
     int unroll_ = 4;       // single-vector applies: gathers issued this many entries at a time
     bool global_dict_ = false;
+    int wg_ = 512;
     CsrInfo csr_;
     DeviceBuffer row_ptr_, col_, vals_, dict_re_, dict_im_;
 };

@@ -9,6 +9,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <cstdio>
 #include <thread>
 #include <tuple>
 
@@ -284,6 +285,7 @@ ServeStats serve(const std::vector<Request>& reqs, const ServeOptions& o, std::v
     std::uint64_t reserved = 0;
     int in_flight = 0;
     ServeStats st;
+    double last_progress = 0.0;
     const auto t0 = clk::now();
     auto now_s = [&] { return std::chrono::duration<double>(clk::now() - t0).count(); };
     auto worker = [&](int slot) {
@@ -344,6 +346,14 @@ ServeStats serve(const std::vector<Request>& reqs, const ServeOptions& o, std::v
                 st.sector_hits += hits;
                 if (res.status == 0) ++st.completed;
                 else ++st.failed;
+                const double now = now_s();
+                if (o.progress_s > 0.0 && now - last_progress >= o.progress_s) {
+                    last_progress = now;
+                    std::fprintf(stderr, "[serve] %.1f s: %d of %zu done, %d in flight, %.2f GiB reserved\n", now,
+                                 st.completed + st.failed, reqs.size(), in_flight,
+                                 static_cast<double>(reserved) / (1ull << 30));
+                    std::fflush(stderr);
+                }
             }
             cv.notify_all();
         }

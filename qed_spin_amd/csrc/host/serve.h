@@ -38,6 +38,7 @@ struct ServeOptions {
     std::vector<double> betas;               ///< thermal temperatures (inverse)
     std::vector<double> omega;               ///< dynamics frequency grid
     double eta = 0.05;
+    double progress_s = 0.0;   ///< > 0: a line on stderr at most this often as requests complete
 };
 
 struct RequestResult {

@@ -52,6 +52,7 @@ class ServeOptions(C.Structure):
         ("n_omega", C.c_int),
         ("omega", C.POINTER(C.c_double)),
         ("eta", C.c_double),
+        ("progress_s", C.c_double),
     ]
 
 
@@ -89,7 +90,7 @@ class ServeResult:
 
 def serve(requests: List[dict], *, slots: int = 8, batched: bool = True, cache_sectors: bool = True,
           realtime: bool = True, hbm_budget: int = 0, cache_budget: int = 0, T=None, omega=None, eta: float = 0.05,
-          device: str = "gpu") -> ServeResult:
+          device: str = "gpu", progress_s: float = 0.0) -> ServeResult:
     resolve_device(device, "serve")
     if not requests:
         raise InvalidRequest("serve: no request")
@@ -121,6 +122,7 @@ def serve(requests: List[dict], *, slots: int = 8, batched: bool = True, cache_s
     o.n_omega = omega.size
     o.omega = _lib.ptr(omega, C.c_double)
     o.eta = float(eta)
+    o.progress_s = float(progress_s)
     stride = max(max(int(r.get("k", 4)) for r in requests), 3 * betas.size, omega.size)
     values = np.zeros((n, stride))
     e0 = np.zeros(n)
