@@ -221,7 +221,7 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=5.0)
     ap.add_argument("--serve-slots", type=int, default=8)
     ap.add_argument("--serve-nmax", type=int, default=30)
-    ap.add_argument("--serve-plain-requests", type=int, default=40)
+    ap.add_argument("--serve-plain-requests", type=int, default=100)
     ap.add_argument("--serve-realtime-requests", type=int, default=0)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 1 GiB device to device (the known-size kernel the HBM counters are scaled by)")

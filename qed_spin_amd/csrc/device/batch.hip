@@ -326,9 +326,13 @@ __global__ void k_pack_csr(CsrView A, u64 row_off, u64 col_off, u64 nnz_off, u64
     row_ptr[row_off + r] = nnz_off + A.row_ptr[r];
     if (r == static_cast<u64>(A.rows)) return;
     for (u64 e = A.row_ptr[r]; e < A.row_ptr[r + 1]; ++e) {
-        col[nnz_off + e] = static_cast<u32>(col_off + A.col[e]);
+        const u32 w = A.col[e];
+        col[nnz_off + e] = static_cast<u32>(col_off + (A.width == 3 ? (w & 0xFFFFFFu) : w));
         Zd v;
-        if (A.width == 1) {
+        if (A.width == 3) {
+            const u32 id = w >> 24;
+            v = Zd{A.dict_re[id], A.values_real ? 0.0 : A.dict_im[id]};
+        } else if (A.width == 1) {
             const u32 id = static_cast<const u8*>(A.vals)[e];
             v = Zd{A.dict_re[id], A.values_real ? 0.0 : A.dict_im[id]};
         } else if (A.width == 2) {

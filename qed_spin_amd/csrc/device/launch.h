@@ -62,7 +62,7 @@ struct CsrView {
     const u64* row_ptr = nullptr;
     const u32* col = nullptr;
     const void* vals = nullptr;    // u8 / u16 ids, or full values
-    int width = 0;                 // 1: u8 ids, 2: u16 ids, 0: full values
+    int width = 0;                 // 1: u8 ids, 2: u16 ids, 3: u8 ids packed in col's top byte, 0: full values
     const double* dict_re = nullptr;
     const double* dict_im = nullptr;
     int lanes = 16;                // lanes per row: 4, 8, 16, 32 or 64
@@ -78,6 +78,8 @@ struct CsrView {
 void csr_spmv(const CsrView& A, bool real, const VecSet& v, hipStream_t st);
 /// Block-interleaved multi-vector SpMV: X, Y are rows x nv, entry (row, vector) at row * nv + vector.
 void csr_spmv_block(const CsrView& A, bool real, int nv, const void* X, void* Y, hipStream_t st);
+/// col[e] |= id[e] << 24 (columns below 2^24): the packed u8-id layout.
+void pack_col_ids(u32* col, const u8* id, u64 nnz, hipStream_t st);
 
 // ---- blas.hip (vectors of T = double or Zd; n entries) -------------------------------------------
 /// Partial-sum scratch a reduction needs (device), sized by blas_scratch_bytes.

@@ -26,7 +26,7 @@ struct Dict {
 
 template <int WIDTH, bool VREAL>
 __device__ __forceinline__ Dict stage_dict(const CsrView& A, double* sre, double* sim) {
-    if constexpr (WIDTH == 1) {
+    if constexpr (WIDTH == 1 || WIDTH == 3) {
         if (A.lds_dict) {
             for (int i = threadIdx.x; i < A.dict_n; i += blockDim.x) {
                 sre[i] = A.dict_re[i];
@@ -78,9 +78,25 @@ inline unsigned grid_blocks(const CsrView& A, u64 threads) {
     return static_cast<unsigned>(b);
 }
 
+// Column and value of entry e. WIDTH 3: the u8 id packed in the top byte of the column word
+// (columns below 2^24), one 4-byte load per entry.
+template <int WIDTH, bool VREAL>
+__device__ __forceinline__ void load_entry(const CsrView& A, const Dict& d, u64 e, u32& c, double& vr, double& vi) {
+    if constexpr (WIDTH == 3) {
+        const u32 w = A.nt ? __builtin_nontemporal_load(A.col + e) : A.col[e];
+        c = w & 0xFFFFFFu;
+        const u32 id = w >> 24;
+        vr = d.re[id];
+        if constexpr (!VREAL) vi = d.im[id];
+    } else {
+        c = A.nt ? __builtin_nontemporal_load(A.col + e) : A.col[e];
+        value_at<WIDTH, VREAL>(A, d, e, vr, vi);
+    }
+}
+
 template <int WIDTH, bool VREAL, bool REAL, int L, int NV, int U>
 __global__ __launch_bounds__(1024) void k_spmv(CsrView A, Ptrs<NV> v) {
-    __shared__ double sre[WIDTH == 1 ? 256 : 1], sim[WIDTH == 1 && !VREAL ? 256 : 1];
+    __shared__ double sre[(WIDTH == 1 || WIDTH == 3) ? 256 : 1], sim[(WIDTH == 1 || WIDTH == 3) && !VREAL ? 256 : 1];
     const Dict d = stage_dict<WIDTH, VREAL>(A, sre, sim);
     const i64 gid = row_block(A) * blockDim.x + threadIdx.x;
     const i64 row = gid / L;
@@ -111,9 +127,8 @@ __global__ __launch_bounds__(1024) void k_spmv(CsrView A, Ptrs<NV> v) {
                 double vr[U], vi[U];
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
-                    c[u] = A.nt ? __builtin_nontemporal_load(A.col + e + u * L) : A.col[e + u * L];
                     vi[u] = 0.0;
-                    value_at<WIDTH, VREAL>(A, d, e + u * L, vr[u], vi[u]);
+                    load_entry<WIDTH, VREAL>(A, d, e + u * L, c[u], vr[u], vi[u]);
                 }
                 if constexpr (REAL && NV == 1) {
                     double x[U];
@@ -128,9 +143,9 @@ __global__ __launch_bounds__(1024) void k_spmv(CsrView A, Ptrs<NV> v) {
             }
         }
         for (; e < e1; e += L) {
-            const u32 c = A.nt ? __builtin_nontemporal_load(A.col + e) : A.col[e];
+            u32 c;
             double vr, vi = 0.0;
-            value_at<WIDTH, VREAL>(A, d, e, vr, vi);
+            load_entry<WIDTH, VREAL>(A, d, e, c, vr, vi);
             accumulate(c, vr, vi);
         }
     }
@@ -154,7 +169,7 @@ __global__ __launch_bounds__(1024) void k_spmv(CsrView A, Ptrs<NV> v) {
 // Block-interleaved vectors: X[row * NV + k].
 template <int WIDTH, bool VREAL, bool REAL, int L, int NV>
 __global__ __launch_bounds__(256) void k_spmv_block(CsrView A, const void* X, void* Y) {
-    __shared__ double sre[WIDTH == 1 ? 256 : 1], sim[WIDTH == 1 && !VREAL ? 256 : 1];
+    __shared__ double sre[(WIDTH == 1 || WIDTH == 3) ? 256 : 1], sim[(WIDTH == 1 || WIDTH == 3) && !VREAL ? 256 : 1];
     const Dict d = stage_dict<WIDTH, VREAL>(A, sre, sim);
     const i64 gid = row_block(A) * blockDim.x + threadIdx.x;
     const i64 row = gid / L;
@@ -165,9 +180,10 @@ __global__ __launch_bounds__(256) void k_spmv_block(CsrView A, const void* X, vo
     if (row < A.rows) {
         const u64 e0 = A.row_ptr[row], e1 = A.row_ptr[row + 1];
         for (u64 e = e0 + lane; e < e1; e += L) {
-            const u64 c = A.nt ? __builtin_nontemporal_load(A.col + e) : A.col[e];
+            u32 c32;
             double vr, vi = 0.0;
-            value_at<WIDTH, VREAL>(A, d, e, vr, vi);
+            load_entry<WIDTH, VREAL>(A, d, e, c32, vr, vi);
+            const u64 c = c32;
 #pragma unroll
             for (int k = 0; k < NV; ++k) {
                 if constexpr (REAL) {
@@ -273,6 +289,7 @@ void dispatch(const CsrView& A, bool vreal, bool real, const VecSet* v, int nv, 
     auto by_width = [&](auto vr, auto xr) {
         if (A.width == 1) go(std::integral_constant<int, 1>{}, vr, xr);
         else if (A.width == 2) go(std::integral_constant<int, 2>{}, vr, xr);
+        else if (A.width == 3) go(std::integral_constant<int, 3>{}, vr, xr);
         else go(std::integral_constant<int, 0>{}, vr, xr);
     };
     if (real) by_width(T{}, T{});
@@ -280,7 +297,19 @@ void dispatch(const CsrView& A, bool vreal, bool real, const VecSet* v, int nv, 
     else by_width(F{}, F{});
 }
 
+__global__ void k_pack_col_ids(u32* col, const u8* id, u64 nnz) {
+    for (u64 e = blockIdx.x * static_cast<u64>(blockDim.x) + threadIdx.x; e < nnz; e += static_cast<u64>(gridDim.x) * blockDim.x)
+        col[e] |= static_cast<u32>(id[e]) << 24;
+}
+
 }  // namespace
+
+void pack_col_ids(u32* col, const u8* id, u64 nnz, hipStream_t st) {
+    if (!nnz) return;
+    u64 g = (nnz + 255) / 256;
+    if (g > 65536) g = 65536;
+    hipLaunchKernelGGL(k_pack_col_ids, dim3(static_cast<unsigned>(g)), dim3(256), 0, st, col, id, nnz);
+}
 
 void csr_spmv(const CsrView& A, bool real, const VecSet& v, hipStream_t st) {
     if (A.rows == 0 || v.k == 0) return;

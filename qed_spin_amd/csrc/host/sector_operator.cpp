@@ -112,6 +112,15 @@ void SectorOperator::build_csr(std::uint64_t budget) {
                       real_, D, st);
         QSA_HIP(hipStreamSynchronize(st));
     }
+    // u8 ids with columns below 2^24: the id rides in the column word's top byte (one 4-byte load
+    // per entry instead of 4 + 1)
+    if (width == 1 && col_dim() <= (std::int64_t{1} << 24)) {
+        dev::pack_col_ids(col_.get<std::uint32_t>(), vals_.get<std::uint8_t>(), nnz, st);
+        QSA_HIP(hipGetLastError());
+        QSA_HIP(hipStreamSynchronize(st));
+        vals_.release();
+        packed_ = true;
+    }
     csr_.nnz = static_cast<std::int64_t>(nnz);
     csr_.width = width == 0 ? (real_ ? 8 : 16) : width;
     csr_.bytes = row_ptr_.bytes() + col_.bytes() + vals_.bytes() + dict_re_.bytes() + dict_im_.bytes();
@@ -150,12 +159,14 @@ void SectorOperator::autotune() {
         return ms;
     };
     float best1 = 1e30f;
-    int l1 = csr_.lanes, u1 = unroll_;
+    int l1 = csr_.lanes, u1 = unroll_, w1 = wg_;
     for (int L : {4, 8, 16})
-    for (int U : {2, 4}) {
+    for (int U : {2, 4})
+    for (int WG : {512, 1024}) {
         csr_.lanes = L;
         lanes_block_ = L;
         unroll_ = U;
+        wg_ = WG;
         const float t1 = time_it([&] {
             dev::VecSet v;
             v.k = 1;
@@ -167,6 +178,7 @@ void SectorOperator::autotune() {
             best1 = t1;
             l1 = L;
             u1 = U;
+            w1 = WG;
         }
     }
     (void)hipEventDestroy(a);
@@ -175,6 +187,7 @@ void SectorOperator::autotune() {
     csr_.lanes = l1;
     lanes_block_ = l1;
     unroll_ = u1;
+    wg_ = w1;
 }
 
 dev::CsrView SectorOperator::csr_view(bool block) const {
@@ -183,7 +196,7 @@ dev::CsrView SectorOperator::csr_view(bool block) const {
     A.row_ptr = row_ptr_.get<std::uint64_t>();
     A.col = col_.get<std::uint32_t>();
     A.vals = vals_.get();
-    A.width = (csr_.width == 1 || csr_.width == 2) ? csr_.width : 0;
+    A.width = packed_ ? 3 : (csr_.width == 1 || csr_.width == 2) ? csr_.width : 0;
     A.dict_re = dict_re_.get<double>();
     A.dict_im = dict_im_.get<double>();
     A.lanes = block ? lanes_block_ : csr_.lanes;
@@ -243,12 +256,22 @@ void SectorOperator::csr_download_rows(std::int64_t r0, std::int64_t r1, std::ve
     col.resize(nnz);
     if (nnz) copy_to_host(col.data(), col_.get<std::uint32_t>() + e0, nnz * 4);
     val.assign(2 * nnz, 0.0);
+    std::vector<std::uint8_t> packed_ids;
+    if (packed_) {
+        packed_ids.resize(nnz);
+        for (std::size_t e = 0; e < nnz; ++e) {
+            packed_ids[e] = static_cast<std::uint8_t>(col[e] >> 24);
+            col[e] &= 0xFFFFFFu;
+        }
+    }
     if (csr_.width == 1 || csr_.width == 2) {
         std::vector<double> dr(static_cast<std::size_t>(csr_.dict)), di(static_cast<std::size_t>(csr_.dict));
         copy_to_host(dr.data(), dict_re_.get(), dr.size() * 8);
         copy_to_host(di.data(), dict_im_.get(), di.size() * 8);
         std::vector<std::uint16_t> ids(nnz);
-        if (csr_.width == 1) {
+        if (packed_) {
+            for (std::size_t e = 0; e < nnz; ++e) ids[e] = packed_ids[e];
+        } else if (csr_.width == 1) {
             std::vector<std::uint8_t> b(nnz);
             if (nnz) copy_to_host(b.data(), vals_.get<std::uint8_t>() + e0, nnz);
             for (std::size_t e = 0; e < nnz; ++e) ids[e] = b[e];

@@ -64,9 +64,9 @@ public:
         csr_.lanes = lanes;
         lanes_block_ = lanes;
     }
-    /// Time the single-vector SpMV at every lanes-per-row width (4, 8, 16) and gather depth (2, 4) on
-    /// this matrix and keep the fastest (block applies share the width); a few milliseconds on a
-    /// large sector.
+    /// Time the single-vector SpMV at every lanes-per-row width (4, 8, 16), gather depth (2, 4) and
+    /// workgroup size (512, 1024) on this matrix and keep the fastest (block applies share the
+    /// width); a few tens of milliseconds on a large sector.
     void autotune();
     int lanes_block() const { return lanes_block_; }
     /// The CSR as the kernels read it (CSR lane only).
@@ -99,6 +99,7 @@ private:
     bool tune_ = true;
     int unroll_ = 4;       // single-vector applies: gathers issued this many entries at a time
     bool global_dict_ = false;
+    bool packed_ = false;   // u8 ids in the column words' top byte
     int wg_ = 512;
     CsrInfo csr_;
     DeviceBuffer row_ptr_, col_, vals_, dict_re_, dict_im_;
