@@ -220,6 +220,13 @@ int qedh_ftlm_dynamics(qedh_matvec source_h, int n_targets, const qedh_matvec* t
                        int samples, int krylov, uint64_t seed, double breakdown_tol, int n_t,
                        const double* temperatures, int n_omega, const double* omega, double eta, double* Z,
                        double* S, double* e_min, int* samples_done);
+/* The FTLM start vectors of samples first_sample .. first_sample + count - 1 of seed `base`, drawn
+ * on the device: each std::mt19937(low 32 bits of splitmix64(base + phi (s + 1))) ->
+ * std::normal_distribution<double>(0, 1), entry i = (n_2i, n_2i+1), not normalised (the reference's
+ * sample_engine and gaussian_vector, include/ed/thermal/sample_seed.h:37-42, 83-89, as ftlm.h:316
+ * and ftlm_dynamics.h:157 draw them). out: device, [count][n] complex (re, im). The entries equal the
+ * host's up to the last bit of the logarithm. */
+int qedh_gaussian_vectors(uint64_t base, uint64_t first_sample, int count, int64_t n, double* out);
 /* Exact thermodynamics of small blocks (the reference's host dense lane for blocks of at most
  * dense_max_dim states, src/engine/thermal.cpp:197-200, curves of include/ed/thermal/curves.h:38-65):
  * each block's CSR expanded to a dense Hermitian matrix on the host and its whole spectrum solved.

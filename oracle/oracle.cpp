@@ -597,6 +597,88 @@ void orc_gaussian_vector(u64 base, u64 sample, i64 n, double* out) {
     for (i64 i = 0; i < 2 * n; ++i) out[i] *= inv;
 }
 
+// The same draw without normalisation (entries of std::normal_distribution as drawn).
+void orc_gaussian_raw(u64 base, u64 sample, i64 n, double* out) {
+    std::mt19937 gen(static_cast<std::mt19937::result_type>(splitmix(base, sample)));
+    std::normal_distribution<double> nd(0.0, 1.0);
+    for (i64 i = 0; i < n; ++i) {
+        out[2 * i] = nd(gen);
+        out[2 * i + 1] = nd(gen);
+    }
+}
+
+// The draw restated from the published algorithms it is built of (no <random>), so the device
+// generator's steps are checked on the CPU against the library:
+//  - MT19937 (Matsumoto and Nishimura 1998): seeding x_0 = seed, x_i = 1812433253 (x_{i-1} xor
+//    (x_{i-1} >> 30)) + i; the recurrence x_{k+624} = x_{k+397} xor twist(upper bit of x_k, lower 31
+//    bits of x_{k+1}) with the matrix 0x9908b0df; tempering by 11 / 7 & 0x9d2c5680 / 15 & 0xefc60000 / 18;
+//  - generate_canonical<double, 53> of a 32-bit engine (libstdc++ 11, bits/random.tcc:3348-3380):
+//    u = (w0 + 2^32 w1) / 2^64, rounded once, 1 - 2^-53 when it reaches 1;
+//  - Marsaglia's polar method (bits/random.tcc:1795-1835): x = 2 u - 1, y = 2 u' - 1 until
+//    0 < r2 = x^2 + y^2 <= 1, then y m and x m (in that order) with m = sqrt(-2 ln r2 / r2).
+struct OrcMt {
+    std::uint32_t x[624];
+    int i = 624;
+    explicit OrcMt(std::uint32_t seed) {
+        x[0] = seed;
+        for (std::uint32_t k = 1; k < 624; ++k) x[k] = 1812433253u * (x[k - 1] ^ (x[k - 1] >> 30)) + k;
+    }
+    std::uint32_t next() {
+        if (i == 624) {
+            for (int k = 0; k < 624; ++k) {
+                const std::uint32_t y = (x[k] & 0x80000000u) | (x[(k + 1) % 624] & 0x7fffffffu);
+                x[k] = x[(k + 397) % 624] ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
+            }
+            i = 0;
+        }
+        std::uint32_t y = x[i++];
+        y ^= y >> 11;
+        y ^= (y << 7) & 0x9d2c5680u;
+        y ^= (y << 15) & 0xefc60000u;
+        y ^= y >> 18;
+        return y;
+    }
+    bool swap_words = false;   // a planted slip for the tests: the high word drawn first
+    double canonical() {
+        std::uint32_t lo = next();
+        std::uint32_t hi = next();
+        if (swap_words) std::swap(lo, hi);
+        double sum = static_cast<double>(lo);
+        sum = sum + static_cast<double>(hi) * 4294967296.0;
+        const double r = sum / 18446744073709551616.0;
+        return r >= 1.0 ? std::nextafter(1.0, 0.0) : r;
+    }
+};
+
+// The k-th output (k >= 1) of MT19937 seeded with `seed`.
+std::uint32_t orc_mt19937_nth(std::uint32_t seed, i64 k) {
+    OrcMt g(seed);
+    std::uint32_t y = 0;
+    for (i64 j = 0; j < k; ++j) y = g.next();
+    return y;
+}
+
+// n complex entries of the polar method on MT19937(seed), unnormalised; returns the candidates drawn.
+// slip != 0 plants a mistake for the tests (1: words of a uniform swapped, 2: (x m, y m) per entry).
+i64 orc_gaussian_polar(std::uint32_t seed, i64 n, double* out, int slip) {
+    OrcMt g(seed);
+    g.swap_words = slip == 1;
+    i64 tried = 0;
+    for (i64 i = 0; i < n; ++i) {
+        double x, y, r2;
+        do {
+            x = 2.0 * g.canonical() - 1.0;
+            y = 2.0 * g.canonical() - 1.0;
+            r2 = x * x + y * y;
+            ++tried;
+        } while (r2 > 1.0 || r2 == 0.0);
+        const double m = std::sqrt(-2.0 * std::log(r2) / r2);
+        out[2 * i] = (slip == 2 ? x : y) * m;
+        out[2 * i + 1] = (slip == 2 ? y : x) * m;
+    }
+    return tried;
+}
+
 // ---- continued fraction (include/ed/dynamics/cf.h:39-59) --------------------------------------
 void orc_continued_fraction(int M, const double* alpha, const double* beta, int nw, const double* omega, double eta,
                             double norm_sq, double* S) {

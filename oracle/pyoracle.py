@@ -69,6 +69,12 @@ def lib():
         L.orc_block_seed.argtypes = [C.c_uint64, C.c_uint64]
         L.orc_gaussian_vector.restype = None
         L.orc_gaussian_vector.argtypes = [C.c_uint64, C.c_uint64, C.c_int64, P_D]
+        L.orc_gaussian_raw.restype = None
+        L.orc_gaussian_raw.argtypes = [C.c_uint64, C.c_uint64, C.c_int64, P_D]
+        L.orc_mt19937_nth.restype = C.c_uint32
+        L.orc_mt19937_nth.argtypes = [C.c_uint32, C.c_int64]
+        L.orc_gaussian_polar.restype = C.c_int64
+        L.orc_gaussian_polar.argtypes = [C.c_uint32, C.c_int64, P_D, C.c_int]
         L.orc_continued_fraction.restype = None
         L.orc_continued_fraction.argtypes = [C.c_int, P_D, P_D, C.c_int, P_D, C.c_double, C.c_double, P_D]
         _lib = L
@@ -216,6 +222,30 @@ def gaussian_vector(base, sample, n):
     out = np.zeros(2 * n)
     lib().orc_gaussian_vector(int(base), int(sample), int(n), _p(out, C.c_double))
     return out.view(np.complex128)
+
+
+def gaussian_raw(base, sample, n):
+    """Sample `sample` of seed `base` as std::normal_distribution draws it (not normalised)."""
+    out = np.zeros(2 * n)
+    lib().orc_gaussian_raw(int(base), int(sample), int(n), _p(out, C.c_double))
+    return out.view(np.complex128)
+
+
+def mt19937_nth(seed, k):
+    return int(lib().orc_mt19937_nth(int(seed) & 0xFFFFFFFF, int(k)))
+
+
+def gaussian_polar(seed32, n, slip=0):
+    """The restated MT19937 -> polar-method draw; returns (entries, candidates drawn). slip plants a
+    mistake (1: the words of a uniform swapped, 2: (x m, y m) per entry) for the tests."""
+    out = np.zeros(2 * n)
+    tried = lib().orc_gaussian_polar(int(seed32) & 0xFFFFFFFF, int(n), _p(out, C.c_double), int(slip))
+    return out.view(np.complex128), int(tried)
+
+
+def seed32(base, sample):
+    """The 32-bit Mersenne Twister seed of sample `sample` of seed `base`."""
+    return int(lib().orc_block_seed(int(base), int(sample))) & 0xFFFFFFFF
 
 
 def continued_fraction(alpha, beta, omega, eta, norm_sq):

@@ -161,6 +161,12 @@ void batch_normalize(const BatchView& B, Zd* V, double* partial, double* inv, do
 /// V[(row0 + i) R + s] = src[i] for i < n.
 void batch_place(Zd* V, int R, u64 row0, int s, const Zd* src, u64 n, hipStream_t st);
 void batch_fill_zero(Zd* x, u64 n, hipStream_t st);
+
+// ---- sample.hip ------------------------------------------------------------------------------
+/// Gaussian start vectors: vector p (block p / R, slot p % R) of std::mt19937(seeds[p]) ->
+/// std::normal_distribution<double>, entry i = (n_2i, n_2i+1) written to
+/// V[(row_begin[p / R] + i) R + p % R] for i < dims[p / R]; `vectors` workgroups. Not normalised.
+void sample_gauss(Zd* V, int R, const u64* row_begin, const u64* dims, const u32* seeds, u32 vectors, hipStream_t st);
 void pack_csr(const CsrView& A, u64 row_off, u64 col_off, u64 nnz_off, u64* row_ptr, u32* col, Zd* val,
               hipStream_t st);
 void batch_advance(const BatchView& B, hipStream_t st);

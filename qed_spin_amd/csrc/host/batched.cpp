@@ -4,8 +4,6 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
-#include <condition_variable>
-#include <mutex>
 #include <thread>
 
 #include "tridiag.h"
@@ -56,28 +54,6 @@ template <class T> void put(DeviceBuffer& b, const std::vector<T>& h, hipStream_
     if (!h.empty()) QSA_HIP(hipMemcpyAsync(b.get(), h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st));
     QSA_HIP(hipStreamSynchronize(st));
 }
-
-/// K slots of pinned host memory, n complex entries each (hipHostMalloc; freed with the object).
-class PinnedRing {
-public:
-    PinnedRing(std::size_t k, std::size_t n) : slots_(k, nullptr) {
-        for (auto& p : slots_) {
-            void* q = nullptr;
-            QSA_HIP(hipHostMalloc(&q, std::max<std::size_t>(n, 1) * sizeof(cplx), hipHostMallocDefault));
-            p = static_cast<cplx*>(q);
-        }
-    }
-    ~PinnedRing() {
-        for (auto* p : slots_)
-            if (p) (void)hipHostFree(p);
-    }
-    PinnedRing(const PinnedRing&) = delete;
-    PinnedRing& operator=(const PinnedRing&) = delete;
-    cplx* at(std::size_t i) const { return slots_[i]; }
-
-private:
-    std::vector<cplx*> slots_;
-};
 
 /// A stream of its own at the default priority (graph capture needs a stream other than the null one).
 struct OwnStream {
@@ -202,6 +178,22 @@ void BatchedLanczos::normalize(std::vector<double>* norm2) {
         norm2->resize(pairs);
         copy_to_host(norm2->data(), n2.get(), pairs * 8);
     }
+}
+
+void BatchedLanczos::sample_start(const std::vector<std::uint32_t>& seeds) {
+    const std::size_t nb = ops_.size(), pairs = nb * static_cast<std::size_t>(R_);
+    if (seeds.size() != pairs) throw Error(kInvalid, "batched lanczos: one seed per pair");
+    const hipStream_t st = engine_stream();
+    std::vector<std::uint64_t> dims(nb);
+    for (std::size_t b = 0; b < nb; ++b) dims[b] = row_off_[b + 1] - row_off_[b];
+    DeviceBuffer rb_d, dim_d, seed_d;
+    put(rb_d, row_off_, st);
+    put(dim_d, dims, st);
+    put(seed_d, seeds, st);
+    dev::sample_gauss(V0_.get<dev::Zd>(), R_, rb_d.get<std::uint64_t>(), dim_d.get<std::uint64_t>(),
+                      seed_d.get<std::uint32_t>(), static_cast<std::uint32_t>(pairs), st);
+    QSA_HIP(hipGetLastError());
+    normalize(nullptr);
 }
 
 void BatchedLanczos::apply(const cplx* X, cplx* Y) const {
@@ -345,97 +337,20 @@ std::vector<Curves> batched_ftlm(const std::vector<std::shared_ptr<const SectorO
     std::vector<std::vector<SampleMoments>> mom(nb);
     std::vector<double> alpha, beta;
     std::vector<int> steps;
-    std::uint64_t max_dim = 0;
-    for (std::size_t b = 0; b < nb; ++b) max_dim = std::max<std::uint64_t>(max_dim, L.dim(static_cast<int>(b)));
-    DeviceBuffer staging(max_dim * static_cast<std::size_t>(R) * sizeof(cplx));
-    PinnedRing ring(4, max_dim * static_cast<std::size_t>(R));
     for (int s0 = 0; s0 < o.samples; s0 += R) {
         const int Rc = std::min(R, o.samples - s0);
-        // Samples of this round: pair (block, s) starts from sample s0 + s of the block's stream; the
-        // unused columns of a short last round hold copies of its first sample (never read back).
-        // The vectors are drawn on the host pool, one (block, sample) per task in block order,
-        // straight into a ring of pinned host slots (block b in slot b mod K, sample-major); this
-        // thread uploads each block once its R vectors are drawn and places them in the batch layout
-        // on the device, which frees the slot for block b + K. Drawing and uploading overlap.
+        // Samples of this round: pair (block, s) starts from sample s0 + s of the block's stream (the
+        // 32-bit Mersenne Twister seed of sample_engine), drawn on the device; the unused columns of
+        // a short last round repeat its first sample (never read back).
         const auto t_seed = clock_t_::now();
-        L.clear_start();
-        constexpr std::size_t K = 4;
-        const std::size_t slot_elems = static_cast<std::size_t>(max_dim) * static_cast<std::size_t>(R);
-        std::unique_ptr<std::atomic<int>[]> left(new std::atomic<int>[nb]);
-        for (std::size_t b = 0; b < nb; ++b) left[b].store(R);
-        std::mutex mu;
-        std::condition_variable cv;
-        std::size_t uploaded = 0;   // blocks whose slot is free again
-        std::atomic<std::size_t> next{0};
-        std::atomic<bool> failed{false};
-        std::exception_ptr failure;
-        const std::size_t tasks = nb * static_cast<std::size_t>(R);
-        auto draw = [&] {
-            for (;;) {
-                const std::size_t i = next.fetch_add(1);
-                if (i >= tasks || failed.load()) return;
-                const std::size_t b = i / R;
-                const int s = static_cast<int>(i % R);
-                {
-                    std::unique_lock<std::mutex> lk(mu);
-                    cv.wait(lk, [&] { return b < uploaded + K || failed.load(); });
-                }
-                if (failed.load()) return;
-                try {
-                    const std::uint64_t n = L.dim(static_cast<int>(b));
-                    const int sample = s0 + (s < Rc ? s : 0);
-                    std::mt19937 eng = sample_engine(o.block_seeds[b], static_cast<std::uint64_t>(sample));
-                    const std::vector<cplx> v = gaussian_vector(n, eng);
-                    std::copy(v.begin(), v.end(), ring.at(b % K) + static_cast<std::size_t>(s) * n);
-                } catch (...) {
-                    std::lock_guard<std::mutex> lk(mu);
-                    if (!failed.exchange(true)) failure = std::current_exception();
-                    cv.notify_all();
-                    return;
-                }
-                if (left[b].fetch_sub(1) == 1) {
-                    std::lock_guard<std::mutex> lk(mu);
-                    cv.notify_all();
-                }
+        std::vector<std::uint32_t> seeds(nb * static_cast<std::size_t>(R));
+        for (std::size_t b = 0; b < nb; ++b)
+            for (int s = 0; s < R; ++s) {
+                const int sample = s0 + (s < Rc ? s : 0);
+                seeds[b * R + s] = static_cast<std::uint32_t>(
+                    splitmix_seed(o.block_seeds[b], static_cast<std::uint64_t>(sample)));
             }
-        };
-        std::vector<std::thread> pool;
-        const unsigned T = host_threads();
-        for (unsigned w = 0; w < T; ++w) pool.emplace_back(draw);
-        auto stop = [&] {
-            {
-                std::lock_guard<std::mutex> lk(mu);
-                failed.store(true);
-                cv.notify_all();
-            }
-            for (auto& th : pool) th.join();
-        };
-        try {
-            for (std::size_t b = 0; b < nb; ++b) {
-                {
-                    std::unique_lock<std::mutex> lk(mu);
-                    cv.wait(lk, [&] { return left[b].load() == 0 || failed.load(); });
-                }
-                if (failed.load()) break;
-                const std::uint64_t n = L.dim(static_cast<int>(b));
-                QSA_HIP(hipMemcpyAsync(staging.get(), ring.at(b % K), n * static_cast<std::size_t>(R) * sizeof(cplx),
-                                       hipMemcpyHostToDevice, engine_stream()));
-                for (int s = 0; s < R; ++s) L.place(static_cast<int>(b), s, staging.get<cplx>() + static_cast<std::size_t>(s) * n);
-                QSA_HIP(hipStreamSynchronize(engine_stream()));
-                {
-                    std::lock_guard<std::mutex> lk(mu);
-                    uploaded = b + 1;
-                }
-                cv.notify_all();
-            }
-        } catch (...) {
-            stop();
-            throw;
-        }
-        for (auto& th : pool) th.join();
-        if (failure) std::rethrow_exception(failure);
-        (void)slot_elems;
-        L.normalize(nullptr);
+        L.sample_start(seeds);
         t.seed_s += secs(t_seed, clock_t_::now());
         L.run(o.krylov, o.breakdown_tol, /*local=*/true, o.use_graph, alpha, beta, steps, t);
         const auto t_post = clock_t_::now();

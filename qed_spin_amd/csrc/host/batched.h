@@ -40,6 +40,10 @@ public:
     void clear_start();
     void place(int b, int s, const cplx* device_vector);
     void normalize(std::vector<double>* norm2);
+    /// Start vectors drawn on the device: pair (b, s) gets the Gaussian vector of the 32-bit
+    /// Mersenne Twister seed seeds[b R + s] (std::mt19937 -> std::normal_distribution, as the host
+    /// draws it), then every vector is normalised.
+    void sample_start(const std::vector<std::uint32_t>& seeds);
     /// M Lanczos steps without a kept basis, with breakdown_tol[b] per block: local = LocalDGKS3
     /// with K = 1 (FTLM), else no reorthogonalisation (the continued fraction). The step is captured
     /// as a hipGraph when use_graph. live (optional, B x R): the pairs to run. Returns

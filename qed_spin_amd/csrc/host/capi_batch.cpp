@@ -380,3 +380,25 @@ extern "C" int qedh_ftlm_dynamics(qedh_matvec source_h, int n_targets, const qed
         if (samples_done) *samples_done = static_cast<int>(r.samples_done);
     });
 }
+
+extern "C" int qedh_gaussian_vectors(uint64_t base, uint64_t first_sample, int count, int64_t n, double* out) {
+    return guarded_batch([&] {
+        if (!out || count < 1 || n < 1) throw qsa::Error(qsa::kInvalid, "gaussian vectors: bad arguments");
+        const hipStream_t st = qsa::engine_stream();
+        const std::size_t c = static_cast<std::size_t>(count);
+        std::vector<std::uint64_t> rb(c), dims(c, static_cast<std::uint64_t>(n));
+        std::vector<std::uint32_t> seeds(c);
+        for (std::size_t i = 0; i < c; ++i) {
+            rb[i] = i * static_cast<std::uint64_t>(n);
+            seeds[i] = static_cast<std::uint32_t>(qsa::splitmix_seed(base, first_sample + i));
+        }
+        qsa::DeviceBuffer rb_d(c * 8), dim_d(c * 8), seed_d(c * 4);
+        qsa::copy_to_device(rb_d.get(), rb.data(), c * 8);
+        qsa::copy_to_device(dim_d.get(), dims.data(), c * 8);
+        qsa::copy_to_device(seed_d.get(), seeds.data(), c * 4);
+        qsa::dev::sample_gauss(reinterpret_cast<qsa::dev::Zd*>(out), 1, rb_d.get<std::uint64_t>(),
+                               dim_d.get<std::uint64_t>(), seed_d.get<std::uint32_t>(), static_cast<std::uint32_t>(c), st);
+        QSA_HIP(hipGetLastError());
+        QSA_HIP(hipStreamSynchronize(st));
+    });
+}

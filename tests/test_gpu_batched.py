@@ -120,6 +120,27 @@ def test_batch_lanczos_graph_equals_eager_bitwise():
     assert np.array_equal(sg, se) and np.array_equal(sg, sg2)
 
 
+@pytest.mark.parametrize("base,first,count,n", [(11, 0, 3, 1), (11, 2, 2, 155), (7, 0, 4, 157), (2**63 + 5, 37, 3, 100003)])
+def test_device_gaussian_matches_library_draw(base, first, count, n):
+    """The FTLM start vectors drawn on the device against std::mt19937 -> std::normal_distribution
+    on the host: every entry within 1e-14 relative (the device logarithm may differ from the host's
+    in the last bit; every other step is exact), most of them bit for bit."""
+    from qed_spin_amd import _lib
+    from qed_spin_amd.sector import DeviceArray
+
+    say("device gaussian", base, first, count, n)
+    d = DeviceArray(count * n)
+    _lib.call("qedh_gaussian_vectors", base, first, count, n, d.ptr)
+    got = d.to_host().reshape(count, n)
+    exact = 0
+    for c in range(count):
+        want = po.gaussian_raw(base, first + c, n)
+        for part in (np.real, np.imag):
+            g, w = part(got[c]), part(want)
+            assert np.all(np.abs(g - w) <= 1e-14 * np.abs(w)), (c, np.max(np.abs(g - w)))
+            exact += int(np.sum(g == w))
+    assert exact >= count * n   # at least half of the 2 count n components bit for bit
+
 @pytest.mark.parametrize("samples", [6, 70])
 def test_ftlm_batched_matches_per_block_driver(samples):
     say("ftlm_batched vs ftlm, samples", samples)

@@ -212,6 +212,30 @@ def test_seeded_gaussian_is_unit_and_reproducible():
     assert abs(np.linalg.norm(a) - 1.0) < 1e-14
 
 
+def test_mt19937_known_answer():
+    # The C++ standard's check of std::mt19937 ([rand.predef]): the 10000th output of the
+    # default-seeded (5489) engine is 4123659995.
+    assert po.mt19937_nth(5489, 10000) == 4123659995
+    assert po.mt19937_nth(5489, 1) == 3499211612   # the first output of MT19937 at its default seed
+
+
+@pytest.mark.parametrize("base,sample,n", [(11, 0, 1), (11, 0, 155), (11, 3, 157), (7, 1, 5000), (2**63 + 5, 39, 20011)])
+def test_polar_restatement_equals_library_draw(base, sample, n):
+    """The device generator's algorithm restated (MT19937 words, 53-bit canonical uniforms, the polar
+    method, (y m, x m) per kept candidate) equals std::normal_distribution on std::mt19937 bit for
+    bit on the CPU (same libm log on both sides)."""
+    want = po.gaussian_raw(base, sample, n)
+    got, tried = po.gaussian_polar(po.seed32(base, sample), n)
+    assert np.array_equal(got, want)
+    # about pi / 4 of the candidates are kept
+    if n >= 5000:
+        assert abs(n / tried - np.pi / 4) < 0.02
+    # planted slips in the restatement -- the words of a uniform swapped, the kept pair's normals in
+    # the other order -- are told apart
+    for slip in (1, 2):
+        assert not np.array_equal(po.gaussian_polar(po.seed32(base, sample), n, slip)[0], want)
+
+
 def _reference_records():
     import json
 
