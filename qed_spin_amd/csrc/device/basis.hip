@@ -25,12 +25,15 @@ __device__ __forceinline__ u64 binom_at(const SectorView& S, int n, int k) {
 /// full space.
 __device__ u64 state_of_rank(const SectorView& S, u64 rank) {
     if (S.n_up < 0) return rank;
+    // colex unranking: the i-th highest set bit is the largest p with C(p, i) <= the remaining rank;
+    // the positions decrease, so p only moves down (at most n_sites + n_up table reads)
     u64 s = 0;
-    for (int i = S.n_up - 1; i >= 0; --i) {
-        int p = i;
-        while (p + 1 < S.n_sites && binom_at(S, p + 1, i + 1) <= rank) ++p;
+    int p = S.n_sites - 1;
+    for (int i = S.n_up; i >= 1; --i) {
+        while (binom_at(S, p, i) > rank) --p;
         s |= 1ULL << p;
-        rank -= binom_at(S, p, i + 1);
+        rank -= binom_at(S, p, i);
+        --p;
     }
     return s;
 }
@@ -104,6 +107,33 @@ __global__ __launch_bounds__(256) void k_orbit_flags(SectorView S, u64 total, u6
     if ((threadIdx.x & 63) == 0 && my_orbits) atomicAdd(orbits, my_orbits);
 }
 
+// The same bitmap with one lane per state (a wave's ballot is one word): for subspaces small enough
+// that one thread per 64 states leaves most of the device idle. The orbit count is summed per
+// workgroup in LDS, one atomic per workgroup.
+template <bool SLC>
+__global__ __launch_bounds__(256) void k_orbit_flags_lane(SectorView S, u64 total, u64 n_words, u64* kept,
+                                                          unsigned long long* orbits) {
+    __shared__ unsigned long long wg_orbits;
+    if (threadIdx.x == 0) wg_orbits = 0;
+    __syncthreads();
+    const u64 r = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x;
+    bool rep = false, keep = false;
+    if (r < total) {
+        const u64 s = state_of_rank(S, r);
+        rep = is_rep<SLC>(S, s);
+        keep = rep && norm_sq(S, s) > 1e-12;
+    }
+    const u64 fk = __ballot(keep);
+    const u64 fr = __ballot(rep);
+    const u64 w = r >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        if (w < n_words) kept[w] = fk;
+        if (fr) atomicAdd(&wg_orbits, static_cast<unsigned long long>(__popcll(fr)));
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && wg_orbits) atomicAdd(orbits, wg_orbits);
+}
+
 __global__ void k_popcount_words(const u64* kept, u64 n_words, u64* counts) {
     const u64 w = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (w < n_words) counts[w] = static_cast<u64>(__popcll(kept[w]));
@@ -129,6 +159,21 @@ __global__ __launch_bounds__(256) void k_orbit_emit(SectorView S, u64 total, u64
     }
 }
 
+// One lane per state: a kept state's index is its word's offset plus the kept bits below it.
+__global__ __launch_bounds__(256) void k_orbit_emit_lane(SectorView S, u64 total, const u64* kept, const u64* offsets,
+                                                         u64* reps, double* inv_norms) {
+    const u64 r = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= total) return;
+    const u64 w = r >> 6;
+    const int bit = static_cast<int>(r & 63);
+    const u64 f = kept[w];
+    if (!((f >> bit) & 1ULL)) return;
+    const u64 off = offsets[w] + static_cast<u64>(__popcll(f & ((1ULL << bit) - 1ULL)));
+    const u64 s = state_of_rank(S, r);
+    reps[off] = s;
+    inv_norms[off] = 1.0 / sqrt(norm_sq(S, s));
+}
+
 __global__ void k_rank_fill(SectorView S, i32* table) {
     const i64 i = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= S.dim) return;
@@ -139,11 +184,24 @@ __global__ void k_rank_fill(SectorView S, i32* table) {
 
 inline unsigned blocks_for(u64 n, unsigned t) { return static_cast<unsigned>((n + t - 1) / t); }
 
+// Up to this many states of the subspace the orbit scan takes one lane per state, above it one
+// thread per 64 states stepping by Gosper's rule (a rank -> state unranking per state costs more
+// than a step once the device is full).
+constexpr u64 kLanePerStateMax = u64{1} << 26;
+
 }  // namespace
 
 // The kept-representative bitmap and the orbit count of the subspace (before the character filter).
 void orbit_flags(const SectorView& S, u64 total, u64* kept, unsigned long long* orbits, hipStream_t st) {
     const u64 n_words = (total + 63) / 64;
+    if (total <= kLanePerStateMax) {
+        const dim3 grid(blocks_for(total, 256));
+        if (S.slc_L > 0)
+            hipLaunchKernelGGL(k_orbit_flags_lane<true>, grid, dim3(256), 0, st, S, total, n_words, kept, orbits);
+        else
+            hipLaunchKernelGGL(k_orbit_flags_lane<false>, grid, dim3(256), 0, st, S, total, n_words, kept, orbits);
+        return;
+    }
     if (S.slc_L > 0)
         hipLaunchKernelGGL(k_orbit_flags<true>, dim3(blocks_for(n_words, 256)), dim3(256), 0, st, S, total, n_words,
                            kept, orbits);
@@ -169,6 +227,11 @@ void scan_words(const u64* kept, u64 n_words, u64* counts, u64* offsets, void* t
 void orbit_emit(const SectorView& S, u64 total, const u64* kept, const u64* offsets, u64* reps, double* inv_norms,
                 hipStream_t st) {
     const u64 n_words = (total + 63) / 64;
+    if (total <= kLanePerStateMax) {
+        hipLaunchKernelGGL(k_orbit_emit_lane, dim3(blocks_for(total, 256)), dim3(256), 0, st, S, total, kept, offsets,
+                           reps, inv_norms);
+        return;
+    }
     hipLaunchKernelGGL(k_orbit_emit, dim3(blocks_for(n_words, 256)), dim3(256), 0, st, S, total, n_words, kept,
                        offsets, reps, inv_norms);
 }
