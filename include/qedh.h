@@ -45,6 +45,12 @@ int qedh_memcpy_dtoh(void* dst, const void* src, size_t bytes);
 int qedh_memcpy_dtod(void* dst, const void* src, size_t bytes);
 int qedh_memset(void* dst, int value, size_t bytes);
 int qedh_synchronize(void);
+/* The calling thread's stream for every engine launch, copy and allocation: own != 0 a non-blocking
+ * stream of the thread's own (created on first use, destroyed when the thread ends), 0 the null
+ * stream. Calls that return results synchronise their stream first, so objects built on one
+ * thread's stream are complete when the call returns. Lets host threads build sectors and
+ * operators concurrently (the reference builds blocks one after another; no counterpart). */
+int qedh_thread_stream(int own);
 
 /* ---- operators: canonical spin-1/2 terms (src/ops/algebra.cpp, include/ed/ops/algebra.h) ----- */
 int qedh_operator_create(int n_sites, qedh_operator* out);

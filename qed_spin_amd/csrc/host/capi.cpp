@@ -94,21 +94,45 @@ int qedh_free(void* ptr) {
 }
 int qedh_memcpy_htod(void* dst, const void* src, size_t bytes) {
     return guarded([&] {
-        if (bytes) QSA_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+        qsa::copy_to_device(dst, src, bytes);
     });
 }
 int qedh_memcpy_dtoh(void* dst, const void* src, size_t bytes) {
     return guarded([&] {
-        if (bytes) QSA_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+        qsa::copy_to_host(dst, src, bytes);
     });
 }
 int qedh_memset(void* dst, int value, size_t bytes) {
     return guarded([&] {
-        if (bytes) QSA_HIP(hipMemset(dst, value, bytes));
+        if (!bytes) return;
+        QSA_HIP(hipMemsetAsync(dst, value, bytes, qsa::engine_stream()));
+        QSA_HIP(hipStreamSynchronize(qsa::engine_stream()));
     });
 }
 int qedh_synchronize(void) {
-    return guarded([&] { QSA_HIP(hipDeviceSynchronize()); });
+    return guarded([&] {
+        QSA_HIP(hipStreamSynchronize(qsa::engine_stream()));
+        QSA_HIP(hipDeviceSynchronize());
+    });
+}
+int qedh_thread_stream(int own) {
+    return guarded([&] {
+        if (!own) {
+            qsa::detail::tls_stream() = nullptr;
+            return;
+        }
+        qsa::require_device();
+        // created on the thread's first request, destroyed when the thread ends
+        struct Owned {
+            hipStream_t s = nullptr;
+            ~Owned() {
+                if (s) (void)hipStreamDestroy(s);
+            }
+        };
+        thread_local Owned owned;
+        if (!owned.s) QSA_HIP(hipStreamCreateWithFlags(&owned.s, hipStreamNonBlocking));
+        qsa::detail::tls_stream() = owned.s;
+    });
 }
 
 int qedh_operator_create(int n_sites, qedh_operator* out) {

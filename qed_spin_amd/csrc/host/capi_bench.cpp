@@ -111,7 +111,9 @@ extern "C" int qedh_matvec_set_tuning(qedh_matvec m, int lanes, int flags) {
 
 extern "C" int qedh_memcpy_dtod(void* dst, const void* src, size_t bytes) {
     try {
-        if (bytes) QSA_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+        if (!bytes) return 0;
+        QSA_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, qsa::engine_stream()));
+        QSA_HIP(hipStreamSynchronize(qsa::engine_stream()));
         return 0;
     } catch (const qsa::Error& e) {
         qsa::capi_set_error(e.what());

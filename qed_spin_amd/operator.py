@@ -62,8 +62,19 @@ class Operator:
         return self
 
     # -- engine handles ---------------------------------------------------------------------------
+    def __getstate__(self):
+        d = dict(self.__dict__)
+        d.pop("_engine", None)   # the engine handle stays with this object (copies rebuild their own)
+        return d
+
     def _handle(self) -> "EngineOperator":
-        return EngineOperator(self)
+        """The engine's copy of the terms, rebuilt only when the products changed since the last call."""
+        key = (int(self.num_sites), tuple(self.products))
+        cached = self.__dict__.get("_engine")
+        if cached is None or cached[0] != key:
+            cached = (key, EngineOperator(self))
+            self.__dict__["_engine"] = cached
+        return cached[1]
 
     def norm_bound(self) -> float:
         return self._handle().info()[1]

@@ -4,8 +4,9 @@ RepSectorMatVec / CrossSectorMatVec device lanes, behind include/qedh.h)."""
 from __future__ import annotations
 
 import ctypes as C
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass
-from typing import Optional, Sequence
+from typing import Callable, List, Optional, Sequence
 
 import numpy as np
 
@@ -15,6 +16,23 @@ from .operator import Operator, Program
 
 LANE_AUTO, LANE_GATHER, LANE_CSR = 0, 1, 2
 DEFAULT_RANK_BUDGET = 16 << 30   # dense rank table up to 16 GiB of HBM
+AUTOTUNE_MIN_ROWS = 1 << 18      # CSR operators from this many rows time their SpMV variants when built
+BUILD_THREADS = 8
+
+
+def _own_stream() -> None:
+    _lib.call("qedh_thread_stream", 1)
+
+
+def parallel_build(fn: Callable, items: Sequence, threads: int = BUILD_THREADS) -> List:
+    """fn(item) for every item on host threads, each with a HIP stream of its own (builds of small
+    sectors and operators are launch- and synchronisation-bound, so several in flight overlap);
+    results in item order. Every build call synchronises its stream before it returns."""
+    items = list(items)
+    if len(items) <= 1 or threads <= 1:
+        return [fn(x) for x in items]
+    with ThreadPoolExecutor(max_workers=min(threads, len(items)), initializer=_own_stream) as ex:
+        return list(ex.map(fn, items))
 DEFAULT_CSR_BUDGET = 64 << 30
 
 

@@ -23,7 +23,7 @@ import numpy as np
 from . import _lib
 from .errors import DeviceUnavailable, DeviceUnsupported, InvalidRequest
 from .operator import Operator
-from .sector import DEFAULT_CSR_BUDGET, LANE_CSR, Sector, SectorMatvec
+from .sector import AUTOTUNE_MIN_ROWS, DEFAULT_CSR_BUDGET, LANE_CSR, Sector, SectorMatvec, parallel_build
 from .symmetry import Symmetry
 
 LCG_MUL = 6364136223846793005
@@ -139,15 +139,22 @@ class ThermalResult:
 
 def walk_blocks(H: Operator, sym: Symmetry, csr_budget: int = DEFAULT_CSR_BUDGET):
     """The non-empty blocks of the walk in the reference's order (n_up ascending, then momentum),
-    each with its sector operator on the device CSR lane."""
-    out = []
-    for blk in sym.blocks(H.num_sites):
-        sec = Sector(blk.group, blk.n_up)
-        if sec.dim == 0:
-            continue
-        m = SectorMatvec(H, sec, lane=LANE_CSR, csr_budget=csr_budget)
-        out.append((blk, sec, m))
-    return out
+    each with its sector operator on the device CSR lane. Sectors and the operators of small
+    sectors are built on several host threads at once; operators large enough to time their SpMV
+    variants are built one after another, so the timings see an otherwise idle device."""
+    blks = sym.blocks(H.num_sites)
+    H._handle()
+    secs = parallel_build(lambda b: Sector(b.group, b.n_up), blks)
+    keep = [(b, s) for b, s in zip(blks, secs) if s.dim > 0]
+    mats = [None] * len(keep)
+    small = [i for i, (_, s) in enumerate(keep) if s.dim < AUTOTUNE_MIN_ROWS]
+    make = lambda i: SectorMatvec(H, keep[i][1], lane=LANE_CSR, csr_budget=csr_budget)
+    for i, m in zip(small, parallel_build(make, small)):
+        mats[i] = m
+    for i in range(len(keep)):
+        if mats[i] is None:
+            mats[i] = make(i)
+    return [(b, s, m) for (b, s), m in zip(keep, mats)]
 
 
 def thermal(H: Operator, T, *, sym: Optional[Symmetry] = None, method: str = "ftlm", samples: int = 40,
