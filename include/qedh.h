@@ -36,6 +36,10 @@ const char* qedh_last_error(void);
 /* Visible HIP devices; 0 without a device (replaces ed::cuda_device_count / has_cuda_build,
  * python/qed/_verbs/_device.py:18-29). */
 int qedh_device_count(int* count);
+/* Free and total bytes of the current device (hipMemGetInfo): the free-memory probe of the
+ * reference's placement, DeviceProbe::free_bytes = ed::core::available_device_bytes
+ * (include/ed/core/select_backend.h:79-83, 112-160); DeviceUnavailable without a device. */
+int qedh_device_memory(uint64_t* free_bytes, uint64_t* total_bytes);
 /* Device memory for callers without their own allocator (replaces CudaBackend::allocate /
  * deallocate / copy_from_host / copy_to_host, include/ed/gpu/cuda_backend.cuh:104-230). */
 int qedh_malloc(void** ptr, size_t bytes);

@@ -46,6 +46,7 @@ class MatvecInfo(C.Structure):
 # name -> argument types (every entry point returns int status)
 _SIGNATURES = {
     "qedh_device_count": [c_int_p],
+    "qedh_device_memory": [c_u64_p, c_u64_p],
     "qedh_malloc": [C.POINTER(C.c_void_p), C.c_size_t],
     "qedh_free": [C.c_void_p],
     "qedh_memcpy_htod": [C.c_void_p, C.c_void_p, C.c_size_t],
@@ -161,6 +162,13 @@ def device_count() -> int:
     n = C.c_int(0)
     call("qedh_device_count", C.byref(n))
     return int(n.value)
+
+
+def device_memory():
+    """(free, total) bytes of the current device."""
+    f, t = C.c_uint64(0), C.c_uint64(0)
+    call("qedh_device_memory", C.byref(f), C.byref(t))
+    return int(f.value), int(t.value)
 
 
 def ptr(a, ctype):

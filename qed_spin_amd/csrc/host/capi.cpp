@@ -79,6 +79,18 @@ int qedh_device_count(int* count) {
     });
 }
 
+int qedh_device_memory(uint64_t* free_bytes, uint64_t* total_bytes) {
+    return guarded([&] {
+        need(free_bytes, "free_bytes");
+        need(total_bytes, "total_bytes");
+        qsa::require_device();
+        size_t f = 0, t = 0;
+        QSA_HIP(hipMemGetInfo(&f, &t));
+        *free_bytes = f;
+        *total_bytes = t;
+    });
+}
+
 int qedh_malloc(void** ptr, size_t bytes) {
     return guarded([&] {
         need(ptr, "ptr");

@@ -209,3 +209,39 @@ def test_majumdar_ghosh_n32_full_size():
     assert abs(x @ hy - hx @ y) <= 1e-12 * abs(x @ hy)
     vals, conv, _ = m.lowest(1)
     assert conv and abs(vals[0] + 12.0) <= 1e-10 * 12.0
+
+
+def test_device_memory_probe():
+    say("device memory probe")
+    from qed_spin_amd import _lib
+
+    free, total = _lib.device_memory()
+    assert 0 < free <= total
+    assert total >= 64 << 30   # an MI355X holds 288 GB
+
+
+def test_sectors_built_on_threads_with_own_streams_equal_sequential_builds():
+    """Sectors and CSR operators built on host threads, each with a stream of its own, are the
+    same bit for bit as the ones built one after another on the calling thread."""
+    say("threaded builds")
+    from qed_spin_amd.sector import parallel_build
+
+    names = NAMES[:6]
+    seq = [fixture_sector(load_fixture(n)).export() for n in names]
+    par = parallel_build(lambda n: fixture_sector(load_fixture(n)).export(), names, threads=3)
+    for (r1, i1), (r2, i2) in zip(seq, par):
+        assert np.array_equal(r1, r2) and np.array_equal(bits(i1), bits(i2))
+    n = 14
+    H = qs.HamiltonianBuilder(n).heisenberg([(i, (i + 1) % n) for i in range(n)], 1.0).to_operator()
+    blks = qs.Symmetry(sz=[n // 2]).blocks(n)
+
+    def csr(b):
+        s = qs.Sector(b.group, b.n_up)
+        m = qs.SectorMatvec(H, s, lane=qs.LANE_CSR)
+        return m.export_csr()
+
+    one = [csr(b) for b in blks]
+    many = parallel_build(csr, blks, threads=4)
+    for a, b in zip(one, many):
+        for x, y in zip(a, b):
+            assert np.array_equal(np.asarray(x), np.asarray(y))
