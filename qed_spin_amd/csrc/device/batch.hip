@@ -2,8 +2,10 @@
 // advance one Lanczos step together.
 //
 // Layout: the blocks' CSRs are packed into one block-diagonal CSR (rows and columns offset by the
-// block's first row, values in full); a batch vector holds R entries per row, X[row R + s], so the
-// R vectors of a row are one contiguous line. Rows are cut into tiles of at most kBatchTileRows rows
+// block's first row; values as 16-bit ids into the blocks' dictionaries laid end to end when every
+// block has one and they hold at most 65536 values together, else in full); a batch vector holds R
+// entries per row, X[row R + s], so the R vectors of a row are one contiguous line. Rows are cut
+// into tiles of at most kBatchTileRows rows
 // of one block, one workgroup of 256 threads per tile.
 //
 // Vector kernels: thread t of a tile is (slot t / R, vector t % R); the 256 / R slots take the
@@ -36,6 +38,14 @@ __device__ __forceinline__ bool pair_live(const BatchView& B, u32 block, int s) 
     return !B.active || B.active[static_cast<u64>(block) * B.R + s] != 0;
 }
 
+__device__ __forceinline__ Zd entry_value(const BatchView& B, u64 e) {
+    if (B.vid) {
+        const u32 id = B.vid[e];
+        return Zd{B.dict_re[id], B.dict_im[id]};
+    }
+    return B.val[e];
+}
+
 // One thread per (row, vector), R >= 8.
 __global__ __launch_bounds__(kThreads) void k_batch_spmv_wide(BatchView B, const Zd* X, Zd* Y) {
     const Tile t = B.tiles[blockIdx.x];
@@ -47,7 +57,7 @@ __global__ __launch_bounds__(kThreads) void k_batch_spmv_wide(BatchView B, const
         double ar = 0.0, ai = 0.0;
         for (u64 e = e0; e < e1; ++e) {
             const u64 c = B.col[e];
-            const Zd v = B.val[e];
+            const Zd v = entry_value(B, e);
             const Zd x = X[c * R + s];
             ar += v.re * x.re - v.im * x.im;
             ai += v.re * x.im + v.im * x.re;
@@ -75,7 +85,7 @@ __global__ __launch_bounds__(kThreads) void k_batch_spmv_narrow(BatchView B, con
             const u64 e0 = B.row_ptr[r], e1 = B.row_ptr[r + 1];
             for (u64 e = e0 + lane; e < e1; e += LR) {
                 const u64 c = B.col[e];
-                const Zd v = B.val[e];
+                const Zd v = entry_value(B, e);
                 const Zd x = X[c * R + s];
                 ar += v.re * x.re - v.im * x.im;
                 ai += v.re * x.im + v.im * x.re;
@@ -127,7 +137,7 @@ __global__ __launch_bounds__(kThreads) void k_batch_spmv_wide_fused(BatchView B,
             double ar = 0.0, ai = 0.0;
             for (u64 e = e0; e < e1; ++e) {
                 const u64 c = B.col[e];
-                const Zd v = B.val[e];
+                const Zd v = entry_value(B, e);
                 const Zd x = Vcur[c * R + s];
                 ar += v.re * x.re - v.im * x.im;
                 ai += v.re * x.im + v.im * x.re;
@@ -319,8 +329,10 @@ __global__ void k_fill_zd(Zd* x, u64 n) {
 }
 
 // Pack one block's CSR into the global one: rows shifted by row_off, columns by col_off, values
-// expanded from the dictionary (width 1 / 2) or copied (width 0; double when real).
-__global__ void k_pack_csr(CsrView A, u64 row_off, u64 col_off, u64 nnz_off, u64* row_ptr, u32* col, Zd* val) {
+// expanded from the dictionary (width 1 / 2 / 3) or copied (width 0; double when real) -- or, with a
+// packed dictionary, the ids offset by the block's first entry in it.
+__global__ void k_pack_csr(CsrView A, u64 row_off, u64 col_off, u64 nnz_off, u64* row_ptr, u32* col, Zd* val,
+                           u16* vid, u32 dict_off) {
     const u64 r = blockIdx.x * static_cast<u64>(blockDim.x) + threadIdx.x;
     if (r > static_cast<u64>(A.rows)) return;
     row_ptr[row_off + r] = nnz_off + A.row_ptr[r];
@@ -328,6 +340,13 @@ __global__ void k_pack_csr(CsrView A, u64 row_off, u64 col_off, u64 nnz_off, u64
     for (u64 e = A.row_ptr[r]; e < A.row_ptr[r + 1]; ++e) {
         const u32 w = A.col[e];
         col[nnz_off + e] = static_cast<u32>(col_off + (A.width == 3 ? (w & 0xFFFFFFu) : w));
+        if (vid) {   // the caller checked that the block has ids and the offset ids fit 16 bits
+            const u32 id = A.width == 3 ? (w >> 24)
+                           : A.width == 1 ? static_cast<u32>(static_cast<const u8*>(A.vals)[e])
+                                          : static_cast<u32>(static_cast<const u16*>(A.vals)[e]);
+            vid[nnz_off + e] = static_cast<u16>(dict_off + id);
+            continue;
+        }
         Zd v;
         if (A.width == 3) {
             const u32 id = w >> 24;
@@ -365,8 +384,16 @@ void batch_spmv(const BatchView& B, const Zd* X, Zd* Y, hipStream_t st) {
         A.rows = static_cast<i64>(B.rows);
         A.row_ptr = B.row_ptr;
         A.col = B.col;
-        A.vals = B.val;
-        A.width = 0;
+        if (B.vid) {
+            A.vals = B.vid;
+            A.width = 2;
+            A.dict_re = B.dict_re;
+            A.dict_im = B.dict_im;
+            A.dict_n = static_cast<int>(B.dict_n);
+        } else {
+            A.vals = B.val;
+            A.width = 0;
+        }
         A.values_real = false;
         A.lanes = 8;
         A.xcd = 1;
@@ -430,11 +457,11 @@ void batch_fill_zero(Zd* x, u64 n, hipStream_t st) {
     hipLaunchKernelGGL(k_fill_zd, dim3(grid_for(n)), dim3(256), 0, st, x, n);
 }
 
-void pack_csr(const CsrView& A, u64 row_off, u64 col_off, u64 nnz_off, u64* row_ptr, u32* col, Zd* val,
-              hipStream_t st) {
+void pack_csr(const CsrView& A, u64 row_off, u64 col_off, u64 nnz_off, u64* row_ptr, u32* col, Zd* val, u16* vid,
+              u32 dict_off, hipStream_t st) {
     const u64 n = static_cast<u64>(A.rows) + 1;
     hipLaunchKernelGGL(k_pack_csr, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, st, A, row_off,
-                       col_off, nnz_off, row_ptr, col, val);
+                       col_off, nnz_off, row_ptr, col, val, vid, dict_off);
 }
 
 }  // namespace qsa::dev

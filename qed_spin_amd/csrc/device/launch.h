@@ -135,7 +135,11 @@ struct BatchView {
     const u32* tile_begin = nullptr;    // n_blocks + 1
     const u64* row_ptr = nullptr;       // packed block-diagonal CSR
     const u32* col = nullptr;
-    const Zd* val = nullptr;
+    const Zd* val = nullptr;            // the values in full, or (vid set) ids into the dictionary
+    const u16* vid = nullptr;
+    const double* dict_re = nullptr;
+    const double* dict_im = nullptr;
+    u32 dict_n = 0;
     int* step = nullptr;                // the current step (device; advanced by batch_advance)
     int* active = nullptr;              // n_blocks x R: the pair is still running
     int* steps = nullptr;               // n_blocks x R: steps recorded
@@ -167,7 +171,9 @@ void batch_fill_zero(Zd* x, u64 n, hipStream_t st);
 /// std::normal_distribution<double>, entry i = (n_2i, n_2i+1) written to
 /// V[(row_begin[p / R] + i) R + p % R] for i < dims[p / R]; `vectors` workgroups. Not normalised.
 void sample_gauss(Zd* V, int R, const u64* row_begin, const u64* dims, const u32* seeds, u32 vectors, hipStream_t st);
-void pack_csr(const CsrView& A, u64 row_off, u64 col_off, u64 nnz_off, u64* row_ptr, u32* col, Zd* val,
-              hipStream_t st);
+/// One block's CSR into the packed one: rows and columns offset, values in full into val, or (vid
+/// set, the block's CSR holding dictionary ids) the ids offset by dict_off into vid.
+void pack_csr(const CsrView& A, u64 row_off, u64 col_off, u64 nnz_off, u64* row_ptr, u32* col, Zd* val, u16* vid,
+              u32 dict_off, hipStream_t st);
 void batch_advance(const BatchView& B, hipStream_t st);
 }  // namespace qsa::dev

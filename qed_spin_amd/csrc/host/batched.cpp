@@ -90,10 +90,44 @@ BatchedLanczos::BatchedLanczos(std::vector<std::shared_ptr<const SectorOperator>
     const hipStream_t st = engine_stream();
     row_ptr_.reset((rows_ + 1) * 8);
     col_.reset(std::max<std::uint64_t>(nnz_, 1) * 4);
-    val_.reset(std::max<std::uint64_t>(nnz_, 1) * sizeof(dev::Zd));
+    // Values: with fewer than 8 vectors per block (the value stream is a large share of the bytes)
+    // and when every block stores dictionary ids and the dictionaries hold at most 2^16 values
+    // together, 16-bit ids into their concatenation (2 bytes per entry instead of 16); else in full
+    // (with 8 or more vectors the gathers dominate and a full value is one broadcast load).
+    std::vector<std::uint32_t> dict_off(ops_.size(), 0);
+    bool ids = R_ < 8;
+    std::uint64_t dict_total = 0;
+    for (std::size_t b = 0; ids && b < ops_.size(); ++b) {
+        const dev::CsrView A = ops_[b]->device_csr();
+        if (A.width < 1 || A.width > 3 || A.dict_n < 1 || !A.dict_re || (!A.values_real && !A.dict_im)) {
+            ids = false;
+            break;
+        }
+        dict_off[b] = static_cast<std::uint32_t>(dict_total);
+        dict_total += static_cast<std::uint64_t>(A.dict_n);
+    }
+    if (dict_total > (std::uint64_t{1} << 16)) ids = false;
+    if (ids) {
+        dict_n_ = static_cast<std::uint32_t>(dict_total);
+        vid_.reset(std::max<std::uint64_t>(nnz_, 1) * 2);
+        dict_re_.reset(dict_total * 8);
+        dict_im_.reset(dict_total * 8);
+        for (std::size_t b = 0; b < ops_.size(); ++b) {
+            const dev::CsrView A = ops_[b]->device_csr();
+            const std::size_t n = static_cast<std::size_t>(A.dict_n) * 8;
+            QSA_HIP(hipMemcpyAsync(dict_re_.get<double>() + dict_off[b], A.dict_re, n, hipMemcpyDeviceToDevice, st));
+            if (A.values_real)
+                dev::fill_zero(false, dict_im_.get<double>() + dict_off[b], static_cast<std::size_t>(A.dict_n), st);
+            else
+                QSA_HIP(hipMemcpyAsync(dict_im_.get<double>() + dict_off[b], A.dict_im, n, hipMemcpyDeviceToDevice, st));
+        }
+    } else {
+        val_.reset(std::max<std::uint64_t>(nnz_, 1) * sizeof(dev::Zd));
+    }
     for (std::size_t b = 0; b < ops_.size(); ++b)
         dev::pack_csr(ops_[b]->device_csr(), row_off_[b], row_off_[b], nnz_off[b], row_ptr_.get<std::uint64_t>(),
-                      col_.get<std::uint32_t>(), val_.get<dev::Zd>(), st);
+                      col_.get<std::uint32_t>(), ids ? nullptr : val_.get<dev::Zd>(),
+                      ids ? vid_.get<std::uint16_t>() : nullptr, dict_off[b], st);
     QSA_HIP(hipGetLastError());
     // Tiles: every block cut into runs of at most kBatchTileRows rows, blocks in order.
     tile_begin_.assign(1, 0);
@@ -140,6 +174,12 @@ dev::BatchView BatchedLanczos::view() const {
     B.row_ptr = row_ptr_.get<std::uint64_t>();
     B.col = col_.get<std::uint32_t>();
     B.val = val_.get<dev::Zd>();
+    if (dict_n_) {
+        B.vid = vid_.get<std::uint16_t>();
+        B.dict_re = dict_re_.get<double>();
+        B.dict_im = dict_im_.get<double>();
+        B.dict_n = dict_n_;
+    }
     B.step = step_d_.get<int>();
     B.active = active_d_.get<int>();
     B.steps = steps_d_.get<int>();
