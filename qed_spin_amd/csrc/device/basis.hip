@@ -184,17 +184,20 @@ __global__ void k_rank_fill(SectorView S, i32* table) {
 
 inline unsigned blocks_for(u64 n, unsigned t) { return static_cast<unsigned>((n + t - 1) / t); }
 
-// Up to this many states of the subspace the orbit scan takes one lane per state, above it one
-// thread per 64 states stepping by Gosper's rule (a rank -> state unranking per state costs more
-// than a step once the device is full).
+// Up to these many states of the subspace the orbit scan (flags) and the emission take one lane per
+// state, above them one thread per 64 states stepping by Gosper's rule: an unranking per state costs
+// more than a step once a thread per word fills the device (measured: at 4e7 states the flag pass
+// takes 1.6 ms a word per thread and 2.5 ms a state per lane, the emission, which unranks only the
+// kept states, 1.4 ms and 0.6 ms).
 constexpr u64 kLanePerStateMax = u64{1} << 26;
+constexpr u64 kLaneFlagsMax = u64{1} << 24;
 
 }  // namespace
 
 // The kept-representative bitmap and the orbit count of the subspace (before the character filter).
 void orbit_flags(const SectorView& S, u64 total, u64* kept, unsigned long long* orbits, hipStream_t st) {
     const u64 n_words = (total + 63) / 64;
-    if (total <= kLanePerStateMax) {
+    if (total <= kLaneFlagsMax) {
         const dim3 grid(blocks_for(total, 256));
         if (S.slc_L > 0)
             hipLaunchKernelGGL(k_orbit_flags_lane<true>, grid, dim3(256), 0, st, S, total, n_words, kept, orbits);

@@ -318,11 +318,12 @@ extern "C" int qedh_serve(const qedh_request_t* reqs, int n, const qedh_serve_op
             if (dims) dims[i] = r.dim;
         }
         if (stats) {
-            const double s[12] = {st.wall_s, st.solves_per_s, st.p50_s, st.p99_s, st.mean_s, st.max_s,
+            const double s[16] = {st.wall_s, st.solves_per_s, st.p50_s, st.p99_s, st.mean_s, st.max_s,
                                   static_cast<double>(st.completed), static_cast<double>(st.failed),
                                   static_cast<double>(st.peak_reserved), static_cast<double>(st.peak_in_flight),
-                                  static_cast<double>(st.sector_builds), static_cast<double>(st.sector_hits)};
-            std::copy(s, s + 12, stats);
+                                  static_cast<double>(st.sector_builds), static_cast<double>(st.sector_hits),
+                                  st.build_s, st.service_s[0], st.service_s[1], st.service_s[2]};
+            std::copy(s, s + 16, stats);
         }
     });
 }

@@ -168,6 +168,11 @@ double kbreak(double sH) { return 64.0 * std::numeric_limits<double>::epsilon() 
 /// One request on the calling thread's engine stream.
 void solve(const Request& r, const ServeOptions& o, SectorCache& cache, RequestResult& res, int& builds, int& hits) {
     const int N = r.n_sites, n_up = N / 2;
+    auto tb = clk::now();
+    auto build_done = [&] {   // build phases end: their device work is finished before the clock stops
+        QSA_HIP(hipStreamSynchronize(engine_stream()));
+        res.build_s += std::chrono::duration<double>(clk::now() - tb).count();
+    };
     bool built = false;
     auto src = cache.get(N, n_up, 0, built);
     (built ? builds : hits) += 1;
@@ -178,6 +183,7 @@ void solve(const Request& r, const ServeOptions& o, SectorCache& cache, RequestR
     auto hprog = std::make_shared<DeviceProgram>(compile_operator(H.dagger()));
     // fixed SpMV width: a request's result must not depend on what else runs
     auto Hop = std::make_shared<SectorOperator>(src, src, hprog, sH, kLaneCsr, ~0ull, /*tune=*/false);
+    build_done();
     const std::size_t n = static_cast<std::size_t>(src->dim());
     HipBackend<cplx> be;
     auto applyH = [&](const cplx* in, cplx* out) { Hop->apply(in, out, true); };
@@ -219,6 +225,7 @@ void solve(const Request& r, const ServeOptions& o, SectorCache& cache, RequestR
         // S^z(q) = N^-1/2 sum_j e^{-i q j} S^z_j, q = 2 pi m / N nearest pi; it takes momentum 0 to -m
         const int m = N / 2;
         const int mt = (N - m) % N;
+        tb = clk::now();
         bool b2 = false;
         auto tgt = cache.get(N, n_up, mt, b2);
         (b2 ? builds : hits) += 1;
@@ -235,6 +242,7 @@ void solve(const Request& r, const ServeOptions& o, SectorCache& cache, RequestR
         auto phi = be.make_zero_vector(nt);
         X.apply(gs.get(), phi.get(), true);
         SectorOperator Ht(tgt, tgt, hprog, sH, kLaneCsr, ~0ull, false);
+        build_done();
         auto applyT = [&](const cplx* in, cplx* out) { Ht.apply(in, out, true); };
         const CfResult cf = cf_from_vector(be, applyT, nt, phi.get(), o.omega, o.eta,
                                            static_cast<std::size_t>(std::max(2, r.krylov)), true, e0, kbreak(sH));
@@ -344,6 +352,9 @@ ServeStats serve(const std::vector<Request>& reqs, const ServeOptions& o, std::v
                 --in_flight;
                 st.sector_builds += builds;
                 st.sector_hits += hits;
+                st.build_s += res.build_s;
+                const int kind = reqs[idx].kind;
+                if (kind >= 0 && kind < 3) st.service_s[kind] += res.finish_s - res.start_s;
                 if (res.status == 0) ++st.completed;
                 else ++st.failed;
                 const double now = now_s();

@@ -45,6 +45,7 @@ struct RequestResult {
     std::vector<double> values;  ///< eigs: k levels; thermal: lnZ, E, V per beta; dynamics: S(omega)
     double e0 = 0.0;
     double start_s = 0.0, finish_s = 0.0, latency_s = 0.0;
+    double build_s = 0.0;         ///< of start..finish: sectors, programs and CSRs built for it
     std::uint64_t reserved_bytes = 0;
     std::int64_t dim = 0;
     int slot = -1;
@@ -59,6 +60,8 @@ struct ServeStats {
     std::uint64_t peak_reserved = 0;
     int peak_in_flight = 0;
     int sector_builds = 0, sector_hits = 0;
+    double build_s = 0.0;                      ///< summed over requests
+    double service_s[3] = {0.0, 0.0, 0.0};     ///< start..finish summed per kind
 };
 
 /// The reserved HBM bytes of a request (sector, CSR, Krylov vectors), before anything is built.

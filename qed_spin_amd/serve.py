@@ -129,7 +129,7 @@ def serve(requests: List[dict], *, slots: int = 8, batched: bool = True, cache_s
     timing = np.zeros((n, 4))
     status = np.zeros(n, np.int32)
     dims = np.zeros(n, np.int64)
-    stats = np.zeros(12)
+    stats = np.zeros(16)
     _lib.call("qedh_serve", arr, n, C.byref(o), int(stride), _lib.ptr(values, C.c_double), _lib.ptr(e0, C.c_double),
               _lib.ptr(timing, C.c_double), _lib.ptr(status, C.c_int), _lib.ptr(dims, C.c_int64),
               _lib.ptr(stats, C.c_double))
@@ -141,6 +141,7 @@ def serve(requests: List[dict], *, slots: int = 8, batched: bool = True, cache_s
             width = min(width, int(dims[i])) if dims[i] > 0 else width
         out_vals.append(values[i, :width].copy())
     keys = ["wall_s", "solves_per_s", "p50_s", "p99_s", "mean_s", "max_s", "completed", "failed", "peak_reserved",
-            "peak_in_flight", "sector_builds", "sector_hits"]
+            "peak_in_flight", "sector_builds", "sector_hits", "build_s", "service_eigs_s", "service_thermal_s",
+            "service_dynamics_s"]
     return ServeResult(values=out_vals, e0=e0, timing=timing, status=status, dims=dims,
                        stats=dict(zip(keys, stats.tolist())))
