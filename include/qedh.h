@@ -237,6 +237,12 @@ int qedh_ftlm_dynamics(qedh_matvec source_h, int n_targets, const qedh_matvec* t
  * and ftlm_dynamics.h:157 draw them). out: device, [count][n] complex (re, im). The entries equal the
  * host's up to the last bit of the logarithm. */
 int qedh_gaussian_vectors(uint64_t base, uint64_t first_sample, int count, int64_t n, double* out);
+/* The whole spectrum of small blocks, ascending, by a dense solve on the host (the reference's
+ * HostDense lane of the eigs walk: solve_block_dense, src/engine/block_solve.cpp:199-262, taken for
+ * blocks at or below lowest_dense_floor, block_solve.cpp:164-197): each block's CSR expanded to a
+ * dense Hermitian matrix, Householder tridiagonalisation and implicit QL. values: the blocks'
+ * spectra one after another (dim of each block entries); blocks of at most 8192 states. */
+int qedh_block_spectrum(const qedh_matvec* blocks, int n_blocks, double* values);
 /* Exact thermodynamics of small blocks (the reference's host dense lane for blocks of at most
  * dense_max_dim states, src/engine/thermal.cpp:197-200, curves of include/ed/thermal/curves.h:38-65):
  * each block's CSR expanded to a dense Hermitian matrix on the host and its whole spectrum solved.
@@ -279,8 +285,10 @@ typedef struct {
 } qedh_serve_options_t;
 /* Run the stream. values[n][stride]: eigs levels / thermal lnZ, E, V per beta / dynamics S(omega);
  * e0[n]; timing[n][4] = arrival, start, finish, latency (s); status[n] (0 ok); dims[n];
- * stats[12] = wall s, solves/s, p50, p99, mean, max latency, completed, failed, peak reserved bytes,
- * peak in flight, sector builds, sector cache hits. Every output but values may be NULL. */
+ * stats[16] = wall s, solves/s, p50, p99, mean, max latency, completed, failed, peak reserved bytes,
+ * peak in flight, sector builds, sector cache hits, seconds of the requests spent building sectors,
+ * programs and CSRs (summed), and start-to-finish seconds summed per kind (eigs, thermal, dynamics).
+ * Every output but values may be NULL. */
 int qedh_serve(const qedh_request_t* reqs, int n, const qedh_serve_options_t* opt, int stride, double* values,
                double* e0, double* timing, int* status, int64_t* dims, double* stats);
 /* The HBM bytes a request reserves at admission. */

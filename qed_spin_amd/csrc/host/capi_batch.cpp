@@ -130,38 +130,62 @@ extern "C" int qedh_ftlm_batched(const qedh_matvec* blocks, int n_blocks, int sa
 /* Exact thermodynamics of small blocks: each block's CSR is expanded to a dense Hermitian matrix on
  * the host and its whole spectrum solved (the reference's host dense lane for blocks at most
  * dense_max_dim, src/engine/thermal.cpp:197-200), blocks in parallel on the host threads. */
+namespace {
+
+/// Every block's whole spectrum, ascending: each CSR expanded to a dense Hermitian matrix on this
+/// thread (device calls stay on one thread), the matrices solved on the host pool.
+std::vector<std::vector<double>> dense_spectra(const std::vector<std::shared_ptr<const qsa::SectorOperator>>& ops,
+                                               const char* who) {
+    for (const auto& op : ops) {
+        if (!op->same()) throw qsa::Error(qsa::kInvalid, std::string(who) + ": a block must be square");
+        if (op->lane() != qsa::kLaneCsr)
+            throw qsa::Error(qsa::kDeviceUnsupported, std::string(who) + ": a block needs its CSR (csr lane)");
+        if (op->dim() > 8192) throw qsa::Error(qsa::kInvalid, std::string(who) + ": block above 8192 states");
+    }
+    std::vector<std::vector<std::complex<double>>> dense(ops.size());
+    for (std::size_t b = 0; b < ops.size(); ++b) {
+        std::vector<std::uint64_t> rp;
+        std::vector<std::uint32_t> col;
+        std::vector<double> val;
+        ops[b]->csr_download(rp, col, val);
+        const std::size_t n = static_cast<std::size_t>(ops[b]->dim());
+        dense[b].assign(n * n, std::complex<double>(0.0, 0.0));
+        for (std::size_t r = 0; r < n; ++r)
+            for (std::uint64_t e = rp[r]; e < rp[r + 1]; ++e)
+                dense[b][col[e] * n + r] += std::complex<double>(val[2 * e], val[2 * e + 1]);
+    }
+    std::vector<std::vector<double>> ev(ops.size());
+    qsa::parallel_blocks(ops.size(), [&](std::size_t b) {
+        const std::size_t n = static_cast<std::size_t>(ops[b]->dim());
+        ev[b] = qsa::hermitian_eigvals_householder(std::move(dense[b]), n);
+    });
+    return ev;
+}
+
+}  // namespace
+
+extern "C" int qedh_block_spectrum(const qedh_matvec* blocks, int n_blocks, double* values) {
+    return guarded_batch([&] {
+        if (!values) throw qsa::Error(qsa::kInvalid, "qedh_block_spectrum: null argument");
+        const auto ev = dense_spectra(gather_blocks(blocks, n_blocks), "qedh_block_spectrum");
+        std::size_t at = 0;
+        for (const auto& e : ev) {
+            std::copy(e.begin(), e.end(), values + at);
+            at += e.size();
+        }
+    });
+}
+
 extern "C" int qedh_exact_thermal(const qedh_matvec* blocks, int n_blocks, int n_beta, const double* betas,
                                   double* lnZ, double* E, double* V, double* e_min) {
     return guarded_batch([&] {
         if (!betas || !lnZ || !E || !V || n_beta < 1)
             throw qsa::Error(qsa::kInvalid, "qedh_exact_thermal: null argument or no temperature");
         const auto ops = gather_blocks(blocks, n_blocks);
-        for (const auto& op : ops) {
-            if (!op->same()) throw qsa::Error(qsa::kInvalid, "qedh_exact_thermal: a block must be square");
-            if (op->lane() != qsa::kLaneCsr)
-                throw qsa::Error(qsa::kDeviceUnsupported, "qedh_exact_thermal: a block needs its CSR (csr lane)");
-            if (op->dim() > 8192) throw qsa::Error(qsa::kInvalid, "qedh_exact_thermal: block above 8192 states");
-        }
-        // Download on this thread (device calls stay on one thread), solve on the pool.
-        std::vector<std::vector<std::complex<double>>> dense(ops.size());
-        for (std::size_t b = 0; b < ops.size(); ++b) {
-            std::vector<std::uint64_t> rp;
-            std::vector<std::uint32_t> col;
-            std::vector<double> val;
-            ops[b]->csr_download(rp, col, val);
-            const std::size_t n = static_cast<std::size_t>(ops[b]->dim());
-            dense[b].assign(n * n, std::complex<double>(0.0, 0.0));
-            for (std::size_t r = 0; r < n; ++r)
-                for (std::uint64_t e = rp[r]; e < rp[r + 1]; ++e)
-                    dense[b][col[e] * n + r] += std::complex<double>(val[2 * e], val[2 * e + 1]);
-        }
+        const auto ev = dense_spectra(ops, "qedh_exact_thermal");
         const std::vector<double> bt(betas, betas + n_beta);
         std::vector<qsa::Curves> out(ops.size());
-        qsa::parallel_blocks(ops.size(), [&](std::size_t b) {
-            const std::size_t n = static_cast<std::size_t>(ops[b]->dim());
-            const auto ev = qsa::hermitian_eigvals_householder(std::move(dense[b]), n);
-            out[b] = qsa::exact_curves(ev, bt);
-        });
+        qsa::parallel_blocks(ops.size(), [&](std::size_t b) { out[b] = qsa::exact_curves(ev[b], bt); });
         for (int b = 0; b < n_blocks; ++b) {
             for (int i = 0; i < n_beta; ++i) {
                 lnZ[b * n_beta + i] = out[b].lnZ[i];

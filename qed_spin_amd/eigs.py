@@ -1,0 +1,95 @@
+"""The eigs verb on the HIP backend: the k lowest levels of H over the blocks of a symmetry walk (the
+reference's qed.eigs, python/qed/_verbs/eigs.py, driving src/engine/eigs.cpp:360-450 per block).
+
+Per block (in the walk's order): want = min(k, dim) levels; blocks of at most two states, or at
+most the dense floor, are solved densely on the host (the reference's HostDense lane,
+src/engine/block_solve.cpp:164-262); the others on the device, k = 1 by the lowest-level scan and
+k > 1 by Krylov-Schur (block_solve.cpp:263-655). The levels of every block are merged and the
+lowest k returned, ascending.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import time
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional
+
+import numpy as np
+
+from . import _lib
+from .errors import InvalidRequest
+from .operator import Operator
+from .sector import DEFAULT_CSR_BUDGET
+from .symmetry import Symmetry
+from .thermal import _handles, resolve_device, walk_blocks
+
+# the automatic dense floor's ceiling (src/engine/internal.h:815) and the dense lane's limit here
+AUTO_DENSE_CEILING = 8192
+
+
+def dense_floor(k: int, dense_max_dim: Optional[int]) -> int:
+    """The largest block solved densely (lowest_dense_floor, block_solve.cpp:164-197): an explicit
+    dense_max_dim as given; automatically 4 max(40 k, 400), at most 8192. (The reference also caps
+    it by the host memory a dense solve may take; at 8192 states that is 1.6 GB, always available.)"""
+    if dense_max_dim is not None and dense_max_dim >= 0:
+        return min(int(dense_max_dim), AUTO_DENSE_CEILING)
+    return min(4 * max(40 * int(k), 400), AUTO_DENSE_CEILING)
+
+
+@dataclass
+class EigsResult:
+    energies: np.ndarray
+    blocks: List[dict] = field(default_factory=list)
+    converged: bool = True
+    timings: Dict[str, float] = field(default_factory=dict)
+
+
+def eigs(H: Operator, k: int = 4, *, sym: Optional[Symmetry] = None, device: str = "gpu",
+         dense_max_dim: Optional[int] = None, max_iter: int = 0,
+         csr_budget: int = DEFAULT_CSR_BUDGET) -> EigsResult:
+    """The k lowest levels of H over the blocks of `sym` (default: every (Sz, k) block of a ring)."""
+    resolve_device(device, "eigs")
+    if int(k) < 1:
+        raise InvalidRequest("eigs: k must be >= 1")
+    k = int(k)
+    sym = sym if sym is not None else Symmetry()
+    t0 = time.perf_counter()
+    blocks = walk_blocks(H, sym, csr_budget)
+    if not blocks:
+        raise InvalidRequest("eigs: the symmetry selection holds no state")
+    t_build = time.perf_counter() - t0
+    floor = dense_floor(k, dense_max_dim)
+    recs = []
+    dense_idx = [i for i, (_, s, _) in enumerate(blocks) if s.dim <= 2 or s.dim <= floor]
+    levels = {}
+    t1 = time.perf_counter()
+    if dense_idx:
+        mats = [blocks[i][2] for i in dense_idx]
+        dims = [m.dim for m in mats]
+        vals = np.zeros(int(sum(dims)))
+        _lib.call("qedh_block_spectrum", _handles(mats), len(mats), _lib.ptr(vals, C.c_double))
+        at = 0
+        for i, d in zip(dense_idx, dims):
+            levels[i] = (vals[at: at + min(k, d)].copy(), True, "host-dense")
+            at += d
+    t_dense = time.perf_counter() - t1
+    t2 = time.perf_counter()
+    converged = True
+    for i, (blk, sec, m) in enumerate(blocks):
+        if i in levels:
+            continue
+        want = min(k, sec.dim)
+        v, conv, _ = m.lowest(want, max_iter)
+        converged = converged and conv
+        levels[i] = (np.asarray(v), conv, "device-krylov")
+    t_krylov = time.perf_counter() - t2
+    allv = []
+    for i, (blk, sec, _) in enumerate(blocks):
+        v, conv, lane = levels[i]
+        recs.append({"label": blk.label, "dim": sec.dim, "lane": lane, "levels": v, "converged": conv})
+        allv.extend(v.tolist())
+    energies = np.sort(np.asarray(allv))[:k]
+    return EigsResult(energies=energies, blocks=recs, converged=converged,
+                      timings={"build_s": t_build, "dense_s": t_dense, "krylov_s": t_krylov,
+                               "total_s": time.perf_counter() - t0})

@@ -1,0 +1,59 @@
+"""GPU parity of the eigs verb: the k = 4 lowest levels of j1j2_chain12 against the reference's own
+recorded levels (tests/python/golden/refs/api-2026-09/cpu.json.gz, copied to
+tests/golden/fixtures/reference_records.json) within 1e-10, over the whole space, every (Sz, k)
+block, the Sz blocks without spatial symmetry and the Sz = 0 block alone; each with the automatic
+dense floor (small blocks solved densely on the host) and with every block above two states on the
+device Krylov lanes (dense_max_dim = 0). Every case prints its name before it touches the device."""
+
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import FIXTURES
+
+import qed_spin_amd as qs
+from qed_spin_amd import eigs as ev
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "none": qs.Symmetry.none(),
+    "auto": qs.Symmetry(),
+    "spatial=none": qs.Symmetry(translations=False),
+    "sz=6": qs.Symmetry(sz=[6], translations=False),
+    "auto/sz=6": qs.Symmetry(sz=[6]),
+}
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def _records():
+    with open(os.path.join(FIXTURES, "reference_records.json")) as f:
+        return json.load(f)["records"]
+
+
+def _h12():
+    n = 12
+    b = qs.HamiltonianBuilder(n)
+    b.heisenberg([(i, (i + 1) % n) for i in range(n)], 1.0)
+    b.heisenberg([(i, (i + 2) % n) for i in range(n)], 0.5)
+    return b.to_operator()
+
+
+@pytest.mark.parametrize("dense_max_dim", [None, 0])
+@pytest.mark.parametrize("key", list(CASES))
+def test_eigs_reproduces_reference_records(key, dense_max_dim):
+    say("eigs", key, "dense_max_dim", dense_max_dim)
+    want = _records()[f"j1j2_chain12/api/eigs/{key}/k=4"]["values"]["energies"]
+    r = ev.eigs(_h12(), 4, sym=CASES[key], dense_max_dim=dense_max_dim)
+    assert r.converged
+    assert np.allclose(r.energies, want, rtol=1e-10, atol=1e-10), (r.energies, want)
+    lanes = {b["lane"] for b in r.blocks}
+    if dense_max_dim == 0:
+        assert "device-krylov" in lanes

@@ -294,3 +294,21 @@ def test_exact_thermal_reproduces_reference_golden_record():
         assert abs(E - r["E"][i]) < 1e-10 * max(1.0, abs(E))
         assert abs(b * b * V - r["C"][i]) < 1e-10 * max(1.0, abs(r["C"][i]))
         assert abs(np.log(Z) - b * e0 + b * E - r["S"][i]) < 1e-10 * max(1.0, abs(r["S"][i]))
+
+
+def test_reference_eigs_records_are_the_dense_spectrum():
+    """The reference's recorded eigs levels of j1j2_chain12 (k = 4) are the lowest levels of the
+    oracle's dense matrix: of the whole space (no symmetry, every symmetry, Sz blocks without spatial
+    symmetry) and of the Sz = 0 block alone (popcount-6 states of the same matrix)."""
+    rec = _reference_records()["records"]
+    s, row_ptr, col, val = _full_space_j1j2_chain12()
+    Hd = dense_csr(row_ptr, col, val, s.dim)
+    ev = np.linalg.eigvalsh(Hd)
+    for key in ("none", "auto", "spatial=none"):
+        got = rec[f"j1j2_chain12/api/eigs/{key}/k=4"]["values"]["energies"]
+        assert np.allclose(got, ev[:4], rtol=1e-10, atol=1e-10), key
+    sz0 = np.array([i for i in range(1 << 12) if bin(i).count("1") == 6])
+    ev0 = np.linalg.eigvalsh(Hd[np.ix_(sz0, sz0)])
+    for key in ("sz=6", "auto/sz=6"):
+        got = rec[f"j1j2_chain12/api/eigs/{key}/k=4"]["values"]["energies"]
+        assert np.allclose(got, ev0[:4], rtol=1e-10, atol=1e-10), key
