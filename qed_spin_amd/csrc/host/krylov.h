@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <future>
 #include <limits>
 #include <numeric>
 #include <random>
@@ -39,6 +40,41 @@ template <class Scalar> std::vector<Scalar> gaussian_start(std::size_t n, std::u
     }
     return h;
 }
+
+/// Successive Gaussian vectors of one std::mt19937_64 stream, a fresh normal_distribution per
+/// vector (krylov_schur.h:74-75, 240-276). With prefetch the first vector is drawn on a host thread
+/// from construction on, while the caller's device work runs; later ones are drawn when taken.
+template <class Scalar> class SeedStream {
+public:
+    SeedStream(std::uint64_t seed, std::size_t n, bool prefetch) : gen_(seed), n_(n) {
+        if (prefetch) next_ = std::async(std::launch::async, [this] { return draw(); });
+    }
+    ~SeedStream() {
+        if (next_.valid()) next_.wait();
+    }
+    SeedStream(const SeedStream&) = delete;
+    SeedStream& operator=(const SeedStream&) = delete;
+    std::vector<Scalar> take() { return next_.valid() ? next_.get() : draw(); }
+
+private:
+    std::vector<Scalar> draw() {
+        std::vector<Scalar> h(n_);
+        std::normal_distribution<double> nd(0.0, 1.0);
+        for (auto& z : h) {
+            if constexpr (std::is_same_v<Scalar, double>) {
+                z = nd(gen_);
+            } else {
+                const double re = nd(gen_);
+                const double im = nd(gen_);
+                z = Scalar(re, im);
+            }
+        }
+        return h;
+    }
+    std::mt19937_64 gen_;
+    std::size_t n_;
+    std::future<std::vector<Scalar>> next_;
+};
 
 /// The lowest LEVELS of Ritz values w[0..m) (ascending): values within 1e-9 scale of a level's first
 /// copy are that level, converged when any copy's Paige bound is <= 1e-7 scale; the walk stops at
@@ -262,19 +298,12 @@ KsResult krylov_schur(const HipBackend<Scalar>& be, Matvec&& matvec, std::size_t
         }
         return kBudget;
     };
-    std::mt19937_64 fresh_gen(0xF8E5A7C3ULL);
-    auto fresh_seed = [&](std::mt19937_64& gen) {
-        std::vector<Scalar> host(n);
-        std::normal_distribution<double> nd(0.0, 1.0);
-        for (auto& z : host) {
-            if constexpr (std::is_same_v<Scalar, double>) {
-                z = nd(gen);
-            } else {
-                const double re = nd(gen);
-                const double im = nd(gen);
-                z = Scalar(re, im);
-            }
-        }
+    // fresh deflated starts, and the degeneracy probe's starts (its first one drawn on a host thread
+    // while the search runs)
+    SeedStream<Scalar> fresh_gen(0xF8E5A7C3ULL, n, false);
+    SeedStream<Scalar> probe_gen(0xDE6E4E7AULL, n, opts.probe_degeneracy);
+    auto fresh_seed = [&](SeedStream<Scalar>& gen) {
+        const std::vector<Scalar> host = gen.take();
         be.copy_from_host(host.data(), col(0), n);
     };
     be.copy(seed, col(0), n);
@@ -300,7 +329,6 @@ KsResult krylov_schur(const HipBackend<Scalar>& be, Matvec&& matvec, std::size_t
     }
     bool converged = !stalled && found_vals.size() >= k;
     if (!found_vals.empty() && !exhausted && opts.probe_degeneracy) {
-        std::mt19937_64 probe_gen(0xDE6E4E7AULL);
         for (std::size_t round = 0; round < k; ++round) {
             const double top = *std::max_element(found_vals.begin(), found_vals.end());
             const double gap = std::max(10.0 * opts.tolerance, 1e-8 * std::abs(top));
@@ -349,7 +377,8 @@ KsResult krylov_schur(const HipBackend<Scalar>& be, Matvec&& matvec, std::size_t
 /// tolerance kLockRel * s_H, breakdown kBreakdownRel * s_H.
 template <class Scalar, class Matvec>
 BlockSolution krylov_schur_lane(const HipBackend<Scalar>& be, Matvec&& H, std::size_t n, std::size_t k,
-                                double norm_bound, std::uint64_t max_iter, Scalar* vectors_out = nullptr) {
+                                double norm_bound, std::uint64_t max_iter, Scalar* vectors_out = nullptr,
+                                std::future<std::vector<Scalar>>* start = nullptr) {
     const std::uint64_t budget = max_iter > 0 ? max_iter : std::max<std::uint64_t>(200u * k, 2000u);
     const std::size_t per_cycle =
         std::min<std::size_t>(krylov_subspace_dim(k, 2 * k + 60, n, 0), static_cast<std::size_t>(std::max<std::uint64_t>(budget, k + 1)));
@@ -364,7 +393,9 @@ BlockSolution krylov_schur_lane(const HipBackend<Scalar>& be, Matvec&& H, std::s
     o.tolerance = kLockRel * scale;
     o.breakdown_tol = kBreakdownRel * scale;
     o.max_subspace_vectors = per_cycle;
-    const auto h = gaussian_start<Scalar>(n, kKsStartSeed);
+    // the start: gaussian_start(n, kKsStartSeed), or the same vector the caller began drawing earlier
+    const auto h = start && start->valid() ? start->get() : gaussian_start<Scalar>(n, kKsStartSeed);
+    if (h.size() != n) throw Error(kInvalid, "krylov_schur: start vector of the wrong length");
     auto v0 = be.make_zero_vector(n);
     be.copy_from_host(h.data(), v0.get(), n);
     const KsResult r = krylov_schur(be, H, n, v0.get(), o, vectors_out, k);
