@@ -237,16 +237,20 @@ int qedh_ftlm_dynamics(qedh_matvec source_h, int n_targets, const qedh_matvec* t
  * and ftlm_dynamics.h:157 draw them). out: device, [count][n] complex (re, im). The entries equal the
  * host's up to the last bit of the logarithm. */
 int qedh_gaussian_vectors(uint64_t base, uint64_t first_sample, int count, int64_t n, double* out);
-/* The whole spectrum of small blocks, ascending, by a dense solve on the host (the reference's
- * HostDense lane of the eigs walk: solve_block_dense, src/engine/block_solve.cpp:199-262, taken for
- * blocks at or below lowest_dense_floor, block_solve.cpp:164-197): each block's CSR expanded to a
- * dense Hermitian matrix, Householder tridiagonalisation and implicit QL. values: the blocks'
- * spectra one after another (dim of each block entries); blocks of at most 8192 states. */
-int qedh_block_spectrum(const qedh_matvec* blocks, int n_blocks, double* values);
-/* Exact thermodynamics of small blocks (the reference's host dense lane for blocks of at most
+/* The whole spectrum of blocks, ascending, by a dense solve (the reference's dense lanes of the eigs
+ * and spectrum walks: solve_block_dense, src/engine/block_solve.cpp:164-262, and the DenseBatch lane,
+ * src/gpu/little_group.cu:38-211): blocks of at least device_min_dim states (< 0: 1024, the
+ * reference's kDeviceDenseMinDim under 'auto'; 0: every block) and every block above 8192 on the
+ * device (CSR expanded to a dense matrix on the device, rocSOLVER zheevd in strided batches per
+ * dimension), the others on the host (Householder tridiagonalisation and implicit QL, blocks in
+ * parallel). values: the blocks' spectra one after another (dim of each block entries); blocks of at
+ * most 32768 states. */
+int qedh_block_spectrum(const qedh_matvec* blocks, int n_blocks, int64_t device_min_dim, double* values);
+/* Exact thermodynamics of blocks solved densely (the reference's dense lanes for blocks of at most
  * dense_max_dim states, src/engine/thermal.cpp:197-200, curves of include/ed/thermal/curves.h:38-65):
- * each block's CSR expanded to a dense Hermitian matrix on the host and its whole spectrum solved.
- * lnZ/E/V[n_blocks][n_beta], e_min[n_blocks]; blocks of at most 8192 states. */
+ * each block's whole spectrum as qedh_block_spectrum solves it (blocks from 1024 states on the
+ * device), then its curves. lnZ/E/V[n_blocks][n_beta], e_min[n_blocks]; blocks of at most 32768
+ * states. */
 int qedh_exact_thermal(const qedh_matvec* blocks, int n_blocks, int n_beta, const double* betas, double* lnZ,
                        double* E, double* V, double* e_min);
 /* The system's curves from its blocks' (src/engine/thermal.cpp:597-640): weights[n_blocks]

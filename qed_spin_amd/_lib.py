@@ -116,7 +116,7 @@ _SIGNATURES = {
         C.c_int, c_dbl_p, C.c_int, c_dbl_p, C.c_double, c_dbl_p, c_dbl_p, c_dbl_p, c_int_p,
     ],
     "qedh_gaussian_vectors": [C.c_uint64, C.c_uint64, C.c_int, C.c_int64, C.c_void_p],
-    "qedh_block_spectrum": [C.POINTER(C.c_void_p), C.c_int, c_dbl_p],
+    "qedh_block_spectrum": [C.POINTER(C.c_void_p), C.c_int, C.c_int64, c_dbl_p],
     "qedh_exact_thermal": [
         C.POINTER(C.c_void_p), C.c_int, C.c_int, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p,
     ],

@@ -328,6 +328,40 @@ __global__ void k_fill_zd(Zd* x, u64 n) {
         x[i] = Zd{0.0, 0.0};
 }
 
+// Entry e of a sector CSR: its column and its value (dictionary ids of width 1 / 2 / 3, or in full).
+__device__ __forceinline__ void decode_entry(const CsrView& A, u64 e, u32& c, Zd& v) {
+    const u32 w = A.col[e];
+    c = A.width == 3 ? (w & 0xFFFFFFu) : w;
+    if (A.width == 3) {
+        const u32 id = w >> 24;
+        v = Zd{A.dict_re[id], A.values_real ? 0.0 : A.dict_im[id]};
+    } else if (A.width == 1) {
+        const u32 id = static_cast<const u8*>(A.vals)[e];
+        v = Zd{A.dict_re[id], A.values_real ? 0.0 : A.dict_im[id]};
+    } else if (A.width == 2) {
+        const u32 id = static_cast<const u16*>(A.vals)[e];
+        v = Zd{A.dict_re[id], A.values_real ? 0.0 : A.dict_im[id]};
+    } else if (A.values_real) {
+        v = Zd{static_cast<const double*>(A.vals)[e], 0.0};
+    } else {
+        v = static_cast<const Zd*>(A.vals)[e];
+    }
+}
+
+// A square sector CSR as a dense column-major matrix (out zeroed first): out[c n + r] = H[r, c], one
+// thread per row (a merged row holds each column once).
+__global__ void k_csr_to_dense(CsrView A, Zd* out) {
+    const u64 r = blockIdx.x * static_cast<u64>(blockDim.x) + threadIdx.x;
+    const u64 n = static_cast<u64>(A.rows);
+    if (r >= n) return;
+    for (u64 e = A.row_ptr[r]; e < A.row_ptr[r + 1]; ++e) {
+        u32 c;
+        Zd v;
+        decode_entry(A, e, c, v);
+        out[static_cast<u64>(c) * n + r] = v;
+    }
+}
+
 // Pack one block's CSR into the global one: rows shifted by row_off, columns by col_off, values
 // expanded from the dictionary (width 1 / 2 / 3) or copied (width 0; double when real) -- or, with a
 // packed dictionary, the ids offset by the block's first entry in it.
@@ -446,6 +480,13 @@ void batch_normalize(const BatchView& B, Zd* V, double* partial, double* inv, do
 void batch_place(Zd* V, int R, u64 row0, int s, const Zd* src, u64 n, hipStream_t st) {
     if (!n) return;
     hipLaunchKernelGGL(k_batch_place, dim3(grid_for(n)), dim3(256), 0, st, V, R, row0, s, src, n);
+}
+
+void csr_to_dense(const CsrView& A, Zd* out, hipStream_t st) {
+    const u64 n = static_cast<u64>(A.rows);
+    if (!n) return;
+    batch_fill_zero(out, n * n, st);
+    hipLaunchKernelGGL(k_csr_to_dense, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, st, A, out);
 }
 
 void batch_advance(const BatchView& B, hipStream_t st) {

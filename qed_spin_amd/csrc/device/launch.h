@@ -176,4 +176,6 @@ void sample_gauss(Zd* V, int R, const u64* row_begin, const u64* dims, const u32
 void pack_csr(const CsrView& A, u64 row_off, u64 col_off, u64 nnz_off, u64* row_ptr, u32* col, Zd* val, u16* vid,
               u32 dict_off, hipStream_t st);
 void batch_advance(const BatchView& B, hipStream_t st);
+/// A square sector CSR (rows x rows) as a dense column-major complex matrix, zero-filled first.
+void csr_to_dense(const CsrView& A, Zd* out, hipStream_t st);
 }  // namespace qsa::dev

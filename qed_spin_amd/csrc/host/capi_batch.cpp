@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "batched.h"
+#include "dense_device.h"
 #include "ftlm_dynamics.h"
 #include "serve.h"
 #include "tridiag.h"
@@ -132,42 +133,61 @@ extern "C" int qedh_ftlm_batched(const qedh_matvec* blocks, int n_blocks, int sa
  * dense_max_dim, src/engine/thermal.cpp:197-200), blocks in parallel on the host threads. */
 namespace {
 
-/// Every block's whole spectrum, ascending: each CSR expanded to a dense Hermitian matrix on this
-/// thread (device calls stay on one thread), the matrices solved on the host pool.
+/// Every block's whole spectrum, ascending. Blocks of at least device_min_dim states (and every
+/// block above the host limit of 8192) on the device lane (rocSOLVER, batched per dimension); the
+/// others expanded to dense Hermitian matrices on this thread (device calls stay on one thread)
+/// and solved on the host pool.
 std::vector<std::vector<double>> dense_spectra(const std::vector<std::shared_ptr<const qsa::SectorOperator>>& ops,
-                                               const char* who) {
-    for (const auto& op : ops) {
+                                               const char* who, std::int64_t device_min_dim) {
+    std::vector<const qsa::SectorOperator*> dev_ops;
+    std::vector<std::size_t> dev_idx, host_idx;
+    for (std::size_t b = 0; b < ops.size(); ++b) {
+        const auto& op = ops[b];
         if (!op->same()) throw qsa::Error(qsa::kInvalid, std::string(who) + ": a block must be square");
         if (op->lane() != qsa::kLaneCsr)
             throw qsa::Error(qsa::kDeviceUnsupported, std::string(who) + ": a block needs its CSR (csr lane)");
-        if (op->dim() > 8192) throw qsa::Error(qsa::kInvalid, std::string(who) + ": block above 8192 states");
+        if (op->dim() > qsa::kDenseDeviceMaxDim)
+            throw qsa::Error(qsa::kInvalid, std::string(who) + ": block above " +
+                                                std::to_string(qsa::kDenseDeviceMaxDim) + " states");
+        if (op->dim() >= device_min_dim || op->dim() > 8192) {
+            dev_ops.push_back(op.get());
+            dev_idx.push_back(b);
+        } else {
+            host_idx.push_back(b);
+        }
     }
-    std::vector<std::vector<std::complex<double>>> dense(ops.size());
-    for (std::size_t b = 0; b < ops.size(); ++b) {
+    std::vector<std::vector<double>> ev(ops.size());
+    if (!dev_ops.empty()) {
+        auto d = qsa::device_spectra(dev_ops);
+        for (std::size_t q = 0; q < dev_idx.size(); ++q) ev[dev_idx[q]] = std::move(d[q]);
+    }
+    std::vector<std::vector<std::complex<double>>> dense(host_idx.size());
+    for (std::size_t q = 0; q < host_idx.size(); ++q) {
+        const auto& op = ops[host_idx[q]];
         std::vector<std::uint64_t> rp;
         std::vector<std::uint32_t> col;
         std::vector<double> val;
-        ops[b]->csr_download(rp, col, val);
-        const std::size_t n = static_cast<std::size_t>(ops[b]->dim());
-        dense[b].assign(n * n, std::complex<double>(0.0, 0.0));
+        op->csr_download(rp, col, val);
+        const std::size_t n = static_cast<std::size_t>(op->dim());
+        dense[q].assign(n * n, std::complex<double>(0.0, 0.0));
         for (std::size_t r = 0; r < n; ++r)
             for (std::uint64_t e = rp[r]; e < rp[r + 1]; ++e)
-                dense[b][col[e] * n + r] += std::complex<double>(val[2 * e], val[2 * e + 1]);
+                dense[q][col[e] * n + r] += std::complex<double>(val[2 * e], val[2 * e + 1]);
     }
-    std::vector<std::vector<double>> ev(ops.size());
-    qsa::parallel_blocks(ops.size(), [&](std::size_t b) {
-        const std::size_t n = static_cast<std::size_t>(ops[b]->dim());
-        ev[b] = qsa::hermitian_eigvals_householder(std::move(dense[b]), n);
+    qsa::parallel_blocks(host_idx.size(), [&](std::size_t q) {
+        const std::size_t n = static_cast<std::size_t>(ops[host_idx[q]]->dim());
+        ev[host_idx[q]] = qsa::hermitian_eigvals_householder(std::move(dense[q]), n);
     });
     return ev;
 }
 
 }  // namespace
 
-extern "C" int qedh_block_spectrum(const qedh_matvec* blocks, int n_blocks, double* values) {
+extern "C" int qedh_block_spectrum(const qedh_matvec* blocks, int n_blocks, int64_t device_min_dim, double* values) {
     return guarded_batch([&] {
         if (!values) throw qsa::Error(qsa::kInvalid, "qedh_block_spectrum: null argument");
-        const auto ev = dense_spectra(gather_blocks(blocks, n_blocks), "qedh_block_spectrum");
+        const auto ev = dense_spectra(gather_blocks(blocks, n_blocks), "qedh_block_spectrum",
+                                      device_min_dim < 0 ? qsa::kDeviceDenseMinDim : device_min_dim);
         std::size_t at = 0;
         for (const auto& e : ev) {
             std::copy(e.begin(), e.end(), values + at);
@@ -182,7 +202,7 @@ extern "C" int qedh_exact_thermal(const qedh_matvec* blocks, int n_blocks, int n
         if (!betas || !lnZ || !E || !V || n_beta < 1)
             throw qsa::Error(qsa::kInvalid, "qedh_exact_thermal: null argument or no temperature");
         const auto ops = gather_blocks(blocks, n_blocks);
-        const auto ev = dense_spectra(ops, "qedh_exact_thermal");
+        const auto ev = dense_spectra(ops, "qedh_exact_thermal", qsa::kDeviceDenseMinDim);
         const std::vector<double> bt(betas, betas + n_beta);
         std::vector<qsa::Curves> out(ops.size());
         qsa::parallel_blocks(ops.size(), [&](std::size_t b) { out[b] = qsa::exact_curves(ev[b], bt); });

@@ -6,6 +6,10 @@ most the dense floor, are solved densely on the host (the reference's HostDense 
 src/engine/block_solve.cpp:164-262); the others on the device, k = 1 by the lowest-level scan and
 k > 1 by Krylov-Schur (block_solve.cpp:263-655). The levels of every block are merged and the
 lowest k returned, ascending.
+
+The spectrum verb (python/qed/_verbs/spectrum.py; the DenseBatch walk of src/engine/walk.h) solves
+every block densely -- from 1024 states on the device (rocSOLVER zheevd, batched per dimension),
+below on the host pool -- and returns all eigenvalues, ascending.
 """
 
 from __future__ import annotations
@@ -24,8 +28,12 @@ from .sector import DEFAULT_CSR_BUDGET
 from .symmetry import Symmetry
 from .thermal import _handles, resolve_device, walk_blocks
 
-# the automatic dense floor's ceiling (src/engine/internal.h:815) and the dense lane's limit here
+# the automatic dense floor's ceiling (src/engine/internal.h:815)
 AUTO_DENSE_CEILING = 8192
+# dense blocks from this size are solved on the device (kDeviceDenseMinDim, include/ed/core/device.h)
+DEVICE_DENSE_MIN_DIM = 1024
+# the dense lanes' largest block
+DENSE_MAX_DIM = 32768
 
 
 def dense_floor(k: int, dense_max_dim: Optional[int]) -> int:
@@ -68,10 +76,10 @@ def eigs(H: Operator, k: int = 4, *, sym: Optional[Symmetry] = None, device: str
         mats = [blocks[i][2] for i in dense_idx]
         dims = [m.dim for m in mats]
         vals = np.zeros(int(sum(dims)))
-        _lib.call("qedh_block_spectrum", _handles(mats), len(mats), _lib.ptr(vals, C.c_double))
+        _lib.call("qedh_block_spectrum", _handles(mats), len(mats), C.c_int64(-1), _lib.ptr(vals, C.c_double))
         at = 0
         for i, d in zip(dense_idx, dims):
-            levels[i] = (vals[at: at + min(k, d)].copy(), True, "host-dense")
+            levels[i] = (vals[at: at + min(k, d)].copy(), True, "device-dense" if d >= DEVICE_DENSE_MIN_DIM else "host-dense")
             at += d
     t_dense = time.perf_counter() - t1
     t2 = time.perf_counter()
@@ -93,3 +101,41 @@ def eigs(H: Operator, k: int = 4, *, sym: Optional[Symmetry] = None, device: str
     return EigsResult(energies=energies, blocks=recs, converged=converged,
                       timings={"build_s": t_build, "dense_s": t_dense, "krylov_s": t_krylov,
                                "total_s": time.perf_counter() - t0})
+
+
+@dataclass
+class SpectrumResult:
+    eigenvalues: np.ndarray
+    blocks: List[dict] = field(default_factory=list)
+    timings: Dict[str, float] = field(default_factory=dict)
+
+
+def spectrum(H: Operator, *, sym: Optional[Symmetry] = None, device: str = "gpu",
+             device_min_dim: Optional[int] = None, csr_budget: int = DEFAULT_CSR_BUDGET) -> SpectrumResult:
+    """Every eigenvalue of H over the blocks of `sym`, ascending. device_min_dim: blocks from this
+    many states are solved on the device (default 1024; 0 sends every block there)."""
+    resolve_device(device, "spectrum")
+    sym = sym if sym is not None else Symmetry()
+    t0 = time.perf_counter()
+    blocks = walk_blocks(H, sym, csr_budget)
+    if not blocks:
+        raise InvalidRequest("spectrum: the symmetry selection holds no state")
+    big = [s.dim for _, s, _ in blocks if s.dim > DENSE_MAX_DIM]
+    if big:
+        raise InvalidRequest(f"spectrum: a block of {big[0]} states is above the dense lanes' {DENSE_MAX_DIM}")
+    t_build = time.perf_counter() - t0
+    mats = [m for _, _, m in blocks]
+    dims = [m.dim for m in mats]
+    vals = np.zeros(int(sum(dims)))
+    dmin = -1 if device_min_dim is None else int(device_min_dim)
+    t1 = time.perf_counter()
+    _lib.call("qedh_block_spectrum", _handles(mats), len(mats), C.c_int64(dmin), _lib.ptr(vals, C.c_double))
+    t_solve = time.perf_counter() - t1
+    recs, at = [], 0
+    lim = DEVICE_DENSE_MIN_DIM if dmin < 0 else dmin
+    for (blk, sec, _), d in zip(blocks, dims):
+        recs.append({"label": blk.label, "dim": d, "lane": "device-dense" if (d >= lim or d > 8192) else "host-dense",
+                     "eigenvalues": vals[at: at + d].copy()})
+        at += d
+    return SpectrumResult(eigenvalues=np.sort(vals), blocks=recs,
+                          timings={"build_s": t_build, "solve_s": t_solve, "total_s": time.perf_counter() - t0})

@@ -13,7 +13,8 @@ J1 = 1 on (i, i+1), J2 = 0.5 on (i, i+2). Cases (cases.py:138-206): thermal at
 T = linspace(0.25, 3, 8), method exact / ftlm (samples 8, krylov 40, seed 11), symmetry auto / none;
 dynamics of S^z(pi) = N^-1/2 sum_i e^{i pi i} S^z_i at omega = linspace(-1, 4, 41), eta = 0.15,
 T = 0 with krylov 80, T = 1 with krylov 40, samples 6, seed 11; eigs k = 4 (cases.py:95-110) over
-the whole space, every symmetry block, the Sz blocks without spatial symmetry, and Sz = 0 alone.
+the whole space, every symmetry block, the Sz blocks without spatial symmetry, and Sz = 0 alone;
+the whole spectrum over the same selections.
 """
 
 from __future__ import annotations
@@ -37,6 +38,11 @@ KEYS = [
     "j1j2_chain12/api/eigs/spatial=none/k=4",
     "j1j2_chain12/api/eigs/sz=6/k=4",
     "j1j2_chain12/api/eigs/auto/sz=6/k=4",
+    "j1j2_chain12/api/spectrum/none",
+    "j1j2_chain12/api/spectrum/auto",
+    "j1j2_chain12/api/spectrum/spatial=none",
+    "j1j2_chain12/api/spectrum/sz=6",
+    "j1j2_chain12/api/spectrum/auto/sz=6",
 ]
 
 

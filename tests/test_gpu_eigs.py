@@ -57,3 +57,33 @@ def test_eigs_reproduces_reference_records(key, dense_max_dim):
     lanes = {b["lane"] for b in r.blocks}
     if dense_max_dim == 0:
         assert "device-krylov" in lanes
+
+
+@pytest.mark.parametrize("device_min_dim", [None, 0])
+@pytest.mark.parametrize("key", list(CASES))
+def test_spectrum_reproduces_reference_records(key, device_min_dim):
+    """The whole spectrum over the walk (dense blocks from 1024 states on the device by default,
+    every block on the device with device_min_dim = 0) against the reference's recorded spectrum."""
+    say("spectrum", key, "device_min_dim", device_min_dim)
+    want = np.sort(np.array(_records()[f"j1j2_chain12/api/spectrum/{key}"]["values"]["eigenvalues"]))
+    r = ev.spectrum(_h12(), sym=CASES[key], device_min_dim=device_min_dim)
+    assert r.eigenvalues.size == want.size
+    assert np.allclose(r.eigenvalues, want, rtol=1e-10, atol=1e-10), np.max(np.abs(r.eigenvalues - want))
+    if device_min_dim == 0 or key == "none":
+        assert any(b["lane"] == "device-dense" for b in r.blocks)
+
+
+def test_exact_thermal_device_lane_matches_host_lane():
+    """Exact thermodynamics of the 4096-state block (device lane: 4096 >= 1024) against the reference's
+    recorded exact curves of the whole space."""
+    say("exact thermal on the device lane")
+    from qed_spin_amd import thermal as th
+
+    with open(os.path.join(FIXTURES, "reference_records.json")) as f:
+        rec = json.load(f)
+    r = rec["records"]["j1j2_chain12/api/thermal/exact/none"]["values"]
+    T = np.array(r["T"])
+    res = th.thermal(_h12(), T, sym=qs.Symmetry.none(), dense_max_dim=8192, seed=1)
+    assert res.blocks[0]["exact"] and res.blocks[0]["dim"] == 4096
+    for key in ("E", "C", "S"):
+        assert np.allclose(getattr(res, key), r[key], rtol=1e-10, atol=1e-10), key

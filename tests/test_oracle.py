@@ -312,3 +312,20 @@ def test_reference_eigs_records_are_the_dense_spectrum():
     for key in ("sz=6", "auto/sz=6"):
         got = rec[f"j1j2_chain12/api/eigs/{key}/k=4"]["values"]["energies"]
         assert np.allclose(got, ev0[:4], rtol=1e-10, atol=1e-10), key
+
+
+def test_reference_spectrum_records_are_the_dense_spectrum():
+    """The reference's recorded whole spectra of j1j2_chain12 are the oracle's dense spectrum: the
+    whole space under any symmetry selection, and the Sz = 0 block."""
+    rec = _reference_records()["records"]
+    s, row_ptr, col, val = _full_space_j1j2_chain12()
+    Hd = dense_csr(row_ptr, col, val, s.dim)
+    ev = np.linalg.eigvalsh(Hd)
+    for key in ("none", "auto", "spatial=none"):
+        got = np.array(rec[f"j1j2_chain12/api/spectrum/{key}"]["values"]["eigenvalues"])
+        assert got.size == 4096 and np.allclose(np.sort(got), ev, rtol=1e-10, atol=1e-10), key
+    sz0 = np.array([i for i in range(1 << 12) if bin(i).count("1") == 6])
+    ev0 = np.linalg.eigvalsh(Hd[np.ix_(sz0, sz0)])
+    for key in ("sz=6", "auto/sz=6"):
+        got = np.array(rec[f"j1j2_chain12/api/spectrum/{key}"]["values"]["eigenvalues"])
+        assert np.allclose(np.sort(got), ev0, rtol=1e-10, atol=1e-10), key
