@@ -160,6 +160,37 @@ def run_dynamics(args):
     return out
 
 
+def run_spectrum(args):
+    """The whole spectrum of the Heisenberg ring over every (Sz, k) block: the dense lanes (blocks
+    from 1024 states on the device, rocSOLVER zheevd batched per dimension; smaller ones on the host
+    pool). Reported: wall time, the blocks on each lane and the ground-state energy."""
+    import numpy as np
+
+    import qed_spin_amd as qs
+    from qed_spin_amd import eigs as ev
+
+    n = args.spectrum_sites
+    H = qs.HamiltonianBuilder(n).heisenberg([(i, (i + 1) % n) for i in range(n)], 1.0).to_operator()
+    t0 = time.perf_counter()
+    r = ev.spectrum(H, sym=qs.Symmetry())
+    wall = time.perf_counter() - t0
+    dev = [b["dim"] for b in r.blocks if b["lane"] == "device-dense"]
+    host = [b["dim"] for b in r.blocks if b["lane"] == "host-dense"]
+    out = {
+        "workload": f"whole spectrum of the Heisenberg ring N={n} over every (Sz, k) block",
+        "eigenvalues": int(r.eigenvalues.size),
+        "blocks_device": len(dev),
+        "blocks_host": len(host),
+        "largest_block": int(max(dev + host)),
+        "wall_s": round(wall, 3),
+        "timings": {k: round(v, 3) for k, v in r.timings.items()},
+        "E0": float(r.eigenvalues[0]),
+        "trace": float(np.sum(r.eigenvalues)),
+    }
+    log(f"spectrum: {out}")
+    return out
+
+
 def run_serve(args):
     """The mixed request stream: served at saturation (every request there at 0: solves/s), served
     with Poisson arrivals (latency p50/p99), and the plain path (one request after another, no
@@ -169,7 +200,8 @@ def run_serve(args):
 
     def brief(st):
         keys = ("wall_s", "solves_per_s", "p50_s", "p99_s", "mean_s", "completed", "failed", "peak_in_flight",
-                "peak_reserved", "sector_builds", "sector_hits")
+                "peak_reserved", "sector_builds", "sector_hits", "build_s", "service_eigs_s", "service_thermal_s",
+                "service_dynamics_s")
         return {k: (round(st[k], 4) if isinstance(st[k], float) else st[k]) for k in keys}
 
     reqs = sv.mixed_stream(n=args.serve_requests, rate=args.serve_rate, seed=3, n_max=args.serve_nmax)
@@ -214,9 +246,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--workload", default="matvec,serve",
-                    help="matvec (the headline) and a comma list of ftlm, dynamics, serve")
+                    help="matvec (the headline) and a comma list of ftlm, dynamics, serve, spectrum")
     ap.add_argument("--ftlm-sites", type=int, default=24)
     ap.add_argument("--dyn-sites", type=int, default=28)
+    ap.add_argument("--spectrum-sites", type=int, default=18)
     ap.add_argument("--serve-requests", type=int, default=200)
     ap.add_argument("--serve-rate", type=float, default=5.0)
     ap.add_argument("--serve-slots", type=int, default=8)
@@ -368,6 +401,8 @@ def main():
             out["dynamics"] = run_dynamics(args)
         elif w == "serve":
             out["serve"] = run_serve(args)
+        elif w == "spectrum":
+            out["spectrum"] = run_spectrum(args)
         else:
             raise SystemExit(f"unknown workload {w!r}")
     print(json.dumps(out))
