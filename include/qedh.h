@@ -47,6 +47,9 @@ int qedh_free(void* ptr);
 int qedh_memcpy_htod(void* dst, const void* src, size_t bytes);
 int qedh_memcpy_dtoh(void* dst, const void* src, size_t bytes);
 int qedh_memcpy_dtod(void* dst, const void* src, size_t bytes);
+/* out[0] + i out[1] = x^H y of two device vectors of n entries (complex or real), reduced on the
+ * device in a fixed order (BasicBackend::dot, include/ed/matvec/backend.h:56-179). */
+int qedh_vdot(const void* x, const void* y, int64_t n, int complex_vectors, double* out);
 int qedh_memset(void* dst, int value, size_t bytes);
 int qedh_synchronize(void);
 /* The calling thread's stream for every engine launch, copy and allocation: own != 0 a non-blocking

@@ -55,6 +55,7 @@ _SIGNATURES = {
     "qedh_memset": [C.c_void_p, C.c_int, C.c_size_t],
     "qedh_synchronize": [],
     "qedh_thread_stream": [C.c_int],
+    "qedh_vdot": [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, c_dbl_p],
     "qedh_operator_create": [C.c_int, C.POINTER(C.c_void_p)],
     "qedh_operator_add_product": [C.c_void_p, C.c_char_p, c_int_p, C.c_double, C.c_double],
     "qedh_operator_info": [C.c_void_p, c_i64_p, c_dbl_p, c_int_p],

@@ -447,3 +447,24 @@ extern "C" int qedh_gaussian_vectors(uint64_t base, uint64_t first_sample, int c
         QSA_HIP(hipStreamSynchronize(st));
     });
 }
+
+extern "C" int qedh_vdot(const void* x, const void* y, int64_t n, int complex_vectors, double* out) {
+    return guarded_batch([&] {
+        if (!x || !y || !out || n < 0) throw qsa::Error(qsa::kInvalid, "qedh_vdot: bad arguments");
+        if (n == 0) {
+            out[0] = out[1] = 0.0;
+            return;
+        }
+        if (complex_vectors) {
+            qsa::HipBackend<qsa::cplx> be;
+            const qsa::cplx d = be.dot(static_cast<const qsa::cplx*>(x), static_cast<const qsa::cplx*>(y),
+                                       static_cast<std::size_t>(n));
+            out[0] = d.real();
+            out[1] = d.imag();
+        } else {
+            qsa::HipBackend<double> be;
+            out[0] = be.dot(static_cast<const double*>(x), static_cast<const double*>(y), static_cast<std::size_t>(n));
+            out[1] = 0.0;
+        }
+    });
+}

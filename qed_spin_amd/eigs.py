@@ -7,6 +7,12 @@ src/engine/block_solve.cpp:164-262); the others on the device, k = 1 by the lowe
 k > 1 by Krylov-Schur (block_solve.cpp:263-655). The levels of every block are merged and the
 lowest k returned, ascending.
 
+The expect verb (python/qed/_verbs/expect.py) returns, for each of the k lowest levels, the
+expectation values <v|O|v> of observables in the level's eigenvector v (the reference sweeps the
+observables' programs over the level vectors, rep_matrix_elements, include/ed/ops/program.h:135-160):
+here each observable's lambda-projected program between two handles of the level's sector is
+applied to v on the device (gather lane) and <v|O v> / <v|v> reduced on the device.
+
 The spectrum verb (python/qed/_verbs/spectrum.py; the DenseBatch walk of src/engine/walk.h) solves
 every block densely -- from 1024 states on the device (rocSOLVER zheevd, batched per dimension),
 below on the host pool -- and returns all eigenvalues, ascending.
@@ -24,7 +30,7 @@ import numpy as np
 from . import _lib
 from .errors import InvalidRequest
 from .operator import Operator
-from .sector import DEFAULT_CSR_BUDGET
+from .sector import DEFAULT_CSR_BUDGET, LANE_GATHER, Sector, SectorMatvec
 from .symmetry import Symmetry
 from .thermal import _handles, resolve_device, walk_blocks
 
@@ -139,3 +145,67 @@ def spectrum(H: Operator, *, sym: Optional[Symmetry] = None, device: str = "gpu"
         at += d
     return SpectrumResult(eigenvalues=np.sort(vals), blocks=recs,
                           timings={"build_s": t_build, "solve_s": t_solve, "total_s": time.perf_counter() - t0})
+
+
+@dataclass
+class ExpectResult:
+    energies: np.ndarray          # the k lowest levels, ascending
+    values: np.ndarray            # [level, observable]: <v|O|v> / <v|v> (real part)
+    labels: List[str] = field(default_factory=list)   # the block of each level
+    timings: Dict[str, float] = field(default_factory=dict)
+
+
+def _vdot(x_ptr, y_ptr, n: int) -> complex:
+    out = np.zeros(2)
+    _lib.call("qedh_vdot", x_ptr, y_ptr, C.c_int64(int(n)), 1, _lib.ptr(out, C.c_double))
+    return complex(out[0], out[1])
+
+
+def expect(H: Operator, observables, k: int = 4, *, sym: Optional[Symmetry] = None, device: str = "gpu",
+           csr_budget: int = DEFAULT_CSR_BUDGET) -> ExpectResult:
+    """For each of the k lowest levels of H over the blocks of `sym`: its energy and <v|O|v> of every
+    observable O (an Operator or a list) in its eigenvector v (Krylov-Schur eigenpairs per block on
+    the device). A level degenerate within its block has no unique vector; the values are then those
+    of the vector the solver returns."""
+    from .dynamics import _DevView, _lib_alloc
+
+    resolve_device(device, "expect")
+    obs = [observables] if isinstance(observables, Operator) else list(observables)
+    if not obs or int(k) < 1:
+        raise InvalidRequest("expect: at least one observable and k >= 1")
+    k = int(k)
+    sym = sym if sym is not None else Symmetry()
+    t0 = time.perf_counter()
+    blocks = walk_blocks(H, sym, csr_budget)
+    if not blocks:
+        raise InvalidRequest("expect: the symmetry selection holds no state")
+    rows, bufs = [], {}
+    for i, (blk, sec, m) in enumerate(blocks):
+        want = min(k, sec.dim)
+        buf = _lib_alloc(want * sec.dim * 16)
+        vals = np.zeros(want)
+        n, conv = C.c_int(), C.c_int()
+        _lib.call("qedh_block_eigenpairs", m.h, int(want), C.c_int64(0), _lib.ptr(vals, C.c_double), buf.ptr,
+                  C.byref(n), C.byref(conv))
+        bufs[i] = buf
+        rows.extend((float(vals[j]), i, j) for j in range(n.value))
+    rows.sort(key=lambda r: r[0])
+    rows = rows[:k]
+    t_solve = time.perf_counter() - t0
+    ops, twins = {}, {}
+    values = np.zeros((len(rows), len(obs)))
+    for li, (e, i, j) in enumerate(rows):
+        blk, sec, _ = blocks[i]
+        if i not in twins:   # the observables are not group invariant: their lambda-projected programs
+            twins[i] = Sector(blk.group, blk.n_up)   # between two handles of the sector
+        v = _DevView(bufs[i].ptr.value, j * sec.dim * 16)
+        y = _lib_alloc(sec.dim * 16)
+        norm2 = _vdot(v.ptr, v.ptr, sec.dim).real
+        for o, O in enumerate(obs):
+            if (i, o) not in ops:
+                ops[(i, o)] = SectorMatvec(O, twins[i], sec, lane=LANE_GATHER)
+            ops[(i, o)].apply(v, y, complex_vectors=True)
+            values[li, o] = (_vdot(v.ptr, y.ptr, sec.dim) / norm2).real
+    return ExpectResult(energies=np.array([r[0] for r in rows]), values=values,
+                        labels=[blocks[i][0].label for _, i, _ in rows],
+                        timings={"solve_s": t_solve, "total_s": time.perf_counter() - t0})

@@ -14,7 +14,7 @@ T = linspace(0.25, 3, 8), method exact / ftlm (samples 8, krylov 40, seed 11), s
 dynamics of S^z(pi) = N^-1/2 sum_i e^{i pi i} S^z_i at omega = linspace(-1, 4, 41), eta = 0.15,
 T = 0 with krylov 80, T = 1 with krylov 40, samples 6, seed 11; eigs k = 4 (cases.py:95-110) over
 the whole space, every symmetry block, the Sz blocks without spatial symmetry, and Sz = 0 alone;
-the whole spectrum over the same selections.
+the whole spectrum over the same selections; expect of S_0 . S_1 in the 4 lowest levels.
 """
 
 from __future__ import annotations
@@ -43,6 +43,7 @@ KEYS = [
     "j1j2_chain12/api/spectrum/spatial=none",
     "j1j2_chain12/api/spectrum/sz=6",
     "j1j2_chain12/api/spectrum/auto/sz=6",
+    "j1j2_chain12/api/expect/auto/S0.S1/k=4",
 ]
 
 

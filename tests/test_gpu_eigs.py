@@ -87,3 +87,20 @@ def test_exact_thermal_device_lane_matches_host_lane():
     assert res.blocks[0]["exact"] and res.blocks[0]["dim"] == 4096
     for key in ("E", "C", "S"):
         assert np.allclose(getattr(res, key), r[key], rtol=1e-10, atol=1e-10), key
+
+
+def test_expect_reproduces_reference_record():
+    """<S_0 . S_1> in the eigenvectors of the 4 lowest levels over every (Sz, k) block against the
+    reference's recorded expect of j1j2_chain12: each recorded (energy, bond) pair matched by one
+    level (energies within 1e-10; bonds within 1e-8, the accuracy of eigenvectors locked at a 1e-10
+    residual), the two ground states (momentum 0 and pi) told apart by their bonds."""
+    say("expect S0.S1")
+    rec = _records()["j1j2_chain12/api/expect/auto/S0.S1/k=4"]["values"]
+    bond = qs.HamiltonianBuilder(12).heisenberg([(0, 1)], 1.0).to_operator()
+    r = ev.expect(_h12(), [bond], 4, sym=qs.Symmetry())
+    assert np.allclose(r.energies, rec["energy"], rtol=1e-10, atol=1e-10)
+    used = set()
+    for e, b in zip(rec["energy"], rec["bond"]):
+        hit = [i for i in range(4) if i not in used and abs(r.energies[i] - e) < 1e-9 and abs(r.values[i, 0] - b) < 1e-8]
+        assert hit, (e, b, r.energies, r.values[:, 0])
+        used.add(hit[0])

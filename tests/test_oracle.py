@@ -329,3 +329,47 @@ def test_reference_spectrum_records_are_the_dense_spectrum():
     for key in ("sz=6", "auto/sz=6"):
         got = np.array(rec[f"j1j2_chain12/api/spectrum/{key}"]["values"]["eigenvalues"])
         assert np.allclose(np.sort(got), ev0, rtol=1e-10, atol=1e-10), key
+
+
+def _sz0_momentum_levels(Hd, n=12, n_up=6, per_k=2):
+    """(energy, <S_0 . S_1>) of the lowest levels of every momentum block of the Sz = 0 states of a
+    dense ring Hamiltonian, by projecting onto translation eigenspaces (a check independent of the
+    sector code)."""
+    st = np.array([i for i in range(1 << n) if bin(i).count("1") == n_up])
+    pos = {int(s): i for i, s in enumerate(st)}
+    H0 = Hd[np.ix_(st, st)]
+    d = st.size
+    T = np.zeros((d, d))
+    for i, s in enumerate(st):
+        t = ((int(s) << 1) | (int(s) >> (n - 1))) & ((1 << n) - 1)   # site j -> j + 1
+        T[pos[t], i] = 1.0
+    B = np.zeros((d, d))
+    for i, s in enumerate(st):
+        b0, b1 = int(s) & 1, (int(s) >> 1) & 1
+        B[i, i] = 0.25 if b0 == b1 else -0.25
+        if b0 != b1:
+            B[pos[int(s) ^ 3], i] = 0.5
+    out = []
+    Tp = [np.linalg.matrix_power(T, a) for a in range(n)]
+    for k in range(n):
+        P = sum(np.exp(-2j * np.pi * k * a / n) * Tp[a] for a in range(n)) / n
+        w, U = np.linalg.eigh((P + P.conj().T) / 2)
+        Q = U[:, w > 0.5]
+        e, V = np.linalg.eigh(Q.conj().T @ H0 @ Q)
+        for j in range(min(per_k, e.size)):
+            v = Q @ V[:, j]
+            out.append((e[j], float(np.real(np.vdot(v, B @ v)))))
+    return out
+
+
+def test_reference_expect_record_matches_momentum_resolved_dense_states():
+    """The reference's recorded <S_0 . S_1> in the 4 lowest levels of j1j2_chain12: every recorded
+    (energy, bond) pair is a momentum eigenstate's of the oracle's dense Sz = 0 block (translation
+    invariant states carry one value of the bond; the triplet's members share theirs)."""
+    r = _reference_records()["records"]["j1j2_chain12/api/expect/auto/S0.S1/k=4"]["values"]
+    s, row_ptr, col, val = _full_space_j1j2_chain12()
+    lv = _sz0_momentum_levels(dense_csr(row_ptr, col, val, s.dim))
+    for e, b in zip(r["energy"], r["bond"]):
+        assert any(abs(e - e2) < 1e-9 and abs(b - b2) < 1e-9 for e2, b2 in lv), (e, b)
+    # the two ground states (momentum 0 and pi) carry different bonds, so the matching tells them apart
+    assert abs(r["bond"][0] - r["bond"][1]) > 1e-3
