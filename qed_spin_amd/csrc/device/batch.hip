@@ -100,15 +100,17 @@ __global__ __launch_bounds__(kThreads) void k_batch_spmv_narrow(BatchView B, con
     }
 }
 
-// The slots' sums of each vector in slot order into partial[tile][s] (re, im). Every thread of the
-// workgroup calls it.
-__device__ __forceinline__ void tile_partial(const BatchView& B, double pr, double pi, double* partial) {
+// The slots' sums of each vector in slot order into partial[tile][s] (re, im); with only, just the
+// vectors whose flag is set. Every thread of the workgroup calls it.
+__device__ __forceinline__ void tile_partial(const BatchView& B, double pr, double pi, double* partial,
+                                             const int* only = nullptr) {
     __shared__ double sr[kThreads], si[kThreads];
     const int R = B.R, slots = kThreads / R;
     sr[threadIdx.x] = pr;
     si[threadIdx.x] = pi;
     __syncthreads();
-    if (static_cast<int>(threadIdx.x) < R) {
+    if (static_cast<int>(threadIdx.x) < R &&
+        (!only || only[static_cast<u64>(B.tiles[blockIdx.x].block) * R + threadIdx.x])) {
         double a = 0.0, b = 0.0;
         for (int q = 0; q < slots; ++q) {
             a += sr[q * R + threadIdx.x];
@@ -159,19 +161,28 @@ __global__ __launch_bounds__(kThreads) void k_batch_spmv_wide_fused(BatchView B,
 
 // mode 0: W -= beta V_prev (skipped at step 0), then partial of <V_cur, W> (alpha);
 // mode 1: W -= alpha V_cur, then partial of <V_cur, W> (the local overlap);
+// mode 5: mode 1, and the partial of ||W||^2 into partial2 as well (what mode 2 sums when it does
+//         not correct W);
 // mode 4: W -= alpha V_cur, then partial of ||W||^2 (no reorthogonalisation);
-// mode 2: W -= ov V_cur when |ov| > threshold (from step 1 on), then partial of ||W||^2;
+// mode 2: W -= ov V_cur when |ov| > threshold (from step 1 on), then partial of ||W||^2; with
+//         B.fix (after mode 5) only the pairs flagged there, their norms into partial2;
 // mode 3: W *= 1 / beta_next (the next basis vector).
 // Pairs no longer running are left alone.
 __global__ __launch_bounds__(kThreads) void k_batch_vec(BatchView B, int mode, const Zd* Vprev, const Zd* Vcur, Zd* W,
                                                         const double* coef, const Zd* zcoef, double threshold,
-                                                        double* partial) {
+                                                        double* partial, double* partial2) {
     const Tile t = B.tiles[blockIdx.x];
     const int R = B.R, slots = kThreads / R;
     const int slot = threadIdx.x / R, s = threadIdx.x % R;
-    const bool live = slot < slots && pair_live(B, t.block, s);
     const u64 bs = static_cast<u64>(t.block) * R + s;
-    double pr = 0.0, pi = 0.0;
+    if (mode == 2 && B.fix) {
+        // nothing to correct in this tile's block: the whole workgroup leaves (the usual case)
+        bool any = false;
+        for (int q = 0; q < R; ++q) any = any || B.fix[static_cast<u64>(t.block) * R + q] != 0;
+        if (!any) return;
+    }
+    const bool live = slot < slots && pair_live(B, t.block, s) && !(mode == 2 && B.fix && !B.fix[bs]);
+    double pr = 0.0, pi = 0.0, pn = 0.0;
     if (live) {
         const int step = *B.step;
         for (u64 r = t.row_begin + slot; r < t.row_end; r += slots) {
@@ -187,15 +198,16 @@ __global__ __launch_bounds__(kThreads) void k_batch_vec(BatchView B, int mode, c
                 const Zd d = cdot(Vcur[i], w);
                 pr += d.re;
                 pi += d.im;
-            } else if (mode == 1 || mode == 4) {
+            } else if (mode == 1 || mode == 4 || mode == 5) {
                 const double a = coef[bs];
                 const Zd v = Vcur[i];
                 w = Zd{w.re - a * v.re, w.im - a * v.im};
                 W[i] = w;
-                if (mode == 1) {
+                if (mode != 4) {
                     const Zd d = cdot(v, w);
                     pr += d.re;
                     pi += d.im;
+                    if (mode == 5) pn += w.re * w.re + w.im * w.im;
                 } else {
                     pr += w.re * w.re + w.im * w.im;
                 }
@@ -213,7 +225,15 @@ __global__ __launch_bounds__(kThreads) void k_batch_vec(BatchView B, int mode, c
             }
         }
     }
-    if (mode != 3) tile_partial(B, pr, pi, partial);
+    if (mode == 5) {
+        tile_partial(B, pr, pi, partial);
+        __syncthreads();   // the LDS of the first sums is read before the second overwrites it
+        tile_partial(B, pn, 0.0, partial2);
+    } else if (mode == 2 && B.fix) {
+        tile_partial(B, pr, pi, partial2, B.fix);
+    } else if (mode != 3) {
+        tile_partial(B, pr, pi, partial);
+    }
 }
 
 // The sum over block b's tiles of partial[tile][s] (re, im), one workgroup of 256 threads per
@@ -247,7 +267,7 @@ __device__ __forceinline__ void block_sum(const BatchView& B, int b, int s, cons
 __global__ __launch_bounds__(kThreads) void k_batch_finish(BatchView B, int kind, const double* partial,
                                                            double* alpha_hist, double* beta_hist, Zd* ov,
                                                            double* cur_alpha, double* cur_beta, double* inv_beta,
-                                                           const double* breakdown_tol) {
+                                                           const double* breakdown_tol, double threshold) {
     const int R = B.R;
     const u64 bs = blockIdx.x;
     const int b = static_cast<int>(bs / R), s = static_cast<int>(bs % R);
@@ -262,6 +282,7 @@ __global__ __launch_bounds__(kThreads) void k_batch_finish(BatchView B, int kind
         cur_alpha[bs] = a;
     } else if (kind == 1) {
         ov[bs] = Zd{a, c};
+        if (B.fix) B.fix[bs] = (step > 0 && sqrt(a * a + c * c) > threshold) ? 1 : 0;
     } else {
         const double beta = sqrt(a);
         beta_hist[hist] = beta;
@@ -457,16 +478,17 @@ bool batch_spmv_fused(const BatchView& B, const Zd* Vprev, const Zd* Vcur, Zd* W
 }
 
 void batch_vec(const BatchView& B, int mode, const Zd* Vprev, const Zd* Vcur, Zd* W, const double* coef,
-               const Zd* zcoef, double threshold, double* partial, hipStream_t st) {
+               const Zd* zcoef, double threshold, double* partial, hipStream_t st, double* partial2) {
     if (!B.n_tiles) return;
     hipLaunchKernelGGL(k_batch_vec, dim3(B.n_tiles), dim3(kThreads), 0, st, B, mode, Vprev, Vcur, W, coef, zcoef,
-                       threshold, partial);
+                       threshold, partial, partial2);
 }
 
 void batch_finish(const BatchView& B, int kind, const double* partial, double* alpha_hist, double* beta_hist, Zd* ov,
-                  double* cur_alpha, double* cur_beta, double* inv_beta, const double* breakdown_tol, hipStream_t st) {
+                  double* cur_alpha, double* cur_beta, double* inv_beta, const double* breakdown_tol, hipStream_t st,
+                  double threshold) {
     hipLaunchKernelGGL(k_batch_finish, dim3(B.n_blocks * static_cast<u32>(B.R)), dim3(kThreads), 0, st, B, kind, partial,
-                       alpha_hist, beta_hist, ov, cur_alpha, cur_beta, inv_beta, breakdown_tol);
+                       alpha_hist, beta_hist, ov, cur_alpha, cur_beta, inv_beta, breakdown_tol, threshold);
 }
 
 void batch_normalize(const BatchView& B, Zd* V, double* partial, double* inv, double* norm2, hipStream_t st) {

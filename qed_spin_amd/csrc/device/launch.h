@@ -144,6 +144,7 @@ struct BatchView {
     int* active = nullptr;              // n_blocks x R: the pair is still running
     int* steps = nullptr;               // n_blocks x R: steps recorded
     const int* cap = nullptr;           // n_blocks: Krylov steps of the block (min(M, dim))
+    int* fix = nullptr;                 // n_blocks x R: the local overlap asks for a correction (or null)
 };
 constexpr int kBatchTileRows = 256;
 constexpr int kBatchMaxR = 64;
@@ -152,14 +153,17 @@ void batch_spmv(const BatchView& B, const Zd* X, Zd* Y, hipStream_t st);
 /// kernel, the same arithmetic); false (nothing launched) when R < 8.
 bool batch_spmv_fused(const BatchView& B, const Zd* Vprev, const Zd* Vcur, Zd* W, const double* beta, double* partial,
                       hipStream_t st);
-/// mode 0: W -= beta V_prev, <V_cur, W>; 1: W -= alpha V_cur, <V_cur, W>; 4: W -= alpha V_cur,
-/// ||W||^2; 2: local reorthogonalisation, ||W||^2; 3: W *= 1 / beta. Partials per tile and vector.
+/// mode 0: W -= beta V_prev, <V_cur, W>; 1: W -= alpha V_cur, <V_cur, W>; 5: mode 1 and ||W||^2
+/// into partial2; 4: W -= alpha V_cur, ||W||^2; 2: local reorthogonalisation, ||W||^2 (with B.fix:
+/// only the flagged pairs, into partial2); 3: W *= 1 / beta. Partials per tile and vector.
 void batch_vec(const BatchView& B, int mode, const Zd* Vprev, const Zd* Vcur, Zd* W, const double* coef,
-               const Zd* zcoef, double threshold, double* partial, hipStream_t st);
-/// kind 0: alpha; 1: the overlap; 2: beta (ends breaking-down or capped pairs). The step is read
-/// from B.step on the device.
+               const Zd* zcoef, double threshold, double* partial, hipStream_t st, double* partial2 = nullptr);
+/// kind 0: alpha; 1: the overlap (and, with B.fix, the flag of the pairs whose |overlap| exceeds
+/// threshold from step 1 on); 2: beta (ends breaking-down or capped pairs). The step is read from
+/// B.step on the device.
 void batch_finish(const BatchView& B, int kind, const double* partial, double* alpha_hist, double* beta_hist, Zd* ov,
-                  double* cur_alpha, double* cur_beta, double* inv_beta, const double* breakdown_tol, hipStream_t st);
+                  double* cur_alpha, double* cur_beta, double* inv_beta, const double* breakdown_tol, hipStream_t st,
+                  double threshold = 0.0);
 /// V scaled to unit norm per (block, vector); inv = 1 / norm, norm2 (may be null) = norm^2.
 void batch_normalize(const BatchView& B, Zd* V, double* partial, double* inv, double* norm2, hipStream_t st);
 /// V[(row0 + i) R + s] = src[i] for i < n.

@@ -148,6 +148,9 @@ BatchedLanczos::BatchedLanczos(std::vector<std::shared_ptr<const SectorOperator>
     V1_.reset(vec_bytes);
     W_.reset(vec_bytes);
     partial_.reset(tiles_.size() * 64 * 2 * sizeof(double));
+    partial2_.reset(tiles_.size() * 64 * 2 * sizeof(double));
+    fix_.reset(ops_.size() * static_cast<std::size_t>(R_) * sizeof(int));
+    put(fix_, std::vector<int>(ops_.size() * static_cast<std::size_t>(R_), 0), st);
     const std::size_t pairs = ops_.size() * static_cast<std::size_t>(R_);
     ov_.reset(pairs * sizeof(dev::Zd));
     cur_a_.reset(pairs * 8);
@@ -184,6 +187,7 @@ dev::BatchView BatchedLanczos::view() const {
     B.active = active_d_.get<int>();
     B.steps = steps_d_.get<int>();
     B.cap = cap_d_.get<int>();
+    B.fix = fix_.get<int>();
     return B;
 }
 
@@ -281,6 +285,7 @@ void BatchedLanczos::run(int M, const std::vector<double>& breakdown_tol, bool l
     const dev::BatchView B = view();
     dev::Zd* V[3] = {V1_.get<dev::Zd>(), V0_.get<dev::Zd>(), W_.get<dev::Zd>()};
     double* partial = partial_.get<double>();
+    double* partial2 = partial2_.get<double>();
     double* ah = alpha_h_.get<double>();
     double* bh = beta_h_.get<double>();
     dev::Zd* ov = ov_.get<dev::Zd>();
@@ -299,13 +304,16 @@ void BatchedLanczos::run(int M, const std::vector<double>& breakdown_tol, bool l
         }
         dev::batch_finish(B, 0, partial, ah, bh, ov, ca, cb, ib, tol, st);
         if (local) {
-            dev::batch_vec(B, 1, nullptr, c, w, ca, nullptr, 0.0, partial, st);
-            dev::batch_finish(B, 1, partial, ah, bh, ov, ca, cb, ib, tol, st);
-            dev::batch_vec(B, 2, nullptr, c, w, nullptr, ov, thr, partial, st);
+            // W -= alpha V with the local overlap and ||W||^2 in one pass; the correction pass then
+            // touches only the pairs whose overlap asks for one (their norms are summed again)
+            dev::batch_vec(B, 5, nullptr, c, w, ca, nullptr, 0.0, partial, st, partial2);
+            dev::batch_finish(B, 1, partial, ah, bh, ov, ca, cb, ib, tol, st, thr);
+            dev::batch_vec(B, 2, nullptr, c, w, nullptr, ov, thr, partial, st, partial2);
+            dev::batch_finish(B, 2, partial2, ah, bh, ov, ca, cb, ib, tol, st);
         } else {
             dev::batch_vec(B, 4, nullptr, c, w, ca, nullptr, 0.0, partial, st);
+            dev::batch_finish(B, 2, partial, ah, bh, ov, ca, cb, ib, tol, st);
         }
-        dev::batch_finish(B, 2, partial, ah, bh, ov, ca, cb, ib, tol, st);
         dev::batch_vec(B, 3, nullptr, nullptr, w, ib, nullptr, 0.0, partial, st);
         dev::batch_advance(B, st);
     };
