@@ -124,8 +124,16 @@ __device__ __forceinline__ void tile_partial(const BatchView& B, double pr, doub
 // The SpMV of R >= 8 with the first recurrence pass in its epilogue: W = H V_cur - beta V_prev
 // (beta of this step; skipped at step 0) and the partial of <V_cur, W>, the same arithmetic as
 // batch_spmv followed by batch_vec mode 0, without writing and re-reading W in between.
+//
+// With sprev / scur the stored vectors are the unnormalised ones and V = U f with the pair's factor
+// f (1 / beta of the step that made it): every read multiplies by f, which rounds exactly as the
+// stored product of a separate scaling pass would, so the results are the same bit for bit and the
+// pass is saved.
+__device__ __forceinline__ Zd scaled(Zd v, double f) { return Zd{v.re * f, v.im * f}; }
+
 __global__ __launch_bounds__(kThreads) void k_batch_spmv_wide_fused(BatchView B, const Zd* Vprev, const Zd* Vcur,
-                                                                    Zd* W, const double* beta, double* partial) {
+                                                                    Zd* W, const double* beta, double* partial,
+                                                                    const double* sprev, const double* scur) {
     const Tile t = B.tiles[blockIdx.x];
     const int R = B.R, slots = kThreads / R;
     const int slot = threadIdx.x / R, s = threadIdx.x % R;
@@ -133,25 +141,27 @@ __global__ __launch_bounds__(kThreads) void k_batch_spmv_wide_fused(BatchView B,
     double pr = 0.0, pi = 0.0;
     if (live) {
         const int step = *B.step;
-        const double b = beta[static_cast<u64>(t.block) * R + s];
+        const u64 bs = static_cast<u64>(t.block) * R + s;
+        const double b = beta[bs];
+        const double fp = sprev ? sprev[bs] : 1.0, fc = scur ? scur[bs] : 1.0;
         for (u64 r = t.row_begin + slot; r < t.row_end; r += slots) {
             const u64 e0 = B.row_ptr[r], e1 = B.row_ptr[r + 1];
             double ar = 0.0, ai = 0.0;
             for (u64 e = e0; e < e1; ++e) {
                 const u64 c = B.col[e];
                 const Zd v = entry_value(B, e);
-                const Zd x = Vcur[c * R + s];
+                const Zd x = scaled(Vcur[c * R + s], fc);
                 ar += v.re * x.re - v.im * x.im;
                 ai += v.re * x.im + v.im * x.re;
             }
             const u64 i = r * R + s;
             Zd w{ar, ai};
             if (step > 0) {
-                const Zd p = Vprev[i];
+                const Zd p = scaled(Vprev[i], fp);
                 w = Zd{w.re - b * p.re, w.im - b * p.im};
             }
             W[i] = w;
-            const Zd d = cdot(Vcur[i], w);
+            const Zd d = cdot(scaled(Vcur[i], fc), w);
             pr += d.re;
             pi += d.im;
         }
@@ -170,7 +180,7 @@ __global__ __launch_bounds__(kThreads) void k_batch_spmv_wide_fused(BatchView B,
 // Pairs no longer running are left alone.
 __global__ __launch_bounds__(kThreads) void k_batch_vec(BatchView B, int mode, const Zd* Vprev, const Zd* Vcur, Zd* W,
                                                         const double* coef, const Zd* zcoef, double threshold,
-                                                        double* partial, double* partial2) {
+                                                        double* partial, double* partial2, const double* vscale) {
     const Tile t = B.tiles[blockIdx.x];
     const int R = B.R, slots = kThreads / R;
     const int slot = threadIdx.x / R, s = threadIdx.x % R;
@@ -185,6 +195,7 @@ __global__ __launch_bounds__(kThreads) void k_batch_vec(BatchView B, int mode, c
     double pr = 0.0, pi = 0.0, pn = 0.0;
     if (live) {
         const int step = *B.step;
+        const double fc = vscale ? vscale[bs] : 1.0;   // V_cur = stored V_cur times fc
         for (u64 r = t.row_begin + slot; r < t.row_end; r += slots) {
             const u64 i = r * R + s;
             Zd w = W[i];
@@ -195,12 +206,12 @@ __global__ __launch_bounds__(kThreads) void k_batch_vec(BatchView B, int mode, c
                     w = Zd{w.re - b * p.re, w.im - b * p.im};
                     W[i] = w;
                 }
-                const Zd d = cdot(Vcur[i], w);
+                const Zd d = cdot(scaled(Vcur[i], fc), w);
                 pr += d.re;
                 pi += d.im;
             } else if (mode == 1 || mode == 4 || mode == 5) {
                 const double a = coef[bs];
-                const Zd v = Vcur[i];
+                const Zd v = scaled(Vcur[i], fc);
                 w = Zd{w.re - a * v.re, w.im - a * v.im};
                 W[i] = w;
                 if (mode != 4) {
@@ -214,7 +225,7 @@ __global__ __launch_bounds__(kThreads) void k_batch_vec(BatchView B, int mode, c
             } else if (mode == 2) {
                 const Zd ov = zcoef[bs];
                 if (step > 0 && sqrt(ov.re * ov.re + ov.im * ov.im) > threshold) {
-                    const Zd m = cmul(ov, Vcur[i]);
+                    const Zd m = cmul(ov, scaled(Vcur[i], fc));
                     w = Zd{w.re - m.re, w.im - m.im};
                     W[i] = w;
                 }
@@ -469,19 +480,20 @@ void batch_spmv(const BatchView& B, const Zd* X, Zd* Y, hipStream_t st) {
 }
 
 bool batch_spmv_fused(const BatchView& B, const Zd* Vprev, const Zd* Vcur, Zd* W, const double* beta, double* partial,
-                      hipStream_t st) {
+                      hipStream_t st, const double* sprev, const double* scur) {
     if (B.R < 8) return false;
     if (!B.n_tiles) return true;
     hipLaunchKernelGGL(k_batch_spmv_wide_fused, dim3(B.n_tiles), dim3(kThreads), 0, st, B, Vprev, Vcur, W, beta,
-                       partial);
+                       partial, sprev, scur);
     return true;
 }
 
 void batch_vec(const BatchView& B, int mode, const Zd* Vprev, const Zd* Vcur, Zd* W, const double* coef,
-               const Zd* zcoef, double threshold, double* partial, hipStream_t st, double* partial2) {
+               const Zd* zcoef, double threshold, double* partial, hipStream_t st, double* partial2,
+               const double* vscale) {
     if (!B.n_tiles) return;
     hipLaunchKernelGGL(k_batch_vec, dim3(B.n_tiles), dim3(kThreads), 0, st, B, mode, Vprev, Vcur, W, coef, zcoef,
-                       threshold, partial, partial2);
+                       threshold, partial, partial2, vscale);
 }
 
 void batch_finish(const BatchView& B, int kind, const double* partial, double* alpha_hist, double* beta_hist, Zd* ov,

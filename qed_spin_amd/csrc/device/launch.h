@@ -150,14 +150,17 @@ constexpr int kBatchTileRows = 256;
 constexpr int kBatchMaxR = 64;
 void batch_spmv(const BatchView& B, const Zd* X, Zd* Y, hipStream_t st);
 /// W = H V_cur - beta V_prev with the partials of <V_cur, W> (batch_spmv + batch_vec mode 0 in one
-/// kernel, the same arithmetic); false (nothing launched) when R < 8.
+/// kernel, the same arithmetic); false (nothing launched) when R < 8. sprev / scur: per-pair factors
+/// of the stored vectors (V = stored times factor), or null (1).
 bool batch_spmv_fused(const BatchView& B, const Zd* Vprev, const Zd* Vcur, Zd* W, const double* beta, double* partial,
-                      hipStream_t st);
+                      hipStream_t st, const double* sprev = nullptr, const double* scur = nullptr);
 /// mode 0: W -= beta V_prev, <V_cur, W>; 1: W -= alpha V_cur, <V_cur, W>; 5: mode 1 and ||W||^2
 /// into partial2; 4: W -= alpha V_cur, ||W||^2; 2: local reorthogonalisation, ||W||^2 (with B.fix:
-/// only the flagged pairs, into partial2); 3: W *= 1 / beta. Partials per tile and vector.
+/// only the flagged pairs, into partial2); 3: W *= 1 / beta. Partials per tile and vector. vscale:
+/// per-pair factor of the stored V_cur, or null (1).
 void batch_vec(const BatchView& B, int mode, const Zd* Vprev, const Zd* Vcur, Zd* W, const double* coef,
-               const Zd* zcoef, double threshold, double* partial, hipStream_t st, double* partial2 = nullptr);
+               const Zd* zcoef, double threshold, double* partial, hipStream_t st, double* partial2 = nullptr,
+               const double* vscale = nullptr);
 /// kind 0: alpha; 1: the overlap (and, with B.fix, the flag of the pairs whose |overlap| exceeds
 /// threshold from step 1 on); 2: beta (ends breaking-down or capped pairs). The step is read from
 /// B.step on the device.

@@ -280,6 +280,7 @@ void BatchedLanczos::run(int M, const std::vector<double>& breakdown_tol, bool l
     dev::fill_zero(false, alpha_h_.get(), hist, st);
     dev::fill_zero(false, beta_h_.get(), hist, st);
     dev::fill_zero(false, cur_b_.get(), pairs, st);
+    put(scale_, std::vector<double>(3 * pairs, 1.0), st);
     QSA_HIP(hipStreamSynchronize(st));
 
     const dev::BatchView B = view();
@@ -294,11 +295,21 @@ void BatchedLanczos::run(int M, const std::vector<double>& breakdown_tol, bool l
     double* ib = inv_b_.get<double>();
     const double* tol = tol_d_.get<double>();
     const double thr = 1.49011611938476562e-08;   // sqrt(eps): LocalDGKS3's overlap threshold
+    // With the fused SpMV (R >= 8) the next vector is never scaled in place: each of the three
+    // buffers keeps a per-pair factor (1 / beta of the step that wrote it; 1 for the normalised
+    // start) that every read applies, which rounds as the scaling pass would (same results, one
+    // pass over the vectors fewer per step).
+    const bool defer = R_ >= 8;
+    const std::size_t np = pairs;
+    double* S[3] = {scale_.get<double>(), scale_.get<double>() + np, scale_.get<double>() + 2 * np};
     auto step = [&](int phase) {
         dev::Zd* p = V[phase % 3];
         dev::Zd* c = V[(phase + 1) % 3];
         dev::Zd* w = V[(phase + 2) % 3];
-        if (!dev::batch_spmv_fused(B, p, c, w, cb, partial, st)) {
+        const double* sp = defer ? S[phase % 3] : nullptr;
+        const double* sc = defer ? S[(phase + 1) % 3] : nullptr;
+        double* sw = defer ? S[(phase + 2) % 3] : ib;
+        if (!dev::batch_spmv_fused(B, p, c, w, cb, partial, st, sp, sc)) {
             dev::batch_spmv(B, c, w, st);
             dev::batch_vec(B, 0, p, c, w, cb, nullptr, 0.0, partial, st);
         }
@@ -306,15 +317,15 @@ void BatchedLanczos::run(int M, const std::vector<double>& breakdown_tol, bool l
         if (local) {
             // W -= alpha V with the local overlap and ||W||^2 in one pass; the correction pass then
             // touches only the pairs whose overlap asks for one (their norms are summed again)
-            dev::batch_vec(B, 5, nullptr, c, w, ca, nullptr, 0.0, partial, st, partial2);
+            dev::batch_vec(B, 5, nullptr, c, w, ca, nullptr, 0.0, partial, st, partial2, sc);
             dev::batch_finish(B, 1, partial, ah, bh, ov, ca, cb, ib, tol, st, thr);
-            dev::batch_vec(B, 2, nullptr, c, w, nullptr, ov, thr, partial, st, partial2);
-            dev::batch_finish(B, 2, partial2, ah, bh, ov, ca, cb, ib, tol, st);
+            dev::batch_vec(B, 2, nullptr, c, w, nullptr, ov, thr, partial, st, partial2, sc);
+            dev::batch_finish(B, 2, partial2, ah, bh, ov, ca, cb, sw, tol, st);
         } else {
-            dev::batch_vec(B, 4, nullptr, c, w, ca, nullptr, 0.0, partial, st);
-            dev::batch_finish(B, 2, partial, ah, bh, ov, ca, cb, ib, tol, st);
+            dev::batch_vec(B, 4, nullptr, c, w, ca, nullptr, 0.0, partial, st, nullptr, sc);
+            dev::batch_finish(B, 2, partial, ah, bh, ov, ca, cb, sw, tol, st);
         }
-        dev::batch_vec(B, 3, nullptr, nullptr, w, ib, nullptr, 0.0, partial, st);
+        if (!defer) dev::batch_vec(B, 3, nullptr, nullptr, w, ib, nullptr, 0.0, partial, st);
         dev::batch_advance(B, st);
     };
     const int Mrun = *std::max_element(cap.begin(), cap.end());

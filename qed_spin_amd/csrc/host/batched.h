@@ -65,7 +65,8 @@ private:
     DeviceBuffer row_ptr_, col_, val_, vid_, dict_re_, dict_im_, tiles_d_, tile_begin_d_, cap_d_, active_d_, steps_d_,
         step_d_;
     std::uint32_t dict_n_ = 0;
-    DeviceBuffer V0_, V1_, W_, partial_, partial2_, fix_, alpha_h_, beta_h_, ov_, cur_a_, cur_b_, inv_b_, tol_d_;
+    DeviceBuffer V0_, V1_, W_, partial_, partial2_, fix_, scale_, alpha_h_, beta_h_, ov_, cur_a_, cur_b_, inv_b_,
+        tol_d_;
     friend struct BatchedAccess;
 };
 
