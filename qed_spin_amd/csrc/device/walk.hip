@@ -135,14 +135,21 @@ __device__ int merged_row(const SectorView& R, const SectorView& C, bool same, c
 
 __device__ __forceinline__ u64 bits_of(double x) { return static_cast<u64>(__double_as_longlong(x)); }
 
-__device__ __forceinline__ u64 fingerprint(u64 re, u64 im) {
-    u64 z = re * 0x9E3779B97F4A7C15ULL ^ (im + 0x632BE59BD9B4E019ULL + (re << 7));
-    z ^= z >> 31;
+__device__ __forceinline__ u64 mix64(u64 z) {
+    z ^= z >> 30;
     z *= 0xBF58476D1CE4E5B9ULL;
-    z ^= z >> 29;
+    z ^= z >> 27;
     z *= 0x94D049BB133111EBULL;
-    z ^= z >> 32;
-    return z | 1ULL;
+    return z ^ (z >> 31);
+}
+
+// The dictionary key of a value: each word through its own bijective mix before they are combined,
+// so structured pairs (v and -v: both sign bits flipped) do not cancel; 0 marks an empty slot.
+__device__ __forceinline__ u64 fingerprint(u64 re, u64 im) {
+    const u64 a = mix64(re ^ 0x9E3779B97F4A7C15ULL);
+    const u64 b = mix64(im ^ 0xD1B54A32D192ED03ULL);
+    const u64 z = mix64(a ^ ((b << 32) | (b >> 32)));
+    return z ? z : 1ULL;
 }
 
 template <bool SLC>

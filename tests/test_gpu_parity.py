@@ -245,3 +245,26 @@ def test_sectors_built_on_threads_with_own_streams_equal_sequential_builds():
     for a, b in zip(one, many):
         for x, y in zip(a, b):
             assert np.array_equal(np.asarray(x), np.asarray(y))
+
+
+def test_dictionary_keeps_values_and_their_negatives_apart():
+    """A momentum sector whose merged values hold pairs v, -v (J1-J2 ring N = 22, k = 1: 345 distinct
+    values) keeps its value dictionary (ids, not full values), and its entries are the oracle's bit
+    for bit (as a multiset: the sector may list its representatives in key order)."""
+    say("dictionary of v and -v")
+    n = 22
+    b = qs.HamiltonianBuilder(n)
+    b.heisenberg([(i, (i + 1) % n) for i in range(n)], 1.0)
+    b.heisenberg([(i, (i + 2) % n) for i in range(n)], 0.3)
+    H = b.to_operator()
+    g = sym.chain_group(n, 1)
+    s = qs.Sector(g, n // 2)
+    m = qs.SectorMatvec(H, s, lane=qs.LANE_CSR)
+    assert m.info["value_bytes"] <= 2 and m.info["dict_size"] == 345
+    os_ = po.OracleSector(n, n // 2, g.perms, g.chars)
+    rp, col, val = po.sector_csr(qs.Program(H, adjoint=True).export(), os_)
+    r2, c2, v2 = m.export_csr()
+    assert len(c2) == len(col)
+    w1 = np.sort(bits(np.asarray(val).view(np.float64)).reshape(-1, 2).view([("re", np.uint64), ("im", np.uint64)]), axis=0)
+    w2 = np.sort(bits(np.asarray(v2).view(np.float64)).reshape(-1, 2).view([("re", np.uint64), ("im", np.uint64)]), axis=0)
+    assert np.array_equal(w1, w2)
