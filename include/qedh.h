@@ -181,6 +181,11 @@ int qedh_block_lowest(qedh_matvec m, int k, int64_t max_iter, double* values, in
 /* The lowest k eigenpairs of the block (Krylov-Schur with locking at 1e-10 s_H, the vectors of
  * solve_block_eigenpairs, src/engine/block_solve.cpp:600-655): values[k] ascending, vectors: a
  * DEVICE buffer of k x dim complex entries (vector i at vectors + i dim), normalised. */
+/* The pruning estimate of a block (estimate_lowest, src/engine/block_solve.cpp:630-655, as the eigs
+ * walk prunes with it, src/engine/eigs.cpp:129-155, 450-560): 40 Lanczos steps without
+ * reorthogonalisation from the fixed start seeded 0xE57A7E; *theta the lowest Ritz value (an upper
+ * bound on the block's lowest level), *residual its residual bound; -inf when nothing ran. */
+int qedh_block_estimate(qedh_matvec m, double* theta, double* residual, int64_t* applies);
 int qedh_block_eigenpairs(qedh_matvec m, int k, int64_t max_iter, double* values, void* vectors, int* n_values,
                           int* converged);
 /* FTLM of the block: samples from splitmix64(seed + phi (s + 1)); lnZ/E/V per beta; *e_min. */

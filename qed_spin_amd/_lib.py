@@ -90,6 +90,7 @@ _SIGNATURES = {
     "qedh_matvec_benchmark": [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, c_dbl_p],
     "qedh_lanczos": [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, c_dbl_p, c_dbl_p, c_int_p, c_dbl_p],
     "qedh_block_lowest": [C.c_void_p, C.c_int, C.c_int64, c_dbl_p, c_int_p, c_int_p, c_i64_p],
+    "qedh_block_estimate": [C.c_void_p, c_dbl_p, c_dbl_p, C.POINTER(C.c_int64)],
     "qedh_block_eigenpairs": [C.c_void_p, C.c_int, C.c_int64, c_dbl_p, C.c_void_p, c_int_p, c_int_p],
     "qedh_ftlm": [
         C.c_void_p, C.c_int, C.c_int, C.c_uint64, C.c_int, c_dbl_p, C.c_double, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p

@@ -463,6 +463,29 @@ int qedh_block_lowest(qedh_matvec m, int k, int64_t max_iter, double* values, in
     });
 }
 
+int qedh_block_estimate(qedh_matvec m, double* theta, double* residual, int64_t* applies) {
+    return guarded([&] {
+        need(m, "matvec");
+        need(theta, "theta");
+        need(residual, "residual");
+        const auto& op = *m->op;
+        if (!op.same()) throw qsa::Error(qsa::kInvalid, "estimate: the operator maps two sectors");
+        const std::size_t n = static_cast<std::size_t>(op.dim());
+        auto run = [&](auto tag) {
+            using S = decltype(tag);
+            qsa::HipBackend<S> be;
+            const bool cv = qsa::HipBackend<S>::kComplex;
+            auto H = [&](const S* in, S* out) { op.apply(in, out, cv); };
+            const qsa::BlockEstimate e = qsa::estimate_lowest(be, H, n);
+            *theta = e.theta;
+            *residual = e.residual;
+            if (applies) *applies = static_cast<int64_t>(e.applies);
+        };
+        if (op.is_real()) run(double{});
+        else run(qsa::cplx{});
+    });
+}
+
 int qedh_block_eigenpairs(qedh_matvec m, int k, int64_t max_iter, double* values, void* vectors, int* n_values,
                           int* converged) {
     return guarded([&] {

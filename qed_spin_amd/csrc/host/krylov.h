@@ -158,6 +158,38 @@ BlockSolution lowest_scan(const HipBackend<Scalar>& be, Matvec&& H, std::size_t 
     return sol;
 }
 
+/// The pruning estimate of a block (block_solve.cpp:630-655): 40 Lanczos steps without
+/// reorthogonalisation from the fixed start gaussian_start(n, 0xE57A7E); theta is the lowest Ritz
+/// value (an upper bound on the block's lowest level), residual |beta_m| |z_(m-1),0| its residual
+/// bound. The caller prunes on theta - residual.
+struct BlockEstimate {
+    double theta = -std::numeric_limits<double>::infinity();
+    double residual = 0.0;
+    std::size_t applies = 0;
+};
+inline constexpr std::uint64_t kEstimateSeed = 0xE57A7EULL;
+
+template <class Scalar, class Matvec>
+BlockEstimate estimate_lowest(const HipBackend<Scalar>& be, Matvec&& H, std::size_t n) {
+    BlockEstimate est;
+    if (n == 0) return est;
+    LanczosOptions<Scalar> o;
+    o.max_iter = std::min<std::size_t>(40, n);
+    o.reorth = Reorth::None;
+    o.keep_basis = false;
+    const auto h = gaussian_start<Scalar>(n, kEstimateSeed);
+    auto v0 = be.make_zero_vector(n);
+    be.copy_from_host(h.data(), v0.get(), n);
+    auto r = lanczos_run(be, H, n, v0.get(), o);
+    est.applies = r.timings.applies;
+    if (r.alpha.empty()) return est;
+    const std::size_t m = r.alpha.size();
+    const TridiagEig t = tridiag_eig(r.alpha, r.beta, m, true);
+    est.theta = t.values.front();
+    est.residual = r.beta.size() > m ? std::abs(r.beta[m]) * std::abs(t.z(m - 1, 0)) : 0.0;
+    return est;
+}
+
 struct KsOptions {
     std::size_t num_eigs = 1;
     std::size_t max_iter = 100;        ///< cycle length hint

@@ -26,7 +26,7 @@ import numpy as np
 from . import _lib
 from .errors import DeviceUnsupported, InvalidRequest
 from .operator import Operator, Program
-from .sector import DEFAULT_CSR_BUDGET, LANE_GATHER, Sector, SectorMatvec
+from .sector import parallel_build, DEFAULT_CSR_BUDGET, LANE_GATHER, Sector, SectorMatvec
 from .symmetry import Symmetry
 from .thermal import KBREAKDOWN_REL, _handles, resolve_device, walk_blocks
 
@@ -72,18 +72,21 @@ class DynamicsResult:
     engine: Dict[str, float] = field(default_factory=dict)
 
 
-def ground_manifold(blocks, window: float, k_max: int = 16, only=None):
+def ground_manifold(blocks, window: float, k_max: int = 16, only=None, prune: bool = True, norm_bound: float = 1.0):
     """(E0, [(block index, device buffer, vector index, energy)]): every level within `window` of the
-    lowest level of the walk. Each block's lowest level comes from the k = 1 scan; the blocks within
-    reach of the lowest then give their eigenpairs (Krylov-Schur, k doubled until a level outside the
-    window shows or k_max), and E0 is the lowest eigenvalue among them."""
-    lows = []
-    for i, (blk, sec, m) in enumerate(blocks):
-        if only is not None and i not in only:
-            lows.append(math.inf)
-            continue
-        vals, _, _ = m.lowest(1)
-        lows.append(vals[0] if len(vals) else math.inf)
+    lowest level of the walk (src/engine/dynamics.cpp:147-193). Each block's lowest level comes from a
+    k = 1 eigs walk with the window (blocks whose 40-step estimate lies far above are pruned, as the
+    reference prunes by default); the blocks within reach of the lowest then give their eigenpairs
+    (Krylov-Schur, k doubled until a level outside the window shows or k_max), and E0 is the lowest
+    eigenvalue among them."""
+    from .eigs import dense_floor, walk_levels
+
+    idx = [i for i in range(len(blocks)) if only is None or i in only]
+    lv, _ = walk_levels([blocks[i] for i in idx], 1, floor=dense_floor(1, None), prune=prune, window=window,
+                        norm_bound=norm_bound)
+    lows = [math.inf] * len(blocks)
+    for q, (v, _, _) in lv.items():
+        lows[idx[q]] = float(v[0]) if len(v) else math.inf
     e_scan = min(lows)
     slack = window + 1e-9 * max(1.0, abs(e_scan))
     found = []
@@ -162,8 +165,10 @@ def _dynamics_finite_T(H, probes, omega, T, eta, blocks, krylov, samples, seed, 
     for blk, _, _ in blocks:
         n_irr[blk.n_up] = max(n_irr.get(blk.n_up, 0), (blk.momentum or 0) + 1)
     tol = KBREAKDOWN_REL * (H.norm_bound() or 1.0)
-    per_src = []
-    for b in (range(len(blocks)) if sources is None else sources):
+    for O in probes:
+        O._handle()
+
+    def one_source(b):
         blk, sec, m = blocks[b]
         irr = blk.group.G if blk.momentum is not None else 1
         key = ((sec.n_up + 1) * 3 + 0) * irr + (blk.momentum or 0)
@@ -183,7 +188,11 @@ def _dynamics_finite_T(H, probes, omega, T, eta, blocks, krylov, samples, seed, 
         Sp = np.zeros((P, nT, nW), np.complex128)
         for q, (p, _, _) in enumerate(reach):
             Sp[p] += S[q, :, :, 0] + 1j * S[q, :, :, 1]
-        per_src.append((emin.value, Z, Sp))
+        return emin.value, Z, Sp
+
+    # sources are independent: several at once on host threads, each with a stream of its own (a
+    # small source's Lanczos runs are launch-bound); the sum below runs in source order
+    per_src = parallel_build(one_source, list(range(len(blocks)) if sources is None else sources))
     ref = min(e for e, _, _ in per_src)
     out = []
     for p in range(P):
@@ -236,7 +245,8 @@ def dynamics(H: Operator, probes, omega, *, eta: float = 0.1, T=None, sym: Optio
     sH = H.norm_bound() or 1.0
     window = degeneracy_tol * sH
     t1 = time.perf_counter()
-    e0, states = ground_manifold(blocks, window, only=None if ground_blocks is None else set(ground_blocks))
+    e0, states = ground_manifold(blocks, window, only=None if ground_blocks is None else set(ground_blocks),
+                                 norm_bound=sH)
     t_gm = time.perf_counter() - t1
     # probe images: (state, probe) -> every block it reaches
     t2 = time.perf_counter()

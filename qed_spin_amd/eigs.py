@@ -1,10 +1,11 @@
 """The eigs verb on the HIP backend: the k lowest levels of H over the blocks of a symmetry walk (the
 reference's qed.eigs, python/qed/_verbs/eigs.py, driving src/engine/eigs.cpp:360-450 per block).
 
-Per block (in the walk's order): want = min(k, dim) levels; blocks of at most two states, or at
-most the dense floor, are solved densely on the host (the reference's HostDense lane,
-src/engine/block_solve.cpp:164-262); the others on the device, k = 1 by the lowest-level scan and
-k > 1 by Krylov-Schur (block_solve.cpp:263-655). The levels of every block are merged and the
+Per block: want = min(k, dim) levels; blocks of at most two states, or at most the dense floor, are
+solved densely (the reference's dense lanes, src/engine/block_solve.cpp:164-262: host pool, or the
+device from 1024 states); the others on the device, k = 1 by the lowest-level scan and k > 1 by
+Krylov-Schur (block_solve.cpp:263-655). Blocks above 64 states are first estimated and pruned as
+the reference's walk prunes them (walk_levels). The levels of every solved block are merged and the
 lowest k returned, ascending.
 
 The expect verb (python/qed/_verbs/expect.py) returns, for each of the k lowest levels, the
@@ -21,6 +22,7 @@ below on the host pool -- and returns all eigenvalues, ascending.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import time
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional
@@ -30,7 +32,7 @@ import numpy as np
 from . import _lib
 from .errors import InvalidRequest
 from .operator import Operator
-from .sector import DEFAULT_CSR_BUDGET, LANE_GATHER, Sector, SectorMatvec
+from .sector import DEFAULT_CSR_BUDGET, LANE_GATHER, Sector, SectorMatvec, parallel_build
 from .symmetry import Symmetry
 from .thermal import _handles, resolve_device, walk_blocks
 
@@ -59,10 +61,70 @@ class EigsResult:
     timings: Dict[str, float] = field(default_factory=dict)
 
 
+PRUNE_MIN_DIM = 64   # blocks above this are estimated before they are solved (eigs.cpp:129-132)
+
+
+def _solve_block(m, dim: int, want: int, floor: int, max_iter: int):
+    """The `want` lowest levels of one block: densely (host or device lane) at or below the dense
+    floor, else the device Krylov lanes. (values, converged, lane)"""
+    if dim <= 2 or dim <= floor:
+        vals = np.zeros(dim)
+        _lib.call("qedh_block_spectrum", _handles([m]), 1, C.c_int64(-1), _lib.ptr(vals, C.c_double))
+        return vals[:want].copy(), True, "device-dense" if dim >= DEVICE_DENSE_MIN_DIM else "host-dense"
+    v, conv, _ = m.lowest(want, max_iter)
+    return np.asarray(v), conv, "device-krylov"
+
+
+def walk_levels(blocks, k: int, *, floor: int, prune: bool = True, prune_margin: float = 0.02, window: float = 0.0,
+                norm_bound: float = 1.0, max_iter: int = 0):
+    """The levels of the blocks of a walk as the reference's eigs walk finds them
+    (src/engine/eigs.cpp:360-560): blocks of at most max(min(dense floor, 64), 2) states are solved
+    outright; every larger block first gets its pruning estimate (40 Lanczos steps, theta_1 - |r_1|,
+    on host threads with streams of their own), then the blocks are taken in order of increasing
+    estimate and solved only when the estimate lies within prune_margin (relative to |E_k|, floored at
+    5% of s_H) or `window` of the k-th level found so far. Returns {block index: (levels, converged,
+    lane)} for the blocks solved and the number pruned."""
+    levels = {}
+    small = max(min(floor, PRUNE_MIN_DIM), 2)
+    outright = [i for i, (_, s, _) in enumerate(blocks) if not prune or s.dim <= small]
+    cand = [i for i, (_, s, _) in enumerate(blocks) if prune and s.dim > small]
+    dense = [i for i in outright if blocks[i][1].dim <= 2 or blocks[i][1].dim <= floor]
+    if dense:
+        mats = [blocks[i][2] for i in dense]
+        dims = [mm.dim for mm in mats]
+        vals = np.zeros(int(sum(dims)))
+        _lib.call("qedh_block_spectrum", _handles(mats), len(mats), C.c_int64(-1), _lib.ptr(vals, C.c_double))
+        at = 0
+        for i, d in zip(dense, dims):
+            levels[i] = (vals[at: at + min(k, d)].copy(), True,
+                         "device-dense" if d >= DEVICE_DENSE_MIN_DIM else "host-dense")
+            at += d
+    for i in outright:
+        if i not in levels:
+            blk, sec, m = blocks[i]
+            levels[i] = _solve_block(m, sec.dim, min(k, sec.dim), floor, max_iter)
+    est = parallel_build(lambda i: blocks[i][2].estimate(), cand)
+    order = sorted(range(len(cand)), key=lambda q: (est[q], cand[q]))
+    margin_floor = 0.05 * (norm_bound if norm_bound > 0 else 1.0)
+    pruned = 0
+    for q in order:
+        i = cand[q]
+        found = np.sort(np.concatenate([v for v, _, _ in levels.values()])) if levels else np.zeros(0)
+        kth = found[k - 1] if found.size >= k else math.inf
+        if est[q] > kth + max(prune_margin * max(margin_floor, abs(kth)), window):
+            pruned += 1
+            continue
+        blk, sec, m = blocks[i]
+        levels[i] = _solve_block(m, sec.dim, min(k, sec.dim), floor, max_iter)
+    return levels, pruned
+
+
 def eigs(H: Operator, k: int = 4, *, sym: Optional[Symmetry] = None, device: str = "gpu",
-         dense_max_dim: Optional[int] = None, max_iter: int = 0,
-         csr_budget: int = DEFAULT_CSR_BUDGET) -> EigsResult:
-    """The k lowest levels of H over the blocks of `sym` (default: every (Sz, k) block of a ring)."""
+         dense_max_dim: Optional[int] = None, max_iter: int = 0, prune: bool = True, prune_margin: float = 0.02,
+         window: float = 0.0, csr_budget: int = DEFAULT_CSR_BUDGET) -> EigsResult:
+    """The k lowest levels of H over the blocks of `sym` (default: every (Sz, k) block of a ring).
+    prune (the reference's default): blocks whose 40-step estimate lies far above the k-th level
+    found so far are not solved."""
     resolve_device(device, "eigs")
     if int(k) < 1:
         raise InvalidRequest("eigs: k must be >= 1")
@@ -73,39 +135,23 @@ def eigs(H: Operator, k: int = 4, *, sym: Optional[Symmetry] = None, device: str
     if not blocks:
         raise InvalidRequest("eigs: the symmetry selection holds no state")
     t_build = time.perf_counter() - t0
-    floor = dense_floor(k, dense_max_dim)
-    recs = []
-    dense_idx = [i for i, (_, s, _) in enumerate(blocks) if s.dim <= 2 or s.dim <= floor]
-    levels = {}
     t1 = time.perf_counter()
-    if dense_idx:
-        mats = [blocks[i][2] for i in dense_idx]
-        dims = [m.dim for m in mats]
-        vals = np.zeros(int(sum(dims)))
-        _lib.call("qedh_block_spectrum", _handles(mats), len(mats), C.c_int64(-1), _lib.ptr(vals, C.c_double))
-        at = 0
-        for i, d in zip(dense_idx, dims):
-            levels[i] = (vals[at: at + min(k, d)].copy(), True, "device-dense" if d >= DEVICE_DENSE_MIN_DIM else "host-dense")
-            at += d
-    t_dense = time.perf_counter() - t1
-    t2 = time.perf_counter()
-    converged = True
-    for i, (blk, sec, m) in enumerate(blocks):
-        if i in levels:
-            continue
-        want = min(k, sec.dim)
-        v, conv, _ = m.lowest(want, max_iter)
-        converged = converged and conv
-        levels[i] = (np.asarray(v), conv, "device-krylov")
-    t_krylov = time.perf_counter() - t2
-    allv = []
+    levels, pruned = walk_levels(blocks, k, floor=dense_floor(k, dense_max_dim), prune=prune,
+                                 prune_margin=prune_margin, window=window, norm_bound=H.norm_bound(),
+                                 max_iter=max_iter)
+    t_solve = time.perf_counter() - t1
+    recs, allv, converged = [], [], True
     for i, (blk, sec, _) in enumerate(blocks):
+        if i not in levels:
+            recs.append({"label": blk.label, "dim": sec.dim, "lane": "pruned", "levels": np.zeros(0), "converged": True})
+            continue
         v, conv, lane = levels[i]
+        converged = converged and conv
         recs.append({"label": blk.label, "dim": sec.dim, "lane": lane, "levels": v, "converged": conv})
         allv.extend(v.tolist())
     energies = np.sort(np.asarray(allv))[:k]
     return EigsResult(energies=energies, blocks=recs, converged=converged,
-                      timings={"build_s": t_build, "dense_s": t_dense, "krylov_s": t_krylov,
+                      timings={"build_s": t_build, "solve_s": t_solve, "pruned_blocks": pruned,
                                "total_s": time.perf_counter() - t0})
 
 

@@ -247,6 +247,12 @@ class SectorMatvec:
                   C.byref(n), C.byref(conv), C.byref(applies))
         return vals[: n.value], bool(conv.value), int(applies.value)
 
+    def estimate(self) -> float:
+        """The pruning estimate theta_1 - |r_1| of the block's lowest level (40 Lanczos steps)."""
+        th, res, ap = C.c_double(), C.c_double(), C.c_int64()
+        _lib.call("qedh_block_estimate", self.h, C.byref(th), C.byref(res), C.byref(ap))
+        return th.value - res.value
+
     def ftlm(self, samples: int, krylov: int, seed: int, betas, breakdown_tol: float = 0.0):
         betas = np.ascontiguousarray(betas, np.float64)
         nT = betas.size

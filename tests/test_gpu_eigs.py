@@ -46,17 +46,23 @@ def _h12():
     return b.to_operator()
 
 
+@pytest.mark.parametrize("prune", [True, False])
 @pytest.mark.parametrize("dense_max_dim", [None, 0])
 @pytest.mark.parametrize("key", list(CASES))
-def test_eigs_reproduces_reference_records(key, dense_max_dim):
-    say("eigs", key, "dense_max_dim", dense_max_dim)
+def test_eigs_reproduces_reference_records(key, dense_max_dim, prune):
+    say("eigs", key, "dense_max_dim", dense_max_dim, "prune", prune)
     want = _records()[f"j1j2_chain12/api/eigs/{key}/k=4"]["values"]["energies"]
-    r = ev.eigs(_h12(), 4, sym=CASES[key], dense_max_dim=dense_max_dim)
+    r = ev.eigs(_h12(), 4, sym=CASES[key], dense_max_dim=dense_max_dim, prune=prune)
     assert r.converged
     assert np.allclose(r.energies, want, rtol=1e-10, atol=1e-10), (r.energies, want)
     lanes = {b["lane"] for b in r.blocks}
     if dense_max_dim == 0:
         assert "device-krylov" in lanes
+    if not prune:
+        assert r.timings["pruned_blocks"] == 0 and "pruned" not in lanes
+    elif key == "spatial=none":
+        # the Sz blocks of 220-924 states far from the lowest levels are estimated and skipped
+        assert r.timings["pruned_blocks"] > 0
 
 
 @pytest.mark.parametrize("device_min_dim", [None, 0])
