@@ -160,6 +160,40 @@ def run_dynamics(args):
     return out
 
 
+def run_reference_dynamics(args):
+    """The reference's two published dynamics cases, same inputs (its bench/cases.py:105-132):
+    chain30_dyn0 -- T = 0 S^z(q = 2 pi 10 / 30) of the Heisenberg ring N = 30 in Sz = 0 over every
+    momentum block, krylov 200, omega = linspace(-1, 7, 401), eta 0.05 (published: 57.6 s on a
+    32-core node); chain24_dynT -- T = 1 FTLM S^z(q = 2 pi 8 / 24) of the ring N = 24 over every
+    (Sz, k) source block, 5 samples, krylov 80, omega = linspace(-3, 7, 301), eta 0.1 (published:
+    ~85 s on a 32-core node). The published numbers are context: another machine."""
+    import numpy as np
+
+    import qed_spin_amd as qs
+    from qed_spin_amd import dynamics as dy
+
+    out = {}
+    n = 30
+    H = qs.HamiltonianBuilder(n).heisenberg([(i, (i + 1) % n) for i in range(n)], 1.0).to_operator()
+    w = np.linspace(-1, 7, 401)
+    t0 = time.perf_counter()
+    r = dy.dynamics(H, [dy.sz_q(n, 2 * np.pi * 10 / n)], w, eta=0.05, krylov=200, sym=qs.Symmetry(sz=[15]))
+    out["chain30_dyn0"] = {"wall_s": round(time.perf_counter() - t0, 3), "weight": float(np.trapezoid(r.S[0], w)),
+                           "e0": r.e0, "timings": {k: round(v, 3) for k, v in r.timings.items()},
+                           "published_cpu_s": 57.6}
+    log(f"chain30_dyn0: {out['chain30_dyn0']}")
+    n = 24
+    H = qs.HamiltonianBuilder(n).heisenberg([(i, (i + 1) % n) for i in range(n)], 1.0).to_operator()
+    w = np.linspace(-3, 7, 301)
+    t0 = time.perf_counter()
+    r = dy.dynamics(H, [dy.sz_q(n, 2 * np.pi * 8 / n)], w, eta=0.1, krylov=80, T=[1.0], samples=5, seed=11,
+                    sym=qs.Symmetry())
+    out["chain24_dynT"] = {"wall_s": round(time.perf_counter() - t0, 3),
+                           "weight": float(np.trapezoid(r.S[0][0].real, w)), "published_cpu_s": 85.0}
+    log(f"chain24_dynT: {out['chain24_dynT']}")
+    return out
+
+
 def run_spectrum(args):
     """The whole spectrum of the Heisenberg ring over every (Sz, k) block: the dense lanes (blocks
     from 1024 states on the device, rocSOLVER zheevd batched per dimension; smaller ones on the host
@@ -246,7 +280,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--workload", default="matvec,serve",
-                    help="matvec (the headline) and a comma list of ftlm, dynamics, serve, spectrum")
+                    help="matvec (the headline) and a comma list of ftlm, dynamics, serve, spectrum, refdyn")
     ap.add_argument("--ftlm-sites", type=int, default=24)
     ap.add_argument("--dyn-sites", type=int, default=28)
     ap.add_argument("--spectrum-sites", type=int, default=18)
@@ -403,6 +437,8 @@ def main():
             out["serve"] = run_serve(args)
         elif w == "spectrum":
             out["spectrum"] = run_spectrum(args)
+        elif w == "refdyn":
+            out["reference_dynamics"] = run_reference_dynamics(args)
         else:
             raise SystemExit(f"unknown workload {w!r}")
     print(json.dumps(out))
