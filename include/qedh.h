@@ -97,6 +97,15 @@ int qedh_program_destroy(qedh_program p);
 int qedh_sector_create(int n_sites, int n_up, int group_size, const int* perms, const uint64_t* flips,
                        const double* chars, int sublattice_mode, int relaxed, uint64_t rank_budget_bytes,
                        qedh_sector* out);
+/* A sector from representatives and 1/norms computed elsewhere -- the reference's RepSectorData
+ * (reps, inv_norms of include/ed/basis/rep_sector.h:134-408) handed over as they are: the group's
+ * tables (LUT, sublattice code per sublattice_mode / relaxed as in qedh_sector_create) and the rank
+ * table are built on the device, and every representative is checked to be a state of the subspace
+ * whose lookup finds itself (plain or key order, as the group's code has it); InvalidRequest names
+ * the first that does not. The 1/norms are taken as given. */
+int qedh_sector_import(int n_sites, int n_up, int group_size, const int* perms, const uint64_t* flips,
+                       const double* chars, int sublattice_mode, int relaxed, int64_t n_reps, const uint64_t* reps,
+                       const double* inv_norms, uint64_t rank_budget_bytes, qedh_sector* out);
 /* dim, orbits before the character filter, sublattice block size (0: plain order), build seconds. */
 int qedh_sector_info(qedh_sector s, int64_t* dim, int64_t* n_orbits, int* sublattice_block, double* build_seconds);
 int qedh_sector_export(qedh_sector s, uint64_t* reps, double* inv_norms);
@@ -132,6 +141,10 @@ int qedh_matvec_info(qedh_matvec m, qedh_matvec_info_t* info);
 /* out = O in (bind_cuda, include/ed/matvec/linear_operator.h:51-54); complex_vectors != 0 applies a
  * real operator to complex vectors. */
 int qedh_matvec_apply(qedh_matvec m, const void* in, void* out, int complex_vectors);
+/* The same on host buffers (the reference's LinearOperator::apply(const Complex* in, Complex* out,
+ * size_t n), include/ed/matvec/linear_operator.h:32-77): in copied to the device, applied, out copied
+ * back; real doubles when the operator is real and complex_vectors is 0, else complex pairs. */
+int qedh_matvec_apply_host(qedh_matvec m, const void* in, void* out, int complex_vectors);
 /* outs[i] = O ins[i] (bind_cuda_multi; k-vector applies equal single applies bit for bit). */
 int qedh_matvec_apply_multi(qedh_matvec m, const void* const* ins, void* const* outs, int k, int complex_vectors);
 /* Block-interleaved X[row * nv + i], nv in {1, 2, 4, 8}; CSR lane. */

@@ -69,6 +69,10 @@ _SIGNATURES = {
         C.POINTER(C.c_void_p),
     ],
     "qedh_program_destroy": [C.c_void_p],
+    "qedh_sector_import": [
+        C.c_int, C.c_int, C.c_int, c_int_p, c_u64_p, c_dbl_p, C.c_int, C.c_int, C.c_int64, c_u64_p, c_dbl_p, C.c_uint64,
+        C.POINTER(C.c_void_p),
+    ],
     "qedh_sector_create": [
         C.c_int, C.c_int, C.c_int, c_int_p, c_u64_p, c_dbl_p, C.c_int, C.c_int, C.c_uint64, C.POINTER(C.c_void_p)
     ],
@@ -80,6 +84,7 @@ _SIGNATURES = {
     "qedh_matvec_create": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.c_uint64, C.POINTER(C.c_void_p)],
     "qedh_matvec_info": [C.c_void_p, C.POINTER(MatvecInfo)],
     "qedh_matvec_apply": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int],
+    "qedh_matvec_apply_host": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int],
     "qedh_matvec_apply_multi": [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_int, C.c_int],
     "qedh_matvec_apply_block": [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int],
     "qedh_matvec_set_lanes": [C.c_void_p, C.c_int],

@@ -290,6 +290,28 @@ int qedh_sector_create(int n_sites, int n_up, int group_size, const int* perms, 
         *out = new qedh_sector_s{std::make_shared<qsa::DeviceSector>(std::move(g), rank_budget_bytes)};
     });
 }
+int qedh_sector_import(int n_sites, int n_up, int group_size, const int* perms, const uint64_t* flips,
+                       const double* chars, int sublattice_mode, int relaxed, int64_t n_reps, const uint64_t* reps,
+                       const double* inv_norms, uint64_t rank_budget_bytes, qedh_sector* out) {
+    return guarded([&] {
+        need(perms, "perms");
+        need(chars, "chars");
+        need(out, "out");
+        if (n_sites < 1 || n_sites > 63 || group_size < 1) throw qsa::Error(qsa::kInvalid, "sector: bad sizes");
+        qsa::GroupSpec g;
+        g.n_sites = n_sites;
+        g.n_up = n_up;
+        g.G = group_size;
+        g.perms.assign(perms, perms + static_cast<std::size_t>(group_size) * n_sites);
+        if (flips) g.flips.assign(flips, flips + group_size);
+        else g.flips.assign(group_size, 0ULL);
+        g.chars.resize(group_size);
+        for (int i = 0; i < group_size; ++i) g.chars[i] = qsa::cplx(chars[2 * i], chars[2 * i + 1]);
+        g.slc = qsa::build_sublattice(g.perms, g.flips, group_size, n_sites, sublattice_mode, relaxed != 0);
+        *out = new qedh_sector_s{
+            std::make_shared<qsa::DeviceSector>(std::move(g), reps, inv_norms, n_reps, rank_budget_bytes)};
+    });
+}
 int qedh_sector_info(qedh_sector s, int64_t* dim, int64_t* n_orbits, int* sublattice_block, double* build_seconds) {
     return guarded([&] {
         need(s, "sector");
@@ -359,6 +381,21 @@ int qedh_matvec_apply(qedh_matvec m, const void* in, void* out, int complex_vect
     return guarded([&] {
         need(m, "matvec");
         m->op->apply(in, out, complex_vectors != 0);
+    });
+}
+int qedh_matvec_apply_host(qedh_matvec m, const void* in, void* out, int complex_vectors) {
+    return guarded([&] {
+        need(m, "matvec");
+        need(in, "in");
+        need(out, "out");
+        const auto& op = *m->op;
+        const std::size_t es = (op.is_real() && !complex_vectors) ? 8 : 16;
+        const std::size_t nin = static_cast<std::size_t>(op.col_dim()) * es, nout = static_cast<std::size_t>(op.dim()) * es;
+        qsa::DeviceBuffer x(nin ? nin : 8), y(nout ? nout : 8);
+        qsa::copy_to_device(x.get(), in, nin);
+        op.apply(x.get(), y.get(), complex_vectors != 0);
+        QSA_HIP(hipGetLastError());
+        qsa::copy_to_host(out, y.get(), nout);
     });
 }
 int qedh_matvec_apply_multi(qedh_matvec m, const void* const* ins, void* const* outs, int k, int complex_vectors) {

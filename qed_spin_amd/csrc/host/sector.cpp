@@ -54,9 +54,8 @@ void require_device() {
     }
 }
 
-DeviceSector::DeviceSector(GroupSpec spec, std::uint64_t rank_budget_bytes) : spec_(std::move(spec)) {
+void DeviceSector::setup_tables() {
     require_device();
-    const auto t0 = std::chrono::steady_clock::now();
     const int N = spec_.n_sites, G = spec_.G;
     if (N < 1 || N > 63) throw Error(kInvalid, "sector: n_sites must be in [1, 63]");
     if (G < 1) throw Error(kInvalid, "sector: the group needs at least the identity");
@@ -100,6 +99,29 @@ DeviceSector::DeviceSector(GroupSpec spec, std::uint64_t rank_budget_bytes) : sp
         cand_off_ = upload(spec_.slc.cand_off.data(), spec_.slc.cand_off.size());
         cand_ = upload(spec_.slc.cand.data(), spec_.slc.cand.size());
     }
+}
+
+void DeviceSector::build_rank_table(std::uint64_t total, std::uint64_t rank_budget_bytes) {
+    const hipStream_t st = engine_stream();
+    const std::uint64_t table_bytes = total * 4;
+    if (dim_ > 0 && table_bytes <= rank_budget_bytes) {
+        rank_table_.reset(table_bytes);
+        QSA_HIP(hipMemsetAsync(rank_table_.get(), 0xFF, table_bytes, st));
+        dev::SectorView v2 = view();
+        v2.rank_table = nullptr;
+        dev::rank_fill(v2, rank_table_.get<dev::i32>(), st);
+        QSA_HIP(hipGetLastError());
+        stats_.rank_table = true;
+    }
+    QSA_HIP(hipStreamSynchronize(st));
+    stats_.device_bytes = lut_.bytes() + reps_.bytes() + inv_norms_.bytes() + rank_table_.bytes() + lead_.bytes() +
+                          cand_off_.bytes() + cand_.bytes();
+}
+
+DeviceSector::DeviceSector(GroupSpec spec, std::uint64_t rank_budget_bytes) : spec_(std::move(spec)) {
+    const auto t0 = std::chrono::steady_clock::now();
+    setup_tables();
+    const int N = spec_.n_sites;
 
     const std::uint64_t total = spec_.n_up >= 0 ? binom_host(N, spec_.n_up) : (1ULL << N);
     stats_.subspace_dim = total;
@@ -132,19 +154,52 @@ DeviceSector::DeviceSector(GroupSpec spec, std::uint64_t rank_budget_bytes) : sp
                         inv_norms_.get<double>(), st);
         QSA_HIP(hipGetLastError());
     }
-    const std::uint64_t table_bytes = total * 4;
-    if (dim_ > 0 && table_bytes <= rank_budget_bytes) {
-        rank_table_.reset(table_bytes);
-        QSA_HIP(hipMemsetAsync(rank_table_.get(), 0xFF, table_bytes, st));
-        dev::SectorView v2 = view();
-        v2.rank_table = nullptr;
-        dev::rank_fill(v2, rank_table_.get<dev::i32>(), st);
-        QSA_HIP(hipGetLastError());
-        stats_.rank_table = true;
+    build_rank_table(total, rank_budget_bytes);
+    stats_.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+DeviceSector::DeviceSector(GroupSpec spec, const std::uint64_t* reps, const double* inv_norms, std::int64_t n,
+                           std::uint64_t rank_budget_bytes)
+    : spec_(std::move(spec)) {
+    const auto t0 = std::chrono::steady_clock::now();
+    setup_tables();
+    const int N = spec_.n_sites;
+    if (n < 0 || (n > 0 && (!reps || !inv_norms))) throw Error(kInvalid, "sector import: no representatives");
+    if (n > 0x7FFFFFFFLL) throw Error(kDeviceUnsupported, "sector import: more than 2^31 representatives");
+    const std::uint64_t total = spec_.n_up >= 0 ? binom_host(N, spec_.n_up) : (1ULL << N);
+    const std::uint64_t mask = N == 64 ? ~0ULL : ((1ULL << N) - 1ULL);
+    for (std::int64_t i = 0; i < n; ++i) {
+        if ((reps[i] & ~mask) != 0 || (spec_.n_up >= 0 && __builtin_popcountll(reps[i]) != spec_.n_up))
+            throw Error(kInvalid, "sector import: representative " + std::to_string(i) + " is not a state of the subspace");
+        if (!(inv_norms[i] > 0.0)) throw Error(kInvalid, "sector import: 1/norm " + std::to_string(i) + " is not positive");
     }
-    QSA_HIP(hipStreamSynchronize(st));
-    stats_.device_bytes = lut_.bytes() + reps_.bytes() + inv_norms_.bytes() + rank_table_.bytes() + lead_.bytes() +
-                          cand_off_.bytes() + cand_.bytes();
+    // ascending states (the lookup's order; with the sublattice code engaged each representative is
+    // its orbit's least image in key order, but the table stays in state order)
+    for (std::int64_t i = 1; i < n; ++i)
+        if (!(reps[i - 1] < reps[i]))
+            throw Error(kInvalid, "sector import: representative " + std::to_string(i) +
+                                      " is out of order (the lookup needs them strictly ascending)");
+    stats_.subspace_dim = total;
+    stats_.orbits = -1;   // not scanned
+    dim_ = n;
+    reps_ = upload(reps, static_cast<std::size_t>(n));
+    inv_norms_ = upload(inv_norms, static_cast<std::size_t>(n));
+    build_rank_table(total, rank_budget_bytes);
+    // every representative must be its own orbit's representative and find itself: the order
+    // (plain or key order) and the canonical form are the ones this group's lookup uses
+    if (n > 0) {
+        const hipStream_t st = engine_stream();
+        DeviceBuffer idx(static_cast<std::size_t>(n) * 8), proj(static_cast<std::size_t>(n) * 16);
+        dev::index_proj_batch(view(), reps_.get<std::uint64_t>(), n, idx.get<dev::i64>(), proj.get<double>(), st);
+        QSA_HIP(hipGetLastError());
+        std::vector<std::int64_t> ih(static_cast<std::size_t>(n));
+        copy_to_host(ih.data(), idx.get(), ih.size() * 8);
+        for (std::int64_t i = 0; i < n; ++i)
+            if (ih[static_cast<std::size_t>(i)] != i)
+                throw Error(kInvalid, "sector import: representative " + std::to_string(i) +
+                                          " is not its orbit's representative in this group's order (it maps to " +
+                                          std::to_string(ih[static_cast<std::size_t>(i)]) + ")");
+    }
     stats_.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 

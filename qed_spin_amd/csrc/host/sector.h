@@ -24,6 +24,11 @@ public:
     /// Enumerate the sector of `spec` on the device. `rank_budget_bytes` caps the dense rank ->
     /// index table (C(N, n_up) x 4 bytes); above it lookups binary-search the representatives.
     DeviceSector(GroupSpec spec, std::uint64_t rank_budget_bytes);
+    /// Adopt representatives and 1/norms computed elsewhere (the reference's RepSectorData): the
+    /// group tables and the rank table are built here; every representative is checked to be a
+    /// state of the subspace and to find itself through the group's lookup (order and canonical form).
+    DeviceSector(GroupSpec spec, const std::uint64_t* reps, const double* inv_norms, std::int64_t n,
+                 std::uint64_t rank_budget_bytes);
 
     const GroupSpec& spec() const { return spec_; }
     std::int64_t dim() const { return dim_; }
@@ -36,6 +41,8 @@ public:
     std::vector<double> inv_norms() const;
 
 private:
+    void setup_tables();
+    void build_rank_table(std::uint64_t total, std::uint64_t rank_budget_bytes);
     GroupSpec spec_;
     std::int64_t dim_ = 0;
     SectorBuildStats stats_;

@@ -93,7 +93,9 @@ class Sector:
     reverse lookup (n_up < 0: the full 2^N space)."""
 
     def __init__(self, group: GroupSpec, n_up: int, sublattice: int = -1, relaxed: bool = True,
-                 rank_budget: int = DEFAULT_RANK_BUDGET):
+                 rank_budget: int = DEFAULT_RANK_BUDGET, reps=None, inv_norms=None):
+        """reps / inv_norms given: adopt them (the reference's RepSectorData) instead of enumerating;
+        each representative is checked to find itself through this group's lookup."""
         perms = np.ascontiguousarray(group.perms, dtype=np.int32)
         chars = np.ascontiguousarray(np.asarray(group.chars, np.complex128).view(np.float64))
         flips = None if group.flips is None else np.ascontiguousarray(group.flips, dtype=np.uint64)
@@ -102,19 +104,18 @@ class Sector:
         self.group = group
         self.n_up = int(n_up)
         h = C.c_void_p()
-        _lib.call(
-            "qedh_sector_create",
-            int(perms.shape[1]),
-            int(n_up),
-            int(perms.shape[0]),
-            perms.ctypes.data_as(C.POINTER(C.c_int)),
-            None if flips is None else flips.ctypes.data_as(C.POINTER(C.c_uint64)),
-            chars.ctypes.data_as(C.POINTER(C.c_double)),
-            int(sublattice),
-            int(bool(relaxed)),
-            C.c_uint64(int(rank_budget)),
-            C.byref(h),
-        )
+        head = (int(perms.shape[1]), int(n_up), int(perms.shape[0]), perms.ctypes.data_as(C.POINTER(C.c_int)),
+                None if flips is None else flips.ctypes.data_as(C.POINTER(C.c_uint64)),
+                chars.ctypes.data_as(C.POINTER(C.c_double)), int(sublattice), int(bool(relaxed)))
+        if reps is None:
+            _lib.call("qedh_sector_create", *head, C.c_uint64(int(rank_budget)), C.byref(h))
+        else:
+            r = np.ascontiguousarray(reps, np.uint64)
+            w = np.ascontiguousarray(inv_norms, np.float64)
+            if r.shape != w.shape or r.ndim != 1:
+                raise InvalidRequest("sector: reps and inv_norms must be 1-D of one length")
+            _lib.call("qedh_sector_import", *head, C.c_int64(r.size), _lib.ptr(r, C.c_uint64), _lib.ptr(w, C.c_double),
+                      C.c_uint64(int(rank_budget)), C.byref(h))
         self.h = h
         d, o, b, s = C.c_int64(), C.c_int64(), C.c_int(), C.c_double()
         _lib.call("qedh_sector_info", self.h, C.byref(d), C.byref(o), C.byref(b), C.byref(s))

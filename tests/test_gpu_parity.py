@@ -268,3 +268,52 @@ def test_dictionary_keeps_values_and_their_negatives_apart():
     w1 = np.sort(bits(np.asarray(val).view(np.float64)).reshape(-1, 2).view([("re", np.uint64), ("im", np.uint64)]), axis=0)
     w2 = np.sort(bits(np.asarray(v2).view(np.float64)).reshape(-1, 2).view([("re", np.uint64), ("im", np.uint64)]), axis=0)
     assert np.array_equal(w1, w2)
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_imported_reference_sector_gives_the_reference_csr(name):
+    """The reference's own representatives and 1/norms (its RepSectorData, written by its code into the
+    fixture) adopted as they are (qedh_sector_import): the CSR built on them is the reference's bit for
+    bit, and a representative out of order is refused by name."""
+    say("sector import", name)
+    fx = load_fixture(name)
+    g = qs.GroupSpec(perms=fx["perms"], chars=fx["chars"], flips=fx["flips"])
+    s = qs.Sector(g, int(fx["n_up"]), sublattice=int(fx["sublattice_mode"]), reps=fx["reps"], inv_norms=fx["inv_norms"])
+    assert s.dim == len(fx["reps"])
+    reps, inv = s.export()
+    assert np.array_equal(reps, fx["reps"]) and np.array_equal(bits(inv), bits(fx["inv_norms"]))
+    m = qs.SectorMatvec(fixture_operator(fx), s, lane=qs.LANE_CSR)
+    row_ptr, col, val = m.export_csr()
+    assert np.array_equal(row_ptr, fx["csr_row_ptr"]) and np.array_equal(col, fx["csr_col"])
+    ref = fx["csr_val"].view(np.complex128)
+    if m.is_real:
+        assert np.array_equal(bits(val.real), bits(ref.real))
+    else:
+        assert np.array_equal(bits(val.view(np.float64)), bits(ref.view(np.float64)))
+    if s.dim >= 3:
+        bad = np.array(fx["reps"], np.uint64).copy()
+        bad[[1, 2]] = bad[[2, 1]]
+        with pytest.raises(qs.errors.InvalidRequest):
+            qs.Sector(g, int(fx["n_up"]), sublattice=int(fx["sublattice_mode"]), reps=bad, inv_norms=fx["inv_norms"])
+
+
+def test_apply_on_host_buffers_equals_device_apply():
+    """qedh_matvec_apply_host (the reference's LinearOperator::apply on host buffers) gives the device
+    apply's output bit for bit, real and complex vectors."""
+    import ctypes as C
+
+    from qed_spin_amd import _lib
+
+    say("apply on host buffers")
+    fx = load_fixture(NAMES[0])
+    s = fixture_sector(fx)
+    m = qs.SectorMatvec(fixture_operator(fx), s, lane=qs.LANE_CSR)
+    rng = np.random.default_rng(5)
+    for cv in (False, True):
+        dt = m.dtype(cv)
+        x = rng.normal(size=m.dim) + (1j * rng.normal(size=m.dim) if dt == np.complex128 else 0.0)
+        x = np.ascontiguousarray(x, dt)
+        y = np.zeros(m.dim, dt)
+        _lib.call("qedh_matvec_apply_host", m.h, x.ctypes.data_as(C.c_void_p), y.ctypes.data_as(C.c_void_p), int(cv))
+        want = m.apply_host(x, complex_vectors=cv)
+        assert np.array_equal(bits(np.asarray(y).view(np.float64)), bits(np.asarray(want).view(np.float64)))
