@@ -114,6 +114,14 @@ int qedh_sector_key_bits(qedh_sector s, int* key_bit, int* block);
 /* index_and_projection of n host states on the device (rep_symmetry_basis_policy.h:221-241). */
 int qedh_sector_index_and_projection(qedh_sector s, const uint64_t* states, int64_t n, int64_t* index,
                                      double* proj);
+/* Sector vectors expanded into a basis without symmetry (ed::sectors::expand, src/engine/eigs.cpp:699-747,
+ * behind EigResult.vectors(basis=...), python/qed/_verbs/eigs.py:52-78): u is a DEVICE buffer of nvec
+ * vectors of dim complex entries; out a DEVICE buffer of nvec x out_dim complex entries, out_dim =
+ * C(N, n_up) (full = 0: the Sz sector, states in ascending order = colex rank) or 2^N (full != 0,
+ * N <= 34). Each vector is normalised to unit length; norms[nvec] (or NULL) gets the norms before.
+ * Refused: a sector without fixed n_up into the Sz basis, an image outside the Sz sector, a vector
+ * the expansion annihilates. */
+int qedh_sector_expand(qedh_sector s, const void* u, int nvec, int full, void* out, double* norms);
 int qedh_sector_destroy(qedh_sector s);
 
 /* ---- sector operators (RepSectorMatVec / CrossSectorMatVec device lanes,
@@ -191,14 +199,14 @@ int qedh_lanczos(qedh_matvec m, const void* v0_host, int complex_vectors, int ma
  * values[k]; *n_values found; *converged; *applies. */
 int qedh_block_lowest(qedh_matvec m, int k, int64_t max_iter, double* values, int* n_values, int* converged,
                       int64_t* applies);
-/* The lowest k eigenpairs of the block (Krylov-Schur with locking at 1e-10 s_H, the vectors of
- * solve_block_eigenpairs, src/engine/block_solve.cpp:600-655): values[k] ascending, vectors: a
- * DEVICE buffer of k x dim complex entries (vector i at vectors + i dim), normalised. */
 /* The pruning estimate of a block (estimate_lowest, src/engine/block_solve.cpp:630-655, as the eigs
  * walk prunes with it, src/engine/eigs.cpp:129-155, 450-560): 40 Lanczos steps without
  * reorthogonalisation from the fixed start seeded 0xE57A7E; *theta the lowest Ritz value (an upper
  * bound on the block's lowest level), *residual its residual bound; -inf when nothing ran. */
 int qedh_block_estimate(qedh_matvec m, double* theta, double* residual, int64_t* applies);
+/* The lowest k eigenpairs of the block (Krylov-Schur with locking at 1e-10 s_H, the vectors of
+ * solve_block_eigenpairs, src/engine/block_solve.cpp:600-655): values[k] ascending, vectors: a
+ * DEVICE buffer of k x dim complex entries (vector i at vectors + i dim), normalised. */
 int qedh_block_eigenpairs(qedh_matvec m, int k, int64_t max_iter, double* values, void* vectors, int* n_values,
                           int* converged);
 /* FTLM of the block: samples from splitmix64(seed + phi (s + 1)); lnZ/E/V per beta; *e_min. */

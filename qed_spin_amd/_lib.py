@@ -80,6 +80,7 @@ _SIGNATURES = {
     "qedh_sector_export": [C.c_void_p, c_u64_p, c_dbl_p],
     "qedh_sector_key_bits": [C.c_void_p, c_int_p, c_int_p],
     "qedh_sector_index_and_projection": [C.c_void_p, c_u64_p, C.c_int64, c_i64_p, c_dbl_p],
+    "qedh_sector_expand": [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, c_dbl_p],
     "qedh_sector_destroy": [C.c_void_p],
     "qedh_matvec_create": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.c_uint64, C.POINTER(C.c_void_p)],
     "qedh_matvec_info": [C.c_void_p, C.POINTER(MatvecInfo)],

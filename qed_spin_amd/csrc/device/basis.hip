@@ -182,6 +182,32 @@ __global__ void k_rank_fill(SectorView S, i32* table) {
     table[r] = static_cast<i32>(i);
 }
 
+// A sector vector in the Sz sector (full: the 2^N space), before normalisation
+// (src/engine/eigs.cpp:699-740): psi[index(g r_a)] += (u_a inv_norm_a) conj(chi(g)), g ascending.
+// One thread per representative; orbits are disjoint, so each entry of psi has a single writer and
+// the stabiliser's repeated images add up in element order. An image outside the Sz sector (a flip
+// that changes the magnetisation) is not written; it raises *bad.
+__global__ __launch_bounds__(256) void k_expand(SectorView S, const Zd* u, int full, Zd* psi, unsigned* bad) {
+    const i64 a = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (a >= S.dim) return;
+    const Zd ua = u[a];
+    if (ua.re == 0.0 && ua.im == 0.0) return;
+    const double w = S.inv_norms[a];
+    const double yr = ua.re * w, yi = ua.im * w;
+    const u64 r = S.reps[a];
+    for (int g = 0; g < S.G; ++g) {
+        const u64 t = apply_element(S, g, r);
+        if (!full && popc(t) != S.n_up) {
+            *bad = 1u;
+            continue;
+        }
+        const i64 at = full ? static_cast<i64>(t) : rank_of(S, t);
+        const double cr = S.chi_re[g], ci = -S.chi_im[g];
+        psi[at].re += yr * cr - yi * ci;
+        psi[at].im += yr * ci + yi * cr;
+    }
+}
+
 inline unsigned blocks_for(u64 n, unsigned t) { return static_cast<unsigned>((n + t - 1) / t); }
 
 // Up to these many states of the subspace the orbit scan (flags) and the emission take one lane per
@@ -242,6 +268,12 @@ void orbit_emit(const SectorView& S, u64 total, const u64* kept, const u64* offs
 void rank_fill(const SectorView& S, i32* table, hipStream_t st) {
     if (S.dim == 0) return;
     hipLaunchKernelGGL(k_rank_fill, dim3(blocks_for(static_cast<u64>(S.dim), 256)), dim3(256), 0, st, S, table);
+}
+
+void sector_expand(const SectorView& S, const Zd* u, bool full, Zd* psi, unsigned* bad, hipStream_t st) {
+    if (S.dim <= 0) return;
+    hipLaunchKernelGGL(k_expand, dim3(blocks_for(static_cast<u64>(S.dim), 256)), dim3(256), 0, st, S, u,
+                       full ? 1 : 0, psi, bad);
 }
 
 }  // namespace qsa::dev

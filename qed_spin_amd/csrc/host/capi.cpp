@@ -609,6 +609,55 @@ extern "C" int qedh_sector_index_and_projection(qedh_sector s, const uint64_t* s
     });
 }
 
+extern "C" int qedh_sector_expand(qedh_sector s, const void* u, int nvec, int full, void* out, double* norms) {
+    return guarded([&] {
+        need(s, "sector");
+        need(u, "u");
+        need(out, "out");
+        if (nvec < 1) throw qsa::Error(qsa::kInvalid, "expand: nvec must be >= 1");
+        const qsa::dev::SectorView S = s->sector->view();
+        const int N = S.n_sites;
+        if (full && N > 34)
+            throw qsa::Error(qsa::kResourceLimit, "expand: the full 2^N space is limited to N <= 34 (use the Sz basis)");
+        if (!full && S.n_up < 0) throw qsa::Error(qsa::kInvalid, "expand: a sector without fixed n_up expands only into the full space");
+        std::uint64_t out_dim = 1;
+        if (full) {
+            out_dim = std::uint64_t{1} << N;
+        } else {
+            for (int i = 1; i <= S.n_up; ++i)
+                out_dim = out_dim * static_cast<std::uint64_t>(N - S.n_up + i) / static_cast<std::uint64_t>(i);
+        }
+        const hipStream_t st = qsa::engine_stream();
+        qsa::DeviceBuffer bad(sizeof(unsigned)), scratch(qsa::dev::blas_scratch_bytes(1)),
+            nrm(static_cast<std::size_t>(nvec) * sizeof(double));
+        QSA_HIP(hipMemsetAsync(bad.get(), 0, sizeof(unsigned), st));
+        const auto* uin = static_cast<const qsa::dev::Zd*>(u);
+        auto* psi = static_cast<qsa::dev::Zd*>(out);
+        for (int v = 0; v < nvec; ++v) {
+            qsa::dev::Zd* p = psi + static_cast<std::size_t>(v) * out_dim;
+            qsa::dev::fill_zero(true, p, out_dim, st);
+            qsa::dev::sector_expand(S, uin + static_cast<std::size_t>(v) * static_cast<std::size_t>(S.dim), full != 0,
+                                    p, bad.get<unsigned>(), st);
+            qsa::dev::nrm2(true, p, out_dim, scratch.get(), nrm.get<double>() + v, st);
+        }
+        QSA_HIP(hipGetLastError());
+        unsigned bad_h = 0;
+        std::vector<double> nh(static_cast<std::size_t>(nvec));
+        qsa::copy_to_host(&bad_h, bad.get(), sizeof(unsigned));
+        qsa::copy_to_host(nh.data(), nrm.get(), nh.size() * sizeof(double));
+        if (bad_h) throw qsa::Error(qsa::kInvalid, "expand: the sector is not inside the Sz sector n_up");
+        for (int v = 0; v < nvec; ++v) {
+            if (!(nh[static_cast<std::size_t>(v)] > 0.0))
+                throw qsa::Error(qsa::kInvalid, "expand: the expansion annihilated the vector");
+            qsa::dev::scale(true, qsa::dev::Zd{1.0 / nh[static_cast<std::size_t>(v)], 0.0},
+                            psi + static_cast<std::size_t>(v) * out_dim, out_dim, st);
+            if (norms) norms[v] = nh[static_cast<std::size_t>(v)];
+        }
+        QSA_HIP(hipGetLastError());
+        QSA_HIP(hipStreamSynchronize(st));
+    });
+}
+
 namespace qsa {
 void capi_set_error(const std::string& m) { g_error = m; }
 int capi_status(const std::exception& e) {

@@ -30,9 +30,10 @@ from typing import Dict, List, Optional
 import numpy as np
 
 from . import _lib
+from . import levels as lv
 from .errors import InvalidRequest
 from .operator import Operator
-from .sector import DEFAULT_CSR_BUDGET, LANE_GATHER, Sector, SectorMatvec, parallel_build
+from .sector import DEFAULT_CSR_BUDGET, LANE_GATHER, DeviceArray, Sector, SectorMatvec, parallel_build
 from .symmetry import Symmetry
 from .thermal import _handles, resolve_device, walk_blocks
 
@@ -55,10 +56,104 @@ def dense_floor(k: int, dense_max_dim: Optional[int]) -> int:
 
 @dataclass
 class EigsResult:
+    """The k lowest levels (energies, ascending) and the walk's blocks. `levels`: one dict per
+    returned level (energy, n_up, momentum, parity, dim, label, the index of its vector or -1).
+    With vectors=True each level's eigenvector is kept in its sector basis: vectors() expands them,
+    expect() and correlations() evaluate observables in them, save() writes the result to an .npz
+    file that load_eigs() restores."""
+
     energies: np.ndarray
     blocks: List[dict] = field(default_factory=list)
     converged: bool = True
     timings: Dict[str, float] = field(default_factory=dict)
+    k: int = 0
+    n_sites: int = 0
+    levels: List[dict] = field(default_factory=list)
+    _bases: Dict[int, "lv.LevelBasis"] = field(default_factory=dict, repr=False)
+    _vectors: List[tuple] = field(default_factory=list, repr=False)   # (basis key, sector vector)
+    _cache: Dict = field(default_factory=dict, repr=False, compare=False)
+
+    def _need_vectors(self):
+        if not any(L.get("vector", -1) >= 0 for L in self.levels):
+            raise InvalidRequest("no vectors: call eigs(..., vectors=True)")
+
+    def vectors(self, basis: str = "full", n_up: Optional[int] = None) -> List[np.ndarray]:
+        """Unit eigenvectors of the returned levels, in level order: basis="full" the 2^N
+        computational basis (bit i of a state is site i, set = up), basis="sz" the Sz sector n_up
+        (its states in ascending order); levels of another n_up have no component there and are
+        left out (the reference's EigResult.vectors, python/qed/_verbs/eigs.py:52-78). Vectors of
+        one degenerate level come from one block's eigensolver and are orthonormal; vectors of
+        different blocks are orthogonal by their quantum numbers."""
+        self._need_vectors()
+        if basis not in ("full", "sz"):
+            raise InvalidRequest("basis must be 'full' or 'sz'")
+        if basis == "sz" and n_up is None:
+            raise InvalidRequest("basis='sz' needs n_up")
+        out = []
+        for L in self.levels:
+            if L.get("vector", -1) < 0:
+                continue
+            key, v = self._vectors[L["vector"]]
+            B = self._bases[key]
+            if basis == "sz" and B.n_up != int(n_up):
+                continue
+            out.extend(lv.expand(B, [v], full=(basis == "full")))
+        return out
+
+    def expect(self, ops) -> np.ndarray:
+        """<v|O|v> of each level's vector v for every O: a complex array [levels, ops]."""
+        self._need_vectors()
+        obs = [ops] if isinstance(ops, Operator) else list(ops)
+        out = np.zeros((len(self.levels), len(obs)), np.complex128)
+        for i, L in enumerate(self.levels):
+            if L.get("vector", -1) < 0:
+                out[i] = np.nan
+                continue
+            key, v = self._vectors[L["vector"]]
+            out[i] = lv.expect_in(self._bases[key], v, obs, self._cache.setdefault(("expect", key), {}))
+        return out
+
+    def correlations(self, A, B=None) -> np.ndarray:
+        """<A_a^dag B_b> in each level's vector for every pair (B=None: B = A): a complex array
+        [levels, len(A), len(B)] (the reference's EigResult.correlations,
+        python/qed/_verbs/eigs.py:91-98, per returned vector)."""
+        self._need_vectors()
+        As = [A] if isinstance(A, Operator) else list(A)
+        Bs = As if B is None else ([B] if isinstance(B, Operator) else list(B))
+        out = np.zeros((len(self.levels), len(As), len(Bs)), np.complex128)
+        for i, L in enumerate(self.levels):
+            if L.get("vector", -1) < 0:
+                out[i] = np.nan
+                continue
+            key, v = self._vectors[L["vector"]]
+            out[i] = lv.correlations_in(self._bases[key], v, As, Bs, self._cache.setdefault(("corr", key), {}))
+        return out
+
+    def save(self, path) -> None:
+        """Write the result to an .npz file (format_version 2: the levels, each basis once, the
+        vectors in their sector basis); load_eigs() restores it and vectors(), expect() and
+        correlations() then work without H."""
+        np.savez_compressed(path, **lv.to_arrays(self))
+
+
+def load_eigs(path) -> EigsResult:
+    """An EigsResult written by EigsResult.save (python/qed/_verbs/eigs.py:191-215): format 1 (the
+    old spin convention) and files that are not EigResult files are refused; the bases are imported
+    onto the device when a vector is first used."""
+    with np.load(path, allow_pickle=False) as f:
+        d = {key: f[key] for key in f.files}
+    version = int(d["format_version"]) if "format_version" in d else 0
+    if version == 1:
+        raise InvalidRequest(f"{path}: an EigResult file of format 1, written when a set bit meant spin down; "
+                             "its states and n_up labels mean the opposite now -- recompute it")
+    if version != lv.FORMAT_VERSION:
+        raise InvalidRequest(f"{path}: not an EigResult file (format_version 2)")
+    levels, bases, vectors, n_sites, rs = lv.from_arrays(d)
+    energies = np.asarray(d["energies"], np.float64).reshape(-1) if "energies" in d else np.zeros(0)
+    k = int(d["k"]) if "k" in d else len(energies)
+    return EigsResult(energies=energies, blocks=[], converged=bool(rs[2]) if rs.size > 2 else True,
+                      timings={"pruned_blocks": int(rs[6]) if rs.size > 6 else 0}, k=k, n_sites=n_sites,
+                      levels=levels, _bases={i: b for i, b in enumerate(bases)}, _vectors=vectors)
 
 
 PRUNE_MIN_DIM = 64   # blocks above this are estimated before they are solved (eigs.cpp:129-132)
@@ -119,12 +214,35 @@ def walk_levels(blocks, k: int, *, floor: int, prune: bool = True, prune_margin:
     return levels, pruned
 
 
+def _level_vectors(blocks, rows, max_iter: int):
+    """Eigenvectors of the selected levels: per block, as many Krylov-Schur eigenpairs on the device
+    as the block has levels among the rows (ascending, so the t-th row of a block takes its t-th
+    vector). {block: [sector vectors]}, {block: LevelBasis}"""
+    from collections import Counter
+
+    got, bases = {}, {}
+    for i, c in sorted(Counter(i for _, i in rows).items()):
+        blk, sec, m = blocks[i]
+        buf = DeviceArray(c * sec.dim)
+        vals = np.zeros(c)
+        n, conv = C.c_int(), C.c_int()
+        _lib.call("qedh_block_eigenpairs", m.h, int(c), C.c_int64(int(max_iter)), _lib.ptr(vals, C.c_double),
+                  buf.ptr, C.byref(n), C.byref(conv))
+        if n.value < c:
+            raise RuntimeError(f"eigs: block {blk.label} gave {n.value} eigenpairs of the {c} asked")
+        host = buf.to_host()
+        got[i] = [host[j * sec.dim:(j + 1) * sec.dim].copy() for j in range(c)]
+        bases[i] = lv.LevelBasis.of(blk.group, sec)
+    return got, bases
+
+
 def eigs(H: Operator, k: int = 4, *, sym: Optional[Symmetry] = None, device: str = "gpu",
          dense_max_dim: Optional[int] = None, max_iter: int = 0, prune: bool = True, prune_margin: float = 0.02,
-         window: float = 0.0, csr_budget: int = DEFAULT_CSR_BUDGET) -> EigsResult:
+         window: float = 0.0, vectors: bool = False, csr_budget: int = DEFAULT_CSR_BUDGET) -> EigsResult:
     """The k lowest levels of H over the blocks of `sym` (default: every (Sz, k) block of a ring).
     prune (the reference's default): blocks whose 40-step estimate lies far above the k-th level
-    found so far are not solved."""
+    found so far are not solved. vectors: also keep each returned level's eigenvector (Krylov-Schur
+    eigenpairs of its block on the device, in the block's sector basis)."""
     resolve_device(device, "eigs")
     if int(k) < 1:
         raise InvalidRequest("eigs: k must be >= 1")
@@ -140,7 +258,7 @@ def eigs(H: Operator, k: int = 4, *, sym: Optional[Symmetry] = None, device: str
                                  prune_margin=prune_margin, window=window, norm_bound=H.norm_bound(),
                                  max_iter=max_iter)
     t_solve = time.perf_counter() - t1
-    recs, allv, converged = [], [], True
+    recs, converged = [], True
     for i, (blk, sec, _) in enumerate(blocks):
         if i not in levels:
             recs.append({"label": blk.label, "dim": sec.dim, "lane": "pruned", "levels": np.zeros(0), "converged": True})
@@ -148,11 +266,26 @@ def eigs(H: Operator, k: int = 4, *, sym: Optional[Symmetry] = None, device: str
         v, conv, lane = levels[i]
         converged = converged and conv
         recs.append({"label": blk.label, "dim": sec.dim, "lane": lane, "levels": v, "converged": conv})
-        allv.extend(v.tolist())
-    energies = np.sort(np.asarray(allv))[:k]
+    rows = sorted((float(e), i) for i, (v, _, _) in levels.items() for e in v)[:k]
+    energies = np.array([e for e, _ in rows])
+    t2 = time.perf_counter()
+    got, bases = _level_vectors(blocks, rows, max_iter) if vectors else ({}, {})
+    lvls, vecs, used = [], [], {}
+    for e, i in rows:
+        blk, sec, _ = blocks[i]
+        L = {"energy": e, "n_up": blk.n_up, "momentum": blk.momentum, "parity": blk.parity, "dim": sec.dim,
+             "label": blk.label, "vector": -1}
+        if vectors:
+            t = used.get(i, 0)
+            used[i] = t + 1
+            L["vector"] = len(vecs)
+            vecs.append((i, got[i][t]))
+        lvls.append(L)
     return EigsResult(energies=energies, blocks=recs, converged=converged,
                       timings={"build_s": t_build, "solve_s": t_solve, "pruned_blocks": pruned,
-                               "total_s": time.perf_counter() - t0})
+                               "vectors_s": time.perf_counter() - t2 if vectors else 0.0,
+                               "total_s": time.perf_counter() - t0},
+                      k=k, n_sites=H.num_sites, levels=lvls, _bases=bases, _vectors=vecs)
 
 
 @dataclass
