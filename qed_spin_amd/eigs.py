@@ -236,18 +236,71 @@ def _level_vectors(blocks, rows, max_iter: int):
     return got, bases
 
 
+def spin_squared(N: int) -> Operator:
+    """S^2 = 3N/4 + sum_{i<j} (2 S^z_i S^z_j + S^+_i S^-_j + S^-_i S^+_j)."""
+    P = Operator(N)
+    P.add_product("I", (0,), 0.75 * N)
+    for i in range(N):
+        for j in range(i + 1, N):
+            P.add_product("zz", (i, j), 2.0)
+            P.add_product("+-", (i, j), 1.0)
+            P.add_product("-+", (i, j), 1.0)
+    return P
+
+
+def two_spin(total_spin, N: int) -> int:
+    """2S of a total_spin request: "min" is 0 for even N and 1 for odd N."""
+    if isinstance(total_spin, str):
+        if total_spin != "min":
+            raise InvalidRequest(f"total_spin must be a number or 'min', got {total_spin!r}")
+        return N % 2
+    two_S = int(round(2.0 * float(total_spin)))
+    if abs(2.0 * float(total_spin) - two_S) > 1e-12 or two_S < 0 or two_S > N or (N + two_S) % 2:
+        raise InvalidRequest(f"total_spin {total_spin} is not a spin of {N} spins 1/2")
+    return two_S
+
+
+def spin_tower_operator(H: Operator, two_S: int) -> Operator:
+    """H + mu (S^2 - S(S+1)) with mu = (s_H + 1) / (S + 1): H commutes with S^2, so every level of
+    spin S keeps its energy, and in the Sz = S sector (where every state has spin >= S) each level of
+    spin S' > S rises by mu (S'(S'+1) - S(S+1)) >= 2 (s_H + 1), above every level of spin S (the
+    reference's penalty operator, src/engine/tower.cpp)."""
+    N = H.num_sites
+    S = 0.5 * two_S
+    mu = (H.norm_bound() + 1.0) / (S + 1.0)
+    lam = S * (S + 1.0)
+    out = Operator(N, products=list(H.products))
+    for ops, sites, c in spin_squared(N).products:
+        out.add_product(ops, sites, mu * c)
+    out.add_product("I", (0,), -mu * lam)
+    return out
+
+
 def eigs(H: Operator, k: int = 4, *, sym: Optional[Symmetry] = None, device: str = "gpu",
          dense_max_dim: Optional[int] = None, max_iter: int = 0, prune: bool = True, prune_margin: float = 0.02,
-         window: float = 0.0, vectors: bool = False, csr_budget: int = DEFAULT_CSR_BUDGET) -> EigsResult:
+         window: float = 0.0, vectors: bool = False, total_spin=None,
+         csr_budget: int = DEFAULT_CSR_BUDGET) -> EigsResult:
     """The k lowest levels of H over the blocks of `sym` (default: every (Sz, k) block of a ring).
     prune (the reference's default): blocks whose 40-step estimate lies far above the k-th level
     found so far are not solved. vectors: also keep each returned level's eigenvector (Krylov-Schur
-    eigenpairs of its block on the device, in the block's sector basis)."""
+    eigenpairs of its block on the device, in the block's sector basis). total_spin (a spin S or
+    "min", H SU(2) symmetric): the k lowest levels of total spin S, each multiplet once, solved in
+    the Sz = S sector of `sym`'s spatial blocks under the penalty operator (spin_tower_operator)."""
     resolve_device(device, "eigs")
     if int(k) < 1:
         raise InvalidRequest("eigs: k must be >= 1")
     k = int(k)
     sym = sym if sym is not None else Symmetry()
+    two_S = None
+    if total_spin is not None:
+        N = H.num_sites
+        two_S = two_spin(total_spin, N)
+        n_up = (N + two_S) // 2
+        if sym.sz == "off" or (sym.sz != "all" and [int(x) for x in np.atleast_1d(sym.sz)] != [n_up]):
+            raise InvalidRequest(f"eigs: total_spin {two_S / 2} is solved in the Sz sector n_up = {n_up}; "
+                                 f"the symmetry asks for sz={sym.sz!r}")
+        sym = Symmetry(sz=[n_up], momenta=sym.momenta, translations=sym.translations, reflection=sym.reflection)
+        H = spin_tower_operator(H, two_S)
     t0 = time.perf_counter()
     blocks = walk_blocks(H, sym, csr_budget)
     if not blocks:
@@ -275,6 +328,8 @@ def eigs(H: Operator, k: int = 4, *, sym: Optional[Symmetry] = None, device: str
         blk, sec, _ = blocks[i]
         L = {"energy": e, "n_up": blk.n_up, "momentum": blk.momentum, "parity": blk.parity, "dim": sec.dim,
              "label": blk.label, "vector": -1}
+        if two_S is not None:
+            L["total_spin"] = two_S / 2
         if vectors:
             t = used.get(i, 0)
             used[i] = t + 1

@@ -44,6 +44,8 @@ KEYS = [
     "j1j2_chain12/api/spectrum/sz=6",
     "j1j2_chain12/api/spectrum/auto/sz=6",
     "j1j2_chain12/api/expect/auto/S0.S1/k=4",
+    "j1j2_chain12/api/eigs/total_spin=min/k=4",
+    "j1j2_chain12/api/eigs/auto/vectors/k=3",
 ]
 
 

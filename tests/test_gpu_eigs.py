@@ -110,3 +110,36 @@ def test_expect_reproduces_reference_record():
         hit = [i for i in range(4) if i not in used and abs(r.energies[i] - e) < 1e-9 and abs(r.values[i, 0] - b) < 1e-8]
         assert hit, (e, b, r.energies, r.values[:, 0])
         used.add(hit[0])
+
+
+@pytest.mark.parametrize("dense_max_dim", [None, 0])
+@pytest.mark.parametrize("sym", ["spatial=none", "auto"])
+def test_total_spin_min_reproduces_the_record(sym, dense_max_dim):
+    """eigs(total_spin="min"): the 4 lowest singlets of j1j2_chain12 (the reference records them over
+    the Sz = 0 sector without spatial symmetry) within 1e-10 -- in that sector and over its momentum
+    blocks, by the dense lanes and by Krylov-Schur on the penalised operator."""
+    say("total_spin=min", sym, "dense_max_dim", dense_max_dim)
+    want = _records()["j1j2_chain12/api/eigs/total_spin=min/k=4"]["values"]["energies"]
+    s = qs.Symmetry(translations=False) if sym == "spatial=none" else qs.Symmetry()
+    r = ev.eigs(_h12(), 4, sym=s, total_spin="min", dense_max_dim=dense_max_dim)
+    assert r.converged
+    assert np.allclose(r.energies, want, rtol=1e-10, atol=1e-10), (r.energies, want)
+    assert all(L["n_up"] == 6 and L["total_spin"] == 0.0 for L in r.levels)
+    if dense_max_dim == 0:
+        assert "device-krylov" in {b["lane"] for b in r.blocks}
+
+
+def test_vectors_record_count_and_rayleigh():
+    """eigs(k = 3, vectors=True).vectors() over every block: the reference records 3 vectors for
+    j1j2_chain12; each is a unit eigenvector whose Rayleigh quotient is its level's energy."""
+    say("vectors record")
+    want = _records()["j1j2_chain12/api/eigs/auto/vectors/k=3"]["values"]["count"]
+    H = _h12()
+    r = ev.eigs(H, 3, vectors=True)
+    assert len(r.vectors()) == want
+    vs = [v for n in sorted({L["n_up"] for L in r.levels}) for v in r.vectors("sz", n_up=n)]
+    assert len(vs) == want
+    for v in vs:
+        assert abs(np.linalg.norm(v) - 1.0) < 1e-12
+    ex = r.expect([H])
+    assert np.allclose(ex[:, 0].real, r.energies, atol=1e-10)

@@ -216,20 +216,26 @@ def test_every_level_of_a_small_ring(tmp_path):
 
 
 def test_sublattice_bases_survive_the_file(tmp_path):
-    """A dihedral sector large enough for the sublattice key order: its fingerprint is written and
-    checked again on load; the loaded vectors equal the originals."""
+    """A sector whose representatives are least in the sublattice key order (the dihedral group of
+    the N = 16 ring, sublattice code forced on): its fingerprint is written and checked again on
+    load; the loaded basis expands a vector exactly as the original; a wrong fingerprint is refused."""
     say("sublattice file")
     N = 16
-    H = _ring(N, 0.3)
-    r = ev.eigs(H, 2, sym=qs.Symmetry(sz=[N // 2], momenta=[0], reflection=1), vectors=True)
-    B = next(iter(r._bases.values()))
-    assert B.sublattice != 0      # 32 elements: the sublattice key order is engaged
+    g = sym.dihedral_group(N, 0, 1)
+    sec = qs.Sector(g, N // 2, sublattice=1)
+    B = lv.LevelBasis.of(g, sec)
+    assert B.sublattice != 0
+    rng = np.random.default_rng(5)
+    u = rng.normal(size=sec.dim) + 1j * rng.normal(size=sec.dim)
+    r = ev.EigsResult(energies=np.array([0.0]), k=1, n_sites=N, _bases={0: B}, _vectors=[(0, u / np.linalg.norm(u))],
+                      levels=[{"energy": 0.0, "n_up": N // 2, "momentum": 0, "parity": 1, "dim": sec.dim,
+                               "label": "n_up=8,k=0,R=+1", "vector": 0}])
     p = tmp_path / "slc.npz"
     r.save(p)
     s = ev.load_eigs(p)
     a, b = r.vectors("sz", n_up=N // 2), s.vectors("sz", n_up=N // 2)
-    for x, y in zip(a, b):
-        assert np.abs(x - y).max() < 1e-14
+    assert len(a) == len(b) == 1
+    assert np.abs(a[0] - b[0]).max() < 1e-14
     with np.load(p) as f:
         d = {k: f[k] for k in f.files}
     d["basis0_sublattice"] = np.array([B.sublattice ^ 1], np.uint64)
@@ -237,3 +243,32 @@ def test_sublattice_bases_survive_the_file(tmp_path):
     np.savez(bad, **d)
     with pytest.raises(errors.InvalidRequest):
         ev.load_eigs(bad).vectors("full")
+
+
+@pytest.mark.parametrize("total_spin", [0.5, 1.5])
+def test_total_spin_against_the_dense_spin_subspace(total_spin):
+    """N = 9 J1-J2 ring: the 4 lowest levels of spin S against eigvalsh of H restricted to the
+    S(S+1) eigenspace of the dense S^2 in the Sz = S sector (1e-10); their vectors carry
+    <S^2> = S(S+1) (1e-9)."""
+    say("total_spin dense", total_spin)
+    N = 9
+    H = _ring(N, 0.3)
+    two_S = int(2 * total_spin)
+    n_up = (N + two_S) // 2
+    states = [x for x in range(1 << N) if bin(x).count("1") == n_up]
+    S2 = ev.spin_squared(N)
+    Hd = po.dense_matrix(H.products, N, states)
+    S2d = po.dense_matrix(S2.products, N, states)
+    w, Q = np.linalg.eigh(S2d)
+    lam = total_spin * (total_spin + 1)
+    Q = Q[:, np.abs(w - lam) < 1e-8]
+    want = np.linalg.eigvalsh(Q.conj().T @ Hd @ Q)[:4]
+    for dmd in (None, 0):
+        r = ev.eigs(H, 4, total_spin=total_spin, vectors=True, dense_max_dim=dmd)
+        assert np.allclose(r.energies, want, atol=1e-10), (dmd, r.energies, want)
+        s2 = r.expect([S2])[:, 0].real
+        assert np.allclose(s2, lam, atol=1e-9), s2
+    with pytest.raises(errors.InvalidRequest):
+        ev.eigs(H, 2, total_spin=1.0)            # not a spin of 9 spins 1/2
+    with pytest.raises(errors.InvalidRequest):
+        ev.eigs(H, 2, total_spin=0.5, sym=qs.Symmetry(sz=[4]))

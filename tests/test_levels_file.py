@@ -137,3 +137,44 @@ def test_a_result_without_vectors_says_so():
                  lambda: r.correlations(Operator(4).add_product("z", (0,), 1.0))):
         with pytest.raises(errors.InvalidRequest):
             call()
+
+
+def test_spin_squared_is_the_casimir():
+    """S^2 of three spins 1/2: eigenvalues 3/4 (two doublets) and 15/4 (the quartet), four each;
+    of two: 0 (singlet) once and 2 (triplet) three times."""
+    w3 = np.linalg.eigvalsh(po.dense_matrix(ev.spin_squared(3).products, 3))
+    assert np.allclose(w3, [0.75] * 4 + [3.75] * 4)
+    w2 = np.linalg.eigvalsh(po.dense_matrix(ev.spin_squared(2).products, 2))
+    assert np.allclose(w2, [0.0, 2.0, 2.0, 2.0])
+
+
+def test_total_spin_requests_are_checked():
+    assert ev.two_spin("min", 12) == 0 and ev.two_spin("min", 9) == 1
+    assert ev.two_spin(1.5, 9) == 3 and ev.two_spin(2, 12) == 4
+    for bad, N in ((1.0, 9), (0.5, 12), (0.3, 12), (7, 12), (-1, 12), ("max", 12)):
+        with pytest.raises(errors.InvalidRequest):
+            ev.two_spin(bad, N)
+
+
+def test_total_spin_record_is_the_lowest_singlets():
+    """The reference's j1j2_chain12 total_spin=min record (Sz = 0 sector, no spatial symmetry) is the
+    4 lowest eigenvalues of H on the S^2 = 0 eigenspace of the dense Sz = 0 block, each singlet once."""
+    import json
+    import os
+
+    from conftest import FIXTURES
+
+    with open(os.path.join(FIXTURES, "reference_records.json")) as f:
+        want = json.load(f)["records"]["j1j2_chain12/api/eigs/total_spin=min/k=4"]["values"]["energies"]
+    N = 12
+    H = Operator(N)
+    for d, J in ((1, 1.0), (2, 0.5)):
+        for i in range(N):
+            j = (i + d) % N
+            H.add_product("zz", (i, j), J).add_product("+-", (i, j), 0.5 * J).add_product("-+", (i, j), 0.5 * J)
+    states = [x for x in range(1 << N) if bin(x).count("1") == N // 2]
+    Hd = po.dense_matrix(H.products, N, states)
+    w, Q = np.linalg.eigh(po.dense_matrix(ev.spin_squared(N).products, N, states))
+    Q = Q[:, np.abs(w) < 1e-8]
+    got = np.linalg.eigvalsh(Q.conj().T @ Hd @ Q)[:4]
+    assert np.allclose(got, want, atol=1e-10)
