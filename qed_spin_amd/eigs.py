@@ -284,8 +284,9 @@ def eigs(H: Operator, k: int = 4, *, sym: Optional[Symmetry] = None, device: str
     prune (the reference's default): blocks whose 40-step estimate lies far above the k-th level
     found so far are not solved. vectors: also keep each returned level's eigenvector (Krylov-Schur
     eigenpairs of its block on the device, in the block's sector basis). total_spin (a spin S or
-    "min", H SU(2) symmetric): the k lowest levels of total spin S, each multiplet once, solved in
-    the Sz = S sector of `sym`'s spatial blocks under the penalty operator (spin_tower_operator)."""
+    "min", H SU(2) symmetric): the k lowest energies of total spin S, each multiplet repeated 2S + 1
+    times as the reference counts it, solved once in the Sz = S sector of `sym`'s spatial blocks
+    under the penalty operator (spin_tower_operator); a level's vector is its Sz = S member."""
     resolve_device(device, "eigs")
     if int(k) < 1:
         raise InvalidRequest("eigs: k must be >= 1")
@@ -300,7 +301,12 @@ def eigs(H: Operator, k: int = 4, *, sym: Optional[Symmetry] = None, device: str
             raise InvalidRequest(f"eigs: total_spin {two_S / 2} is solved in the Sz sector n_up = {n_up}; "
                                  f"the symmetry asks for sz={sym.sz!r}")
         sym = Symmetry(sz=[n_up], momenta=sym.momenta, translations=sym.translations, reflection=sym.reflection)
+        # spin-S levels lie at or below s_H, every penalised one at or above s_H + 2
+        spin_cut = H.norm_bound() + 1.0
         H = spin_tower_operator(H, two_S)
+    k_out = k
+    if two_S:   # a multiplet counts 2S + 1 in the energies; each is solved once
+        k = -(-k // (two_S + 1))
     t0 = time.perf_counter()
     blocks = walk_blocks(H, sym, csr_budget)
     if not blocks:
@@ -319,8 +325,12 @@ def eigs(H: Operator, k: int = 4, *, sym: Optional[Symmetry] = None, device: str
         v, conv, lane = levels[i]
         converged = converged and conv
         recs.append({"label": blk.label, "dim": sec.dim, "lane": lane, "levels": v, "converged": conv})
-    rows = sorted((float(e), i) for i, (v, _, _) in levels.items() for e in v)[:k]
-    energies = np.array([e for e, _ in rows])
+    rows = sorted((float(e), i) for i, (v, _, _) in levels.items() for e in v)
+    if two_S is not None:   # a tower with fewer than k multiplets: the penalised levels are not returned
+        rows = [r for r in rows if r[0] < spin_cut]
+    rows = rows[:k]
+    mult = 1 if two_S is None else two_S + 1
+    energies = np.repeat(np.array([e for e, _ in rows]), mult)[:k_out]
     t2 = time.perf_counter()
     got, bases = _level_vectors(blocks, rows, max_iter) if vectors else ({}, {})
     lvls, vecs, used = [], [], {}
@@ -330,6 +340,7 @@ def eigs(H: Operator, k: int = 4, *, sym: Optional[Symmetry] = None, device: str
              "label": blk.label, "vector": -1}
         if two_S is not None:
             L["total_spin"] = two_S / 2
+            L["multiplicity"] = mult
         if vectors:
             t = used.get(i, 0)
             used[i] = t + 1
@@ -340,7 +351,7 @@ def eigs(H: Operator, k: int = 4, *, sym: Optional[Symmetry] = None, device: str
                       timings={"build_s": t_build, "solve_s": t_solve, "pruned_blocks": pruned,
                                "vectors_s": time.perf_counter() - t2 if vectors else 0.0,
                                "total_s": time.perf_counter() - t0},
-                      k=k, n_sites=H.num_sites, levels=lvls, _bases=bases, _vectors=vecs)
+                      k=k_out, n_sites=H.num_sites, levels=lvls, _bases=bases, _vectors=vecs)
 
 
 @dataclass

@@ -235,7 +235,7 @@ def to_arrays(result) -> Dict[str, np.ndarray]:
     d["level_energy"] = np.array([L["energy"] for L in levels], np.float64)
     d["level_mirror"] = np.full(nl, -1, np.int64)
     d["level_fold"] = np.zeros(nl, np.int64)
-    d["level_multiplicity"] = np.ones(nl, np.int64)
+    d["level_multiplicity"] = np.array([L.get("multiplicity", 1) for L in levels], np.int64)
     d["level_vector"] = np.array([L.get("vector", -1) for L in levels], np.int64)
     tag = np.zeros((nl, TAG_WIDTH), np.int64)
     mchi, moff = [], [0]
@@ -243,7 +243,7 @@ def to_arrays(result) -> Dict[str, np.ndarray]:
         m = L.get("momentum")
         par = L.get("parity")
         tag[i] = [L["n_up"], 0, 0 if m is None else m, -1 if m is None else m, 0,
-                  -1 if par is None else (0 if par > 0 else 1), 1, 1, 0, L["dim"], 1]
+                  -1 if par is None else (0 if par > 0 else 1), 1, 1, 0, L["dim"], L.get("multiplicity", 1)]
         if m is not None:
             mchi.append(np.exp(2j * np.pi * m / result.n_sites))
         moff.append(len(mchi))

@@ -69,6 +69,18 @@ def main():
     with open(dst, "w") as f:
         json.dump(out, f, indent=1)
     print("wrote", dst)
+    # the small audit models: every eigs / spectrum record whose symmetry selection maps onto this
+    # backend's (audit_models.record_keys)
+    sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+    from tests.golden import audit_models as am
+
+    keys = sorted(k for m in am.MODELS for verb, label, k in am.record_keys(m, recs)
+                  if verb == "eigs" or label == "none" or label.startswith("sz="))
+    audit = {"source": out["source"], "reference_sha": out["reference_sha"], "records": {k: recs[k] for k in keys}}
+    dst = os.path.join(HERE, "fixtures", "audit_records.json")
+    with open(dst, "w") as f:
+        json.dump(audit, f, separators=(",", ":"))
+    print("wrote", dst, len(keys), "records")
 
 
 if __name__ == "__main__":

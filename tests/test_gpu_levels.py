@@ -247,9 +247,9 @@ def test_sublattice_bases_survive_the_file(tmp_path):
 
 @pytest.mark.parametrize("total_spin", [0.5, 1.5])
 def test_total_spin_against_the_dense_spin_subspace(total_spin):
-    """N = 9 J1-J2 ring: the 4 lowest levels of spin S against eigvalsh of H restricted to the
-    S(S+1) eigenspace of the dense S^2 in the Sz = S sector (1e-10); their vectors carry
-    <S^2> = S(S+1) (1e-9)."""
+    """N = 9 J1-J2 ring: the 4 lowest energies of spin S (each multiplet counted 2S + 1 times, as
+    the reference counts them) against eigvalsh of H restricted to the S(S+1) eigenspace of the
+    dense S^2 in the Sz = S sector (1e-10); the levels' vectors carry <S^2> = S(S+1) (1e-9)."""
     say("total_spin dense", total_spin)
     N = 9
     H = _ring(N, 0.3)
@@ -262,10 +262,12 @@ def test_total_spin_against_the_dense_spin_subspace(total_spin):
     w, Q = np.linalg.eigh(S2d)
     lam = total_spin * (total_spin + 1)
     Q = Q[:, np.abs(w - lam) < 1e-8]
-    want = np.linalg.eigvalsh(Q.conj().T @ Hd @ Q)[:4]
+    want = np.repeat(np.linalg.eigvalsh(Q.conj().T @ Hd @ Q), two_S + 1)[:4]   # a multiplet counts 2S + 1
     for dmd in (None, 0):
         r = ev.eigs(H, 4, total_spin=total_spin, vectors=True, dense_max_dim=dmd)
         assert np.allclose(r.energies, want, atol=1e-10), (dmd, r.energies, want)
+        assert len(r.levels) == -(-4 // (two_S + 1))
+        assert all(L["multiplicity"] == two_S + 1 for L in r.levels)
         s2 = r.expect([S2])[:, 0].real
         assert np.allclose(s2, lam, atol=1e-9), s2
     with pytest.raises(errors.InvalidRequest):
