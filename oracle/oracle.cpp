@@ -587,8 +587,8 @@ void orc_gaussian_vector(u64 base, u64 sample, i64 n, double* out) {
     std::normal_distribution<double> nd(0.0, 1.0);
     double s2 = 0.0;
     for (i64 i = 0; i < n; ++i) {
+        const double im = nd(gen);   // GCC evaluates Complex(nd(gen), nd(gen))'s second argument first
         const double re = nd(gen);
-        const double im = nd(gen);
         out[2 * i] = re;
         out[2 * i + 1] = im;
         s2 += re * re + im * im;
@@ -602,8 +602,8 @@ void orc_gaussian_raw(u64 base, u64 sample, i64 n, double* out) {
     std::mt19937 gen(static_cast<std::mt19937::result_type>(splitmix(base, sample)));
     std::normal_distribution<double> nd(0.0, 1.0);
     for (i64 i = 0; i < n; ++i) {
+        out[2 * i + 1] = nd(gen);    // the imaginary part is drawn first (see orc_gaussian_vector)
         out[2 * i] = nd(gen);
-        out[2 * i + 1] = nd(gen);
     }
 }
 
@@ -673,8 +673,8 @@ i64 orc_gaussian_polar(std::uint32_t seed, i64 n, double* out, int slip) {
             ++tried;
         } while (r2 > 1.0 || r2 == 0.0);
         const double m = std::sqrt(-2.0 * std::log(r2) / r2);
-        out[2 * i] = (slip == 2 ? x : y) * m;
-        out[2 * i + 1] = (slip == 2 ? y : x) * m;
+        out[2 * i + 1] = (slip == 2 ? x : y) * m;   // the first normal of the pair: the imaginary part
+        out[2 * i] = (slip == 2 ? y : x) * m;
     }
     return tried;
 }

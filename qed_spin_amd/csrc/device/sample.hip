@@ -95,8 +95,8 @@ __global__ __launch_bounds__(kSampleThreads) void k_sample_gauss(Zd* V, int R, c
             keep = !(r2 > 1.0 || r2 == 0.0);
             if (keep) {
                 const double m = sqrt(-2.0 * log(r2) / r2);
-                re = y * m;
-                im = x * m;
+                im = y * m;   // the pair's first normal is the entry's imaginary part (the reference's
+                re = x * m;   // compiler evaluates Complex(nd(gen), nd(gen))'s second argument first)
             }
         }
         const u64 mask = __ballot(keep);

@@ -23,8 +23,10 @@ inline constexpr double kLockRel = 1e-10;
 inline constexpr std::uint64_t kKsStartSeed = 0x51ED0B70ULL;
 inline double scale_or_one(double b) { return b > 0.0 ? b : 1.0; }
 
-/// A unit-variance Gaussian start from std::mt19937_64(seed), (re, im) per complex entry
-/// (block_solve.cpp:268-277 staged_seed, lanczos.h gaussian_entry).
+/// A unit-variance Gaussian start from std::mt19937_64(seed) (block_solve.cpp:268-277 staged_seed,
+/// lanczos.h gaussian_entry). A complex entry is the reference's Complex(nd(gen), nd(gen)); C++
+/// leaves the order of the two argument evaluations unspecified and the reference's compiler (GCC on
+/// x86-64) evaluates the second first: the first draw is the imaginary part.
 template <class Scalar> std::vector<Scalar> gaussian_start(std::size_t n, std::uint64_t seed) {
     std::vector<Scalar> h(n);
     std::mt19937_64 gen(seed);
@@ -33,8 +35,10 @@ template <class Scalar> std::vector<Scalar> gaussian_start(std::size_t n, std::u
         if constexpr (std::is_same_v<Scalar, double>) {
             c = nd(gen);
         } else {
-            const double re = nd(gen);
+            // Complex(nd(gen), nd(gen)): the reference's compiler evaluates the second argument
+            // first, so the first draw is the imaginary part
             const double im = nd(gen);
+            const double re = nd(gen);
             c = Scalar(re, im);
         }
     }
@@ -63,9 +67,9 @@ private:
         for (auto& z : h) {
             if constexpr (std::is_same_v<Scalar, double>) {
                 z = nd(gen_);
-            } else {
-                const double re = nd(gen_);
+            } else {   // imaginary part first, as gaussian_start
                 const double im = nd(gen_);
+                const double re = nd(gen_);
                 z = Scalar(re, im);
             }
         }

@@ -30,13 +30,17 @@ inline std::mt19937 sample_engine(std::uint64_t base, std::uint64_t s) {
     return std::mt19937(static_cast<std::mt19937::result_type>(splitmix_seed(base, s)));
 }
 
-/// n i.i.d. complex Gaussians Complex(nd(gen), nd(gen)) from one normal_distribution, normalised.
+/// n i.i.d. complex Gaussians Complex(nd(gen), nd(gen)) from one normal_distribution, normalised
+/// (sample_seed.h:78-88). The reference's compiler (GCC on x86-64) evaluates the constructor's
+/// second argument first, so each entry's first draw is its imaginary part: for a real H the two
+/// orders give the same Lanczos coefficients (one vector is i conj of the other), for a complex H
+/// only this one reproduces the reference's samples.
 inline std::vector<cplx> gaussian_vector(std::size_t n, std::mt19937& gen) {
     std::normal_distribution<double> nd(0.0, 1.0);
     std::vector<cplx> v(n);
     for (auto& x : v) {
-        const double re = nd(gen);
         const double im = nd(gen);
+        const double re = nd(gen);
         x = cplx(re, im);
     }
     // The 2-norm with the scaling of the reference BLAS dznrm2 (scale / sum-of-squares update).

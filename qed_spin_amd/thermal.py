@@ -21,7 +21,7 @@ from typing import Dict, List, Optional, Sequence
 import numpy as np
 
 from . import _lib
-from .errors import DeviceUnavailable, DeviceUnsupported, InvalidRequest
+from .errors import DeviceUnavailable, DeviceUnsupported, InvalidRequest, ResourceLimit
 from .operator import Operator
 from .sector import AUTOTUNE_MIN_ROWS, DEFAULT_CSR_BUDGET, LANE_CSR, Sector, SectorMatvec, parallel_build
 from .symmetry import Symmetry
@@ -31,6 +31,9 @@ LCG_ADD = 1442695040888963407
 _U64 = (1 << 64) - 1
 KBREAKDOWN_REL = 64.0 * np.finfo(np.float64).eps
 
+
+# method='exact': the dense lanes' largest block (16 GiB as a dense complex matrix on the device)
+EXACT_MAX_DIM = 32768
 
 def next_block_seed(seed: int) -> int:
     """The next block's seed of the thermal walk (thermal.cpp:471)."""
@@ -162,10 +165,11 @@ def thermal(H: Operator, T, *, sym: Optional[Symmetry] = None, method: str = "ft
             use_graph: bool = True, csr_budget: int = DEFAULT_CSR_BUDGET) -> ThermalResult:
     """FTLM thermodynamics of H at temperatures T over the blocks of `sym` (default: every (Sz, k)
     block of a ring). seed 0 draws one. dense_max_dim (default 512, the reference's ThermalSpec
-    default): blocks up to this dimension are solved exactly; 0 samples every block."""
+    default): blocks up to this dimension are solved exactly; 0 samples every block. method="exact":
+    every block's whole spectrum through the dense lanes (blocks of at most EXACT_MAX_DIM states)."""
     resolve_device(device, "thermal")
-    if method != "ftlm":
-        raise DeviceUnsupported(f"thermal: method {method!r} is not on the device path (ftlm only)")
+    if method not in ("ftlm", "exact"):
+        raise DeviceUnsupported(f"thermal: method {method!r} is not on the device path (ftlm and exact only)")
     T = np.asarray(T, np.float64).ravel()
     if T.size == 0 or np.any(~(T > 0)):
         raise InvalidRequest("thermal: temperatures must be positive")
@@ -174,6 +178,8 @@ def thermal(H: Operator, T, *, sym: Optional[Symmetry] = None, method: str = "ft
     dmax = 512 if dense_max_dim is None else int(dense_max_dim)
     if dmax < 0:
         raise InvalidRequest(f"dense_max_dim must be >= 0 or None, got {dense_max_dim}")
+    if method == "exact":   # every block through the dense lanes
+        dmax = EXACT_MAX_DIM
     betas = 1.0 / T
     sym = sym if sym is not None else Symmetry()
     s = int(seed) if seed else int.from_bytes(os.urandom(8), "little")
@@ -182,6 +188,8 @@ def thermal(H: Operator, T, *, sym: Optional[Symmetry] = None, method: str = "ft
     t_build = time.perf_counter() - t0
     if not blocks:
         raise InvalidRequest("thermal: the symmetry selection holds no state")
+    if method == "exact" and max(sec.dim for _, sec, _ in blocks) > EXACT_MAX_DIM:
+        raise ResourceLimit(f"thermal: method 'exact' solves every block densely, at most {EXACT_MAX_DIM} states")
     recs = []
     for blk, sec, m in blocks:
         s = next_block_seed(s)

@@ -76,6 +76,8 @@ def main():
 
     keys = sorted(k for m in am.MODELS for verb, label, k in am.record_keys(m, recs)
                   if verb == "eigs" or label == "none" or label.startswith("sz="))
+    keys += sorted(k for m in am.MODELS for k in (f"{m}/api/thermal/exact/none", f"{m}/api/thermal/exact/auto",
+                                                 f"{m}/api/thermal/ftlm/none") if k in recs)
     audit = {"source": out["source"], "reference_sha": out["reference_sha"], "records": {k: recs[k] for k in keys}}
     dst = os.path.join(HERE, "fixtures", "audit_records.json")
     with open(dst, "w") as f:

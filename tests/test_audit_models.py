@@ -57,3 +57,54 @@ def test_restated_model_has_the_recorded_spectrum(name):
             assert np.allclose(got, ws[: got.size], atol=1e-10), key
         else:
             assert np.abs(ws - np.sort(v["eigenvalues"])).max() < 1e-10, key
+
+
+THERMAL = ["tfim10", "tri_chiral4x3"]
+
+
+@pytest.mark.parametrize("name", THERMAL)
+def test_thermal_records_are_the_dense_thermodynamics(name):
+    """The recorded exact curves (whole system, not per site) are E = <E>, C = (<E^2> - <E>^2) / T^2,
+    S = ln Z + <E> / T of the dense spectrum (1e-10; C 1e-9, a difference of large moments)."""
+    recs = _records()
+    H = am.operator(name)
+    w = np.linalg.eigvalsh(po.dense_matrix(H.products, H.num_sites))
+    for label in ("none", "auto"):
+        r = recs[f"{name}/api/thermal/exact/{label}"]["values"]
+        b = 1.0 / np.asarray(r["T"])
+        x = np.exp(-np.outer(b, w - w[0]))
+        Z = x.sum(1)
+        E = (x * w).sum(1) / Z
+        C = ((x * w * w).sum(1) / Z - E * E) * b * b
+        S = np.log(Z) + b * (E - w[0])
+        assert np.abs(E - r["E"]).max() < 1e-10 and np.abs(S - r["S"]).max() < 1e-10
+        assert np.abs(C - r["C"]).max() < 1e-9
+
+
+def test_complex_ftlm_record_pins_the_draw_order():
+    """tri_chiral4x3's recorded FTLM (a complex H over the whole 4096-state space, samples 8,
+    krylov 40, seed 11) through the oracle's FTLM within 1e-10. Its sample vectors take each entry's
+    first normal as the imaginary part (the reference's Complex(nd(gen), nd(gen)) under GCC's
+    right-to-left argument evaluation); drawn the other way round the curves miss by ~5e-2. A real
+    H cannot tell the two orders apart."""
+    import qed_spin_amd as qs
+
+    r = _records()["tri_chiral4x3/api/thermal/ftlm/none"]["values"]
+    H = am.operator("tri_chiral4x3")
+    s = po.OracleSector(12, -1, np.arange(12, dtype=np.int32)[None, :], [1.0])
+    row_ptr, col, val = po.sector_csr(qs.Program(H, adjoint=True).export(), s)
+    betas = 1.0 / np.array(r["T"])
+    seed = (11 * 6364136223846793005 + 1442695040888963407) % (1 << 64)
+    E = {}
+    for order in ("im first", "re first"):
+        moments = []
+        for k in range(8):
+            v0 = po.gaussian_vector(seed, k, s.dim)
+            if order == "re first":
+                v0 = v0.imag + 1j * v0.real
+            alpha, beta = po.lanczos(row_ptr, col, val, v0, 40, 2, 64 * np.finfo(float).eps * H.norm_bound())
+            ritz, V = po.tridiag_eig(alpha, beta)
+            moments.append(po.ftlm_moments(ritz, np.abs(V[0]) ** 2, betas))
+        E[order] = po.ftlm_combine(np.array(moments), s.dim, betas)[1]
+    assert np.abs(E["im first"] - r["E"]).max() < 1e-10
+    assert np.abs(E["re first"] - r["E"]).max() > 1e-3

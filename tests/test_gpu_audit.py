@@ -49,3 +49,32 @@ def test_audit_record(name, verb, label, key):
         assert r.converged
         assert r.energies.size == want.size, (dmd, r.energies, want)
         assert np.allclose(r.energies, want, rtol=1e-10, atol=1e-10), (dmd, r.energies, want)
+
+
+THERMAL = [m for m in ("tfim10", "tri_chiral4x3") if f"{m}/api/thermal/exact/none" in RECS]
+
+
+@pytest.mark.parametrize("name", THERMAL)
+def test_audit_thermal_records(name):
+    """Exact thermodynamics over the whole space and over this backend's blocks (the reference's
+    'auto' records the same curves) within 1e-10 (C 1e-9); FTLM over the whole space with the
+    recorded samples 8, krylov 40, seed 11 within 1e-8 -- for tri_chiral4x3 a complex H, whose
+    sample vectors are complex."""
+    from qed_spin_amd import thermal as th
+
+    print("[case] thermal", name, flush=True)
+    H = am.operator(name)
+    u1 = am.MODELS[name][2]
+    for label, sym in (("none", am.symmetry(name, "none")[0]), ("auto", am.symmetry(name, "auto")[0])):
+        r = RECS[f"{name}/api/thermal/exact/{label}"]["values"]
+        res = th.thermal(H, np.array(r["T"]), sym=sym, method="exact")
+        for key, got, tol in (("E", res.E, 1e-10), ("C", res.C, 1e-9), ("S", res.S, 1e-10)):
+            assert np.abs(got - np.array(r[key])).max() < tol, (label, key)
+    r = RECS[f"{name}/api/thermal/ftlm/none"]["values"]
+    res = th.thermal(H, np.array(r["T"]), sym=am.symmetry(name, "none")[0], samples=8, krylov=40, seed=11)
+    assert len(res.blocks) == 1 and not res.blocks[0]["exact"]
+    for key, got in (("E", res.E), ("C", res.C), ("S", res.S)):
+        err = np.abs(got - np.array(r[key])).max()
+        print(f"  {key}: max |diff| {err:.3e}", flush=True)
+        assert np.allclose(got, r[key], rtol=1e-8, atol=1e-8), key
+    assert u1 or name == "tfim10"
