@@ -260,20 +260,61 @@ def two_spin(total_spin, N: int) -> int:
     return two_S
 
 
-def spin_tower_operator(H: Operator, two_S: int) -> Operator:
-    """H + mu (S^2 - S(S+1)) with mu = (s_H + 1) / (S + 1): H commutes with S^2, so every level of
-    spin S keeps its energy, and in the Sz = S sector (where every state has spin >= S) each level of
-    spin S' > S rises by mu (S'(S'+1) - S(S+1)) >= 2 (s_H + 1), above every level of spin S (the
-    reference's penalty operator, src/engine/tower.cpp)."""
+def spin_tower_operator(H: Operator, two_S: int, mu: Optional[float] = None) -> Operator:
+    """H + mu (S^2 - S(S+1)) (the reference's penalty operator, src/engine/tower.cpp). H commutes with
+    S^2, so every level of spin S keeps its energy; in the Sz = S sector every state has spin >= S and
+    a level of spin S' > S rises by mu (S'(S'+1) - S(S+1)) >= 2 (S + 1) mu. The default mu =
+    (s_H + 1) / (S + 1) lifts every such level above every level of spin S."""
     N = H.num_sites
     S = 0.5 * two_S
-    mu = (H.norm_bound() + 1.0) / (S + 1.0)
+    if mu is None:
+        mu = (H.norm_bound() + 1.0) / (S + 1.0)
     lam = S * (S + 1.0)
     out = Operator(N, products=list(H.products))
     for ops, sites, c in spin_squared(N).products:
         out.add_product(ops, sites, mu * c)
     out.add_product("I", (0,), -mu * lam)
     return out
+
+
+def _tower(H: Operator, k: int, sym: Symmetry, two_S: int, **kw) -> EigsResult:
+    """The k lowest energies of total spin S (each multiplet counted 2S + 1 times) through the
+    penalty operator in the Sz = S sector. A small penalty keeps the operator's spectral width, and
+    so the Krylov lanes' convergence, close to H's; it is raised until the levels it returns are
+    provably of spin S: every level of a higher spin lies at or above e_min + 2 (S + 1) mu (e_min the
+    lowest level of H in the sector), so levels below that are of spin S. At mu = (s_H + 1) / (S + 1)
+    the bound exceeds every level of spin S and the answer is final."""
+    N = H.num_sites
+    S = 0.5 * two_S
+    n_up = (N + two_S) // 2
+    if sym.sz == "off" or (sym.sz != "all" and [int(x) for x in np.atleast_1d(sym.sz)] != [n_up]):
+        raise InvalidRequest(f"eigs: total_spin {S} is solved in the Sz sector n_up = {n_up}; "
+                             f"the symmetry asks for sz={sym.sz!r}")
+    sym = Symmetry(sz=[n_up], momenta=sym.momenta, translations=sym.translations, reflection=sym.reflection)
+    mult = two_S + 1
+    k_lv = -(-k // mult)
+    t0 = time.perf_counter()
+    base = {key: kw[key] for key in ("device", "dense_max_dim", "max_iter", "prune", "prune_margin", "csr_budget")}
+    e_min = float(eigs(H, 1, sym=sym, **base).energies[0])
+    s_H = H.norm_bound()
+    mu_safe = (s_H + 1.0) / (S + 1.0)
+    mu = mu_safe / 64.0
+    while True:
+        r = eigs(spin_tower_operator(H, two_S, mu), k_lv, sym=sym, **kw)
+        cut = e_min + 2.0 * (S + 1.0) * mu - 1e-9 * max(1.0, s_H)
+        good = [L for L in r.levels if L["energy"] < cut]
+        if len(good) == len(r.levels) == k_lv or mu >= mu_safe:
+            break
+        mu = min(mu_safe, 8.0 * mu)
+    for L in good:
+        L["total_spin"] = S
+        L["multiplicity"] = mult
+    r.levels = good
+    r.energies = np.repeat(np.array([L["energy"] for L in good]), mult)[:k]
+    r.k = k
+    r.n_sites = N
+    r.timings.update({"total_s": time.perf_counter() - t0, "penalty": mu})
+    return r
 
 
 def eigs(H: Operator, k: int = 4, *, sym: Optional[Symmetry] = None, device: str = "gpu",
@@ -292,21 +333,10 @@ def eigs(H: Operator, k: int = 4, *, sym: Optional[Symmetry] = None, device: str
         raise InvalidRequest("eigs: k must be >= 1")
     k = int(k)
     sym = sym if sym is not None else Symmetry()
-    two_S = None
     if total_spin is not None:
-        N = H.num_sites
-        two_S = two_spin(total_spin, N)
-        n_up = (N + two_S) // 2
-        if sym.sz == "off" or (sym.sz != "all" and [int(x) for x in np.atleast_1d(sym.sz)] != [n_up]):
-            raise InvalidRequest(f"eigs: total_spin {two_S / 2} is solved in the Sz sector n_up = {n_up}; "
-                                 f"the symmetry asks for sz={sym.sz!r}")
-        sym = Symmetry(sz=[n_up], momenta=sym.momenta, translations=sym.translations, reflection=sym.reflection)
-        # spin-S levels lie at or below s_H, every penalised one at or above s_H + 2
-        spin_cut = H.norm_bound() + 1.0
-        H = spin_tower_operator(H, two_S)
-    k_out = k
-    if two_S:   # a multiplet counts 2S + 1 in the energies; each is solved once
-        k = -(-k // (two_S + 1))
+        return _tower(H, k, sym, two_spin(total_spin, H.num_sites), device=device, dense_max_dim=dense_max_dim,
+                      max_iter=max_iter, prune=prune, prune_margin=prune_margin, window=window, vectors=vectors,
+                      csr_budget=csr_budget)
     t0 = time.perf_counter()
     blocks = walk_blocks(H, sym, csr_budget)
     if not blocks:
@@ -325,12 +355,8 @@ def eigs(H: Operator, k: int = 4, *, sym: Optional[Symmetry] = None, device: str
         v, conv, lane = levels[i]
         converged = converged and conv
         recs.append({"label": blk.label, "dim": sec.dim, "lane": lane, "levels": v, "converged": conv})
-    rows = sorted((float(e), i) for i, (v, _, _) in levels.items() for e in v)
-    if two_S is not None:   # a tower with fewer than k multiplets: the penalised levels are not returned
-        rows = [r for r in rows if r[0] < spin_cut]
-    rows = rows[:k]
-    mult = 1 if two_S is None else two_S + 1
-    energies = np.repeat(np.array([e for e, _ in rows]), mult)[:k_out]
+    rows = sorted((float(e), i) for i, (v, _, _) in levels.items() for e in v)[:k]
+    energies = np.array([e for e, _ in rows])
     t2 = time.perf_counter()
     got, bases = _level_vectors(blocks, rows, max_iter) if vectors else ({}, {})
     lvls, vecs, used = [], [], {}
@@ -338,9 +364,6 @@ def eigs(H: Operator, k: int = 4, *, sym: Optional[Symmetry] = None, device: str
         blk, sec, _ = blocks[i]
         L = {"energy": e, "n_up": blk.n_up, "momentum": blk.momentum, "parity": blk.parity, "dim": sec.dim,
              "label": blk.label, "vector": -1}
-        if two_S is not None:
-            L["total_spin"] = two_S / 2
-            L["multiplicity"] = mult
         if vectors:
             t = used.get(i, 0)
             used[i] = t + 1
@@ -351,7 +374,7 @@ def eigs(H: Operator, k: int = 4, *, sym: Optional[Symmetry] = None, device: str
                       timings={"build_s": t_build, "solve_s": t_solve, "pruned_blocks": pruned,
                                "vectors_s": time.perf_counter() - t2 if vectors else 0.0,
                                "total_s": time.perf_counter() - t0},
-                      k=k_out, n_sites=H.num_sites, levels=lvls, _bases=bases, _vectors=vecs)
+                      k=k, n_sites=H.num_sites, levels=lvls, _bases=bases, _vectors=vecs)
 
 
 @dataclass

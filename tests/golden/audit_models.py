@@ -1,4 +1,5 @@
-"""The reference's small audit models (tests/python/support/models.py:505-594, make_audit_models),
+"""The reference's small audit models (tests/python/support/models.py:505-646: make_audit_models,
+the square and triangular tori, the NLCE clusters),
 restated as this backend's Operators, with the symmetry selections of its golden cases
 (tests/python/golden/cases.py:60-110) mapped onto this backend's Symmetry.
 
@@ -97,6 +98,44 @@ def _tri_chiral():
     return heisenberg(nn) + heisenberg(nnn, 0.2) + chiral(tri, 0.5)
 
 
+def square_torus(Lx, Ly):
+    """Nearest-neighbour bonds of the periodic Lx x Ly square lattice, each once."""
+    out = set()
+    for y in range(Ly):
+        for x in range(Lx):
+            i = x + Lx * y
+            for j in (((x + 1) % Lx) + Lx * y, x + Lx * ((y + 1) % Ly)):
+                if i != j:
+                    out.add((min(i, j), max(i, j)))
+    return sorted(out)
+
+
+def square_j1j2(L, J2):
+    """models.py:604-617: J1 on the nearest, J2 on the diagonal bonds of the L x L torus."""
+    def idx(x, y):
+        return (x % L) + L * (y % L)
+
+    nn, nnn = set(), set()
+    for y in range(L):
+        for x in range(L):
+            i = idx(x, y)
+            for j in (idx(x + 1, y), idx(x, y + 1)):
+                nn.add((min(i, j), max(i, j)))
+            for j in (idx(x + 1, y + 1), idx(x - 1, y + 1)):
+                nnn.add((min(i, j), max(i, j)))
+    return heisenberg(sorted(nn)) + heisenberg(sorted(nnn), J2)
+
+
+_HEX = [(0, i) for i in range(1, 7)] + [(i, i % 6 + 1) for i in range(1, 7)]
+_TRI6 = [(0, 1), (1, 2), (0, 3), (1, 3), (1, 4), (2, 4), (3, 4), (3, 5), (4, 5)]
+_STRIP = [(i, i + 1) for i in range(0, 9)] + [(i, i + 2) for i in range(0, 8)]
+
+
+def _tri_chiral_3x3():
+    nn, _, tri = triangular_torus(3, 3)
+    return heisenberg(nn) + chiral(tri, 0.5)
+
+
 # name: (N, products, u1, su2, translation-invariant ring)
 MODELS = {
     "dimer": (2, lambda: heisenberg([(0, 1)]), True, True, True),
@@ -111,6 +150,17 @@ MODELS = {
     "random_real8": (8, lambda: random_real(8, 1), True, False, False),
     "random_complex8": (8, lambda: random_complex(8, 2), True, False, False),
     "tri_chiral4x3": (12, _tri_chiral, True, True, False),
+    "square4x3": (12, lambda: heisenberg(square_torus(4, 3)), True, True, False),
+    "square4x4_j2=1": (16, lambda: square_j1j2(4, 1.0), True, True, False),
+    "tri_chiral_3x3": (9, _tri_chiral_3x3, True, True, False),
+    # the open clusters of the NLCE engine (models.py:630-646)
+    "nlce_triangle3": (3, lambda: heisenberg([(0, 1), (1, 2), (2, 0)]), True, True, False),
+    "nlce_rhombus4": (4, lambda: heisenberg([(0, 1), (1, 2), (2, 3), (3, 0), (1, 3)]), True, True, False),
+    "nlce_bowtie5": (5, lambda: heisenberg([(0, 1), (1, 2), (2, 0), (2, 3), (3, 4), (4, 2)]), True, True, False),
+    "nlce_hexagon7": (7, lambda: heisenberg(_HEX), True, True, False),
+    "nlce_hexagon7_xxz": (7, lambda: heisenberg(_HEX, 1.0, 0.75), True, False, False),
+    "nlce_triangle6": (6, lambda: heisenberg(_TRI6), True, True, False),
+    "nlce_strip10": (10, lambda: heisenberg(_STRIP), True, True, False),
 }
 
 
