@@ -77,7 +77,9 @@ def main():
     keys = sorted(k for m in am.MODELS for verb, label, k in am.record_keys(m, recs)
                   if verb == "eigs" or (am.MODELS[m][0] <= 12 and (label == "none" or label.startswith("sz="))))
     keys += sorted(k for m in am.MODELS for k in (f"{m}/api/thermal/exact/none", f"{m}/api/thermal/exact/auto",
-                                                 f"{m}/api/thermal/ftlm/none") if k in recs)
+                                                 f"{m}/api/thermal/ftlm/none", f"{m}/api/dynamics/T=0/none/SzQ",
+                                                 f"{m}/api/dynamics/T=0/auto/SzQ", f"{m}/api/dynamics/T=1/none/SzQ")
+                   if k in recs)
     audit = {"source": out["source"], "reference_sha": out["reference_sha"], "records": {k: recs[k] for k in keys}}
     dst = os.path.join(HERE, "fixtures", "audit_records.json")
     with open(dst, "w") as f:

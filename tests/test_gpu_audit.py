@@ -78,3 +78,35 @@ def test_audit_thermal_records(name):
         print(f"  {key}: max |diff| {err:.3e}", flush=True)
         assert np.allclose(got, r[key], rtol=1e-8, atol=1e-8), key
     assert u1 or name == "tfim10"
+
+
+DYN = [k for k in RECS if "/dynamics/" in k]
+
+
+@pytest.mark.parametrize("key", DYN)
+def test_audit_dynamics_records(key):
+    """S^z(Q) = N^-1/2 sum_i e^{i pi i} S^z_i (site i = x + Lx y) of square4x3 over the whole space:
+    T = 0 (krylov 80; the reference's 'auto' walk records the same curve) and T = 1 (FTLM, samples
+    6, krylov 40, seed 11) within 1e-8 of the records, omega = linspace(-1, 4, 41), eta = 0.15."""
+    from qed_spin_amd import dynamics as dy
+    from qed_spin_amd.operator import Operator
+
+    print("[case]", key, flush=True)
+    name = key.split("/api/")[0]
+    H = am.operator(name)
+    N = H.num_sites
+    O = Operator(N)
+    for i in range(N):
+        O.add_product("z", (i,), np.exp(1j * np.pi * i) / np.sqrt(N))
+    omega = np.linspace(-1.0, 4.0, 41)
+    want = np.array(RECS[key]["values"]["S"])
+    none = am.symmetry(name, "none")[0]
+    if "/T=0/" in key:
+        r = dy.dynamics(H, O, omega, eta=0.15, sym=none, krylov=80)
+    else:
+        r = dy.dynamics(H, O, omega, eta=0.15, T=[1.0], sym=none, krylov=40, samples=6, seed=11)
+    got = r.S[0][0] if "/T=1/" in key else r.S[0]
+    got = np.asarray(got).real.ravel()
+    err = np.abs(got - want).max()
+    print(f"  max |diff| {err:.3e}", flush=True)
+    assert np.allclose(got, want, rtol=1e-8, atol=1e-8)
