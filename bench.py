@@ -180,11 +180,22 @@ def run_dynamics(args):
     return out
 
 
+def _ref_chain(n, J2=0.35):
+    """The reference's bench model chain(N) (tests/python/support/models.py:92-110, imported by
+    bench/cases.py through grid.models): the J1-J2 ring, J1 = 1, J2 = 0.35."""
+    import qed_spin_amd as qs
+
+    b = qs.HamiltonianBuilder(n)
+    b.heisenberg([(i, (i + 1) % n) for i in range(n)], 1.0)
+    b.heisenberg([(i, (i + 2) % n) for i in range(n)], J2)
+    return b.to_operator()
+
+
 def run_reference_dynamics(args):
-    """The reference's two published dynamics cases, same inputs (its bench/cases.py:105-132):
-    chain30_dyn0 -- T = 0 S^z(q = 2 pi 10 / 30) of the Heisenberg ring N = 30 in Sz = 0 over every
-    momentum block, krylov 200, omega = linspace(-1, 7, 401), eta 0.05 (published: 57.6 s on a
-    32-core node); chain24_dynT -- T = 1 FTLM S^z(q = 2 pi 8 / 24) of the ring N = 24 over every
+    """The reference's two published dynamics cases, same inputs (its bench/cases.py:105-132, model
+    chain(N): the J1-J2 ring with J2 = 0.35): chain30_dyn0 -- T = 0 S^z(q = 2 pi 10 / 30), N = 30,
+    Sz = 0, every momentum block, krylov 200, omega = linspace(-1, 7, 401), eta 0.05 (published:
+    57.6 s on a 32-core node); chain24_dynT -- T = 1 FTLM S^z(q = 2 pi 8 / 24), N = 24, every
     (Sz, k) source block, 5 samples, krylov 80, omega = linspace(-3, 7, 301), eta 0.1 (published:
     ~85 s on a 32-core node). The published numbers are context: another machine."""
     import numpy as np
@@ -194,7 +205,7 @@ def run_reference_dynamics(args):
 
     out = {}
     n = 30
-    H = qs.HamiltonianBuilder(n).heisenberg([(i, (i + 1) % n) for i in range(n)], 1.0).to_operator()
+    H = _ref_chain(n)
     w = np.linspace(-1, 7, 401)
     t0 = time.perf_counter()
     r = dy.dynamics(H, [dy.sz_q(n, 2 * np.pi * 10 / n)], w, eta=0.05, krylov=200, sym=qs.Symmetry(sz=[15]))
@@ -203,7 +214,7 @@ def run_reference_dynamics(args):
                            "published_cpu_s": 57.6}
     log(f"chain30_dyn0: {out['chain30_dyn0']}")
     n = 24
-    H = qs.HamiltonianBuilder(n).heisenberg([(i, (i + 1) % n) for i in range(n)], 1.0).to_operator()
+    H = _ref_chain(n)
     w = np.linspace(-3, 7, 301)
     t0 = time.perf_counter()
     r = dy.dynamics(H, [dy.sz_q(n, 2 * np.pi * 8 / n)], w, eta=0.1, krylov=80, T=[1.0], samples=5, seed=11,
@@ -211,6 +222,35 @@ def run_reference_dynamics(args):
     out["chain24_dynT"] = {"wall_s": round(time.perf_counter() - t0, 3),
                            "weight": float(np.trapezoid(r.S[0][0].real, w)), "published_cpu_s": 85.0}
     log(f"chain24_dynT: {out['chain24_dynT']}")
+    return out
+
+
+def run_reference_ftlm(args):
+    """The reference's published GPU thermal case chain28_ftlm_gpu, same inputs (bench/cases.py:68-88):
+    FTLM of chain(28) (J1-J2 ring, J2 = 0.35) over every (Sz, k) block, 4 samples, krylov 60, seed
+    7, T = linspace(0.1, 4, 20) (published: 24.3 s on one H100)."""
+    import numpy as np
+
+    import qed_spin_amd as qs
+    from qed_spin_amd import thermal as th
+
+    from qed_spin_amd import _lib
+
+    n = 28
+    H = _ref_chain(n)
+    T = np.linspace(0.1, 4.0, 20)
+    _lib.device_count()     # the runtime's start-up is not part of the solve
+    _lib.call("qedh_synchronize")
+    t0 = time.perf_counter()
+    r = th.thermal(H, T, sym=qs.Symmetry(), samples=4, krylov=60, seed=7)
+    wall = time.perf_counter() - t0
+    out = {"chain28_ftlm": {"wall_s": round(wall, 3), "E(Tmin)": float(r.E[0]), "C_max": float(np.max(r.C)),
+                            "build_split": {k: round(v, 3) for k, v in th.WALK_TIMES.items()},
+                            "blocks": len(r.blocks), "sampled_blocks": sum(1 for b in r.blocks if not b["exact"]),
+                            "timings": {k: round(v, 3) for k, v in r.timings.items()},
+                            "engine": {k: round(v, 4) for k, v in r.engine.items()},
+                            "published_gpu_s": 24.3, "published_hw": "1x H100"}}
+    log(f"chain28_ftlm: {out['chain28_ftlm']}")
     return out
 
 
@@ -300,7 +340,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--workload", default="matvec,serve",
-                    help="matvec (the headline) and a comma list of ftlm, dynamics, serve, spectrum, refdyn")
+                    help="matvec (the headline) and a comma list of ftlm, dynamics, serve, spectrum, refdyn, refftlm")
     ap.add_argument("--ftlm-sites", type=int, default=24)
     ap.add_argument("--dyn-sites", type=int, default=28)
     ap.add_argument("--spectrum-sites", type=int, default=18)
@@ -459,6 +499,8 @@ def main():
             out["spectrum"] = run_spectrum(args)
         elif w == "refdyn":
             out["reference_dynamics"] = run_reference_dynamics(args)
+        elif w == "refftlm":
+            out["reference_ftlm"] = run_reference_ftlm(args)
         else:
             raise SystemExit(f"unknown workload {w!r}")
     print(json.dumps(out))

@@ -140,6 +140,10 @@ class ThermalResult:
     engine: Dict[str, float] = field(default_factory=dict)
 
 
+# the last walk's split of its build time (sectors, then operators), for the benchmarks
+WALK_TIMES: Dict[str, float] = {}
+
+
 def walk_blocks(H: Operator, sym: Symmetry, csr_budget: int = DEFAULT_CSR_BUDGET):
     """The non-empty blocks of the walk in the reference's order (n_up ascending, then momentum),
     each with its sector operator on the device CSR lane. Sectors and the operators of small
@@ -147,7 +151,9 @@ def walk_blocks(H: Operator, sym: Symmetry, csr_budget: int = DEFAULT_CSR_BUDGET
     variants are built one after another, so the timings see an otherwise idle device."""
     blks = sym.blocks(H.num_sites)
     H._handle()
+    t0 = time.perf_counter()
     secs = parallel_build(lambda b: Sector(b.group, b.n_up), blks)
+    WALK_TIMES["sectors_s"] = time.perf_counter() - t0
     keep = [(b, s) for b, s in zip(blks, secs) if s.dim > 0]
     mats = [None] * len(keep)
     small = [i for i, (_, s) in enumerate(keep) if s.dim < AUTOTUNE_MIN_ROWS]
@@ -157,6 +163,7 @@ def walk_blocks(H: Operator, sym: Symmetry, csr_budget: int = DEFAULT_CSR_BUDGET
     for i in range(len(keep)):
         if mats[i] is None:
             mats[i] = make(i)
+    WALK_TIMES["operators_s"] = time.perf_counter() - t0 - WALK_TIMES["sectors_s"]
     return [(b, s, m) for (b, s), m in zip(keep, mats)]
 
 
