@@ -128,6 +128,9 @@ int qedh_sector_destroy(qedh_sector s);
  * src/engine/internal.h:163-609; make_sector_matvec_gpu_rep, make_cross_matvec_gpu_rep,
  * build_sector_csr_gpu, csr_matvec_gpu(_multi), include/ed/gpu/rep_matvec.h:43-101) ----------- */
 enum { QEDH_LANE_AUTO = 0, QEDH_LANE_GATHER = 1, QEDH_LANE_CSR = 2 };
+/* OR-ed into the lane of qedh_matvec_create (no counterpart in the reference): skip the SpMV autotuning
+ * of a large CSR operator (for operators whose applies run through a batched engine instead). */
+enum { QEDH_LANE_NO_TUNE = 0x100 };
 typedef struct {
     int64_t rows, cols;
     int is_real;          /* vectors are double */

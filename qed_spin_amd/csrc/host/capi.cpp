@@ -353,8 +353,10 @@ int qedh_matvec_create(qedh_sector rows, qedh_sector cols, qedh_program prog, do
         need(prog, "program");
         need(out, "out");
         auto c = (cols && cols != rows) ? cols->sector : rows->sector;
+        const bool tune = (lane & QEDH_LANE_NO_TUNE) == 0;
         *out = new qedh_matvec_s{std::make_shared<qsa::SectorOperator>(rows->sector, c, device_program(prog), norm_bound,
-                                                                      lane, csr_budget_bytes)};
+                                                                      lane & ~QEDH_LANE_NO_TUNE, csr_budget_bytes,
+                                                                      tune)};
     });
 }
 int qedh_matvec_info(qedh_matvec m, qedh_matvec_info_t* info) {

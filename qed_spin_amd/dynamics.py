@@ -231,8 +231,11 @@ def dynamics(H: Operator, probes, omega, *, eta: float = 0.1, T=None, sym: Optio
         raise InvalidRequest("dynamics: an omega grid and eta > 0 are required")
     sym = sym if sym is not None else Symmetry()
     t0 = time.perf_counter()
+    finite_T = T is not None and len(np.atleast_1d(T)) > 0
     if blocks is None:
-        blocks = walk_blocks(H, sym, csr_budget)
+        # T = 0: the continued fractions run in the batched engine and the ground search's applies are
+        # few, so the blocks keep the default SpMV shape; T > 0 applies every source block many times
+        blocks = walk_blocks(H, sym, csr_budget, tune=finite_T)
     if not blocks:
         raise InvalidRequest("dynamics: the symmetry selection holds no state")
     t_walk = time.perf_counter() - t0

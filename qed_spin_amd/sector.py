@@ -15,6 +15,7 @@ from .errors import InvalidRequest
 from .operator import Operator, Program
 
 LANE_AUTO, LANE_GATHER, LANE_CSR = 0, 1, 2
+LANE_NO_TUNE = 0x100             # OR-ed into the lane: skip the SpMV autotuning
 DEFAULT_RANK_BUDGET = 16 << 30   # dense rank table up to 16 GiB of HBM
 AUTOTUNE_MIN_ROWS = 1 << 18      # CSR operators from this many rows time their SpMV variants when built
 BUILD_THREADS = 8
@@ -160,7 +161,9 @@ class SectorMatvec:
     The program is the walk of O^dagger (for two sectors, lambda-projected with rows as ket)."""
 
     def __init__(self, op: Operator, rows: Sector, cols: Optional[Sector] = None, lane: int = LANE_AUTO,
-                 csr_budget: int = DEFAULT_CSR_BUDGET):
+                 csr_budget: int = DEFAULT_CSR_BUDGET, tune: bool = True):
+        """tune=False: a large CSR operator keeps the default SpMV shape instead of timing the
+        variants (for operators applied only through a batched engine)."""
         self.op = op
         self.rows = rows
         self.cols = cols if cols is not None else rows
@@ -170,7 +173,8 @@ class SectorMatvec:
             self.program = Program(op, adjoint=True, src=rows, tgt=self.cols)
         h = C.c_void_p()
         _lib.call("qedh_matvec_create", rows.h, None if self.cols is rows else self.cols.h, self.program.h,
-                  C.c_double(op.norm_bound()), int(lane), C.c_uint64(int(csr_budget)), C.byref(h))
+                  C.c_double(op.norm_bound()), int(lane) | (0 if tune else LANE_NO_TUNE), C.c_uint64(int(csr_budget)),
+                  C.byref(h))
         self.h = h
         self.info = self._info()
 

@@ -14,6 +14,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import threading
 import time
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional, Sequence
@@ -143,27 +144,48 @@ class ThermalResult:
 # the last walk's split of its build time (sectors, then operators), for the benchmarks
 WALK_TIMES: Dict[str, float] = {}
 
+# Device objects of finished walks being released on a thread of their own: a walk's hundreds of
+# sectors and operators take seconds to free, which need not delay the verb's return. The next walk
+# waits for them first, so their memory is back in the pool before it allocates.
+_RELEASING: List[threading.Thread] = []
 
-def walk_blocks(H: Operator, sym: Symmetry, csr_budget: int = DEFAULT_CSR_BUDGET):
+
+def release_later(objs: list) -> None:
+    """Drop the list's references (and so free the device objects only it holds) on another thread."""
+    t = threading.Thread(target=objs.clear, name="qsa-release")
+    t.start()
+    _RELEASING.append(t)
+
+
+def wait_released() -> None:
+    while _RELEASING:
+        _RELEASING.pop().join()
+
+
+def walk_blocks(H: Operator, sym: Symmetry, csr_budget: int = DEFAULT_CSR_BUDGET, tune: bool = True):
     """The non-empty blocks of the walk in the reference's order (n_up ascending, then momentum),
     each with its sector operator on the device CSR lane. Sectors and the operators of small
     sectors are built on several host threads at once; operators large enough to time their SpMV
-    variants are built one after another, so the timings see an otherwise idle device."""
+    variants are built one after another, so the timings see an otherwise idle device. tune=False
+    (walks whose applies all run through a batched engine, which packs the blocks' CSRs and has
+    kernels of its own): no operator times its variants and all are built on the threads."""
     blks = sym.blocks(H.num_sites)
     H._handle()
+    wait_released()
     t0 = time.perf_counter()
     secs = parallel_build(lambda b: Sector(b.group, b.n_up), blks)
     WALK_TIMES["sectors_s"] = time.perf_counter() - t0
     keep = [(b, s) for b, s in zip(blks, secs) if s.dim > 0]
     mats = [None] * len(keep)
-    small = [i for i, (_, s) in enumerate(keep) if s.dim < AUTOTUNE_MIN_ROWS]
-    make = lambda i: SectorMatvec(H, keep[i][1], lane=LANE_CSR, csr_budget=csr_budget)
+    small = [i for i, (_, s) in enumerate(keep) if s.dim < AUTOTUNE_MIN_ROWS or not tune]
+    make = lambda i: SectorMatvec(H, keep[i][1], lane=LANE_CSR, csr_budget=csr_budget, tune=tune)
     for i, m in zip(small, parallel_build(make, small)):
         mats[i] = m
     for i in range(len(keep)):
         if mats[i] is None:
             mats[i] = make(i)
     WALK_TIMES["operators_s"] = time.perf_counter() - t0 - WALK_TIMES["sectors_s"]
+    release_later(secs)    # the empty sectors (the kept ones live on in the walk)
     return [(b, s, m) for (b, s), m in zip(keep, mats)]
 
 
@@ -191,7 +213,7 @@ def thermal(H: Operator, T, *, sym: Optional[Symmetry] = None, method: str = "ft
     sym = sym if sym is not None else Symmetry()
     s = int(seed) if seed else int.from_bytes(os.urandom(8), "little")
     t0 = time.perf_counter()
-    blocks = walk_blocks(H, sym, csr_budget)
+    blocks = walk_blocks(H, sym, csr_budget, tune=False)   # every apply runs in the batched engine
     t_build = time.perf_counter() - t0
     if not blocks:
         raise InvalidRequest("thermal: the symmetry selection holds no state")
@@ -223,6 +245,8 @@ def thermal(H: Operator, T, *, sym: Optional[Symmetry] = None, method: str = "ft
     c = combine(betas, [r["weight"] for r in recs], lnZ, E, V)
     for i, r in enumerate(recs):
         r.update(lnZ=lnZ[i].copy(), E=E[i].copy(), V=V[i].copy(), e_min=float(emin[i]))
+    mats = None
+    release_later(blocks)
     return ThermalResult(
         T=T, lnZ=c["lnZ"], E=c["E"], C=c["C"], S=c["S"], F=c["F"], e0=float(np.min(emin)),
         total_dim=int(sum(r["dim"] for r in recs)), blocks=recs,
