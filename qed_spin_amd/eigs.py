@@ -33,9 +33,9 @@ from . import _lib
 from . import levels as lv
 from .errors import InvalidRequest
 from .operator import Operator
-from .sector import DEFAULT_CSR_BUDGET, LANE_GATHER, DeviceArray, Sector, SectorMatvec, parallel_build
+from .sector import DEFAULT_CSR_BUDGET, LANE_CSR, LANE_GATHER, DeviceArray, Sector, SectorMatvec, parallel_build
 from .symmetry import Symmetry
-from .thermal import _handles, resolve_device, walk_blocks
+from .thermal import _handles, resolve_device, wait_released, walk_blocks
 
 # the automatic dense floor's ceiling (src/engine/internal.h:815)
 AUTO_DENSE_CEILING = 8192
@@ -277,6 +277,63 @@ def spin_tower_operator(H: Operator, two_S: int, mu: Optional[float] = None) -> 
     return out
 
 
+def _build_blocks(H: Operator, blks, csr_budget: int, tune: bool):
+    """(block, sector, operator) of each block, built on the build threads."""
+    secs = parallel_build(lambda b: Sector(b.group, b.n_up), blks)
+    mats = parallel_build(lambda s: SectorMatvec(H, s, lane=LANE_CSR, csr_budget=csr_budget, tune=tune), secs)
+    return [(b, s, m) for b, s, m in zip(blks, secs, mats)]
+
+
+def _stream_levels(H: Operator, k: int, sym: Symmetry, *, floor: int, prune: bool, prune_margin: float, window: float,
+                   max_iter: int, csr_budget: int):
+    """walk_levels without the resident walk: each build thread takes one block at a time -- its
+    sector, its operator and (above the outright size) its pruning estimate -- and frees it; the
+    blocks solved outright are then built again together and solved as walk_levels solves them, and
+    the candidates, by increasing estimate, are built again one by one when not pruned. Returns
+    ([(block, dim)], {index: (levels, converged, lane)}, pruned, {}, build seconds)."""
+    t0 = time.perf_counter()
+    blks = sym.blocks(H.num_sites)
+    H._handle()
+    wait_released()
+    small = max(min(floor, PRUNE_MIN_DIM), 2)
+
+    def first(b):
+        sec = Sector(b.group, b.n_up)
+        if sec.dim == 0 or sec.dim <= small:
+            return sec.dim, None
+        if not prune:        # every larger block is solved, one at a time
+            return sec.dim, -math.inf
+        return sec.dim, SectorMatvec(H, sec, lane=LANE_CSR, csr_budget=csr_budget, tune=False).estimate()
+
+    scan = parallel_build(first, blks)
+    keep = [i for i, (d, _) in enumerate(scan) if d > 0]
+    if not keep:
+        raise InvalidRequest("eigs: the symmetry selection holds no state")
+    meta = [(blks[i], scan[i][0]) for i in keep]
+    est = {j: scan[i][1] for j, i in enumerate(keep) if scan[i][1] is not None}
+    t_build = time.perf_counter() - t0
+    out = [j for j in range(len(meta)) if j not in est]
+    levels = {}
+    if out:
+        built = _build_blocks(H, [meta[j][0] for j in out], csr_budget, tune=False)
+        lv_out, _ = walk_levels(built, k, floor=floor, prune=False, max_iter=max_iter)
+        levels = {out[q]: v for q, v in lv_out.items()}
+        del built
+    margin_floor = 0.05 * (H.norm_bound() or 1.0)
+    pruned = 0
+    for j in sorted(est, key=lambda q: (est[q], q)):
+        found = np.sort(np.concatenate([v for v, _, _ in levels.values()])) if levels else np.zeros(0)
+        kth = found[k - 1] if found.size >= k else math.inf
+        if prune and est[j] > kth + max(prune_margin * max(margin_floor, abs(kth)), window):
+            pruned += 1
+            continue
+        blk, dim = meta[j]
+        (_, sec, m), = _build_blocks(H, [blk], csr_budget, tune=True)
+        levels[j] = _solve_block(m, dim, min(k, dim), floor, max_iter)
+        del sec, m
+    return meta, levels, pruned, {}, t_build
+
+
 def _tower(H: Operator, k: int, sym: Symmetry, two_S: int, **kw) -> EigsResult:
     """The k lowest energies of total spin S (each multiplet counted 2S + 1 times) through the
     penalty operator in the Sz = S sector. A small penalty keeps the operator's spectral width, and
@@ -294,7 +351,8 @@ def _tower(H: Operator, k: int, sym: Symmetry, two_S: int, **kw) -> EigsResult:
     mult = two_S + 1
     k_lv = -(-k // mult)
     t0 = time.perf_counter()
-    base = {key: kw[key] for key in ("device", "dense_max_dim", "max_iter", "prune", "prune_margin", "csr_budget")}
+    base = {key: kw[key] for key in ("device", "dense_max_dim", "max_iter", "prune", "prune_margin", "stream",
+                                     "csr_budget")}
     e_min = float(eigs(H, 1, sym=sym, **base).energies[0])
     s_H = H.norm_bound()
     mu_safe = (s_H + 1.0) / (S + 1.0)
@@ -319,7 +377,7 @@ def _tower(H: Operator, k: int, sym: Symmetry, two_S: int, **kw) -> EigsResult:
 
 def eigs(H: Operator, k: int = 4, *, sym: Optional[Symmetry] = None, device: str = "gpu",
          dense_max_dim: Optional[int] = None, max_iter: int = 0, prune: bool = True, prune_margin: float = 0.02,
-         window: float = 0.0, vectors: bool = False, total_spin=None,
+         window: float = 0.0, vectors: bool = False, total_spin=None, stream: bool = False,
          csr_budget: int = DEFAULT_CSR_BUDGET) -> EigsResult:
     """The k lowest levels of H over the blocks of `sym` (default: every (Sz, k) block of a ring).
     prune (the reference's default): blocks whose 40-step estimate lies far above the k-th level
@@ -327,7 +385,9 @@ def eigs(H: Operator, k: int = 4, *, sym: Optional[Symmetry] = None, device: str
     eigenpairs of its block on the device, in the block's sector basis). total_spin (a spin S or
     "min", H SU(2) symmetric): the k lowest energies of total spin S, each multiplet repeated 2S + 1
     times as the reference counts it, solved once in the Sz = S sector of `sym`'s spatial blocks
-    under the penalty operator (spin_tower_operator); a level's vector is its Sz = S member."""
+    under the penalty operator (spin_tower_operator); a level's vector is its Sz = S member.
+    stream: hold at most one block per build thread on the device (walks whose blocks do not fit the
+    device together); the estimates, the pruning and the levels are those of the resident walk."""
     resolve_device(device, "eigs")
     if int(k) < 1:
         raise InvalidRequest("eigs: k must be >= 1")
@@ -336,33 +396,43 @@ def eigs(H: Operator, k: int = 4, *, sym: Optional[Symmetry] = None, device: str
     if total_spin is not None:
         return _tower(H, k, sym, two_spin(total_spin, H.num_sites), device=device, dense_max_dim=dense_max_dim,
                       max_iter=max_iter, prune=prune, prune_margin=prune_margin, window=window, vectors=vectors,
-                      csr_budget=csr_budget)
+                      stream=stream, csr_budget=csr_budget)
     t0 = time.perf_counter()
-    blocks = walk_blocks(H, sym, csr_budget)
-    if not blocks:
-        raise InvalidRequest("eigs: the symmetry selection holds no state")
-    t_build = time.perf_counter() - t0
-    t1 = time.perf_counter()
-    levels, pruned = walk_levels(blocks, k, floor=dense_floor(k, dense_max_dim), prune=prune,
-                                 prune_margin=prune_margin, window=window, norm_bound=H.norm_bound(),
-                                 max_iter=max_iter)
+    floor = dense_floor(k, dense_max_dim)
+    if stream:
+        meta, levels, pruned, blocks, t_build = _stream_levels(H, k, sym, floor=floor, prune=prune,
+                                                               prune_margin=prune_margin, window=window,
+                                                               max_iter=max_iter, csr_budget=csr_budget)
+        t1 = t0 + t_build
+    else:
+        blocks = walk_blocks(H, sym, csr_budget)
+        if not blocks:
+            raise InvalidRequest("eigs: the symmetry selection holds no state")
+        meta = [(blk, sec.dim) for blk, sec, _ in blocks]
+        t_build = time.perf_counter() - t0
+        t1 = time.perf_counter()
+        levels, pruned = walk_levels(blocks, k, floor=floor, prune=prune, prune_margin=prune_margin, window=window,
+                                     norm_bound=H.norm_bound(), max_iter=max_iter)
     t_solve = time.perf_counter() - t1
     recs, converged = [], True
-    for i, (blk, sec, _) in enumerate(blocks):
+    for i, (blk, dim) in enumerate(meta):
         if i not in levels:
-            recs.append({"label": blk.label, "dim": sec.dim, "lane": "pruned", "levels": np.zeros(0), "converged": True})
+            recs.append({"label": blk.label, "dim": dim, "lane": "pruned", "levels": np.zeros(0), "converged": True})
             continue
         v, conv, lane = levels[i]
         converged = converged and conv
-        recs.append({"label": blk.label, "dim": sec.dim, "lane": lane, "levels": v, "converged": conv})
+        recs.append({"label": blk.label, "dim": dim, "lane": lane, "levels": v, "converged": conv})
     rows = sorted((float(e), i) for i, (v, _, _) in levels.items() for e in v)[:k]
     energies = np.array([e for e, _ in rows])
     t2 = time.perf_counter()
+    if vectors and stream:   # the blocks of the returned levels, built again
+        need = sorted({i for _, i in rows})
+        blocks = dict(zip(need, _build_blocks(H, [meta[i][0] for i in need], csr_budget, tune=True)))
     got, bases = _level_vectors(blocks, rows, max_iter) if vectors else ({}, {})
     lvls, vecs, used = [], [], {}
     for e, i in rows:
-        blk, sec, _ = blocks[i]
-        L = {"energy": e, "n_up": blk.n_up, "momentum": blk.momentum, "parity": blk.parity, "dim": sec.dim,
+        blk, dim = meta[i]
+        L = {"energy": e, "n_up": blk.n_up, "momentum": blk.momentum, "parity": blk.parity, "dim": dim,
              "label": blk.label, "vector": -1}
         if vectors:
             t = used.get(i, 0)

@@ -57,6 +57,16 @@ def dihedral_group(N: int, m: int, parity: int) -> GroupSpec:
     return GroupSpec(perms=perms, chars=chars)
 
 
+def with_spin_flip(g: GroupSpec, z: int) -> GroupSpec:
+    """The group g times the global spin flip (every site flipped), the flipped elements' characters
+    z chi(g): the sector of flip parity z (only for n_up = N / 2, where the flip keeps Sz)."""
+    N = g.N
+    full = np.uint64((1 << N) - 1)
+    base = np.zeros(g.G, np.uint64) if g.flips is None else np.asarray(g.flips, np.uint64)
+    return GroupSpec(perms=np.concatenate([g.perms, g.perms]), chars=np.concatenate([g.chars, z * g.chars]),
+                     flips=np.concatenate([base, base ^ full]))
+
+
 def trivial_group(N: int) -> GroupSpec:
     return GroupSpec(perms=np.arange(N, dtype=np.int32)[None, :], chars=np.ones(1, np.complex128))
 
@@ -69,6 +79,7 @@ class Block:
     n_up: int
     momentum: Optional[int] = None
     parity: Optional[int] = None
+    flip: Optional[int] = None
 
     @property
     def label(self) -> str:
@@ -77,6 +88,8 @@ class Block:
             parts.append(f"k={self.momentum}")
         if self.parity is not None:
             parts.append(f"R={self.parity:+d}")
+        if self.flip is not None:
+            parts.append(f"Z={self.flip:+d}")
         return ",".join(parts)
 
 
@@ -92,6 +105,7 @@ class Symmetry:
     momenta: object = "all"
     translations: bool = True
     reflection: Optional[int] = None
+    spin_flip: Optional[object] = None   # None, +1, -1 or "all" (both parities); applies at n_up = N / 2
 
     @classmethod
     def none(cls) -> "Symmetry":
@@ -106,15 +120,22 @@ class Symmetry:
             nups = [int(x) for x in self.sz]
         else:
             nups = [int(self.sz)]
+        if self.spin_flip not in (None, 1, -1, "all"):
+            raise ValueError(f"spin_flip must be None, +1, -1 or 'all', got {self.spin_flip!r}")
         out = []
         for n_up in nups:
             if not self.translations:
-                out.append(Block(trivial_group(N), n_up))
-                continue
-            ms = range(N) if self.momenta in ("all", None) else [int(m) % N for m in self.momenta]
-            for m in ms:
-                if self.reflection is not None:
-                    out.append(Block(dihedral_group(N, m, int(self.reflection)), n_up, m, int(self.reflection)))
-                else:
-                    out.append(Block(chain_group(N, m), n_up, m))
+                blocks = [Block(trivial_group(N), n_up)]
+            else:
+                blocks = []
+                ms = range(N) if self.momenta in ("all", None) else [int(m) % N for m in self.momenta]
+                for m in ms:
+                    if self.reflection is not None:
+                        blocks.append(Block(dihedral_group(N, m, int(self.reflection)), n_up, m, int(self.reflection)))
+                    else:
+                        blocks.append(Block(chain_group(N, m), n_up, m))
+            if self.spin_flip is not None and 2 * n_up == N:
+                zs = (1, -1) if self.spin_flip == "all" else (int(self.spin_flip),)
+                blocks = [Block(with_spin_flip(b.group, z), b.n_up, b.momentum, b.parity, z) for b in blocks for z in zs]
+            out.extend(blocks)
         return out

@@ -143,3 +143,33 @@ def test_vectors_record_count_and_rayleigh():
         assert abs(np.linalg.norm(v) - 1.0) < 1e-12
     ex = r.expect([H])
     assert np.allclose(ex[:, 0].real, r.energies, atol=1e-10)
+
+
+@pytest.mark.parametrize("prune", [True, False])
+@pytest.mark.parametrize("key", ["auto", "sz=6", "auto/sz=6"])
+def test_streamed_walk_reproduces_reference_records(key, prune):
+    """eigs(stream=True) -- one block per build thread on the device at a time, estimates and pruning
+    as the resident walk -- gives the recorded levels of j1j2_chain12 (1e-10) and the same pruned
+    count as the resident walk."""
+    say("eigs streamed", key, "prune", prune)
+    want = _records()[f"j1j2_chain12/api/eigs/{key}/k=4"]["values"]["energies"]
+    r = ev.eigs(_h12(), 4, sym=CASES[key], prune=prune, stream=True, dense_max_dim=0)
+    assert r.converged
+    assert np.allclose(r.energies, want, rtol=1e-10, atol=1e-10), (r.energies, want)
+    w = ev.eigs(_h12(), 4, sym=CASES[key], prune=prune, dense_max_dim=0)
+    assert r.timings["pruned_blocks"] == w.timings["pruned_blocks"]
+    assert [b["lane"] for b in r.blocks] == [b["lane"] for b in w.blocks]
+
+
+def test_spin_flip_blocks_reproduce_the_records():
+    """Translations times the spin flip at Sz = 0 (both parities): the recorded Sz = 0 levels of
+    j1j2_chain12, resident and streamed, with vectors (unit, Rayleigh quotient = energy)."""
+    say("spin flip blocks")
+    want = _records()["j1j2_chain12/api/eigs/auto/sz=6/k=4"]["values"]["energies"]
+    H = _h12()
+    for stream in (False, True):
+        r = ev.eigs(H, 4, sym=qs.Symmetry(sz=[6], spin_flip="all"), stream=stream, vectors=True)
+        assert np.allclose(r.energies, want, rtol=1e-10, atol=1e-10), (stream, r.energies, want)
+        assert len(r.blocks) == 24 and all("Z=" in b["label"] for b in r.blocks)
+        ex = r.expect([H])[:, 0].real
+        assert np.allclose(ex, r.energies, atol=1e-10)
