@@ -254,6 +254,33 @@ def run_reference_ftlm(args):
     return out
 
 
+def run_reference_eigs(args):
+    """The reference's published GPU eigs case chain32_abelian_eigs_gpu, same inputs
+    (bench/cases.py:62-65): the lowest level of chain(32) (J1-J2 ring, J2 = 0.35) at Sz = 0 over every
+    momentum and spin-flip block (64 blocks of ~9.4e6 states), pruned as the reference's walk prunes
+    (published: 64.9 s on one H100). The blocks are streamed: one per build thread on the device at
+    a time."""
+    import qed_spin_amd as qs
+    from qed_spin_amd import _lib
+    from qed_spin_amd import eigs as ev
+
+    n = 32
+    H = _ref_chain(n)
+    _lib.device_count()
+    _lib.call("qedh_synchronize")
+    t0 = time.perf_counter()
+    r = ev.eigs(H, 1, sym=qs.Symmetry(sz=[16], spin_flip="all"), stream=True)
+    wall = time.perf_counter() - t0
+    solved = [b for b in r.blocks if b["lane"] != "pruned"]
+    out = {"chain32_abelian_eigs": {"wall_s": round(wall, 3), "E0": float(r.energies[0]), "blocks": len(r.blocks),
+                                    "solved_blocks": len(solved), "pruned_blocks": r.timings["pruned_blocks"],
+                                    "max_block_dim": max(b["dim"] for b in r.blocks),
+                                    "timings": {k: round(float(v), 3) for k, v in r.timings.items()},
+                                    "published_gpu_s": 64.9, "published_hw": "1x H100"}}
+    log(f"chain32_abelian_eigs: {out['chain32_abelian_eigs']}")
+    return out
+
+
 def run_spectrum(args):
     """The whole spectrum of the Heisenberg ring over every (Sz, k) block: the dense lanes (blocks
     from 1024 states on the device, rocSOLVER zheevd batched per dimension; smaller ones on the host
@@ -340,7 +367,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--workload", default="matvec,serve",
-                    help="matvec (the headline) and a comma list of ftlm, dynamics, serve, spectrum, refdyn, refftlm")
+                    help="matvec (the headline) and a comma list of ftlm, dynamics, serve, spectrum, refdyn, refftlm, refeigs")
     ap.add_argument("--ftlm-sites", type=int, default=24)
     ap.add_argument("--dyn-sites", type=int, default=28)
     ap.add_argument("--spectrum-sites", type=int, default=18)
@@ -501,6 +528,8 @@ def main():
             out["reference_dynamics"] = run_reference_dynamics(args)
         elif w == "refftlm":
             out["reference_ftlm"] = run_reference_ftlm(args)
+        elif w == "refeigs":
+            out["reference_eigs"] = run_reference_eigs(args)
         else:
             raise SystemExit(f"unknown workload {w!r}")
     print(json.dumps(out))

@@ -296,6 +296,19 @@ def _stream_levels(H: Operator, k: int, sym: Symmetry, *, floor: int, prune: boo
     H._handle()
     wait_released()
     small = max(min(floor, PRUNE_MIN_DIM), 2)
+    # A real H maps the momentum-m block onto the complex conjugate of the momentum-(N - m) block
+    # (time reversal: same representatives and norms, conjugate characters), so the two have one
+    # spectrum; the second of each such pair is neither built nor solved: it takes its partner's
+    # dimension, estimate and levels.
+    N = H.num_sites
+    partner = {}
+    if H.is_real():
+        where = {(b.n_up, b.momentum, b.parity, b.flip): i for i, b in enumerate(blks)}
+        for i, b in enumerate(blks):
+            if b.momentum is not None and 2 * b.momentum > N:
+                p = where.get((b.n_up, N - b.momentum, b.parity, b.flip))
+                if p is not None and np.allclose(np.asarray(blks[p].group.chars).conj(), b.group.chars):
+                    partner[i] = p
 
     def first(b):
         sec = Sector(b.group, b.n_up)
@@ -305,20 +318,29 @@ def _stream_levels(H: Operator, k: int, sym: Symmetry, *, floor: int, prune: boo
             return sec.dim, -math.inf
         return sec.dim, SectorMatvec(H, sec, lane=LANE_CSR, csr_budget=csr_budget, tune=False).estimate()
 
-    scan = parallel_build(first, blks)
+    primary = [i for i in range(len(blks)) if i not in partner]
+    scan = dict(zip(primary, parallel_build(first, [blks[i] for i in primary])))
+    for i, p in partner.items():
+        scan[i] = scan[p]
+    scan = [scan[i] for i in range(len(blks))]
     keep = [i for i, (d, _) in enumerate(scan) if d > 0]
+    jmap = {i: j for j, i in enumerate(keep)}
+    mirror = {jmap[i]: jmap[p] for i, p in partner.items() if i in jmap}
     if not keep:
         raise InvalidRequest("eigs: the symmetry selection holds no state")
     meta = [(blks[i], scan[i][0]) for i in keep]
     est = {j: scan[i][1] for j, i in enumerate(keep) if scan[i][1] is not None}
     t_build = time.perf_counter() - t0
-    out = [j for j in range(len(meta)) if j not in est]
+    out = [j for j in range(len(meta)) if j not in est and j not in mirror]
     levels = {}
     if out:
         built = _build_blocks(H, [meta[j][0] for j in out], csr_budget, tune=False)
         lv_out, _ = walk_levels(built, k, floor=floor, prune=False, max_iter=max_iter)
         levels = {out[q]: v for q, v in lv_out.items()}
         del built
+    for j, p in mirror.items():
+        if j not in est:
+            levels[j] = levels[p]
     margin_floor = 0.05 * (H.norm_bound() or 1.0)
     pruned = 0
     for j in sorted(est, key=lambda q: (est[q], q)):
@@ -326,6 +348,12 @@ def _stream_levels(H: Operator, k: int, sym: Symmetry, *, floor: int, prune: boo
         kth = found[k - 1] if found.size >= k else math.inf
         if prune and est[j] > kth + max(prune_margin * max(margin_floor, abs(kth)), window):
             pruned += 1
+            continue
+        if j in mirror:      # its partner comes first in the (estimate, index) order
+            if mirror[j] in levels:
+                levels[j] = levels[mirror[j]]
+            else:
+                pruned += 1
             continue
         blk, dim = meta[j]
         (_, sec, m), = _build_blocks(H, [blk], csr_budget, tune=True)
