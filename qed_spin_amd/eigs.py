@@ -35,7 +35,7 @@ from .errors import InvalidRequest
 from .operator import Operator
 from .sector import DEFAULT_CSR_BUDGET, LANE_CSR, LANE_GATHER, DeviceArray, Sector, SectorMatvec, parallel_build
 from .symmetry import Symmetry
-from .thermal import _handles, resolve_device, wait_released, walk_blocks
+from .thermal import _handles, resolve_device, time_reversal_partners, wait_released, walk_blocks
 
 # the automatic dense floor's ceiling (src/engine/internal.h:815)
 AUTO_DENSE_CEILING = 8192
@@ -300,15 +300,7 @@ def _stream_levels(H: Operator, k: int, sym: Symmetry, *, floor: int, prune: boo
     # (time reversal: same representatives and norms, conjugate characters), so the two have one
     # spectrum; the second of each such pair is neither built nor solved: it takes its partner's
     # dimension, estimate and levels.
-    N = H.num_sites
-    partner = {}
-    if H.is_real():
-        where = {(b.n_up, b.momentum, b.parity, b.flip): i for i, b in enumerate(blks)}
-        for i, b in enumerate(blks):
-            if b.momentum is not None and 2 * b.momentum > N:
-                p = where.get((b.n_up, N - b.momentum, b.parity, b.flip))
-                if p is not None and np.allclose(np.asarray(blks[p].group.chars).conj(), b.group.chars):
-                    partner[i] = p
+    partner = time_reversal_partners(H, blks)
 
     def first(b):
         sec = Sector(b.group, b.n_up)

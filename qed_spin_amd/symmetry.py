@@ -80,6 +80,7 @@ class Block:
     momentum: Optional[int] = None
     parity: Optional[int] = None
     flip: Optional[int] = None
+    weight: int = 1      # blocks it stands for (2: it and its time-reversal partner)
 
     @property
     def label(self) -> str:

@@ -13,6 +13,7 @@ variance (thermal.cpp:597-640).
 from __future__ import annotations
 
 import ctypes as C
+import dataclasses
 import os
 import threading
 import time
@@ -162,14 +163,39 @@ def wait_released() -> None:
         _RELEASING.pop().join()
 
 
-def walk_blocks(H: Operator, sym: Symmetry, csr_budget: int = DEFAULT_CSR_BUDGET, tune: bool = True):
+def time_reversal_partners(H: Operator, blks) -> Dict[int, int]:
+    """{index of a momentum-m block with 2m > N: index of its momentum-(N - m) partner} for a real H:
+    time reversal maps the one block onto the complex conjugate of the other (same representatives
+    and norms, conjugate characters), so the two have one spectrum and one trace of any function of H."""
+    partner: Dict[int, int] = {}
+    if not blks or not H.is_real():
+        return partner
+    N = H.num_sites
+    where = {(b.n_up, b.momentum, b.parity, b.flip): i for i, b in enumerate(blks)}
+    for i, b in enumerate(blks):
+        if b.momentum is not None and 2 * b.momentum > N:
+            p = where.get((b.n_up, N - b.momentum, b.parity, b.flip))
+            if p is not None and np.allclose(np.asarray(blks[p].group.chars).conj(), b.group.chars):
+                partner[i] = p
+    return partner
+
+
+def walk_blocks(H: Operator, sym: Symmetry, csr_budget: int = DEFAULT_CSR_BUDGET, tune: bool = True,
+                fold: bool = False):
     """The non-empty blocks of the walk in the reference's order (n_up ascending, then momentum),
     each with its sector operator on the device CSR lane. Sectors and the operators of small
     sectors are built on several host threads at once; operators large enough to time their SpMV
     variants are built one after another, so the timings see an otherwise idle device. tune=False
     (walks whose applies all run through a batched engine, which packs the blocks' CSRs and has
-    kernels of its own): no operator times its variants and all are built on the threads."""
+    kernels of its own): no operator times its variants and all are built on the threads. fold (a
+    real H): the second block of each time-reversal pair is left out and the first stands for both
+    (Block.weight 2), as the reference's walk folds them."""
     blks = sym.blocks(H.num_sites)
+    if fold:
+        partner = time_reversal_partners(H, blks)
+        for p in partner.values():
+            blks[p] = dataclasses.replace(blks[p], weight=2 * blks[p].weight)
+        blks = [b for i, b in enumerate(blks) if i not in partner]
     H._handle()
     wait_released()
     t0 = time.perf_counter()
@@ -193,7 +219,8 @@ def thermal(H: Operator, T, *, sym: Optional[Symmetry] = None, method: str = "ft
             krylov: int = 100, seed: int = 0, device: str = "gpu", dense_max_dim: Optional[int] = None,
             use_graph: bool = True, csr_budget: int = DEFAULT_CSR_BUDGET) -> ThermalResult:
     """FTLM thermodynamics of H at temperatures T over the blocks of `sym` (default: every (Sz, k)
-    block of a ring). seed 0 draws one. dense_max_dim (default 512, the reference's ThermalSpec
+    block of a ring; with a real H each time-reversal pair of momentum blocks is one block of weight
+    2, as the reference's walk folds them). seed 0 draws one. dense_max_dim (default 512, the reference's ThermalSpec
     default): blocks up to this dimension are solved exactly; 0 samples every block. method="exact":
     every block's whole spectrum through the dense lanes (blocks of at most EXACT_MAX_DIM states)."""
     resolve_device(device, "thermal")
@@ -213,7 +240,8 @@ def thermal(H: Operator, T, *, sym: Optional[Symmetry] = None, method: str = "ft
     sym = sym if sym is not None else Symmetry()
     s = int(seed) if seed else int.from_bytes(os.urandom(8), "little")
     t0 = time.perf_counter()
-    blocks = walk_blocks(H, sym, csr_budget, tune=False)   # every apply runs in the batched engine
+    # every apply runs in the batched engine; time-reversal pairs fold into one block of weight 2
+    blocks = walk_blocks(H, sym, csr_budget, tune=False, fold=True)
     t_build = time.perf_counter() - t0
     if not blocks:
         raise InvalidRequest("thermal: the symmetry selection holds no state")
@@ -222,7 +250,8 @@ def thermal(H: Operator, T, *, sym: Optional[Symmetry] = None, method: str = "ft
     recs = []
     for blk, sec, m in blocks:
         s = next_block_seed(s)
-        recs.append({"label": blk.label, "dim": sec.dim, "seed": s, "exact": sec.dim <= dmax, "weight": 1.0})
+        recs.append({"label": blk.label, "dim": sec.dim, "seed": s, "exact": sec.dim <= dmax,
+                     "weight": float(blk.weight)})
     nb, nT = len(blocks), betas.size
     lnZ, E, V, emin = np.zeros((nb, nT)), np.zeros((nb, nT)), np.zeros((nb, nT)), np.zeros(nb)
     ex = [i for i, r in enumerate(recs) if r["exact"]]
@@ -249,6 +278,6 @@ def thermal(H: Operator, T, *, sym: Optional[Symmetry] = None, method: str = "ft
     release_later(blocks)
     return ThermalResult(
         T=T, lnZ=c["lnZ"], E=c["E"], C=c["C"], S=c["S"], F=c["F"], e0=float(np.min(emin)),
-        total_dim=int(sum(r["dim"] for r in recs)), blocks=recs,
+        total_dim=int(sum(r["dim"] * r["weight"] for r in recs)), blocks=recs,
         timings={"build_s": t_build, "exact_s": t_exact, "ftlm_s": t_ftlm, "total_s": time.perf_counter() - t0},
         engine=engine)

@@ -228,14 +228,18 @@ def test_thermal_mixed_walk_matches_oracle():
     res = th.thermal(H, T, sym=qs.Symmetry(), samples=samples, krylov=krylov, seed=seed, dense_max_dim=dmax)
     assert res.total_dim == 1 << N
     assert any(b["exact"] for b in res.blocks) and any(not b["exact"] for b in res.blocks)
-    # the oracle, block by block in the same walk order with the same seed stream
+    # the oracle, block by block in the same walk order with the same seed stream; H is real, so
+    # the momentum-k block with 2k > N is the time reversal of the (N - k) block and folds into it
     seed_b = seed
-    lnZ, E, V = [], [], []
+    lnZ, E, V, wts = [], [], [], []
     for n_up in range(N + 1):
         for k in range(N):
+            if 2 * k > N:
+                continue
             s, A = _oracle_block(H.products, N, n_up, k)
             if s.dim == 0:
                 continue
+            wts.append(1.0 if k in (0, N // 2) else 2.0)
             seed_b = th.next_block_seed(seed_b)
             if s.dim <= dmax:
                 ev = np.linalg.eigvalsh(A)
@@ -265,13 +269,14 @@ def test_thermal_mixed_walk_matches_oracle():
             lnZ.append(l_)
             E.append(e_)
             V.append(v_)
-    lnZ, E, V = np.array(lnZ), np.array(E), np.array(V)
+    lnZ, E, V, wts = np.array(lnZ), np.array(E), np.array(V), np.array(wts)
     assert lnZ.shape[0] == len(res.blocks)
     for i, b in enumerate(res.blocks):
+        assert b["weight"] == wts[i], b["label"]
         assert np.allclose(b["lnZ"], lnZ[i], rtol=1e-8, atol=1e-8), b["label"]
         assert np.allclose(b["E"], E[i], rtol=1e-8, atol=1e-8), b["label"]
     mx = lnZ.max(0)
-    p = np.exp(lnZ - mx)
+    p = wts[:, None] * np.exp(lnZ - mx)
     z = p.sum(0)
     Et = (p * E).sum(0) / z
     var = (p * (V + (E - Et) ** 2)).sum(0) / z
