@@ -35,7 +35,7 @@ from .errors import InvalidRequest
 from .operator import Operator
 from .sector import DEFAULT_CSR_BUDGET, LANE_CSR, LANE_GATHER, DeviceArray, Sector, SectorMatvec, parallel_build
 from .symmetry import Symmetry
-from .thermal import _handles, resolve_device, time_reversal_partners, wait_released, walk_blocks
+from .thermal import _handles, release_later, resolve_device, time_reversal_partners, wait_released, walk_blocks
 
 # the automatic dense floor's ceiling (src/engine/internal.h:815)
 AUTO_DENSE_CEILING = 8192
@@ -477,11 +477,12 @@ class SpectrumResult:
 def spectrum(H: Operator, *, sym: Optional[Symmetry] = None, device: str = "gpu",
              device_min_dim: Optional[int] = None, csr_budget: int = DEFAULT_CSR_BUDGET) -> SpectrumResult:
     """Every eigenvalue of H over the blocks of `sym`, ascending. device_min_dim: blocks from this
-    many states are solved on the device (default 1024; 0 sends every block there)."""
+    many states are solved on the device (default 1024; 0 sends every block there). With a real H a
+    time-reversal pair of momentum blocks is solved once and its eigenvalues counted twice."""
     resolve_device(device, "spectrum")
     sym = sym if sym is not None else Symmetry()
     t0 = time.perf_counter()
-    blocks = walk_blocks(H, sym, csr_budget)
+    blocks = walk_blocks(H, sym, csr_budget, tune=False, fold=True)
     if not blocks:
         raise InvalidRequest("spectrum: the symmetry selection holds no state")
     big = [s.dim for _, s, _ in blocks if s.dim > DENSE_MAX_DIM]
@@ -495,13 +496,16 @@ def spectrum(H: Operator, *, sym: Optional[Symmetry] = None, device: str = "gpu"
     t1 = time.perf_counter()
     _lib.call("qedh_block_spectrum", _handles(mats), len(mats), C.c_int64(dmin), _lib.ptr(vals, C.c_double))
     t_solve = time.perf_counter() - t1
-    recs, at = [], 0
+    recs, at, every = [], 0, []
     lim = DEVICE_DENSE_MIN_DIM if dmin < 0 else dmin
     for (blk, sec, _), d in zip(blocks, dims):
-        recs.append({"label": blk.label, "dim": d, "lane": "device-dense" if (d >= lim or d > 8192) else "host-dense",
+        recs.append({"label": blk.label, "dim": d, "weight": blk.weight,
+                     "lane": "device-dense" if (d >= lim or d > 8192) else "host-dense",
                      "eigenvalues": vals[at: at + d].copy()})
+        every.extend([vals[at: at + d]] * blk.weight)
         at += d
-    return SpectrumResult(eigenvalues=np.sort(vals), blocks=recs,
+    release_later(blocks)
+    return SpectrumResult(eigenvalues=np.sort(np.concatenate(every)), blocks=recs,
                           timings={"build_s": t_build, "solve_s": t_solve, "total_s": time.perf_counter() - t0})
 
 
