@@ -366,7 +366,7 @@ def main():
     ap.add_argument("--no-lanczos", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--lanes", type=int, default=0)
-    ap.add_argument("--workload", default="matvec,serve",
+    ap.add_argument("--workload", default="matvec,serve,refftlm,refeigs",
                     help="matvec (the headline) and a comma list of ftlm, dynamics, serve, spectrum, refdyn, refftlm, refeigs")
     ap.add_argument("--ftlm-sites", type=int, default=24)
     ap.add_argument("--dyn-sites", type=int, default=28)
