@@ -277,6 +277,19 @@ def spin_tower_operator(H: Operator, two_S: int, mu: Optional[float] = None) -> 
     return out
 
 
+def _resident_bytes(N: int, sym: Symmetry) -> float:
+    """A resident walk's device footprint, roughly: per block its dense rank table (4 bytes per state
+    of its Sz subspace) and per state about 48 bytes of representatives, norms and CSR (the blocks'
+    states add up to the selected Sz subspaces, or a share of them under momentum blocks)."""
+    from math import comb
+
+    total = 0.0
+    for b in sym.blocks(N):
+        sub = comb(N, b.n_up) if b.n_up >= 0 else 2 ** N
+        total += 4.0 * sub + 48.0 * sub * b.weight / max(1, b.group.G)
+    return total
+
+
 def _build_blocks(H: Operator, blks, csr_budget: int, tune: bool):
     """(block, sector, operator) of each block, built on the build threads."""
     secs = parallel_build(lambda b: Sector(b.group, b.n_up), blks)
@@ -397,7 +410,7 @@ def _tower(H: Operator, k: int, sym: Symmetry, two_S: int, **kw) -> EigsResult:
 
 def eigs(H: Operator, k: int = 4, *, sym: Optional[Symmetry] = None, device: str = "gpu",
          dense_max_dim: Optional[int] = None, max_iter: int = 0, prune: bool = True, prune_margin: float = 0.02,
-         window: float = 0.0, vectors: bool = False, total_spin=None, stream: bool = False,
+         window: float = 0.0, vectors: bool = False, total_spin=None, stream: Optional[bool] = None,
          csr_budget: int = DEFAULT_CSR_BUDGET) -> EigsResult:
     """The k lowest levels of H over the blocks of `sym` (default: every (Sz, k) block of a ring).
     prune (the reference's default): blocks whose 40-step estimate lies far above the k-th level
@@ -407,7 +420,8 @@ def eigs(H: Operator, k: int = 4, *, sym: Optional[Symmetry] = None, device: str
     times as the reference counts it, solved once in the Sz = S sector of `sym`'s spatial blocks
     under the penalty operator (spin_tower_operator); a level's vector is its Sz = S member.
     stream: hold at most one block per build thread on the device (walks whose blocks do not fit the
-    device together); the estimates, the pruning and the levels are those of the resident walk."""
+    device together); the estimates, the pruning and the levels are those of the resident walk.
+    None: stream when the resident walk's estimated footprint exceeds half the free device memory."""
     resolve_device(device, "eigs")
     if int(k) < 1:
         raise InvalidRequest("eigs: k must be >= 1")
@@ -419,6 +433,8 @@ def eigs(H: Operator, k: int = 4, *, sym: Optional[Symmetry] = None, device: str
                       stream=stream, csr_budget=csr_budget)
     t0 = time.perf_counter()
     floor = dense_floor(k, dense_max_dim)
+    if stream is None:
+        stream = _resident_bytes(H.num_sites, sym) > 0.5 * _lib.device_memory()[0]
     if stream:
         meta, levels, pruned, blocks, t_build = _stream_levels(H, k, sym, floor=floor, prune=prune,
                                                                prune_margin=prune_margin, window=window,
