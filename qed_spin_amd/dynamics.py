@@ -28,7 +28,7 @@ from .errors import DeviceUnsupported, InvalidRequest
 from .operator import Operator, Program
 from .sector import parallel_build, DEFAULT_CSR_BUDGET, LANE_GATHER, Sector, SectorMatvec
 from .symmetry import Symmetry
-from .thermal import KBREAKDOWN_REL, _handles, resolve_device, walk_blocks
+from .thermal import KBREAKDOWN_REL, _handles, release_later, resolve_device, walk_blocks
 
 
 def sz_q(N: int, q: float, sign: int = -1) -> Operator:
@@ -232,6 +232,7 @@ def dynamics(H: Operator, probes, omega, *, eta: float = 0.1, T=None, sym: Optio
     sym = sym if sym is not None else Symmetry()
     t0 = time.perf_counter()
     finite_T = T is not None and len(np.atleast_1d(T)) > 0
+    own = blocks is None
     if blocks is None:
         # T = 0: the continued fractions run in the batched engine and the ground search's applies are
         # few, so the blocks keep the default SpMV shape; T > 0 applies every source block many times
@@ -243,6 +244,8 @@ def dynamics(H: Operator, probes, omega, *, eta: float = 0.1, T=None, sym: Optio
         if np.any(~(np.atleast_1d(T) > 0)):
             raise InvalidRequest("dynamics: temperatures must be positive")
         S, ref = _dynamics_finite_T(H, probes, omega, T, eta, blocks, krylov, samples, seed, sources)
+        if own:
+            release_later(blocks)
         return DynamicsResult(omega=omega, S=S, e0=ref, ground_manifold=0, target_sectors=0,
                               timings={"walk_s": t_walk, "total_s": time.perf_counter() - t0})
     sH = H.norm_bound() or 1.0
@@ -304,7 +307,11 @@ def dynamics(H: Operator, probes, omega, *, eta: float = 0.1, T=None, sym: Optio
         engine = dict(zip(["pack_s", "steps_s", "host_s", "total_s", "applies", "rows", "nnz"], stats.tolist()))
     for p in range(len(probes)):
         S[p] /= float(len(states))
+    n_states = len(states)
+    states = images = keep = None
+    if own:
+        release_later(blocks)
     return DynamicsResult(
-        omega=omega, S=S, e0=e0, ground_manifold=len(states), target_sectors=len(targets),
+        omega=omega, S=S, e0=e0, ground_manifold=n_states, target_sectors=len(targets),
         timings={"walk_s": t_walk, "ground_s": t_gm, "probe_s": t_probe, "total_s": time.perf_counter() - t0},
         engine=engine)

@@ -476,6 +476,8 @@ def eigs(H: Operator, k: int = 4, *, sym: Optional[Symmetry] = None, device: str
             L["vector"] = len(vecs)
             vecs.append((i, got[i][t]))
         lvls.append(L)
+    if isinstance(blocks, list):
+        release_later(blocks)      # the levels' bases keep the sectors they need
     return EigsResult(energies=energies, blocks=recs, converged=converged,
                       timings={"build_s": t_build, "solve_s": t_solve, "pruned_blocks": pruned,
                                "vectors_s": time.perf_counter() - t2 if vectors else 0.0,
