@@ -28,7 +28,7 @@ from .errors import DeviceUnsupported, InvalidRequest
 from .operator import Operator, Program
 from .sector import parallel_build, DEFAULT_CSR_BUDGET, LANE_GATHER, Sector, SectorMatvec
 from .symmetry import Symmetry
-from .thermal import KBREAKDOWN_REL, _handles, release_later, resolve_device, walk_blocks
+from .thermal import KBREAKDOWN_REL, _handles, release_later, resolve_device, time_reversal_partners, walk_blocks
 
 
 def sz_q(N: int, q: float, sign: int = -1) -> Operator:
@@ -72,21 +72,29 @@ class DynamicsResult:
     engine: Dict[str, float] = field(default_factory=dict)
 
 
-def ground_manifold(blocks, window: float, k_max: int = 16, only=None, prune: bool = True, norm_bound: float = 1.0):
+def ground_manifold(blocks, window: float, k_max: int = 16, only=None, prune: bool = True, norm_bound: float = 1.0,
+                    partners=None):
     """(E0, [(block index, device buffer, vector index, energy)]): every level within `window` of the
     lowest level of the walk (src/engine/dynamics.cpp:147-193). Each block's lowest level comes from a
     k = 1 eigs walk with the window (blocks whose 40-step estimate lies far above are pruned, as the
     reference prunes by default); the blocks within reach of the lowest then give their eigenpairs
     (Krylov-Schur, k doubled until a level outside the window shows or k_max), and E0 is the lowest
-    eigenvalue among them."""
+    eigenvalue among them. partners {mirror block: block} (time-reversal pairs of a real H): a mirror
+    block is not scanned; it takes its partner's lowest level (its eigenpairs, when within reach, are
+    its own)."""
     from .eigs import dense_floor, walk_levels
 
-    idx = [i for i in range(len(blocks)) if only is None or i in only]
+    partners = {} if partners is None else partners
+    chosen = [i for i in range(len(blocks)) if only is None or i in only]
+    idx = [i for i in chosen if not (i in partners and partners[i] in chosen)]
     lv, _ = walk_levels([blocks[i] for i in idx], 1, floor=dense_floor(1, None), prune=prune, window=window,
                         norm_bound=norm_bound)
     lows = [math.inf] * len(blocks)
     for q, (v, _, _) in lv.items():
         lows[idx[q]] = float(v[0]) if len(v) else math.inf
+    for i in chosen:
+        if i not in idx:
+            lows[i] = lows[partners[i]]
     e_scan = min(lows)
     slack = window + 1e-9 * max(1.0, abs(e_scan))
     found = []
@@ -251,8 +259,9 @@ def dynamics(H: Operator, probes, omega, *, eta: float = 0.1, T=None, sym: Optio
     sH = H.norm_bound() or 1.0
     window = degeneracy_tol * sH
     t1 = time.perf_counter()
+    partners = time_reversal_partners(H, [b for b, _, _ in blocks])
     e0, states = ground_manifold(blocks, window, only=None if ground_blocks is None else set(ground_blocks),
-                                 norm_bound=sH)
+                                 norm_bound=sH, partners=partners)
     t_gm = time.perf_counter() - t1
     # probe images: (state, probe) -> every block it reaches
     t2 = time.perf_counter()
