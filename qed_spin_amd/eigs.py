@@ -83,7 +83,10 @@ class EigsResult:
         (its states in ascending order); levels of another n_up have no component there and are
         left out (the reference's EigResult.vectors, python/qed/_verbs/eigs.py:52-78). Vectors of
         one degenerate level come from one block's eigensolver and are orthonormal; vectors of
-        different blocks are orthogonal by their quantum numbers."""
+        different blocks are orthogonal by their quantum numbers. A level of a spin tower
+        (total_spin=S) is a whole multiplet: basis="full" gives its members from Sz = S down (as many as
+        the k energies still need), basis="sz" the member of that n_up, each lowered from the solved
+        Sz = S member by S^-."""
         self._need_vectors()
         if basis not in ("full", "sz"):
             raise InvalidRequest("basis must be 'full' or 'sz'")
@@ -95,6 +98,17 @@ class EigsResult:
                 continue
             key, v = self._vectors[L["vector"]]
             B = self._bases[key]
+            if "total_spin" in L and L.get("multiplicity", 1) > 1:
+                bottom = self.n_sites - B.n_up          # n_up of the Sz = -S member
+                if basis == "full":
+                    if len(out) >= self.k:
+                        break
+                    lowest = max(bottom, B.n_up - (self.k - len(out)) + 1)
+                    out.extend(lv.tower_members(B, v, True, lowest, self._cache.setdefault(("tower", key), {})))
+                elif bottom <= int(n_up) <= B.n_up:
+                    out.append(lv.tower_members(B, v, False, int(n_up),
+                                                self._cache.setdefault(("tower", key), {}))[-1])
+                continue
             if basis == "sz" and B.n_up != int(n_up):
                 continue
             out.extend(lv.expand(B, [v], full=(basis == "full")))

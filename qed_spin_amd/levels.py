@@ -222,6 +222,51 @@ def correlations_in(basis: LevelBasis, vec: np.ndarray, A: Sequence[Operator], B
     return Cm
 
 
+def tower_members(basis: LevelBasis, vec: np.ndarray, full: bool, lowest_n_up: int, cache: Dict) -> List[np.ndarray]:
+    """The members of a spin multiplet from its top member v (the sector vector at n_up = basis.n_up):
+    v expanded into its Sz sector, then S^- = sum_i S^-_i applied on the device (gather lane between
+    the Sz sectors n and n - 1) down to n_up = lowest_n_up, each member normalised (the reference's
+    multiplet, src/engine/eigs.cpp:749-860, total S^- for an SU(2) multiplet). Returned top first, in
+    the Sz sectors (states ascending) or, full, in the 2^N space."""
+    N = basis.N
+    psi = expand(basis, [vec], full=False)[0]
+    secs = cache.setdefault("sz", {})
+
+    def sz_sector(n):
+        if n not in secs:
+            secs[n] = Sector(trivial_group(N), n)
+        return secs[n]
+
+    def place(n, p):
+        if not full:
+            return p
+        reps, _ = sz_sector(n).export()
+        out = np.zeros(1 << N, np.complex128)
+        out[reps.astype(np.int64)] = p
+        return out
+
+    members = [place(basis.n_up, psi)]
+    lower = cache.get("S-")
+    if lower is None:
+        lower = Operator(N)
+        for i in range(N):
+            lower.add_product("-", (i,), 1.0)
+        cache["S-"] = lower
+    x = DeviceArray.from_host(psi)
+    for n in range(basis.n_up, lowest_n_up, -1):
+        tgt = sz_sector(n - 1)
+        y = DeviceArray(tgt.dim)
+        SectorMatvec(lower, tgt, sz_sector(n), lane=LANE_GATHER).apply(x, y, complex_vectors=True)
+        p = y.to_host()
+        nrm = np.linalg.norm(p)
+        if not nrm > 0.0:
+            raise InvalidRequest("vectors: S^- annihilated a member above the bottom of its multiplet")
+        p /= nrm
+        members.append(place(n - 1, p))
+        x = DeviceArray.from_host(p)
+    return members
+
+
 # ---- the .npz file ----------------------------------------------------------------------------------
 
 def to_arrays(result) -> Dict[str, np.ndarray]:

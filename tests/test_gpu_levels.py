@@ -274,3 +274,34 @@ def test_total_spin_against_the_dense_spin_subspace(total_spin):
         ev.eigs(H, 2, total_spin=1.0)            # not a spin of 9 spins 1/2
     with pytest.raises(errors.InvalidRequest):
         ev.eigs(H, 2, total_spin=0.5, sym=qs.Symmetry(sz=[4]))
+
+
+@pytest.mark.parametrize("total_spin", [0.5, 1.5])
+def test_tower_vectors_complete_the_multiplets(total_spin):
+    """vectors() of a spin tower: the k = 4 vectors are the multiplets' members from Sz = S down,
+    lowered by S^- on the device -- orthonormal eigenvectors of the dense H with the returned
+    energies, <S^2> = S(S+1), <S^z> stepping down by one; basis='sz' gives the member of that n_up,
+    the full vector restricted to it."""
+    say("tower vectors", total_spin)
+    N = 9
+    H = _ring(N, 0.3)
+    r = ev.eigs(H, 4, total_spin=total_spin, vectors=True)
+    V = np.array(r.vectors("full")).T
+    assert V.shape == (1 << N, 4)
+    assert np.abs(V.conj().T @ V - np.eye(4)).max() < 1e-10
+    Hd = _dense(H, N)
+    S2 = _dense(ev.spin_squared(N), N)
+    Sz = _dense(qs.Operator(N).add_terms([(1.0, (("z", i),)) for i in range(N)]), N)
+    lam = total_spin * (total_spin + 1)
+    two_S = int(2 * total_spin)
+    for i in range(4):
+        v = V[:, i]
+        e = r.energies[i]
+        assert np.linalg.norm(Hd @ v - e * v) < 1e-8, i
+        assert abs(np.vdot(v, S2 @ v).real - lam) < 1e-9, i
+        assert abs(np.vdot(v, Sz @ v).real - (total_spin - (i % (two_S + 1)))) < 1e-9, i
+    n_low = (N + two_S) // 2 - 1            # one step below the top member
+    idx = np.array([x for x in range(1 << N) if bin(x).count("1") == n_low])
+    sz = r.vectors("sz", n_up=n_low)
+    assert len(sz) == len(r.levels)
+    assert np.abs(sz[0] - V[idx, 1]).max() < 1e-12
