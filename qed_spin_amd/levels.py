@@ -324,7 +324,9 @@ def to_arrays(result) -> Dict[str, np.ndarray]:
     d["spec_residues"] = np.zeros(0, np.int64)
     d["spec_only_k0"] = np.zeros(0, np.int64)
     d["spec_only_irrep"] = np.zeros(0, np.int64)
-    d["spec_scalars"] = np.array([-1, 0, 1, 0, 0, -1, result.n_sites], np.int64)
+    towers = [L["total_spin"] for L in levels if "total_spin" in L]
+    two_S = int(round(2 * towers[0])) if towers else -1   # spec_scalars[5]: 2S of a total_spin request
+    d["spec_scalars"] = np.array([-1, 0, 1, 0, 0, two_S, result.n_sites], np.int64)
     total = int(sum(r["dim"] for r in result.blocks))
     d["result_scalars"] = np.array([total, 0, int(bool(result.converged)), 0, 0, 0,
                                     int(result.timings.get("pruned_blocks", 0))], np.int64)
@@ -416,7 +418,9 @@ def from_arrays(d: Dict[str, np.ndarray]):
         vectors.append((int(vbasis[i]), v))
     if np.any(vector < -1) or np.any(vector >= len(vectors)):
         _fail("a level names a vector that is not in the file")
-    n_sites = int(_get(d, "spec_scalars", np.int64)[6]) if "spec_scalars" in d else (bases[0].N if bases else 0)
+    spec = _get(d, "spec_scalars", np.int64) if "spec_scalars" in d else np.zeros(0, np.int64)
+    n_sites = int(spec[6]) if spec.size > 6 else (bases[0].N if bases else 0)
+    two_S = int(spec[5]) if spec.size > 5 else -1
     levels = []
     for i in range(nl):
         t = tag[i]
@@ -426,5 +430,7 @@ def from_arrays(d: Dict[str, np.ndarray]):
         levels.append({"energy": float(energy[i]), "n_up": int(t[0]), "momentum": mom,
                        "parity": None if t[5] < 0 else (1 if t[5] == 0 else -1), "dim": int(t[9]),
                        "multiplicity": int(mult[i]), "vector": int(vector[i]), "label": f"n_up={int(t[0])}"})
+        if two_S >= 0:
+            levels[-1]["total_spin"] = two_S / 2
     rs = _get(d, "result_scalars", np.int64) if "result_scalars" in d else np.zeros(7, np.int64)
     return levels, bases, vectors, n_sites, rs

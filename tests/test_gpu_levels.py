@@ -305,3 +305,22 @@ def test_tower_vectors_complete_the_multiplets(total_spin):
     sz = r.vectors("sz", n_up=n_low)
     assert len(sz) == len(r.levels)
     assert np.abs(sz[0] - V[idx, 1]).max() < 1e-12
+
+
+def test_tower_result_survives_the_file(tmp_path):
+    """A total_spin result saved and loaded: 2S travels in spec_scalars[5], the levels keep their
+    multiplicities, and the loaded result's vectors() completes the multiplets as the original's."""
+    say("tower file")
+    N = 9
+    r = ev.eigs(_ring(N, 0.3), 4, total_spin=0.5, vectors=True)
+    p = tmp_path / "tower.npz"
+    r.save(p)
+    with np.load(p) as f:
+        assert int(f["spec_scalars"][5]) == 1
+        assert list(f["level_multiplicity"]) == [2] * len(r.levels)
+    s = ev.load_eigs(p)
+    assert all(L["total_spin"] == 0.5 for L in s.levels)
+    a, b = r.vectors("full"), s.vectors("full")
+    assert len(a) == len(b) == 4
+    for x, y in zip(a, b):
+        assert np.abs(x - y).max() < 1e-13
