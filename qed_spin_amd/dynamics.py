@@ -98,9 +98,10 @@ def ground_manifold(blocks, window: float, k_max: int = 16, only=None, prune: bo
     e_scan = min(lows)
     slack = window + 1e-9 * max(1.0, abs(e_scan))
     found = []
-    for i, (blk, sec, m) in enumerate(blocks):
+    for i in range(len(blocks)):
         if lows[i] > e_scan + slack:
             continue
+        blk, sec, m = blocks[i]
         k = 2
         while True:
             k = min(k, sec.dim)
@@ -244,7 +245,8 @@ def dynamics(H: Operator, probes, omega, *, eta: float = 0.1, T=None, sym: Optio
     if blocks is None:
         # T = 0: the continued fractions run in the batched engine and the ground search's applies are
         # few, so the blocks keep the default SpMV shape; T > 0 applies every source block many times
-        blocks = walk_blocks(H, sym, csr_budget, tune=finite_T)
+        # (T = 0 builds a block's operator only when the ground search or a probe image needs it)
+        blocks = walk_blocks(H, sym, csr_budget, tune=finite_T, lazy=not finite_T)
     if not blocks:
         raise InvalidRequest("dynamics: the symmetry selection holds no state")
     t_walk = time.perf_counter() - t0

@@ -163,6 +163,27 @@ def wait_released() -> None:
         _RELEASING.pop().join()
 
 
+class LazyMatvec:
+    """A block's sector operator built when first used (any attribute): a walk whose blocks are not
+    all applied -- a T = 0 dynamics run needs the ground search's blocks and the probes' targets --
+    builds only those. Safe to first use from several threads."""
+
+    def __init__(self, make):
+        self._make = make
+        self._m = None
+        self._lock = threading.Lock()
+
+    def _get(self):
+        if self._m is None:
+            with self._lock:
+                if self._m is None:
+                    self._m = self._make()
+        return self._m
+
+    def __getattr__(self, name):
+        return getattr(self._get(), name)
+
+
 def time_reversal_partners(H: Operator, blks) -> Dict[int, int]:
     """{index of a momentum-m block with 2m > N: index of its momentum-(N - m) partner} for a real H:
     time reversal maps the one block onto the complex conjugate of the other (same representatives
@@ -181,7 +202,7 @@ def time_reversal_partners(H: Operator, blks) -> Dict[int, int]:
 
 
 def walk_blocks(H: Operator, sym: Symmetry, csr_budget: int = DEFAULT_CSR_BUDGET, tune: bool = True,
-                fold: bool = False):
+                fold: bool = False, lazy: bool = False):
     """The non-empty blocks of the walk in the reference's order (n_up ascending, then momentum),
     each with its sector operator on the device CSR lane. Sectors and the operators of small
     sectors are built on several host threads at once; operators large enough to time their SpMV
@@ -189,7 +210,8 @@ def walk_blocks(H: Operator, sym: Symmetry, csr_budget: int = DEFAULT_CSR_BUDGET
     (walks whose applies all run through a batched engine, which packs the blocks' CSRs and has
     kernels of its own): no operator times its variants and all are built on the threads. fold (a
     real H): the second block of each time-reversal pair is left out and the first stands for both
-    (Block.weight 2), as the reference's walk folds them."""
+    (Block.weight 2), as the reference's walk folds them. lazy: the operators are LazyMatvec, built
+    on first use."""
     blks = sym.blocks(H.num_sites)
     if fold:
         partner = time_reversal_partners(H, blks)
@@ -205,6 +227,9 @@ def walk_blocks(H: Operator, sym: Symmetry, csr_budget: int = DEFAULT_CSR_BUDGET
     mats = [None] * len(keep)
     small = [i for i, (_, s) in enumerate(keep) if s.dim < AUTOTUNE_MIN_ROWS or not tune]
     make = lambda i: SectorMatvec(H, keep[i][1], lane=LANE_CSR, csr_budget=csr_budget, tune=tune)
+    if lazy:
+        mats = [LazyMatvec(lambda i=i: make(i)) for i in range(len(keep))]
+        small = []
     for i, m in zip(small, parallel_build(make, small)):
         mats[i] = m
     for i in range(len(keep)):
