@@ -185,18 +185,21 @@ def _solve_block(m, dim: int, want: int, floor: int, max_iter: int):
 
 
 def walk_levels(blocks, k: int, *, floor: int, prune: bool = True, prune_margin: float = 0.02, window: float = 0.0,
-                norm_bound: float = 1.0, max_iter: int = 0):
+                norm_bound: float = 1.0, max_iter: int = 0, partners=None):
     """The levels of the blocks of a walk as the reference's eigs walk finds them
     (src/engine/eigs.cpp:360-560): blocks of at most max(min(dense floor, 64), 2) states are solved
     outright; every larger block first gets its pruning estimate (40 Lanczos steps, theta_1 - |r_1|,
     on host threads with streams of their own), then the blocks are taken in order of increasing
     estimate and solved only when the estimate lies within prune_margin (relative to |E_k|, floored at
     5% of s_H) or `window` of the k-th level found so far. Returns {block index: (levels, converged,
-    lane)} for the blocks solved and the number pruned."""
+    lane)} for the blocks solved and the number pruned. partners {mirror: block} (time-reversal pairs
+    of a real H, indices into blocks): a mirror is neither estimated nor solved; it takes its
+    partner's estimate and levels (its partner precedes it in the (estimate, index) order)."""
+    partners = {} if partners is None else partners
     levels = {}
     small = max(min(floor, PRUNE_MIN_DIM), 2)
-    outright = [i for i, (_, s, _) in enumerate(blocks) if not prune or s.dim <= small]
-    cand = [i for i, (_, s, _) in enumerate(blocks) if prune and s.dim > small]
+    outright = [i for i, (_, s, _) in enumerate(blocks) if (not prune or s.dim <= small) and i not in partners]
+    cand = [i for i, (_, s, _) in enumerate(blocks) if prune and s.dim > small and i not in partners]
     dense = [i for i in outright if blocks[i][1].dim <= 2 or blocks[i][1].dim <= floor]
     if dense:
         mats = [blocks[i][2] for i in dense]
@@ -212,7 +215,14 @@ def walk_levels(blocks, k: int, *, floor: int, prune: bool = True, prune_margin:
         if i not in levels:
             blk, sec, m = blocks[i]
             levels[i] = _solve_block(m, sec.dim, min(k, sec.dim), floor, max_iter)
+    for j, p in partners.items():      # mirrors of blocks solved outright
+        if p in levels:
+            levels[j] = levels[p]
     est = parallel_build(lambda i: blocks[i][2].estimate(), cand)
+    for j, p in partners.items():      # mirrors of candidates share their estimate
+        if p in cand:
+            cand.append(j)
+            est.append(est[cand.index(p)])
     order = sorted(range(len(cand)), key=lambda q: (est[q], cand[q]))
     margin_floor = 0.05 * (norm_bound if norm_bound > 0 else 1.0)
     pruned = 0
@@ -222,6 +232,12 @@ def walk_levels(blocks, k: int, *, floor: int, prune: bool = True, prune_margin:
         kth = found[k - 1] if found.size >= k else math.inf
         if est[q] > kth + max(prune_margin * max(margin_floor, abs(kth)), window):
             pruned += 1
+            continue
+        if i in partners:
+            if partners[i] in levels:
+                levels[i] = levels[partners[i]]
+            else:
+                pruned += 1
             continue
         blk, sec, m = blocks[i]
         levels[i] = _solve_block(m, sec.dim, min(k, sec.dim), floor, max_iter)
@@ -455,14 +471,16 @@ def eigs(H: Operator, k: int = 4, *, sym: Optional[Symmetry] = None, device: str
                                                                max_iter=max_iter, csr_budget=csr_budget)
         t1 = t0 + t_build
     else:
-        blocks = walk_blocks(H, sym, csr_budget)
+        # operators built on first use: time-reversal mirrors and pruned blocks' solves need none
+        blocks = walk_blocks(H, sym, csr_budget, tune=False, lazy=True)
         if not blocks:
             raise InvalidRequest("eigs: the symmetry selection holds no state")
         meta = [(blk, sec.dim) for blk, sec, _ in blocks]
+        partners = time_reversal_partners(H, [b for b, _, _ in blocks])
         t_build = time.perf_counter() - t0
         t1 = time.perf_counter()
         levels, pruned = walk_levels(blocks, k, floor=floor, prune=prune, prune_margin=prune_margin, window=window,
-                                     norm_bound=H.norm_bound(), max_iter=max_iter)
+                                     norm_bound=H.norm_bound(), max_iter=max_iter, partners=partners)
     t_solve = time.perf_counter() - t1
     recs, converged = [], True
     for i, (blk, dim) in enumerate(meta):
