@@ -53,11 +53,54 @@ int qedh_vdot(const void* x, const void* y, int64_t n, int complex_vectors, doub
 int qedh_memset(void* dst, int value, size_t bytes);
 int qedh_synchronize(void);
 /* The calling thread's stream for every engine launch, copy and allocation: own != 0 a non-blocking
- * stream of the thread's own (created on first use, destroyed when the thread ends), 0 the null
- * stream. Calls that return results synchronise their stream first, so objects built on one
- * thread's stream are complete when the call returns. Lets host threads build sectors and
- * operators concurrently (the reference builds blocks one after another; no counterpart). */
+ * stream of the engine's pool (taken on first use, handed back drained -- never destroyed -- when
+ * the thread ends), 0 the null stream. Calls that return results synchronise their stream first,
+ * so objects built on one thread's stream are complete when the call returns; an object dropped on
+ * any thread frees its device memory only after the work every engine stream has enqueued so far.
+ * Lets host threads build sectors and operators concurrently (the reference keeps every thread on
+ * the legacy default stream, include/ed/matvec/batcher.h:21-23; no counterpart). */
 int qedh_thread_stream(int own);
+/* Cross-stream fences taken so far (one per shared object released) and streams in the pool
+ * (no counterpart; tests of the release rule). Either pointer may be NULL. */
+int qedh_stream_stats(uint64_t* fences, int64_t* pooled_streams);
+
+/* ---- the Krylov drivers' vector backend on device pointers (BasicBackend<Scalar>,
+ * include/ed/matvec/backend.h:56-179; its device implementation CudaBackend,
+ * include/ed/gpu/cuda_backend.cuh:104-415). complex_vectors != 0: Scalar = complex<double>
+ * (interleaved re, im; scalars passed as double[2]), else double (scalars double[1]). Each call
+ * runs on the calling thread's engine stream and returns when done; reductions are deterministic
+ * (fixed grid and order). ------------------------------------------------------------------- */
+/* BasicBackend::fill_zero (backend.h:73). */
+int qedh_fill_zero(int complex_vectors, void* x, size_t n);
+/* y <- alpha x + y (BasicBackend::axpy, backend.h:89). */
+int qedh_axpy(int complex_vectors, const double* alpha, const void* x, void* y, size_t n);
+/* x <- alpha x (BasicBackend::scale, backend.h:90). */
+int qedh_scale(int complex_vectors, const double* alpha, void* x, size_t n);
+/* y <- alpha x + beta y (BasicBackend::axpby, backend.h:96). */
+int qedh_axpby(int complex_vectors, const double* alpha, const void* x, const double* beta, void* y, size_t n);
+/* out = x^H y, conjugate on the left (BasicBackend::dot, backend.h:91). */
+int qedh_dot(int complex_vectors, const void* x, const void* y, size_t n, double* out);
+/* out = ||x||_2 (BasicBackend::nrm2, backend.h:92). */
+int qedh_nrm2(int complex_vectors, const void* x, size_t n, double* out);
+/* y <- y + alpha x, then out = z^H y, one pass (BasicBackend::axpy_dot, backend.h:108-112). */
+int qedh_axpy_dot(int complex_vectors, const double* alpha, const void* x, void* y, const void* z, size_t n,
+                  double* out);
+/* y <- y + alpha x, then out = ||y||_2, one pass (BasicBackend::axpy_nrm2, backend.h:113-116). */
+int qedh_axpy_nrm2(int complex_vectors, const double* alpha, const void* x, void* y, size_t n, double* out);
+/* coeffs_out[j] = basis[j]^H v for j < num_basis; basis: a HOST array of DEVICE vectors, coeffs_out
+ * a host array of num_basis scalars; one pass over v feeds every column (BasicBackend::dot_many,
+ * backend.h:133-137). */
+int qedh_dot_many(int complex_vectors, const void* const* basis, size_t num_basis, const void* v, size_t n,
+                  double* coeffs_out);
+/* v += sum_j alphas[j] basis[j] (host alphas, host array of device vectors; BasicBackend::axpy_many,
+ * backend.h:141-145). */
+int qedh_axpy_many(int complex_vectors, const double* alphas, const void* const* basis, size_t num_basis, void* v,
+                   size_t n);
+/* C = alpha op(A) op(B) + beta C, column-major device matrices, op 'N' / 'T' / 'C' (BasicBackend::gemm,
+ * backend.h:156-161): the library GEMM (rocBLAS zgemm / dgemm) on the engine stream. alpha / beta NULL:
+ * 1 / 0. */
+int qedh_gemm(int complex_vectors, char op_a, char op_b, size_t m, size_t n, size_t k, const double* alpha,
+              const void* A, size_t lda, const void* B, size_t ldb, const double* beta, void* C, size_t ldc);
 
 /* ---- operators: canonical spin-1/2 terms (src/ops/algebra.cpp, include/ed/ops/algebra.h) ----- */
 int qedh_operator_create(int n_sites, qedh_operator* out);
@@ -187,7 +230,8 @@ int qedh_matvec_export_csr_rows(qedh_matvec m, int64_t r0, int64_t r1, uint64_t*
 int qedh_matvec_destroy(qedh_matvec m);
 /* Average seconds of one apply over `iters` timed applies (after `warmup`) of k vectors at once
  * (k = 1: the single apply; 2..8 multi-vector; block != 0: block-interleaved), measured with HIP
- * events on the stream the kernels run on. Vectors are allocated and filled inside. */
+ * events on the stream the kernels run on. Vectors are allocated inside and filled with random unit
+ * vectors (Gaussian entries, normalised; the block form normalises the whole block). */
 int qedh_matvec_benchmark(qedh_matvec m, int k, int block, int complex_vectors, int warmup, int iters,
                           double* avg_seconds);
 

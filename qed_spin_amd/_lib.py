@@ -56,6 +56,21 @@ _SIGNATURES = {
     "qedh_synchronize": [],
     "qedh_thread_stream": [C.c_int],
     "qedh_vdot": [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, c_dbl_p],
+    "qedh_stream_stats": [c_u64_p, c_i64_p],
+    "qedh_fill_zero": [C.c_int, C.c_void_p, C.c_size_t],
+    "qedh_axpy": [C.c_int, c_dbl_p, C.c_void_p, C.c_void_p, C.c_size_t],
+    "qedh_scale": [C.c_int, c_dbl_p, C.c_void_p, C.c_size_t],
+    "qedh_axpby": [C.c_int, c_dbl_p, C.c_void_p, c_dbl_p, C.c_void_p, C.c_size_t],
+    "qedh_dot": [C.c_int, C.c_void_p, C.c_void_p, C.c_size_t, c_dbl_p],
+    "qedh_nrm2": [C.c_int, C.c_void_p, C.c_size_t, c_dbl_p],
+    "qedh_axpy_dot": [C.c_int, c_dbl_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, c_dbl_p],
+    "qedh_axpy_nrm2": [C.c_int, c_dbl_p, C.c_void_p, C.c_void_p, C.c_size_t, c_dbl_p],
+    "qedh_dot_many": [C.c_int, C.POINTER(C.c_void_p), C.c_size_t, C.c_void_p, C.c_size_t, c_dbl_p],
+    "qedh_axpy_many": [C.c_int, c_dbl_p, C.POINTER(C.c_void_p), C.c_size_t, C.c_void_p, C.c_size_t],
+    "qedh_gemm": [
+        C.c_int, C.c_char, C.c_char, C.c_size_t, C.c_size_t, C.c_size_t, c_dbl_p, C.c_void_p, C.c_size_t,
+        C.c_void_p, C.c_size_t, c_dbl_p, C.c_void_p, C.c_size_t,
+    ],
     "qedh_operator_create": [C.c_int, C.POINTER(C.c_void_p)],
     "qedh_operator_add_product": [C.c_void_p, C.c_char_p, c_int_p, C.c_double, C.c_double],
     "qedh_operator_info": [C.c_void_p, c_i64_p, c_dbl_p, c_int_p],

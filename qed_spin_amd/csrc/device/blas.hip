@@ -66,6 +66,25 @@ template <bool C> __global__ void k_fill_zero(void* x, std::size_t n) {
         V<C>::store(x, i, Zd{0.0, 0.0});
 }
 
+// Entry i of a Gaussian test vector: two uniforms from splitmix64 of (seed, i), Box-Muller.
+__device__ __forceinline__ u64 mix64(u64 z) {
+    z += 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+template <bool C> __global__ void k_fill_gauss(void* x, std::size_t n, u64 seed) {
+    for (std::size_t i = blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+        const u64 a = mix64(seed ^ (2 * i)), b = mix64(seed ^ (2 * i + 1));
+        const double u1 = (static_cast<double>(a >> 11) + 0.5) * 0x1.0p-53;
+        const double u2 = static_cast<double>(b >> 11) * 0x1.0p-53;
+        const double r = sqrt(-2.0 * log(u1));
+        const double t = 6.283185307179586 * u2;
+        V<C>::store(x, i, Zd{r * cos(t), r * sin(t)});
+    }
+}
+
 template <bool C> __global__ void k_axpby(Zd a, const void* x, Zd b, void* y, std::size_t n, int mode) {
     // mode 0: y += a x; 1: y *= a; 2: y = a x + b y
     for (std::size_t i = blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x; i < n;
@@ -136,6 +155,17 @@ __global__ __launch_bounds__(kThreads) void k_finish(const double* partial, int 
     finish_pairs(partial, nb, out_kind, out);
 }
 
+// The columns of a Gram-Schmidt block: column j at base + j stride (bytes), or table[j] when a
+// table of device pointers is given (the reference's array-of-vectors form).
+struct Cols {
+    const char* base;
+    std::size_t stride;
+    const void* const* table;
+    __device__ __forceinline__ const void* at(int j) const {
+        return table ? table[j] : static_cast<const void*>(base + static_cast<std::size_t>(j) * stride);
+    }
+};
+
 // dot_many: block b owns a row range; the rows come in tiles of kTile staged in LDS (the w tile),
 // wavefront v owns the columns j = v, v + 4, ...: its lanes read 64 consecutive rows of column j
 // per load (coalesced), sum, and the wave total accumulates into the column's LDS slot (one owner:
@@ -144,7 +174,7 @@ constexpr int kManyCols = 256;   // columns per launch
 constexpr int kTile = 1024;      // rows per LDS tile
 
 template <bool C>
-__global__ __launch_bounds__(kThreads) void k_dot_many(const void* Vb, std::size_t ld, int m, const void* w,
+__global__ __launch_bounds__(kThreads) void k_dot_many(Cols Vb, int m, const void* w,
                                                        std::size_t n, std::size_t rows_per_block, double* partial,
                                                        int nb, int want_norm) {
     __shared__ Zd sw[kTile];
@@ -164,11 +194,11 @@ __global__ __launch_bounds__(kThreads) void k_dot_many(const void* Vb, std::size
         }
         __syncthreads();
         for (int j = wv; j < m; j += kWaves) {
-            const std::size_t base = static_cast<std::size_t>(j) * ld + t0;
+            const void* cj = Vb.at(j);
             double pr = 0.0, pi = 0.0;
 #pragma unroll 4
             for (int i = ln; i < cnt; i += 64) {
-                const Zd vj = V<C>::load(Vb, base + i);
+                const Zd vj = V<C>::load(cj, t0 + i);
                 if constexpr (C) {
                     const Zd d = cdot(vj, sw[i]);
                     pr += d.re;
@@ -205,7 +235,7 @@ __global__ __launch_bounds__(kThreads) void k_dot_many_finish(const double* part
 
 // w += sign sum_j c_j V_j over grid-strided rows, four columns per step; the partial of ||w||^2 after.
 template <bool C>
-__global__ __launch_bounds__(kThreads) void k_axpy_many(const Zd* coeffs, const void* Vb, std::size_t ld, int m,
+__global__ __launch_bounds__(kThreads) void k_axpy_many(const Zd* coeffs, Cols Vb, int m,
                                                         void* w, std::size_t n, double* partial, int want_norm,
                                                         double sign) {
     __shared__ Zd sc[kManyCols];
@@ -217,10 +247,10 @@ __global__ __launch_bounds__(kThreads) void k_axpy_many(const Zd* coeffs, const 
         Zd acc = V<C>::load(w, i);
         int j = 0;
         for (; j + 4 <= m; j += 4) {
-            const Zd v0 = V<C>::load(Vb, static_cast<std::size_t>(j) * ld + i);
-            const Zd v1 = V<C>::load(Vb, static_cast<std::size_t>(j + 1) * ld + i);
-            const Zd v2 = V<C>::load(Vb, static_cast<std::size_t>(j + 2) * ld + i);
-            const Zd v3 = V<C>::load(Vb, static_cast<std::size_t>(j + 3) * ld + i);
+            const Zd v0 = V<C>::load(Vb.at(j), i);
+            const Zd v1 = V<C>::load(Vb.at(j + 1), i);
+            const Zd v2 = V<C>::load(Vb.at(j + 2), i);
+            const Zd v3 = V<C>::load(Vb.at(j + 3), i);
             if constexpr (C) {
                 acc = cadd(acc, cmul(sc[j], v0));
                 acc = cadd(acc, cmul(sc[j + 1], v1));
@@ -234,7 +264,7 @@ __global__ __launch_bounds__(kThreads) void k_axpy_many(const Zd* coeffs, const 
             }
         }
         for (; j < m; ++j) {
-            const Zd vj = V<C>::load(Vb, static_cast<std::size_t>(j) * ld + i);
+            const Zd vj = V<C>::load(Vb.at(j), i);
             acc = C ? cadd(acc, cmul(sc[j], vj)) : Zd{acc.re + sc[j].re * vj.re, 0.0};
         }
         V<C>::store(w, i, acc);
@@ -289,6 +319,13 @@ void fill_zero(bool c, void* x, std::size_t n, hipStream_t st) {
     else hipLaunchKernelGGL(k_fill_zero<false>, dim3(g), dim3(kThreads), 0, st, x, n);
 }
 
+void fill_gaussian(bool c, void* x, std::size_t n, u64 seed, hipStream_t st) {
+    if (!n) return;
+    const unsigned g = grid_for(n);
+    if (c) hipLaunchKernelGGL(k_fill_gauss<true>, dim3(g), dim3(kThreads), 0, st, x, n, seed);
+    else hipLaunchKernelGGL(k_fill_gauss<false>, dim3(g), dim3(kThreads), 0, st, x, n, seed);
+}
+
 static void launch_axpby(bool c, Zd a, const void* x, Zd b, void* y, std::size_t n, int mode, hipStream_t st) {
     if (!n) return;
     const unsigned g = grid_for(n);
@@ -330,8 +367,8 @@ void axpy_nrm2(bool c, Zd a, const void* x, void* y, std::size_t n, void* scratc
     reduce(c, 1, a, x, y, nullptr, n, 1, scratch, 1, result, st);
 }
 
-void dot_many(bool c, const void* V, std::size_t ld, std::size_t m, const void* w, std::size_t n, void* scratch,
-              Zd* coeffs, double* wnorm2, hipStream_t st) {
+static void dot_many_cols(bool c, const void* V, std::size_t ld, const void* const* table, std::size_t m,
+                          const void* w, std::size_t n, void* scratch, Zd* coeffs, double* wnorm2, hipStream_t st) {
     double* part = static_cast<double*>(scratch);
     const std::size_t elem = c ? sizeof(Zd) : sizeof(double);
     std::size_t done = 0;
@@ -342,13 +379,14 @@ void dot_many(bool c, const void* V, std::size_t ld, std::size_t m, const void* 
     const std::size_t rpb = (n + nb - 1) / nb;
     do {
         const int mc = static_cast<int>((m - done) < static_cast<std::size_t>(kManyCols) ? (m - done) : kManyCols);
-        const void* Vc = static_cast<const char*>(V) + done * ld * elem;
+        const Cols Vc{table ? nullptr : static_cast<const char*>(V) + done * ld * elem, ld * elem,
+                      table ? table + done : nullptr};
         const int want = (first && wnorm2) ? 1 : 0;
         if (c)
-            hipLaunchKernelGGL(k_dot_many<true>, dim3(static_cast<unsigned>(nb)), dim3(kThreads), 0, st, Vc, ld, mc, w,
+            hipLaunchKernelGGL(k_dot_many<true>, dim3(static_cast<unsigned>(nb)), dim3(kThreads), 0, st, Vc, mc, w,
                                n, rpb, part, static_cast<int>(nb), want);
         else
-            hipLaunchKernelGGL(k_dot_many<false>, dim3(static_cast<unsigned>(nb)), dim3(kThreads), 0, st, Vc, ld, mc,
+            hipLaunchKernelGGL(k_dot_many<false>, dim3(static_cast<unsigned>(nb)), dim3(kThreads), 0, st, Vc, mc,
                                w, n, rpb, part, static_cast<int>(nb), want);
         hipLaunchKernelGGL(k_dot_many_finish, dim3(static_cast<unsigned>(mc + want)), dim3(kThreads), 0, st, part,
                            static_cast<int>(nb), mc, coeffs + done, want ? wnorm2 : nullptr);
@@ -357,8 +395,18 @@ void dot_many(bool c, const void* V, std::size_t ld, std::size_t m, const void* 
     } while (done < m);
 }
 
-void axpy_many(bool c, const Zd* coeffs, const void* V, std::size_t ld, std::size_t m, void* w, std::size_t n,
-               void* scratch, double* wnorm2, hipStream_t st, double sign) {
+void dot_many(bool c, const void* V, std::size_t ld, std::size_t m, const void* w, std::size_t n, void* scratch,
+              Zd* coeffs, double* wnorm2, hipStream_t st) {
+    dot_many_cols(c, V, ld, nullptr, m, w, n, scratch, coeffs, wnorm2, st);
+}
+void dot_many_table(bool c, const void* const* table, std::size_t m, const void* w, std::size_t n, void* scratch,
+                    Zd* coeffs, hipStream_t st) {
+    if (m) dot_many_cols(c, nullptr, 0, table, m, w, n, scratch, coeffs, nullptr, st);
+}
+
+static void axpy_many_cols(bool c, const Zd* coeffs, const void* V, std::size_t ld, const void* const* table,
+                           std::size_t m, void* w, std::size_t n, void* scratch, double* wnorm2, hipStream_t st,
+                           double sign) {
     double* part = static_cast<double*>(scratch);
     const std::size_t elem = c ? sizeof(Zd) : sizeof(double);
     const unsigned g = grid_for(n);
@@ -367,16 +415,26 @@ void axpy_many(bool c, const Zd* coeffs, const void* V, std::size_t ld, std::siz
         const int mc = static_cast<int>((m - done) < static_cast<std::size_t>(kManyCols) ? (m - done) : kManyCols);
         const bool last = done + static_cast<std::size_t>(mc) >= m;
         const int want = (last && wnorm2) ? 1 : 0;
-        const void* Vc = static_cast<const char*>(V) + done * ld * elem;
+        const Cols Vc{table ? nullptr : static_cast<const char*>(V) + done * ld * elem, ld * elem,
+                      table ? table + done : nullptr};
         if (c)
-            hipLaunchKernelGGL(k_axpy_many<true>, dim3(g), dim3(kThreads), 0, st, coeffs + done, Vc, ld, mc, w, n, part,
+            hipLaunchKernelGGL(k_axpy_many<true>, dim3(g), dim3(kThreads), 0, st, coeffs + done, Vc, mc, w, n, part,
                                want, sign);
         else
-            hipLaunchKernelGGL(k_axpy_many<false>, dim3(g), dim3(kThreads), 0, st, coeffs + done, Vc, ld, mc, w, n,
+            hipLaunchKernelGGL(k_axpy_many<false>, dim3(g), dim3(kThreads), 0, st, coeffs + done, Vc, mc, w, n,
                                part, want, sign);
         if (want) hipLaunchKernelGGL(k_finish, dim3(1), dim3(kThreads), 0, st, part, static_cast<int>(g), 2, wnorm2);
         done += static_cast<std::size_t>(mc);
     } while (done < m);
+}
+
+void axpy_many(bool c, const Zd* coeffs, const void* V, std::size_t ld, std::size_t m, void* w, std::size_t n,
+               void* scratch, double* wnorm2, hipStream_t st, double sign) {
+    axpy_many_cols(c, coeffs, V, ld, nullptr, m, w, n, scratch, wnorm2, st, sign);
+}
+void axpy_many_table(bool c, const Zd* coeffs, const void* const* table, std::size_t m, void* w, std::size_t n,
+                     hipStream_t st) {
+    if (m) axpy_many_cols(c, coeffs, nullptr, 0, table, m, w, n, nullptr, nullptr, st, 1.0);
 }
 
 void block_rotate(bool c, const void* Vm, std::size_t ld, std::size_t mm, const double* S, std::size_t cols, void* out,

@@ -88,6 +88,9 @@ void pack_col_ids(u32* col, const u8* id, u64 nnz, hipStream_t st);
 /// Partial-sum scratch a reduction needs (device), sized by blas_scratch_bytes.
 std::size_t blas_scratch_bytes(std::size_t max_cols);
 void fill_zero(bool cplx, void* x, std::size_t n, hipStream_t st);
+/// x[i] = standard normal entries (complex: both parts) from a counter hash of (seed, i): test and
+/// benchmark vectors, not the FTLM samples.
+void fill_gaussian(bool cplx, void* x, std::size_t n, u64 seed, hipStream_t st);
 void axpy(bool cplx, Zd a, const void* x, void* y, std::size_t n, hipStream_t st);
 void scale(bool cplx, Zd a, void* x, std::size_t n, hipStream_t st);
 void axpby(bool cplx, Zd a, const void* x, Zd b, void* y, std::size_t n, hipStream_t st);
@@ -109,6 +112,13 @@ void dot_many(bool cplx, const void* V, std::size_t ld, std::size_t m, const voi
 /// when non-null.
 void axpy_many(bool cplx, const Zd* coeffs, const void* V, std::size_t ld, std::size_t m, void* w, std::size_t n,
                void* scratch, double* wnorm2, hipStream_t st, double sign = 1.0);
+
+/// dot_many / axpy_many over columns given as a device array of m device pointers (the reference's
+/// array-of-vectors form, BasicBackend::dot_many / axpy_many). axpy_many_table: w += sum_j c_j V_j.
+void dot_many_table(bool cplx, const void* const* table, std::size_t m, const void* w, std::size_t n, void* scratch,
+                    Zd* coeffs, hipStream_t st);
+void axpy_many_table(bool cplx, const Zd* coeffs, const void* const* table, std::size_t m, void* w, std::size_t n,
+                     hipStream_t st);
 
 }  // namespace qsa::dev
 

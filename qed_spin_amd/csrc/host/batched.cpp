@@ -55,13 +55,11 @@ template <class T> void put(DeviceBuffer& b, const std::vector<T>& h, hipStream_
     QSA_HIP(hipStreamSynchronize(st));
 }
 
-/// A stream of its own at the default priority (graph capture needs a stream other than the null one).
+/// A pooled stream at the default priority (graph capture needs a stream other than the null one).
 struct OwnStream {
     hipStream_t s = nullptr;
-    OwnStream() { QSA_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)); }
-    ~OwnStream() {
-        if (s) (void)hipStreamDestroy(s);
-    }
+    OwnStream() : s(pooled_stream_acquire()) {}
+    ~OwnStream() { pooled_stream_release(s); }
 };
 
 }  // namespace

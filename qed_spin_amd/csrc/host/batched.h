@@ -26,6 +26,9 @@ struct BatchedTimings {
 class BatchedLanczos {
 public:
     BatchedLanczos(std::vector<std::shared_ptr<const SectorOperator>> blocks, int R);
+    ~BatchedLanczos() { fence_before_release(); }
+    BatchedLanczos(const BatchedLanczos&) = delete;
+    BatchedLanczos& operator=(const BatchedLanczos&) = delete;
     int n_blocks() const { return static_cast<int>(ops_.size()); }
     int R() const { return R_; }
     std::uint64_t rows() const { return rows_; }

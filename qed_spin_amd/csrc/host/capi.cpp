@@ -134,15 +134,14 @@ int qedh_thread_stream(int own) {
             return;
         }
         qsa::require_device();
-        // created on the thread's first request, destroyed when the thread ends
+        // taken from the engine's pool on the thread's first request and handed back (drained,
+        // not destroyed) when the thread ends
         struct Owned {
             hipStream_t s = nullptr;
-            ~Owned() {
-                if (s) (void)hipStreamDestroy(s);
-            }
+            ~Owned() { qsa::pooled_stream_release(s); }
         };
         thread_local Owned owned;
-        if (!owned.s) QSA_HIP(hipStreamCreateWithFlags(&owned.s, hipStreamNonBlocking));
+        if (!owned.s) owned.s = qsa::pooled_stream_acquire();
         qsa::detail::tls_stream() = owned.s;
     });
 }

@@ -5,6 +5,7 @@
 #include <string>
 #include <vector>
 
+#include "../device/launch.h"
 #include "sector_operator.h"
 
 struct qedh_matvec_s {
@@ -27,18 +28,27 @@ extern "C" int qedh_matvec_benchmark(qedh_matvec m, int k, int block, int comple
         const std::size_t rows = static_cast<std::size_t>(op.dim()), cols = static_cast<std::size_t>(op.col_dim());
         const hipStream_t st = qsa::engine_stream();
         std::vector<qsa::DeviceBuffer> xs, ys;
-        qsa::DeviceBuffer X, Y;
+        qsa::DeviceBuffer X, Y, part(qsa::dev::blas_scratch_bytes(1)), nrm(8);
         std::vector<const void*> ins;
         std::vector<void*> outs;
+        // random unit input vectors (Gaussian entries, normalised on the device)
+        auto random_unit = [&](void* x, std::size_t n, std::uint64_t seed) {
+            qsa::dev::fill_gaussian(cv, x, n, seed, st);
+            qsa::dev::nrm2(cv, x, n, part.get(), nrm.get<double>(), st);
+            double h = 1.0;
+            QSA_HIP(hipMemcpyAsync(&h, nrm.get(), 8, hipMemcpyDeviceToHost, st));
+            QSA_HIP(hipStreamSynchronize(st));
+            if (h > 0.0) qsa::dev::scale(cv, qsa::dev::Zd{1.0 / h, 0.0}, x, n, st);
+        };
         if (block) {
             X.reset(cols * k * elem);
             Y.reset(rows * k * elem);
-            QSA_HIP(hipMemsetAsync(X.get(), 0, X.bytes(), st));
+            random_unit(X.get(), cols * k, 0x51ED2701ULL);
         } else {
             for (int i = 0; i < k; ++i) {
                 xs.emplace_back(cols * elem);
                 ys.emplace_back(rows * elem);
-                QSA_HIP(hipMemsetAsync(xs.back().get(), 0, xs.back().bytes(), st));
+                random_unit(xs.back().get(), cols, 0x51ED2701ULL + static_cast<std::uint64_t>(i));
                 ins.push_back(xs.back().get());
                 outs.push_back(ys.back().get());
             }

@@ -64,9 +64,30 @@ private:
 
 /// Stream-ordered device memory on the engine stream (the device's default pool, kept by the
 /// process instead of returned at every synchronisation). A buffer is freed in the order of the
-/// freeing thread's stream: work of another stream that still reads it must have been waited for.
+/// freeing thread's stream: an object whose buffers other streams may still read calls
+/// fence_before_release() first (every shared object's destructor does).
 void device_alloc(void** p, std::size_t bytes);
 void device_free(void* p) noexcept;
+
+/// A non-blocking stream from the engine's pool (created on demand, never destroyed while the
+/// process runs); release drains it and hands it back for the next thread or slot.
+hipStream_t pooled_stream_acquire();
+void pooled_stream_release(hipStream_t s) noexcept;
+/// The calling thread's engine stream waits for all work enqueued so far on every pooled stream
+/// and on the null stream. Objects that several threads may have used call it before freeing
+/// their device memory (hipFreeAsync orders a free on the freeing stream only).
+void fence_engine_streams();
+/// fence_engine_streams for destructors: errors are swallowed (and cleared).
+inline void fence_before_release() noexcept {
+    try {
+        fence_engine_streams();
+    } catch (...) {
+        (void)hipGetLastError();
+    }
+}
+/// Fences taken so far (tests), and the pooled streams created.
+std::uint64_t fence_count();
+std::size_t pooled_stream_count();
 
 /// Synchronous copies through the engine stream.
 inline void copy_to_host(void* dst, const void* src, std::size_t bytes) {

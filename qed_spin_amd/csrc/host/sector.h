@@ -30,6 +30,10 @@ public:
     DeviceSector(GroupSpec spec, const std::uint64_t* reps, const double* inv_norms, std::int64_t n,
                  std::uint64_t rank_budget_bytes);
 
+    ~DeviceSector() { fence_before_release(); }
+    DeviceSector(const DeviceSector&) = delete;
+    DeviceSector& operator=(const DeviceSector&) = delete;
+
     const GroupSpec& spec() const { return spec_; }
     std::int64_t dim() const { return dim_; }
     const SectorBuildStats& stats() const { return stats_; }
@@ -53,6 +57,9 @@ private:
 /// A compiled program in device memory.
 struct DeviceProgram {
     explicit DeviceProgram(const struct Program& p);
+    ~DeviceProgram() { fence_before_release(); }
+    DeviceProgram(const DeviceProgram&) = delete;
+    DeviceProgram& operator=(const DeviceProgram&) = delete;
     dev::ProgramView view() const;
     std::uint32_t n_groups = 0;
     std::size_t n_terms = 0;

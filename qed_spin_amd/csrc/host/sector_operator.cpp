@@ -158,13 +158,13 @@ void SectorOperator::autotune() {
         QSA_HIP(hipEventElapsedTime(&ms, a, b));
         return ms;
     };
+    // Only choices that leave every result bit unchanged are timed: the gather depth and the
+    // workgroup size. The lanes per row set the summation order of a row, so they stay what the
+    // mean row length gives and an apply does not depend on timing noise.
     float best1 = 1e30f;
-    int l1 = csr_.lanes, u1 = unroll_, w1 = wg_;
-    for (int L : {4, 8, 16})
-    for (int U : {2, 4})
+    int u1 = unroll_, w1 = wg_;
+    for (int U : {2, 4, 8})
     for (int WG : {512, 1024}) {
-        csr_.lanes = L;
-        lanes_block_ = L;
         unroll_ = U;
         wg_ = WG;
         const float t1 = time_it([&] {
@@ -176,7 +176,6 @@ void SectorOperator::autotune() {
         });
         if (t1 < best1) {
             best1 = t1;
-            l1 = L;
             u1 = U;
             w1 = WG;
         }
@@ -184,8 +183,6 @@ void SectorOperator::autotune() {
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     QSA_HIP(hipGetLastError());
-    csr_.lanes = l1;
-    lanes_block_ = l1;
     unroll_ = u1;
     wg_ = w1;
 }

@@ -29,12 +29,16 @@ struct CsrInfo {
 
 class SectorOperator {
 public:
-    /// tune: time the SpMV widths on a large CSR and keep the fastest (else the width follows the
-    /// mean row length only, so the summation order -- and the last bits of every apply -- do not
-    /// depend on timing).
+    /// tune: time the SpMV's gather depth and workgroup size on a large CSR and keep the fastest;
+    /// the lanes per row follow the mean row length either way, so the summation order -- and the
+    /// last bits of every apply -- never depend on timing.
     SectorOperator(std::shared_ptr<const DeviceSector> rows, std::shared_ptr<const DeviceSector> cols,
                    std::shared_ptr<const DeviceProgram> prog, double norm_bound, int lane,
                    std::uint64_t csr_budget_bytes, bool tune = true);
+
+    ~SectorOperator() { fence_before_release(); }
+    SectorOperator(const SectorOperator&) = delete;
+    SectorOperator& operator=(const SectorOperator&) = delete;
 
     std::int64_t dim() const { return rows_->dim(); }
     std::int64_t col_dim() const { return cols_->dim(); }
@@ -64,9 +68,9 @@ public:
         csr_.lanes = lanes;
         lanes_block_ = lanes;
     }
-    /// Time the single-vector SpMV at every lanes-per-row width (4, 8, 16), gather depth (2, 4) and
-    /// workgroup size (512, 1024) on this matrix and keep the fastest (block applies share the
-    /// width); a few tens of milliseconds on a large sector.
+    /// Time the single-vector SpMV at every gather depth (2, 4, 8) and workgroup size (512, 1024) on
+    /// this matrix and keep the fastest (both leave the results bit for bit unchanged); a few tens
+    /// of milliseconds on a large sector.
     void autotune();
     int lanes_block() const { return lanes_block_; }
     /// The CSR as the kernels read it (CSR lane only).

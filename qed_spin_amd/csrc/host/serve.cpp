@@ -305,9 +305,10 @@ ServeStats serve(const std::vector<Request>& reqs, const ServeOptions& o, std::v
     auto now_s = [&] { return std::chrono::duration<double>(clk::now() - t0).count(); };
     auto worker = [&](int slot) {
         hipStream_t stream = nullptr;
-        if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) {
-            (void)hipGetLastError();
-            stream = nullptr;
+        try {
+            stream = pooled_stream_acquire();
+        } catch (const Error&) {
+            stream = nullptr;   // the slot runs on the null stream
         }
         StreamScope scope(stream);
         for (;;) {
@@ -318,7 +319,7 @@ ServeStats serve(const std::vector<Request>& reqs, const ServeOptions& o, std::v
                 for (;;) {
                     if (next >= order.size()) {
                         lk.unlock();
-                        if (stream) (void)hipStreamDestroy(stream);
+                        pooled_stream_release(stream);
                         return;
                     }
                     const Request& r = reqs[order[next]];

@@ -184,6 +184,17 @@ def _solve_block(m, dim: int, want: int, floor: int, max_iter: int):
     return np.asarray(v), conv, "device-krylov"
 
 
+def walk_order(est, cand, partners) -> list:
+    """Positions q of cand in the order a pruned walk takes them: increasing estimate est[q], ties by
+    block index, each mirror of a time-reversal pair (partners {mirror: block}) right after its
+    partner whatever the two blocks' indices -- the mirror shares the partner's estimate and then its
+    decision (solved or pruned), so the walk does not depend on the order the momenta were given in."""
+    def key(q):
+        i = cand[q]
+        return (est[q], partners.get(i, i), i in partners)
+    return sorted(range(len(cand)), key=key)
+
+
 def walk_levels(blocks, k: int, *, floor: int, prune: bool = True, prune_margin: float = 0.02, window: float = 0.0,
                 norm_bound: float = 1.0, max_iter: int = 0, partners=None):
     """The levels of the blocks of a walk as the reference's eigs walk finds them
@@ -194,7 +205,7 @@ def walk_levels(blocks, k: int, *, floor: int, prune: bool = True, prune_margin:
     5% of s_H) or `window` of the k-th level found so far. Returns {block index: (levels, converged,
     lane)} for the blocks solved and the number pruned. partners {mirror: block} (time-reversal pairs
     of a real H, indices into blocks): a mirror is neither estimated nor solved; it takes its
-    partner's estimate and levels (its partner precedes it in the (estimate, index) order)."""
+    partner's estimate and levels, or is pruned with it (walk_order)."""
     partners = {} if partners is None else partners
     levels = {}
     small = max(min(floor, PRUNE_MIN_DIM), 2)
@@ -223,21 +234,20 @@ def walk_levels(blocks, k: int, *, floor: int, prune: bool = True, prune_margin:
         if p in cand:
             cand.append(j)
             est.append(est[cand.index(p)])
-    order = sorted(range(len(cand)), key=lambda q: (est[q], cand[q]))
     margin_floor = 0.05 * (norm_bound if norm_bound > 0 else 1.0)
     pruned = 0
-    for q in order:
+    for q in walk_order(est, cand, partners):
         i = cand[q]
-        found = np.sort(np.concatenate([v for v, _, _ in levels.values()])) if levels else np.zeros(0)
-        kth = found[k - 1] if found.size >= k else math.inf
-        if est[q] > kth + max(prune_margin * max(margin_floor, abs(kth)), window):
-            pruned += 1
-            continue
-        if i in partners:
+        if i in partners:    # one spectrum with its partner, which was just taken or pruned
             if partners[i] in levels:
                 levels[i] = levels[partners[i]]
             else:
                 pruned += 1
+            continue
+        found = np.sort(np.concatenate([v for v, _, _ in levels.values()])) if levels else np.zeros(0)
+        kth = found[k - 1] if found.size >= k else math.inf
+        if est[q] > kth + max(prune_margin * max(margin_floor, abs(kth)), window):
+            pruned += 1
             continue
         blk, sec, m = blocks[i]
         levels[i] = _solve_block(m, sec.dim, min(k, sec.dim), floor, max_iter)
@@ -378,17 +388,19 @@ def _stream_levels(H: Operator, k: int, sym: Symmetry, *, floor: int, prune: boo
             levels[j] = levels[p]
     margin_floor = 0.05 * (H.norm_bound() or 1.0)
     pruned = 0
-    for j in sorted(est, key=lambda q: (est[q], q)):
-        found = np.sort(np.concatenate([v for v, _, _ in levels.values()])) if levels else np.zeros(0)
-        kth = found[k - 1] if found.size >= k else math.inf
-        if prune and est[j] > kth + max(prune_margin * max(margin_floor, abs(kth)), window):
-            pruned += 1
-            continue
-        if j in mirror:      # its partner comes first in the (estimate, index) order
+    ests = sorted(est)
+    for q in walk_order([est[j] for j in ests], ests, mirror):
+        j = ests[q]
+        if j in mirror:      # one spectrum with its partner, which was just taken or pruned
             if mirror[j] in levels:
                 levels[j] = levels[mirror[j]]
             else:
                 pruned += 1
+            continue
+        found = np.sort(np.concatenate([v for v, _, _ in levels.values()])) if levels else np.zeros(0)
+        kth = found[k - 1] if found.size >= k else math.inf
+        if prune and est[j] > kth + max(prune_margin * max(margin_floor, abs(kth)), window):
+            pruned += 1
             continue
         blk, dim = meta[j]
         (_, sec, m), = _build_blocks(H, [blk], csr_budget, tune=True)
