@@ -107,11 +107,38 @@ def cpu_baseline(m, D, w_id, target_s=10.0):
     }
 
 
+def _oracle_lanczos_rate(m, reorth, steps, target_s, max_rows=None):
+    """The oracle's Lanczos (the reference's kernel restated, one thread, complex CSR) on the CSR of
+    matvec m (its first max_rows rows when the whole block would take too long): rows x steps per
+    second over repeated runs for about target_s seconds."""
+    import numpy as np
+
+    from oracle import pyoracle as po
+
+    row_ptr, col, val = m.export_csr()
+    rp = row_ptr.astype(np.int64)
+    n = m.dim
+    v0 = np.random.default_rng(7).normal(size=n) + 1j * np.random.default_rng(8).normal(size=n)
+    reps, t0 = 0, time.perf_counter()
+    done = 0
+    while True:
+        a, _ = po.lanczos(rp, col.astype(np.int64), val, v0, steps, reorth=reorth)
+        done += len(a)
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= target_s:
+            break
+    return n * done / el, {"rows": n, "nnz": int(rp[-1]), "runs": reps, "steps_per_run": steps, "seconds": el}
+
+
 def run_ftlm(args):
     """FTLM thermodynamics of the XXZ ring N=24 (Jxy = 1, Delta = 0.5) over every (Sz, k) block:
     40 samples per block (seed 7), Lanczos depth 100, 40 temperatures linspace(0.1, 4); blocks of at
     most 512 states exactly, all others through the batched engine in one packed multi-vector matvec
-    per step."""
+    per step. roofline: the packed step's algorithmic bytes (rows*8 + nnz*(4 + w) + R*(nnz*16 +
+    rows*16), w the stored value bytes) over the measured step time; cpu_baseline: the oracle's
+    LocalDGKS3 Lanczos (the reference's FTLM sample kernel restated) on one thread over the block
+    with the most rows, in the same unit (row-vector Lanczos steps per second)."""
     import numpy as np
 
     import qed_spin_amd as qs
@@ -126,9 +153,26 @@ def run_ftlm(args):
     e = r.engine
     R = 40
     rows, nnz = e.get("rows", 0.0), e.get("nnz", 0.0)
-    step_bytes = rows * 8 + nnz * (4 + 16) + R * (nnz * 16 + rows * 16)
-    gbs = step_bytes * 100 / e["steps_s"] / 1e9 if e.get("steps_s") else None
+    wval = e.get("value_bytes", 16.0) or 16.0
+    step_bytes = rows * 8 + nnz * (4 + wval) + R * (nnz * 16 + rows * 16)
+    steps = e.get("steps", 100.0) or 100.0
+    gbs = step_bytes * steps / e["steps_s"] / 1e9 if e.get("steps_s") else None
+    row_steps = rows * R * steps / e["steps_s"] if e.get("steps_s") else None
+    cpu = None
+    if not args.no_cpu:
+        blocks = th.walk_blocks(H, qs.Symmetry(sz=[n // 2], momenta=[0]), tune=False)
+        rate, sample = _oracle_lanczos_rate(blocks[0][2], 2, 100, 10.0)
+        th.release_later(blocks)
+        cpu = {"value": round(rate, 1), "unit": "row-vector Lanczos steps/s", "cores": 1, "kind": "port",
+               "sample": f"the oracle's LocalDGKS3 Lanczos, depth 100, on the (Sz=0, k=0) block ({sample['rows']} "
+                         f"rows, {sample['nnz']} entries), {sample['runs']} runs in {sample['seconds']:.1f} s",
+               "cpu": _cpu_model()}
     out = {
+        "throughput": {"value": round(row_steps, 1) if row_steps else None, "unit": "row-vector Lanczos steps/s"},
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 1) if gbs else None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs / HBM_PEAK_GBS, 4) if gbs else None, "traffic": None,
+                     "kernel": "packed SpMV + recurrence per step (batched engine)"},
+        "cpu_baseline": cpu,
         "workload": f"FTLM XXZ ring N={n}, Delta=0.5, every (Sz, k) block, 40 samples, depth 100, 40 T",
         "blocks": len(r.blocks),
         "sampled_blocks": sum(1 for b in r.blocks if not b["exact"]),
@@ -165,7 +209,30 @@ def run_dynamics(args):
     probes = [dy.sz_q(n, 2 * np.pi * m / n) for m in range(n)]
     r = dy.dynamics(H, probes, omega, eta=0.05, krylov=200, blocks=blocks, ground_blocks=k0)
     wall = time.perf_counter() - t0
+    e = r.engine
+    rows, nnz = e.get("rows", 0.0), e.get("nnz", 0.0)
+    wval = e.get("value_bytes", 2.0) or 2.0
+    step_bytes = rows * 8 + nnz * (4 + wval) + (nnz * 16 + rows * 16)
+    steps = e.get("steps", 200.0) or 200.0
+    gbs = step_bytes * steps / e["steps_s"] / 1e9 if e.get("steps_s") else None
+    row_steps = rows * steps / e["steps_s"] if e.get("steps_s") else None
+    cpu = None
+    if not args.no_cpu:
+        tgt = [b for b in blocks if b[0].momentum == n // 2][0]
+        rate, sample = _oracle_lanczos_rate(tgt[2], 0, 20, 10.0)
+        cpu = {"value": round(rate, 1), "unit": "row-vector Lanczos steps/s", "cores": 1, "kind": "port",
+               "sample": f"the oracle's Lanczos without reorthogonalisation (the continued fraction's), 20 steps, on "
+                         f"the (Sz=0, k=pi) block ({sample['rows']} rows, {sample['nnz']} entries), "
+                         f"{sample['runs']} runs in {sample['seconds']:.1f} s",
+               "cpu": _cpu_model()}
+    th.release_later(blocks)
     out = {
+        "throughput": {"value": round(row_steps, 1) if row_steps else None, "unit": "row-vector Lanczos steps/s"},
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 1) if gbs else None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs / HBM_PEAK_GBS, 4) if gbs else None, "traffic": None,
+                     "kernel": "packed SpMV + recurrence per step (batched engine, 28 right-hand sides)"},
+        "cpu_baseline": cpu,
+        "norm_sum": float(np.sum(r.norms)),
         "workload": f"T=0 S^z(q) of the Heisenberg ring N={n}, all {n} q, krylov 200, 400 omega, eta 0.05",
         "e0": r.e0,
         "ground_manifold": r.ground_manifold,
@@ -366,17 +433,17 @@ def main():
     ap.add_argument("--no-lanczos", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--lanes", type=int, default=0)
-    ap.add_argument("--workload", default="matvec,serve,refftlm,refeigs",
+    ap.add_argument("--workload", default="matvec,ftlm,dynamics,serve,refftlm,refeigs",
                     help="matvec (the headline) and a comma list of ftlm, dynamics, serve, spectrum, refdyn, refftlm, refeigs")
     ap.add_argument("--ftlm-sites", type=int, default=24)
     ap.add_argument("--dyn-sites", type=int, default=28)
     ap.add_argument("--spectrum-sites", type=int, default=18)
-    ap.add_argument("--serve-requests", type=int, default=200)
-    ap.add_argument("--serve-rate", type=float, default=5.0)
+    ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-rate", type=float, default=4.0)
     ap.add_argument("--serve-slots", type=int, default=8)
     ap.add_argument("--serve-nmax", type=int, default=30)
-    ap.add_argument("--serve-plain-requests", type=int, default=100)
-    ap.add_argument("--serve-realtime-requests", type=int, default=0)
+    ap.add_argument("--serve-plain-requests", type=int, default=50)
+    ap.add_argument("--serve-realtime-requests", type=int, default=300)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 1 GiB device to device (the known-size kernel the HBM counters are scaled by)")
     args = ap.parse_args()

@@ -149,6 +149,15 @@ int qedh_sector_create(int n_sites, int n_up, int group_size, const int* perms, 
 int qedh_sector_import(int n_sites, int n_up, int group_size, const int* perms, const uint64_t* flips,
                        const double* chars, int sublattice_mode, int relaxed, int64_t n_reps, const uint64_t* reps,
                        const double* inv_norms, uint64_t rank_budget_bytes, qedh_sector* out);
+/* The sublattice key-order tables of a group without building a sector (host only, no device):
+ * the reference's SublatticeCode::of (include/ed/basis/sublattice_code.h:187-230). *block gets L
+ * (0: not engaged; nothing else is written). key_bit[n_sites]; lead[(n_sites / L) 2^L] the least
+ * leading-block pattern reachable from each block pattern; cand_off[(n_sites / L) 2^L + 1] and
+ * cand[*n_cand] the elements reaching it, ascending. lead / cand_off / cand may be NULL (sizes
+ * only); a cand_cap below the count is InvalidRequest. */
+int qedh_sublattice_tables(int n_sites, int group_size, const int* perms, const uint64_t* flips, int sublattice_mode,
+                           int relaxed, int* block, int* key_bit, uint16_t* lead, uint32_t* cand_off, uint16_t* cand,
+                           int64_t cand_cap, int64_t* n_cand);
 /* dim, orbits before the character filter, sublattice block size (0: plain order), build seconds. */
 int qedh_sector_info(qedh_sector s, int64_t* dim, int64_t* n_orbits, int* sublattice_block, double* build_seconds);
 int qedh_sector_export(qedh_sector s, uint64_t* reps, double* inv_norms);
@@ -204,7 +213,9 @@ int qedh_matvec_apply_multi(qedh_matvec m, const void* const* ins, void* const* 
 /* Block-interleaved X[row * nv + i], nv in {1, 2, 4, 8}; CSR lane. */
 int qedh_matvec_apply_block(qedh_matvec m, int nv, const void* X, void* Y, int complex_vectors);
 int qedh_matvec_set_lanes(qedh_matvec m, int lanes);
-/* SpMV tuning (no counterpart in the reference): lanes per row (0 keeps the current) and flags:
+/* SpMV tuning (no counterpart in the reference). Large tuned operators also keep a SELL-64 copy of
+ * the CSR (64-row slices, one lane per row, sums in entry order) and apply through it; the lanes
+ * per row then do not apply. Lanes per row (0 keeps the current) and flags:
  * non-temporal streaming of the CSR arrays, one contiguous row range per XCD, the single-vector
  * gathers issued two, four or eight entries at a time, u8-id dictionaries read from global memory
  * instead of LDS, 256 or 1024 threads per workgroup instead of 512 (every flag leaves the results
@@ -218,7 +229,10 @@ enum {
     QEDH_TUNE_GLOBAL_DICT = 16,
     QEDH_TUNE_WG256 = 32,
     QEDH_TUNE_WG1024 = 64,
-    QEDH_TUNE_UNROLL2 = 128
+    QEDH_TUNE_UNROLL2 = 128,
+    QEDH_TUNE_UNROLL16 = 256,   /* SELL-64 layout: sixteen gathers in flight per lane */
+    QEDH_TUNE_CSR = 512,        /* apply through the CSR layout even where a SELL-64 copy exists */
+    QEDH_TUNE_SELL = 1024       /* build the SELL-64 copy now, whatever the operator's size */
 };
 int qedh_matvec_set_tuning(qedh_matvec m, int lanes, int flags);
 /* The merged CSR (download_csr): row_ptr[rows+1], col[nnz], val[2 nnz] (re, im). */

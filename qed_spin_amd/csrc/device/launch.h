@@ -76,6 +76,10 @@ struct CsrView {
     int dict_n = 0;                // dictionary entries (u8 / u16 ids)
     int lds_dict = 1;              // u8 ids: stage the dictionary in LDS
     int wg = 256;                  // single-vector SpMV: threads per workgroup (256, 512, 1024)
+    // SELL-64 layout (sell_ptr non-null): col / vals hold the entries slice by slice, entry k of row
+    // 64 s + j at sell_ptr[s] + 64 k + j; sell_len[row] is the row's entry count. One lane per row.
+    const u64* sell_ptr = nullptr;
+    const u16* sell_len = nullptr;
 };
 /// out_v = A in_v (real: double vectors; else complex vectors, with real or complex values).
 void csr_spmv(const CsrView& A, bool real, const VecSet& v, hipStream_t st);
@@ -84,9 +88,24 @@ void csr_spmv_block(const CsrView& A, bool real, int nv, const void* X, void* Y,
 /// col[e] |= id[e] << 24 (columns below 2^24): the packed u8-id layout.
 void pack_col_ids(u32* col, const u8* id, u64 nnz, hipStream_t st);
 
+/// The SELL-64 copy of a CSR (rows in slices of 64, each slice column-major and padded to its
+/// longest row). sell_lengths: sell_len[row] and the slice sizes (entries, 64 x the longest row)
+/// into slice_words[0 .. n_slices) with slice_words[n_slices] = 0; *too_long (device u32, zeroed
+/// by the caller) becomes nonzero when a row has more than 65535 entries. Then an exclusive scan of
+/// slice_words gives sell_ptr (sell_scan), and sell_fill copies columns and value ids / values
+/// (value_bytes 0: none, 1, 2, 8 or 16) into the slices.
+void sell_lengths(const u64* row_ptr, i64 rows, u16* sell_len, u64* slice_words, u32* too_long, hipStream_t st);
+std::size_t sell_scan_temp_bytes(i64 n_slices);
+void sell_scan(const u64* slice_words, i64 n_slices, u64* sell_ptr, void* temp, std::size_t temp_bytes,
+               hipStream_t st);
+void sell_fill(const u64* row_ptr, const u32* col, const void* vals, int value_bytes, i64 rows, const u64* sell_ptr,
+               u32* sell_col, void* sell_vals, hipStream_t st);
+
 // ---- blas.hip (vectors of T = double or Zd; n entries) -------------------------------------------
-/// Partial-sum scratch a reduction needs (device), sized by blas_scratch_bytes.
+/// Partial-sum scratch a reduction needs (device), sized by blas_scratch_bytes; never more than
+/// kBlasScratchMaxBytes (1024 blocks x 257 columns x one complex partial).
 std::size_t blas_scratch_bytes(std::size_t max_cols);
+inline constexpr std::size_t kBlasScratchMaxBytes = std::size_t{1024} * 257 * 2 * sizeof(double);
 void fill_zero(bool cplx, void* x, std::size_t n, hipStream_t st);
 /// x[i] = standard normal entries (complex: both parts) from a counter hash of (seed, i): test and
 /// benchmark vectors, not the FTLM samples.

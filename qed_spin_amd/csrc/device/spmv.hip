@@ -5,6 +5,8 @@
 // is deterministic and a k-vector apply equals k single applies bit for bit (the contract of the
 // reference's csr_spmv, src/gpu/rep_matvec.cu:1003-1036). Values come from a dictionary by u8 or
 // u16 id (the reference's reduced_csr.h layout) or are stored in full.
+#include <hipcub/hipcub.hpp>
+
 #include "dev_common.h"
 #include "launch.h"
 
@@ -213,6 +215,148 @@ __global__ __launch_bounds__(256) void k_spmv_block(CsrView A, const void* X, vo
     }
 }
 
+// ---- SELL-64 ------------------------------------------------------------------------------------
+// Slice s holds rows 64 s .. 64 s + 63 column-major: entry k of row 64 s + j sits at
+// sell_ptr[s] + 64 k + j, so the k-th entries of a wavefront's 64 rows are one coalesced 256-byte
+// load. A slice is padded to its longest row; the padding is never read (each lane stops at its
+// own row length). One lane owns a row and sums its entries in CSR order, one after another: the
+// sum depends neither on the workgroup size nor on how many gathers are in flight, and a k-vector
+// apply equals k single applies bit for bit. U entries per round: their column words first, then
+// the U gathers together, then the sums in entry order.
+template <int WIDTH, bool VREAL, bool REAL, int NV, int U>
+__global__ __launch_bounds__(1024) void k_sell(CsrView A, Ptrs<NV> v) {
+    __shared__ double sre[(WIDTH == 1 || WIDTH == 3) ? 256 : 1], sim[(WIDTH == 1 || WIDTH == 3) && !VREAL ? 256 : 1];
+    const Dict d = stage_dict<WIDTH, VREAL>(A, sre, sim);
+    const i64 row = row_block(A) * blockDim.x + threadIdx.x;
+    if (row >= A.rows) return;
+    const u32 n = A.nt ? __builtin_nontemporal_load(A.sell_len + row) : A.sell_len[row];
+    const u64 base = A.sell_ptr[row >> 6] + static_cast<u64>(row & 63);
+    double ar[NV], ai[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) ar[k] = ai[k] = 0.0;
+    auto accumulate = [&](u32 c, double vr, double vi) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            if constexpr (REAL) {
+                ar[k] += vr * static_cast<const double*>(v.in[k])[c];
+            } else {
+                const Zd x = static_cast<const Zd*>(v.in[k])[c];
+                ar[k] += vr * x.re - vi * x.im;
+                ai[k] += vr * x.im + vi * x.re;
+            }
+        }
+    };
+    u32 k = 0;
+    if constexpr (U > 1) {
+        for (; k + U <= n; k += U) {
+            u32 c[U];
+            double vr[U], vi[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                vi[u] = 0.0;
+                load_entry<WIDTH, VREAL>(A, d, base + static_cast<u64>(k + u) * 64, c[u], vr[u], vi[u]);
+            }
+            if constexpr (REAL && NV == 1) {
+                double x[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) x[u] = static_cast<const double*>(v.in[0])[c[u]];
+#pragma unroll
+                for (int u = 0; u < U; ++u) ar[0] += vr[u] * x[u];
+            } else {
+#pragma unroll
+                for (int u = 0; u < U; ++u) accumulate(c[u], vr[u], vi[u]);
+            }
+        }
+    }
+    for (; k < n; ++k) {
+        u32 c;
+        double vr, vi = 0.0;
+        load_entry<WIDTH, VREAL>(A, d, base + static_cast<u64>(k) * 64, c, vr, vi);
+        accumulate(c, vr, vi);
+    }
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        if constexpr (REAL) static_cast<double*>(v.out[q])[row] = ar[q];
+        else static_cast<Zd*>(v.out[q])[row] = Zd{ar[q], ai[q]};
+    }
+}
+
+// Block-interleaved vectors X[row * NV + k] on the SELL layout (the same order per row).
+template <int WIDTH, bool VREAL, bool REAL, int NV>
+__global__ __launch_bounds__(256) void k_sell_block(CsrView A, const void* X, void* Y) {
+    __shared__ double sre[(WIDTH == 1 || WIDTH == 3) ? 256 : 1], sim[(WIDTH == 1 || WIDTH == 3) && !VREAL ? 256 : 1];
+    const Dict d = stage_dict<WIDTH, VREAL>(A, sre, sim);
+    const i64 row = row_block(A) * blockDim.x + threadIdx.x;
+    if (row >= A.rows) return;
+    const u32 n = A.sell_len[row];
+    const u64 base = A.sell_ptr[row >> 6] + static_cast<u64>(row & 63);
+    double ar[NV], ai[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) ar[q] = ai[q] = 0.0;
+    for (u32 k = 0; k < n; ++k) {
+        u32 c32;
+        double vr, vi = 0.0;
+        load_entry<WIDTH, VREAL>(A, d, base + static_cast<u64>(k) * 64, c32, vr, vi);
+        const u64 c = c32;
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+            if constexpr (REAL) {
+                ar[q] += vr * static_cast<const double*>(X)[c * NV + q];
+            } else {
+                const Zd x = static_cast<const Zd*>(X)[c * NV + q];
+                ar[q] += vr * x.re - vi * x.im;
+                ai[q] += vr * x.im + vi * x.re;
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        if constexpr (REAL) static_cast<double*>(Y)[row * NV + q] = ar[q];
+        else static_cast<Zd*>(Y)[row * NV + q] = Zd{ar[q], ai[q]};
+    }
+}
+
+template <int WIDTH, bool VREAL, bool REAL> void launch_sell_set(const CsrView& A, const VecSet& v, hipStream_t st) {
+    const dim3 grid(grid_blocks(A, static_cast<u64>(A.rows))), block(block_threads(A));
+    int done = 0;
+    while (done < v.k) {
+        const int rest = v.k - done;
+        const int nv = rest >= 8 ? 8 : rest >= 4 ? 4 : rest >= 2 ? 2 : 1;
+        auto go = [&](auto tag) {
+            constexpr int NV = decltype(tag)::value;
+            Ptrs<NV> p;
+            for (int k = 0; k < NV; ++k) {
+                p.in[k] = v.in[done + k];
+                p.out[k] = v.out[done + k];
+            }
+            if (NV == 1 && A.unroll >= 16) hipLaunchKernelGGL((k_sell<WIDTH, VREAL, REAL, NV, 16>), grid, block, 0, st, A, p);
+            else if (NV == 1 && A.unroll >= 8) hipLaunchKernelGGL((k_sell<WIDTH, VREAL, REAL, NV, 8>), grid, block, 0, st, A, p);
+            else if (A.unroll >= 4) hipLaunchKernelGGL((k_sell<WIDTH, VREAL, REAL, NV, 4>), grid, block, 0, st, A, p);
+            else if (A.unroll >= 2) hipLaunchKernelGGL((k_sell<WIDTH, VREAL, REAL, NV, 2>), grid, block, 0, st, A, p);
+            else hipLaunchKernelGGL((k_sell<WIDTH, VREAL, REAL, NV, 1>), grid, block, 0, st, A, p);
+        };
+        if (nv == 8) go(std::integral_constant<int, 8>{});
+        else if (nv == 4) go(std::integral_constant<int, 4>{});
+        else if (nv == 2) go(std::integral_constant<int, 2>{});
+        else go(std::integral_constant<int, 1>{});
+        done += nv;
+    }
+}
+
+template <int WIDTH, bool VREAL, bool REAL>
+void launch_sell_block(const CsrView& A, int nv, const void* X, void* Y, hipStream_t st) {
+    CsrView B = A;
+    B.wg = 256;
+    const dim3 grid(grid_blocks(B, static_cast<u64>(A.rows))), block(256);
+    switch (nv) {
+    case 1: hipLaunchKernelGGL((k_sell_block<WIDTH, VREAL, REAL, 1>), grid, block, 0, st, B, X, Y); break;
+    case 2: hipLaunchKernelGGL((k_sell_block<WIDTH, VREAL, REAL, 2>), grid, block, 0, st, B, X, Y); break;
+    case 4: hipLaunchKernelGGL((k_sell_block<WIDTH, VREAL, REAL, 4>), grid, block, 0, st, B, X, Y); break;
+    case 8: hipLaunchKernelGGL((k_sell_block<WIDTH, VREAL, REAL, 8>), grid, block, 0, st, B, X, Y); break;
+    default: break;
+    }
+}
+
 template <int WIDTH, bool VREAL, bool REAL, int L> void launch_set(const CsrView& A, const VecSet& v, hipStream_t st) {
     const u64 threads = static_cast<u64>(A.rows) * L;
     const dim3 grid(grid_blocks(A, threads)), block(block_threads(A));
@@ -281,8 +425,13 @@ void dispatch(const CsrView& A, bool vreal, bool real, const VecSet* v, int nv, 
     auto go = [&](auto w, auto vr, auto xr) {
         constexpr int W = decltype(w)::value;
         constexpr bool VR = decltype(vr)::value, XR = decltype(xr)::value;
-        if constexpr (BLOCK) pick_lanes_block<W, VR, XR>(A, nv, X, Y, st);
-        else pick_lanes<W, VR, XR>(A, *v, st);
+        if (A.sell_ptr) {
+            if constexpr (BLOCK) launch_sell_block<W, VR, XR>(A, nv, X, Y, st);
+            else launch_sell_set<W, VR, XR>(A, *v, st);
+        } else {
+            if constexpr (BLOCK) pick_lanes_block<W, VR, XR>(A, nv, X, Y, st);
+            else pick_lanes<W, VR, XR>(A, *v, st);
+        }
     };
     using T = std::true_type;
     using F = std::false_type;
@@ -302,7 +451,69 @@ __global__ void k_pack_col_ids(u32* col, const u8* id, u64 nnz) {
         col[e] |= static_cast<u32>(id[e]) << 24;
 }
 
+// One thread per row; a wavefront is one slice (the grid's blocks are multiples of 64 threads).
+__global__ __launch_bounds__(256) void k_sell_lengths(const u64* row_ptr, i64 rows, u16* sell_len, u64* slice_words,
+                                                      u32* too_long) {
+    const i64 r = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x;
+    u32 n = 0;
+    if (r < rows) {
+        const u64 len = row_ptr[r + 1] - row_ptr[r];
+        if (len > 0xFFFFu) atomicOr(too_long, 1u);
+        n = len > 0xFFFFu ? 0xFFFFu : static_cast<u32>(len);
+        sell_len[r] = static_cast<u16>(n);
+    }
+    u32 m = n;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const u32 o = static_cast<u32>(__shfl_xor(static_cast<int>(m), off, 64));
+        m = o > m ? o : m;
+    }
+    if ((threadIdx.x & 63) == 0 && r < rows) slice_words[r >> 6] = static_cast<u64>(m) * 64;
+}
+
+__global__ __launch_bounds__(256) void k_sell_fill(const u64* row_ptr, const u32* col, const void* vals, int vb,
+                                                   i64 rows, const u64* sell_ptr, u32* sell_col, void* sell_vals) {
+    const i64 r = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    const u64 e0 = row_ptr[r], n = row_ptr[r + 1] - e0;
+    const u64 base = sell_ptr[r >> 6] + static_cast<u64>(r & 63);
+    for (u64 k = 0; k < n; ++k) {
+        const u64 dst = base + k * 64, src = e0 + k;
+        sell_col[dst] = col[src];
+        if (vb == 1) static_cast<u8*>(sell_vals)[dst] = static_cast<const u8*>(vals)[src];
+        else if (vb == 2) static_cast<u16*>(sell_vals)[dst] = static_cast<const u16*>(vals)[src];
+        else if (vb == 8) static_cast<double*>(sell_vals)[dst] = static_cast<const double*>(vals)[src];
+        else if (vb == 16) static_cast<Zd*>(sell_vals)[dst] = static_cast<const Zd*>(vals)[src];
+    }
+}
+
 }  // namespace
+
+void sell_lengths(const u64* row_ptr, i64 rows, u16* sell_len, u64* slice_words, u32* too_long, hipStream_t st) {
+    if (rows <= 0) return;
+    const u64 g = (static_cast<u64>(rows) + 255) / 256;
+    hipLaunchKernelGGL(k_sell_lengths, dim3(static_cast<unsigned>(g)), dim3(256), 0, st, row_ptr, rows, sell_len,
+                       slice_words, too_long);
+}
+
+std::size_t sell_scan_temp_bytes(i64 n_slices) {
+    std::size_t bytes = 0;
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, static_cast<const u64*>(nullptr), static_cast<u64*>(nullptr),
+                                           static_cast<int>(n_slices + 1));
+    return bytes;
+}
+
+void sell_scan(const u64* slice_words, i64 n_slices, u64* sell_ptr, void* temp, std::size_t temp_bytes, hipStream_t st) {
+    (void)hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, slice_words, sell_ptr, static_cast<int>(n_slices + 1), st);
+}
+
+void sell_fill(const u64* row_ptr, const u32* col, const void* vals, int value_bytes, i64 rows, const u64* sell_ptr,
+               u32* sell_col, void* sell_vals, hipStream_t st) {
+    if (rows <= 0) return;
+    const u64 g = (static_cast<u64>(rows) + 255) / 256;
+    hipLaunchKernelGGL(k_sell_fill, dim3(static_cast<unsigned>(g)), dim3(256), 0, st, row_ptr, col, vals, value_bytes,
+                       rows, sell_ptr, sell_col, sell_vals);
+}
 
 void pack_col_ids(u32* col, const u8* id, u64 nnz, hipStream_t st) {
     if (!nnz) return;

@@ -92,6 +92,27 @@ public:
         return {std::sqrt(h[0]), std::sqrt(h[1])};
     }
 
+    /// Two unconditional classical Gram-Schmidt passes of w against the m >= 1 contiguous columns
+    /// of V. Returns the coefficient of the last column summed over both passes (V_{m-1}^H w as w
+    /// was) and ||w|| after, read back together in one synchronisation.
+    std::pair<Scalar, double> cgs2_tail(const Scalar* V, std::size_t ld, std::size_t m, Scalar* w, std::size_t n) const {
+        if (coeff_.bytes() < 2 * m * sizeof(dev::Zd)) coeff_.reset(std::max<std::size_t>(m, 64) * 2 * sizeof(dev::Zd));
+        dev::Zd* c1 = coeff_.get<dev::Zd>();
+        dev::Zd* c2 = c1 + m;
+        double* norm = result_.get<double>() + 8;
+        dev::dot_many(kComplex, V, ld, m, w, n, scratch_.get(), c1, nullptr, st());
+        dev::axpy_many(kComplex, c1, V, ld, m, w, n, scratch_.get(), nullptr, st(), -1.0);
+        dev::dot_many(kComplex, V, ld, m, w, n, scratch_.get(), c2, nullptr, st());
+        dev::axpy_many(kComplex, c2, V, ld, m, w, n, scratch_.get(), norm, st(), -1.0);
+        dev::Zd tail[2];
+        double nn = 0.0;
+        QSA_HIP(hipMemcpyAsync(&tail[0], c1 + (m - 1), sizeof(dev::Zd), hipMemcpyDeviceToHost, st()));
+        QSA_HIP(hipMemcpyAsync(&tail[1], c2 + (m - 1), sizeof(dev::Zd), hipMemcpyDeviceToHost, st()));
+        QSA_HIP(hipMemcpyAsync(&nn, norm, sizeof(double), hipMemcpyDeviceToHost, st()));
+        QSA_HIP(hipStreamSynchronize(st()));
+        return {from_z(dev::Zd{tail[0].re + tail[1].re, tail[0].im + tail[1].im}), std::sqrt(nn)};
+    }
+
     /// coeffs_out[j] = V_j^H v for the m columns of V (host result).
     void dot_many_block(const Scalar* V, std::size_t ld, std::size_t m, const Scalar* v, std::size_t n,
                         Scalar* coeffs_out) const {

@@ -343,9 +343,19 @@ void BatchedLanczos::run(int M, const std::vector<double>& breakdown_tol, bool l
     try {
         if (use_graph) {
             for (int ph = 0; ph < 3 && ph < Mrun; ++ph) {
-                QSA_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-                step(ph);
-                QSA_HIP(hipStreamEndCapture(st, &graph[ph]));
+                {
+                    const auto no_fence = capture_guard();   // no fence touches st while it captures
+                    QSA_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                    try {
+                        step(ph);
+                    } catch (...) {
+                        hipGraph_t broken = nullptr;
+                        (void)hipStreamEndCapture(st, &broken);
+                        if (broken) (void)hipGraphDestroy(broken);
+                        throw;
+                    }
+                    QSA_HIP(hipStreamEndCapture(st, &graph[ph]));
+                }
                 QSA_HIP(hipGraphInstantiate(&exec[ph], graph[ph], nullptr, nullptr, 0));
             }
         }
@@ -391,7 +401,7 @@ std::vector<Curves> batched_ftlm(const std::vector<std::shared_ptr<const SectorO
     const int R = std::min(o.samples, dev::kBatchMaxR);
     BatchedLanczos L(blocks, R);
     t.pack_s += secs(t_pack, clock_t_::now());
-    std::vector<std::vector<SampleMoments>> mom(nb);
+    std::vector<RitzPool> pools(nb);
     std::vector<double> alpha, beta;
     std::vector<int> steps;
     for (int s0 = 0; s0 < o.samples; s0 += R) {
@@ -412,7 +422,7 @@ std::vector<Curves> batched_ftlm(const std::vector<std::shared_ptr<const SectorO
         L.run(o.krylov, o.breakdown_tol, /*local=*/true, o.use_graph, alpha, beta, steps, t);
         const auto t_post = clock_t_::now();
         const std::size_t pairs = nb * static_cast<std::size_t>(R);
-        std::vector<SampleMoments> round(nb * static_cast<std::size_t>(Rc));
+        std::vector<std::vector<double>> ritz(nb * static_cast<std::size_t>(Rc)), weight(ritz.size());
         parallel_for(nb * static_cast<std::size_t>(Rc), [&](std::size_t i) {
             const std::size_t b = i / Rc, s = i % Rc;
             const std::size_t pr = b * R + s;
@@ -422,16 +432,15 @@ std::vector<Curves> batched_ftlm(const std::vector<std::shared_ptr<const SectorO
                 a[j] = alpha[static_cast<std::size_t>(j) * pairs + pr];
                 bn[j] = beta[static_cast<std::size_t>(j) * pairs + pr];
             }
-            std::vector<double> values, weights;
-            tridiag_weights(a.data(), bn.data(), static_cast<std::size_t>(m), values, weights);
-            round[i] = sample_moments(values, weights, o.betas);
+            tridiag_weights(a.data(), bn.data(), static_cast<std::size_t>(m), ritz[i], weight[i]);
         });
-        for (std::size_t b = 0; b < nb; ++b)
-            for (int s = 0; s < Rc; ++s) mom[b].push_back(std::move(round[b * Rc + s]));
+        for (std::size_t b = 0; b < nb; ++b)   // samples pooled in sample order
+            for (int s = 0; s < Rc; ++s)
+                if (!ritz[b * Rc + s].empty()) pools[b].add_sample(ritz[b * Rc + s], weight[b * Rc + s]);
         t.post_s += secs(t_post, clock_t_::now());
     }
     std::vector<Curves> out(nb);
-    for (std::size_t b = 0; b < nb; ++b) out[b] = combine_samples(mom[b], L.dim(static_cast<int>(b)), o.betas);
+    for (std::size_t b = 0; b < nb; ++b) out[b] = pools[b].curves(L.dim(static_cast<int>(b)), o.betas);
     t.total_s += secs(t_all, clock_t_::now());
     return out;
 }

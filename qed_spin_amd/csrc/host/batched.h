@@ -10,7 +10,7 @@
 #include <vector>
 
 #include "sector_operator.h"
-#include "spectral.h"
+#include "thermo.h"
 
 namespace qsa {
 

@@ -6,10 +6,10 @@
 #include <string>
 
 #include "algebra.h"
-#include "krylov.h"
+#include "eigsolve.h"
 #include "sector.h"
 #include "sector_operator.h"
-#include "spectral.h"
+#include "thermo.h"
 
 struct qedh_operator_s {
     qsa::SpinOperator op;
@@ -311,6 +311,30 @@ int qedh_sector_import(int n_sites, int n_up, int group_size, const int* perms, 
             std::make_shared<qsa::DeviceSector>(std::move(g), reps, inv_norms, n_reps, rank_budget_bytes)};
     });
 }
+int qedh_sublattice_tables(int n_sites, int group_size, const int* perms, const uint64_t* flips, int sublattice_mode,
+                           int relaxed, int* block, int* key_bit, uint16_t* lead, uint32_t* cand_off, uint16_t* cand,
+                           int64_t cand_cap, int64_t* n_cand) {
+    return guarded([&] {
+        need(perms, "perms");
+        need(block, "block");
+        if (n_sites < 1 || n_sites > 63 || group_size < 1) throw qsa::Error(qsa::kInvalid, "sublattice: bad sizes");
+        std::vector<int> P(perms, perms + static_cast<std::size_t>(group_size) * n_sites);
+        std::vector<uint64_t> F = flips ? std::vector<uint64_t>(flips, flips + group_size)
+                                        : std::vector<uint64_t>(static_cast<std::size_t>(group_size), 0ULL);
+        const qsa::SublatticeTables t = qsa::build_sublattice(P, F, group_size, n_sites, sublattice_mode, relaxed != 0);
+        *block = t.L;
+        if (n_cand) *n_cand = static_cast<int64_t>(t.cand.size());
+        if (!t.engaged()) return;
+        if (key_bit) std::memcpy(key_bit, t.key_bit.data(), t.key_bit.size() * sizeof(int));
+        if (lead) std::memcpy(lead, t.lead.data(), t.lead.size() * sizeof(uint16_t));
+        if (cand_off) std::memcpy(cand_off, t.cand_off.data(), t.cand_off.size() * sizeof(uint32_t));
+        if (cand) {
+            if (cand_cap < static_cast<int64_t>(t.cand.size()))
+                throw qsa::Error(qsa::kInvalid, "sublattice: cand_cap below the candidate count");
+            std::memcpy(cand, t.cand.data(), t.cand.size() * sizeof(uint16_t));
+        }
+    });
+}
 int qedh_sector_info(qedh_sector s, int64_t* dim, int64_t* n_orbits, int* sublattice_block, double* build_seconds) {
     return guarded([&] {
         need(s, "sector");
@@ -374,7 +398,7 @@ int qedh_matvec_info(qedh_matvec m, qedh_matvec_info_t* info) {
         info->dict_size = c.dict;
         info->csr_bytes = c.bytes;
         info->build_seconds = c.build_seconds;
-        info->lanes = c.lanes;
+        info->lanes = m->op->sell() ? 1 : c.lanes;   // SELL-64: one lane per row
         info->norm_bound = o.norm_bound();
     });
 }
@@ -490,7 +514,7 @@ int qedh_block_lowest(qedh_matvec m, int k, int64_t max_iter, double* values, in
             qsa::BlockSolution sol;
             const std::uint64_t budget = max_iter > 0 ? static_cast<std::uint64_t>(max_iter) : 0;
             if (kk > 1) sol = qsa::krylov_schur_lane(be, H, n, kk, op.norm_bound(), budget);
-            else sol = qsa::lowest_scan(be, H, n, budget > 0 ? budget : std::max<std::uint64_t>(400, 0));
+            else sol = qsa::lowest_scan(be, H, n, budget > 0 ? budget : 400, op.norm_bound());
             *n_values = static_cast<int>(sol.values.size());
             for (std::size_t i = 0; i < sol.values.size(); ++i) values[i] = sol.values[i];
             *converged = sol.converged ? 1 : 0;

@@ -21,7 +21,7 @@
 
 #include "lanczos.h"
 #include "sector_operator.h"
-#include "spectral.h"
+#include "thermo.h"
 #include "tridiag.h"
 
 namespace qsa {
@@ -49,9 +49,6 @@ struct FtlmDynResult {
     double E_min = 0.0;
     std::size_t samples_done = 0;
 };
-
-/// splitmix64 of base + phi (key + 1): the seed of one source sector (sample_seed.h:48-53).
-inline std::uint64_t block_seed(std::uint64_t base, std::uint64_t key) { return splitmix_seed(base, key); }
 
 inline FtlmDynResult ftlm_dynamics_source(const SectorOperator& Hsrc, const std::vector<FtlmDynTarget>& targets,
                                           const FtlmDynOptions& o) {

@@ -5,6 +5,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -77,6 +78,10 @@ void pooled_stream_release(hipStream_t s) noexcept;
 /// and on the null stream. Objects that several threads may have used call it before freeing
 /// their device memory (hipFreeAsync orders a free on the freeing stream only).
 void fence_engine_streams();
+/// Held from hipStreamBeginCapture to hipStreamEndCapture on a pooled stream: no fence records an
+/// event on any engine stream meanwhile. Only kernel launches may happen under it (no allocation,
+/// no stream acquisition: the lock is the pool's own).
+std::unique_lock<std::mutex> capture_guard();
 /// fence_engine_streams for destructors: errors are swallowed (and cleared).
 inline void fence_before_release() noexcept {
     try {

@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <vector>
 
 namespace qsa {
 
@@ -133,8 +134,54 @@ void SectorOperator::build_csr(std::uint64_t budget) {
     csr_.lanes = L;
     lanes_block_ = L;
     lane_ = kLaneCsr;
-    if (tune_ && rows >= (std::int64_t{1} << 18)) autotune();
+    if (tune_ && rows >= (std::int64_t{1} << 18)) {
+        build_sell();
+        autotune();
+    }
     csr_.build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+bool SectorOperator::build_sell() {
+    if (lane_ != kLaneCsr || dim() == 0) return false;
+    if (sell_) return true;
+    const hipStream_t st = engine_stream();
+    const std::int64_t rows = dim(), slices = (rows + 63) / 64;
+    DeviceBuffer words(static_cast<std::size_t>(slices + 1) * 8), flag(4);
+    sell_len_.reset(static_cast<std::size_t>(rows) * 2);
+    sell_ptr_.reset(static_cast<std::size_t>(slices + 1) * 8);
+    QSA_HIP(hipMemsetAsync(words.get(), 0, words.bytes(), st));
+    QSA_HIP(hipMemsetAsync(flag.get(), 0, 4, st));
+    dev::sell_lengths(row_ptr_.get<std::uint64_t>(), rows, sell_len_.get<std::uint16_t>(), words.get<std::uint64_t>(),
+                      flag.get<std::uint32_t>(), st);
+    const std::size_t tb = dev::sell_scan_temp_bytes(slices);
+    DeviceBuffer temp(tb ? tb : 8);
+    dev::sell_scan(words.get<std::uint64_t>(), slices, sell_ptr_.get<std::uint64_t>(), temp.get(), tb, st);
+    QSA_HIP(hipGetLastError());
+    std::uint64_t total = 0;
+    std::uint32_t too_long = 0;
+    QSA_HIP(hipMemcpyAsync(&total, sell_ptr_.get<std::uint64_t>() + slices, 8, hipMemcpyDeviceToHost, st));
+    QSA_HIP(hipMemcpyAsync(&too_long, flag.get(), 4, hipMemcpyDeviceToHost, st));
+    QSA_HIP(hipStreamSynchronize(st));
+    if (too_long) {
+        sell_len_.release();
+        sell_ptr_.release();
+        return false;
+    }
+    // the value stream beside the column words: u8 / u16 ids, or full values (none when packed)
+    const int vb = packed_ ? 0 : (csr_.width == 1 || csr_.width == 2) ? csr_.width : (real_ ? 8 : 16);
+    sell_col_.reset(std::max<std::uint64_t>(total, 1) * 4);
+    if (vb) sell_vals_.reset(std::max<std::uint64_t>(total, 1) * static_cast<std::size_t>(vb));
+    QSA_HIP(hipMemsetAsync(sell_col_.get(), 0, sell_col_.bytes(), st));
+    if (vb) QSA_HIP(hipMemsetAsync(sell_vals_.get(), 0, sell_vals_.bytes(), st));
+    dev::sell_fill(row_ptr_.get<std::uint64_t>(), col_.get<std::uint32_t>(), vals_.get(), vb, rows,
+                   sell_ptr_.get<std::uint64_t>(), sell_col_.get<std::uint32_t>(), sell_vals_.get(), st);
+    QSA_HIP(hipGetLastError());
+    QSA_HIP(hipStreamSynchronize(st));
+    sell_ = true;
+    unroll_ = 8;
+    wg_ = 256;
+    csr_.bytes += sell_ptr_.bytes() + sell_len_.bytes() + sell_col_.bytes() + sell_vals_.bytes();
+    return true;
 }
 
 void SectorOperator::autotune() {
@@ -161,10 +208,13 @@ void SectorOperator::autotune() {
     // Only choices that leave every result bit unchanged are timed: the gather depth and the
     // workgroup size. The lanes per row set the summation order of a row, so they stay what the
     // mean row length gives and an apply does not depend on timing noise.
+    // On the SELL-64 copy one lane owns a row, so no choice changes a sum.
     float best1 = 1e30f;
     int u1 = unroll_, w1 = wg_;
-    for (int U : {2, 4, 8})
-    for (int WG : {512, 1024}) {
+    const std::vector<int> depths = sell() ? std::vector<int>{4, 8, 16} : std::vector<int>{2, 4, 8};
+    const std::vector<int> groups = sell() ? std::vector<int>{256, 512, 1024} : std::vector<int>{512, 1024};
+    for (int U : depths)
+    for (int WG : groups) {
         unroll_ = U;
         wg_ = WG;
         const float t1 = time_it([&] {
@@ -172,7 +222,7 @@ void SectorOperator::autotune() {
             v.k = 1;
             v.in[0] = x.get();
             v.out[0] = y.get();
-            dev::csr_spmv(csr_view(), real_, v, st);
+            dev::csr_spmv(csr_view(false, sell()), real_, v, st);
         });
         if (t1 < best1) {
             best1 = t1;
@@ -187,12 +237,18 @@ void SectorOperator::autotune() {
     wg_ = w1;
 }
 
-dev::CsrView SectorOperator::csr_view(bool block) const {
+dev::CsrView SectorOperator::csr_view(bool block, bool sell) const {
     dev::CsrView A;
     A.rows = dim();
     A.row_ptr = row_ptr_.get<std::uint64_t>();
     A.col = col_.get<std::uint32_t>();
     A.vals = vals_.get();
+    if (sell && sell_) {
+        A.col = sell_col_.get<std::uint32_t>();
+        A.vals = sell_vals_.get();
+        A.sell_ptr = sell_ptr_.get<std::uint64_t>();
+        A.sell_len = sell_len_.get<std::uint16_t>();
+    }
     A.width = packed_ ? 3 : (csr_.width == 1 || csr_.width == 2) ? csr_.width : 0;
     A.dict_re = dict_re_.get<double>();
     A.dict_im = dict_im_.get<double>();
@@ -224,7 +280,7 @@ void SectorOperator::apply_multi(const void* const* ins, void* const* outs, int 
             v.in[i] = ins[done + i];
             v.out[i] = outs[done + i];
         }
-        if (lane_ == kLaneCsr) dev::csr_spmv(csr_view(), real_vectors, v, st);
+        if (lane_ == kLaneCsr) dev::csr_spmv(csr_view(false, sell()), real_vectors, v, st);
         else dev::walk_gather(rows_->view(), cols_->view(), same(), prog_->view(), real_vectors, v, st);
     }
     QSA_HIP(hipGetLastError());
@@ -233,7 +289,7 @@ void SectorOperator::apply_multi(const void* const* ins, void* const* outs, int 
 void SectorOperator::apply_block(int nv, const void* X, void* Y, bool complex_vectors) const {
     if (lane_ != kLaneCsr) throw Error(kDeviceUnsupported, "apply_block needs the CSR lane");
     if (nv != 1 && nv != 2 && nv != 4 && nv != 8) throw Error(kInvalid, "apply_block: nv must be 1, 2, 4 or 8");
-    dev::csr_spmv_block(csr_view(true), real_ && !complex_vectors, nv, X, Y, engine_stream());
+    dev::csr_spmv_block(csr_view(true, sell()), real_ && !complex_vectors, nv, X, Y, engine_stream());
     QSA_HIP(hipGetLastError());
 }
 

@@ -88,9 +88,18 @@ public:
     void set_workgroup(int wg) { wg_ = wg; }
     bool xcd_rows() const { return xcd_; }
 
+    /// Whether applies run on the SELL-64 copy of the CSR (one lane per row; built for the large
+    /// operators that time their SpMV).
+    bool sell() const { return sell_ && use_sell_; }
+    /// Apply through the SELL-64 copy when one exists (default), or through the CSR.
+    void set_use_sell(bool on) { use_sell_ = on; }
+    /// Build the SELL-64 copy now (tests); false when a row is too long for it.
+    bool build_sell();
+
 private:
     void build_csr(std::uint64_t budget);
-    dev::CsrView csr_view(bool block = false) const;
+    /// The CSR as a kernel reads it; with sell, the SELL-64 copy (single and multi-vector applies).
+    dev::CsrView csr_view(bool block = false, bool sell = false) const;
     int lanes_block_ = 16;
 
     std::shared_ptr<const DeviceSector> rows_, cols_;
@@ -107,6 +116,8 @@ private:
     int wg_ = 512;
     CsrInfo csr_;
     DeviceBuffer row_ptr_, col_, vals_, dict_re_, dict_im_;
+    bool sell_ = false, use_sell_ = true;
+    DeviceBuffer sell_ptr_, sell_len_, sell_col_, sell_vals_;
 };
 
 }  // namespace qsa

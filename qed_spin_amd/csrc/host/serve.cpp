@@ -16,10 +16,10 @@
 
 #include "algebra.h"
 #include "batched.h"
-#include "krylov.h"
+#include "eigsolve.h"
 #include "sector.h"
 #include "sector_operator.h"
-#include "spectral.h"
+#include "thermo.h"
 
 namespace qsa {
 
@@ -179,12 +179,6 @@ void solve(const Request& r, const ServeOptions& o, SectorCache& cache, RequestR
     (built ? builds : hits) += 1;
     res.dim = src->dim();
     if (src->dim() == 0) throw Error(kInvalid, "serve: the sector is empty");
-    // the Krylov-Schur start of eigs and dynamics, drawn on a host thread while the CSR is built
-    std::future<std::vector<cplx>> ks_start;
-    if (r.kind == kReqEigs || r.kind == kReqDynamics) {
-        const std::size_t nd = static_cast<std::size_t>(src->dim());
-        ks_start = std::async(std::launch::async, [nd] { return gaussian_start<cplx>(nd, kKsStartSeed); });
-    }
     const SpinOperator H = chain_model(N, r.j1, r.j2, r.chi);
     const double sH = H.l1_norm();
     auto hprog = std::make_shared<DeviceProgram>(compile_operator(H.dagger()));
@@ -196,7 +190,7 @@ void solve(const Request& r, const ServeOptions& o, SectorCache& cache, RequestR
     auto applyH = [&](const cplx* in, cplx* out) { Hop->apply(in, out, true); };
     if (r.kind == kReqEigs) {
         const std::size_t k = std::min<std::size_t>(static_cast<std::size_t>(std::max(1, r.k)), n);
-        const BlockSolution sol = krylov_schur_lane(be, applyH, n, k, sH, 0, static_cast<cplx*>(nullptr), &ks_start);
+        const BlockSolution sol = krylov_schur_lane(be, applyH, n, k, sH, 0, static_cast<cplx*>(nullptr));
         res.values = sol.values;
         res.e0 = sol.values.empty() ? 0.0 : sol.values.front();
     } else if (r.kind == kReqThermal) {
@@ -225,7 +219,7 @@ void solve(const Request& r, const ServeOptions& o, SectorCache& cache, RequestR
         res.e0 = c.e_min;
     } else if (r.kind == kReqDynamics) {
         auto gs = be.make_zero_vector(n);
-        const BlockSolution sol = krylov_schur_lane(be, applyH, n, 1, sH, 0, gs.get(), &ks_start);
+        const BlockSolution sol = krylov_schur_lane(be, applyH, n, 1, sH, 0, gs.get());
         if (sol.values.empty()) throw Error(kConvergence, "serve: no ground state");
         const double e0 = sol.values.front();
         res.e0 = e0;

@@ -13,6 +13,12 @@
 // memory would go back to the pool while kernels of another stream still read it. fence() makes
 // the calling thread's stream wait for everything enqueued so far on every engine stream and on
 // the null stream; the destructors of shared objects call it before their buffers are freed.
+//
+// Fences and graph captures exclude each other: a fence records an event on every pooled stream,
+// and an event recorded on a stream that another thread is capturing becomes a node of that capture
+// (the fencing stream's wait on it then joins the capture, and the free that follows would be
+// recorded into the other thread's graph). A capture therefore holds the pool's lock from its
+// begin to its end (capture_guard), and a fence takes the same lock.
 #include <atomic>
 #include <mutex>
 #include <vector>
@@ -97,6 +103,8 @@ void fence_engine_streams() {
     }
     P.fences.fetch_add(1, std::memory_order_relaxed);
 }
+
+std::unique_lock<std::mutex> capture_guard() { return std::unique_lock<std::mutex>(pool().mu); }
 
 std::uint64_t fence_count() { return pool().fences.load(std::memory_order_relaxed); }
 

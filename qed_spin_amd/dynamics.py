@@ -70,6 +70,7 @@ class DynamicsResult:
     target_sectors: int
     timings: Dict[str, float] = field(default_factory=dict)
     engine: Dict[str, float] = field(default_factory=dict)
+    norms: Optional[np.ndarray] = None   # T = 0: per probe, ||O|g>||^2 averaged over the ground manifold
 
 
 def ground_manifold(blocks, window: float, k_max: int = 16, only=None, prune: bool = True, norm_bound: float = 1.0,
@@ -290,6 +291,7 @@ def dynamics(H: Operator, probes, omega, *, eta: float = 0.1, T=None, sym: Optio
                 keep.append(X)
     t_probe = time.perf_counter() - t2
     S = [np.zeros(omega.size) for _ in probes]
+    norms = np.zeros(len(probes))
     engine = {}
     targets = sorted({t for t, _, _ in images})
     if images:
@@ -315,9 +317,11 @@ def dynamics(H: Operator, probes, omega, *, eta: float = 0.1, T=None, sym: Optio
                   None, None, _lib.ptr(stats, C.c_double))
         for i, p in owner:
             S[p] += out[i]
+            norms[p] += nsq[i]
         engine = dict(zip(["pack_s", "steps_s", "host_s", "total_s", "applies", "rows", "nnz"], stats.tolist()))
     for p in range(len(probes)):
         S[p] /= float(len(states))
+        norms[p] /= float(len(states))
     n_states = len(states)
     states = images = keep = None
     if own:
@@ -325,4 +329,4 @@ def dynamics(H: Operator, probes, omega, *, eta: float = 0.1, T=None, sym: Optio
     return DynamicsResult(
         omega=omega, S=S, e0=e0, ground_manifold=n_states, target_sectors=len(targets),
         timings={"walk_s": t_walk, "ground_s": t_gm, "probe_s": t_probe, "total_s": time.perf_counter() - t0},
-        engine=engine)
+        engine=engine, norms=norms)
