@@ -96,6 +96,15 @@ int qedh_dot_many(int complex_vectors, const void* const* basis, size_t num_basi
  * backend.h:141-145). */
 int qedh_axpy_many(int complex_vectors, const double* alphas, const void* const* basis, size_t num_basis, void* v,
                    size_t n);
+/* qedh_dot_many without any allocation. No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). Same arguments and results as qedh_dot_many (bit for bit: the same kernels);
+ * its column table, coefficients and partial sums live in a fixed device array the library holds on
+ * each device until the process ends (about 4.3 MB). num_basis above QEDH_NOALLOC_MAX_BASIS is
+ * InvalidRequest. Calls on one device share that array: calls that overlap on one device race. */
+#define QEDH_NOALLOC_MAX_BASIS 4096
+int qedh_dot_many_noalloc(int complex_vectors, const void* const* basis, size_t num_basis, const void* v, size_t n,
+                          double* coeffs_out);
 /* C = alpha op(A) op(B) + beta C, column-major device matrices, op 'N' / 'T' / 'C' (BasicBackend::gemm,
  * backend.h:156-161): the library GEMM (rocBLAS zgemm / dgemm) on the engine stream. alpha / beta NULL:
  * 1 / 0. */

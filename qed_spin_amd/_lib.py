@@ -67,6 +67,7 @@ _SIGNATURES = {
     "qedh_axpy_nrm2": [C.c_int, c_dbl_p, C.c_void_p, C.c_void_p, C.c_size_t, c_dbl_p],
     "qedh_dot_many": [C.c_int, C.POINTER(C.c_void_p), C.c_size_t, C.c_void_p, C.c_size_t, c_dbl_p],
     "qedh_axpy_many": [C.c_int, c_dbl_p, C.POINTER(C.c_void_p), C.c_size_t, C.c_void_p, C.c_size_t],
+    "qedh_dot_many_noalloc": [C.c_int, C.POINTER(C.c_void_p), C.c_size_t, C.c_void_p, C.c_size_t, c_dbl_p],
     "qedh_gemm": [
         C.c_int, C.c_char, C.c_char, C.c_size_t, C.c_size_t, C.c_size_t, c_dbl_p, C.c_void_p, C.c_size_t,
         C.c_void_p, C.c_size_t, c_dbl_p, C.c_void_p, C.c_size_t,
