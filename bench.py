@@ -247,6 +247,40 @@ def run_dynamics(args):
     return out
 
 
+def run_dynamics_finite_T(args):
+    """A bounded sample of the T > 0 half of the dynamics configuration (the whole half -- every
+    (Sz, k) source of the N = 28 ring, 28 probes each, 40 samples, krylov 200 -- keeps ~2e17
+    row-steps of work): the Heisenberg ring N = 28, source (Sz = 0, k = 0), probe S^z(pi), 40
+    samples (seed 11), krylov 200, T = 0.5 and 1, omega = linspace(0, 4, 400), eta 0.05, through the
+    batched kept-basis engine (source and target Lanczos of all samples in lockstep, overlaps as one
+    GEMM per sample)."""
+    import numpy as np
+
+    import qed_spin_amd as qs
+    from qed_spin_amd import dynamics as dy
+    from qed_spin_amd import thermal as th
+
+    n = args.dyn_sites
+    H = qs.HamiltonianBuilder(n).heisenberg([(i, (i + 1) % n) for i in range(n)], 1.0).to_operator()
+    omega = np.linspace(0.0, 4.0, 400)
+    blocks = th.walk_blocks(H, qs.Symmetry(sz=[n // 2]), tune=False)
+    k0 = [i for i, (b, s, m) in enumerate(blocks) if b.momentum == 0]
+    t0 = time.perf_counter()
+    r = dy.dynamics(H, [dy.sz_q(n, np.pi)], omega, eta=0.05, T=[0.5, 1.0], samples=40, seed=11, krylov=200,
+                    blocks=blocks, sources=k0)
+    wall = time.perf_counter() - t0
+    dim = blocks[k0[0]][1].dim
+    th.release_later(blocks)
+    e = r.engine
+    out = {"workload": f"FTLM S^z(pi) of the Heisenberg ring N={n}, source (Sz=0, k=0) ({dim} states), 40 samples, "
+                       f"krylov 200, T = 0.5, 1",
+           "wall_s": round(wall, 3), "engine": {k: round(v, 3) for k, v in e.items()},
+           "row_steps_per_s": round(2 * 40 * 200 * dim / e["lanczos_s"], 1) if e.get("lanczos_s") else None,
+           "weight_T1": float(np.trapezoid(r.S[0][1].real, omega))}
+    log(f"dynamics T>0 sample: {out}")
+    return out
+
+
 def _ref_chain(n, J2=0.35):
     """The reference's bench model chain(N) (tests/python/support/models.py:92-110, imported by
     bench/cases.py through grid.models): the J1-J2 ring, J1 = 1, J2 = 0.35."""
@@ -433,8 +467,8 @@ def main():
     ap.add_argument("--no-lanczos", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--lanes", type=int, default=0)
-    ap.add_argument("--workload", default="matvec,ftlm,dynamics,serve,refftlm,refeigs",
-                    help="matvec (the headline) and a comma list of ftlm, dynamics, serve, spectrum, refdyn, refftlm, refeigs")
+    ap.add_argument("--workload", default="matvec,ftlm,dynamics,dynT,serve,refftlm,refeigs",
+                    help="matvec (the headline) and a comma list of ftlm, dynamics, dynT, serve, spectrum, refdyn, refftlm, refeigs")
     ap.add_argument("--ftlm-sites", type=int, default=24)
     ap.add_argument("--dyn-sites", type=int, default=28)
     ap.add_argument("--spectrum-sites", type=int, default=18)
@@ -587,6 +621,8 @@ def main():
             out["ftlm"] = run_ftlm(args)
         elif w == "dynamics":
             out["dynamics"] = run_dynamics(args)
+        elif w == "dynT":
+            out["dynamics_finite_T"] = run_dynamics_finite_T(args)
         elif w == "serve":
             out["serve"] = run_serve(args)
         elif w == "spectrum":

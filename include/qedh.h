@@ -241,7 +241,9 @@ enum {
     QEDH_TUNE_UNROLL2 = 128,
     QEDH_TUNE_UNROLL16 = 256,   /* SELL-64 layout: sixteen gathers in flight per lane */
     QEDH_TUNE_CSR = 512,        /* apply through the CSR layout even where a SELL-64 copy exists */
-    QEDH_TUNE_SELL = 1024       /* build the SELL-64 copy now, whatever the operator's size */
+    QEDH_TUNE_SELL = 1024,      /* build the SELL-64 copy now, whatever the operator's size */
+    QEDH_TUNE_SELL_SUMS1 = 2048, /* SELL-64: one sum per row instead of four in a fixed tree (other bits) */
+    QEDH_TUNE_SELL_DIAG = 4096  /* SELL-64 rebuilt with each row's diagonal entry first (other bits) */
 };
 int qedh_matvec_set_tuning(qedh_matvec m, int lanes, int flags);
 /* The merged CSR (download_csr): row_ptr[rows+1], col[nnz], val[2 nnz] (re, im). */
@@ -318,13 +320,27 @@ int qedh_ftlm_batched(const qedh_matvec* blocks, int n_blocks, int samples, int 
 int qedh_cf_batched(const qedh_matvec* blocks, int n_blocks, int R, const void* const* starts, int krylov,
                     int has_shift, double energy_shift, double breakdown_tol, int n_omega, const double* omega,
                     double eta, int use_graph, double* S, double* norm_sq, double* shifts, int* steps, double* stats);
-/* Finite-temperature dynamics of one source sector (ftlm_dynamics_kernel,
- * include/ed/dynamics/ftlm_dynamics.h:110-397, for an autocorrelation A = B): samples from
- * splitmix64(seed + phi (s + 1)) (seed: the source's block seed), Lanczos with kept bases on the
- * source and on every target (full CGS2 up to max(4 krylov, 2^16) states, LocalDGKS3 above),
- * probes[t] from the source to target t (its H: target_h[t]). Outputs, un-normalised (times the
- * source dimension): Z[n_t], S[n_targets][n_t][n_omega] complex (re, im); *e_min the samples'
- * common reference; *samples_done. */
+/* Finite-temperature dynamics of a whole walk of source sectors (ftlm_dynamics_kernel,
+ * include/ed/dynamics/ftlm_dynamics.h:110-397, for every source of the T > 0 dynamics verb,
+ * src/engine/dynamics.cpp:693-1257; autocorrelations A = B). Source b's samples are drawn from
+ * splitmix64(source_seeds[b] + phi (s + 1)) (std::mt19937 -> std::normal_distribution, on the
+ * device); a channel c is a probe channel_probe[c] (CSR lane) from source channel_source[c] to a
+ * target sector whose H is channel_h[c]. All sources' samples and all channels' target runs advance
+ * through the packed multi-vector engine with kept bases (full CGS2 on sectors of at most
+ * max(4 krylov, 2^16) states, LocalDGKS3 on larger ones); the overlaps <A v_a|w_b> are products of
+ * the kept bases split over row chunks, summed per pair in a fixed order; waves of sources keep the
+ * kept vectors within budget_bytes (0: half the device memory free at the call). Outputs,
+ * un-normalised (times each source's dimension / samples done): Z[n_sources][n_t], S[n_channels][n_t][n_omega] complex
+ * (re, im); e_min[n_sources] each source's samples' common reference and samples_done[n_sources]
+ * (each may be NULL); stats[5] = Lanczos s, overlap s, host s, total s, waves (or NULL). */
+int qedh_ftlm_dynamics_walk(int n_sources, const qedh_matvec* source_h, const uint64_t* source_seeds,
+                            const double* breakdown_tol, int n_channels, const int* channel_source,
+                            const qedh_matvec* channel_h, const qedh_matvec* channel_probe, int samples, int krylov,
+                            int n_t, const double* temperatures, int n_omega, const double* omega, double eta,
+                            uint64_t budget_bytes, double* Z, double* S, double* e_min, int* samples_done,
+                            double* stats);
+/* The same for one source sector: qedh_ftlm_dynamics_walk with n_sources = 1, the targets as its
+ * channels, seed its block seed; Z[n_t], S[n_targets][n_t][n_omega], *e_min, *samples_done. */
 int qedh_ftlm_dynamics(qedh_matvec source_h, int n_targets, const qedh_matvec* target_h, const qedh_matvec* probes,
                        int samples, int krylov, uint64_t seed, double breakdown_tol, int n_t,
                        const double* temperatures, int n_omega, const double* omega, double eta, double* Z,

@@ -143,6 +143,11 @@ _SIGNATURES = {
         C.c_void_p, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_uint64, C.c_double,
         C.c_int, c_dbl_p, C.c_int, c_dbl_p, C.c_double, c_dbl_p, c_dbl_p, c_dbl_p, c_int_p,
     ],
+    "qedh_ftlm_dynamics_walk": [
+        C.c_int, C.POINTER(C.c_void_p), c_u64_p, c_dbl_p, C.c_int, c_int_p, C.POINTER(C.c_void_p),
+        C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_int, c_dbl_p, C.c_int, c_dbl_p, C.c_double, C.c_uint64, c_dbl_p,
+        c_dbl_p, c_dbl_p, c_int_p, c_dbl_p,
+    ],
     "qedh_gaussian_vectors": [C.c_uint64, C.c_uint64, C.c_int, C.c_int64, C.c_void_p],
     "qedh_block_spectrum": [C.POINTER(C.c_void_p), C.c_int, C.c_int64, c_dbl_p],
     "qedh_exact_thermal": [

@@ -432,6 +432,147 @@ __global__ void k_pack_csr(CsrView A, u64 row_off, u64 col_off, u64 nnz_off, u64
     }
 }
 
+// Full Gram-Schmidt against kept basis layers (layer k at basis + k * layer, the batch layout), for
+// the blocks flagged in full. Coefficients: one workgroup per (block, k); thread (slot, s) sums
+// <V_k, W> over the block's rows r = first + slot, first + slot + slots, ..., then the slots' sums
+// in slot order: coef[(b * kmax + k) * R + s]. Deterministic, independent of the other blocks.
+__global__ __launch_bounds__(kThreads) void k_batch_gs_coef(BatchView B, const Zd* basis, u64 layer, int K, int kmax,
+                                                            const Zd* W, const int* full, Zd* coef) {
+    const u32 b = blockIdx.x / static_cast<u32>(K);
+    const int k = static_cast<int>(blockIdx.x % static_cast<u32>(K));
+    if (!full[b]) return;   // the whole workgroup
+    const int R = B.R, slots = kThreads / R;
+    const int slot = threadIdx.x / R, s = threadIdx.x % R;
+    const u64 first = B.tiles[B.tile_begin[b]].row_begin, last = B.tiles[B.tile_begin[b + 1] - 1].row_end;
+    double pr = 0.0, pi = 0.0;
+    if (slot < slots && pair_live(B, b, s)) {
+        const Zd* V = basis + static_cast<u64>(k) * layer;
+        for (u64 r = first + slot; r < last; r += slots) {
+            const u64 i = r * R + s;
+            const Zd d = cdot(V[i], W[i]);
+            pr += d.re;
+            pi += d.im;
+        }
+    }
+    __shared__ double sr[kThreads], si[kThreads];
+    sr[threadIdx.x] = pr;
+    si[threadIdx.x] = pi;
+    __syncthreads();
+    if (static_cast<int>(threadIdx.x) < R) {
+        double a = 0.0, c = 0.0;
+        for (int q = 0; q < slots; ++q) {
+            a += sr[q * R + threadIdx.x];
+            c += si[q * R + threadIdx.x];
+        }
+        coef[(static_cast<u64>(b) * kmax + k) * R + threadIdx.x] = Zd{a, c};
+    }
+}
+
+// W -= sum_{k < K} coef_k V_k on the tiles of the flagged blocks, k ascending.
+__global__ __launch_bounds__(kThreads) void k_batch_gs_update(BatchView B, const Zd* basis, u64 layer, int K, int kmax,
+                                                              Zd* W, const int* full, const Zd* coef) {
+    const Tile t = B.tiles[blockIdx.x];
+    if (!full[t.block]) return;
+    const int R = B.R, slots = kThreads / R;
+    const int slot = threadIdx.x / R, s = threadIdx.x % R;
+    if (slot >= slots || !pair_live(B, t.block, s)) return;
+    const Zd* c = coef + static_cast<u64>(t.block) * kmax * R + s;
+    for (u64 r = t.row_begin + slot; r < t.row_end; r += slots) {
+        const u64 i = r * R + s;
+        Zd w = W[i];
+        for (int k = 0; k < K; ++k) {
+            const Zd m = cmul(c[static_cast<u64>(k) * R], basis[static_cast<u64>(k) * layer + i]);
+            w = Zd{w.re - m.re, w.im - m.im};
+        }
+        W[i] = w;
+    }
+}
+
+// Layers [K][rows][R] to [R][K][rows]: each pair's vectors contiguous, layer after layer (the
+// column-major panels a library GEMM reads).
+__global__ void k_batch_layers_transpose(const Zd* in, u64 rows, int R, int K, Zd* out) {
+    const u64 n = rows * static_cast<u64>(R) * static_cast<u64>(K);
+    for (u64 i = blockIdx.x * static_cast<u64>(blockDim.x) + threadIdx.x; i < n; i += static_cast<u64>(gridDim.x) * blockDim.x) {
+        const u64 s = i % static_cast<u64>(R), kr = i / static_cast<u64>(R);
+        const u64 r = kr % rows, k = kr / rows;
+        out[(s * static_cast<u64>(K) + k) * rows + r] = in[i];
+    }
+}
+
+// Kept-basis overlaps X^H Y per chunk of rows: a workgroup owns a 64 x 64 tile of the (a, b) product
+// of one chunk. 32-row slices of its 64 X and 64 Y vectors are staged in LDS with the vector index
+// fastest, and thread (tx, ty) accumulates a = a0 + ty + 16 i, b = b0 + tx + 16 j (i, j < 4) over the
+// chunk's rows in order, with explicit fused multiply-adds: the 16 threads of a row of the wave read
+// 16 consecutive Y entries (no bank conflicts) and share 4 X entries (broadcasts).
+constexpr int kOvTile = 64, kOvRows = 32;
+__global__ __launch_bounds__(256) void k_overlap_chunk(const Zd* X, u64 ldx, const Zd* Y, u64 ldy,
+                                                       const OverlapChunk* chunks, const int* ma, const int* mb,
+                                                       int kmax, Zd* partial) {
+    __shared__ Zd xs[kOvRows][kOvTile + 1], ys[kOvRows][kOvTile + 1];   // padded: the row-major stores spread over banks
+    const OverlapChunk ch = chunks[blockIdx.x];
+    const int a0 = blockIdx.y * kOvTile, b0 = blockIdx.z * kOvTile;
+    const int na = ma[ch.pair], nb = mb[ch.pair];
+    if (a0 >= na || b0 >= nb) return;   // the whole workgroup: tile outside this pair's product
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    double acc_r[4][4], acc_i[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc_r[i][j] = acc_i[i][j] = 0.0;
+    for (u32 r0 = 0; r0 < ch.rows; r0 += kOvRows) {
+        for (int idx = threadIdx.x; idx < kOvTile * kOvRows; idx += 256) {
+            const int k = idx / kOvRows, rr = idx % kOvRows;   // consecutive threads: consecutive rows
+            const u32 row = r0 + static_cast<u32>(rr);
+            const bool in_row = row < ch.rows;
+            xs[rr][k] = (in_row && a0 + k < na) ? X[ch.x_base + static_cast<u64>(a0 + k) * ldx + row] : Zd{0.0, 0.0};
+            ys[rr][k] = (in_row && b0 + k < nb) ? Y[ch.y_base + static_cast<u64>(b0 + k) * ldy + row] : Zd{0.0, 0.0};
+        }
+        __syncthreads();
+        for (int rr = 0; rr < kOvRows; ++rr) {
+            Zd xv[4], yv[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) xv[i] = xs[rr][ty + 16 * i];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) yv[j] = ys[rr][tx + 16 * j];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    acc_r[i][j] = fma(xv[i].re, yv[j].re, fma(xv[i].im, yv[j].im, acc_r[i][j]));
+                    acc_i[i][j] = fma(xv[i].re, yv[j].im, fma(-xv[i].im, yv[j].re, acc_i[i][j]));
+                }
+        }
+        __syncthreads();
+    }
+    Zd* out = partial + static_cast<u64>(blockIdx.x) * kmax * kmax;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int a = a0 + ty + 16 * i, b = b0 + tx + 16 * j;
+            if (a < na && b < nb) out[static_cast<u64>(b) * na + a] = Zd{acc_r[i][j], acc_i[i][j]};
+        }
+}
+
+// omega[p](a, b) = the sum of the pair's chunk partials in chunk order; one thread per (pair, a, b).
+__global__ void k_overlap_reduce(const Zd* partial, const u32* chunk_begin, const int* ma, const int* mb, u32 n_pairs,
+                                 int kmax, Zd* omega) {
+    const u64 per = static_cast<u64>(kmax) * kmax;
+    for (u64 i = blockIdx.x * static_cast<u64>(blockDim.x) + threadIdx.x; i < n_pairs * per;
+         i += static_cast<u64>(gridDim.x) * blockDim.x) {
+        const u32 p = static_cast<u32>(i / per);
+        const u64 e = i % per;
+        if (e >= static_cast<u64>(ma[p]) * mb[p]) continue;
+        double re = 0.0, im = 0.0;
+        for (u32 q = chunk_begin[p]; q < chunk_begin[p + 1]; ++q) {
+            const Zd v = partial[static_cast<u64>(q) * per + e];
+            re += v.re;
+            im += v.im;
+        }
+        omega[static_cast<u64>(p) * per + e] = Zd{re, im};
+    }
+}
+
 unsigned grid_for(u64 n) {
     u64 g = (n + 255) / 256;
     if (g > 65536) g = 65536;
@@ -521,6 +662,41 @@ void csr_to_dense(const CsrView& A, Zd* out, hipStream_t st) {
     if (!n) return;
     batch_fill_zero(out, n * n, st);
     hipLaunchKernelGGL(k_csr_to_dense, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, st, A, out);
+}
+
+void batch_gram_schmidt(const BatchView& B, const Zd* basis, u64 layer, int K, int kmax, Zd* W, const int* full,
+                        Zd* coef, hipStream_t st) {
+    if (!B.n_tiles || K < 1) return;
+    hipLaunchKernelGGL(k_batch_gs_coef, dim3(B.n_blocks * static_cast<u32>(K)), dim3(kThreads), 0, st, B, basis, layer,
+                       K, kmax, W, full, coef);
+    hipLaunchKernelGGL(k_batch_gs_update, dim3(B.n_tiles), dim3(kThreads), 0, st, B, basis, layer, K, kmax, W, full,
+                       coef);
+}
+
+void batch_norm_partial(const BatchView& B, const Zd* V, double* partial, hipStream_t st) {
+    if (!B.n_tiles) return;
+    hipLaunchKernelGGL(k_batch_norm_partial, dim3(B.n_tiles), dim3(kThreads), 0, st, B, V, partial);
+}
+
+void batch_layers_transpose(const Zd* in, u64 rows, int R, int K, Zd* out, hipStream_t st) {
+    const u64 n = rows * static_cast<u64>(R) * static_cast<u64>(K);
+    if (!n) return;
+    hipLaunchKernelGGL(k_batch_layers_transpose, dim3(grid_for(n)), dim3(256), 0, st, in, rows, R, K, out);
+}
+
+void overlap_chunks(const Zd* X, u64 ldx, const Zd* Y, u64 ldy, const OverlapChunk* chunks, u32 n_chunks,
+                    const int* ma, const int* mb, int kmax, Zd* partial, hipStream_t st) {
+    if (!n_chunks) return;
+    const unsigned tiles = static_cast<unsigned>((kmax + kOvTile - 1) / kOvTile);
+    hipLaunchKernelGGL(k_overlap_chunk, dim3(n_chunks, tiles, tiles), dim3(256), 0, st, X, ldx, Y, ldy, chunks, ma, mb,
+                       kmax, partial);
+}
+
+void overlap_reduce(const Zd* partial, const u32* chunk_begin, const int* ma, const int* mb, u32 n_pairs, int kmax,
+                    Zd* omega, hipStream_t st) {
+    if (!n_pairs) return;
+    hipLaunchKernelGGL(k_overlap_reduce, dim3(grid_for(static_cast<u64>(n_pairs) * kmax * kmax)), dim3(256), 0, st,
+                       partial, chunk_begin, ma, mb, n_pairs, kmax, omega);
 }
 
 void batch_advance(const BatchView& B, hipStream_t st) {

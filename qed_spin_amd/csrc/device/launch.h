@@ -56,6 +56,16 @@ bool csr_count(const SectorView& R, const SectorView& C, bool same, const Progra
 /// when real) at row_ptr.
 void csr_fill(const SectorView& R, const SectorView& C, bool same, const ProgramView& P, const u64* row_ptr,
               u32* col, void* vals, int width, bool real, const DictTable& D, hipStream_t st);
+/// The build in one walk: each row merged once into a stage (entry e of row r at
+/// stage[e * rows + r], at most `acting groups` entries per row) with its length; then the
+/// dictionary from the staged values (csr_stage_dict, the same set of values csr_count inserts) and,
+/// after the row scan, the CSR from the stage (csr_compact: what csr_fill writes, bit for bit).
+/// false (nothing launched) when the program has more groups than a row can hold.
+bool csr_walk(const SectorView& R, const SectorView& C, bool same, const ProgramView& P, u32* row_len, u32* stage_col,
+              Zd* stage_val, hipStream_t st);
+void csr_stage_dict(i64 rows, const u32* row_len, const Zd* stage_val, const DictTable& D, hipStream_t st);
+void csr_compact(i64 rows, const u64* row_ptr, const u32* stage_col, const Zd* stage_val, u32* col, void* vals,
+                 int width, bool real, const DictTable& D, hipStream_t st);
 std::size_t row_scan_temp_bytes(i64 rows);
 void row_scan(const u32* row_len, i64 rows, u64* row_ptr, void* temp, std::size_t temp_bytes, hipStream_t st);
 
@@ -80,6 +90,7 @@ struct CsrView {
     // 64 s + j at sell_ptr[s] + 64 k + j; sell_len[row] is the row's entry count. One lane per row.
     const u64* sell_ptr = nullptr;
     const u16* sell_len = nullptr;
+    int sell_acc = 1;              // SELL: partial sums per row (1, or 4 combined by a fixed tree)
 };
 /// out_v = A in_v (real: double vectors; else complex vectors, with real or complex values).
 void csr_spmv(const CsrView& A, bool real, const VecSet& v, hipStream_t st);
@@ -99,7 +110,7 @@ std::size_t sell_scan_temp_bytes(i64 n_slices);
 void sell_scan(const u64* slice_words, i64 n_slices, u64* sell_ptr, void* temp, std::size_t temp_bytes,
                hipStream_t st);
 void sell_fill(const u64* row_ptr, const u32* col, const void* vals, int value_bytes, i64 rows, const u64* sell_ptr,
-               u32* sell_col, void* sell_vals, hipStream_t st);
+               u32* sell_col, void* sell_vals, bool diag_first, bool packed, hipStream_t st);
 
 // ---- blas.hip (vectors of T = double or Zd; n entries) -------------------------------------------
 /// Partial-sum scratch a reduction needs (device), sized by blas_scratch_bytes; never more than
@@ -215,6 +226,31 @@ void sample_gauss(Zd* V, int R, const u64* row_begin, const u64* dims, const u32
 void pack_csr(const CsrView& A, u64 row_off, u64 col_off, u64 nnz_off, u64* row_ptr, u32* col, Zd* val, u16* vid,
               u32 dict_off, hipStream_t st);
 void batch_advance(const BatchView& B, hipStream_t st);
+/// One classical Gram-Schmidt pass of the batch vectors W against kept layers 0 .. K-1 (layer k at
+/// basis + k * layer entries, the batch layout), only for the blocks whose full[b] is set:
+/// coef[(b * kmax + k) * R + s] = <V_k, W> per pair (fixed-order sums), then W -= sum_k coef_k V_k.
+void batch_gram_schmidt(const BatchView& B, const Zd* basis, u64 layer, int K, int kmax, Zd* W, const int* full,
+                        Zd* coef, hipStream_t st);
+/// partial[tile][s] = the tile's part of ||V_s||^2 (what batch_finish kind 2 sums into beta).
+void batch_norm_partial(const BatchView& B, const Zd* V, double* partial, hipStream_t st);
+/// Layers [K][rows][R] to [R][K][rows] (each pair's K vectors contiguous, for a library GEMM).
+void batch_layers_transpose(const Zd* in, u64 rows, int R, int K, Zd* out, hipStream_t st);
+
+/// One chunk of rows of one (channel, sample) pair of the kept-basis overlaps: its rows start at
+/// x_base / y_base (element offsets into the panels; layer k of the pair at base + k * ld).
+struct OverlapChunk {
+    u64 x_base, y_base;
+    u32 rows, pair;
+};
+/// partial[q] (column-major ma x mb of chunk q's pair, at partial + q * kmax * kmax) = the chunk's
+/// X^H Y over its rows in order (fused multiply-adds).
+void overlap_chunks(const Zd* X, u64 ldx, const Zd* Y, u64 ldy, const OverlapChunk* chunks, u32 n_chunks,
+                    const int* ma, const int* mb, int kmax, Zd* partial, hipStream_t st);
+/// omega[p] (column-major ma[p] x mb[p], at omega + p * kmax * kmax) = the sum, in chunk order, of
+/// the pair's chunk products partial[q] (q = chunk_begin[p] .. chunk_begin[p + 1], each column-major
+/// ma[p] x mb[p] at partial + q * kmax * kmax).
+void overlap_reduce(const Zd* partial, const u32* chunk_begin, const int* ma, const int* mb, u32 n_pairs, int kmax,
+                    Zd* omega, hipStream_t st);
 /// A square sector CSR (rows x rows) as a dense column-major complex matrix, zero-filled first.
 void csr_to_dense(const CsrView& A, Zd* out, hipStream_t st);
 }  // namespace qsa::dev

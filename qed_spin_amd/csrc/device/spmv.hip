@@ -223,26 +223,37 @@ __global__ __launch_bounds__(256) void k_spmv_block(CsrView A, const void* X, vo
 // sum depends neither on the workgroup size nor on how many gathers are in flight, and a k-vector
 // apply equals k single applies bit for bit. U entries per round: their column words first, then
 // the U gathers together, then the sums in entry order.
-template <int WIDTH, bool VREAL, bool REAL, int NV, int U>
+// ACC partial sums per row and vector: entry k goes to sum k mod ACC (k in the row's order), and
+// the sums meet in a fixed tree, (s0 + s2) + (s1 + s3) for ACC = 4 -- the order a 4-lane butterfly
+// gives, so the block kernel's 4 lanes per row produce the same bits.
+template <int ACC> __device__ __forceinline__ double tree(const double* s) {
+    if constexpr (ACC == 4) return (s[0] + s[2]) + (s[1] + s[3]);
+    else return s[0];
+}
+
+template <int WIDTH, bool VREAL, bool REAL, int NV, int U, int ACC>
 __global__ __launch_bounds__(1024) void k_sell(CsrView A, Ptrs<NV> v) {
+    static_assert(U % ACC == 0 || U == 1, "unroll in whole rounds of the partial sums");
     __shared__ double sre[(WIDTH == 1 || WIDTH == 3) ? 256 : 1], sim[(WIDTH == 1 || WIDTH == 3) && !VREAL ? 256 : 1];
     const Dict d = stage_dict<WIDTH, VREAL>(A, sre, sim);
     const i64 row = row_block(A) * blockDim.x + threadIdx.x;
     if (row >= A.rows) return;
     const u32 n = A.nt ? __builtin_nontemporal_load(A.sell_len + row) : A.sell_len[row];
     const u64 base = A.sell_ptr[row >> 6] + static_cast<u64>(row & 63);
-    double ar[NV], ai[NV];
+    double ar[NV][ACC], ai[NV][ACC];
 #pragma unroll
-    for (int k = 0; k < NV; ++k) ar[k] = ai[k] = 0.0;
-    auto accumulate = [&](u32 c, double vr, double vi) {
+    for (int k = 0; k < NV; ++k)
+#pragma unroll
+        for (int j = 0; j < ACC; ++j) ar[k][j] = ai[k][j] = 0.0;
+    auto accumulate = [&](int j, u32 c, double vr, double vi) {
 #pragma unroll
         for (int k = 0; k < NV; ++k) {
             if constexpr (REAL) {
-                ar[k] += vr * static_cast<const double*>(v.in[k])[c];
+                ar[k][j] += vr * static_cast<const double*>(v.in[k])[c];
             } else {
                 const Zd x = static_cast<const Zd*>(v.in[k])[c];
-                ar[k] += vr * x.re - vi * x.im;
-                ai[k] += vr * x.im + vi * x.re;
+                ar[k][j] += vr * x.re - vi * x.im;
+                ai[k][j] += vr * x.im + vi * x.re;
             }
         }
     };
@@ -261,10 +272,10 @@ __global__ __launch_bounds__(1024) void k_sell(CsrView A, Ptrs<NV> v) {
 #pragma unroll
                 for (int u = 0; u < U; ++u) x[u] = static_cast<const double*>(v.in[0])[c[u]];
 #pragma unroll
-                for (int u = 0; u < U; ++u) ar[0] += vr[u] * x[u];
+                for (int u = 0; u < U; ++u) ar[0][u % ACC] += vr[u] * x[u];
             } else {
 #pragma unroll
-                for (int u = 0; u < U; ++u) accumulate(c[u], vr[u], vi[u]);
+                for (int u = 0; u < U; ++u) accumulate(u % ACC, c[u], vr[u], vi[u]);
             }
         }
     }
@@ -272,47 +283,62 @@ __global__ __launch_bounds__(1024) void k_sell(CsrView A, Ptrs<NV> v) {
         u32 c;
         double vr, vi = 0.0;
         load_entry<WIDTH, VREAL>(A, d, base + static_cast<u64>(k) * 64, c, vr, vi);
-        accumulate(c, vr, vi);
+        accumulate(static_cast<int>(k % ACC), c, vr, vi);
     }
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
-        if constexpr (REAL) static_cast<double*>(v.out[q])[row] = ar[q];
-        else static_cast<Zd*>(v.out[q])[row] = Zd{ar[q], ai[q]};
+        if constexpr (REAL) static_cast<double*>(v.out[q])[row] = tree<ACC>(ar[q]);
+        else static_cast<Zd*>(v.out[q])[row] = Zd{tree<ACC>(ar[q]), tree<ACC>(ai[q])};
     }
 }
 
-// Block-interleaved vectors X[row * NV + k] on the SELL layout (the same order per row).
-template <int WIDTH, bool VREAL, bool REAL, int NV>
+// Block-interleaved vectors X[row * NV + k] on the SELL layout. ACC lanes share a row, lane l
+// taking entries k = l, l + ACC, ... in order (the single-vector kernel's partial sums), combined
+// by the xor butterfly whose order is that kernel's tree: the same bits as single applies.
+template <int WIDTH, bool VREAL, bool REAL, int NV, int ACC>
 __global__ __launch_bounds__(256) void k_sell_block(CsrView A, const void* X, void* Y) {
     __shared__ double sre[(WIDTH == 1 || WIDTH == 3) ? 256 : 1], sim[(WIDTH == 1 || WIDTH == 3) && !VREAL ? 256 : 1];
     const Dict d = stage_dict<WIDTH, VREAL>(A, sre, sim);
-    const i64 row = row_block(A) * blockDim.x + threadIdx.x;
-    if (row >= A.rows) return;
-    const u32 n = A.sell_len[row];
-    const u64 base = A.sell_ptr[row >> 6] + static_cast<u64>(row & 63);
+    const i64 gid = row_block(A) * blockDim.x + threadIdx.x;
+    const i64 row = gid / ACC;
+    const int lane = static_cast<int>(gid % ACC);
     double ar[NV], ai[NV];
 #pragma unroll
     for (int q = 0; q < NV; ++q) ar[q] = ai[q] = 0.0;
-    for (u32 k = 0; k < n; ++k) {
-        u32 c32;
-        double vr, vi = 0.0;
-        load_entry<WIDTH, VREAL>(A, d, base + static_cast<u64>(k) * 64, c32, vr, vi);
-        const u64 c = c32;
+    if (row < A.rows) {
+        const u32 n = A.sell_len[row];
+        const u64 base = A.sell_ptr[row >> 6] + static_cast<u64>(row & 63);
+        for (u32 k = static_cast<u32>(lane); k < n; k += ACC) {
+            u32 c32;
+            double vr, vi = 0.0;
+            load_entry<WIDTH, VREAL>(A, d, base + static_cast<u64>(k) * 64, c32, vr, vi);
+            const u64 c = c32;
 #pragma unroll
-        for (int q = 0; q < NV; ++q) {
-            if constexpr (REAL) {
-                ar[q] += vr * static_cast<const double*>(X)[c * NV + q];
-            } else {
-                const Zd x = static_cast<const Zd*>(X)[c * NV + q];
-                ar[q] += vr * x.re - vi * x.im;
-                ai[q] += vr * x.im + vi * x.re;
+            for (int q = 0; q < NV; ++q) {
+                if constexpr (REAL) {
+                    ar[q] += vr * static_cast<const double*>(X)[c * NV + q];
+                } else {
+                    const Zd x = static_cast<const Zd*>(X)[c * NV + q];
+                    ar[q] += vr * x.re - vi * x.im;
+                    ai[q] += vr * x.im + vi * x.re;
+                }
             }
         }
     }
 #pragma unroll
-    for (int q = 0; q < NV; ++q) {
-        if constexpr (REAL) static_cast<double*>(Y)[row * NV + q] = ar[q];
-        else static_cast<Zd*>(Y)[row * NV + q] = Zd{ar[q], ai[q]};
+    for (int off = ACC / 2; off > 0; off >>= 1) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+            ar[q] += __shfl_xor(ar[q], off, ACC);
+            if constexpr (!REAL) ai[q] += __shfl_xor(ai[q], off, ACC);
+        }
+    }
+    if (row < A.rows && lane == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+            if constexpr (REAL) static_cast<double*>(Y)[row * NV + q] = ar[q];
+            else static_cast<Zd*>(Y)[row * NV + q] = Zd{ar[q], ai[q]};
+        }
     }
 }
 
@@ -329,11 +355,17 @@ template <int WIDTH, bool VREAL, bool REAL> void launch_sell_set(const CsrView& 
                 p.in[k] = v.in[done + k];
                 p.out[k] = v.out[done + k];
             }
-            if (NV == 1 && A.unroll >= 16) hipLaunchKernelGGL((k_sell<WIDTH, VREAL, REAL, NV, 16>), grid, block, 0, st, A, p);
-            else if (NV == 1 && A.unroll >= 8) hipLaunchKernelGGL((k_sell<WIDTH, VREAL, REAL, NV, 8>), grid, block, 0, st, A, p);
-            else if (A.unroll >= 4) hipLaunchKernelGGL((k_sell<WIDTH, VREAL, REAL, NV, 4>), grid, block, 0, st, A, p);
-            else if (A.unroll >= 2) hipLaunchKernelGGL((k_sell<WIDTH, VREAL, REAL, NV, 2>), grid, block, 0, st, A, p);
-            else hipLaunchKernelGGL((k_sell<WIDTH, VREAL, REAL, NV, 1>), grid, block, 0, st, A, p);
+            if (A.sell_acc == 4) {
+                if (NV == 1 && A.unroll >= 16) hipLaunchKernelGGL((k_sell<WIDTH, VREAL, REAL, NV, 16, 4>), grid, block, 0, st, A, p);
+                else if (NV == 1 && A.unroll >= 8) hipLaunchKernelGGL((k_sell<WIDTH, VREAL, REAL, NV, 8, 4>), grid, block, 0, st, A, p);
+                else hipLaunchKernelGGL((k_sell<WIDTH, VREAL, REAL, NV, 4, 4>), grid, block, 0, st, A, p);
+            } else {
+                if (NV == 1 && A.unroll >= 16) hipLaunchKernelGGL((k_sell<WIDTH, VREAL, REAL, NV, 16, 1>), grid, block, 0, st, A, p);
+                else if (NV == 1 && A.unroll >= 8) hipLaunchKernelGGL((k_sell<WIDTH, VREAL, REAL, NV, 8, 1>), grid, block, 0, st, A, p);
+                else if (A.unroll >= 4) hipLaunchKernelGGL((k_sell<WIDTH, VREAL, REAL, NV, 4, 1>), grid, block, 0, st, A, p);
+                else if (A.unroll >= 2) hipLaunchKernelGGL((k_sell<WIDTH, VREAL, REAL, NV, 2, 1>), grid, block, 0, st, A, p);
+                else hipLaunchKernelGGL((k_sell<WIDTH, VREAL, REAL, NV, 1, 1>), grid, block, 0, st, A, p);
+            }
         };
         if (nv == 8) go(std::integral_constant<int, 8>{});
         else if (nv == 4) go(std::integral_constant<int, 4>{});
@@ -343,18 +375,23 @@ template <int WIDTH, bool VREAL, bool REAL> void launch_sell_set(const CsrView& 
     }
 }
 
-template <int WIDTH, bool VREAL, bool REAL>
-void launch_sell_block(const CsrView& A, int nv, const void* X, void* Y, hipStream_t st) {
+template <int WIDTH, bool VREAL, bool REAL, int ACC>
+void launch_sell_block_acc(const CsrView& A, int nv, const void* X, void* Y, hipStream_t st) {
     CsrView B = A;
     B.wg = 256;
-    const dim3 grid(grid_blocks(B, static_cast<u64>(A.rows))), block(256);
+    const dim3 grid(grid_blocks(B, static_cast<u64>(A.rows) * ACC)), block(256);
     switch (nv) {
-    case 1: hipLaunchKernelGGL((k_sell_block<WIDTH, VREAL, REAL, 1>), grid, block, 0, st, B, X, Y); break;
-    case 2: hipLaunchKernelGGL((k_sell_block<WIDTH, VREAL, REAL, 2>), grid, block, 0, st, B, X, Y); break;
-    case 4: hipLaunchKernelGGL((k_sell_block<WIDTH, VREAL, REAL, 4>), grid, block, 0, st, B, X, Y); break;
-    case 8: hipLaunchKernelGGL((k_sell_block<WIDTH, VREAL, REAL, 8>), grid, block, 0, st, B, X, Y); break;
+    case 1: hipLaunchKernelGGL((k_sell_block<WIDTH, VREAL, REAL, 1, ACC>), grid, block, 0, st, B, X, Y); break;
+    case 2: hipLaunchKernelGGL((k_sell_block<WIDTH, VREAL, REAL, 2, ACC>), grid, block, 0, st, B, X, Y); break;
+    case 4: hipLaunchKernelGGL((k_sell_block<WIDTH, VREAL, REAL, 4, ACC>), grid, block, 0, st, B, X, Y); break;
+    case 8: hipLaunchKernelGGL((k_sell_block<WIDTH, VREAL, REAL, 8, ACC>), grid, block, 0, st, B, X, Y); break;
     default: break;
     }
+}
+template <int WIDTH, bool VREAL, bool REAL>
+void launch_sell_block(const CsrView& A, int nv, const void* X, void* Y, hipStream_t st) {
+    if (A.sell_acc == 4) launch_sell_block_acc<WIDTH, VREAL, REAL, 4>(A, nv, X, Y, st);
+    else launch_sell_block_acc<WIDTH, VREAL, REAL, 1>(A, nv, X, Y, st);
 }
 
 template <int WIDTH, bool VREAL, bool REAL, int L> void launch_set(const CsrView& A, const VecSet& v, hipStream_t st) {
@@ -471,14 +508,24 @@ __global__ __launch_bounds__(256) void k_sell_lengths(const u64* row_ptr, i64 ro
     if ((threadIdx.x & 63) == 0 && r < rows) slice_words[r >> 6] = static_cast<u64>(m) * 64;
 }
 
+// diag_first (a square operator): a row's diagonal entry, when it has one, moves to its first slot,
+// the others keeping their order -- the k = 0 loads of a wavefront then read x[row] for 64
+// consecutive rows, one coalesced line instead of 64 gathers.
 __global__ __launch_bounds__(256) void k_sell_fill(const u64* row_ptr, const u32* col, const void* vals, int vb,
-                                                   i64 rows, const u64* sell_ptr, u32* sell_col, void* sell_vals) {
+                                                   i64 rows, const u64* sell_ptr, u32* sell_col, void* sell_vals,
+                                                   int diag_first, int packed) {
     const i64 r = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (r >= rows) return;
     const u64 e0 = row_ptr[r], n = row_ptr[r + 1] - e0;
     const u64 base = sell_ptr[r >> 6] + static_cast<u64>(r & 63);
+    u64 diag = n;   // the diagonal entry's position in the row (n: none)
+    if (diag_first)
+        for (u64 k = 0; k < n && diag == n; ++k)
+            if ((packed ? (col[e0 + k] & 0xFFFFFFu) : col[e0 + k]) == static_cast<u32>(r)) diag = k;
     for (u64 k = 0; k < n; ++k) {
-        const u64 dst = base + k * 64, src = e0 + k;
+        // slot k takes the diagonal first, then the row's other entries in order
+        const u64 from = diag == n ? k : (k == 0 ? diag : (k <= diag ? k - 1 : k));
+        const u64 dst = base + k * 64, src = e0 + from;
         sell_col[dst] = col[src];
         if (vb == 1) static_cast<u8*>(sell_vals)[dst] = static_cast<const u8*>(vals)[src];
         else if (vb == 2) static_cast<u16*>(sell_vals)[dst] = static_cast<const u16*>(vals)[src];
@@ -508,11 +555,11 @@ void sell_scan(const u64* slice_words, i64 n_slices, u64* sell_ptr, void* temp, 
 }
 
 void sell_fill(const u64* row_ptr, const u32* col, const void* vals, int value_bytes, i64 rows, const u64* sell_ptr,
-               u32* sell_col, void* sell_vals, hipStream_t st) {
+               u32* sell_col, void* sell_vals, bool diag_first, bool packed, hipStream_t st) {
     if (rows <= 0) return;
     const u64 g = (static_cast<u64>(rows) + 255) / 256;
     hipLaunchKernelGGL(k_sell_fill, dim3(static_cast<unsigned>(g)), dim3(256), 0, st, row_ptr, col, vals, value_bytes,
-                       rows, sell_ptr, sell_col, sell_vals);
+                       rows, sell_ptr, sell_col, sell_vals, diag_first ? 1 : 0, packed ? 1 : 0);
 }
 
 void pack_col_ids(u32* col, const u8* id, u64 nnz, hipStream_t st) {

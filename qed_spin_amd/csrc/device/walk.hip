@@ -253,7 +253,156 @@ __global__ __launch_bounds__(256) void k_csr_fill(SectorView R, SectorView C, in
     }
 }
 
+// ---- one walk: the merged rows staged, then the dictionary and the CSR from the stage ---------
+// Entry e of row r sits at stage[e * rows + r] (a column per entry slot: the threads of a wave
+// write and read consecutive rows together); the walk, the most expensive part of the build, runs
+// once instead of once per pass.
+template <bool SLC>
+__global__ __launch_bounds__(256) void k_csr_walk(SectorView R, SectorView C, int same, ProgramView P, u32* row_len,
+                                                  u32* stage_col, Zd* stage_val) {
+    const i64 r = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= R.dim) return;
+    u32 cols[kMaxRow];
+    double vre[kMaxRow], vim[kMaxRow];
+    const int n = merged_row<SLC>(R, C, same != 0, P, r, cols, vre, vim);
+    row_len[r] = static_cast<u32>(n);
+    const u64 rows = static_cast<u64>(R.dim);
+    for (int e = 0; e < n; ++e) {
+        stage_col[static_cast<u64>(e) * rows + r] = cols[e];
+        stage_val[static_cast<u64>(e) * rows + r] = Zd{vre[e], vim[e]};
+    }
+}
+
+// The dictionary from the staged values: per workgroup an LDS table of its rows' distinct values,
+// then into the global table (the same set of values as the walking count pass inserts).
+__global__ __launch_bounds__(256) void k_csr_stage_dict(i64 rows_n, const u32* row_len, const Zd* stage_val,
+                                                        DictTable D) {
+    __shared__ unsigned long long s_fp[kLdsSlots];
+    __shared__ u64 s_re[kLdsSlots], s_im[kLdsSlots];
+    __shared__ int s_full;
+    for (int i = threadIdx.x; i < kLdsSlots; i += blockDim.x) s_fp[i] = 0ULL;
+    if (threadIdx.x == 0) s_full = 0;
+    __syncthreads();
+    const i64 r = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r < rows_n) {
+        const int n = static_cast<int>(row_len[r]);
+        const u64 rows = static_cast<u64>(rows_n);
+        for (int e = 0; e < n; ++e) {
+            const Zd v = stage_val[static_cast<u64>(e) * rows + r];
+            const u64 re = bits_of(v.re), im = bits_of(v.im);
+            const u64 fp = fingerprint(re, im);
+            const int h = static_cast<int>((fp >> 17) & (kLdsSlots - 1));
+            bool placed = false;
+            for (int p = 0; p < kLdsSlots && !placed; ++p) {
+                const int i = (h + p) & (kLdsSlots - 1);
+                const unsigned long long old = atomicCAS(&s_fp[i], 0ULL, static_cast<unsigned long long>(fp));
+                if (old == 0ULL) {
+                    s_re[i] = re;
+                    s_im[i] = im;
+                    placed = true;
+                } else if (old == fp) {
+                    placed = true;
+                }
+            }
+            if (!placed) s_full = 1;
+        }
+    }
+    __syncthreads();
+    if (s_full) {
+        if (threadIdx.x == 0) atomicOr(D.flags, 1u);
+        return;
+    }
+    for (int i = threadIdx.x; i < kLdsSlots; i += blockDim.x) {
+        const u64 fp = s_fp[i];
+        if (!fp) continue;
+        const int h = static_cast<int>((fp >> 17) & (DictTable::kSlots - 1));
+        bool placed = false;
+        for (int p = 0; p < 4096 && !placed; ++p) {
+            const int j = (h + p) & (DictTable::kSlots - 1);
+            const unsigned long long old =
+                atomicCAS(reinterpret_cast<unsigned long long*>(&D.fp[j]), 0ULL, static_cast<unsigned long long>(fp));
+            if (old == 0ULL) {
+                D.re[j] = s_re[i];
+                D.im[j] = s_im[i];
+                atomicAdd(D.count, 1u);
+                placed = true;
+            } else if (old == fp) {
+                placed = true;
+            }
+        }
+        if (!placed) atomicOr(D.flags, 1u);
+    }
+}
+
+// The CSR from the stage: columns, and value ids (width 1 / 2) or full values (width 0).
+template <int WIDTH, bool REAL>
+__global__ __launch_bounds__(256) void k_csr_compact(i64 rows_n, const u64* row_ptr, const u32* stage_col,
+                                                     const Zd* stage_val, u32* col, void* vals, DictTable D) {
+    const i64 r = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= rows_n) return;
+    const u64 rows = static_cast<u64>(rows_n);
+    const u64 e0 = row_ptr[r], n = row_ptr[r + 1] - e0;
+    for (u64 e = 0; e < n; ++e) {
+        col[e0 + e] = stage_col[e * rows + r];
+        const Zd v = stage_val[e * rows + r];
+        if constexpr (WIDTH == 0) {
+            if constexpr (REAL) static_cast<double*>(vals)[e0 + e] = v.re;
+            else static_cast<Zd*>(vals)[e0 + e] = v;
+        } else {
+            const u64 re = bits_of(v.re), im = bits_of(v.im);
+            const u64 fp = fingerprint(re, im);
+            const int h = static_cast<int>((fp >> 17) & (DictTable::kSlots - 1));
+            int slot = -1;
+            for (int p = 0; p < 4096; ++p) {
+                const int j = (h + p) & (DictTable::kSlots - 1);
+                const u64 f = D.fp[j];
+                if (f == fp) {
+                    slot = j;
+                    break;
+                }
+                if (f == 0) break;
+            }
+            u32 id = 0;
+            if (slot < 0 || D.re[slot] != re || D.im[slot] != im) atomicOr(D.flags, 2u);
+            else id = D.slot_id[slot];
+            if constexpr (WIDTH == 1) static_cast<u8*>(vals)[e0 + e] = static_cast<u8>(id);
+            else static_cast<u16*>(vals)[e0 + e] = static_cast<u16>(id);
+        }
+    }
+}
+
 }  // namespace
+
+bool csr_walk(const SectorView& R, const SectorView& C, bool same, const ProgramView& P, u32* row_len, u32* stage_col,
+              Zd* stage_val, hipStream_t st) {
+    if (P.n_groups > static_cast<u32>(kMaxRow)) return false;
+    if (R.dim == 0) return true;
+    const dim3 grid(blocks_for(static_cast<u64>(R.dim), 256)), block(256);
+    if (C.slc_L > 0)
+        hipLaunchKernelGGL(k_csr_walk<true>, grid, block, 0, st, R, C, same ? 1 : 0, P, row_len, stage_col, stage_val);
+    else
+        hipLaunchKernelGGL(k_csr_walk<false>, grid, block, 0, st, R, C, same ? 1 : 0, P, row_len, stage_col, stage_val);
+    return true;
+}
+
+void csr_stage_dict(i64 rows, const u32* row_len, const Zd* stage_val, const DictTable& D, hipStream_t st) {
+    if (rows <= 0) return;
+    hipLaunchKernelGGL(k_csr_stage_dict, dim3(blocks_for(static_cast<u64>(rows), 256)), dim3(256), 0, st, rows, row_len,
+                       stage_val, D);
+}
+
+void csr_compact(i64 rows, const u64* row_ptr, const u32* stage_col, const Zd* stage_val, u32* col, void* vals,
+                 int width, bool real, const DictTable& D, hipStream_t st) {
+    if (rows <= 0) return;
+    const dim3 grid(blocks_for(static_cast<u64>(rows), 256)), block(256);
+#define QSA_COMPACT(W, RE) \
+    hipLaunchKernelGGL((k_csr_compact<W, RE>), grid, block, 0, st, rows, row_ptr, stage_col, stage_val, col, vals, D)
+    if (width == 1) QSA_COMPACT(1, false);
+    else if (width == 2) QSA_COMPACT(2, false);
+    else if (real) QSA_COMPACT(0, true);
+    else QSA_COMPACT(0, false);
+#undef QSA_COMPACT
+}
 
 void walk_gather(const SectorView& R, const SectorView& C, bool same, const ProgramView& P, bool real,
                  const VecSet& v, hipStream_t st) {

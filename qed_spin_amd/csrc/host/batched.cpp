@@ -389,6 +389,83 @@ void BatchedLanczos::run(int M, const std::vector<double>& breakdown_tol, bool l
     t.nnz = nnz_;
 }
 
+void BatchedLanczos::run_kept(int M, const std::vector<double>& breakdown_tol, const std::vector<int>& full,
+                              cplx* basis_c, std::vector<double>& alpha, std::vector<double>& beta,
+                              std::vector<int>& steps, BatchedTimings& t, const std::vector<int>* live) {
+    if (M < 1) throw Error(kInvalid, "batched lanczos: at least one step");
+    const std::size_t nb = ops_.size(), pairs = nb * static_cast<std::size_t>(R_);
+    if (breakdown_tol.size() != nb || full.size() != nb)
+        throw Error(kInvalid, "batched lanczos: one breakdown tolerance and one reorthogonalisation flag per block");
+    if (live && live->size() != pairs) throw Error(kInvalid, "batched lanczos: one live flag per pair");
+    const hipStream_t st = engine_stream();
+    const auto t0 = clock_t_::now();
+    const std::size_t hist = static_cast<std::size_t>(M) * pairs;
+    std::vector<int> cap(nb), zeros(pairs, 0);
+    for (std::size_t b = 0; b < nb; ++b) cap[b] = static_cast<int>(std::min<std::uint64_t>(M, dim(static_cast<int>(b))));
+    put(cap_d_, cap, st);
+    put(tol_d_, breakdown_tol, st);
+    put(active_d_, live ? *live : std::vector<int>(pairs, 1), st);
+    put(steps_d_, zeros, st);
+    put(step_d_, std::vector<int>{0}, st);
+    put(fix_, zeros, st);
+    DeviceBuffer full_d;
+    put(full_d, full, st);
+    const bool any_full = std::any_of(full.begin(), full.end(), [](int f) { return f != 0; });
+    DeviceBuffer coef(any_full ? nb * static_cast<std::size_t>(M + 1) * R_ * sizeof(dev::Zd) : 8);
+    alpha_h_.reset(hist * 8);
+    beta_h_.reset(hist * 8);
+    dev::fill_zero(false, alpha_h_.get(), hist, st);
+    dev::fill_zero(false, beta_h_.get(), hist, st);
+    dev::fill_zero(false, cur_b_.get(), pairs, st);
+    const std::uint64_t layer = rows_ * static_cast<std::uint64_t>(R_);
+    auto* basis = reinterpret_cast<dev::Zd*>(basis_c);
+    QSA_HIP(hipMemcpyAsync(basis, V0_.get(), layer * sizeof(dev::Zd), hipMemcpyDeviceToDevice, st));
+    const dev::BatchView B = view();
+    double* partial = partial_.get<double>();
+    double* partial2 = partial2_.get<double>();
+    double* ah = alpha_h_.get<double>();
+    double* bh = beta_h_.get<double>();
+    dev::Zd* ov = ov_.get<dev::Zd>();
+    double* ca = cur_a_.get<double>();
+    double* cb = cur_b_.get<double>();
+    double* ib = inv_b_.get<double>();
+    const double* tol = tol_d_.get<double>();
+    const double thr = 1.49011611938476562e-08;   // sqrt(eps): LocalDGKS3's overlap threshold
+    const int Mrun = *std::max_element(cap.begin(), cap.end());
+    for (int j = 0; j < Mrun; ++j) {
+        const dev::Zd* Vp = basis + static_cast<std::uint64_t>(j > 0 ? j - 1 : 0) * layer;
+        const dev::Zd* Vc = basis + static_cast<std::uint64_t>(j) * layer;
+        dev::Zd* W = basis + static_cast<std::uint64_t>(j + 1) * layer;
+        dev::batch_spmv(B, Vc, W, st);
+        dev::batch_vec(B, 0, Vp, Vc, W, cb, nullptr, 0.0, partial, st);
+        dev::batch_finish(B, 0, partial, ah, bh, ov, ca, cb, ib, tol, st);
+        dev::batch_vec(B, 5, nullptr, Vc, W, ca, nullptr, 0.0, partial, st, partial2);
+        dev::batch_finish(B, 1, partial, ah, bh, ov, ca, cb, ib, tol, st, thr);
+        dev::batch_vec(B, 2, nullptr, Vc, W, nullptr, ov, thr, partial, st, partial2);
+        if (any_full) {
+            for (int pass = 0; pass < 2; ++pass)
+                dev::batch_gram_schmidt(B, basis, layer, j + 1, M + 1, W, full_d.get<int>(), coef.get<dev::Zd>(), st);
+            dev::batch_norm_partial(B, W, partial2, st);
+        }
+        dev::batch_finish(B, 2, partial2, ah, bh, ov, ca, cb, ib, tol, st);
+        dev::batch_vec(B, 3, nullptr, nullptr, W, ib, nullptr, 0.0, partial, st);
+        dev::batch_advance(B, st);
+    }
+    QSA_HIP(hipGetLastError());
+    QSA_HIP(hipStreamSynchronize(st));
+    t.steps_s += secs(t0, clock_t_::now());
+    t.steps += Mrun;
+    alpha.resize(hist);
+    beta.resize(hist);
+    steps.resize(pairs);
+    copy_to_host(alpha.data(), ah, hist * 8);
+    copy_to_host(beta.data(), bh, hist * 8);
+    copy_to_host(steps.data(), steps_d_.get(), pairs * sizeof(int));
+    for (int s : steps) t.applies += static_cast<std::uint64_t>(s);
+    t.rows = rows_;
+    t.nnz = nnz_;
+}
+
 std::vector<Curves> batched_ftlm(const std::vector<std::shared_ptr<const SectorOperator>>& blocks,
                                  const BatchedFtlmOptions& o, BatchedTimings& t) {
     const auto t_all = clock_t_::now();

@@ -54,8 +54,25 @@ public:
     void run(int M, const std::vector<double>& breakdown_tol, bool local, bool use_graph, std::vector<double>& alpha,
              std::vector<double>& beta, std::vector<int>& steps, BatchedTimings& t,
              const std::vector<int>* live = nullptr);
+    /// M steps keeping every normalised vector: layer j of basis (device, (M + 1) layers of
+    /// rows() x R entries, the batch layout) gets V_j, layer 0 the start vectors already placed and
+    /// normalised (sample_start, or clear_start / place / normalize). Every pair runs LocalDGKS3
+    /// (K = 1); the blocks flagged in full also take two classical Gram-Schmidt passes against all
+    /// their kept layers before beta (the reference's FullCGS2 runs). Eager launches: the number of
+    /// layers a pass reads grows each step. Outputs as run().
+    void run_kept(int M, const std::vector<double>& breakdown_tol, const std::vector<int>& full, cplx* basis,
+                  std::vector<double>& alpha, std::vector<double>& beta, std::vector<int>& steps, BatchedTimings& t,
+                  const std::vector<int>* live = nullptr);
     /// Y = H X on batch vectors (device pointers, rows() x R complex each).
     void apply(const cplx* X, cplx* Y) const;
+    /// The start vectors (rows() x R, device): written by the caller, then normalize().
+    cplx* start_buffer() { return V0_.get<cplx>(); }
+    /// The packed layout as the kernels read it (tiles, R); every pair live.
+    dev::BatchView packed_view() const {
+        dev::BatchView B = view();
+        B.active = nullptr;
+        return B;
+    }
 
 private:
     dev::BatchView view() const;

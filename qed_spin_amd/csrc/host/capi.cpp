@@ -134,11 +134,12 @@ int qedh_thread_stream(int own) {
             return;
         }
         qsa::require_device();
-        // taken from the engine's pool on the thread's first request and handed back (drained,
-        // not destroyed) when the thread ends
+        // taken from the engine's pool on the thread's first request and handed back when the
+        // thread ends, with no HIP call at thread exit (a profiler's per-thread state may already
+        // be gone then)
         struct Owned {
             hipStream_t s = nullptr;
-            ~Owned() { qsa::pooled_stream_release(s); }
+            ~Owned() { qsa::pooled_stream_return(s); }
         };
         thread_local Owned owned;
         if (!owned.s) owned.s = qsa::pooled_stream_acquire();

@@ -9,6 +9,7 @@
 #include <rocblas/rocblas.h>
 
 #include <complex>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -46,14 +47,46 @@ qsa::dev::Zd scalar_of(const double* a, int cplx) {
     return qsa::dev::Zd{a[0], cplx ? a[1] : 0.0};
 }
 
+/// Per-thread device objects (reduction scratch, a rocBLAS handle) come from process-wide pools and
+/// go back to them when the thread ends; nothing is freed or destroyed at thread exit, where a
+/// profiler's or the runtime's own per-thread state may already be gone.
+template <class T> struct ThreadPool {
+    std::mutex mu;
+    std::vector<T*> idle;
+    T* take() {
+        std::lock_guard<std::mutex> lock(mu);
+        if (idle.empty()) return nullptr;
+        T* t = idle.back();
+        idle.pop_back();
+        return t;
+    }
+    void give(T* t) noexcept {
+        if (!t) return;
+        std::lock_guard<std::mutex> lock(mu);
+        idle.push_back(t);
+    }
+};
+template <class T> ThreadPool<T>& pool_of() {
+    static ThreadPool<T>* p = new ThreadPool<T>;   // never destroyed
+    return *p;
+}
+template <class T> struct Lease {
+    T* t = nullptr;
+    ~Lease() { pool_of<T>().give(t); }
+};
+
 /// Reduction scratch and result slots of the calling thread (its engine stream only).
 struct ThreadScratch {
     qsa::DeviceBuffer partial, result;
     ThreadScratch() : partial(qsa::dev::blas_scratch_bytes(256)), result(64) {}
 };
 ThreadScratch& scratch() {
-    thread_local ThreadScratch s;
-    return s;
+    thread_local Lease<ThreadScratch> lease;
+    if (!lease.t) {
+        lease.t = pool_of<ThreadScratch>().take();
+        if (!lease.t) lease.t = new ThreadScratch;
+    }
+    return *lease.t;
 }
 
 void sync() { QSA_HIP(hipStreamSynchronize(qsa::engine_stream())); }
@@ -80,18 +113,26 @@ rocblas_operation op_of(char c, int cplx) {
     }
 }
 
-/// The calling thread's rocBLAS handle (created on first use, kept for the thread's life).
+/// The calling thread's rocBLAS handle (from the pool on first use, back to it when the thread ends).
+struct BlasHandle {
+    rocblas_handle h = nullptr;
+};
 rocblas_handle blas_handle() {
-    struct H {
-        rocblas_handle h = nullptr;
-        ~H() {
-            if (h) (void)rocblas_destroy_handle(h);
+    thread_local Lease<BlasHandle> lease;
+    if (!lease.t) {
+        lease.t = pool_of<BlasHandle>().take();
+        if (!lease.t) {
+            auto* fresh = new BlasHandle;
+            const rocblas_status s = rocblas_create_handle(&fresh->h);
+            if (s != rocblas_status_success) {
+                delete fresh;
+                rb(s, "rocblas_create_handle");
+            }
+            lease.t = fresh;
         }
-    };
-    thread_local H handle;
-    if (!handle.h) rb(rocblas_create_handle(&handle.h), "rocblas_create_handle");
-    rb(rocblas_set_stream(handle.h, qsa::engine_stream()), "rocblas_set_stream");
-    return handle.h;
+    }
+    rb(rocblas_set_stream(lease.t->h, qsa::engine_stream()), "rocblas_set_stream");
+    return lease.t->h;
 }
 
 }  // namespace

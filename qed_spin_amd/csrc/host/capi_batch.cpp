@@ -11,7 +11,7 @@
 
 #include "batched.h"
 #include "dense_device.h"
-#include "ftlm_dynamics.h"
+#include "dyn_walk.h"
 #include "serve.h"
 #include "tridiag.h"
 
@@ -386,44 +386,76 @@ extern "C" int qedh_request_footprint(const qedh_request_t* req, int batched, ui
     });
 }
 
+namespace {
+
+/// The walk's results into the C-ABI's flat arrays.
+void write_walk(const qsa::DynWalkResult& r, std::size_t nT, std::size_t nW, double* Z, double* S, double* e_min,
+                int* samples_done) {
+    for (std::size_t b = 0; b < r.Z.size(); ++b)
+        for (std::size_t i = 0; i < nT; ++i) Z[b * nT + i] = r.Z[b][i];
+    if (S)
+        for (std::size_t c = 0; c < r.S.size(); ++c)
+            for (std::size_t q = 0; q < nT * nW; ++q) {
+                S[(c * nT * nW + q) * 2] = r.S[c][q].real();
+                S[(c * nT * nW + q) * 2 + 1] = r.S[c][q].imag();
+            }
+    for (std::size_t b = 0; b < r.e_min.size(); ++b) {
+        if (e_min) e_min[b] = r.e_min[b];
+        if (samples_done) samples_done[b] = r.samples_done[b];
+    }
+}
+
+}  // namespace
+
+extern "C" int qedh_ftlm_dynamics_walk(int n_sources, const qedh_matvec* source_h, const uint64_t* source_seeds,
+                                       const double* breakdown_tol, int n_channels, const int* channel_source,
+                                       const qedh_matvec* channel_h, const qedh_matvec* channel_probe, int samples,
+                                       int krylov, int n_t, const double* temperatures, int n_omega,
+                                       const double* omega, double eta, uint64_t budget_bytes, double* Z, double* S,
+                                       double* e_min, int* samples_done, double* stats) {
+    return guarded_batch([&] {
+        if (n_sources < 1 || !source_h || !source_seeds || !breakdown_tol || n_channels < 0 ||
+            (n_channels > 0 && (!channel_source || !channel_h || !channel_probe || !S)) || samples < 1 || krylov < 1 ||
+            n_t < 1 || !temperatures || n_omega < 1 || !omega || !Z)
+            throw qsa::Error(qsa::kInvalid, "qedh_ftlm_dynamics_walk: bad arguments");
+        const auto src = gather_blocks(source_h, n_sources);
+        std::vector<qsa::DynChannel> ch(static_cast<std::size_t>(n_channels));
+        for (int c = 0; c < n_channels; ++c) {
+            if (!channel_h[c] || !channel_probe[c]) throw qsa::Error(qsa::kInvalid, "qedh_ftlm_dynamics_walk: null channel");
+            ch[c].source = channel_source[c];
+            ch[c].H = channel_h[c]->op;
+            ch[c].A = channel_probe[c]->op;
+        }
+        qsa::DynWalkOptions o;
+        o.samples = samples;
+        o.krylov = krylov;
+        o.source_seeds.assign(source_seeds, source_seeds + n_sources);
+        o.breakdown_tol.assign(breakdown_tol, breakdown_tol + n_sources);
+        o.temperatures.assign(temperatures, temperatures + n_t);
+        o.omega.assign(omega, omega + n_omega);
+        o.eta = eta;
+        o.budget_bytes = budget_bytes;
+        const qsa::DynWalkResult r = qsa::ftlm_dynamics_walk(src, ch, o);
+        write_walk(r, static_cast<std::size_t>(n_t), static_cast<std::size_t>(n_omega), Z, S, e_min, samples_done);
+        if (stats) {
+            stats[0] = r.lanczos_s;
+            stats[1] = r.overlap_s;
+            stats[2] = r.host_s;
+            stats[3] = r.total_s;
+            stats[4] = static_cast<double>(r.waves);
+        }
+    });
+}
+
 extern "C" int qedh_ftlm_dynamics(qedh_matvec source_h, int n_targets, const qedh_matvec* target_h,
                                   const qedh_matvec* probes, int samples, int krylov, uint64_t seed,
                                   double breakdown_tol, int n_t, const double* temperatures, int n_omega,
                                   const double* omega, double eta, double* Z, double* S, double* e_min,
                                   int* samples_done) {
-    return guarded_batch([&] {
-        if (!source_h || n_targets < 0 || (n_targets > 0 && (!target_h || !probes)) || samples < 1 || krylov < 1 ||
-            n_t < 1 || !temperatures || n_omega < 1 || !omega || !Z || (n_targets > 0 && !S))
-            throw qsa::Error(qsa::kInvalid, "qedh_ftlm_dynamics: bad arguments");
-        const auto& Hs = *source_h->op;
-        if (!Hs.same()) throw qsa::Error(qsa::kInvalid, "qedh_ftlm_dynamics: the source operator must be square");
-        std::vector<qsa::FtlmDynTarget> tg;
-        for (int t = 0; t < n_targets; ++t) {
-            if (!target_h[t] || !probes[t]) throw qsa::Error(qsa::kInvalid, "qedh_ftlm_dynamics: null target");
-            const auto& Ht = *target_h[t]->op;
-            const auto& A = *probes[t]->op;
-            if (A.col_dim() != Hs.dim() || A.dim() != Ht.dim() || !Ht.same())
-                throw qsa::Error(qsa::kInvalid, "qedh_ftlm_dynamics: a probe does not map the source to its target");
-            tg.push_back({&Ht, &A});
-        }
-        qsa::FtlmDynOptions o;
-        o.krylov = static_cast<std::size_t>(krylov);
-        o.samples = static_cast<std::size_t>(samples);
-        o.breakdown_tol = breakdown_tol;
-        o.eta = eta;
-        o.seed = seed;
-        o.temperatures.assign(temperatures, temperatures + n_t);
-        o.omega.assign(omega, omega + n_omega);
-        const qsa::FtlmDynResult r = qsa::ftlm_dynamics_source(Hs, tg, o);
-        for (int i = 0; i < n_t; ++i) Z[i] = r.Z[i];
-        for (int t = 0; t < n_targets; ++t)
-            for (std::size_t q = 0; q < r.S[t].size(); ++q) {
-                S[(static_cast<std::size_t>(t) * r.S[t].size() + q) * 2] = r.S[t][q].real();
-                S[(static_cast<std::size_t>(t) * r.S[t].size() + q) * 2 + 1] = r.S[t][q].imag();
-            }
-        if (e_min) *e_min = r.E_min;
-        if (samples_done) *samples_done = static_cast<int>(r.samples_done);
-    });
+    std::vector<int> src(static_cast<std::size_t>(std::max(n_targets, 0)), 0);
+    const double tol = breakdown_tol > 0.0 ? breakdown_tol : 1e-300;
+    return qedh_ftlm_dynamics_walk(1, &source_h, &seed, &tol, n_targets, src.data(), target_h, probes, samples, krylov,
+                                   n_t, temperatures, n_omega, omega, eta, 0, Z, S, e_min, samples_done, nullptr);
 }
 
 extern "C" int qedh_gaussian_vectors(uint64_t base, uint64_t first_sample, int count, int64_t n, double* out) {

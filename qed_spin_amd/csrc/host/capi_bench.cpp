@@ -109,7 +109,8 @@ extern "C" int qedh_matvec_set_tuning(qedh_matvec m, int lanes, int flags) {
         m->op->set_xcd_rows((flags & QEDH_TUNE_XCD_ROWS) != 0);
         m->op->set_unroll((flags & QEDH_TUNE_UNROLL16) ? 16 : (flags & QEDH_TUNE_UNROLL8) ? 8 : (flags & QEDH_TUNE_UNROLL) ? 4
                           : (flags & QEDH_TUNE_UNROLL2) ? 2 : 1);
-        if (flags & QEDH_TUNE_SELL) m->op->build_sell();
+        if ((flags & QEDH_TUNE_SELL) || m->op->sell()) m->op->build_sell((flags & QEDH_TUNE_SELL_DIAG) != 0);
+        m->op->set_sell_sums((flags & QEDH_TUNE_SELL_SUMS1) ? 1 : 4);
         m->op->set_use_sell((flags & QEDH_TUNE_CSR) == 0);
         m->op->set_workgroup((flags & QEDH_TUNE_WG1024) ? 1024 : (flags & QEDH_TUNE_WG256) ? 256 : 512);
         m->op->set_global_dict((flags & QEDH_TUNE_GLOBAL_DICT) != 0);

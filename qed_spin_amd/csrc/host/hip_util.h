@@ -74,6 +74,10 @@ void device_free(void* p) noexcept;
 /// process runs); release drains it and hands it back for the next thread or slot.
 hipStream_t pooled_stream_acquire();
 void pooled_stream_release(hipStream_t s) noexcept;
+/// Hands a stream back without any HIP call (for thread-exit destructors, where a profiler's or the
+/// runtime's own per-thread state may already be gone): work still queued on it stays ordered
+/// before whatever the next owner enqueues.
+void pooled_stream_return(hipStream_t s) noexcept;
 /// The calling thread's engine stream waits for all work enqueued so far on every pooled stream
 /// and on the null stream. Objects that several threads may have used call it before freeing
 /// their device memory (hipFreeAsync orders a free on the freeing stream only).

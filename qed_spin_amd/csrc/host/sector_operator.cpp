@@ -45,8 +45,32 @@ void SectorOperator::build_csr(std::uint64_t budget) {
     D.flags = dcnt.get<unsigned>() + 1;
     QSA_HIP(hipMemsetAsync(dfp.get(), 0, dfp.bytes(), st));
     QSA_HIP(hipMemsetAsync(dcnt.get(), 0, 8, st));
-    if (!dev::csr_count(R, C, same(), P, row_len.get<std::uint32_t>(), D, true, st)) return;   // too many groups
+    // One walk into a stage of (acting groups) slots per row when it fits -- at most 16 GiB and a
+    // quarter of the free device memory -- else two walks (count, then fill).
+    const std::uint64_t stage_bytes = static_cast<std::uint64_t>(rows) * P.n_groups * (4 + sizeof(dev::Zd));
+    std::size_t free_b = 0, total_b = 0;
+    QSA_HIP(hipMemGetInfo(&free_b, &total_b));
+    const bool staged = stage_bytes <= (std::uint64_t{16} << 30) && stage_bytes <= free_b / 4;
+    DeviceBuffer stage_col, stage_val;
+    if (staged) {
+        stage_col.reset(std::max<std::uint64_t>(static_cast<std::uint64_t>(rows) * P.n_groups, 1) * 4);
+        stage_val.reset(std::max<std::uint64_t>(static_cast<std::uint64_t>(rows) * P.n_groups, 1) * sizeof(dev::Zd));
+        if (!dev::csr_walk(R, C, same(), P, row_len.get<std::uint32_t>(), stage_col.get<std::uint32_t>(),
+                           stage_val.get<dev::Zd>(), st))
+            return;   // too many groups
+        dev::csr_stage_dict(rows, row_len.get<std::uint32_t>(), stage_val.get<dev::Zd>(), D, st);
+    } else if (!dev::csr_count(R, C, same(), P, row_len.get<std::uint32_t>(), D, true, st)) {
+        return;   // too many groups
+    }
     QSA_HIP(hipGetLastError());
+    auto fill = [&](int w) {
+        if (staged)
+            dev::csr_compact(rows, row_ptr_.get<std::uint64_t>(), stage_col.get<std::uint32_t>(),
+                             stage_val.get<dev::Zd>(), col_.get<std::uint32_t>(), vals_.get(), w, real_, D, st);
+        else
+            dev::csr_fill(R, C, same(), P, row_ptr_.get<std::uint64_t>(), col_.get<std::uint32_t>(), vals_.get(), w,
+                          real_, D, st);
+    };
     row_ptr_.reset(static_cast<std::size_t>(rows + 1) * 8);
     const std::size_t tb = dev::row_scan_temp_bytes(rows);
     DeviceBuffer temp(tb ? tb : 8);
@@ -98,8 +122,7 @@ void SectorOperator::build_csr(std::uint64_t budget) {
         dict_im_ = upload(di.data(), di.size());
         csr_.dict = static_cast<std::int64_t>(dr.size());
     }
-    dev::csr_fill(R, C, same(), P, row_ptr_.get<std::uint64_t>(), col_.get<std::uint32_t>(), vals_.get(), width,
-                  real_, D, st);
+    fill(width);
     QSA_HIP(hipGetLastError());
     QSA_HIP(hipMemcpyAsync(cnt, dcnt.get(), 8, hipMemcpyDeviceToHost, st));
     QSA_HIP(hipStreamSynchronize(st));
@@ -109,8 +132,7 @@ void SectorOperator::build_csr(std::uint64_t budget) {
         dict_re_.release();
         dict_im_.release();
         csr_.dict = 0;
-        dev::csr_fill(R, C, same(), P, row_ptr_.get<std::uint64_t>(), col_.get<std::uint32_t>(), vals_.get(), 0,
-                      real_, D, st);
+        fill(0);
         QSA_HIP(hipStreamSynchronize(st));
     }
     // u8 ids with columns below 2^24: the id rides in the column word's top byte (one 4-byte load
@@ -141,9 +163,16 @@ void SectorOperator::build_csr(std::uint64_t budget) {
     csr_.build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
-bool SectorOperator::build_sell() {
+bool SectorOperator::build_sell(bool diag_first) {
     if (lane_ != kLaneCsr || dim() == 0) return false;
-    if (sell_) return true;
+    if (sell_ && diag_first == sell_diag_first_) return true;
+    if (sell_) {   // rebuilt in the other entry order
+        csr_.bytes -= sell_ptr_.bytes() + sell_len_.bytes() + sell_col_.bytes() + sell_vals_.bytes();
+        sell_ = false;
+        sell_col_.release();
+        sell_vals_.release();
+    }
+    sell_diag_first_ = diag_first;
     const hipStream_t st = engine_stream();
     const std::int64_t rows = dim(), slices = (rows + 63) / 64;
     DeviceBuffer words(static_cast<std::size_t>(slices + 1) * 8), flag(4);
@@ -174,7 +203,8 @@ bool SectorOperator::build_sell() {
     QSA_HIP(hipMemsetAsync(sell_col_.get(), 0, sell_col_.bytes(), st));
     if (vb) QSA_HIP(hipMemsetAsync(sell_vals_.get(), 0, sell_vals_.bytes(), st));
     dev::sell_fill(row_ptr_.get<std::uint64_t>(), col_.get<std::uint32_t>(), vals_.get(), vb, rows,
-                   sell_ptr_.get<std::uint64_t>(), sell_col_.get<std::uint32_t>(), sell_vals_.get(), st);
+                   sell_ptr_.get<std::uint64_t>(), sell_col_.get<std::uint32_t>(), sell_vals_.get(),
+                   sell_diag_first_ && same(), packed_, st);
     QSA_HIP(hipGetLastError());
     QSA_HIP(hipStreamSynchronize(st));
     sell_ = true;
@@ -248,6 +278,7 @@ dev::CsrView SectorOperator::csr_view(bool block, bool sell) const {
         A.vals = sell_vals_.get();
         A.sell_ptr = sell_ptr_.get<std::uint64_t>();
         A.sell_len = sell_len_.get<std::uint16_t>();
+        A.sell_acc = sell_acc_;
     }
     A.width = packed_ ? 3 : (csr_.width == 1 || csr_.width == 2) ? csr_.width : 0;
     A.dict_re = dict_re_.get<double>();

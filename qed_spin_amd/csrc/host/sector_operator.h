@@ -93,8 +93,12 @@ public:
     bool sell() const { return sell_ && use_sell_; }
     /// Apply through the SELL-64 copy when one exists (default), or through the CSR.
     void set_use_sell(bool on) { use_sell_ = on; }
-    /// Build the SELL-64 copy now (tests); false when a row is too long for it.
-    bool build_sell();
+    /// Build the SELL-64 copy now (tests), or again in the other entry order; false when a row is
+    /// too long for it. diag_first: a row's diagonal entry takes its first slot.
+    bool build_sell(bool diag_first = false);
+    /// SELL-64: partial sums per row, 4 (the default, combined by a fixed tree) or 1; sets the
+    /// summation order.
+    void set_sell_sums(int acc) { sell_acc_ = acc == 4 ? 4 : 1; }
 
 private:
     void build_csr(std::uint64_t budget);
@@ -116,7 +120,8 @@ private:
     int wg_ = 512;
     CsrInfo csr_;
     DeviceBuffer row_ptr_, col_, vals_, dict_re_, dict_im_;
-    bool sell_ = false, use_sell_ = true;
+    bool sell_ = false, use_sell_ = true, sell_diag_first_ = false;
+    int sell_acc_ = 4;   // four partial sums per row: as fast single, and the block kernel's 4 lanes per row
     DeviceBuffer sell_ptr_, sell_len_, sell_col_, sell_vals_;
 };
 

@@ -78,6 +78,13 @@ hipStream_t pooled_stream_acquire() {
     return s;
 }
 
+void pooled_stream_return(hipStream_t s) noexcept {
+    if (!s) return;
+    Pool& P = pool();
+    std::lock_guard<std::mutex> lock(P.mu);
+    P.idle.push_back(s);
+}
+
 void pooled_stream_release(hipStream_t s) noexcept {
     if (!s) return;
     (void)hipStreamSynchronize(s);

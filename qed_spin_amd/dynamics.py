@@ -26,7 +26,7 @@ import numpy as np
 from . import _lib
 from .errors import DeviceUnsupported, InvalidRequest
 from .operator import Operator, Program
-from .sector import parallel_build, DEFAULT_CSR_BUDGET, LANE_GATHER, Sector, SectorMatvec
+from .sector import parallel_build, projected_program, DEFAULT_CSR_BUDGET, LANE_CSR, LANE_GATHER, Sector, SectorMatvec
 from .symmetry import Symmetry
 from .thermal import KBREAKDOWN_REL, _handles, release_later, resolve_device, time_reversal_partners, walk_blocks
 
@@ -149,74 +149,79 @@ def block_seed(base: int, key: int) -> int:
 
 
 def _reach(blocks, b, O, delta):
-    """(target index, probe operator on the device) for every block the probe reaches from block b."""
+    """(target index, probe operator on the device CSR) for every block the probe reaches from block b."""
     blk_b, sec_b, _ = blocks[b]
     out = []
     for t, (blk_t, sec_t, _) in enumerate(blocks):
         if sec_b.n_up >= 0 and sec_t.n_up != sec_b.n_up + delta:
             continue
-        tgt = sec_t if t != b else Sector(blk_t.group, blk_t.n_up)
-        X = SectorMatvec(O, tgt, sec_b)
-        if X.program.counts()[2] == 0:
+        prog = projected_program(O, sec_t, sec_b)   # no part between the two: nothing to build
+        if prog.counts()[2] == 0:
             continue
-        out.append((t, X))
+        tgt = sec_t if t != b else Sector(blk_t.group, blk_t.n_up)
+        out.append((t, SectorMatvec(O, tgt, sec_b, lane=LANE_CSR, tune=False, program=prog)))
     return out
 
 
 def _dynamics_finite_T(H, probes, omega, T, eta, blocks, krylov, samples, seed, sources):
-    """FTLM over every source sector (dynamics.cpp:693-1257): per source its samples' Z and S from
-    qedh_ftlm_dynamics, then S_O(omega, T) = sum_src w_src S_src / sum_src w_src Z_src with
+    """FTLM over every source sector (dynamics.cpp:693-1257) in one batched run
+    (qedh_ftlm_dynamics_walk): per source its samples' Z and per (source, probe, target) channel its
+    S, then S_O(omega, T) = sum_src w_src S_src / sum_src w_src Z_src with
     w_src = e^{-beta (emin_src - ref)}."""
     T = np.ascontiguousarray(np.atleast_1d(T), np.float64)
     nT, nW, P = T.size, omega.size, len(probes)
     deltas = [_delta_up(O) for O in probes]
     s0 = int(seed) if seed else int.from_bytes(__import__("os").urandom(8), "little")
-    n_irr = {}
-    for blk, _, _ in blocks:
-        n_irr[blk.n_up] = max(n_irr.get(blk.n_up, 0), (blk.momentum or 0) + 1)
     tol = KBREAKDOWN_REL * (H.norm_bound() or 1.0)
     for O in probes:
         O._handle()
-
-    def one_source(b):
+    src = list(range(len(blocks))) if sources is None else list(sources)
+    seeds, chan_src, chan_h, chan_a, chan_probe, keep = [], [], [], [], [], []
+    for i, b in enumerate(src):
         blk, sec, m = blocks[b]
         irr = blk.group.G if blk.momentum is not None else 1
         key = ((sec.n_up + 1) * 3 + 0) * irr + (blk.momentum or 0)
-        reach = []
+        seeds.append(block_seed(s0, key))
         for p, O in enumerate(probes):
             for t, X in _reach(blocks, b, O, deltas[p]):
-                reach.append((p, t, X))
-        nt = len(reach)
-        th = (C.c_void_p * max(nt, 1))(*[blocks[t][2].h.value for _, t, _ in reach] or [None])
-        ph = (C.c_void_p * max(nt, 1))(*[X.h.value for _, _, X in reach] or [None])
-        Z = np.zeros(nT)
-        S = np.zeros((max(nt, 1), nT, nW, 2))
-        emin, done = C.c_double(), C.c_int()
-        _lib.call("qedh_ftlm_dynamics", m.h, nt, th, ph, int(samples), int(krylov), C.c_uint64(block_seed(s0, key)),
-                  C.c_double(tol), nT, _lib.ptr(T, C.c_double), nW, _lib.ptr(omega, C.c_double), C.c_double(eta),
-                  _lib.ptr(Z, C.c_double), _lib.ptr(S, C.c_double), C.byref(emin), C.byref(done))
-        Sp = np.zeros((P, nT, nW), np.complex128)
-        for q, (p, _, _) in enumerate(reach):
-            Sp[p] += S[q, :, :, 0] + 1j * S[q, :, :, 1]
-        return emin.value, Z, Sp
-
-    # sources are independent: several at once on host threads, each with a stream of its own (a
-    # small source's Lanczos runs are launch-bound); the sum below runs in source order
-    per_src = parallel_build(one_source, list(range(len(blocks)) if sources is None else sources))
-    ref = min(e for e, _, _ in per_src)
+                chan_src.append(i)
+                chan_h.append(blocks[t][2].h.value)
+                chan_a.append(X.h.value)
+                chan_probe.append(p)
+                keep.append(X)
+    nS, nC = len(src), len(chan_src)
+    sh = (C.c_void_p * nS)(*[blocks[b][2].h.value for b in src])
+    seed_a = np.array(seeds, np.uint64)
+    tol_a = np.full(nS, tol)
+    cs = np.array(chan_src or [0], np.int32)
+    ch = (C.c_void_p * max(nC, 1))(*(chan_h or [None]))
+    ca = (C.c_void_p * max(nC, 1))(*(chan_a or [None]))
+    Z = np.zeros((nS, nT))
+    S = np.zeros((max(nC, 1), nT, nW, 2))
+    emin = np.zeros(nS)
+    done = np.zeros(nS, np.int32)
+    stats = np.zeros(5)
+    _lib.call("qedh_ftlm_dynamics_walk", nS, sh, _lib.ptr(seed_a, C.c_uint64), _lib.ptr(tol_a, C.c_double), nC,
+              cs.ctypes.data_as(C.POINTER(C.c_int)), ch, ca, int(samples), int(krylov), nT, _lib.ptr(T, C.c_double), nW,
+              _lib.ptr(omega, C.c_double), C.c_double(eta), C.c_uint64(0), _lib.ptr(Z, C.c_double),
+              _lib.ptr(S, C.c_double), _lib.ptr(emin, C.c_double), done.ctypes.data_as(C.POINTER(C.c_int)),
+              _lib.ptr(stats, C.c_double))
+    keep = None
+    ref = float(np.min(emin))
     out = []
     for p in range(P):
         Sres = np.zeros((nT, nW), np.complex128)
         for it in range(nT):
             beta = 1.0 / T[it]
-            Zp = 0.0
-            for e, Z, Sp in per_src:
-                w = math.exp(-beta * (e - ref))
-                Zp += w * Z[it]
-                Sres[it] += w * Sp[p, it]
+            w = np.exp(-beta * (emin - ref))
+            Zp = float(np.sum(w * Z[:, it]))
+            for c in range(nC):
+                if chan_probe[c] == p:
+                    Sres[it] += w[chan_src[c]] * (S[c, it, :, 0] + 1j * S[c, it, :, 1])
             Sres[it] /= Zp
         out.append(Sres)
-    return out, ref
+    engine = dict(zip(["lanczos_s", "overlap_s", "host_s", "total_s", "waves"], stats.tolist()))
+    return out, ref, engine
 
 
 def dynamics(H: Operator, probes, omega, *, eta: float = 0.1, T=None, sym: Optional[Symmetry] = None,
@@ -254,11 +259,11 @@ def dynamics(H: Operator, probes, omega, *, eta: float = 0.1, T=None, sym: Optio
     if T is not None and len(np.atleast_1d(T)) > 0:
         if np.any(~(np.atleast_1d(T) > 0)):
             raise InvalidRequest("dynamics: temperatures must be positive")
-        S, ref = _dynamics_finite_T(H, probes, omega, T, eta, blocks, krylov, samples, seed, sources)
+        S, ref, engine = _dynamics_finite_T(H, probes, omega, T, eta, blocks, krylov, samples, seed, sources)
         if own:
             release_later(blocks)
         return DynamicsResult(omega=omega, S=S, e0=ref, ground_manifold=0, target_sectors=0,
-                              timings={"walk_s": t_walk, "total_s": time.perf_counter() - t0})
+                              timings={"walk_s": t_walk, "total_s": time.perf_counter() - t0}, engine=engine)
     sH = H.norm_bound() or 1.0
     window = degeneracy_tol * sH
     t1 = time.perf_counter()
@@ -281,10 +286,11 @@ def dynamics(H: Operator, probes, omega, *, eta: float = 0.1, T=None, sym: Optio
                 # The probe is not invariant under the group, so even within the source's own block it
                 # needs the lambda-projected program of two sectors: the target then gets a handle of
                 # its own (the same representatives, enumerated again).
-                tgt = sec_t if t != b else Sector(blk_t.group, blk_t.n_up)
-                X = SectorMatvec(O, tgt, sec_b, lane=LANE_GATHER)
-                if X.program.counts()[2] == 0:
+                prog = projected_program(O, sec_t, sec_b)
+                if prog.counts()[2] == 0:
                     continue
+                tgt = sec_t if t != b else Sector(blk_t.group, blk_t.n_up)
+                X = SectorMatvec(O, tgt, sec_b, lane=LANE_GATHER, program=prog)
                 phi = _lib_alloc(sec_t.dim * 16)
                 X.apply(g, phi, complex_vectors=True)
                 images.append((t, p, phi))

@@ -155,22 +155,33 @@ class Sector:
             self.h = None
 
 
+def projected_program(op: Operator, rows: Sector, cols: Sector) -> Program:
+    """The program of O from sector cols to sector rows (lambda-projected, rows as ket); it depends
+    on the sectors' groups and characters only, so any handles of them serve. counts()[2] == 0:
+    O has no part between the two."""
+    return Program(op, adjoint=True, src=rows, tgt=cols)
+
+
 class SectorMatvec:
     """O restricted to a sector (or from sector `cols` to sector `rows`, one group) on device vectors.
 
     The program is the walk of O^dagger (for two sectors, lambda-projected with rows as ket)."""
 
     def __init__(self, op: Operator, rows: Sector, cols: Optional[Sector] = None, lane: int = LANE_AUTO,
-                 csr_budget: int = DEFAULT_CSR_BUDGET, tune: bool = True):
+                 csr_budget: int = DEFAULT_CSR_BUDGET, tune: bool = True, program: Optional[Program] = None):
         """tune=False: a large CSR operator keeps the default SpMV shape instead of timing the
-        variants (for operators applied only through a batched engine)."""
+        variants (for operators applied only through a batched engine). program: the compiled
+        program when the caller has it already (Program(op, adjoint=True) for one sector,
+        projected_program(op, rows, cols) for two)."""
         self.op = op
         self.rows = rows
         self.cols = cols if cols is not None else rows
-        if self.cols is rows:
+        if program is not None:
+            self.program = program
+        elif self.cols is rows:
             self.program = Program(op, adjoint=True)
         else:
-            self.program = Program(op, adjoint=True, src=rows, tgt=self.cols)
+            self.program = projected_program(op, rows, self.cols)
         h = C.c_void_p()
         _lib.call("qedh_matvec_create", rows.h, None if self.cols is rows else self.cols.h, self.program.h,
                   C.c_double(op.norm_bound()), int(lane) | (0 if tune else LANE_NO_TUNE), C.c_uint64(int(csr_budget)),
