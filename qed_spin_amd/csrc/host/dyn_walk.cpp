@@ -161,28 +161,53 @@ DynWalkResult ftlm_dynamics_walk(const std::vector<std::shared_ptr<const SectorO
 
     const int Rmax = std::min(o.samples, dev::kBatchMaxR);
     const hipStream_t st = engine_stream();
-    // kept vectors of one sample of source b: its basis, and per channel the target basis, the probe
-    // images of the source basis, and both transposed
-    auto per_sample = [&](std::size_t b) {
-        std::uint64_t rows_c = 0;
-        for (int c : chans_of[b]) rows_c += static_cast<std::uint64_t>(channels[c].H->dim());
-        return 16ull * static_cast<std::uint64_t>(M + 1) * (static_cast<std::uint64_t>(sources[b]->dim()) + 4 * rows_c);
+    // The kept vectors live in three device regions that every wave reuses (allocated once: kept
+    // vectors of tens of GB re-allocated per wave in slightly different sizes cost the pool more time
+    // than the wave's work): (1) the source bases, later the probe images transposed; (2) the target
+    // bases; (3) the probe images, later the target bases transposed. Per sample, with r_s the wave's
+    // source rows and r_c its channel rows: 16 (M + 1) (max(r_s, r_c) + 2 r_c) bytes.
+    auto rows_c_of = [&](std::size_t b) {
+        std::uint64_t r = 0;
+        for (int c : chans_of[b]) r += static_cast<std::uint64_t>(channels[c].H->dim());
+        return r;
+    };
+    const std::uint64_t layer_bytes = 16ull * static_cast<std::uint64_t>(M + 1);
+    auto wave_bytes = [&](std::uint64_t rs, std::uint64_t rc) { return layer_bytes * (std::max(rs, rc) + 2 * rc); };
+    struct Wave {
+        std::size_t b0, b1;
+        int R;
     };
     // waves: consecutive sources whose kept vectors for Rmax samples fit the budget (a source too
     // large for that forms a wave of its own and takes fewer samples per round)
-    std::size_t b0 = 0;
-    while (b0 < nS) {
+    std::vector<Wave> waves;
+    std::uint64_t need1 = 0, need2 = 0;   // region sizes, in vectors (region 3 equals region 2)
+    for (std::size_t b0 = 0; b0 < nS;) {
         std::size_t b1 = b0;
-        std::uint64_t bytes = 0;
+        std::uint64_t rs = 0, rc = 0;
         while (b1 < nS) {
-            const std::uint64_t need = per_sample(b1) * static_cast<std::uint64_t>(Rmax);
-            if (b1 > b0 && bytes + need > budget) break;
-            bytes += need;
+            const std::uint64_t rs2 = rs + static_cast<std::uint64_t>(sources[b1]->dim()), rc2 = rc + rows_c_of(b1);
+            if (b1 > b0 && wave_bytes(rs2, rc2) * static_cast<std::uint64_t>(Rmax) > budget) break;
+            rs = rs2;
+            rc = rc2;
             ++b1;
         }
-        std::uint64_t one = 0;
-        for (std::size_t b = b0; b < b1; ++b) one += per_sample(b);
-        const int Rfit = static_cast<int>(std::max<std::uint64_t>(1, std::min<std::uint64_t>(Rmax, budget / one)));
+        const int R = static_cast<int>(
+            std::max<std::uint64_t>(1, std::min<std::uint64_t>(Rmax, budget / std::max<std::uint64_t>(wave_bytes(rs, rc), 1))));
+        waves.push_back(Wave{b0, b1, R});
+        need1 = std::max(need1, std::max(rs, rc) * static_cast<std::uint64_t>(R));
+        need2 = std::max(need2, rc * static_cast<std::uint64_t>(R));
+        b0 = b1;
+    }
+    DeviceBuffer region1(static_cast<std::size_t>(need1 * layer_bytes));
+    DeviceBuffer region2(static_cast<std::size_t>(need2 * layer_bytes));
+    DeviceBuffer region3(static_cast<std::size_t>(need2 * layer_bytes));
+    DeviceBuffer partial, Omega;   // grown when a wave needs more, never shrunk
+    auto ensure = [](DeviceBuffer& b, std::size_t bytes) {
+        if (b.bytes() < bytes) b.reset(bytes);
+    };
+    for (const Wave& wv : waves) {
+        const std::size_t b0 = wv.b0, b1 = wv.b1;
+        const int Rfit = wv.R;
         for (int s0 = 0; s0 < o.samples; s0 += Rfit) {
             const int R = std::min(Rfit, o.samples - s0);
             ++out.waves;
@@ -204,11 +229,11 @@ DynWalkResult ftlm_dynamics_walk(const std::vector<std::shared_ptr<const SectorO
             }
             S.sample_start(seeds);
             const std::uint64_t layer_s = S.rows() * static_cast<std::uint64_t>(R);
-            DeviceBuffer Vs(static_cast<std::size_t>(layer_s) * (M + 1) * sizeof(dev::Zd));
+            dev::Zd* const Vs = region1.get<dev::Zd>();
             std::vector<double> aS, bS;
             std::vector<int> stS;
             BatchedTimings ts;
-            S.run_kept(M, tol_s, full_s, Vs.get<cplx>(), aS, bS, stS, ts);
+            S.run_kept(M, tol_s, full_s, reinterpret_cast<cplx*>(Vs), aS, bS, stS, ts);
             // ---- the channels' target runs, started from the probe images ----------------------
             std::vector<int> wch;
             for (std::size_t b = b0; b < b1; ++b)
@@ -217,7 +242,6 @@ DynWalkResult ftlm_dynamics_walk(const std::vector<std::shared_ptr<const SectorO
             std::vector<double> n2;
             std::vector<double> aC, bC;
             std::vector<int> stC;
-            DeviceBuffer Wt, AVt, Omega;
             std::vector<cplx> omega_h;
             std::vector<std::uint64_t> crow(1, 0);
             if (nc) {
@@ -236,33 +260,30 @@ DynWalkResult ftlm_dynamics_walk(const std::vector<std::shared_ptr<const SectorO
                 }
                 const PackedProbe P(probes, crow, colo, R);
                 BatchedLanczos T(wtgt, R);
-                P.apply(Vs.get<dev::Zd>(), reinterpret_cast<dev::Zd*>(T.start_buffer()));
+                P.apply(Vs, reinterpret_cast<dev::Zd*>(T.start_buffer()));
                 T.normalize(&n2);
                 std::vector<int> live(nc * static_cast<std::size_t>(R));
                 for (std::size_t i = 0; i < live.size(); ++i) live[i] = std::sqrt(n2[i]) >= 1e-14 ? 1 : 0;
                 const std::uint64_t layer_c = T.rows() * static_cast<std::uint64_t>(R);
-                DeviceBuffer Wb(static_cast<std::size_t>(layer_c) * (M + 1) * sizeof(dev::Zd));
+                dev::Zd* const Wb = region2.get<dev::Zd>();
                 BatchedTimings tc;
-                T.run_kept(M, tol_c, full_c, Wb.get<cplx>(), aC, bC, stC, tc, &live);
+                T.run_kept(M, tol_c, full_c, reinterpret_cast<cplx*>(Wb), aC, bC, stC, tc, &live);
                 out.lanczos_s += since(t_lz);
                 // ---- overlaps <A v_a | w_b>: the probe on every source layer, then one GEMM per pair
                 const auto t_ov = clk::now();
                 int mH = 0;
                 for (int v : stS) mH = std::max(mH, v);
-                DeviceBuffer AV(static_cast<std::size_t>(layer_c) * std::max(mH, 1) * sizeof(dev::Zd));
+                // region 3 <- A (region 1); region 1 <- region 3 transposed; region 3 <- region 2 transposed
+                dev::Zd* const AV = region3.get<dev::Zd>();
                 for (int j = 0; j < mH; ++j)
-                    P.apply(Vs.get<dev::Zd>() + static_cast<std::uint64_t>(j) * layer_s,
-                            AV.get<dev::Zd>() + static_cast<std::uint64_t>(j) * layer_c);
-                Vs.release();
-                AVt.reset(AV.bytes());
-                dev::batch_layers_transpose(AV.get<dev::Zd>(), T.rows(), R, std::max(mH, 1), AVt.get<dev::Zd>(), st);
-                AV.release();
-                Wt.reset(Wb.bytes());
-                dev::batch_layers_transpose(Wb.get<dev::Zd>(), T.rows(), R, M + 1, Wt.get<dev::Zd>(), st);
-                Wb.release();
+                    P.apply(Vs + static_cast<std::uint64_t>(j) * layer_s, AV + static_cast<std::uint64_t>(j) * layer_c);
+                dev::Zd* const AVt = region1.get<dev::Zd>();
+                dev::batch_layers_transpose(AV, T.rows(), R, std::max(mH, 1), AVt, st);
+                dev::Zd* const Wt = region3.get<dev::Zd>();
+                dev::batch_layers_transpose(Wb, T.rows(), R, M + 1, Wt, st);
                 QSA_HIP(hipGetLastError());
                 const std::size_t per = static_cast<std::size_t>(M) * static_cast<std::size_t>(M);
-                Omega.reset(nc * static_cast<std::size_t>(R) * per * sizeof(dev::Zd));
+                ensure(Omega, nc * static_cast<std::size_t>(R) * per * sizeof(dev::Zd));
                 // every live pair's rows in chunks of at most 8192: the overlaps as split products
                 const std::uint64_t rows_t = T.rows();
                 const int kav = std::max(mH, 1);
@@ -293,11 +314,11 @@ DynWalkResult ftlm_dynamics_walk(const std::vector<std::shared_ptr<const SectorO
                 }
                 if (!chunks.empty()) {
                     // every chunk's product in one launch, then the chunks of every pair summed in order
-                    DeviceBuffer partial(chunks.size() * per * sizeof(dev::Zd));
+                    ensure(partial, chunks.size() * per * sizeof(dev::Zd));
                     DeviceBuffer chunks_d = upload(chunks.data(), chunks.size());
                     DeviceBuffer begin_d = upload(chunk_begin.data(), chunk_begin.size());
                     DeviceBuffer ma_d = upload(ma.data(), ma.size()), mb_d = upload(mb.data(), mb.size());
-                    dev::overlap_chunks(AVt.get<dev::Zd>(), rows_t, Wt.get<dev::Zd>(), rows_t,
+                    dev::overlap_chunks(AVt, rows_t, Wt, rows_t,
                                         chunks_d.get<dev::OverlapChunk>(), static_cast<std::uint32_t>(chunks.size()),
                                         ma_d.get<int>(), mb_d.get<int>(), M, partial.get<dev::Zd>(), st);
                     dev::overlap_reduce(partial.get<dev::Zd>(), begin_d.get<std::uint32_t>(), ma_d.get<int>(),
@@ -308,9 +329,6 @@ DynWalkResult ftlm_dynamics_walk(const std::vector<std::shared_ptr<const SectorO
                 }
                 omega_h.resize(nc * static_cast<std::size_t>(R) * per);
                 copy_to_host(omega_h.data(), Omega.get(), omega_h.size() * sizeof(cplx));
-                AVt.release();
-                Wt.release();
-                Omega.release();
                 out.overlap_s += since(t_ov);
             } else {
                 out.lanczos_s += since(t_lz);
@@ -390,7 +408,6 @@ DynWalkResult ftlm_dynamics_walk(const std::vector<std::shared_ptr<const SectorO
             });
             out.host_s += since(t_host);
         }
-        b0 = b1;
     }
     // every source's samples combined about their common reference, in sample order
     const auto t_host = clk::now();
