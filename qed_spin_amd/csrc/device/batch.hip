@@ -499,26 +499,44 @@ __global__ void k_batch_layers_transpose(const Zd* in, u64 rows, int R, int K, Z
     }
 }
 
-// Kept-basis overlaps X^H Y per chunk of rows: a workgroup owns a 64 x 64 tile of the (a, b) product
-// of one chunk. 32-row slices of its 64 X and 64 Y vectors are staged in LDS with the vector index
-// fastest, and thread (tx, ty) accumulates a = a0 + ty + 16 i, b = b0 + tx + 16 j (i, j < 4) over the
-// chunk's rows in order, with explicit fused multiply-adds: the 16 threads of a row of the wave read
-// 16 consecutive Y entries (no bank conflicts) and share 4 X entries (broadcasts).
-constexpr int kOvTile = 64, kOvRows = 32;
-__global__ __launch_bounds__(256) void k_overlap_chunk(const Zd* X, u64 ldx, const Zd* Y, u64 ldy,
-                                                       const OverlapChunk* chunks, const int* ma, const int* mb,
-                                                       int kmax, Zd* partial) {
-    __shared__ Zd xs[kOvRows][kOvTile + 1], ys[kOvRows][kOvTile + 1];   // padded: the row-major stores spread over banks
-    const OverlapChunk ch = chunks[blockIdx.x];
-    const int a0 = blockIdx.y * kOvTile, b0 = blockIdx.z * kOvTile;
+// Kept-basis overlaps X^H Y per chunk of rows on the FP64 matrix cores. A workgroup owns a 64 x 64
+// tile of one chunk's (a, b) product; its four waves own 32 x 32 quarters, each four 16 x 16 MFMA
+// tiles with a real and an imaginary accumulator (re += xr yr + xi yi, im += xr yi - xi yr: four
+// v_mfma_f64_16x16x4 per tile and four rows). 16-row slices of the tile's 64 X and 64 Y vectors are
+// staged in LDS with the vector index fastest. Operand layout of v_mfma_f64_16x16x4 (wave64): lane l
+// holds A[l % 16][l / 16] and B[l / 16][l % 16]; accumulator entry v of lane l is
+// D[4 v + l / 16][l % 16]. A wave skips its 16 x 16 tiles that lie outside the pair's ma x mb product.
+// The grid is one-dimensional: workgroup g runs on XCD g % 8, and the tiles of a chunk are
+// consecutive on one XCD, so the chunk's vectors, read by every tile of it, stay in that XCD's L2.
+typedef double mf_acc __attribute__((ext_vector_type(4)));
+constexpr int kOvTile = 64, kOvRows = 16, kXcds = 8;
+__global__ __launch_bounds__(256) void k_overlap_mfma(const Zd* X, u64 ldx, const Zd* Y, u64 ldy,
+                                                      const OverlapChunk* chunks, u32 n_chunks, int tiles,
+                                                      const int* ma, const int* mb, int kmax, Zd* partial) {
+    __shared__ Zd xs[kOvRows][kOvTile + 1], ys[kOvRows][kOvTile + 1];   // padded: the column stores spread over banks
+    const u32 g = blockIdx.x, xcd = g % kXcds, slot = g / kXcds;
+    const u32 per_chunk = static_cast<u32>(tiles * tiles);
+    const u32 q = (slot / per_chunk) * kXcds + xcd;   // the chunk
+    if (q >= n_chunks) return;
+    const u32 tile = slot % per_chunk;
+    const OverlapChunk ch = chunks[q];
+    const int a0 = static_cast<int>(tile / static_cast<u32>(tiles)) * kOvTile;
+    const int b0 = static_cast<int>(tile % static_cast<u32>(tiles)) * kOvTile;
     const int na = ma[ch.pair], nb = mb[ch.pair];
     if (a0 >= na || b0 >= nb) return;   // the whole workgroup: tile outside this pair's product
-    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-    double acc_r[4][4], acc_i[4][4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wa = (wave & 1) * 32, wb = (wave >> 1) * 32;
+    const int li = lane & 15, lk = lane >> 4;
+    bool on[2][2];
+    mf_acc cr[2][2], ci[2][2];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc_r[i][j] = acc_i[i][j] = 0.0;
+        for (int j = 0; j < 2; ++j) {
+            on[i][j] = a0 + wa + 16 * i < na && b0 + wb + 16 * j < nb;   // uniform over the wave
+            cr[i][j] = mf_acc{0.0, 0.0, 0.0, 0.0};
+            ci[i][j] = mf_acc{0.0, 0.0, 0.0, 0.0};
+        }
     for (u32 r0 = 0; r0 < ch.rows; r0 += kOvRows) {
         for (int idx = threadIdx.x; idx < kOvTile * kOvRows; idx += 256) {
             const int k = idx / kOvRows, rr = idx % kOvRows;   // consecutive threads: consecutive rows
@@ -528,30 +546,43 @@ __global__ __launch_bounds__(256) void k_overlap_chunk(const Zd* X, u64 ldx, con
             ys[rr][k] = (in_row && b0 + k < nb) ? Y[ch.y_base + static_cast<u64>(b0 + k) * ldy + row] : Zd{0.0, 0.0};
         }
         __syncthreads();
-        for (int rr = 0; rr < kOvRows; ++rr) {
-            Zd xv[4], yv[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) xv[i] = xs[rr][ty + 16 * i];
+        for (int kk = 0; kk < kOvRows / 4; ++kk) {
+            const int rr = kk * 4 + lk;
+            Zd xv[2], yv[2];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) yv[j] = ys[rr][tx + 16 * j];
+            for (int i = 0; i < 2; ++i) xv[i] = xs[rr][wa + 16 * i + li];
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 2; ++j) yv[j] = ys[rr][wb + 16 * j + li];
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    acc_r[i][j] = fma(xv[i].re, yv[j].re, fma(xv[i].im, yv[j].im, acc_r[i][j]));
-                    acc_i[i][j] = fma(xv[i].re, yv[j].im, fma(-xv[i].im, yv[j].re, acc_i[i][j]));
-                }
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    if (on[i][j]) {
+                        cr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(xv[i].re, yv[j].re, cr[i][j], 0, 0, 0);
+                        ci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(xv[i].re, yv[j].im, ci[i][j], 0, 0, 0);
+                    }
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    if (on[i][j]) {
+                        cr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(xv[i].im, yv[j].im, cr[i][j], 0, 0, 0);
+                        ci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-xv[i].im, yv[j].re, ci[i][j], 0, 0, 0);
+                    }
         }
         __syncthreads();
     }
-    Zd* out = partial + static_cast<u64>(blockIdx.x) * kmax * kmax;
+    Zd* out = partial + static_cast<u64>(q) * kmax * kmax;
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int a = a0 + ty + 16 * i, b = b0 + tx + 16 * j;
-            if (a < na && b < nb) out[static_cast<u64>(b) * na + a] = Zd{acc_r[i][j], acc_i[i][j]};
-        }
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int a = a0 + wa + 16 * i + 4 * v + lk, b = b0 + wb + 16 * j + li;
+                if (a < na && b < nb) out[static_cast<u64>(b) * na + a] = Zd{cr[i][j][v], ci[i][j][v]};
+            }
 }
 
 // omega[p](a, b) = the sum of the pair's chunk partials in chunk order; one thread per (pair, a, b).
@@ -687,9 +718,10 @@ void batch_layers_transpose(const Zd* in, u64 rows, int R, int K, Zd* out, hipSt
 void overlap_chunks(const Zd* X, u64 ldx, const Zd* Y, u64 ldy, const OverlapChunk* chunks, u32 n_chunks,
                     const int* ma, const int* mb, int kmax, Zd* partial, hipStream_t st) {
     if (!n_chunks) return;
-    const unsigned tiles = static_cast<unsigned>((kmax + kOvTile - 1) / kOvTile);
-    hipLaunchKernelGGL(k_overlap_chunk, dim3(n_chunks, tiles, tiles), dim3(256), 0, st, X, ldx, Y, ldy, chunks, ma, mb,
-                       kmax, partial);
+    const int tiles = (kmax + kOvTile - 1) / kOvTile;
+    const u64 groups = (static_cast<u64>(n_chunks) + kXcds - 1) / kXcds * kXcds * static_cast<u64>(tiles * tiles);
+    hipLaunchKernelGGL(k_overlap_mfma, dim3(static_cast<unsigned>(groups)), dim3(256), 0, st, X, ldx, Y, ldy, chunks,
+                       n_chunks, tiles, ma, mb, kmax, partial);
 }
 
 void overlap_reduce(const Zd* partial, const u32* chunk_begin, const int* ma, const int* mb, u32 n_pairs, int kmax,

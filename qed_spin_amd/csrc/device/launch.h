@@ -86,8 +86,9 @@ struct CsrView {
     int dict_n = 0;                // dictionary entries (u8 / u16 ids)
     int lds_dict = 1;              // u8 ids: stage the dictionary in LDS
     int wg = 256;                  // single-vector SpMV: threads per workgroup (256, 512, 1024)
-    // SELL-64 layout (sell_ptr non-null): col / vals hold the entries slice by slice, entry k of row
-    // 64 s + j at sell_ptr[s] + 64 k + j; sell_len[row] is the row's entry count. One lane per row.
+    // SELL-64 layout (sell_ptr non-null): col / vals hold the entries slice by slice in groups of
+    // four, entry k of row 64 s + j at sell_ptr[s] + 256 (k / 4) + 4 j + k % 4 (slices padded to
+    // whole groups); sell_len[row] is the row's entry count. One lane per row.
     const u64* sell_ptr = nullptr;
     const u16* sell_len = nullptr;
     int sell_acc = 1;              // SELL: partial sums per row (1, or 4 combined by a fixed tree)
@@ -243,7 +244,7 @@ struct OverlapChunk {
     u32 rows, pair;
 };
 /// partial[q] (column-major ma x mb of chunk q's pair, at partial + q * kmax * kmax) = the chunk's
-/// X^H Y over its rows in order (fused multiply-adds).
+/// X^H Y over its rows (FP64 matrix-core products, 16-row slices in order).
 void overlap_chunks(const Zd* X, u64 ldx, const Zd* Y, u64 ldy, const OverlapChunk* chunks, u32 n_chunks,
                     const int* ma, const int* mb, int kmax, Zd* partial, hipStream_t st);
 /// omega[p] (column-major ma[p] x mb[p], at omega + p * kmax * kmax) = the sum, in chunk order, of

@@ -216,16 +216,73 @@ __global__ __launch_bounds__(256) void k_spmv_block(CsrView A, const void* X, vo
 }
 
 // ---- SELL-64 ------------------------------------------------------------------------------------
-// Slice s holds rows 64 s .. 64 s + 63 column-major: entry k of row 64 s + j sits at
-// sell_ptr[s] + 64 k + j, so the k-th entries of a wavefront's 64 rows are one coalesced 256-byte
-// load. A slice is padded to its longest row; the padding is never read (each lane stops at its
-// own row length). One lane owns a row and sums its entries in CSR order, one after another: the
-// sum depends neither on the workgroup size nor on how many gathers are in flight, and a k-vector
-// apply equals k single applies bit for bit. U entries per round: their column words first, then
-// the U gathers together, then the sums in entry order.
+// Slice s holds rows 64 s .. 64 s + 63 in groups of four entries: entry k of row 64 s + j sits at
+// sell_ptr[s] + 256 (k / 4) + 4 j + k % 4, so one 16-byte load gives a lane four consecutive
+// entries of its row and the group loads of a wavefront's 64 rows are one contiguous kilobyte. A
+// slice is padded to its longest row, rounded up to whole groups; the padding is zero and no entry
+// beyond a row's length is used. One lane owns a row and sums its entries in CSR order: the sum
+// depends neither on the workgroup size nor on how many gathers are in flight, and a k-vector
+// apply equals k single applies bit for bit. U entries per round: their groups first, then the U
+// gathers together, then the sums in entry order.
 // ACC partial sums per row and vector: entry k goes to sum k mod ACC (k in the row's order), and
 // the sums meet in a fixed tree, (s0 + s2) + (s1 + s3) for ACC = 4 -- the order a 4-lane butterfly
 // gives, so the block kernel's 4 lanes per row produce the same bits.
+__device__ __forceinline__ u64 sell_at(u64 slice, u32 j, u32 k) {
+    return slice + static_cast<u64>(k >> 2) * 256 + static_cast<u64>(j) * 4 + (k & 3u);
+}
+
+typedef u32 u32x4 __attribute__((ext_vector_type(4)));
+typedef u16 u16x4 __attribute__((ext_vector_type(4)));
+typedef u8 u8x4 __attribute__((ext_vector_type(4)));
+
+template <class V> __device__ __forceinline__ V load_vec(const CsrView& A, const void* p) {
+    return A.nt ? __builtin_nontemporal_load(static_cast<const V*>(p)) : *static_cast<const V*>(p);
+}
+
+// The four entries of the group at e (a multiple of four): columns and values.
+template <int WIDTH, bool VREAL>
+__device__ __forceinline__ void load_quad(const CsrView& A, const Dict& d, u64 e, u32* c, double* vr, double* vi) {
+    const u32x4 w = load_vec<u32x4>(A, A.col + e);
+    if constexpr (WIDTH == 3) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            c[u] = w[u] & 0xFFFFFFu;
+            const u32 id = w[u] >> 24;
+            vr[u] = d.re[id];
+            if constexpr (!VREAL) vi[u] = d.im[id];
+        }
+    } else {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) c[u] = w[u];
+        if constexpr (WIDTH == 1) {
+            const u8x4 id = load_vec<u8x4>(A, static_cast<const u8*>(A.vals) + e);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                vr[u] = d.re[id[u]];
+                if constexpr (!VREAL) vi[u] = d.im[id[u]];
+            }
+        } else if constexpr (WIDTH == 2) {
+            const u16x4 id = load_vec<u16x4>(A, static_cast<const u16*>(A.vals) + e);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                vr[u] = A.dict_re[id[u]];
+                if constexpr (!VREAL) vi[u] = A.dict_im[id[u]];
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if constexpr (VREAL) {
+                    vr[u] = static_cast<const double*>(A.vals)[e + u];
+                } else {
+                    const Zd z = static_cast<const Zd*>(A.vals)[e + u];
+                    vr[u] = z.re;
+                    vi[u] = z.im;
+                }
+            }
+        }
+    }
+}
+
 template <int ACC> __device__ __forceinline__ double tree(const double* s) {
     if constexpr (ACC == 4) return (s[0] + s[2]) + (s[1] + s[3]);
     else return s[0];
@@ -233,13 +290,13 @@ template <int ACC> __device__ __forceinline__ double tree(const double* s) {
 
 template <int WIDTH, bool VREAL, bool REAL, int NV, int U, int ACC>
 __global__ __launch_bounds__(1024) void k_sell(CsrView A, Ptrs<NV> v) {
-    static_assert(U % ACC == 0 || U == 1, "unroll in whole rounds of the partial sums");
+    static_assert(U % 4 == 0, "whole groups of four entries per round");
     __shared__ double sre[(WIDTH == 1 || WIDTH == 3) ? 256 : 1], sim[(WIDTH == 1 || WIDTH == 3) && !VREAL ? 256 : 1];
     const Dict d = stage_dict<WIDTH, VREAL>(A, sre, sim);
     const i64 row = row_block(A) * blockDim.x + threadIdx.x;
     if (row >= A.rows) return;
     const u32 n = A.nt ? __builtin_nontemporal_load(A.sell_len + row) : A.sell_len[row];
-    const u64 base = A.sell_ptr[row >> 6] + static_cast<u64>(row & 63);
+    const u64 first = A.sell_ptr[row >> 6] + static_cast<u64>(row & 63) * 4;   // group g at first + 256 g
     double ar[NV][ACC], ai[NV][ACC];
 #pragma unroll
     for (int k = 0; k < NV; ++k)
@@ -258,32 +315,32 @@ __global__ __launch_bounds__(1024) void k_sell(CsrView A, Ptrs<NV> v) {
         }
     };
     u32 k = 0;
-    if constexpr (U > 1) {
-        for (; k + U <= n; k += U) {
-            u32 c[U];
-            double vr[U], vi[U];
+    for (; k + U <= n; k += U) {
+        u32 c[U];
+        double vr[U], vi[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                vi[u] = 0.0;
-                load_entry<WIDTH, VREAL>(A, d, base + static_cast<u64>(k + u) * 64, c[u], vr[u], vi[u]);
-            }
-            if constexpr (REAL && NV == 1) {
-                double x[U];
+        for (int u = 0; u < U; ++u) vi[u] = 0.0;
 #pragma unroll
-                for (int u = 0; u < U; ++u) x[u] = static_cast<const double*>(v.in[0])[c[u]];
+        for (int g = 0; g < U / 4; ++g)
+            load_quad<WIDTH, VREAL>(A, d, first + static_cast<u64>(k / 4 + g) * 256, c + 4 * g, vr + 4 * g, vi + 4 * g);
+        if constexpr (REAL && NV == 1) {
+            double x[U];
 #pragma unroll
-                for (int u = 0; u < U; ++u) ar[0][u % ACC] += vr[u] * x[u];
-            } else {
+            for (int u = 0; u < U; ++u) x[u] = static_cast<const double*>(v.in[0])[c[u]];
 #pragma unroll
-                for (int u = 0; u < U; ++u) accumulate(u % ACC, c[u], vr[u], vi[u]);
-            }
+            for (int u = 0; u < U; ++u) ar[0][u % ACC] += vr[u] * x[u];
+        } else {
+#pragma unroll
+            for (int u = 0; u < U; ++u) accumulate(u % ACC, c[u], vr[u], vi[u]);
         }
     }
-    for (; k < n; ++k) {
-        u32 c;
-        double vr, vi = 0.0;
-        load_entry<WIDTH, VREAL>(A, d, base + static_cast<u64>(k) * 64, c, vr, vi);
-        accumulate(static_cast<int>(k % ACC), c, vr, vi);
+    for (; k < n; k += 4) {   // the last groups, entries below n (k stays a multiple of four)
+        u32 c[4];
+        double vr[4], vi[4] = {0.0, 0.0, 0.0, 0.0};
+        load_quad<WIDTH, VREAL>(A, d, first + static_cast<u64>(k / 4) * 256, c, vr, vi);
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (k + static_cast<u32>(u) < n) accumulate(u % ACC, c[u], vr[u], vi[u]);
     }
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
@@ -307,11 +364,12 @@ __global__ __launch_bounds__(256) void k_sell_block(CsrView A, const void* X, vo
     for (int q = 0; q < NV; ++q) ar[q] = ai[q] = 0.0;
     if (row < A.rows) {
         const u32 n = A.sell_len[row];
-        const u64 base = A.sell_ptr[row >> 6] + static_cast<u64>(row & 63);
+        const u64 slice = A.sell_ptr[row >> 6];
+        const u32 j = static_cast<u32>(row & 63);
         for (u32 k = static_cast<u32>(lane); k < n; k += ACC) {
             u32 c32;
             double vr, vi = 0.0;
-            load_entry<WIDTH, VREAL>(A, d, base + static_cast<u64>(k) * 64, c32, vr, vi);
+            load_entry<WIDTH, VREAL>(A, d, sell_at(slice, j, k), c32, vr, vi);
             const u64 c = c32;
 #pragma unroll
             for (int q = 0; q < NV; ++q) {
@@ -362,9 +420,7 @@ template <int WIDTH, bool VREAL, bool REAL> void launch_sell_set(const CsrView& 
             } else {
                 if (NV == 1 && A.unroll >= 16) hipLaunchKernelGGL((k_sell<WIDTH, VREAL, REAL, NV, 16, 1>), grid, block, 0, st, A, p);
                 else if (NV == 1 && A.unroll >= 8) hipLaunchKernelGGL((k_sell<WIDTH, VREAL, REAL, NV, 8, 1>), grid, block, 0, st, A, p);
-                else if (A.unroll >= 4) hipLaunchKernelGGL((k_sell<WIDTH, VREAL, REAL, NV, 4, 1>), grid, block, 0, st, A, p);
-                else if (A.unroll >= 2) hipLaunchKernelGGL((k_sell<WIDTH, VREAL, REAL, NV, 2, 1>), grid, block, 0, st, A, p);
-                else hipLaunchKernelGGL((k_sell<WIDTH, VREAL, REAL, NV, 1, 1>), grid, block, 0, st, A, p);
+                else hipLaunchKernelGGL((k_sell<WIDTH, VREAL, REAL, NV, 4, 1>), grid, block, 0, st, A, p);
             }
         };
         if (nv == 8) go(std::integral_constant<int, 8>{});
@@ -505,7 +561,7 @@ __global__ __launch_bounds__(256) void k_sell_lengths(const u64* row_ptr, i64 ro
         const u32 o = static_cast<u32>(__shfl_xor(static_cast<int>(m), off, 64));
         m = o > m ? o : m;
     }
-    if ((threadIdx.x & 63) == 0 && r < rows) slice_words[r >> 6] = static_cast<u64>(m) * 64;
+    if ((threadIdx.x & 63) == 0 && r < rows) slice_words[r >> 6] = static_cast<u64>((m + 3u) & ~3u) * 64;
 }
 
 // diag_first (a square operator): a row's diagonal entry, when it has one, moves to its first slot,
@@ -517,7 +573,8 @@ __global__ __launch_bounds__(256) void k_sell_fill(const u64* row_ptr, const u32
     const i64 r = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (r >= rows) return;
     const u64 e0 = row_ptr[r], n = row_ptr[r + 1] - e0;
-    const u64 base = sell_ptr[r >> 6] + static_cast<u64>(r & 63);
+    const u64 slice = sell_ptr[r >> 6];
+    const u32 j = static_cast<u32>(r & 63);
     u64 diag = n;   // the diagonal entry's position in the row (n: none)
     if (diag_first)
         for (u64 k = 0; k < n && diag == n; ++k)
@@ -525,7 +582,7 @@ __global__ __launch_bounds__(256) void k_sell_fill(const u64* row_ptr, const u32
     for (u64 k = 0; k < n; ++k) {
         // slot k takes the diagonal first, then the row's other entries in order
         const u64 from = diag == n ? k : (k == 0 ? diag : (k <= diag ? k - 1 : k));
-        const u64 dst = base + k * 64, src = e0 + from;
+        const u64 dst = sell_at(slice, j, static_cast<u32>(k)), src = e0 + from;
         sell_col[dst] = col[src];
         if (vb == 1) static_cast<u8*>(sell_vals)[dst] = static_cast<const u8*>(vals)[src];
         else if (vb == 2) static_cast<u16*>(sell_vals)[dst] = static_cast<const u16*>(vals)[src];

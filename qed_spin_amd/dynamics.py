@@ -176,20 +176,24 @@ def _dynamics_finite_T(H, probes, omega, T, eta, blocks, krylov, samples, seed, 
     for O in probes:
         O._handle()
     src = list(range(len(blocks))) if sources is None else list(sources)
+    t_probes = time.perf_counter()
     seeds, chan_src, chan_h, chan_a, chan_probe, keep = [], [], [], [], [], []
+    # every source's probe operators, built on the host threads (results in source order)
+    reaches = parallel_build(
+        lambda b: [(p, t, X) for p, O in enumerate(probes) for t, X in _reach(blocks, b, O, deltas[p])], src)
     for i, b in enumerate(src):
         blk, sec, m = blocks[b]
         irr = blk.group.G if blk.momentum is not None else 1
         key = ((sec.n_up + 1) * 3 + 0) * irr + (blk.momentum or 0)
         seeds.append(block_seed(s0, key))
-        for p, O in enumerate(probes):
-            for t, X in _reach(blocks, b, O, deltas[p]):
-                chan_src.append(i)
-                chan_h.append(blocks[t][2].h.value)
-                chan_a.append(X.h.value)
-                chan_probe.append(p)
-                keep.append(X)
+        for p, t, X in reaches[i]:
+            chan_src.append(i)
+            chan_h.append(blocks[t][2].h.value)
+            chan_a.append(X.h.value)
+            chan_probe.append(p)
+            keep.append(X)
     nS, nC = len(src), len(chan_src)
+    t_probes = time.perf_counter() - t_probes
     sh = (C.c_void_p * nS)(*[blocks[b][2].h.value for b in src])
     seed_a = np.array(seeds, np.uint64)
     tol_a = np.full(nS, tol)
@@ -221,6 +225,7 @@ def _dynamics_finite_T(H, probes, omega, T, eta, blocks, krylov, samples, seed, 
             Sres[it] /= Zp
         out.append(Sres)
     engine = dict(zip(["lanczos_s", "overlap_s", "host_s", "total_s", "waves"], stats.tolist()))
+    engine.update(probes_s=t_probes, channels=nC)
     return out, ref, engine
 
 
@@ -249,10 +254,10 @@ def dynamics(H: Operator, probes, omega, *, eta: float = 0.1, T=None, sym: Optio
     finite_T = T is not None and len(np.atleast_1d(T)) > 0
     own = blocks is None
     if blocks is None:
-        # T = 0: the continued fractions run in the batched engine and the ground search's applies are
-        # few, so the blocks keep the default SpMV shape; T > 0 applies every source block many times
-        # (T = 0 builds a block's operator only when the ground search or a probe image needs it)
-        blocks = walk_blocks(H, sym, csr_budget, tune=finite_T, lazy=not finite_T)
+        # every apply of a walk runs in the batched engine (the continued fractions at T = 0, every
+        # Lanczos run at T > 0) or is one of the ground search's few, so no block's SpMV is tuned;
+        # T = 0 builds a block's operator only when the ground search or a probe image needs it
+        blocks = walk_blocks(H, sym, csr_budget, tune=False, lazy=not finite_T)
     if not blocks:
         raise InvalidRequest("dynamics: the symmetry selection holds no state")
     t_walk = time.perf_counter() - t0

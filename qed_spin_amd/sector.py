@@ -4,6 +4,7 @@ RepSectorMatVec / CrossSectorMatVec device lanes, behind include/qedh.h)."""
 from __future__ import annotations
 
 import ctypes as C
+import os
 from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass
 from typing import Callable, List, Optional, Sequence
@@ -18,18 +19,19 @@ LANE_AUTO, LANE_GATHER, LANE_CSR = 0, 1, 2
 LANE_NO_TUNE = 0x100             # OR-ed into the lane: skip the SpMV autotuning
 DEFAULT_RANK_BUDGET = 16 << 30   # dense rank table up to 16 GiB of HBM
 AUTOTUNE_MIN_ROWS = 1 << 18      # CSR operators from this many rows time their SpMV variants when built
-BUILD_THREADS = 8
+BUILD_THREADS = int(os.environ.get("QED_SPIN_AMD_BUILD_THREADS", "8"))   # host threads building sectors and operators
 
 
 def _own_stream() -> None:
     _lib.call("qedh_thread_stream", 1)
 
 
-def parallel_build(fn: Callable, items: Sequence, threads: int = BUILD_THREADS) -> List:
+def parallel_build(fn: Callable, items: Sequence, threads: Optional[int] = None) -> List:
     """fn(item) for every item on host threads, each with a HIP stream of its own (builds of small
     sectors and operators are launch- and synchronisation-bound, so several in flight overlap);
     results in item order. Every build call synchronises its stream before it returns."""
     items = list(items)
+    threads = BUILD_THREADS if threads is None else threads
     if len(items) <= 1 or threads <= 1:
         return [fn(x) for x in items]
     with ThreadPoolExecutor(max_workers=min(threads, len(items)), initializer=_own_stream) as ex:
