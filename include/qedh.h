@@ -329,10 +329,12 @@ int qedh_cf_batched(const qedh_matvec* blocks, int n_blocks, int R, const void* 
  * through the packed multi-vector engine with kept bases (full CGS2 on sectors of at most
  * max(4 krylov, 2^16) states, LocalDGKS3 on larger ones); the overlaps <A v_a|w_b> are products of
  * the kept bases split over row chunks, summed per pair in a fixed order; waves of sources keep the
- * kept vectors within budget_bytes (0: half the device memory free at the call). Outputs,
+ * kept vectors within budget_bytes (0: two fifths of the device's memory, at most nine tenths of what
+ * is free at the call). Outputs,
  * un-normalised (times each source's dimension / samples done): Z[n_sources][n_t], S[n_channels][n_t][n_omega] complex
  * (re, im); e_min[n_sources] each source's samples' common reference and samples_done[n_sources]
- * (each may be NULL); stats[5] = Lanczos s, overlap s, host s, total s, waves (or NULL). */
+ * (each may be NULL); stats[6] = Lanczos s, overlap s, host s, total s, waves, set-up s (the wave plan and
+ * the kept-vector regions; or NULL). */
 int qedh_ftlm_dynamics_walk(int n_sources, const qedh_matvec* source_h, const uint64_t* source_seeds,
                             const double* breakdown_tol, int n_channels, const int* channel_source,
                             const qedh_matvec* channel_h, const qedh_matvec* channel_probe, int samples, int krylov,

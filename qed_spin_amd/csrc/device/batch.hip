@@ -272,6 +272,20 @@ __device__ __forceinline__ void block_sum(const BatchView& B, int b, int s, cons
     c = rc[0];
 }
 
+// The DGKS gate of the second Gram-Schmidt pass, one workgroup per (block, vector): the pass is
+// repeated when the first one left less than kappa of the norm it found.
+__global__ __launch_bounds__(kThreads) void k_batch_dgks_gate(BatchView B, const double* before, const double* after,
+                                                              const int* full, double kappa, int* on) {
+    const u32 b = blockIdx.x / static_cast<u32>(B.R);
+    const int s = static_cast<int>(blockIdx.x % static_cast<u32>(B.R));
+    double n0 = 0.0, n1 = 0.0, unused = 0.0;
+    block_sum(B, static_cast<int>(b), s, before, n0, unused);
+    __syncthreads();   // every thread has read the first sum before the LDS is reused
+    block_sum(B, static_cast<int>(b), s, after, n1, unused);
+    if (threadIdx.x == 0)
+        on[blockIdx.x] = (full[b] && pair_live(B, b, s) && sqrt(n1) < kappa * sqrt(n0)) ? 1 : 0;
+}
+
 // One workgroup per (block, vector). kind 0: alpha (real part) -> alpha_hist[step]; kind 1: the
 // overlap (complex) -> ov; kind 2: beta = sqrt -> beta_hist[step], 1 / beta -> inv_beta; breakdown
 // (beta < tol) or the block's Krylov cap ends the pair's run.
@@ -432,20 +446,31 @@ __global__ void k_pack_csr(CsrView A, u64 row_off, u64 col_off, u64 nnz_off, u64
     }
 }
 
+// Whether block b takes a Gram-Schmidt pass: it is fully reorthogonalised and, with per-pair flags,
+// at least one of its pairs is flagged.
+__device__ __forceinline__ bool gs_block(const BatchView& B, u32 b, const int* full, const int* on) {
+    if (!full[b]) return false;
+    if (!on) return true;
+    bool any = false;
+    for (int q = 0; q < B.R; ++q) any = any || on[static_cast<u64>(b) * B.R + q] != 0;
+    return any;
+}
+
 // Full Gram-Schmidt against kept basis layers (layer k at basis + k * layer, the batch layout), for
-// the blocks flagged in full. Coefficients: one workgroup per (block, k); thread (slot, s) sums
-// <V_k, W> over the block's rows r = first + slot, first + slot + slots, ..., then the slots' sums
-// in slot order: coef[(b * kmax + k) * R + s]. Deterministic, independent of the other blocks.
+// the blocks flagged in full (and, with on, the pairs flagged there). Coefficients: one workgroup
+// per (block, k); thread (slot, s) sums <V_k, W> over the block's rows r = first + slot, first +
+// slot + slots, ..., then the slots' sums in slot order: coef[(b * kmax + k) * R + s].
+// Deterministic, independent of the other blocks.
 __global__ __launch_bounds__(kThreads) void k_batch_gs_coef(BatchView B, const Zd* basis, u64 layer, int K, int kmax,
-                                                            const Zd* W, const int* full, Zd* coef) {
+                                                            const Zd* W, const int* full, const int* on, Zd* coef) {
     const u32 b = blockIdx.x / static_cast<u32>(K);
     const int k = static_cast<int>(blockIdx.x % static_cast<u32>(K));
-    if (!full[b]) return;   // the whole workgroup
+    if (!gs_block(B, b, full, on)) return;   // the whole workgroup
     const int R = B.R, slots = kThreads / R;
     const int slot = threadIdx.x / R, s = threadIdx.x % R;
     const u64 first = B.tiles[B.tile_begin[b]].row_begin, last = B.tiles[B.tile_begin[b + 1] - 1].row_end;
     double pr = 0.0, pi = 0.0;
-    if (slot < slots && pair_live(B, b, s)) {
+    if (slot < slots && pair_live(B, b, s) && (!on || on[static_cast<u64>(b) * R + s])) {
         const Zd* V = basis + static_cast<u64>(k) * layer;
         for (u64 r = first + slot; r < last; r += slots) {
             const u64 i = r * R + s;
@@ -470,12 +495,12 @@ __global__ __launch_bounds__(kThreads) void k_batch_gs_coef(BatchView B, const Z
 
 // W -= sum_{k < K} coef_k V_k on the tiles of the flagged blocks, k ascending.
 __global__ __launch_bounds__(kThreads) void k_batch_gs_update(BatchView B, const Zd* basis, u64 layer, int K, int kmax,
-                                                              Zd* W, const int* full, const Zd* coef) {
+                                                              Zd* W, const int* full, const int* on, const Zd* coef) {
     const Tile t = B.tiles[blockIdx.x];
-    if (!full[t.block]) return;
+    if (!gs_block(B, t.block, full, on)) return;
     const int R = B.R, slots = kThreads / R;
     const int slot = threadIdx.x / R, s = threadIdx.x % R;
-    if (slot >= slots || !pair_live(B, t.block, s)) return;
+    if (slot >= slots || !pair_live(B, t.block, s) || (on && !on[static_cast<u64>(t.block) * R + s])) return;
     const Zd* c = coef + static_cast<u64>(t.block) * kmax * R + s;
     for (u64 r = t.row_begin + slot; r < t.row_end; r += slots) {
         const u64 i = r * R + s;
@@ -696,12 +721,19 @@ void csr_to_dense(const CsrView& A, Zd* out, hipStream_t st) {
 }
 
 void batch_gram_schmidt(const BatchView& B, const Zd* basis, u64 layer, int K, int kmax, Zd* W, const int* full,
-                        Zd* coef, hipStream_t st) {
+                        Zd* coef, hipStream_t st, const int* on) {
     if (!B.n_tiles || K < 1) return;
     hipLaunchKernelGGL(k_batch_gs_coef, dim3(B.n_blocks * static_cast<u32>(K)), dim3(kThreads), 0, st, B, basis, layer,
-                       K, kmax, W, full, coef);
+                       K, kmax, W, full, on, coef);
     hipLaunchKernelGGL(k_batch_gs_update, dim3(B.n_tiles), dim3(kThreads), 0, st, B, basis, layer, K, kmax, W, full,
-                       coef);
+                       on, coef);
+}
+
+void batch_dgks_gate(const BatchView& B, const double* before, const double* after, const int* full, double kappa,
+                     int* on, hipStream_t st) {
+    if (!B.n_blocks) return;
+    hipLaunchKernelGGL(k_batch_dgks_gate, dim3(B.n_blocks * static_cast<u32>(B.R)), dim3(kThreads), 0, st, B, before,
+                       after, full, kappa, on);
 }
 
 void batch_norm_partial(const BatchView& B, const Zd* V, double* partial, hipStream_t st) {

@@ -228,10 +228,16 @@ void pack_csr(const CsrView& A, u64 row_off, u64 col_off, u64 nnz_off, u64* row_
               u32 dict_off, hipStream_t st);
 void batch_advance(const BatchView& B, hipStream_t st);
 /// One classical Gram-Schmidt pass of the batch vectors W against kept layers 0 .. K-1 (layer k at
-/// basis + k * layer entries, the batch layout), only for the blocks whose full[b] is set:
+/// basis + k * layer entries, the batch layout), only for the blocks whose full[b] is set and, with
+/// on (one flag per pair), only for the pairs flagged there:
 /// coef[(b * kmax + k) * R + s] = <V_k, W> per pair (fixed-order sums), then W -= sum_k coef_k V_k.
 void batch_gram_schmidt(const BatchView& B, const Zd* basis, u64 layer, int K, int kmax, Zd* W, const int* full,
-                        Zd* coef, hipStream_t st);
+                        Zd* coef, hipStream_t st, const int* on = nullptr);
+/// The DGKS gate of the second pass ("twice is enough"): on[pair] = full[block], the pair runs, and
+/// ||W|| after the first pass < kappa ||W|| before it, from the tile partials of ||W||^2 before and
+/// after (batch_norm_partial's layout).
+void batch_dgks_gate(const BatchView& B, const double* before, const double* after, const int* full, double kappa,
+                     int* on, hipStream_t st);
 /// partial[tile][s] = the tile's part of ||V_s||^2 (what batch_finish kind 2 sums into beta).
 void batch_norm_partial(const BatchView& B, const Zd* V, double* partial, hipStream_t st);
 /// Layers [K][rows][R] to [R][K][rows] (each pair's K vectors contiguous, for a library GEMM).

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <thread>
 
+#include "lanczos.h"
 #include "tridiag.h"
 
 namespace qsa {
@@ -412,6 +413,7 @@ void BatchedLanczos::run_kept(int M, const std::vector<double>& breakdown_tol, c
     put(full_d, full, st);
     const bool any_full = std::any_of(full.begin(), full.end(), [](int f) { return f != 0; });
     DeviceBuffer coef(any_full ? nb * static_cast<std::size_t>(M + 1) * R_ * sizeof(dev::Zd) : 8);
+    DeviceBuffer gs2(pairs * sizeof(int));   // per pair: the second Gram-Schmidt pass of this step
     alpha_h_.reset(hist * 8);
     beta_h_.reset(hist * 8);
     dev::fill_zero(false, alpha_h_.get(), hist, st);
@@ -443,8 +445,14 @@ void BatchedLanczos::run_kept(int M, const std::vector<double>& breakdown_tol, c
         dev::batch_finish(B, 1, partial, ah, bh, ov, ca, cb, ib, tol, st, thr);
         dev::batch_vec(B, 2, nullptr, Vc, W, nullptr, ov, thr, partial, st, partial2);
         if (any_full) {
-            for (int pass = 0; pass < 2; ++pass)
-                dev::batch_gram_schmidt(B, basis, layer, j + 1, M + 1, W, full_d.get<int>(), coef.get<dev::Zd>(), st);
+            // CGS2 against every kept layer; the second pass only where the first removed more than
+            // 1 - kappa of ||W|| (DGKS, as the reference's FullCGS2 gates it): partial2 holds the
+            // norms before the first pass, partial those after it
+            dev::batch_gram_schmidt(B, basis, layer, j + 1, M + 1, W, full_d.get<int>(), coef.get<dev::Zd>(), st);
+            dev::batch_norm_partial(B, W, partial, st);
+            dev::batch_dgks_gate(B, partial2, partial, full_d.get<int>(), kDgksKappa, gs2.get<int>(), st);
+            dev::batch_gram_schmidt(B, basis, layer, j + 1, M + 1, W, full_d.get<int>(), coef.get<dev::Zd>(), st,
+                                    gs2.get<int>());
             dev::batch_norm_partial(B, W, partial2, st);
         }
         dev::batch_finish(B, 2, partial2, ah, bh, ov, ca, cb, ib, tol, st);

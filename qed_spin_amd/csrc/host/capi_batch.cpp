@@ -443,6 +443,7 @@ extern "C" int qedh_ftlm_dynamics_walk(int n_sources, const qedh_matvec* source_
             stats[2] = r.host_s;
             stats[3] = r.total_s;
             stats[4] = static_cast<double>(r.waves);
+            stats[5] = r.setup_s;
         }
     });
 }

@@ -141,10 +141,10 @@ DynWalkResult ftlm_dynamics_walk(const std::vector<std::shared_ptr<const SectorO
     const std::size_t nT = o.temperatures.size(), nW = o.omega.size();
     const std::size_t full_limit = std::max<std::size_t>(4 * static_cast<std::size_t>(M), kFullReorthMaxDim);
     std::uint64_t budget = o.budget_bytes;
-    if (!budget) {   // half of the device memory free now
+    if (!budget) {   // two fifths of the device's memory, at most nine tenths of what is free now
         std::size_t free_b = 0, total_b = 0;
         QSA_HIP(hipMemGetInfo(&free_b, &total_b));
-        budget = std::max<std::uint64_t>(free_b / 2, std::uint64_t{1} << 30);
+        budget = std::max<std::uint64_t>(std::min<std::uint64_t>(total_b / 5 * 2, free_b / 10 * 9), std::uint64_t{1} << 30);
     }
     std::vector<std::vector<int>> chans_of(nS);
     for (std::size_t c = 0; c < nC; ++c) chans_of[channels[c].source].push_back(static_cast<int>(c));
@@ -162,7 +162,7 @@ DynWalkResult ftlm_dynamics_walk(const std::vector<std::shared_ptr<const SectorO
     const int Rmax = std::min(o.samples, dev::kBatchMaxR);
     const hipStream_t st = engine_stream();
     // The kept vectors live in three device regions that every wave reuses (allocated once: kept
-    // vectors of tens of GB re-allocated per wave in slightly different sizes cost the pool more time
+    // vectors of tens of GB allocated per wave in slightly different sizes cost the pool more time
     // than the wave's work): (1) the source bases, later the probe images transposed; (2) the target
     // bases; (3) the probe images, later the target bases transposed. Per sample, with r_s the wave's
     // source rows and r_c its channel rows: 16 (M + 1) (max(r_s, r_c) + 2 r_c) bytes.
@@ -198,10 +198,20 @@ DynWalkResult ftlm_dynamics_walk(const std::vector<std::shared_ptr<const SectorO
         need2 = std::max(need2, rc * static_cast<std::uint64_t>(R));
         b0 = b1;
     }
-    DeviceBuffer region1(static_cast<std::size_t>(need1 * layer_bytes));
-    DeviceBuffer region2(static_cast<std::size_t>(need2 * layer_bytes));
-    DeviceBuffer region3(static_cast<std::size_t>(need2 * layer_bytes));
+    // One allocation holds the three regions. A walk that needs more than half of the budget takes
+    // the whole budget, so that consecutive large walks ask the stream-ordered pool for the same size
+    // and reuse the block it keeps: memory the pool has given back costs seconds to map again.
+    auto up256 = [](std::uint64_t b) { return (b + 255) / 256 * 256; };
+    const std::uint64_t bytes1 = up256(need1 * layer_bytes), bytes2 = up256(need2 * layer_bytes);
+    std::uint64_t arena_bytes = bytes1 + 2 * bytes2;
+    if (arena_bytes > budget / 2 && arena_bytes < budget) arena_bytes = budget;
+    DeviceBuffer arena(static_cast<std::size_t>(arena_bytes));
+    dev::Zd* const region1 = arena.get<dev::Zd>();
+    dev::Zd* const region2 = reinterpret_cast<dev::Zd*>(arena.get<char>() + bytes1);
+    dev::Zd* const region3 = reinterpret_cast<dev::Zd*>(arena.get<char>() + bytes1 + bytes2);
     DeviceBuffer partial, Omega;   // grown when a wave needs more, never shrunk
+    QSA_HIP(hipStreamSynchronize(st));
+    out.setup_s = since(t_all);
     auto ensure = [](DeviceBuffer& b, std::size_t bytes) {
         if (b.bytes() < bytes) b.reset(bytes);
     };
@@ -229,7 +239,7 @@ DynWalkResult ftlm_dynamics_walk(const std::vector<std::shared_ptr<const SectorO
             }
             S.sample_start(seeds);
             const std::uint64_t layer_s = S.rows() * static_cast<std::uint64_t>(R);
-            dev::Zd* const Vs = region1.get<dev::Zd>();
+            dev::Zd* const Vs = region1;
             std::vector<double> aS, bS;
             std::vector<int> stS;
             BatchedTimings ts;
@@ -265,7 +275,7 @@ DynWalkResult ftlm_dynamics_walk(const std::vector<std::shared_ptr<const SectorO
                 std::vector<int> live(nc * static_cast<std::size_t>(R));
                 for (std::size_t i = 0; i < live.size(); ++i) live[i] = std::sqrt(n2[i]) >= 1e-14 ? 1 : 0;
                 const std::uint64_t layer_c = T.rows() * static_cast<std::uint64_t>(R);
-                dev::Zd* const Wb = region2.get<dev::Zd>();
+                dev::Zd* const Wb = region2;
                 BatchedTimings tc;
                 T.run_kept(M, tol_c, full_c, reinterpret_cast<cplx*>(Wb), aC, bC, stC, tc, &live);
                 out.lanczos_s += since(t_lz);
@@ -274,12 +284,12 @@ DynWalkResult ftlm_dynamics_walk(const std::vector<std::shared_ptr<const SectorO
                 int mH = 0;
                 for (int v : stS) mH = std::max(mH, v);
                 // region 3 <- A (region 1); region 1 <- region 3 transposed; region 3 <- region 2 transposed
-                dev::Zd* const AV = region3.get<dev::Zd>();
+                dev::Zd* const AV = region3;
                 for (int j = 0; j < mH; ++j)
                     P.apply(Vs + static_cast<std::uint64_t>(j) * layer_s, AV + static_cast<std::uint64_t>(j) * layer_c);
-                dev::Zd* const AVt = region1.get<dev::Zd>();
+                dev::Zd* const AVt = region1;
                 dev::batch_layers_transpose(AV, T.rows(), R, std::max(mH, 1), AVt, st);
-                dev::Zd* const Wt = region3.get<dev::Zd>();
+                dev::Zd* const Wt = region3;
                 dev::batch_layers_transpose(Wb, T.rows(), R, M + 1, Wt, st);
                 QSA_HIP(hipGetLastError());
                 const std::size_t per = static_cast<std::size_t>(M) * static_cast<std::size_t>(M);

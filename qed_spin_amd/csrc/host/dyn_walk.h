@@ -16,11 +16,11 @@
 // packed probe applied to the sources' start vectors. The overlaps <A v_a|w_b> of every pair are
 // products of the kept bases laid out as column panels, split over chunks of 8192 rows so that a
 // pair of 1e6 rows fills the device as well as many small pairs do: every chunk of every pair in
-// one launch (64 x 64 tiles, LDS-staged slices, fused multiply-adds), then the chunks of a pair
-// summed in order. (For these shapes -- 40..200 vectors against 40..200, 1e2..1e6 rows -- the
+// one launch (64 x 64 tiles on the FP64 matrix cores, LDS-staged slices, the tiles of a chunk on one
+// XCD), then the chunks of a pair summed in order. (For these shapes -- 40..200 vectors against 40..200, 1e2..1e6 rows -- the
 // library's FP64 GEMM, batched per pair, measured slower.) Only the small tridiagonal algebra and
-// the Lorentzian sums run on the host. Waves are cut so that the kept vectors fit a device-memory budget (by default
-// half of the memory free when the walk starts).
+// the Lorentzian sums run on the host. Waves are cut so that the kept vectors fit a device-memory
+// budget (by default two fifths of the device's memory, at most nine tenths of what is free).
 #pragma once
 
 #include <memory>
@@ -42,7 +42,7 @@ struct DynWalkOptions {
     std::vector<double> breakdown_tol;           ///< per source; a channel's target runs use its source's
     std::vector<double> temperatures, omega;
     double eta = 0.05;
-    std::uint64_t budget_bytes = 0;              ///< device memory for one wave's kept vectors (0: half the free)
+    std::uint64_t budget_bytes = 0;              ///< device memory for the kept vectors (0: 2/5 of the device's, at most 9/10 of the free)
 };
 
 struct DynWalkResult {
@@ -50,7 +50,7 @@ struct DynWalkResult {
     std::vector<std::vector<cplx>> S;            ///< [channel][T nW + w], the same scaling
     std::vector<double> e_min;                   ///< [source] the samples' common reference
     std::vector<int> samples_done;               ///< [source]
-    double lanczos_s = 0.0, overlap_s = 0.0, host_s = 0.0, total_s = 0.0;
+    double lanczos_s = 0.0, overlap_s = 0.0, host_s = 0.0, total_s = 0.0, setup_s = 0.0;
     int waves = 0;
 };
 

@@ -204,7 +204,7 @@ def _dynamics_finite_T(H, probes, omega, T, eta, blocks, krylov, samples, seed, 
     S = np.zeros((max(nC, 1), nT, nW, 2))
     emin = np.zeros(nS)
     done = np.zeros(nS, np.int32)
-    stats = np.zeros(5)
+    stats = np.zeros(6)
     _lib.call("qedh_ftlm_dynamics_walk", nS, sh, _lib.ptr(seed_a, C.c_uint64), _lib.ptr(tol_a, C.c_double), nC,
               cs.ctypes.data_as(C.POINTER(C.c_int)), ch, ca, int(samples), int(krylov), nT, _lib.ptr(T, C.c_double), nW,
               _lib.ptr(omega, C.c_double), C.c_double(eta), C.c_uint64(0), _lib.ptr(Z, C.c_double),
@@ -224,7 +224,7 @@ def _dynamics_finite_T(H, probes, omega, T, eta, blocks, krylov, samples, seed, 
                     Sres[it] += w[chan_src[c]] * (S[c, it, :, 0] + 1j * S[c, it, :, 1])
             Sres[it] /= Zp
         out.append(Sres)
-    engine = dict(zip(["lanczos_s", "overlap_s", "host_s", "total_s", "waves"], stats.tolist()))
+    engine = dict(zip(["lanczos_s", "overlap_s", "host_s", "total_s", "waves", "setup_s"], stats.tolist()))
     engine.update(probes_s=t_probes, channels=nC)
     return out, ref, engine
 
