@@ -570,15 +570,17 @@ def main():
         cpu = cpu_baseline(m, D, w_id)
         log(f"cpu baseline: {cpu}")
 
-    # HBM bytes per launch of this SpMV from the committed PMC summary (FETCH_SIZE and WRITE_SIZE,
-    # each scaled by the same counter of a 1 GiB copy), valid for the lanes-per-row it was taken at
-    traffic = None
+    # HBM bytes per launch of this SpMV from the committed PMC summary (read requests by size and
+    # WRITE_SIZE, calibrated on a 1 GiB copy), valid for the layout it was taken on (lanes per row:
+    # 1 is SELL-64); not measured in this run
+    traffic, traffic_from = None, None
     tf = os.path.join(ROOT, "profiles", "spmv_n32_traffic.json")
     if os.path.exists(tf):
         with open(tf) as f:
             rec = json.load(f)
         if int(rec.get("lanes_per_row", -1)) == int(info["lanes"]):
             traffic = rec.get("hbm_bytes_per_launch")
+            traffic_from = rec.get("profile")
 
     out = {
         "metric": "FP64 sector matvec GB/s (% HBM peak)",
@@ -612,6 +614,7 @@ def main():
             "unit": "GB/s",
             "frac": round(gbs / HBM_PEAK_GBS, 4),
             "traffic": traffic,
+            "traffic_from": traffic_from,
         },
         "multi_vector": multi,
         "lanczos_to_E0": lanczos,
