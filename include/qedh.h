@@ -167,8 +167,30 @@ int qedh_sector_import(int n_sites, int n_up, int group_size, const int* perms, 
 int qedh_sublattice_tables(int n_sites, int group_size, const int* perms, const uint64_t* flips, int sublattice_mode,
                            int relaxed, int* block, int* key_bit, uint16_t* lead, uint32_t* cand_off, uint16_t* cand,
                            int64_t cand_cap, int64_t* n_cand);
-/* dim, orbits before the character filter, sublattice block size (0: plain order), build seconds. */
+/* The sector of an irrep of dimension irrep_dim of the group (the reference's
+ * group_sector_irrep_from_table, src/engine/group_sector.cpp:78-139, with the d > 1 fields of
+ * include/ed/basis/rep_sector.h:198-215): D holds D(g) for every element, group_size x d x d complex
+ * (re, im) row-major, a unitary representation. Every orbit is enumerated (least member, plain or
+ * key order), each representative's stabiliser S read on the device; per stabiliser class C spans
+ * the range of sum_{s in S} conj(D(s)) (rank(C) states per representative, C^dag M C = I), and the
+ * representatives of rank 0 are dropped. The sector's basis states are the partner-0 states
+ * |r; a> = sqrt(d/|G|) sum_j C_r[j][a] sum_g conj(D(g)[0][j]) |g r>, a < rank(r); its dim counts
+ * them. irrep_dim 1: D holds the characters (qedh_sector_create). d above 4: DeviceUnsupported. */
+int qedh_sector_create_irrep(int n_sites, int n_up, int group_size, const int* perms, const uint64_t* flips,
+                             int irrep_dim, const double* D, int sublattice_mode, int relaxed,
+                             uint64_t rank_budget_bytes, qedh_sector* out);
+/* d (1 for a sector of characters), the representatives, the stabiliser classes (0 for d = 1). */
+int qedh_sector_irrep_info(qedh_sector s, int* irrep_dim, int64_t* n_reps, int64_t* n_classes);
+/* A d > 1 sector's fields: rep_class[n_reps], state_offset[n_reps + 1] (the index of each
+ * representative's first state), class_rank[n_classes], class_C[n_classes][d][d] complex (re, im)
+ * row-major (columns from the rank on zero); each may be NULL. d = 1: InvalidRequest. */
+int qedh_sector_irrep_export(qedh_sector s, uint16_t* rep_class, uint64_t* state_offset, uint8_t* class_rank,
+                             double* class_C);
+/* dim (basis states), orbits before the character filter, sublattice block size (0: plain order),
+ * build seconds. */
 int qedh_sector_info(qedh_sector s, int64_t* dim, int64_t* n_orbits, int* sublattice_block, double* build_seconds);
+/* The representatives (n_reps of qedh_sector_irrep_info; dim for d = 1) and their 1/norms (1 for a
+ * d > 1 sector, whose normalisation is in C). */
 int qedh_sector_export(qedh_sector s, uint64_t* reps, double* inv_norms);
 /* key_bit[n_sites] of the sublattice order; returns the block size through *block (0: none). */
 int qedh_sector_key_bits(qedh_sector s, int* key_bit, int* block);

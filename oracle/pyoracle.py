@@ -321,3 +321,53 @@ def sector_basis_matrix(sector: OracleSector, states):
                                     sector.N, int(s))
             E[index[int(t)], r] += np.conj(chi[g]) * sector.inv_norms[r]
     return E / np.sqrt(sector.G)
+
+
+def irrep_fields(N, n_up, perms, D, flips=None):
+    """A sector of an irrep of dimension d > 1 by brute force, from the definitions of the
+    reference's d > 1 sectors (include/ed/basis/rep_sector.h:198-215, src/engine/group_sector.cpp:
+    78-139): every orbit's least member r, its stabiliser S = {g : g r = r}, the projector
+    P = (1/|S|) sum_{s in S} conj(D(s)), C_r = an orthonormal basis of P's range (numpy's eigh,
+    eigenvalue 1) over sqrt(|S|); members of rank 0 dropped. Returns (reps, [C_r as d x rank])."""
+    perms = np.ascontiguousarray(perms, np.int32)
+    G = perms.shape[0]
+    D = np.asarray(D, np.complex128)
+    fl = np.zeros(G, np.uint64) if flips is None else np.asarray(flips, np.uint64)
+    triv = OracleSector(N, n_up, perms, np.ones(G), fl)
+    L = lib()
+    reps, Cs = [], []
+    for r in triv.reps:
+        S = [g for g in range(G)
+             if L.orc_apply_element(_p(np.ascontiguousarray(perms[g]), C.c_int), int(fl[g]), N, int(r)) == int(r)]
+        P = np.conj(D[S]).sum(axis=0) / len(S)
+        w, V = np.linalg.eigh(0.5 * (P + P.conj().T))
+        cols = V[:, w > 0.5]
+        if cols.shape[1] == 0:
+            continue
+        reps.append(int(r))
+        Cs.append(cols / np.sqrt(len(S)))
+    return np.array(reps, np.uint64), Cs
+
+
+def irrep_basis_matrix(N, perms, D, reps, Cs, states, flips=None):
+    """E (len(states) x states of the sector): the columns |r; a> = sqrt(d/|G|) sum_j C_r[j][a]
+    sum_g conj(D(g)[0][j]) |g r>, representative by representative, a < rank(r)."""
+    perms = np.ascontiguousarray(perms, np.int32)
+    G = perms.shape[0]
+    D = np.asarray(D, np.complex128)
+    d = D.shape[1]
+    fl = np.zeros(G, np.uint64) if flips is None else np.asarray(flips, np.uint64)
+    index = {int(s): i for i, s in enumerate(states)}
+    n = sum(Cr.shape[1] for Cr in Cs)
+    E = np.zeros((len(states), n), np.complex128)
+    L = lib()
+    col = 0
+    for r, Cr in zip(reps, Cs):
+        imgs = [index[int(L.orc_apply_element(_p(np.ascontiguousarray(perms[g]), C.c_int), int(fl[g]), N, int(r)))]
+                for g in range(G)]
+        for a in range(Cr.shape[1]):
+            coef = np.conj(D[:, 0, :]) @ Cr[:, a]   # per element: sum_j conj(D(g)[0][j]) C_r[j][a]
+            for g in range(G):
+                E[imgs[g], col] += coef[g]
+            col += 1
+    return E * np.sqrt(d / G)

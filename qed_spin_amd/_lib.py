@@ -96,6 +96,12 @@ _SIGNATURES = {
         C.c_int, C.c_int, c_int_p, c_u64_p, C.c_int, C.c_int, c_int_p, c_int_p, C.c_void_p, c_u32_p, C.c_void_p,
         C.c_int64, c_i64_p,
     ],
+    "qedh_sector_create_irrep": [
+        C.c_int, C.c_int, C.c_int, c_int_p, c_u64_p, C.c_int, c_dbl_p, C.c_int, C.c_int, C.c_uint64,
+        C.POINTER(C.c_void_p),
+    ],
+    "qedh_sector_irrep_info": [C.c_void_p, c_int_p, c_i64_p, c_i64_p],
+    "qedh_sector_irrep_export": [C.c_void_p, C.c_void_p, c_u64_p, C.c_void_p, c_dbl_p],
     "qedh_sector_info": [C.c_void_p, c_i64_p, c_i64_p, c_int_p, c_dbl_p],
     "qedh_sector_export": [C.c_void_p, c_u64_p, c_dbl_p],
     "qedh_sector_key_bits": [C.c_void_p, c_int_p, c_int_p],

@@ -40,6 +40,22 @@ struct SectorView {
     const u64* binom = nullptr;      // 65 x 65 Pascal table, binom[n * 65 + k] = C(n, k)
 };
 
+/// An irrep of dimension d > 1 on a sector (the reference's rep_sector.h d > 1 fields): D(g) per
+/// element, and per representative its stabiliser class -- the class's rank (the representative's
+/// number of states, at most d) and its d x d matrix C (columns from the rank on zero) -- and the
+/// index of its first state. d = 1 leaves every pointer null.
+struct IrrepView {
+    int d = 1;
+    const Zd* D = nullptr;          // G x d x d, row-major
+    const Zd* C = nullptr;          // per class: d x d, row-major
+    const u8* class_rank = nullptr; // per class
+    const u16* rep_class = nullptr; // per representative
+    const u64* offset = nullptr;    // per representative, and one past the last
+};
+
+/// The largest irrep dimension the device kernels hold in registers.
+constexpr int kMaxIrrepDim = 4;
+
 /// A compiled program (program.h layout) in device memory.
 struct ProgramView {
     u32 n_groups = 0;

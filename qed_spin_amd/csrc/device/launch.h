@@ -69,6 +69,23 @@ void csr_compact(i64 rows, const u64* row_ptr, const u32* stage_col, const Zd* s
 std::size_t row_scan_temp_bytes(i64 rows);
 void row_scan(const u32* row_len, i64 rows, u64* row_ptr, void* temp, std::size_t temp_bytes, hipStream_t st);
 
+// ---- irrep.hip -------------------------------------------------------------------------------
+/// masks[r * words + w] bit b: element 64 w + b fixes representative r (words = ceil(G / 64)).
+void stab_masks(const SectorView& S, int words, u64* masks, hipStream_t st);
+/// The operator between two sectors of one irrep of dimension d (2..kMaxIrrepDim), representative by
+/// representative (the reference's for_each_cross_block, include/ed/matvec/sector_rows.h:72-106):
+/// with val null, row_len[state] = the entries of each of a representative's rows (a target reached
+/// several times gives several entries); else the entries into the rows' CSR slots (row_ptr from
+/// the scan of row_len). Returns false (nothing launched) for an unsupported d.
+bool irrep_rows(const SectorView& R, const IrrepView& IR, const SectorView& C, const IrrepView& IC, bool same,
+                const ProgramView& P, u32* row_len, const u64* row_ptr, u32* col, Zd* val, hipStream_t st);
+/// Each row's entries sorted by column, equal columns summed in their order, exact zeros dropped, in
+/// place; the rows' new lengths into row_len.
+void csr_merge_rows(i64 rows, const u64* row_ptr, u32* col, Zd* val, u32* row_len, hipStream_t st);
+/// The first new_ptr[r + 1] - new_ptr[r] entries of every row moved from old_ptr[r] to new_ptr[r].
+void csr_move_rows(i64 rows, const u64* old_ptr, const u64* new_ptr, const u32* col, const Zd* val, u32* ncol, Zd* nval,
+                   hipStream_t st);
+
 // ---- spmv.hip --------------------------------------------------------------------------------
 struct CsrView {
     i64 rows = 0;

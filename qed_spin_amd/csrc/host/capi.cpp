@@ -290,6 +290,55 @@ int qedh_sector_create(int n_sites, int n_up, int group_size, const int* perms, 
         *out = new qedh_sector_s{std::make_shared<qsa::DeviceSector>(std::move(g), rank_budget_bytes)};
     });
 }
+int qedh_sector_create_irrep(int n_sites, int n_up, int group_size, const int* perms, const uint64_t* flips,
+                             int irrep_dim, const double* D, int sublattice_mode, int relaxed,
+                             uint64_t rank_budget_bytes, qedh_sector* out) {
+    return guarded([&] {
+        need(perms, "perms");
+        need(D, "D");
+        need(out, "out");
+        if (n_sites < 1 || n_sites > 63 || group_size < 1 || irrep_dim < 1)
+            throw qsa::Error(qsa::kInvalid, "sector: bad sizes");
+        qsa::GroupSpec g;
+        g.n_sites = n_sites;
+        g.n_up = n_up;
+        g.G = group_size;
+        g.perms.assign(perms, perms + static_cast<std::size_t>(group_size) * n_sites);
+        if (flips) g.flips.assign(flips, flips + group_size);
+        else g.flips.assign(group_size, 0ULL);
+        g.slc = qsa::build_sublattice(g.perms, g.flips, group_size, n_sites, sublattice_mode, relaxed != 0);
+        const std::size_t n = static_cast<std::size_t>(group_size) * irrep_dim * irrep_dim;
+        std::vector<qsa::cplx> Dm(n);
+        for (std::size_t i = 0; i < n; ++i) Dm[i] = qsa::cplx(D[2 * i], D[2 * i + 1]);
+        if (irrep_dim == 1) {   // a one-dimensional irrep: its characters
+            g.chars = std::move(Dm);
+            *out = new qedh_sector_s{std::make_shared<qsa::DeviceSector>(std::move(g), rank_budget_bytes)};
+            return;
+        }
+        *out = new qedh_sector_s{qsa::DeviceSector::make_irrep(std::move(g), irrep_dim, Dm, rank_budget_bytes)};
+    });
+}
+int qedh_sector_irrep_info(qedh_sector s, int* irrep_dim, int64_t* n_reps, int64_t* n_classes) {
+    return guarded([&] {
+        need(s, "sector");
+        const auto& ir = s->sector->irrep();
+        if (irrep_dim) *irrep_dim = ir.d;
+        if (n_reps) *n_reps = s->sector->n_reps();
+        if (n_classes) *n_classes = static_cast<int64_t>(ir.class_rank.size());
+    });
+}
+int qedh_sector_irrep_export(qedh_sector s, uint16_t* rep_class, uint64_t* state_offset, uint8_t* class_rank,
+                             double* class_C) {
+    return guarded([&] {
+        need(s, "sector");
+        const auto& ir = s->sector->irrep();
+        if (ir.d <= 1) throw qsa::Error(qsa::kInvalid, "irrep export: the sector's irrep is one-dimensional");
+        if (rep_class) std::memcpy(rep_class, ir.rep_class.data(), ir.rep_class.size() * 2);
+        if (state_offset) std::memcpy(state_offset, ir.state_offset.data(), ir.state_offset.size() * 8);
+        if (class_rank) std::memcpy(class_rank, ir.class_rank.data(), ir.class_rank.size());
+        if (class_C) std::memcpy(class_C, ir.class_C.data(), ir.class_C.size() * sizeof(qsa::cplx));
+    });
+}
 int qedh_sector_import(int n_sites, int n_up, int group_size, const int* perms, const uint64_t* flips,
                        const double* chars, int sublattice_mode, int relaxed, int64_t n_reps, const uint64_t* reps,
                        const double* inv_norms, uint64_t rank_budget_bytes, qedh_sector* out) {
@@ -624,6 +673,8 @@ extern "C" int qedh_sector_index_and_projection(qedh_sector s, const uint64_t* s
                                                 double* proj) {
     return guarded([&] {
         need(s, "sector");
+        if (s->sector->irrep_dim() > 1)
+            throw qsa::Error(qsa::kDeviceUnsupported, "index_and_projection: the sector's irrep has dimension > 1");
         if (n <= 0) return;
         auto st = qsa::upload(states, static_cast<std::size_t>(n));
         qsa::DeviceBuffer di(static_cast<std::size_t>(n) * 8), dp(static_cast<std::size_t>(n) * 16);
@@ -641,6 +692,8 @@ extern "C" int qedh_sector_expand(qedh_sector s, const void* u, int nvec, int fu
         need(u, "u");
         need(out, "out");
         if (nvec < 1) throw qsa::Error(qsa::kInvalid, "expand: nvec must be >= 1");
+        if (s->sector->irrep_dim() > 1)
+            throw qsa::Error(qsa::kDeviceUnsupported, "expand: sectors of irreps of dimension > 1 are not expanded");
         const qsa::dev::SectorView S = s->sector->view();
         const int N = S.n_sites;
         if (full && N > 34)

@@ -1,5 +1,8 @@
 #include "sector.h"
 
+#include <array>
+#include <cmath>
+#include <map>
 #include <mutex>
 
 #include <chrono>
@@ -239,6 +242,173 @@ std::vector<double> DeviceSector::inv_norms() const {
     std::vector<double> h(static_cast<std::size_t>(dim_));
     if (dim_) copy_to_host(h.data(), inv_norms_.get(), h.size() * 8);
     return h;
+}
+
+namespace {
+
+/// An orthonormal basis of the range of the Hermitian projector P (d x d, row-major): Gram-Schmidt
+/// (twice) over its columns in order, a column kept when a part of norm above 1e-6 remains. The
+/// basis vectors are the columns of U (d x d, row-major; columns from the rank on zero).
+int projector_range(const std::vector<cplx>& P, int d, std::vector<cplx>& U) {
+    const std::size_t dd = static_cast<std::size_t>(d) * static_cast<std::size_t>(d);
+    U.assign(dd, cplx(0.0, 0.0));
+    int rank = 0;
+    std::vector<cplx> v(static_cast<std::size_t>(d));
+    for (int j = 0; j < d; ++j) {
+        for (int i = 0; i < d; ++i) v[i] = P[static_cast<std::size_t>(i) * d + j];
+        for (int pass = 0; pass < 2; ++pass)
+            for (int k = 0; k < rank; ++k) {
+                cplx c(0.0, 0.0);
+                for (int i = 0; i < d; ++i) c += std::conj(U[static_cast<std::size_t>(i) * d + k]) * v[i];
+                for (int i = 0; i < d; ++i) v[i] -= c * U[static_cast<std::size_t>(i) * d + k];
+            }
+        double n2 = 0.0;
+        for (int i = 0; i < d; ++i) n2 += std::norm(v[i]);
+        if (!(n2 > 1e-12)) continue;
+        const double inv = 1.0 / std::sqrt(n2);
+        for (int i = 0; i < d; ++i) U[static_cast<std::size_t>(i) * d + rank] = v[i] * inv;
+        ++rank;
+    }
+    return rank;
+}
+
+}  // namespace
+
+std::shared_ptr<DeviceSector> DeviceSector::make_irrep(GroupSpec spec, int d, const std::vector<cplx>& D,
+                                                       std::uint64_t rank_budget_bytes) {
+    const int G = spec.G;
+    if (d < 2) throw Error(kInvalid, "irrep sector: an irrep of dimension at least 2 (use the characters for d = 1)");
+    if (d > dev::kMaxIrrepDim)
+        throw Error(kDeviceUnsupported, "irrep sector: irreps up to dimension " + std::to_string(dev::kMaxIrrepDim) +
+                                            " (the device kernels hold a block in registers)");
+    if (G < 1 || G > 256) throw Error(kDeviceUnsupported, "irrep sector: groups of 1 to 256 elements");
+    const std::size_t dd = static_cast<std::size_t>(d) * static_cast<std::size_t>(d);
+    if (D.size() != static_cast<std::size_t>(G) * dd) throw Error(kInvalid, "irrep sector: D needs G x d x d entries");
+    for (const cplx& z : D)
+        if (!std::isfinite(z.real()) || !std::isfinite(z.imag())) throw Error(kInvalid, "irrep sector: D is not finite");
+    spec.chars.assign(static_cast<std::size_t>(G), cplx(0.0, 0.0));   // the characters label the sector
+    for (int g = 0; g < G; ++g)
+        for (int i = 0; i < d; ++i) spec.chars[g] += D[static_cast<std::size_t>(g) * dd + static_cast<std::size_t>(i) * (d + 1)];
+    // every orbit's representative: the trivial characters keep them all
+    GroupSpec trivial = spec;
+    trivial.chars.assign(static_cast<std::size_t>(G), cplx(1.0, 0.0));
+    std::vector<std::uint64_t> reps;
+    std::vector<std::uint64_t> masks;
+    const int words = (G + 63) / 64;
+    std::int64_t n = 0, orbits = 0;
+    {
+        const DeviceSector all(std::move(trivial), 0);
+        n = all.dim_;
+        orbits = all.stats_.orbits;
+        reps = all.reps();
+        masks.assign(static_cast<std::size_t>(n) * words, 0);
+        if (n > 0) {
+            DeviceBuffer m(static_cast<std::size_t>(n) * words * 8);
+            dev::stab_masks(all.view(), words, m.get<std::uint64_t>(), engine_stream());
+            QSA_HIP(hipGetLastError());
+            copy_to_host(masks.data(), m.get(), masks.size() * 8);
+        }
+    }
+    // stabiliser classes: P = (1 / |S|) sum_{s in S} conj(D(s)) is a projector; C = an orthonormal
+    // basis of its range over sqrt(|S|), so C^dag (|S| P) C = I
+    IrrepData ir;
+    ir.d = d;
+    ir.D = D;
+    std::map<std::array<std::uint64_t, 4>, int> class_of;
+    std::vector<int> cls(static_cast<std::size_t>(n));
+    std::array<std::uint64_t, 4> last{};
+    int last_class = -1;
+    for (std::int64_t r = 0; r < n; ++r) {
+        std::array<std::uint64_t, 4> key{};
+        for (int w = 0; w < words; ++w) key[w] = masks[static_cast<std::size_t>(r) * words + w];
+        if (last_class >= 0 && key == last) {
+            cls[r] = last_class;
+            continue;
+        }
+        auto it = class_of.find(key);
+        int c;
+        if (it != class_of.end()) {
+            c = it->second;
+        } else {
+            c = static_cast<int>(class_of.size());
+            if (c > 65535) throw Error(kDeviceUnsupported, "irrep sector: more than 65536 stabiliser classes");
+            class_of.emplace(key, c);
+            std::vector<cplx> P(dd, cplx(0.0, 0.0));
+            int size = 0;
+            for (int g = 0; g < G; ++g)
+                if ((key[g / 64] >> (g % 64)) & 1ULL) {
+                    ++size;
+                    for (std::size_t e = 0; e < dd; ++e) P[e] += std::conj(D[static_cast<std::size_t>(g) * dd + e]);
+                }
+            if (size == 0) throw Error(kRuntime, "irrep sector: a representative with an empty stabiliser");
+            std::vector<cplx> H(dd);
+            for (int i = 0; i < d; ++i)
+                for (int j = 0; j < d; ++j)
+                    H[static_cast<std::size_t>(i) * d + j] =
+                        0.5 * (P[static_cast<std::size_t>(i) * d + j] + std::conj(P[static_cast<std::size_t>(j) * d + i])) /
+                        static_cast<double>(size);
+            std::vector<cplx> U;
+            const int rank = projector_range(H, d, U);
+            double tr = 0.0;
+            for (int i = 0; i < d; ++i) tr += H[static_cast<std::size_t>(i) * (d + 1)].real();
+            if (std::abs(tr - rank) > 1e-6)
+                throw Error(kInvalid, "irrep sector: D is not a unitary representation of the group (a stabiliser's "
+                                      "projector has trace " + std::to_string(tr) + " and rank " + std::to_string(rank) + ")");
+            const double sc = 1.0 / std::sqrt(static_cast<double>(size));
+            for (cplx& z : U) z *= sc;
+            ir.class_C.insert(ir.class_C.end(), U.begin(), U.end());
+            ir.class_rank.push_back(static_cast<std::uint8_t>(rank));
+        }
+        cls[r] = c;
+        last = key;
+        last_class = c;
+    }
+    // the representatives that hold states of the irrep
+    std::vector<std::uint64_t> keep;
+    ir.state_offset.assign(1, 0);
+    for (std::int64_t r = 0; r < n; ++r) {
+        const int rank = ir.class_rank[static_cast<std::size_t>(cls[r])];
+        if (rank == 0) continue;
+        keep.push_back(reps[static_cast<std::size_t>(r)]);
+        ir.rep_class.push_back(static_cast<std::uint16_t>(cls[r]));
+        ir.state_offset.push_back(ir.state_offset.back() + static_cast<std::uint64_t>(rank));
+    }
+    const std::vector<double> ones(keep.size(), 1.0);
+    auto out = std::make_shared<DeviceSector>(std::move(spec), keep.data(), ones.data(),
+                                              static_cast<std::int64_t>(keep.size()), rank_budget_bytes);
+    out->stats_.orbits = orbits;
+    out->attach_irrep(std::move(ir));
+    return out;
+}
+
+void DeviceSector::attach_irrep(IrrepData ir) {
+    const hipStream_t st = engine_stream();
+    auto put = [&](DeviceBuffer& b, const void* h, std::size_t bytes) {
+        b.reset(std::max<std::size_t>(bytes, 8));
+        if (bytes) QSA_HIP(hipMemcpyAsync(b.get(), h, bytes, hipMemcpyHostToDevice, st));
+    };
+    put(irrep_D_, ir.D.data(), ir.D.size() * sizeof(cplx));
+    put(class_C_, ir.class_C.data(), ir.class_C.size() * sizeof(cplx));
+    put(class_rank_, ir.class_rank.data(), ir.class_rank.size());
+    put(rep_class_, ir.rep_class.data(), ir.rep_class.size() * 2);
+    put(state_offset_, ir.state_offset.data(), ir.state_offset.size() * 8);
+    QSA_HIP(hipStreamSynchronize(st));
+    states_ = static_cast<std::int64_t>(ir.state_offset.back());
+    stats_.device_bytes += irrep_D_.bytes() + class_C_.bytes() + class_rank_.bytes() + rep_class_.bytes() +
+                           state_offset_.bytes();
+    irrep_ = std::move(ir);
+}
+
+dev::IrrepView DeviceSector::irrep_view() const {
+    dev::IrrepView v;
+    if (irrep_.d <= 1) return v;
+    v.d = irrep_.d;
+    v.D = irrep_D_.get<dev::Zd>();
+    v.C = class_C_.get<dev::Zd>();
+    v.class_rank = class_rank_.get<std::uint8_t>();
+    v.rep_class = rep_class_.get<std::uint16_t>();
+    v.offset = state_offset_.get<std::uint64_t>();
+    return v;
 }
 
 DeviceProgram::DeviceProgram(const Program& p) {

@@ -19,8 +19,29 @@ struct SectorBuildStats {
     bool rank_table = false;
 };
 
+/// An irrep of dimension d > 1 on a sector (the reference's rep_sector.h d > 1 fields): D(g) per
+/// element; per stabiliser class its rank and d x d matrix C (C^dag M C = I on the range of
+/// M = sum over the stabiliser of conj(D(s)), columns from the rank on zero); per representative its
+/// class and the index of its first state. d = 1: the vectors stay empty.
+struct IrrepData {
+    int d = 1;
+    std::vector<cplx> D;                      ///< G x d x d, row-major
+    std::vector<cplx> class_C;                ///< per class: d x d, row-major
+    std::vector<std::uint8_t> class_rank;     ///< per class
+    std::vector<std::uint16_t> rep_class;     ///< per representative
+    std::vector<std::uint64_t> state_offset;  ///< per representative, and one past the last
+};
+
 class DeviceSector {
 public:
+    /// The sector of an irrep of dimension d (2 .. the device kernels' limit) of spec's group:
+    /// D holds D(g) for every element (G x d x d, row-major, a unitary representation); spec's
+    /// characters become the traces of D (the reference's group_sector_irrep_from_table,
+    /// src/engine/group_sector.cpp:78-139). Every orbit is enumerated, each representative's
+    /// stabiliser read on the device, and the representatives whose class has rank 0 dropped.
+    static std::shared_ptr<DeviceSector> make_irrep(GroupSpec spec, int d, const std::vector<cplx>& D,
+                                                    std::uint64_t rank_budget_bytes);
+
     /// Enumerate the sector of `spec` on the device. `rank_budget_bytes` caps the dense rank ->
     /// index table (C(N, n_up) x 4 bytes); above it lookups binary-search the representatives.
     DeviceSector(GroupSpec spec, std::uint64_t rank_budget_bytes);
@@ -35,7 +56,13 @@ public:
     DeviceSector& operator=(const DeviceSector&) = delete;
 
     const GroupSpec& spec() const { return spec_; }
-    std::int64_t dim() const { return dim_; }
+    /// Basis states: the representatives (d = 1), or the sum of their ranks (d > 1).
+    std::int64_t dim() const { return irrep_.d > 1 ? states_ : dim_; }
+    /// Representatives.
+    std::int64_t n_reps() const { return dim_; }
+    int irrep_dim() const { return irrep_.d; }
+    const IrrepData& irrep() const { return irrep_; }
+    dev::IrrepView irrep_view() const;
     const SectorBuildStats& stats() const { return stats_; }
     dev::SectorView view() const;
     bool real_characters() const { return spec_.real_characters(); }
@@ -45,6 +72,7 @@ public:
     std::vector<double> inv_norms() const;
 
 private:
+    void attach_irrep(IrrepData ir);
     void setup_tables();
     void build_rank_table(std::uint64_t total, std::uint64_t rank_budget_bytes);
     GroupSpec spec_;
@@ -52,6 +80,9 @@ private:
     SectorBuildStats stats_;
     DeviceBuffer lut_, flips_, chi_re_, chi_im_, to_key_, lead_, cand_off_, cand_, binom_;
     DeviceBuffer reps_, inv_norms_, rank_table_;
+    IrrepData irrep_;
+    std::int64_t states_ = 0;
+    DeviceBuffer irrep_D_, class_C_, class_rank_, rep_class_, state_offset_;
 };
 
 /// A compiled program in device memory.

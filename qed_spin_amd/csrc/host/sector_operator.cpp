@@ -20,8 +20,20 @@ SectorOperator::SectorOperator(std::shared_ptr<const DeviceSector> rows, std::sh
         throw Error(kInvalid, "SectorOperator: the two sectors come from different groups");
     if (a.slc.engaged() != b.slc.engaged() || a.slc.key_bit != b.slc.key_bit)
         throw Error(kInvalid, "SectorOperator: the two sectors use different representative orders");
+    if (rows_->irrep_dim() != cols_->irrep_dim())
+        throw Error(kInvalid, "SectorOperator: the two sectors carry irreps of different dimensions");
     real_ = prog_->real && rows_->real_characters() && cols_->real_characters();
     lane_ = kLaneGather;
+    if (rows_->irrep_dim() > 1) {
+        // a representative's states mix under the group: only the CSR lane, complex values
+        if (lane == kLaneGather)
+            throw Error(kDeviceUnsupported, "SectorOperator: sectors of irreps of dimension > 1 take the CSR lane");
+        real_ = false;
+        if (dim() > 0) build_csr_irrep(csr_budget_bytes);
+        if (lane_ != kLaneCsr && dim() > 0)
+            throw Error(kResourceLimit, "SectorOperator: the CSR of an irrep sector does not fit its budget");
+        return;
+    }
     if (lane != kLaneGather && dim() > 0) build_csr(csr_budget_bytes);
     if (lane == kLaneCsr && lane_ != kLaneCsr)
         throw Error(kResourceLimit, "SectorOperator: the CSR lane was requested but does not fit its budget");
@@ -151,6 +163,71 @@ void SectorOperator::build_csr(std::uint64_t budget) {
     // About four entries per lane (measured on the N = 32 ring). Block applies use the same width:
     // the lanes set the summation order of a row, so a block apply then equals single applies bit
     // for bit.
+    int L = 4;
+    while (L < 64 && L < mean / 4.0) L *= 2;
+    csr_.lanes = L;
+    lanes_block_ = L;
+    lane_ = kLaneCsr;
+    if (tune_ && rows >= (std::int64_t{1} << 18)) {
+        build_sell();
+        autotune();
+    }
+    csr_.build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void SectorOperator::build_csr_irrep(std::uint64_t budget) {
+    const auto t0 = std::chrono::steady_clock::now();
+    const hipStream_t st = engine_stream();
+    const std::int64_t rows = dim();
+    const dev::SectorView R = rows_->view(), C = cols_->view();
+    const dev::IrrepView IR = rows_->irrep_view(), IC = cols_->irrep_view();
+    const dev::ProgramView P = prog_->view();
+    DeviceBuffer row_len(static_cast<std::size_t>(rows) * 4);
+    if (!dev::irrep_rows(R, IR, C, IC, same(), P, row_len.get<std::uint32_t>(), nullptr, nullptr, nullptr, st))
+        throw Error(kDeviceUnsupported, "SectorOperator: an irrep dimension the device kernels do not hold");
+    QSA_HIP(hipGetLastError());
+    row_ptr_.reset(static_cast<std::size_t>(rows + 1) * 8);
+    const std::size_t tb = dev::row_scan_temp_bytes(rows);
+    DeviceBuffer temp(tb ? tb : 8);
+    dev::row_scan(row_len.get<std::uint32_t>(), rows, row_ptr_.get<std::uint64_t>(), temp.get(), tb, st);
+    QSA_HIP(hipGetLastError());
+    std::uint64_t nnz = 0;
+    QSA_HIP(hipMemcpyAsync(&nnz, row_ptr_.get<std::uint64_t>() + rows, 8, hipMemcpyDeviceToHost, st));
+    QSA_HIP(hipStreamSynchronize(st));
+    const std::uint64_t bytes = static_cast<std::uint64_t>(rows + 1) * 8 + nnz * (4 + sizeof(dev::Zd));
+    if (nnz > 0xFFFFFFFFFFULL || bytes > budget) {
+        row_ptr_.release();
+        return;
+    }
+    col_.reset(std::max<std::uint64_t>(nnz, 1) * 4);
+    vals_.reset(std::max<std::uint64_t>(nnz, 1) * sizeof(dev::Zd));
+    dev::irrep_rows(R, IR, C, IC, same(), P, nullptr, row_ptr_.get<std::uint64_t>(), col_.get<std::uint32_t>(),
+                    vals_.get<dev::Zd>(), st);
+    QSA_HIP(hipGetLastError());
+    // merged: a column once per row (the blocks of one target reached by several connections summed
+    // in walk order), exact zeros dropped
+    dev::csr_merge_rows(rows, row_ptr_.get<std::uint64_t>(), col_.get<std::uint32_t>(), vals_.get<dev::Zd>(),
+                        row_len.get<std::uint32_t>(), st);
+    QSA_HIP(hipGetLastError());
+    DeviceBuffer merged_ptr(static_cast<std::size_t>(rows + 1) * 8);
+    dev::row_scan(row_len.get<std::uint32_t>(), rows, merged_ptr.get<std::uint64_t>(), temp.get(), tb, st);
+    QSA_HIP(hipGetLastError());
+    QSA_HIP(hipMemcpyAsync(&nnz, merged_ptr.get<std::uint64_t>() + rows, 8, hipMemcpyDeviceToHost, st));
+    QSA_HIP(hipStreamSynchronize(st));
+    DeviceBuffer mcol(std::max<std::uint64_t>(nnz, 1) * 4), mval(std::max<std::uint64_t>(nnz, 1) * sizeof(dev::Zd));
+    dev::csr_move_rows(rows, row_ptr_.get<std::uint64_t>(), merged_ptr.get<std::uint64_t>(), col_.get<std::uint32_t>(),
+                       vals_.get<dev::Zd>(), mcol.get<std::uint32_t>(), mval.get<dev::Zd>(), st);
+    QSA_HIP(hipGetLastError());
+    QSA_HIP(hipStreamSynchronize(st));
+    row_ptr_ = std::move(merged_ptr);
+    col_ = std::move(mcol);
+    vals_ = std::move(mval);
+    csr_.nnz = static_cast<std::int64_t>(nnz);
+    csr_.width = 16;
+    csr_.dict = 0;
+    csr_.id_width_ref = 16;
+    csr_.bytes = row_ptr_.bytes() + col_.bytes() + vals_.bytes();
+    const double mean = rows ? static_cast<double>(nnz) / static_cast<double>(rows) : 0.0;
     int L = 4;
     while (L < 64 && L < mean / 4.0) L *= 2;
     csr_.lanes = L;

@@ -102,6 +102,10 @@ public:
 
 private:
     void build_csr(std::uint64_t budget);
+    /// The CSR of sectors of an irrep of dimension d > 1: counted and filled representative by
+    /// representative, full complex values; a target reached by several connections gives several
+    /// entries of one column.
+    void build_csr_irrep(std::uint64_t budget);
     /// The CSR as a kernel reads it; with sell, the SELL-64 copy (single and multi-vector applies).
     dev::CsrView csr_view(bool block = false, bool sell = false) const;
     int lanes_block_ = 16;

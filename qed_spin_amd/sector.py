@@ -96,9 +96,12 @@ class Sector:
     reverse lookup (n_up < 0: the full 2^N space)."""
 
     def __init__(self, group: GroupSpec, n_up: int, sublattice: int = -1, relaxed: bool = True,
-                 rank_budget: int = DEFAULT_RANK_BUDGET, reps=None, inv_norms=None):
+                 rank_budget: int = DEFAULT_RANK_BUDGET, reps=None, inv_norms=None, irrep=None):
         """reps / inv_norms given: adopt them (the reference's RepSectorData) instead of enumerating;
-        each representative is checked to find itself through this group's lookup."""
+        each representative is checked to find itself through this group's lookup. irrep given: the
+        sector of that irrep, D(g) for every element of the group as an array (G, d, d) (a unitary
+        representation; the group's characters are then not used): its basis states are the
+        partner-0 states of the representatives whose stabiliser admits them, `dim` counts them."""
         perms = np.ascontiguousarray(group.perms, dtype=np.int32)
         chars = np.ascontiguousarray(np.asarray(group.chars, np.complex128).view(np.float64))
         flips = None if group.flips is None else np.ascontiguousarray(group.flips, dtype=np.uint64)
@@ -106,11 +109,24 @@ class Sector:
             raise InvalidRequest("sector: perms must be (G, N) and chars (G,)")
         self.group = group
         self.n_up = int(n_up)
+        self.irrep_dim = 1
         h = C.c_void_p()
         head = (int(perms.shape[1]), int(n_up), int(perms.shape[0]), perms.ctypes.data_as(C.POINTER(C.c_int)),
                 None if flips is None else flips.ctypes.data_as(C.POINTER(C.c_uint64)),
                 chars.ctypes.data_as(C.POINTER(C.c_double)), int(sublattice), int(bool(relaxed)))
-        if reps is None:
+        if irrep is not None:
+            D = np.ascontiguousarray(np.asarray(irrep, np.complex128))
+            if D.ndim != 3 or D.shape[0] != perms.shape[0] or D.shape[1] != D.shape[2]:
+                raise InvalidRequest("sector: irrep must be (G, d, d)")
+            if reps is not None:
+                raise InvalidRequest("sector: an irrep sector is enumerated, not imported")
+            Dv = np.ascontiguousarray(D.reshape(-1).view(np.float64))
+            _lib.call("qedh_sector_create_irrep", head[0], head[1], head[2], head[3], head[4], int(D.shape[1]),
+                      _lib.ptr(Dv, C.c_double), int(sublattice), int(bool(relaxed)), C.c_uint64(int(rank_budget)),
+                      C.byref(h))
+            self.irrep_dim = int(D.shape[1])
+            self.irrep = D
+        elif reps is None:
             _lib.call("qedh_sector_create", *head, C.c_uint64(int(rank_budget)), C.byref(h))
         else:
             r = np.ascontiguousarray(reps, np.uint64)
@@ -126,16 +142,36 @@ class Sector:
         self.n_orbits = int(o.value)
         self.sublattice_block = int(b.value)
         self.build_seconds = float(s.value)
+        di, nr, nc = C.c_int(), C.c_int64(), C.c_int64()
+        _lib.call("qedh_sector_irrep_info", self.h, C.byref(di), C.byref(nr), C.byref(nc))
+        self.n_reps = int(nr.value)
+        self.n_classes = int(nc.value)
 
     @property
     def real(self) -> bool:
+        if self.irrep_dim > 1:
+            return False
         return bool(np.all(np.asarray(self.group.chars).imag == 0.0))
 
     def export(self):
-        reps = np.zeros(self.dim, np.uint64)
-        inv = np.zeros(self.dim, np.float64)
+        reps = np.zeros(self.n_reps, np.uint64)
+        inv = np.zeros(self.n_reps, np.float64)
         _lib.call("qedh_sector_export", self.h, _lib.ptr(reps, C.c_uint64), _lib.ptr(inv, C.c_double))
         return reps, inv
+
+    def export_irrep(self):
+        """A d > 1 sector's fields: per representative its stabiliser class and first state, per
+        class its rank and C (d x d; columns from the rank on zero)."""
+        if self.irrep_dim <= 1:
+            raise InvalidRequest("export_irrep: the sector's irrep is one-dimensional")
+        d = self.irrep_dim
+        rep_class = np.zeros(self.n_reps, np.uint16)
+        offset = np.zeros(self.n_reps + 1, np.uint64)
+        rank = np.zeros(self.n_classes, np.uint8)
+        Cm = np.zeros((self.n_classes, d, d), np.complex128)
+        _lib.call("qedh_sector_irrep_export", self.h, rep_class.ctypes.data_as(C.c_void_p), _lib.ptr(offset, C.c_uint64),
+                  rank.ctypes.data_as(C.c_void_p), _lib.ptr(Cm.reshape(-1).view(np.float64), C.c_double))
+        return rep_class, offset, rank, Cm
 
     def key_bits(self):
         kb = np.zeros(self.group.N, np.int32)
@@ -178,6 +214,10 @@ class SectorMatvec:
         self.op = op
         self.rows = rows
         self.cols = cols if cols is not None else rows
+        if getattr(rows, "irrep_dim", 1) > 1 and self.cols is not rows:
+            # the one-representative row walk needs an operator that commutes with the group; a
+            # probe between two irrep sectors is not of that kind
+            raise InvalidRequest("matvec: sectors of irreps of dimension > 1 take one sector and a symmetric operator")
         if program is not None:
             self.program = program
         elif self.cols is rows:

@@ -57,6 +57,60 @@ def dihedral_group(N: int, m: int, parity: int) -> GroupSpec:
     return GroupSpec(perms=perms, chars=chars)
 
 
+def compose(perms: np.ndarray, flips=None) -> np.ndarray:
+    """prod[g1, g2] = the index of the element g1 g2 (U(g1) U(g2) = U(g1 g2); with (g s)_i = s_{p[i]},
+    g1 g2 has p[i] = p2[p1[i]] and flip f1 ^ p1(f2)), or -1 when the product is not in the list."""
+    perms = np.asarray(perms)
+    G, N = perms.shape
+    fl = np.zeros(G, np.uint64) if flips is None else np.asarray(flips, np.uint64)
+
+    def permute(p, mask):   # (g m)_i = m_{p[i]}
+        out = 0
+        for i in range(N):
+            out |= ((int(mask) >> int(p[i])) & 1) << i
+        return out
+
+    index = {(tuple(perms[g]), int(fl[g])): g for g in range(G)}
+    prod = np.full((G, G), -1, np.int64)
+    for a in range(G):
+        for b in range(G):
+            p = perms[b][perms[a]]
+            f = int(fl[a]) ^ permute(perms[a], fl[b])
+            prod[a, b] = index.get((tuple(p), f), -1)
+    return prod
+
+
+def is_representation(perms: np.ndarray, D, flips=None, tol: float = 1e-12) -> bool:
+    """Whether D (G, d, d) is a unitary representation of the group in the composition above."""
+    D = np.asarray(D, np.complex128)
+    prod = compose(perms, flips)
+    if np.any(prod < 0):
+        return False
+    d = D.shape[1]
+    for g in range(D.shape[0]):
+        if np.abs(D[g] @ D[g].conj().T - np.eye(d)).max() > tol:
+            return False
+    return bool(np.abs(D[:, None] @ D[None, :] - D[prod]).max() <= tol)
+
+
+def dihedral_irrep(N: int, m: int):
+    """The two-dimensional irrep E_m (0 < m < N/2) of the dihedral group of the N-ring, for the
+    elements of dihedral_group's order: translations T^a (i -> i + a), then the reflections
+    i -> a - i, which are R T^a with R: i -> -i. On the momentum pair (m, -m): D(T^a) = diag(w^a,
+    w^-a), w = e^{2 pi i m / N}, D(R T^a) = [[0, w^-a], [w^a, 0]]. Returns (the group with the traces
+    as characters, D of shape (2N, 2, 2))."""
+    if not 0 < 2 * m < N:
+        raise ValueError("E_m needs 0 < m < N/2")
+    D = np.zeros((2 * N, 2, 2), np.complex128)
+    for a in range(N):
+        w = root_of_unity(m * a, N)
+        wc = root_of_unity(-m * a, N)
+        D[a] = [[w, 0.0], [0.0, wc]]
+        D[N + a] = [[0.0, wc], [w, 0.0]]
+    perms = np.concatenate([translations(N), reflections(N)])
+    return GroupSpec(perms=perms, chars=np.trace(D, axis1=1, axis2=2)), D
+
+
 def with_spin_flip(g: GroupSpec, z: int) -> GroupSpec:
     """The group g times the global spin flip (every site flipped), the flipped elements' characters
     z chi(g): the sector of flip parity z (only for n_up = N / 2, where the flip keeps Sz)."""
