@@ -101,6 +101,38 @@ def test_irrep_sector_against_the_oracle(N, n_up, m):
     assert np.abs(np.linalg.eigvalsh(A) - np.linalg.eigvalsh(Ak)).max() <= 1e-12 * max(1.0, np.abs(Hs).max())
 
 
+@pytest.mark.parametrize("n,layers,n_up", [(3, 2, 3), (4, 2, 4), (4, 2, 3), (5, 2, 5), (5, 2, 4)])
+def test_standard_irreps_of_symmetric_groups_d2_to_d4(n, layers, n_up):
+    from irrep_groups import layered_heisenberg, layered_symmetric_group, standard_irrep
+    from test_irrep_oracle import isotypic_spectrum
+
+    say("symmetric-group irrep sector", n, layers, n_up)
+    N = n * layers
+    perms, els = layered_symmetric_group(n, layers)
+    D = standard_irrep(els, n)
+    d = D.shape[1]
+    g = sym.GroupSpec(perms=perms, chars=np.trace(D, axis1=1, axis2=2))
+    s = qs.Sector(g, n_up, irrep=D)
+    assert s.irrep_dim == d
+    reps_o, Cs_o = po.irrep_fields(N, n_up, perms, D)
+    reps, _ = s.export()
+    assert np.array_equal(reps, reps_o)
+    rep_class, offset, rank, Cm = s.export_irrep()
+    ranks = np.diff(offset.astype(np.int64))
+    assert list(ranks) == [C.shape[1] for C in Cs_o]
+    Cs = [Cm[rep_class[i]][:, : ranks[i]] for i in range(len(reps))]
+    H = layered_heisenberg(qs, n, layers)
+    Hs, states = sz_hamiltonian(H, N, n_up)
+    E = po.irrep_basis_matrix(N, perms, D, reps, Cs, states)
+    assert np.abs(E.conj().T @ E - np.eye(s.dim)).max() <= 1e-12
+    mv = qs.SectorMatvec(H, s, lane=qs.LANE_CSR, tune=False)
+    assert merged(mv)
+    A = dense(mv)
+    assert np.abs(A - E.conj().T @ Hs @ E).max() <= 1e-12 * max(1.0, np.abs(Hs).max())
+    iso, _ = isotypic_spectrum(Hs, states, N, perms, D)
+    assert np.abs(np.sort(np.repeat(np.linalg.eigvalsh(A), d)) - np.sort(iso)).max() <= 1e-11
+
+
 def test_irrep_sector_n24_lowest_levels_equal_the_momentum_block():
     say("irrep sector N=24, E_5")
     N, m = 24, 5

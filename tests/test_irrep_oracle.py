@@ -82,3 +82,42 @@ def test_rank_follows_the_stabiliser():
     ranks = {int(r): C.shape[1] for r, C in zip(reps, Cs)}
     assert ranks == {0b0011: 1}
     assert sum(ranks.values()) == po.OracleSector(4, 2, sym.translations(4), sym.momentum_chars(4, 1)).dim
+
+
+def isotypic_spectrum(Hs, states, N, perms, D):
+    """Eigenvalues of H on the isotypic subspace of the irrep D in the Sz basis `states` (the range
+    of P = (d / |G|) sum_g conj(chi(g)) U(g)); every level there comes d times."""
+    index = {int(s): i for i, s in enumerate(states)}
+    G, d = D.shape[0], D.shape[1]
+    chi = np.trace(D, axis1=1, axis2=2)
+    P = np.zeros((len(states), len(states)), complex)
+    for g in range(G):
+        for j, s in enumerate(states):
+            t = 0
+            for i in range(N):
+                t |= ((int(s) >> int(perms[g][i])) & 1) << i
+            P[index[t], j] += np.conj(chi[g])
+    P *= d / G
+    w, V = np.linalg.eigh(0.5 * (P + P.conj().T))
+    Q = V[:, w > 0.5]
+    return np.linalg.eigvalsh(Q.conj().T @ Hs @ Q), Q.shape[1]
+
+
+@pytest.mark.parametrize("n,layers,n_up", [(3, 2, 3), (4, 2, 4), (4, 2, 3), (5, 2, 5), (5, 2, 4)])
+def test_standard_irreps_of_symmetric_groups(n, layers, n_up):
+    from irrep_groups import layered_heisenberg, layered_symmetric_group, standard_irrep
+
+    N = n * layers
+    perms, els = layered_symmetric_group(n, layers)
+    D = standard_irrep(els, n)
+    d = D.shape[1]
+    assert sym.is_representation(perms, D)
+    H = layered_heisenberg(qs, n, layers)
+    Hs, states = sz_hamiltonian(H, N, n_up)
+    reps, Cs = po.irrep_fields(N, n_up, perms, D)
+    E = po.irrep_basis_matrix(N, perms, D, reps, Cs, states)
+    assert np.abs(E.conj().T @ E - np.eye(E.shape[1])).max() <= 1e-12
+    iso, rank = isotypic_spectrum(Hs, states, N, perms, D)
+    assert rank == d * E.shape[1]
+    ev = np.linalg.eigvalsh(E.conj().T @ Hs @ E)
+    assert np.abs(np.sort(np.repeat(ev, d)) - np.sort(iso)).max() <= 1e-11
