@@ -189,6 +189,18 @@ int qedh_sector_irrep_export(qedh_sector s, uint16_t* rep_class, uint64_t* state
 /* dim (basis states), orbits before the character filter, sublattice block size (0: plain order),
  * build seconds. */
 int qedh_sector_info(qedh_sector s, int64_t* dim, int64_t* n_orbits, int* sublattice_block, double* build_seconds);
+/* Matrix elements of many observables between many vector pairs in one walk of the bra sector
+ * (rep_matrix_elements, include/ed/ops/program.h:130-161, and its device sweep
+ * src/gpu/rep_matrix_elements.cu:175-340): out[p][o] (complex, re/im) = <bras[p]| O_o |kets[p]>,
+ * programs[o] the program of O_o from the ket sector to the bra sector as qedh_matvec_create takes it
+ * (qedh_program_compile_projected of O_o^dag with the bra sector as src, or qedh_program_compile for
+ * one sector). bras / kets: host arrays of n_pairs device vectors (complex, of each sector's dim);
+ * 1 <= n_pairs <= 4096. The observables' terms are merged into one program, so a flip group costs
+ * one target lookup for all of them; the sums are accumulated in LDS by f64 atomics, so results agree
+ * with other evaluations to rounding (~1e-13 relative), not bit for bit. Sectors of irreps of
+ * dimension > 1: DeviceUnsupported. */
+int qedh_matrix_elements(qedh_sector bra_sector, qedh_sector ket_sector, int n_obs, const qedh_program* programs,
+                         int n_pairs, const void* const* bras, const void* const* kets, double* out);
 /* The representatives (n_reps of qedh_sector_irrep_info; dim for d = 1) and their 1/norms (1 for a
  * d > 1 sector, whose normalisation is in C). */
 int qedh_sector_export(qedh_sector s, uint64_t* reps, double* inv_norms);

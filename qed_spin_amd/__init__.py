@@ -9,7 +9,7 @@ Python surface: operators as term records, sectors, and the verbs with the refer
 
 from . import errors
 from .operator import OP_SMINUS, OP_SPLUS, OP_SZ, HamiltonianBuilder, Operator, Program
-from .sector import LANE_AUTO, LANE_CSR, LANE_GATHER, DeviceArray, GroupSpec, Sector, SectorMatvec
+from .sector import LANE_AUTO, LANE_CSR, LANE_GATHER, DeviceArray, GroupSpec, Sector, SectorMatvec, matrix_elements
 from .symmetry import Symmetry
 
 __all__ = [
@@ -22,6 +22,7 @@ __all__ = [
     "OP_SZ",
     "Sector",
     "SectorMatvec",
+    "matrix_elements",
     "GroupSpec",
     "DeviceArray",
     "Symmetry",

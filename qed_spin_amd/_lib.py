@@ -102,6 +102,10 @@ _SIGNATURES = {
     ],
     "qedh_sector_irrep_info": [C.c_void_p, c_int_p, c_i64_p, c_i64_p],
     "qedh_sector_irrep_export": [C.c_void_p, C.c_void_p, c_u64_p, C.c_void_p, c_dbl_p],
+    "qedh_matrix_elements": [
+        C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p),
+        C.POINTER(C.c_void_p), c_dbl_p,
+    ],
     "qedh_sector_info": [C.c_void_p, c_i64_p, c_i64_p, c_int_p, c_dbl_p],
     "qedh_sector_export": [C.c_void_p, c_u64_p, c_dbl_p],
     "qedh_sector_key_bits": [C.c_void_p, c_int_p, c_int_p],

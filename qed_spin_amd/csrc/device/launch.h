@@ -86,6 +86,20 @@ void csr_merge_rows(i64 rows, const u64* row_ptr, u32* col, Zd* val, u32* row_le
 void csr_move_rows(i64 rows, const u64* old_ptr, const u64* new_ptr, const u32* col, const Zd* val, u32* ncol, Zd* nval,
                    hipStream_t st);
 
+// ---- observe.hip -----------------------------------------------------------------------------
+/// Observables per launch of obs_sweep, times the pairs: the LDS accumulators (64 KB).
+constexpr int kObsCells = 4096;
+/// Workgroups of obs_sweep for a bra sector of `rows` representatives (partial holds 2 x pairs x
+/// nob doubles per workgroup).
+int obs_sweep_blocks(i64 rows);
+/// out[p][o0 .. o0 + nob) (complex, row length n_obs) = <bras[p]| O_o |kets[p]> for the observables
+/// o0 .. o0 + nob of the merged program P (term_obs: each term's observable; R the bra sector, C the
+/// ket sector; same: one sector, the diagonal shortcut). n_pairs x nob <= kObsCells. bras / kets:
+/// device arrays of device vector pointers.
+void obs_sweep(const SectorView& R, const SectorView& C, bool same, const ProgramView& P, const u32* term_obs, int o0,
+               int nob, int n_obs, int n_pairs, const Zd* const* bras, const Zd* const* kets, double* partial,
+               double* out, hipStream_t st);
+
 // ---- spmv.hip --------------------------------------------------------------------------------
 struct CsrView {
     i64 rows = 0;

@@ -4,6 +4,8 @@
 // coefficients bit for bit; the data structures are this project's own.
 #include "algebra.h"
 
+#include <map>
+
 #include <algorithm>
 #include <cmath>
 #include <stdexcept>
@@ -258,6 +260,46 @@ Program compile_projected(const SpinOperator& O, int G, const int* perms, const 
     }
     if (tree.empty()) return Program{{}, {}, {0}, {}, {0}, {}, {}};
     return pack(tree);
+}
+
+MultiProgram merge_programs(const std::vector<const Program*>& programs) {
+    struct Term {
+        std::uint32_t obs;
+        std::uint64_t sign;
+        cplx coeff;
+    };
+    std::map<std::uint64_t, std::map<std::uint64_t, std::vector<Term>>> groups;
+    for (std::size_t o = 0; o < programs.size(); ++o) {
+        const Program& P = *programs[o];
+        for (std::size_t g = 0; g < P.n_groups(); ++g)
+            for (std::uint32_t v = P.group_vbegin[g]; v < P.group_vbegin[g + 1]; ++v)
+                for (std::uint32_t k = P.vsub_tbegin[v]; k < P.vsub_tbegin[v + 1]; ++k)
+                    groups[P.group_flip[g]][P.vsub_val[v]].push_back(
+                        Term{static_cast<std::uint32_t>(o), P.term_sign[k], P.term_coeff[k]});
+    }
+    MultiProgram M;
+    M.n_obs = static_cast<int>(programs.size());
+    Program& P = M.prog;
+    for (const auto& [flip, vals] : groups) {
+        P.group_flip.push_back(flip);
+        P.group_vbegin.push_back(static_cast<std::uint32_t>(P.vsub_val.size()));
+        int setbits = -2;
+        for (const auto& [val, terms] : vals) {
+            const int pc = __builtin_popcountll(val);
+            setbits = setbits == -2 ? pc : (setbits == pc ? setbits : -1);
+            P.vsub_val.push_back(val);
+            P.vsub_tbegin.push_back(static_cast<std::uint32_t>(P.term_sign.size()));
+            for (const Term& t : terms) {
+                P.term_sign.push_back(t.sign);
+                P.term_coeff.push_back(t.coeff);
+                M.term_obs.push_back(t.obs);
+            }
+        }
+        P.group_setbits.push_back(setbits < 0 ? -1 : setbits);
+    }
+    P.group_vbegin.push_back(static_cast<std::uint32_t>(P.vsub_val.size()));
+    P.vsub_tbegin.push_back(static_cast<std::uint32_t>(P.term_sign.size()));
+    return M;
 }
 
 }  // namespace qsa

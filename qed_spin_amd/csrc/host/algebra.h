@@ -85,4 +85,15 @@ Program compile_operator(const SpinOperator& O);
 Program compile_projected(const SpinOperator& O, int G, const int* perms, const std::uint64_t* flips,
                           const cplx* chi_src, const cplx* chi_tgt, int src_nup, int tgt_nup);
 
+/// Several programs (each of one observable, compiled for one pair of sectors) as one: groups by
+/// flip mask ascending, value subgroups ascending, and in a subgroup every observable's terms in
+/// observable order, each tagged with its observable (the multi-observable MaskedProgram of the
+/// reference's compile_program, include/ed/ops/program.h:60-120).
+struct MultiProgram {
+    Program prog;
+    std::vector<std::uint32_t> term_obs;
+    int n_obs = 0;
+};
+MultiProgram merge_programs(const std::vector<const Program*>& programs);
+
 }  // namespace qsa

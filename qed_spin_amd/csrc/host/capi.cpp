@@ -669,6 +669,54 @@ int qedh_cf(qedh_matvec m, const double* phi_host, int krylov, int n_omega, cons
 
 }  // extern "C"
 
+extern "C" int qedh_matrix_elements(qedh_sector bra_sector, qedh_sector ket_sector, int n_obs,
+                                    const qedh_program* programs, int n_pairs, const void* const* bras,
+                                    const void* const* kets, double* out) {
+    return guarded([&] {
+        need(bra_sector, "bra sector");
+        need(ket_sector, "ket sector");
+        need(programs, "programs");
+        need(bras, "bras");
+        need(kets, "kets");
+        need(out, "out");
+        if (n_obs < 1 || n_pairs < 1 || n_pairs > qsa::dev::kObsCells)
+            throw qsa::Error(qsa::kInvalid, "matrix elements: 1 or more observables and 1 to 4096 pairs");
+        const auto& B = *bra_sector->sector;
+        const auto& K = *ket_sector->sector;
+        if (B.spec().perms != K.spec().perms || B.spec().flips != K.spec().flips)
+            throw qsa::Error(qsa::kInvalid, "matrix elements: sectors of different groups");
+        if (B.irrep_dim() > 1 || K.irrep_dim() > 1)
+            throw qsa::Error(qsa::kDeviceUnsupported, "matrix elements: sectors of irreps of dimension > 1");
+        if (B.spec().slc.engaged() != K.spec().slc.engaged() || B.spec().slc.key_bit != K.spec().slc.key_bit)
+            throw qsa::Error(qsa::kInvalid, "matrix elements: the sectors use different representative orders");
+        std::vector<const qsa::Program*> ps;
+        for (int o = 0; o < n_obs; ++o) {
+            need(programs[o], "program");
+            ps.push_back(&programs[o]->prog);
+        }
+        for (int p = 0; p < n_pairs; ++p) {
+            need(bras[p], "bra vector");
+            need(kets[p], "ket vector");
+        }
+        const qsa::MultiProgram M = qsa::merge_programs(ps);
+        const qsa::DeviceProgram dp(M.prog);
+        const qsa::DeviceBuffer tobs = qsa::upload(M.term_obs.data(), M.term_obs.size());
+        const qsa::DeviceBuffer bd = qsa::upload(bras, static_cast<std::size_t>(n_pairs));
+        const qsa::DeviceBuffer kd = qsa::upload(kets, static_cast<std::size_t>(n_pairs));
+        const int nob = std::min(n_obs, qsa::dev::kObsCells / n_pairs);
+        const int blocks = qsa::dev::obs_sweep_blocks(B.n_reps());
+        qsa::DeviceBuffer partial(static_cast<std::size_t>(blocks) * 2 * n_pairs * nob * sizeof(double));
+        qsa::DeviceBuffer dout(static_cast<std::size_t>(n_pairs) * n_obs * 2 * sizeof(double));
+        const hipStream_t st = qsa::engine_stream();
+        for (int o0 = 0; o0 < n_obs; o0 += nob)
+            qsa::dev::obs_sweep(B.view(), K.view(), bra_sector->sector == ket_sector->sector, dp.view(),
+                                tobs.get<std::uint32_t>(), o0, std::min(nob, n_obs - o0), n_obs, n_pairs,
+                                bd.get<const qsa::dev::Zd* const>(), kd.get<const qsa::dev::Zd* const>(),
+                                partial.get<double>(), dout.get<double>(), st);
+        QSA_HIP(hipGetLastError());
+        qsa::copy_to_host(out, dout.get(), dout.bytes());
+    });
+}
 extern "C" int qedh_sector_index_and_projection(qedh_sector s, const uint64_t* states, int64_t n, int64_t* index,
                                                 double* proj) {
     return guarded([&] {

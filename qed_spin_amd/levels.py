@@ -31,7 +31,7 @@ import numpy as np
 from . import _lib
 from .errors import InvalidRequest
 from .operator import Operator
-from .sector import LANE_GATHER, DeviceArray, GroupSpec, Sector, SectorMatvec
+from .sector import LANE_GATHER, DeviceArray, GroupSpec, Sector, SectorMatvec, matrix_elements, projected_program
 from .symmetry import trivial_group
 
 FORMAT_VERSION = 2
@@ -137,22 +137,21 @@ def _vdot(x_ptr, y_ptr, n: int) -> complex:
 
 
 def expect_in(basis: LevelBasis, vec: np.ndarray, ops: Sequence[Operator], cache: Dict) -> np.ndarray:
-    """<v|O|v> (v a unit vector of the sector) of every O: the projected program of O between two
-    handles of the sector, applied on the device (gather lane)."""
+    """<v|O|v> (v a unit vector of the sector) of every O in one walk of the sector (the fused
+    observable sweep, qedh_matrix_elements): each O's projected program between two handles of the
+    sector (an observable need not be group invariant)."""
     sec = basis.sector()
-    if "twin" not in cache:   # an observable need not be group invariant: lambda-projected between two handles
+    if "twin" not in cache:
         cache["twin"] = Sector(basis.group(), basis.n_up, sublattice=1 if basis.sublattice else 0, reps=basis.reps,
                                inv_norms=basis.inv_norms)
     v = DeviceArray.from_host(np.asarray(vec, np.complex128))
-    y = DeviceArray(sec.dim)
-    out = np.zeros(len(ops), np.complex128)
-    for o, O in enumerate(ops):
-        key = ("op", id(O))
+    programs = []
+    for O in ops:
+        key = ("prog", id(O))
         if key not in cache:
-            cache[key] = (O, SectorMatvec(O, cache["twin"], sec, lane=LANE_GATHER))
-        cache[key][1].apply(v, y, complex_vectors=True)
-        out[o] = _vdot(v.ptr, y.ptr, sec.dim)
-    return out
+            cache[key] = (O, projected_program(O, cache["twin"], sec))
+        programs.append(cache[key][1])
+    return matrix_elements(ops, cache["twin"], sec, [v], [v], programs=programs)[0]
 
 
 def split_by_shift(op: Operator) -> Dict[int, Operator]:

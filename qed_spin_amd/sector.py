@@ -200,6 +200,27 @@ def projected_program(op: Operator, rows: Sector, cols: Sector) -> Program:
     return Program(op, adjoint=True, src=rows, tgt=cols)
 
 
+def matrix_elements(ops, bra: "Sector", ket: "Sector", bras, kets, programs=None) -> np.ndarray:
+    """out[p, o] = <bras[p]| O_o |kets[p]> for device vectors (complex DeviceArrays) of the bra and ket
+    sectors: every observable and pair in one walk of the bra sector (qedh_matrix_elements; the
+    reference's rep_matrix_elements). programs: the observables' programs from the ket sector to the
+    bra sector when the caller has them (projected_program(O, bra, ket))."""
+    if programs is None:
+        programs = [projected_program(O, bra, ket) for O in ops]
+    n_obs, n_pairs = len(programs), len(bras)
+    if len(kets) != n_pairs:
+        raise InvalidRequest("matrix_elements: one ket per bra")
+    out = np.zeros((n_pairs, n_obs), np.complex128)
+    if n_obs == 0 or n_pairs == 0:
+        return out
+    ph = (C.c_void_p * n_obs)(*[p.h.value for p in programs])
+    bh = (C.c_void_p * n_pairs)(*[b.ptr.value for b in bras])
+    kh = (C.c_void_p * n_pairs)(*[k.ptr.value for k in kets])
+    _lib.call("qedh_matrix_elements", bra.h, ket.h, n_obs, ph, n_pairs, bh, kh,
+              _lib.ptr(out.reshape(-1).view(np.float64), C.c_double))
+    return out
+
+
 class SectorMatvec:
     """O restricted to a sector (or from sector `cols` to sector `rows`, one group) on device vectors.
 
