@@ -292,6 +292,60 @@ def _ref_chain(n, J2=0.35):
     return b.to_operator()
 
 
+def run_observe(args):
+    """The fused observable sweep on the Heisenberg ring N=28 (sector Sz=0, k=0: 1,432,860 states) for
+    a random unit vector: S_0.S_j (j = 1..14), S^z_j (28) and S^z_i S^z_j (i < j, 378): 420 observables
+    in one walk (qedh_matrix_elements), against one apply and one dot per observable on the device."""
+    import numpy as np
+
+    import qed_spin_amd as qs
+    from qed_spin_amd import _lib
+    from qed_spin_amd import symmetry as sym
+    from qed_spin_amd.sector import LANE_GATHER, projected_program
+
+    N = 28
+    g = sym.chain_group(N, 0)
+    sec = qs.Sector(g, N // 2)
+    reps, inv = sec.export()
+    twin = qs.Sector(g, N // 2, reps=reps, inv_norms=inv)
+    ops = []
+    for j in range(1, N // 2 + 1):
+        O = qs.Operator(N)
+        O.add_product("zz", (0, j), 1.0)
+        O.add_product("+-", (0, j), 0.5)
+        O.add_product("-+", (0, j), 0.5)
+        ops.append(O)
+    ops += [qs.Operator(N).add_product("z", (j,), 1.0) for j in range(N)]
+    ops += [qs.Operator(N).add_product("zz", (i, j), 1.0) for i in range(N) for j in range(i + 1, N)]
+    rng = np.random.default_rng(2)
+    v = rng.normal(size=sec.dim) + 1j * rng.normal(size=sec.dim)
+    v /= np.linalg.norm(v)
+    dv = qs.DeviceArray.from_host(v)
+    t0 = time.perf_counter()
+    progs = [projected_program(O, twin, sec) for O in ops]
+    t_compile = time.perf_counter() - t0
+    qs.matrix_elements(ops, twin, sec, [dv], [dv], programs=progs)   # warm
+    _lib.call("qedh_synchronize")
+    t1 = time.perf_counter()
+    got = qs.matrix_elements(ops, twin, sec, [dv], [dv], programs=progs)[0]
+    t_sweep = time.perf_counter() - t1
+    y = qs.DeviceArray(sec.dim)
+    mvs = [qs.SectorMatvec(O, twin, sec, lane=LANE_GATHER, program=p) for O, p in zip(ops, progs)]
+    _lib.call("qedh_synchronize")
+    t2 = time.perf_counter()
+    ref = np.zeros(len(ops), np.complex128)
+    out = np.zeros(2)
+    for o, m in enumerate(mvs):
+        m.apply(dv, y, complex_vectors=True)
+        _lib.call("qedh_vdot", dv.ptr, y.ptr, C.c_int64(sec.dim), 1, _lib.ptr(out, C.c_double))
+        ref[o] = complex(out[0], out[1])
+    t_loop = time.perf_counter() - t2
+    diff = float(np.max(np.abs(got - ref)) / max(1e-300, np.max(np.abs(ref))))
+    return {"workload": f"Heisenberg ring N={N}, Sz=0, k=0 ({sec.dim} states), {len(ops)} observables, one vector",
+            "sweep_s": round(t_sweep, 4), "per_observable_s": round(t_loop, 4), "speedup": round(t_loop / t_sweep, 1),
+            "compile_s": round(t_compile, 3), "max_rel_difference": diff}
+
+
 def run_reference_dynamics(args):
     """The reference's two published dynamics cases, same inputs (its bench/cases.py:105-132, model
     chain(N): the J1-J2 ring with J2 = 0.35): chain30_dyn0 -- T = 0 S^z(q = 2 pi 10 / 30), N = 30,
@@ -634,6 +688,9 @@ def main():
             out["spectrum"] = run_spectrum(args)
         elif w == "refdyn":
             out["reference_dynamics"] = run_reference_dynamics(args)
+        elif w == "observe":
+            out["observe"] = run_observe(args)
+            log(f"observe: {out['observe']}")
         elif w == "refftlm":
             out["reference_ftlm"] = run_reference_ftlm(args)
         elif w == "refeigs":
