@@ -145,7 +145,8 @@ int qedh_program_destroy(qedh_program p);
  * element XOR mask or NULL; chars: group_size complex characters (re, im); n_up < 0: the full
  * 2^N space. sublattice_mode: -1 the size rule (relaxed != 0: 16 distinct permutations, else 64),
  * 0 never, 1 whenever a block system exists (ED_SYM_SUBLATTICE,
- * include/ed/basis/sublattice_code.h:187-230). rank_budget_bytes caps the dense rank table. */
+ * include/ed/basis/sublattice_code.h:187-230). rank_budget_bytes caps the rank directory (16 bytes per 64 states of the
+ * subspace); above it lookups binary-search the representatives. */
 int qedh_sector_create(int n_sites, int n_up, int group_size, const int* perms, const uint64_t* flips,
                        const double* chars, int sublattice_mode, int relaxed, uint64_t rank_budget_bytes,
                        qedh_sector* out);

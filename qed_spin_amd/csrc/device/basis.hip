@@ -1,5 +1,5 @@
 // GPU-resident sector enumeration: the orbit scan of a fixed-Sz (or full) subspace into ascending
-// representatives with closed-form norms, and the dense rank -> index table.
+// representatives with closed-form norms, and the rank directory (rank -> index).
 //
 // The subspace is cut into 64-state words in rank (= ascending state) order. Pass 1 tests every
 // state for being its orbit's representative and keeps those whose character sum over the
@@ -174,12 +174,20 @@ __global__ __launch_bounds__(256) void k_orbit_emit_lane(SectorView S, u64 total
     inv_norms[off] = 1.0 / sqrt(norm_sq(S, s));
 }
 
-__global__ void k_rank_fill(SectorView S, i32* table) {
+// The rank directory from the kept bitmap and its exclusive scan: record w = (word w, kept before it).
+__global__ void k_rank_pack(const u64* kept, const u64* offsets, u64 n_words, RankRec* dir) {
+    const u64 w = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (w >= n_words) return;
+    dir[w] = RankRec{kept[w], static_cast<u32>(offsets[w]), 0u};
+}
+
+// The bitmap of given representatives (an imported sector): the bit of each one's rank.
+__global__ void k_rank_mark(SectorView S, const u64* reps, i64 n, unsigned long long* kept) {
     const i64 i = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= S.dim) return;
-    const u64 s = S.reps[i];
-    const i64 r = S.n_up >= 0 ? rank_of(S, s) : static_cast<i64>(s);
-    table[r] = static_cast<i32>(i);
+    if (i >= n) return;
+    const u64 s = reps[i];
+    const u64 r = S.n_up >= 0 ? static_cast<u64>(rank_of(S, s)) : s;
+    atomicOr(kept + (r >> 6), 1ULL << (r & 63));
 }
 
 // A sector vector in the Sz sector (full: the 2^N space), before normalisation
@@ -265,9 +273,15 @@ void orbit_emit(const SectorView& S, u64 total, const u64* kept, const u64* offs
                        offsets, reps, inv_norms);
 }
 
-void rank_fill(const SectorView& S, i32* table, hipStream_t st) {
-    if (S.dim == 0) return;
-    hipLaunchKernelGGL(k_rank_fill, dim3(blocks_for(static_cast<u64>(S.dim), 256)), dim3(256), 0, st, S, table);
+void rank_pack(const u64* kept, const u64* offsets, u64 n_words, RankRec* dir, hipStream_t st) {
+    if (n_words == 0) return;
+    hipLaunchKernelGGL(k_rank_pack, dim3(blocks_for(n_words, 256)), dim3(256), 0, st, kept, offsets, n_words, dir);
+}
+
+void rank_mark(const SectorView& S, const u64* reps, i64 n, u64* kept, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_rank_mark, dim3(blocks_for(static_cast<u64>(n), 256)), dim3(256), 0, st, S, reps, n,
+                       reinterpret_cast<unsigned long long*>(kept));
 }
 
 void sector_expand(const SectorView& S, const Zd* u, bool full, Zd* psi, unsigned* bad, hipStream_t st) {

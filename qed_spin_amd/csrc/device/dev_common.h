@@ -20,6 +20,16 @@ struct Zd {
     double re, im;
 };
 
+/// One record of a sector's rank directory: which of 64 consecutive ranks (colex ranks at fixed
+/// n_up, the states themselves otherwise) are representatives, a bit each, and the index of the
+/// first of them. Representatives ascend with their rank, so the index of a representative is the
+/// record's base plus the set bits below its own. 16 bytes per 64 ranks: one aligned load.
+struct alignas(16) RankRec {
+    u64 bits;
+    u32 base;
+    u32 spare;
+};
+
 /// A sector as the kernels read it (all pointers device memory). The representative of an orbit
 /// is its least member (in key order when slc_L > 0); reps ascend by state value.
 struct SectorView {
@@ -36,8 +46,18 @@ struct SectorView {
     const u64* reps = nullptr;       // dim, ascending
     const double* inv_norms = nullptr;
     i64 dim = 0;
-    const i32* rank_table = nullptr; // C(N, n_up) entries (state-indexed 2^N when n_up < 0), or null
+    const RankRec* rank_dir = nullptr; // C(N, n_up) / 64 records (2^N / 64 when n_up < 0), or null
     const u64* binom = nullptr;      // 65 x 65 Pascal table, binom[n * 65 + k] = C(n, k)
+    // The action by nibbles, for kernels that hold it in LDS: nq = ceil(N / 4) positions; nib[(g nq +
+    // q) 16 + v] is the image of nibble q = v under element g's permutation and nib_flip[g] its flip,
+    // both in key positions when slc_L > 0 (the key is linear in the state, so key(g s) is the OR of
+    // the nibbles' keyed images XOR the keyed flip); nib_key / nib_unkey (nq x 16) move a state's
+    // nibbles to key positions and back.
+    int nq = 0;
+    const u64* nib = nullptr;
+    const u64* nib_flip = nullptr;
+    const u64* nib_key = nullptr;
+    const u64* nib_unkey = nullptr;
 };
 
 /// An irrep of dimension d > 1 on a sector (the reference's rep_sector.h d > 1 fields): D(g) per
@@ -138,11 +158,14 @@ __device__ __forceinline__ i64 rank_of(const SectorView& S, u64 s) {
     return r;
 }
 
-/// Index of a representative, or -1: the rank table when present, else a binary search of reps.
+/// Index of a representative, or -1: the rank directory when present, else a binary search of reps.
 __device__ __forceinline__ i64 index_of_rep(const SectorView& S, u64 rb) {
-    if (S.rank_table) {
-        const i64 r = S.n_up >= 0 ? rank_of(S, rb) : static_cast<i64>(rb);
-        return static_cast<i64>(S.rank_table[r]);
+    if (S.rank_dir) {
+        const u64 r = S.n_up >= 0 ? static_cast<u64>(rank_of(S, rb)) : rb;
+        const RankRec e = S.rank_dir[r >> 6];
+        const u64 bit = 1ULL << (r & 63);
+        if (!(e.bits & bit)) return -1;
+        return static_cast<i64>(e.base) + popc(e.bits & (bit - 1ULL));
     }
     i64 lo = 0, hi = S.dim;
     while (lo < hi) {

@@ -17,7 +17,8 @@ void scan_words(const u64* kept, u64 n_words, u64* counts, u64* offsets, void* t
                 hipStream_t st);
 void orbit_emit(const SectorView& S, u64 total, const u64* kept, const u64* offsets, u64* reps, double* inv_norms,
                 hipStream_t st);
-void rank_fill(const SectorView& S, i32* table, hipStream_t st);
+void rank_pack(const u64* kept, const u64* offsets, u64 n_words, RankRec* dir, hipStream_t st);
+void rank_mark(const SectorView& S, const u64* reps, i64 n, u64* kept, hipStream_t st);
 /// psi (zeroed by the caller: C(N, n_up) entries in ascending state order, or 2^N when full) +=
 /// the expansion of the sector vector u; *bad = 1 when an image leaves the Sz sector.
 void sector_expand(const SectorView& S, const Zd* u, bool full, Zd* psi, unsigned* bad, hipStream_t st);

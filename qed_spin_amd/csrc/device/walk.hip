@@ -61,9 +61,231 @@ __global__ __launch_bounds__(256) void k_gather(SectorView R, SectorView C, int 
     }
 }
 
+// ---- the gather lane with the column sector's action in LDS --------------------------------------
+//
+// The walk above reads the group action from the 256-entry byte tables (G x bpw x 256 words: 512 KB
+// at N = 32 with 64 elements), the key tables and the Pascal table in global memory: about 21 L1
+// line accesses per entry, which bound it. Here every workgroup holds the column sector's action by
+// nibbles (G x nq x 16 words, 32-bit when N <= 32: 32 KB at N = 32, G = 64), the keyed flips, the
+// key tables, the characters and the rows of Pascal's triangle the rank needs in LDS (37 KB at N = 32,
+// G = 64), and walks rows in a grid-stride loop so each workgroup of 16 waves loads them once. What stays in global memory per target:
+// the sublattice lead and candidate entries, the rank directory record, the column's 1/norm and x.
+// The arithmetic is the walk above, operation for operation, so the results are its bits.
+
+template <class W> struct LdsAction {
+    const double* chr;
+    const double* chi;
+    const W* img;     // G x nq x 16: images of nibbles (keyed when the sublattice order is engaged)
+    const W* flip;    // G
+    const W* key;     // nq x 16
+    const W* unkey;   // nq x 16
+    const W* binom;   // n_sites x nb: C(b, j)
+    int nq, G, nb;
+};
+
+/// LDS bytes of the action of sector C for words of type W.
+template <class W> std::size_t lds_action_bytes(const SectorView& C) {
+    const std::size_t q16 = static_cast<std::size_t>(C.nq) * 16;
+    std::size_t words = static_cast<std::size_t>(C.G) * q16 + static_cast<std::size_t>(C.G);
+    if (C.slc_L > 0) words += 2 * q16;
+    if (C.n_up >= 0) words += static_cast<std::size_t>(C.n_sites) * static_cast<std::size_t>(C.n_up + 1);
+    return 16 * static_cast<std::size_t>(C.G) + sizeof(W) * words;
+}
+
+template <bool SLC, class W> __device__ LdsAction<W> stage_action(const SectorView& C, unsigned char* raw) {
+    LdsAction<W> A;
+    A.nq = C.nq;
+    A.G = C.G;
+    A.nb = C.n_up + 1;
+    double* chr = reinterpret_cast<double*>(raw);
+    double* chi = chr + C.G;
+    W* img = reinterpret_cast<W*>(chi + C.G);
+    const int n_img = C.G * C.nq * 16;
+    W* flip = img + n_img;
+    W* key = flip + C.G;
+    W* unkey = key + (SLC ? C.nq * 16 : 0);
+    W* bn = unkey + (SLC ? C.nq * 16 : 0);
+    for (int i = threadIdx.x; i < C.G; i += blockDim.x) {
+        chr[i] = C.chi_re[i];
+        chi[i] = C.chi_im[i];
+        flip[i] = static_cast<W>(C.nib_flip[i]);
+    }
+    for (int i = threadIdx.x; i < n_img; i += blockDim.x) img[i] = static_cast<W>(C.nib[i]);
+    if constexpr (SLC) {
+        for (int i = threadIdx.x; i < C.nq * 16; i += blockDim.x) {
+            key[i] = static_cast<W>(C.nib_key[i]);
+            unkey[i] = static_cast<W>(C.nib_unkey[i]);
+        }
+    }
+    if (C.n_up >= 0)
+        for (int i = threadIdx.x; i < C.n_sites * A.nb; i += blockDim.x)
+            bn[i] = static_cast<W>(C.binom[(i / A.nb) * 65 + i % A.nb]);
+    A.chr = chr;
+    A.chi = chi;
+    A.img = img;
+    A.flip = flip;
+    A.key = key;
+    A.unkey = unkey;
+    A.binom = bn;
+    return A;
+}
+
+template <class W> __device__ __forceinline__ W nib_map(const W* tab, int nq, u64 s) {
+    W o = 0;
+    for (int q = 0; q < nq; ++q) o |= tab[(q << 4) | static_cast<int>((s >> (4 * q)) & 15u)];
+    return o;
+}
+
+/// index_and_projection over the LDS action: the same running minimum, ties and character sums.
+template <bool SLC, class W>
+__device__ __forceinline__ i64 lds_index_and_projection(const SectorView& S, const LdsAction<W>& A, u64 t,
+                                                        double& pr, double& pi) {
+    if (S.n_up >= 0 && popc(t) != S.n_up) return -1;
+    W best = ~W(0);
+    double ar = 0.0, ai = 0.0;
+    auto visit = [&](int g) {
+        const W img = nib_map(A.img + g * A.nq * 16, A.nq, t) ^ A.flip[g];
+        if (img < best) {
+            best = img;
+            ar = 0.0 + A.chr[g];
+            ai = 0.0 - A.chi[g];
+        } else if (img == best) {
+            ar += A.chr[g];
+            ai -= A.chi[g];
+        }
+    };
+    u64 rb;
+    if constexpr (!SLC) {
+        for (int g = 0; g < A.G; ++g) visit(g);
+        rb = static_cast<u64>(best);
+    } else {
+        for_each_candidate(S, static_cast<u64>(nib_map(A.key, A.nq, t)), visit);
+        rb = static_cast<u64>(nib_map(A.unkey, A.nq, static_cast<u64>(best)));
+    }
+    if (S.n_up >= 0 && popc(rb) != S.n_up) return -1;
+    i64 idx;
+    if (S.rank_dir) {
+        u64 r = rb;
+        if (S.n_up >= 0) {
+            r = 0;
+            int j = 0;
+            for (u64 x = rb; x; x &= x - 1) {
+                ++j;
+                r += static_cast<u64>(A.binom[(__ffsll(static_cast<long long>(x)) - 1) * A.nb + j]);
+            }
+        }
+        const RankRec e = S.rank_dir[r >> 6];
+        const u64 bit = 1ULL << (r & 63);
+        if (!(e.bits & bit)) return -1;
+        idx = static_cast<i64>(e.base) + popc(e.bits & (bit - 1ULL));
+    } else {
+        idx = index_of_rep(S, rb);
+        if (idx < 0) return -1;
+    }
+    const double w = S.inv_norms[idx];
+    pr = ar * w;
+    pi = ai * w;
+    return idx;
+}
+
+template <bool SLC, bool REAL, int NV, class W>
+__global__ __launch_bounds__(1024) void k_gather_lds(SectorView R, SectorView C, int same, ProgramView P,
+                                                    Ptrs<NV> v) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const LdsAction<W> A = stage_action<SLC, W>(C, lds_raw);
+    __syncthreads();
+    const i64 stride = static_cast<i64>(gridDim.x) * blockDim.x;
+    for (i64 r = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x; r < R.dim; r += stride) {
+        const u64 s = R.reps[r];
+        const double w = R.inv_norms[r];
+        double ar[NV], ai[NV];
+#pragma unroll
+        for (int k = 0; k < NV; ++k) ar[k] = ai[k] = 0.0;
+        for (u32 g = 0; g < P.n_groups; ++g) {
+            u64 t;
+            double hr, hi;
+            if (!connection(P, s, g, t, hr, hi)) continue;
+            double vr, vi;
+            i64 c;
+            if (same != 0 && t == s) {
+                vr = hr;
+                vi = -hi;
+                c = r;
+            } else {
+                double pr, pi;
+                c = lds_index_and_projection<SLC, W>(C, A, t, pr, pi);
+                if (c < 0) continue;
+                const double xr = hr * pr - hi * pi;
+                const double xi = hr * pi + hi * pr;
+                vr = w * xr;
+                vi = w * (-xi);
+            }
+#pragma unroll
+            for (int k = 0; k < NV; ++k) {
+                if constexpr (REAL) {
+                    ar[k] += vr * static_cast<const double*>(v.in[k])[c];
+                } else {
+                    const Zd x = static_cast<const Zd*>(v.in[k])[c];
+                    ar[k] += vr * x.re - vi * x.im;
+                    ai[k] += vr * x.im + vi * x.re;
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            if constexpr (REAL) static_cast<double*>(v.out[k])[r] = ar[k];
+            else static_cast<Zd*>(v.out[k])[r] = Zd{ar[k], ai[k]};
+        }
+    }
+}
+
+/// LDS budget of the gather workgroup's action. Workgroups of 1024 threads (16 waves) share one copy:
+/// two of them per CU (32 waves, eight per SIMD) fit in 160 KB.
+constexpr std::size_t kLdsActionMax = 64 * 1024;
+/// Workgroups of the grid-stride gather: four per CU of a 256-CU device, each loading the action once.
+constexpr u64 kGatherLdsBlocks = 1024;
+
+/// Launch the LDS-action gather when the action fits kLdsActionMax; false when it does not.
+template <bool SLC, bool REAL>
+bool launch_gather_lds(const SectorView& R, const SectorView& C, bool same, const ProgramView& P, const VecSet& v,
+                       hipStream_t st) {
+    if (!C.nib || C.nq <= 0) return false;
+    const bool narrow = C.n_sites <= 32;
+    const std::size_t bytes = narrow ? lds_action_bytes<u32>(C) : lds_action_bytes<u64>(C);
+    if (bytes > kLdsActionMax) return false;
+    const u64 need = blocks_for(static_cast<u64>(R.dim), 1024);
+    const dim3 grid(static_cast<unsigned>(need < kGatherLdsBlocks ? need : kGatherLdsBlocks)), block(1024);
+    int done = 0;
+    while (done < v.k) {
+        const int rest = v.k - done;
+        const int nv = rest >= 8 ? 8 : rest >= 4 ? 4 : rest >= 2 ? 2 : 1;
+        auto go = [&](auto tag) {
+            constexpr int NV = decltype(tag)::value;
+            Ptrs<NV> p;
+            for (int k = 0; k < NV; ++k) {
+                p.in[k] = v.in[done + k];
+                p.out[k] = v.out[done + k];
+            }
+            if (narrow)
+                hipLaunchKernelGGL((k_gather_lds<SLC, REAL, NV, u32>), grid, block, bytes, st, R, C, same ? 1 : 0, P,
+                                   p);
+            else
+                hipLaunchKernelGGL((k_gather_lds<SLC, REAL, NV, u64>), grid, block, bytes, st, R, C, same ? 1 : 0, P,
+                                   p);
+        };
+        if (nv == 8) go(std::integral_constant<int, 8>{});
+        else if (nv == 4) go(std::integral_constant<int, 4>{});
+        else if (nv == 2) go(std::integral_constant<int, 2>{});
+        else go(std::integral_constant<int, 1>{});
+        done += nv;
+    }
+    return true;
+}
+
 template <bool SLC, bool REAL>
 void launch_gather(const SectorView& R, const SectorView& C, bool same, const ProgramView& P, const VecSet& v,
                    hipStream_t st) {
+    if (launch_gather_lds<SLC, REAL>(R, C, same, P, v, st)) return;
     const dim3 grid(blocks_for(static_cast<u64>(R.dim), 256)), block(256);
     int done = 0;
     while (done < v.k) {

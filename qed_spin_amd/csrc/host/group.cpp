@@ -24,6 +24,47 @@ std::vector<std::uint64_t> GroupSpec::byte_lut() const {
     return lut;
 }
 
+std::vector<std::uint64_t> GroupSpec::nibble_tables() const {
+    const int N = n_sites, Q = nq();
+    const bool keyed = slc.engaged();
+    auto keyed_bit = [&](int site) { return keyed ? slc.key_bit[site] : site; };
+    std::vector<std::uint64_t> t(static_cast<std::size_t>(G) * Q * 16 + G + (keyed ? 2 * Q * 16 : 0), 0ULL);
+    for (int g = 0; g < G; ++g) {
+        const int* p = perms.data() + static_cast<std::size_t>(g) * N;
+        int inv[64];
+        for (int i = 0; i < N; ++i) inv[p[i]] = i;   // source bit j lands on output bit inv[j]
+        for (int q = 0; q < Q; ++q)
+            for (int v = 0; v < 16; ++v) {
+                std::uint64_t o = 0;
+                for (int u = 0; u < 4 && 4 * q + u < N; ++u)
+                    if ((v >> u) & 1) o |= 1ULL << keyed_bit(inv[4 * q + u]);
+                t[(static_cast<std::size_t>(g) * Q + q) * 16 + v] = o;
+            }
+        std::uint64_t f = 0;
+        for (int i = 0; i < N; ++i)
+            if ((flips[g] >> i) & 1ULL) f |= 1ULL << keyed_bit(i);
+        t[static_cast<std::size_t>(G) * Q * 16 + g] = f;
+    }
+    if (keyed) {
+        std::uint64_t* key = t.data() + static_cast<std::size_t>(G) * Q * 16 + G;
+        std::uint64_t* unkey = key + Q * 16;
+        int site_of[64];
+        for (int i = 0; i < N; ++i) site_of[slc.key_bit[i]] = i;
+        for (int q = 0; q < Q; ++q)
+            for (int v = 0; v < 16; ++v) {
+                std::uint64_t k = 0, s = 0;
+                for (int u = 0; u < 4 && 4 * q + u < N; ++u)
+                    if ((v >> u) & 1) {
+                        k |= 1ULL << slc.key_bit[4 * q + u];
+                        s |= 1ULL << site_of[4 * q + u];
+                    }
+                key[q * 16 + v] = k;
+                unkey[q * 16 + v] = s;
+            }
+    }
+    return t;
+}
+
 bool GroupSpec::real_characters() const {
     for (const auto& c : chars)
         if (c.imag() != 0.0) return false;

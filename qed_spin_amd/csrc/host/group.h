@@ -44,6 +44,12 @@ struct GroupSpec {
     SublatticeTables slc;
 
     int bpw() const { return (n_sites + 7) / 8; }
+    int nq() const { return (n_sites + 3) / 4; }
+    /// The action by nibbles, one array: the images of nibble q = v under every element's
+    /// permutation ([(g nq + q) 16 + v]), then the G flips, then (sublattice order engaged) the
+    /// nibbles' key positions and their inverse (nq x 16 each); images and flips in key positions
+    /// when the sublattice order is engaged. For kernels that hold the action in LDS.
+    std::vector<std::uint64_t> nibble_tables() const;
     /// Byte lookup tables of the action: lut[(g bpw + b) 256 + v] is the image of byte b = v
     /// (include/ed/basis/compiled_group.h:105-162 layout).
     std::vector<std::uint64_t> byte_lut() const;

@@ -96,6 +96,12 @@ void DeviceSector::setup_tables() {
     for (int n = 0; n <= 64; ++n)
         for (int k = 0; k <= n; ++k) pascal[n * 65 + k] = binom_host(n, k);
     binom_ = upload(pascal.data(), pascal.size());
+    {
+        const std::vector<std::uint64_t> nib = spec_.nibble_tables();
+        const std::size_t body = static_cast<std::size_t>(G) * spec_.nq() * 16;
+        nib_ = upload(nib.data(), nib.size());
+        nib_sizes_ = {body, static_cast<std::size_t>(G), static_cast<std::size_t>(spec_.nq()) * 16};
+    }
     if (spec_.slc.engaged()) {
         to_key_ = upload(spec_.slc.to_key.data(), spec_.slc.to_key.size());
         lead_ = upload(spec_.slc.lead.data(), spec_.slc.lead.size());
@@ -104,20 +110,36 @@ void DeviceSector::setup_tables() {
     }
 }
 
-void DeviceSector::build_rank_table(std::uint64_t total, std::uint64_t rank_budget_bytes) {
+void DeviceSector::build_rank_dir(std::uint64_t total, std::uint64_t rank_budget_bytes, const std::uint64_t* kept,
+                                  const std::uint64_t* offsets) {
     const hipStream_t st = engine_stream();
-    const std::uint64_t table_bytes = total * 4;
-    if (dim_ > 0 && table_bytes <= rank_budget_bytes) {
-        rank_table_.reset(table_bytes);
-        QSA_HIP(hipMemsetAsync(rank_table_.get(), 0xFF, table_bytes, st));
-        dev::SectorView v2 = view();
-        v2.rank_table = nullptr;
-        dev::rank_fill(v2, rank_table_.get<dev::i32>(), st);
+    const std::uint64_t n_words = (total + 63) / 64;
+    const std::uint64_t dir_bytes = n_words * sizeof(dev::RankRec);
+    if (dim_ > 0 && dir_bytes <= rank_budget_bytes) {
+        rank_dir_.reset(dir_bytes);
+        if (kept && offsets) {
+            dev::rank_pack(kept, offsets, n_words, rank_dir_.get<dev::RankRec>(), st);
+        } else {
+            // imported representatives: their bitmap, its scan, then the records
+            DeviceBuffer bits(n_words * 8), counts(n_words * 8), offs(n_words * 8);
+            QSA_HIP(hipMemsetAsync(bits.get(), 0, n_words * 8, st));
+            dev::SectorView v = view();
+            v.rank_dir = nullptr;
+            dev::rank_mark(v, reps_.get<std::uint64_t>(), dim_, bits.get<std::uint64_t>(), st);
+            QSA_HIP(hipGetLastError());
+            const std::size_t tb = dev::scan_words_temp_bytes(n_words);
+            DeviceBuffer temp(tb ? tb : 8);
+            dev::scan_words(bits.get<std::uint64_t>(), n_words, counts.get<std::uint64_t>(),
+                            offs.get<std::uint64_t>(), temp.get(), tb, st);
+            dev::rank_pack(bits.get<std::uint64_t>(), offs.get<std::uint64_t>(), n_words,
+                           rank_dir_.get<dev::RankRec>(), st);
+            QSA_HIP(hipStreamSynchronize(st));
+        }
         QSA_HIP(hipGetLastError());
         stats_.rank_table = true;
     }
     QSA_HIP(hipStreamSynchronize(st));
-    stats_.device_bytes = lut_.bytes() + reps_.bytes() + inv_norms_.bytes() + rank_table_.bytes() + lead_.bytes() +
+    stats_.device_bytes = lut_.bytes() + nib_.bytes() + reps_.bytes() + inv_norms_.bytes() + rank_dir_.bytes() + lead_.bytes() +
                           cand_off_.bytes() + cand_.bytes();
 }
 
@@ -157,7 +179,7 @@ DeviceSector::DeviceSector(GroupSpec spec, std::uint64_t rank_budget_bytes) : sp
                         inv_norms_.get<double>(), st);
         QSA_HIP(hipGetLastError());
     }
-    build_rank_table(total, rank_budget_bytes);
+    build_rank_dir(total, rank_budget_bytes, kept.get<std::uint64_t>(), offsets.get<std::uint64_t>());
     stats_.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
@@ -187,7 +209,7 @@ DeviceSector::DeviceSector(GroupSpec spec, const std::uint64_t* reps, const doub
     dim_ = n;
     reps_ = upload(reps, static_cast<std::size_t>(n));
     inv_norms_ = upload(inv_norms, static_cast<std::size_t>(n));
-    build_rank_table(total, rank_budget_bytes);
+    build_rank_dir(total, rank_budget_bytes, nullptr, nullptr);
     // every representative must be its own orbit's representative and find itself: the order
     // (plain or key order) and the canonical form are the ones this group's lookup uses
     if (n > 0) {
@@ -216,6 +238,13 @@ dev::SectorView DeviceSector::view() const {
     v.flips = flips_.get<std::uint64_t>();
     v.chi_re = chi_re_.get<double>();
     v.chi_im = chi_im_.get<double>();
+    v.nq = spec_.nq();
+    v.nib = nib_.get<std::uint64_t>();
+    v.nib_flip = v.nib + nib_sizes_[0];
+    if (spec_.slc.engaged()) {
+        v.nib_key = v.nib_flip + nib_sizes_[1];
+        v.nib_unkey = v.nib_key + nib_sizes_[2];
+    }
     if (spec_.slc.engaged()) {
         v.slc_L = spec_.slc.L;
         v.slc_m = spec_.slc.m;
@@ -227,7 +256,7 @@ dev::SectorView DeviceSector::view() const {
     v.reps = reps_.get<std::uint64_t>();
     v.inv_norms = inv_norms_.get<double>();
     v.dim = dim_;
-    v.rank_table = rank_table_ ? rank_table_.get<dev::i32>() : nullptr;
+    v.rank_dir = rank_dir_ ? rank_dir_.get<dev::RankRec>() : nullptr;
     v.binom = binom_.get<std::uint64_t>();
     return v;
 }

@@ -2,6 +2,7 @@
 // include/ed/basis/rep_sector.h:134-408, src/gpu/rep_matvec.cu:253-383), built on the device.
 #pragma once
 
+#include <array>
 #include <memory>
 #include <vector>
 
@@ -16,7 +17,7 @@ struct SectorBuildStats {
     std::int64_t orbits = 0;           ///< orbits of the subspace before the character filter
     std::uint64_t subspace_dim = 0;
     std::uint64_t device_bytes = 0;
-    bool rank_table = false;
+    bool rank_table = false;           ///< the rank directory is present (else lookups binary-search)
 };
 
 /// An irrep of dimension d > 1 on a sector (the reference's rep_sector.h d > 1 fields): D(g) per
@@ -42,11 +43,12 @@ public:
     static std::shared_ptr<DeviceSector> make_irrep(GroupSpec spec, int d, const std::vector<cplx>& D,
                                                     std::uint64_t rank_budget_bytes);
 
-    /// Enumerate the sector of `spec` on the device. `rank_budget_bytes` caps the dense rank ->
-    /// index table (C(N, n_up) x 4 bytes); above it lookups binary-search the representatives.
+    /// Enumerate the sector of `spec` on the device. `rank_budget_bytes` caps the rank directory
+    /// (16 bytes per 64 states of the subspace: C(N, n_up) / 4 bytes); above it lookups
+    /// binary-search the representatives.
     DeviceSector(GroupSpec spec, std::uint64_t rank_budget_bytes);
     /// Adopt representatives and 1/norms computed elsewhere (the reference's RepSectorData): the
-    /// group tables and the rank table are built here; every representative is checked to be a
+    /// group tables and the rank directory are built here; every representative is checked to be a
     /// state of the subspace and to find itself through the group's lookup (order and canonical form).
     DeviceSector(GroupSpec spec, const std::uint64_t* reps, const double* inv_norms, std::int64_t n,
                  std::uint64_t rank_budget_bytes);
@@ -74,12 +76,17 @@ public:
 private:
     void attach_irrep(IrrepData ir);
     void setup_tables();
-    void build_rank_table(std::uint64_t total, std::uint64_t rank_budget_bytes);
+    /// The rank directory from the scan's kept bitmap and its exclusive scan, or (both null) from
+    /// the representatives.
+    void build_rank_dir(std::uint64_t total, std::uint64_t rank_budget_bytes, const std::uint64_t* kept,
+                        const std::uint64_t* offsets);
     GroupSpec spec_;
     std::int64_t dim_ = 0;
     SectorBuildStats stats_;
     DeviceBuffer lut_, flips_, chi_re_, chi_im_, to_key_, lead_, cand_off_, cand_, binom_;
-    DeviceBuffer reps_, inv_norms_, rank_table_;
+    DeviceBuffer reps_, inv_norms_, rank_dir_;
+    DeviceBuffer nib_;                          ///< GroupSpec::nibble_tables
+    std::array<std::size_t, 3> nib_sizes_{};    ///< entries of its images, flips and key tables
     IrrepData irrep_;
     std::int64_t states_ = 0;
     DeviceBuffer irrep_D_, class_C_, class_rank_, rep_class_, state_offset_;

@@ -318,15 +318,15 @@ def spin_tower_operator(H: Operator, two_S: int, mu: Optional[float] = None) -> 
 
 
 def _resident_bytes(N: int, sym: Symmetry) -> float:
-    """A resident walk's device footprint, roughly: per block its dense rank table (4 bytes per state
-    of its Sz subspace) and per state about 48 bytes of representatives, norms and CSR (the blocks'
+    """A resident walk's device footprint, roughly: per block its rank directory (16 bytes per 64
+    states of its Sz subspace) and per state about 48 bytes of representatives, norms and CSR (the blocks'
     states add up to the selected Sz subspaces, or a share of them under momentum blocks)."""
     from math import comb
 
     total = 0.0
     for b in sym.blocks(N):
         sub = comb(N, b.n_up) if b.n_up >= 0 else 2 ** N
-        total += 4.0 * sub + 48.0 * sub * b.weight / max(1, b.group.G)
+        total += 0.25 * sub + 48.0 * sub * b.weight / max(1, b.group.G)
     return total
 
 

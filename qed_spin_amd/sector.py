@@ -17,7 +17,7 @@ from .operator import Operator, Program
 
 LANE_AUTO, LANE_GATHER, LANE_CSR = 0, 1, 2
 LANE_NO_TUNE = 0x100             # OR-ed into the lane: skip the SpMV autotuning
-DEFAULT_RANK_BUDGET = 16 << 30   # dense rank table up to 16 GiB of HBM
+DEFAULT_RANK_BUDGET = 16 << 30   # rank directory (C(N, n_up) / 4 bytes) up to 16 GiB of HBM
 AUTOTUNE_MIN_ROWS = 1 << 18      # CSR operators from this many rows time their SpMV variants when built
 BUILD_THREADS = int(os.environ.get("QED_SPIN_AMD_BUILD_THREADS", "8"))   # host threads building sectors and operators
 

@@ -123,3 +123,45 @@ def test_refused_inputs():
 
     with pytest.raises(errors.QEDError):
         sv.serve([dict(kind="eigs", n_sites=20)], hbm_budget=1 << 20)
+
+
+@pytest.mark.parametrize("N,group,n_up,sublattice,stride", [(18, "dihedral", 9, 0, 1), (16, "chain", -1, 0, 1),
+                                                          (20, "dihedral", 10, 1, 3)])
+def test_rank_directory_against_binary_search(N, group, n_up, sublattice, stride):
+    """The rank directory (a bit per state of the subspace and a base per 64 states) and the binary
+    search over the representatives (rank_budget = 0) resolve every lookup alike: the same merged CSR
+    bit for bit, the same gather-lane apply, index_and_projection equal on every state of the subspace
+    (every third one at N = 20, where the sublattice key order is engaged). An imported sector builds
+    its directory from the representatives: the same again."""
+    say("rank directory vs binary search", N, group, n_up, sublattice)
+    g = sym.dihedral_group(N, 0, +1) if group == "dihedral" else sym.chain_group(N, 1)
+    H = _ring(N, 0.4)
+    a = qs.Sector(g, n_up, sublattice=sublattice)
+    b = qs.Sector(g, n_up, sublattice=sublattice, rank_budget=0)
+    ra, ia = a.export()
+    rb, ib = b.export()
+    assert np.array_equal(ra, rb) and np.array_equal(ia, ib)
+    c = qs.Sector(g, n_up, sublattice=sublattice, reps=ra, inv_norms=ia)
+    assert a.sublattice_block == b.sublattice_block == c.sublattice_block
+    assert (a.sublattice_block > 0) == (sublattice == 1)
+    csr = [qs.SectorMatvec(H, s, lane=qs.LANE_CSR).export_csr() for s in (a, b, c)]
+    for other in csr[1:]:
+        for x, y in zip(csr[0], other):
+            assert np.array_equal(np.asarray(x).view(np.uint8), np.asarray(y).view(np.uint8))
+    x = np.random.default_rng(5).normal(size=a.dim) + 1j * np.random.default_rng(6).normal(size=a.dim)
+    ys = [qs.SectorMatvec(H, s, lane=qs.LANE_GATHER).apply_host(x, complex_vectors=True) for s in (a, b, c)]
+    assert np.array_equal(ys[0].view(np.uint8), ys[1].view(np.uint8))
+    assert np.array_equal(ys[0].view(np.uint8), ys[2].view(np.uint8))
+    if n_up >= 0:
+        states = np.array([s for s in range(1 << N) if bin(s).count("1") == n_up], np.uint64)[::stride]
+    else:
+        states = np.arange(1 << N, dtype=np.uint64)[::stride]
+    if n_up >= 0:   # states of another magnetisation find no index
+        states = np.concatenate([states, np.array([(1 << (n_up + 1)) - 1, (1 << (n_up - 1)) - 1], np.uint64)])
+    ja, pa = a.index_and_projection(states)
+    for s in (b, c):
+        jb, pb = s.index_and_projection(states)
+        assert np.array_equal(ja, jb) and np.array_equal(np.asarray(pa).view(np.uint8), np.asarray(pb).view(np.uint8))
+    assert (ja >= 0).sum() > 0
+    if n_up >= 0:
+        assert np.all(ja[-2:] == -1)
