@@ -188,6 +188,25 @@ __device__ __forceinline__ i64 lds_index_and_projection(const SectorView& S, con
     return idx;
 }
 
+/// entry_of over the LDS action: the same operations, so the same bits.
+template <bool SLC, class W>
+__device__ __forceinline__ i64 lds_entry_of(const SectorView& C, const LdsAction<W>& A, bool same, i64 r, u64 s,
+                                            double w, u64 t, double hr, double hi, double& vr, double& vi) {
+    if (same && t == s) {
+        vr = hr;
+        vi = -hi;
+        return r;
+    }
+    double pr, pi;
+    const i64 c = lds_index_and_projection<SLC, W>(C, A, t, pr, pi);
+    if (c < 0) return -1;
+    const double xr = hr * pr - hi * pi;
+    const double xi = hr * pi + hi * pr;
+    vr = w * xr;
+    vi = w * (-xi);
+    return c;
+}
+
 template <bool SLC, bool REAL, int NV, class W>
 __global__ __launch_bounds__(1024) void k_gather_lds(SectorView R, SectorView C, int same, ProgramView P,
                                                     Ptrs<NV> v) {
@@ -206,20 +225,8 @@ __global__ __launch_bounds__(1024) void k_gather_lds(SectorView R, SectorView C,
             double hr, hi;
             if (!connection(P, s, g, t, hr, hi)) continue;
             double vr, vi;
-            i64 c;
-            if (same != 0 && t == s) {
-                vr = hr;
-                vi = -hi;
-                c = r;
-            } else {
-                double pr, pi;
-                c = lds_index_and_projection<SLC, W>(C, A, t, pr, pi);
-                if (c < 0) continue;
-                const double xr = hr * pr - hi * pi;
-                const double xi = hr * pi + hi * pr;
-                vr = w * xr;
-                vi = w * (-xi);
-            }
+            const i64 c = lds_entry_of<SLC, W>(C, A, same != 0, r, s, w, t, hr, hi, vr, vi);
+            if (c < 0) continue;
 #pragma unroll
             for (int k = 0; k < NV; ++k) {
                 if constexpr (REAL) {
@@ -475,24 +482,155 @@ __global__ __launch_bounds__(256) void k_csr_fill(SectorView R, SectorView C, in
     }
 }
 
-// ---- one walk: the merged rows staged, then the dictionary and the CSR from the stage ---------
+// ---- one walk: the rows staged, sorted and merged, then the dictionary and the CSR -------------
 // Entry e of row r sits at stage[e * rows + r] (a column per entry slot: the threads of a wave
-// write and read consecutive rows together); the walk, the most expensive part of the build, runs
-// once instead of once per pass.
-template <bool SLC>
-__global__ __launch_bounds__(256) void k_csr_walk(SectorView R, SectorView C, int same, ProgramView P, u32* row_len,
-                                                  u32* stage_col, Zd* stage_val) {
-    const i64 r = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (r >= R.dim) return;
-    u32 cols[kMaxRow];
-    double vre[kMaxRow], vim[kMaxRow];
-    const int n = merged_row<SLC>(R, C, same != 0, P, r, cols, vre, vim);
-    row_len[r] = static_cast<u32>(n);
-    const u64 rows = static_cast<u64>(R.dim);
-    for (int e = 0; e < n; ++e) {
-        stage_col[static_cast<u64>(e) * rows + r] = cols[e];
-        stage_val[static_cast<u64>(e) * rows + r] = Zd{vre[e], vim[e]};
+// write and read consecutive rows together). The walk (k_csr_emit, one thread per row, the column
+// sector's action in LDS when it fits) writes each row's entries in emission order; k_csr_sort then
+// takes one wavefront per row: a bitonic sort of the keys (column, emission index) across the lanes,
+// equal columns summed in emission order, exact zeros dropped, the row written back in place -- the
+// merged row of the per-thread walk above, entry for entry and bit for bit, without its private
+// arrays.
+
+template <bool SLC, bool LDS, class W>
+__global__ __launch_bounds__(1024) void k_csr_emit(SectorView R, SectorView C, int same, ProgramView P, u32* row_len,
+                                                   u32* stage_col, Zd* stage_val) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    LdsAction<W> A{};
+    if constexpr (LDS) {
+        A = stage_action<SLC, W>(C, lds_raw);
+        __syncthreads();
     }
+    const u64 rows = static_cast<u64>(R.dim);
+    const i64 stride = static_cast<i64>(gridDim.x) * blockDim.x;
+    for (i64 r = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x; r < R.dim; r += stride) {
+        const u64 s = R.reps[r];
+        const double w = R.inv_norms[r];
+        u32 n = 0;
+        for (u32 g = 0; g < P.n_groups; ++g) {
+            u64 t;
+            double hr, hi;
+            if (!connection(P, s, g, t, hr, hi)) continue;
+            double vr, vi;
+            i64 c;
+            if constexpr (LDS) c = lds_entry_of<SLC, W>(C, A, same != 0, r, s, w, t, hr, hi, vr, vi);
+            else c = entry_of<SLC>(C, same != 0, r, s, w, t, hr, hi, vr, vi);
+            if (c < 0) continue;
+            stage_col[static_cast<u64>(n) * rows + static_cast<u64>(r)] = static_cast<u32>(c);
+            stage_val[static_cast<u64>(n) * rows + static_cast<u64>(r)] = Zd{vr, vi};
+            ++n;
+        }
+        row_len[r] = n;
+    }
+}
+
+/// The value of element x (of the row's 64 E sorted elements, element x in lane x % 64, slot x / 64)
+/// read by every lane from wherever it sits.
+template <int E, class T> __device__ __forceinline__ T elem_at(const T (&v)[E], int x) {
+    const int src = x & 63;
+    T out = __shfl(v[0], src, 64);
+    if constexpr (E == 2) {
+        const T hi = __shfl(v[1], src, 64);
+        if (x >= 64) out = hi;
+    }
+    return out;
+}
+
+template <int E>
+__global__ __launch_bounds__(256) void k_csr_sort(i64 rows_n, u32* row_len, u32* stage_col, Zd* stage_val) {
+    const i64 r = (static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+    if (r >= rows_n) return;   // the whole wave
+    const int lane = static_cast<int>(threadIdx.x & 63);
+    const u64 rows = static_cast<u64>(rows_n);
+    const int n = static_cast<int>(row_len[r]);
+    u64 key[E];
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+        const int e = lane + 64 * q;
+        key[q] = e < n ? (static_cast<u64>(stage_col[static_cast<u64>(e) * rows + static_cast<u64>(r)]) << 8) |
+                             static_cast<u64>(e)
+                       : ~0ULL;
+    }
+    // bitonic sort of the 64 E keys, element x = 64 q + lane
+#pragma unroll
+    for (int k = 2; k <= 64 * E; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            if (j >= 64) {
+                // E == 2, k == 128: element x and x + 64 sit in this lane's two slots, ascending
+                const u64 a = key[0] < key[1] ? key[0] : key[1];
+                const u64 b = key[0] < key[1] ? key[1] : key[0];
+                key[0] = a;
+                key[E - 1] = b;
+                continue;
+            }
+#pragma unroll
+            for (int q = 0; q < E; ++q) {
+                const int x = 64 * q + lane;
+                const u64 other = __shfl_xor(key[q], j, 64);
+                const bool up = (x & k) == 0, lower = (x & j) == 0;
+                key[q] = (lower == up) ? (other < key[q] ? other : key[q]) : (other > key[q] ? other : key[q]);
+            }
+        }
+    }
+    u32 col[E];
+    double vr[E], vi[E];
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+        const int x = 64 * q + lane;
+        col[q] = static_cast<u32>(key[q] >> 8);
+        vr[q] = vi[q] = 0.0;
+        if (x < n) {
+            const Zd v = stage_val[(key[q] & 255ULL) * rows + static_cast<u64>(r)];
+            vr[q] = v.re;
+            vi[q] = v.im;
+        }
+    }
+    // a run of equal columns is summed by its first element, left to right (emission order)
+    bool head[E];
+    double sr[E], si[E];
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+        const int x = 64 * q + lane;
+        const u32 prev = elem_at<E>(col, x > 0 ? x - 1 : 0);
+        head[q] = x < n && (x == 0 || prev != col[q]);
+        sr[q] = vr[q];
+        si[q] = vi[q];
+    }
+    for (int d = 1; d < n; ++d) {
+        bool more = false;
+#pragma unroll
+        for (int q = 0; q < E; ++q) {
+            const int x = 64 * q + lane;
+            const int y = x + d < 64 * E ? x + d : x;
+            const u32 cy = elem_at<E>(col, y);
+            const double ry = elem_at<E>(vr, y), iy = elem_at<E>(vi, y);
+            if (head[q] && x + d < n && cy == col[q]) {
+                sr[q] += ry;
+                si[q] += iy;
+                more = true;
+            }
+        }
+        if (!__any(more)) break;
+    }
+    bool keep[E];
+    u64 mask[E];
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+        keep[q] = head[q] && !(sr[q] == 0.0 && si[q] == 0.0);
+        mask[q] = __ballot(keep[q]);
+    }
+    const u64 below = (1ULL << lane) - 1ULL;
+    u32 before = 0;
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+        if (keep[q]) {
+            const u64 pos = before + static_cast<u32>(__popcll(mask[q] & below));
+            stage_col[pos * rows + static_cast<u64>(r)] = col[q];
+            stage_val[pos * rows + static_cast<u64>(r)] = Zd{sr[q], si[q]};
+        }
+        before += static_cast<u32>(__popcll(mask[q]));
+    }
+    if (lane == 0) row_len[r] = before;
 }
 
 // The dictionary from the staged values: per workgroup an LDS table of its rows' distinct values,
@@ -599,11 +737,28 @@ bool csr_walk(const SectorView& R, const SectorView& C, bool same, const Program
               Zd* stage_val, hipStream_t st) {
     if (P.n_groups > static_cast<u32>(kMaxRow)) return false;
     if (R.dim == 0) return true;
-    const dim3 grid(blocks_for(static_cast<u64>(R.dim), 256)), block(256);
-    if (C.slc_L > 0)
-        hipLaunchKernelGGL(k_csr_walk<true>, grid, block, 0, st, R, C, same ? 1 : 0, P, row_len, stage_col, stage_val);
+    const u64 need = blocks_for(static_cast<u64>(R.dim), 1024);
+    const dim3 grid(static_cast<unsigned>(need < kGatherLdsBlocks ? need : kGatherLdsBlocks)), block(1024);
+    const bool slc = C.slc_L > 0, narrow = C.n_sites <= 32;
+    const std::size_t bytes = !C.nib ? kLdsActionMax + 1 : narrow ? lds_action_bytes<u32>(C) : lds_action_bytes<u64>(C);
+#define QSA_EMIT(S, L, W, B) \
+    hipLaunchKernelGGL((k_csr_emit<S, L, W>), grid, block, B, st, R, C, same ? 1 : 0, P, row_len, stage_col, stage_val)
+    if (bytes <= kLdsActionMax) {
+        if (slc && narrow) QSA_EMIT(true, true, u32, bytes);
+        else if (slc) QSA_EMIT(true, true, u64, bytes);
+        else if (narrow) QSA_EMIT(false, true, u32, bytes);
+        else QSA_EMIT(false, true, u64, bytes);
+    } else if (slc) {
+        QSA_EMIT(true, false, u64, 0);
+    } else {
+        QSA_EMIT(false, false, u64, 0);
+    }
+#undef QSA_EMIT
+    const dim3 sgrid(blocks_for(static_cast<u64>(R.dim) * 64, 256)), sblock(256);
+    if (P.n_groups <= 64)
+        hipLaunchKernelGGL(k_csr_sort<1>, sgrid, sblock, 0, st, R.dim, row_len, stage_col, stage_val);
     else
-        hipLaunchKernelGGL(k_csr_walk<false>, grid, block, 0, st, R, C, same ? 1 : 0, P, row_len, stage_col, stage_val);
+        hipLaunchKernelGGL(k_csr_sort<2>, sgrid, sblock, 0, st, R.dim, row_len, stage_col, stage_val);
     return true;
 }
 
