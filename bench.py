@@ -674,6 +674,16 @@ def main():
         multi[str(k)] = {"ms": avg.value * 1e3, "GBps": Bk / avg.value / 1e9}
         log(f"block k={k}: {avg.value * 1e3:.3f} ms -> {Bk / avg.value / 1e9:.1f} GB/s")
 
+    # the on-the-fly (gather) lane on the same sector and program, scored by the same B(1): the lane
+    # the engine falls back on when a CSR does not fit its budget
+    mg = qs.SectorMatvec(op, sector, lane=qs.LANE_GATHER)
+    _lib.call("qedh_matvec_benchmark", mg.h, 1, 0, 0, 2, 10, C.byref(avg))
+    gather = {"ms": round(avg.value * 1e3, 3), "GBps": round(B1 / avg.value / 1e9, 1),
+              "frac": round(B1 / avg.value / 1e9 / HBM_PEAK_GBS, 4),
+              "note": "on-the-fly lane (no CSR), by the merged CSR's B(1)"}
+    log(f"gather lane: {gather}")
+    del mg
+
     lanczos = None
     if not args.no_lanczos:
         rng = np.random.default_rng(1)
@@ -751,6 +761,7 @@ def main():
             "traffic_from": traffic_from,
         },
         "multi_vector": multi,
+        "gather_lane": gather,
         "lanczos_to_E0": lanczos,
         "cpu_baseline": cpu,
     }

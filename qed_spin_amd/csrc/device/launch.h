@@ -49,21 +49,15 @@ struct DictTable {
     unsigned* flags = nullptr;     // bit 0: overflow, bit 1: fingerprint collision
 };
 
-/// Pass 1 of the CSR build: merged row lengths into row_len, distinct values into the dictionary.
-/// Returns false (nothing launched) when the program has more groups than a row can hold.
-bool csr_count(const SectorView& R, const SectorView& C, bool same, const ProgramView& P, u32* row_len,
-               const DictTable& D, bool use_dict, hipStream_t st);
-/// Pass 2: columns and value ids (width 1 or 2) or full values (width 0: complex Zd, or double
-/// when real) at row_ptr.
-void csr_fill(const SectorView& R, const SectorView& C, bool same, const ProgramView& P, const u64* row_ptr,
-              u32* col, void* vals, int width, bool real, const DictTable& D, hipStream_t st);
-/// The build in one walk: each row merged once into a stage (entry e of row r at
-/// stage[e * rows + r], at most `acting groups` entries per row) with its length; then the
-/// dictionary from the staged values (csr_stage_dict, the same set of values csr_count inserts) and,
-/// after the row scan, the CSR from the stage (csr_compact: what csr_fill writes, bit for bit).
-/// false (nothing launched) when the program has more groups than a row can hold.
-bool csr_walk(const SectorView& R, const SectorView& C, bool same, const ProgramView& P, u32* row_len, u32* stage_col,
-              Zd* stage_val, hipStream_t st);
+/// The merged CSR's rows [row0, row0 + n_rows), walked once: each row's entries in emission order into
+/// a stage (entry e of local row i at stage[e * n_rows + i], room for the program's acting groups per
+/// row), then each row sorted by (column, emission index), equal columns summed in emission order and
+/// exact zeros dropped, in place (sector_rows.h:147-158), with its length in row_len[i]. Then the
+/// dictionary from the staged values (csr_stage_dict) and, after the row scan, the CSR from the stage
+/// (csr_compact, row_ptr offset to row0). false (nothing launched) when the program has more groups
+/// than a row can hold.
+bool csr_walk(const SectorView& R, const SectorView& C, bool same, const ProgramView& P, i64 row0, i64 n_rows,
+              u32* row_len, u32* stage_col, Zd* stage_val, hipStream_t st);
 void csr_stage_dict(i64 rows, const u32* row_len, const Zd* stage_val, const DictTable& D, hipStream_t st);
 void csr_compact(i64 rows, const u64* row_ptr, const u32* stage_col, const Zd* stage_val, u32* col, void* vals,
                  int width, bool real, const DictTable& D, hipStream_t st);

@@ -1,5 +1,5 @@
-// Term application in a symmetry sector: the on-the-fly row walk (gather lane) and the two passes
-// of the merged reduced-CSR build.
+// Term application in a symmetry sector: the on-the-fly row walk (gather lane) and the merged
+// reduced-CSR build (a staged walk, a wavefront-per-row sort and merge, the dictionary, compaction).
 //
 // A row r of O between sector R (rows) and sector C (columns) comes from the walk of the program
 // of O^dagger from R's representative s_r: each acting group gives its target t and h, the target
@@ -315,53 +315,6 @@ void launch_gather(const SectorView& R, const SectorView& C, bool same, const Pr
     }
 }
 
-/// The merged entries of row r: columns ascending, equal columns summed in emission order, exact
-/// zeros dropped. Returns the count.
-template <bool SLC>
-__device__ int merged_row(const SectorView& R, const SectorView& C, bool same, const ProgramView& P, i64 r, u32* cols,
-                          double* vre, double* vim) {
-    const u64 s = R.reps[r];
-    const double w = R.inv_norms[r];
-    int n = 0;
-    for (u32 g = 0; g < P.n_groups; ++g) {
-        u64 t;
-        double hr, hi;
-        if (!connection(P, s, g, t, hr, hi)) continue;
-        double vr, vi;
-        const i64 c = entry_of<SLC>(C, same, r, s, w, t, hr, hi, vr, vi);
-        if (c < 0) continue;
-        // stable insertion: after every entry with a column <= c
-        int j = n;
-        while (j > 0 && cols[j - 1] > static_cast<u32>(c)) {
-            cols[j] = cols[j - 1];
-            vre[j] = vre[j - 1];
-            vim[j] = vim[j - 1];
-            --j;
-        }
-        cols[j] = static_cast<u32>(c);
-        vre[j] = vr;
-        vim[j] = vi;
-        ++n;
-    }
-    int o = 0;
-    for (int i = 0; i < n;) {
-        double sr = vre[i], si = vim[i];
-        int k = i + 1;
-        for (; k < n && cols[k] == cols[i]; ++k) {
-            sr += vre[k];
-            si += vim[k];
-        }
-        if (!(sr == 0.0 && si == 0.0)) {
-            cols[o] = cols[i];
-            vre[o] = sr;
-            vim[o] = si;
-            ++o;
-        }
-        i = k;
-    }
-    return o;
-}
-
 __device__ __forceinline__ u64 bits_of(double x) { return static_cast<u64>(__double_as_longlong(x)); }
 
 __device__ __forceinline__ u64 mix64(u64 z) {
@@ -381,110 +334,9 @@ __device__ __forceinline__ u64 fingerprint(u64 re, u64 im) {
     return z ? z : 1ULL;
 }
 
-template <bool SLC>
-__global__ __launch_bounds__(256) void k_csr_count(SectorView R, SectorView C, int same, ProgramView P, u32* row_len,
-                                                   DictTable D, int use_dict) {
-    __shared__ unsigned long long s_fp[kLdsSlots];
-    __shared__ u64 s_re[kLdsSlots], s_im[kLdsSlots];
-    __shared__ int s_full;
-    for (int i = threadIdx.x; i < kLdsSlots; i += blockDim.x) s_fp[i] = 0ULL;
-    if (threadIdx.x == 0) s_full = 0;
-    __syncthreads();
-    const i64 r = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (r < R.dim) {
-        u32 cols[kMaxRow];
-        double vre[kMaxRow], vim[kMaxRow];
-        const int n = merged_row<SLC>(R, C, same != 0, P, r, cols, vre, vim);
-        row_len[r] = static_cast<u32>(n);
-        if (use_dict) {
-            for (int e = 0; e < n; ++e) {
-                const u64 re = bits_of(vre[e]), im = bits_of(vim[e]);
-                const u64 fp = fingerprint(re, im);
-                const int h = static_cast<int>((fp >> 17) & (kLdsSlots - 1));
-                bool placed = false;
-                for (int p = 0; p < kLdsSlots && !placed; ++p) {
-                    const int i = (h + p) & (kLdsSlots - 1);
-                    const unsigned long long old = atomicCAS(&s_fp[i], 0ULL, static_cast<unsigned long long>(fp));
-                    if (old == 0ULL) {
-                        s_re[i] = re;
-                        s_im[i] = im;
-                        placed = true;
-                    } else if (old == fp) {
-                        placed = true;
-                    }
-                }
-                if (!placed) s_full = 1;
-            }
-        }
-    }
-    __syncthreads();
-    if (!use_dict) return;
-    if (s_full) {
-        if (threadIdx.x == 0) atomicOr(D.flags, 1u);
-        return;
-    }
-    for (int i = threadIdx.x; i < kLdsSlots; i += blockDim.x) {
-        const u64 fp = s_fp[i];
-        if (!fp) continue;
-        const int h = static_cast<int>((fp >> 17) & (DictTable::kSlots - 1));
-        bool placed = false;
-        for (int p = 0; p < 4096 && !placed; ++p) {
-            const int j = (h + p) & (DictTable::kSlots - 1);
-            const unsigned long long old =
-                atomicCAS(reinterpret_cast<unsigned long long*>(&D.fp[j]), 0ULL, static_cast<unsigned long long>(fp));
-            if (old == 0ULL) {
-                D.re[j] = s_re[i];
-                D.im[j] = s_im[i];
-                atomicAdd(D.count, 1u);
-                placed = true;
-            } else if (old == fp) {
-                placed = true;
-            }
-        }
-        if (!placed) atomicOr(D.flags, 1u);
-    }
-}
-
-template <bool SLC, int WIDTH, bool REAL>
-__global__ __launch_bounds__(256) void k_csr_fill(SectorView R, SectorView C, int same, ProgramView P,
-                                                  const u64* row_ptr, u32* col, void* vals, DictTable D) {
-    const i64 r = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (r >= R.dim) return;
-    u32 cols[kMaxRow];
-    double vre[kMaxRow], vim[kMaxRow];
-    const int n = merged_row<SLC>(R, C, same != 0, P, r, cols, vre, vim);
-    const u64 e0 = row_ptr[r];
-    for (int e = 0; e < n; ++e) {
-        col[e0 + e] = cols[e];
-        if constexpr (WIDTH == 0) {
-            if constexpr (REAL) static_cast<double*>(vals)[e0 + e] = vre[e];
-            else static_cast<Zd*>(vals)[e0 + e] = Zd{vre[e], vim[e]};
-        } else {
-            const u64 re = bits_of(vre[e]), im = bits_of(vim[e]);
-            const u64 fp = fingerprint(re, im);
-            const int h = static_cast<int>((fp >> 17) & (DictTable::kSlots - 1));
-            int slot = -1;
-            for (int p = 0; p < 4096; ++p) {
-                const int j = (h + p) & (DictTable::kSlots - 1);
-                const u64 f = D.fp[j];
-                if (f == fp) {
-                    slot = j;
-                    break;
-                }
-                if (f == 0) break;
-            }
-            u32 id = 0;
-            if (slot < 0 || D.re[slot] != re || D.im[slot] != im) atomicOr(D.flags, 2u);
-            else id = D.slot_id[slot];
-            if constexpr (WIDTH == 1) static_cast<u8*>(vals)[e0 + e] = static_cast<u8>(id);
-            else static_cast<u16*>(vals)[e0 + e] = static_cast<u16>(id);
-        }
-    }
-}
-
 // ---- one walk: the rows staged, sorted and merged, then the dictionary and the CSR -------------
-// Entry e of row r sits at stage[e * rows + r] (a column per entry slot: the threads of a wave
-// write and read consecutive rows together). The walk (k_csr_emit, one thread per row, the column
+// Entry e of local row i of a chunk of rows sits at stage[e * rows + i] (a column per entry slot:
+// the threads of a wave write and read consecutive rows together). The walk (k_csr_emit, one thread per row, the column
 // sector's action in LDS when it fits) writes each row's entries in emission order; k_csr_sort then
 // takes one wavefront per row: a bitonic sort of the keys (column, emission index) across the lanes,
 // equal columns summed in emission order, exact zeros dropped, the row written back in place -- the
@@ -492,17 +344,18 @@ __global__ __launch_bounds__(256) void k_csr_fill(SectorView R, SectorView C, in
 // arrays.
 
 template <bool SLC, bool LDS, class W>
-__global__ __launch_bounds__(1024) void k_csr_emit(SectorView R, SectorView C, int same, ProgramView P, u32* row_len,
-                                                   u32* stage_col, Zd* stage_val) {
+__global__ __launch_bounds__(1024) void k_csr_emit(SectorView R, SectorView C, int same, ProgramView P, i64 row0,
+                                                   i64 n_rows, u32* row_len, u32* stage_col, Zd* stage_val) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     LdsAction<W> A{};
     if constexpr (LDS) {
         A = stage_action<SLC, W>(C, lds_raw);
         __syncthreads();
     }
-    const u64 rows = static_cast<u64>(R.dim);
+    const u64 rows = static_cast<u64>(n_rows);
     const i64 stride = static_cast<i64>(gridDim.x) * blockDim.x;
-    for (i64 r = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x; r < R.dim; r += stride) {
+    for (i64 i = static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x; i < n_rows; i += stride) {
+        const i64 r = row0 + i;
         const u64 s = R.reps[r];
         const double w = R.inv_norms[r];
         u32 n = 0;
@@ -515,11 +368,11 @@ __global__ __launch_bounds__(1024) void k_csr_emit(SectorView R, SectorView C, i
             if constexpr (LDS) c = lds_entry_of<SLC, W>(C, A, same != 0, r, s, w, t, hr, hi, vr, vi);
             else c = entry_of<SLC>(C, same != 0, r, s, w, t, hr, hi, vr, vi);
             if (c < 0) continue;
-            stage_col[static_cast<u64>(n) * rows + static_cast<u64>(r)] = static_cast<u32>(c);
-            stage_val[static_cast<u64>(n) * rows + static_cast<u64>(r)] = Zd{vr, vi};
+            stage_col[static_cast<u64>(n) * rows + static_cast<u64>(i)] = static_cast<u32>(c);
+            stage_val[static_cast<u64>(n) * rows + static_cast<u64>(i)] = Zd{vr, vi};
             ++n;
         }
-        row_len[r] = n;
+        row_len[i] = n;
     }
 }
 
@@ -535,20 +388,20 @@ template <int E, class T> __device__ __forceinline__ T elem_at(const T (&v)[E], 
     return out;
 }
 
-template <int E>
-__global__ __launch_bounds__(256) void k_csr_sort(i64 rows_n, u32* row_len, u32* stage_col, Zd* stage_val) {
-    const i64 r = (static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
-    if (r >= rows_n) return;   // the whole wave
-    const int lane = static_cast<int>(threadIdx.x & 63);
-    const u64 rows = static_cast<u64>(rows_n);
-    const int n = static_cast<int>(row_len[r]);
-    u64 key[E];
+/// One row (n <= 64 E entries at stage[e * rows + r]) sorted by (column, emission index), equal
+/// columns summed in emission order, exact zeros dropped, written back in place; by one wavefront.
+/// K: the key word, (column << 7) | e (32-bit when the columns are below 2^25).
+template <int E, class K>
+__device__ __forceinline__ void sort_merge_row(u64 rows, u64 r, int n, int lane, u32* row_len, u32* stage_col,
+                                               Zd* stage_val) {
+    constexpr K kPad = ~K(0);
+    K key[E];
 #pragma unroll
     for (int q = 0; q < E; ++q) {
         const int e = lane + 64 * q;
-        key[q] = e < n ? (static_cast<u64>(stage_col[static_cast<u64>(e) * rows + static_cast<u64>(r)]) << 8) |
-                             static_cast<u64>(e)
-                       : ~0ULL;
+        key[q] = e < n ? static_cast<K>((static_cast<K>(stage_col[static_cast<u64>(e) * rows + r]) << 7) |
+                                        static_cast<K>(e))
+                       : kPad;
     }
     // bitonic sort of the 64 E keys, element x = 64 q + lane
 #pragma unroll
@@ -557,8 +410,8 @@ __global__ __launch_bounds__(256) void k_csr_sort(i64 rows_n, u32* row_len, u32*
         for (int j = k >> 1; j > 0; j >>= 1) {
             if (j >= 64) {
                 // E == 2, k == 128: element x and x + 64 sit in this lane's two slots, ascending
-                const u64 a = key[0] < key[1] ? key[0] : key[1];
-                const u64 b = key[0] < key[1] ? key[1] : key[0];
+                const K a = key[0] < key[E - 1] ? key[0] : key[E - 1];
+                const K b = key[0] < key[E - 1] ? key[E - 1] : key[0];
                 key[0] = a;
                 key[E - 1] = b;
                 continue;
@@ -566,7 +419,7 @@ __global__ __launch_bounds__(256) void k_csr_sort(i64 rows_n, u32* row_len, u32*
 #pragma unroll
             for (int q = 0; q < E; ++q) {
                 const int x = 64 * q + lane;
-                const u64 other = __shfl_xor(key[q], j, 64);
+                const K other = __shfl_xor(key[q], j, 64);
                 const bool up = (x & k) == 0, lower = (x & j) == 0;
                 key[q] = (lower == up) ? (other < key[q] ? other : key[q]) : (other > key[q] ? other : key[q]);
             }
@@ -577,40 +430,47 @@ __global__ __launch_bounds__(256) void k_csr_sort(i64 rows_n, u32* row_len, u32*
 #pragma unroll
     for (int q = 0; q < E; ++q) {
         const int x = 64 * q + lane;
-        col[q] = static_cast<u32>(key[q] >> 8);
+        col[q] = static_cast<u32>(key[q] >> 7);
         vr[q] = vi[q] = 0.0;
         if (x < n) {
-            const Zd v = stage_val[(key[q] & 255ULL) * rows + static_cast<u64>(r)];
+            const Zd v = stage_val[static_cast<u64>(key[q] & K(127)) * rows + r];
             vr[q] = v.re;
             vi[q] = v.im;
         }
     }
     // a run of equal columns is summed by its first element, left to right (emission order)
-    bool head[E];
+    bool head[E], cont[E];
     double sr[E], si[E];
 #pragma unroll
     for (int q = 0; q < E; ++q) {
         const int x = 64 * q + lane;
         const u32 prev = elem_at<E>(col, x > 0 ? x - 1 : 0);
+        const u32 next = elem_at<E>(col, x + 1 < 64 * E ? x + 1 : x);
         head[q] = x < n && (x == 0 || prev != col[q]);
+        cont[q] = head[q] && x + 1 < n && next == col[q];
         sr[q] = vr[q];
         si[q] = vi[q];
     }
-    for (int d = 1; d < n; ++d) {
-        bool more = false;
+    bool any_run = false;
 #pragma unroll
-        for (int q = 0; q < E; ++q) {
-            const int x = 64 * q + lane;
-            const int y = x + d < 64 * E ? x + d : x;
-            const u32 cy = elem_at<E>(col, y);
-            const double ry = elem_at<E>(vr, y), iy = elem_at<E>(vi, y);
-            if (head[q] && x + d < n && cy == col[q]) {
-                sr[q] += ry;
-                si[q] += iy;
-                more = true;
+    for (int q = 0; q < E; ++q) any_run = any_run || cont[q];
+    if (__any(any_run)) {
+        for (int d = 1; d < n; ++d) {
+            bool more = false;
+#pragma unroll
+            for (int q = 0; q < E; ++q) {
+                const int x = 64 * q + lane;
+                const int y = x + d < 64 * E ? x + d : x;
+                const u32 cy = elem_at<E>(col, y);
+                const double ry = elem_at<E>(vr, y), iy = elem_at<E>(vi, y);
+                if (head[q] && x + d < n && cy == col[q]) {
+                    sr[q] += ry;
+                    si[q] += iy;
+                    more = true;
+                }
             }
+            if (!__any(more)) break;
         }
-        if (!__any(more)) break;
     }
     bool keep[E];
     u64 mask[E];
@@ -625,12 +485,25 @@ __global__ __launch_bounds__(256) void k_csr_sort(i64 rows_n, u32* row_len, u32*
     for (int q = 0; q < E; ++q) {
         if (keep[q]) {
             const u64 pos = before + static_cast<u32>(__popcll(mask[q] & below));
-            stage_col[pos * rows + static_cast<u64>(r)] = col[q];
-            stage_val[pos * rows + static_cast<u64>(r)] = Zd{sr[q], si[q]};
+            stage_col[pos * rows + r] = col[q];
+            stage_val[pos * rows + r] = Zd{sr[q], si[q]};
         }
         before += static_cast<u32>(__popcll(mask[q]));
     }
     if (lane == 0) row_len[r] = before;
+}
+
+/// One wavefront per row; rows of at most 64 entries take the one-key-per-lane network.
+template <int E, class K>
+__global__ __launch_bounds__(256) void k_csr_sort(i64 rows_n, u32* row_len, u32* stage_col, Zd* stage_val) {
+    const i64 r = (static_cast<i64>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+    if (r >= rows_n) return;   // the whole wave
+    const int lane = static_cast<int>(threadIdx.x & 63);
+    const int n = static_cast<int>(row_len[r]);
+    if (E == 2 && n > 64)
+        sort_merge_row<E, K>(static_cast<u64>(rows_n), static_cast<u64>(r), n, lane, row_len, stage_col, stage_val);
+    else
+        sort_merge_row<1, K>(static_cast<u64>(rows_n), static_cast<u64>(r), n, lane, row_len, stage_col, stage_val);
 }
 
 // The dictionary from the staged values: per workgroup an LDS table of its rows' distinct values,
@@ -733,16 +606,17 @@ __global__ __launch_bounds__(256) void k_csr_compact(i64 rows_n, const u64* row_
 
 }  // namespace
 
-bool csr_walk(const SectorView& R, const SectorView& C, bool same, const ProgramView& P, u32* row_len, u32* stage_col,
-              Zd* stage_val, hipStream_t st) {
+bool csr_walk(const SectorView& R, const SectorView& C, bool same, const ProgramView& P, i64 row0, i64 n_rows,
+              u32* row_len, u32* stage_col, Zd* stage_val, hipStream_t st) {
     if (P.n_groups > static_cast<u32>(kMaxRow)) return false;
-    if (R.dim == 0) return true;
-    const u64 need = blocks_for(static_cast<u64>(R.dim), 1024);
+    if (n_rows <= 0) return true;
+    const u64 need = blocks_for(static_cast<u64>(n_rows), 1024);
     const dim3 grid(static_cast<unsigned>(need < kGatherLdsBlocks ? need : kGatherLdsBlocks)), block(1024);
     const bool slc = C.slc_L > 0, narrow = C.n_sites <= 32;
     const std::size_t bytes = !C.nib ? kLdsActionMax + 1 : narrow ? lds_action_bytes<u32>(C) : lds_action_bytes<u64>(C);
-#define QSA_EMIT(S, L, W, B) \
-    hipLaunchKernelGGL((k_csr_emit<S, L, W>), grid, block, B, st, R, C, same ? 1 : 0, P, row_len, stage_col, stage_val)
+#define QSA_EMIT(S, L, W, B)                                                                                           \
+    hipLaunchKernelGGL((k_csr_emit<S, L, W>), grid, block, B, st, R, C, same ? 1 : 0, P, row0, n_rows, row_len,     \
+                       stage_col, stage_val)
     if (bytes <= kLdsActionMax) {
         if (slc && narrow) QSA_EMIT(true, true, u32, bytes);
         else if (slc) QSA_EMIT(true, true, u64, bytes);
@@ -754,11 +628,15 @@ bool csr_walk(const SectorView& R, const SectorView& C, bool same, const Program
         QSA_EMIT(false, false, u64, 0);
     }
 #undef QSA_EMIT
-    const dim3 sgrid(blocks_for(static_cast<u64>(R.dim) * 64, 256)), sblock(256);
-    if (P.n_groups <= 64)
-        hipLaunchKernelGGL(k_csr_sort<1>, sgrid, sblock, 0, st, R.dim, row_len, stage_col, stage_val);
-    else
-        hipLaunchKernelGGL(k_csr_sort<2>, sgrid, sblock, 0, st, R.dim, row_len, stage_col, stage_val);
+    const dim3 sgrid(blocks_for(static_cast<u64>(n_rows) * 64, 256)), sblock(256);
+    const bool short_cols = C.dim < (i64{1} << 25);
+#define QSA_SORT(E, K) \
+    hipLaunchKernelGGL((k_csr_sort<E, K>), sgrid, sblock, 0, st, n_rows, row_len, stage_col, stage_val)
+    if (P.n_groups <= 64 && short_cols) QSA_SORT(1, u32);
+    else if (P.n_groups <= 64) QSA_SORT(1, u64);
+    else if (short_cols) QSA_SORT(2, u32);
+    else QSA_SORT(2, u64);
+#undef QSA_SORT
     return true;
 }
 
@@ -791,40 +669,6 @@ void walk_gather(const SectorView& R, const SectorView& C, bool same, const Prog
     else launch_gather<false, false>(R, C, same, P, v, st);
 }
 
-bool csr_count(const SectorView& R, const SectorView& C, bool same, const ProgramView& P, u32* row_len,
-               const DictTable& D, bool use_dict, hipStream_t st) {
-    if (P.n_groups > static_cast<u32>(kMaxRow)) return false;
-    if (R.dim == 0) return true;
-    const dim3 grid(blocks_for(static_cast<u64>(R.dim), 256)), block(256);
-    if (C.slc_L > 0)
-        hipLaunchKernelGGL(k_csr_count<true>, grid, block, 0, st, R, C, same ? 1 : 0, P, row_len, D, use_dict ? 1 : 0);
-    else
-        hipLaunchKernelGGL(k_csr_count<false>, grid, block, 0, st, R, C, same ? 1 : 0, P, row_len, D,
-                           use_dict ? 1 : 0);
-    return true;
-}
-
-void csr_fill(const SectorView& R, const SectorView& C, bool same, const ProgramView& P, const u64* row_ptr,
-              u32* col, void* vals, int width, bool real, const DictTable& D, hipStream_t st) {
-    if (R.dim == 0) return;
-    const dim3 grid(blocks_for(static_cast<u64>(R.dim), 256)), block(256);
-    const int sm = same ? 1 : 0;
-    const bool slc = C.slc_L > 0;
-#define QSA_FILL(SL, W, RE) \
-    hipLaunchKernelGGL((k_csr_fill<SL, W, RE>), grid, block, 0, st, R, C, sm, P, row_ptr, col, vals, D)
-    if (slc) {
-        if (width == 1) QSA_FILL(true, 1, false);
-        else if (width == 2) QSA_FILL(true, 2, false);
-        else if (real) QSA_FILL(true, 0, true);
-        else QSA_FILL(true, 0, false);
-    } else {
-        if (width == 1) QSA_FILL(false, 1, false);
-        else if (width == 2) QSA_FILL(false, 2, false);
-        else if (real) QSA_FILL(false, 0, true);
-        else QSA_FILL(false, 0, false);
-    }
-#undef QSA_FILL
-}
 
 namespace {
 struct ToU64 {

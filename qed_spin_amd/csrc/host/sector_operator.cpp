@@ -1,6 +1,7 @@
 #include "sector_operator.h"
 
 #include <chrono>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -57,31 +58,43 @@ void SectorOperator::build_csr(std::uint64_t budget) {
     D.flags = dcnt.get<unsigned>() + 1;
     QSA_HIP(hipMemsetAsync(dfp.get(), 0, dfp.bytes(), st));
     QSA_HIP(hipMemsetAsync(dcnt.get(), 0, 8, st));
-    // One walk into a stage of (acting groups) slots per row when it fits -- at most 16 GiB and a
-    // quarter of the free device memory -- else two walks (count, then fill).
-    const std::uint64_t stage_bytes = static_cast<std::uint64_t>(rows) * P.n_groups * (4 + sizeof(dev::Zd));
+    // The walk stages (acting groups) slots per row. Rows go in chunks whose stage fits at most
+    // 16 GiB and a quarter of the free device memory: with one chunk the CSR is compacted from the
+    // stage the walk left; with several, a first pass over the chunks gives the row lengths and the
+    // dictionary, a second walks them again and compacts each into place.
+    const std::uint64_t slot_bytes = static_cast<std::uint64_t>(std::max<std::uint32_t>(P.n_groups, 1)) *
+                                     (4 + sizeof(dev::Zd));
     std::size_t free_b = 0, total_b = 0;
     QSA_HIP(hipMemGetInfo(&free_b, &total_b));
-    const bool staged = stage_bytes <= (std::uint64_t{16} << 30) && stage_bytes <= free_b / 4;
-    DeviceBuffer stage_col, stage_val;
-    if (staged) {
-        stage_col.reset(std::max<std::uint64_t>(static_cast<std::uint64_t>(rows) * P.n_groups, 1) * 4);
-        stage_val.reset(std::max<std::uint64_t>(static_cast<std::uint64_t>(rows) * P.n_groups, 1) * sizeof(dev::Zd));
-        if (!dev::csr_walk(R, C, same(), P, row_len.get<std::uint32_t>(), stage_col.get<std::uint32_t>(),
-                           stage_val.get<dev::Zd>(), st))
-            return;   // too many groups
-        dev::csr_stage_dict(rows, row_len.get<std::uint32_t>(), stage_val.get<dev::Zd>(), D, st);
-    } else if (!dev::csr_count(R, C, same(), P, row_len.get<std::uint32_t>(), D, true, st)) {
-        return;   // too many groups
+    std::uint64_t cap = std::min<std::uint64_t>(std::uint64_t{16} << 30, free_b / 4);
+    if (const char* e = std::getenv("QED_SPIN_AMD_CSR_STAGE_BYTES")) {   // a lower cap (tests, tight memory)
+        const unsigned long long v = std::strtoull(e, nullptr, 10);
+        if (v > 0) cap = std::min<std::uint64_t>(cap, v);
+    }
+    const std::int64_t chunk = std::min<std::int64_t>(rows, static_cast<std::int64_t>(cap / slot_bytes));
+    if (chunk < std::min<std::int64_t>(rows, 256)) return;   // no room to stage: the gather lane
+    const std::int64_t n_chunks = (rows + chunk - 1) / chunk;
+    DeviceBuffer stage_col(static_cast<std::size_t>(chunk) * std::max<std::uint32_t>(P.n_groups, 1) * 4);
+    DeviceBuffer stage_val(static_cast<std::size_t>(chunk) * std::max<std::uint32_t>(P.n_groups, 1) *
+                           sizeof(dev::Zd));
+    auto walk = [&](std::int64_t c) {
+        const std::int64_t r0 = c * chunk, n = std::min<std::int64_t>(chunk, rows - r0);
+        return dev::csr_walk(R, C, same(), P, r0, n, row_len.get<std::uint32_t>() + r0, stage_col.get<std::uint32_t>(),
+                             stage_val.get<dev::Zd>(), st);
+    };
+    for (std::int64_t c = 0; c < n_chunks; ++c) {
+        if (!walk(c)) return;   // too many groups
+        const std::int64_t r0 = c * chunk, n = std::min<std::int64_t>(chunk, rows - r0);
+        dev::csr_stage_dict(n, row_len.get<std::uint32_t>() + r0, stage_val.get<dev::Zd>(), D, st);
     }
     QSA_HIP(hipGetLastError());
     auto fill = [&](int w) {
-        if (staged)
-            dev::csr_compact(rows, row_ptr_.get<std::uint64_t>(), stage_col.get<std::uint32_t>(),
+        for (std::int64_t c = 0; c < n_chunks; ++c) {
+            const std::int64_t r0 = c * chunk, n = std::min<std::int64_t>(chunk, rows - r0);
+            if (n_chunks > 1) walk(c);
+            dev::csr_compact(n, row_ptr_.get<std::uint64_t>() + r0, stage_col.get<std::uint32_t>(),
                              stage_val.get<dev::Zd>(), col_.get<std::uint32_t>(), vals_.get(), w, real_, D, st);
-        else
-            dev::csr_fill(R, C, same(), P, row_ptr_.get<std::uint64_t>(), col_.get<std::uint32_t>(), vals_.get(), w,
-                          real_, D, st);
+        }
     };
     row_ptr_.reset(static_cast<std::size_t>(rows + 1) * 8);
     const std::size_t tb = dev::row_scan_temp_bytes(rows);

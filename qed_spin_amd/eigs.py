@@ -343,7 +343,8 @@ def _stream_levels(H: Operator, k: int, sym: Symmetry, *, floor: int, prune: boo
     sector, its operator and (above the outright size) its pruning estimate -- and frees it; the
     blocks solved outright are then built again together and solved as walk_levels solves them, and
     the candidates, by increasing estimate, are built again one by one when not pruned. Returns
-    ([(block, dim)], {index: (levels, converged, lane)}, pruned, {}, build seconds)."""
+    ([(block, dim)], {index: (levels, converged, lane)}, pruned, {}, build seconds, {the candidates'
+    rebuild and solve seconds})."""
     t0 = time.perf_counter()
     blks = sym.blocks(H.num_sites)
     H._handle()
@@ -378,6 +379,7 @@ def _stream_levels(H: Operator, k: int, sym: Symmetry, *, floor: int, prune: boo
     t_build = time.perf_counter() - t0
     out = [j for j in range(len(meta)) if j not in est and j not in mirror]
     levels = {}
+    split = {"rebuild_s": 0.0, "rebuild_sector_s": 0.0, "rebuild_csr_s": 0.0, "candidate_solve_s": 0.0}
     if out:
         built = _build_blocks(H, [meta[j][0] for j in out], csr_budget, tune=False)
         lv_out, _ = walk_levels(built, k, floor=floor, prune=False, max_iter=max_iter)
@@ -403,10 +405,18 @@ def _stream_levels(H: Operator, k: int, sym: Symmetry, *, floor: int, prune: boo
             pruned += 1
             continue
         blk, dim = meta[j]
-        (_, sec, m), = _build_blocks(H, [blk], csr_budget, tune=True)
+        tb = time.perf_counter()
+        sec = Sector(blk.group, blk.n_up)
+        tm = time.perf_counter()
+        m = SectorMatvec(H, sec, lane=LANE_CSR, csr_budget=csr_budget, tune=True)
+        ts = time.perf_counter()
         levels[j] = _solve_block(m, dim, min(k, dim), floor, max_iter)
+        split["rebuild_s"] += ts - tb
+        split["rebuild_sector_s"] += tm - tb
+        split["rebuild_csr_s"] += float(m.info.get("build_seconds", 0.0))
+        split["candidate_solve_s"] += time.perf_counter() - ts
         del sec, m
-    return meta, levels, pruned, {}, t_build
+    return meta, levels, pruned, {}, t_build, split
 
 
 def _tower(H: Operator, k: int, sym: Symmetry, two_S: int, **kw) -> EigsResult:
@@ -478,9 +488,9 @@ def eigs(H: Operator, k: int = 4, *, sym: Optional[Symmetry] = None, device: str
     if stream is None:
         stream = _resident_bytes(H.num_sites, sym) > 0.5 * _lib.device_memory()[0]
     if stream:
-        meta, levels, pruned, blocks, t_build = _stream_levels(H, k, sym, floor=floor, prune=prune,
-                                                               prune_margin=prune_margin, window=window,
-                                                               max_iter=max_iter, csr_budget=csr_budget)
+        meta, levels, pruned, blocks, t_build, split = _stream_levels(H, k, sym, floor=floor, prune=prune,
+                                                                      prune_margin=prune_margin, window=window,
+                                                                      max_iter=max_iter, csr_budget=csr_budget)
         t1 = t0 + t_build
     else:
         # operators built on first use: time-reversal mirrors and pruned blocks' solves need none
@@ -493,6 +503,7 @@ def eigs(H: Operator, k: int = 4, *, sym: Optional[Symmetry] = None, device: str
         t1 = time.perf_counter()
         levels, pruned = walk_levels(blocks, k, floor=floor, prune=prune, prune_margin=prune_margin, window=window,
                                      norm_bound=H.norm_bound(), max_iter=max_iter, partners=partners)
+        split = {}
     t_solve = time.perf_counter() - t1
     recs, converged = [], True
     for i, (blk, dim) in enumerate(meta):
@@ -525,7 +536,7 @@ def eigs(H: Operator, k: int = 4, *, sym: Optional[Symmetry] = None, device: str
     return EigsResult(energies=energies, blocks=recs, converged=converged,
                       timings={"build_s": t_build, "solve_s": t_solve, "pruned_blocks": pruned,
                                "vectors_s": time.perf_counter() - t2 if vectors else 0.0,
-                               "total_s": time.perf_counter() - t0},
+                               "total_s": time.perf_counter() - t0, **split},
                       k=k, n_sites=H.num_sites, levels=lvls, _bases=bases, _vectors=vecs)
 
 

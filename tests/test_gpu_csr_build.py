@@ -90,3 +90,31 @@ def test_exact_cancellations_dropped(N, k):
     _, _, nnz_minus = check(cancelling(N, -1.0), g, N // 2)
     _, _, nnz_plus = check(cancelling(N, +1.0), g, N // 2)
     assert nnz_minus < nnz_plus   # columns met and cancelled to exact zeros, which are dropped
+
+
+def j123(N):
+    """Heisenberg couplings at distances 1, 2 and 3 (3 N bonds: rows of up to 3 N + 1 entries)."""
+    b = qs.HamiltonianBuilder(N)
+    for d, J in ((1, 1.0), (2, 0.45), (3, -0.2)):
+        b.heisenberg([(i, (i + d) % N) for i in range(N)], J)
+    return b.to_operator()
+
+
+@pytest.mark.parametrize("model,stage_bytes", [("cancelling", 100_000), ("j123", 600_000)])
+def test_chunked_build_equals_one_walk(model, stage_bytes, monkeypatch):
+    """A stage too small for the whole sector: the rows go in chunks (two passes: lengths and
+    dictionary, then the walk again and the compaction into place). The CSR equals the one-chunk
+    build bit for bit, and the oracle's."""
+    say("csr chunked", model, stage_bytes)
+    N = 20
+    g = sym.chain_group(N, 0)
+    s = qs.Sector(g, N // 2)
+    H = cancelling(N) if model == "cancelling" else j123(N)
+    groups = qs.Program(H).counts()[0]
+    assert s.dim * groups * 20 > 4 * stage_bytes   # several chunks
+    one = qs.SectorMatvec(H, s, lane=qs.LANE_CSR).export_csr()
+    monkeypatch.setenv("QED_SPIN_AMD_CSR_STAGE_BYTES", str(stage_bytes))
+    chunked = qs.SectorMatvec(H, s, lane=qs.LANE_CSR).export_csr()
+    for x, y in zip(one, chunked):
+        assert np.array_equal(np.asarray(x).view(np.uint8), np.asarray(y).view(np.uint8))
+    check(H, g, N // 2)
