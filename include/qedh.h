@@ -340,8 +340,10 @@ int qedh_batch_lanczos(const qedh_matvec* blocks, int n_blocks, int R, int krylo
 /* FTLM of n_blocks blocks at once: block b samples s = 0..samples-1 from
  * splitmix64(block_seeds[b] + phi (s + 1)) as qedh_ftlm does, breakdown_tol[b] per block (NULL:
  * 64 eps times the block's norm bound, kBreakdownRel of include/ed/core/numerics.h:23).
- * lnZ/E/V[n_blocks][n_beta], e_min[n_blocks]. stats[8] (may be NULL) = pack s, seed s, steps s,
- * post s, total s, block-vector applies, packed rows, packed nnz. */
+ * lnZ/E/V[n_blocks][n_beta], e_min[n_blocks]. stats[10] (may be NULL) = pack s, seed s, steps s,
+ * post s, total s, block-vector applies, packed rows, packed nnz, the tile schedule the engine's
+ * autotuner chose (0 in order, 1 grouped by XCD; results are the same bit for bit under either),
+ * its tuning s. QED_SPIN_AMD_BATCH_SCHEDULE=0|1 pins the schedule. */
 int qedh_ftlm_batched(const qedh_matvec* blocks, int n_blocks, int samples, int krylov, const uint64_t* block_seeds,
                       const double* breakdown_tol, int n_beta, const double* betas, int use_graph, double* lnZ,
                       double* E, double* V, double* e_min, double* stats);
@@ -351,7 +353,8 @@ int qedh_ftlm_batched(const qedh_matvec* blocks, int n_blocks, int samples, int 
  * (NULL: no pair); Lanczos without reorthogonalisation, krylov steps, breakdown_tol (<= 0: 1e-12);
  * shift = energy_shift when has_shift, else each pair's lowest Ritz value. Outputs
  * S[n_blocks R][n_omega] (weighted by ||phi||^2), norm_sq, shifts, steps (each may be NULL but S),
- * stats[7] = pack s, steps s, host s, total s, applies, rows, nnz (may be NULL). */
+ * stats[9] = pack s, steps s, host s, total s, applies, rows, nnz, tile schedule, tuning s (may be
+ * NULL). */
 int qedh_cf_batched(const qedh_matvec* blocks, int n_blocks, int R, const void* const* starts, int krylov,
                     int has_shift, double energy_shift, double breakdown_tol, int n_omega, const double* omega,
                     double eta, int use_graph, double* S, double* norm_sq, double* shifts, int* steps, double* stats);

@@ -15,9 +15,9 @@
 // other blocks share the batch.
 //
 // SpMV: for R >= 8 one thread per (row, vector), the R threads of a row reading the same column and
-// value and gathering one contiguous line of R entries; for 2 <= R < 8, 16 threads per row (R
-// rounded up to a power of two, 16 / that many lanes per (row, vector)) splitting the row's entries,
-// summed by a fixed xor butterfly; R = 1 is the sector SpMV of spmv.hip on the packed CSR.
+// value and gathering one contiguous line of R entries; for R < 8, 16 threads per row (R rounded
+// up to a power of two, 16 / that many lanes per (row, vector)) splitting the row's entries, summed
+// by a fixed xor butterfly.
 //
 // The Lanczos scalars alpha, beta, the local-reorthogonalisation overlap, the active flags and the
 // step counter live on the device; a step needs no host synchronisation, so it is captured once
@@ -34,6 +34,10 @@ constexpr int kThreads = 256;
 __device__ __forceinline__ Zd cmul(Zd a, Zd b) { return Zd{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
 __device__ __forceinline__ Zd cdot(Zd a, Zd b) { return Zd{a.re * b.re + a.im * b.im, a.re * b.im - a.im * b.re}; }
 
+/// The tile this workgroup runs: the schedule's entry (a permutation of the tiles), else its own
+/// index. Every tile's work and partial sums are the same whichever workgroup runs it.
+__device__ __forceinline__ u32 tile_id(const BatchView& B) { return B.sched ? B.sched[blockIdx.x] : blockIdx.x; }
+
 __device__ __forceinline__ bool pair_live(const BatchView& B, u32 block, int s) {
     return !B.active || B.active[static_cast<u64>(block) * B.R + s] != 0;
 }
@@ -48,7 +52,7 @@ __device__ __forceinline__ Zd entry_value(const BatchView& B, u64 e) {
 
 // One thread per (row, vector), R >= 8.
 __global__ __launch_bounds__(kThreads) void k_batch_spmv_wide(BatchView B, const Zd* X, Zd* Y) {
-    const Tile t = B.tiles[blockIdx.x];
+    const Tile t = B.tiles[tile_id(B)];
     const int R = B.R, slots = kThreads / R;
     const int slot = threadIdx.x / R, s = threadIdx.x % R;
     if (slot >= slots || !pair_live(B, t.block, s)) return;
@@ -66,15 +70,16 @@ __global__ __launch_bounds__(kThreads) void k_batch_spmv_wide(BatchView B, const
     }
 }
 
-// 16 threads per row for R < 8: RP = R rounded up to a power of two, LR = 16 / RP lanes per
-// (row, vector); lane l of a vector takes entries e0 + l, e0 + l + LR, ...
-template <int RP>
+// R < 8: RP = R rounded up to a power of two, LR = 16 / RP lanes per (row, vector), 16 threads per
+// row; lane l of a vector takes entries e0 + l, e0 + l + LR, ... (four lanes per (row, vector) at
+// R = 1 measured the same on the N = 28 continued fractions).
+template <int RP, int LR>
 __global__ __launch_bounds__(kThreads) void k_batch_spmv_narrow(BatchView B, const Zd* X, Zd* Y) {
-    constexpr int LR = 16 / RP;
-    constexpr int slots = kThreads / 16;
-    const Tile t = B.tiles[blockIdx.x];
+    constexpr int TPR = RP * LR;
+    constexpr int slots = kThreads / TPR;
+    const Tile t = B.tiles[tile_id(B)];
     const int R = B.R;
-    const int slot = threadIdx.x / 16, w = threadIdx.x % 16;
+    const int slot = threadIdx.x / TPR, w = threadIdx.x % TPR;
     const int s = w / LR, lane = w % LR;
     const bool live = s < R && pair_live(B, t.block, s);
     // every thread of a row group runs the same rounds (the butterfly needs all of them)
@@ -110,14 +115,14 @@ __device__ __forceinline__ void tile_partial(const BatchView& B, double pr, doub
     si[threadIdx.x] = pi;
     __syncthreads();
     if (static_cast<int>(threadIdx.x) < R &&
-        (!only || only[static_cast<u64>(B.tiles[blockIdx.x].block) * R + threadIdx.x])) {
+        (!only || only[static_cast<u64>(B.tiles[tile_id(B)].block) * R + threadIdx.x])) {
         double a = 0.0, b = 0.0;
         for (int q = 0; q < slots; ++q) {
             a += sr[q * R + threadIdx.x];
             b += si[q * R + threadIdx.x];
         }
-        partial[(static_cast<u64>(blockIdx.x) * 64 + threadIdx.x) * 2] = a;
-        partial[(static_cast<u64>(blockIdx.x) * 64 + threadIdx.x) * 2 + 1] = b;
+        partial[(static_cast<u64>(tile_id(B)) * 64 + threadIdx.x) * 2] = a;
+        partial[(static_cast<u64>(tile_id(B)) * 64 + threadIdx.x) * 2 + 1] = b;
     }
 }
 
@@ -134,7 +139,7 @@ __device__ __forceinline__ Zd scaled(Zd v, double f) { return Zd{v.re * f, v.im 
 __global__ __launch_bounds__(kThreads) void k_batch_spmv_wide_fused(BatchView B, const Zd* Vprev, const Zd* Vcur,
                                                                     Zd* W, const double* beta, double* partial,
                                                                     const double* sprev, const double* scur) {
-    const Tile t = B.tiles[blockIdx.x];
+    const Tile t = B.tiles[tile_id(B)];
     const int R = B.R, slots = kThreads / R;
     const int slot = threadIdx.x / R, s = threadIdx.x % R;
     const bool live = slot < slots && pair_live(B, t.block, s);
@@ -181,7 +186,7 @@ __global__ __launch_bounds__(kThreads) void k_batch_spmv_wide_fused(BatchView B,
 __global__ __launch_bounds__(kThreads) void k_batch_vec(BatchView B, int mode, const Zd* Vprev, const Zd* Vcur, Zd* W,
                                                         const double* coef, const Zd* zcoef, double threshold,
                                                         double* partial, double* partial2, const double* vscale) {
-    const Tile t = B.tiles[blockIdx.x];
+    const Tile t = B.tiles[tile_id(B)];
     const int R = B.R, slots = kThreads / R;
     const int slot = threadIdx.x / R, s = threadIdx.x % R;
     const u64 bs = static_cast<u64>(t.block) * R + s;
@@ -324,7 +329,7 @@ __global__ __launch_bounds__(kThreads) void k_batch_finish(BatchView B, int kind
 
 // ||V||^2 per pair, then V scaled by 1 / ||V||: the normalisation of the start vectors.
 __global__ __launch_bounds__(kThreads) void k_batch_norm_partial(BatchView B, const Zd* V, double* partial) {
-    const Tile t = B.tiles[blockIdx.x];
+    const Tile t = B.tiles[tile_id(B)];
     const int R = B.R, slots = kThreads / R;
     const int slot = threadIdx.x / R, s = threadIdx.x % R;
     double p = 0.0;
@@ -348,7 +353,7 @@ __global__ __launch_bounds__(kThreads) void k_batch_norm_finish(BatchView B, con
 }
 
 __global__ __launch_bounds__(kThreads) void k_batch_scale(BatchView B, Zd* V, const double* inv) {
-    const Tile t = B.tiles[blockIdx.x];
+    const Tile t = B.tiles[tile_id(B)];
     const int R = B.R, slots = kThreads / R;
     const int slot = threadIdx.x / R, s = threadIdx.x % R;
     if (slot >= slots) return;
@@ -496,7 +501,7 @@ __global__ __launch_bounds__(kThreads) void k_batch_gs_coef(BatchView B, const Z
 // W -= sum_{k < K} coef_k V_k on the tiles of the flagged blocks, k ascending.
 __global__ __launch_bounds__(kThreads) void k_batch_gs_update(BatchView B, const Zd* basis, u64 layer, int K, int kmax,
                                                               Zd* W, const int* full, const int* on, const Zd* coef) {
-    const Tile t = B.tiles[blockIdx.x];
+    const Tile t = B.tiles[tile_id(B)];
     if (!gs_block(B, t.block, full, on)) return;
     const int R = B.R, slots = kThreads / R;
     const int slot = threadIdx.x / R, s = threadIdx.x % R;
@@ -670,10 +675,10 @@ void batch_spmv(const BatchView& B, const Zd* X, Zd* Y, hipStream_t st) {
     }
     const dim3 grid(B.n_tiles), block(kThreads);
     if (B.R >= 8) hipLaunchKernelGGL(k_batch_spmv_wide, grid, block, 0, st, B, X, Y);
-    else if (B.R == 1) hipLaunchKernelGGL(k_batch_spmv_narrow<1>, grid, block, 0, st, B, X, Y);
-    else if (B.R == 2) hipLaunchKernelGGL(k_batch_spmv_narrow<2>, grid, block, 0, st, B, X, Y);
-    else if (B.R <= 4) hipLaunchKernelGGL(k_batch_spmv_narrow<4>, grid, block, 0, st, B, X, Y);
-    else hipLaunchKernelGGL(k_batch_spmv_narrow<8>, grid, block, 0, st, B, X, Y);
+    else if (B.R == 1) hipLaunchKernelGGL((k_batch_spmv_narrow<1, 16>), grid, block, 0, st, B, X, Y);
+    else if (B.R == 2) hipLaunchKernelGGL((k_batch_spmv_narrow<2, 8>), grid, block, 0, st, B, X, Y);
+    else if (B.R <= 4) hipLaunchKernelGGL((k_batch_spmv_narrow<4, 4>), grid, block, 0, st, B, X, Y);
+    else hipLaunchKernelGGL((k_batch_spmv_narrow<8, 2>), grid, block, 0, st, B, X, Y);
 }
 
 bool batch_spmv_fused(const BatchView& B, const Zd* Vprev, const Zd* Vcur, Zd* W, const double* beta, double* partial,

@@ -202,6 +202,7 @@ struct BatchView {
     u32 n_blocks = 0, n_tiles = 0;
     int R = 0;                          // vectors per block (<= 64), X[row R + s]
     const Tile* tiles = nullptr;        // n_tiles, blocks contiguous
+    const u32* sched = nullptr;         // n_tiles: workgroup w runs tile sched[w] (null: tile w)
     const u32* tile_begin = nullptr;    // n_blocks + 1
     const u64* row_ptr = nullptr;       // packed block-diagonal CSR
     const u32* col = nullptr;

@@ -4,6 +4,7 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <thread>
 
 #include "lanczos.h"
@@ -165,6 +166,81 @@ BatchedLanczos::BatchedLanczos(std::vector<std::shared_ptr<const SectorOperator>
     QSA_HIP(hipStreamSynchronize(st));
 }
 
+void BatchedLanczos::set_schedule(int kind) {
+    sched_kind_ = 0;
+    sched_d_.release();
+    if (kind != 1 || tiles_.empty()) return;
+    // blocks to XCDs, largest first onto the XCD with the fewest rows so far; workgroup w runs on XCD
+    // w % 8, so it takes the next tile of that XCD's blocks (of the longest remaining queue when that
+    // XCD has none left)
+    constexpr int kXcd = 8;
+    std::vector<std::size_t> order(ops_.size());
+    for (std::size_t b = 0; b < order.size(); ++b) order[b] = b;
+    std::stable_sort(order.begin(), order.end(), [&](std::size_t a, std::size_t b) { return dim(a) > dim(b); });
+    std::uint64_t load[kXcd] = {};
+    std::vector<std::uint32_t> queue[kXcd];
+    for (const std::size_t b : order) {
+        int x = 0;
+        for (int y = 1; y < kXcd; ++y)
+            if (load[y] < load[x]) x = y;
+        load[x] += dim(static_cast<int>(b));
+        for (std::uint32_t t = tile_begin_[b]; t < tile_begin_[b + 1]; ++t) queue[x].push_back(t);
+    }
+    std::size_t head[kXcd] = {};
+    std::vector<std::uint32_t> sched(tiles_.size());
+    for (std::size_t w = 0; w < sched.size(); ++w) {
+        int x = static_cast<int>(w % kXcd);
+        if (head[x] == queue[x].size()) {
+            for (int y = 0; y < kXcd; ++y)
+                if (queue[y].size() - head[y] > queue[x].size() - head[x]) x = y;
+        }
+        sched[w] = queue[x][head[x]++];
+    }
+    put(sched_d_, sched, engine_stream());
+    QSA_HIP(hipStreamSynchronize(engine_stream()));
+    sched_kind_ = 1;
+}
+
+int BatchedLanczos::tune_schedule() {
+    if (const char* e = std::getenv("QED_SPIN_AMD_BATCH_SCHEDULE")) {   // pinned: 0 or 1
+        set_schedule(std::atoi(e) == 1 ? 1 : 0);
+        return sched_kind_;
+    }
+    if (tiles_.size() < 4096) {
+        set_schedule(0);
+        return 0;
+    }
+    // one warm-up apply, then the two schedules alternately (0, 1, 0, 1), each apply timed
+    const hipStream_t st = engine_stream();
+    hipEvent_t ev[5];
+    for (auto& e : ev) QSA_HIP(hipEventCreate(&e));
+    set_schedule(1);
+    const std::uint32_t* grouped = sched_d_.get<std::uint32_t>();
+    DeviceBuffer keep = std::move(sched_d_);
+    dev::BatchView B = packed_view();
+    B.sched = nullptr;
+    dev::batch_spmv(B, V1_.get<dev::Zd>(), W_.get<dev::Zd>(), st);
+    QSA_HIP(hipEventRecord(ev[0], st));
+    for (int i = 0; i < 4; ++i) {
+        B.sched = (i & 1) ? grouped : nullptr;
+        dev::batch_spmv(B, V1_.get<dev::Zd>(), W_.get<dev::Zd>(), st);
+        QSA_HIP(hipEventRecord(ev[i + 1], st));
+    }
+    QSA_HIP(hipEventSynchronize(ev[4]));
+    float ms[2] = {0.0f, 0.0f};
+    for (int i = 0; i < 4; ++i) {
+        float m = 0.0f;
+        QSA_HIP(hipEventElapsedTime(&m, ev[i], ev[i + 1]));
+        ms[i & 1] += m;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    const int best = ms[1] < ms[0] ? 1 : 0;
+    dev::batch_fill_zero(W_.get<dev::Zd>(), rows_ * R_, st);
+    QSA_HIP(hipStreamSynchronize(st));
+    set_schedule(best);
+    return best;
+}
+
 dev::BatchView BatchedLanczos::view() const {
     dev::BatchView B;
     B.rows = rows_;
@@ -172,6 +248,7 @@ dev::BatchView BatchedLanczos::view() const {
     B.n_tiles = static_cast<std::uint32_t>(tiles_.size());
     B.R = R_;
     B.tiles = tiles_d_.get<dev::Tile>();
+    B.sched = sched_d_ ? sched_d_.get<std::uint32_t>() : nullptr;
     B.tile_begin = tile_begin_d_.get<std::uint32_t>();
     B.row_ptr = row_ptr_.get<std::uint64_t>();
     B.col = col_.get<std::uint32_t>();
@@ -486,6 +563,9 @@ std::vector<Curves> batched_ftlm(const std::vector<std::shared_ptr<const SectorO
     const int R = std::min(o.samples, dev::kBatchMaxR);
     BatchedLanczos L(blocks, R);
     t.pack_s += secs(t_pack, clock_t_::now());
+    const auto t_tune = clock_t_::now();
+    t.schedule = L.tune_schedule();
+    t.tune_s += secs(t_tune, clock_t_::now());
     std::vector<RitzPool> pools(nb);
     std::vector<double> alpha, beta;
     std::vector<int> steps;

@@ -20,6 +20,8 @@ struct BatchedTimings {
     std::uint64_t rows = 0, nnz = 0;
     int steps = 0;
     bool graph = false;
+    int schedule = 0;               ///< the tile schedule the autotuner chose (BatchedLanczos::set_schedule)
+    double tune_s = 0.0;
 };
 
 /// The packed engine: blocks (square sector operators on the CSR lane) x R complex vectors.
@@ -67,6 +69,15 @@ public:
     void apply(const cplx* X, cplx* Y) const;
     /// The start vectors (rows() x R, device): written by the caller, then normalize().
     cplx* start_buffer() { return V0_.get<cplx>(); }
+    /// Which workgroup runs which tile: 0 in order, 1 grouped by XCD (the tiles of a block on one
+    /// XCD, blocks spread over the XCDs by rows, so a block's gathers share that XCD's L2). Every
+    /// schedule gives the same results bit for bit: a tile's work and partial sums do not depend on
+    /// the workgroup that runs it.
+    void set_schedule(int kind);
+    int schedule() const { return sched_kind_; }
+    /// The schedule of the faster packed matvec, timed over a few applies of each (the engine's
+    /// autotuner); returns the chosen kind. Engines of fewer than 4096 tiles keep schedule 0.
+    int tune_schedule();
     /// The packed layout as the kernels read it (tiles, R); every pair live.
     dev::BatchView packed_view() const {
         dev::BatchView B = view();
@@ -85,6 +96,8 @@ private:
     DeviceBuffer row_ptr_, col_, val_, vid_, dict_re_, dict_im_, tiles_d_, tile_begin_d_, cap_d_, active_d_, steps_d_,
         step_d_;
     std::uint32_t dict_n_ = 0;
+    int sched_kind_ = 0;
+    DeviceBuffer sched_d_;
     DeviceBuffer V0_, V1_, W_, partial_, partial2_, fix_, scale_, alpha_h_, beta_h_, ov_, cur_a_, cur_b_, inv_b_,
         tol_d_;
     friend struct BatchedAccess;

@@ -124,6 +124,8 @@ extern "C" int qedh_ftlm_batched(const qedh_matvec* blocks, int n_blocks, int sa
             stats[5] = static_cast<double>(t.applies);
             stats[6] = static_cast<double>(t.rows);
             stats[7] = static_cast<double>(t.nnz);
+            stats[8] = static_cast<double>(t.schedule);
+            stats[9] = t.tune_s;
         }
     });
 }
@@ -261,6 +263,9 @@ extern "C" int qedh_cf_batched(const qedh_matvec* blocks, int n_blocks, int R, c
         const auto t0 = std::chrono::steady_clock::now();
         qsa::BatchedTimings t;
         qsa::BatchedLanczos L(ops, R);
+        const auto t_tune = std::chrono::steady_clock::now();
+        t.schedule = L.tune_schedule();
+        t.tune_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_tune).count();
         L.clear_start();
         const std::size_t pairs = static_cast<std::size_t>(n_blocks) * static_cast<std::size_t>(R);
         for (int b = 0; b < n_blocks; ++b)
@@ -308,6 +313,8 @@ extern "C" int qedh_cf_batched(const qedh_matvec* blocks, int n_blocks, int R, c
             stats[4] = static_cast<double>(t.applies);
             stats[5] = static_cast<double>(t.rows);
             stats[6] = static_cast<double>(t.nnz);
+            stats[7] = static_cast<double>(t.schedule);
+            stats[8] = t.tune_s;
         }
     });
 }

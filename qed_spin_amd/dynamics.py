@@ -321,7 +321,7 @@ def dynamics(H: Operator, probes, omega, *, eta: float = 0.1, T=None, sym: Optio
                 owner.append((ti * R + s, p))
         out = np.zeros((len(targets) * R, omega.size))
         nsq = np.zeros(len(targets) * R)
-        stats = np.zeros(7)
+        stats = np.zeros(9)
         _lib.call("qedh_cf_batched", _handles(mats), len(targets), int(R), starts, int(max(krylov, 2)), 1,
                   C.c_double(e0), C.c_double(KBREAKDOWN_REL * sH), int(omega.size), _lib.ptr(omega, C.c_double),
                   C.c_double(eta), int(bool(use_graph)), _lib.ptr(out, C.c_double), _lib.ptr(nsq, C.c_double),
@@ -329,7 +329,8 @@ def dynamics(H: Operator, probes, omega, *, eta: float = 0.1, T=None, sym: Optio
         for i, p in owner:
             S[p] += out[i]
             norms[p] += nsq[i]
-        engine = dict(zip(["pack_s", "steps_s", "host_s", "total_s", "applies", "rows", "nnz"], stats.tolist()))
+        engine = dict(zip(["pack_s", "steps_s", "host_s", "total_s", "applies", "rows", "nnz", "schedule", "tune_s"],
+                          stats.tolist()))
     for p in range(len(probes)):
         S[p] /= float(len(states))
         norms[p] /= float(len(states))

@@ -91,12 +91,12 @@ def ftlm_batched(mats: Sequence[SectorMatvec], samples: int, krylov: int, seeds:
     tol = None if breakdown_tol is None else np.ascontiguousarray(breakdown_tol, np.float64)
     lnZ, E, V = np.zeros((nb, nT)), np.zeros((nb, nT)), np.zeros((nb, nT))
     emin = np.zeros(nb)
-    stats = np.zeros(8)
+    stats = np.zeros(10)
     _lib.call("qedh_ftlm_batched", _handles(mats), nb, int(samples), int(krylov), _lib.ptr(sd, C.c_uint64),
               _lib.ptr(tol, C.c_double), int(nT), _lib.ptr(betas, C.c_double), int(bool(use_graph)),
               _lib.ptr(lnZ, C.c_double), _lib.ptr(E, C.c_double), _lib.ptr(V, C.c_double),
               _lib.ptr(emin, C.c_double), _lib.ptr(stats, C.c_double))
-    keys = ["pack_s", "seed_s", "steps_s", "post_s", "total_s", "applies", "rows", "nnz"]
+    keys = ["pack_s", "seed_s", "steps_s", "post_s", "total_s", "applies", "rows", "nnz", "schedule", "tune_s"]
     return lnZ, E, V, emin, dict(zip(keys, stats.tolist()))
 
 
