@@ -197,9 +197,10 @@ int qedh_sector_info(qedh_sector s, int64_t* dim, int64_t* n_orbits, int* sublat
  * (qedh_program_compile_projected of O_o^dag with the bra sector as src, or qedh_program_compile for
  * one sector). bras / kets: host arrays of n_pairs device vectors (complex, of each sector's dim);
  * 1 <= n_pairs <= 4096. The observables' terms are merged into one program, so a flip group costs
- * one target lookup for all of them; the sums are accumulated in LDS by f64 atomics, so results agree
- * with other evaluations to rounding (~1e-13 relative), not bit for bit. Sectors of irreps of
- * dimension > 1: DeviceUnsupported. */
+ * one target lookup for all of them; every sum has a fixed order (a wavefront's lanes by a fixed
+ * tree, rows in order, workgroups in order), so a call's results are the same bit for bit from run to
+ * run and whatever other pairs share the call; against other evaluations they agree to rounding
+ * (~1e-13 relative). Sectors of irreps of dimension > 1: DeviceUnsupported. */
 int qedh_matrix_elements(qedh_sector bra_sector, qedh_sector ket_sector, int n_obs, const qedh_program* programs,
                          int n_pairs, const void* const* bras, const void* const* kets, double* out);
 /* The representatives (n_reps of qedh_sector_irrep_info; dim for d = 1) and their 1/norms (1 for a

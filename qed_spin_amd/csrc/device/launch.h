@@ -82,18 +82,20 @@ void csr_move_rows(i64 rows, const u64* old_ptr, const u64* new_ptr, const u32* 
                    hipStream_t st);
 
 // ---- observe.hip -----------------------------------------------------------------------------
-/// Observables per launch of obs_sweep, times the pairs: the LDS accumulators (64 KB).
-constexpr int kObsCells = 4096;
+/// Observables per launch of obs_sweep, times the pairs: the LDS accumulators (16 KB per
+/// one-wavefront workgroup).
+constexpr int kObsCells = 1024;
 /// Workgroups of obs_sweep for a bra sector of `rows` representatives (partial holds 2 x pairs x
-/// nob doubles per workgroup).
+/// nob doubles per workgroup): one wavefront each, at most 8192.
 int obs_sweep_blocks(i64 rows);
 /// out[p][o0 .. o0 + nob) (complex, row length n_obs) = <bras[p]| O_o |kets[p]> for the observables
-/// o0 .. o0 + nob of the merged program P (term_obs: each term's observable; R the bra sector, C the
-/// ket sector; same: one sector, the diagonal shortcut). n_pairs x nob <= kObsCells. bras / kets:
-/// device arrays of device vector pointers.
-void obs_sweep(const SectorView& R, const SectorView& C, bool same, const ProgramView& P, const u32* term_obs, int o0,
-               int nob, int n_obs, int n_pairs, const Zd* const* bras, const Zd* const* kets, double* partial,
-               double* out, hipStream_t st);
+/// o0 .. o0 + nob of the merged program P (term_obs: each term's observable; gobs[gobs_begin[g] ..
+/// gobs_begin[g + 1]): flip group g's observables ascending; R the bra sector, C the ket sector;
+/// same: one sector, the diagonal shortcut). n_pairs x nob <= kObsCells. bras / kets: device arrays
+/// of device vector pointers.
+void obs_sweep(const SectorView& R, const SectorView& C, bool same, const ProgramView& P, const u32* term_obs,
+               const u32* gobs_begin, const u32* gobs, int o0, int nob, int n_obs, int n_pairs,
+               const Zd* const* bras, const Zd* const* kets, double* partial, double* out, hipStream_t st);
 
 // ---- spmv.hip --------------------------------------------------------------------------------
 struct CsrView {

@@ -1,6 +1,7 @@
 // The C-ABI (include/qedh.h): opaque handles over the host engine, exceptions to status codes.
 #include "../../../include/qedh.h"
 
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -679,7 +680,7 @@ extern "C" int qedh_matrix_elements(qedh_sector bra_sector, qedh_sector ket_sect
         need(bras, "bras");
         need(kets, "kets");
         need(out, "out");
-        if (n_obs < 1 || n_pairs < 1 || n_pairs > qsa::dev::kObsCells)
+        if (n_obs < 1 || n_pairs < 1 || n_pairs > 4096)
             throw qsa::Error(qsa::kInvalid, "matrix elements: 1 or more observables and 1 to 4096 pairs");
         const auto& B = *bra_sector->sector;
         const auto& K = *ket_sector->sector;
@@ -701,18 +702,39 @@ extern "C" int qedh_matrix_elements(qedh_sector bra_sector, qedh_sector ket_sect
         const qsa::MultiProgram M = qsa::merge_programs(ps);
         const qsa::DeviceProgram dp(M.prog);
         const qsa::DeviceBuffer tobs = qsa::upload(M.term_obs.data(), M.term_obs.size());
+        // per flip group its observables, ascending (the list every lane of a wave walks together)
+        std::vector<std::uint32_t> gbegin(1, 0), gobs;
+        for (std::size_t g = 0; g + 1 < M.prog.group_vbegin.size(); ++g) {
+            const std::uint32_t k0 = M.prog.vsub_tbegin[M.prog.group_vbegin[g]];
+            const std::uint32_t k1 = M.prog.vsub_tbegin[M.prog.group_vbegin[g + 1]];
+            std::vector<std::uint32_t> os(M.term_obs.begin() + k0, M.term_obs.begin() + k1);
+            std::sort(os.begin(), os.end());
+            os.erase(std::unique(os.begin(), os.end()), os.end());
+            gobs.insert(gobs.end(), os.begin(), os.end());
+            gbegin.push_back(static_cast<std::uint32_t>(gobs.size()));
+        }
+        if (gobs.empty()) gobs.push_back(0);
+        const qsa::DeviceBuffer gb = qsa::upload(gbegin.data(), gbegin.size());
+        const qsa::DeviceBuffer go = qsa::upload(gobs.data(), gobs.size());
         const qsa::DeviceBuffer bd = qsa::upload(bras, static_cast<std::size_t>(n_pairs));
         const qsa::DeviceBuffer kd = qsa::upload(kets, static_cast<std::size_t>(n_pairs));
-        const int nob = std::min(n_obs, qsa::dev::kObsCells / n_pairs);
+        const int pc_max = std::min(n_pairs, qsa::dev::kObsCells);
         const int blocks = qsa::dev::obs_sweep_blocks(B.n_reps());
-        qsa::DeviceBuffer partial(static_cast<std::size_t>(blocks) * 2 * n_pairs * nob * sizeof(double));
+        const int cells_max = std::min(n_pairs * n_obs, qsa::dev::kObsCells);
+        qsa::DeviceBuffer partial(static_cast<std::size_t>(blocks) * 2 * cells_max * sizeof(double));
         qsa::DeviceBuffer dout(static_cast<std::size_t>(n_pairs) * n_obs * 2 * sizeof(double));
         const hipStream_t st = qsa::engine_stream();
-        for (int o0 = 0; o0 < n_obs; o0 += nob)
-            qsa::dev::obs_sweep(B.view(), K.view(), bra_sector->sector == ket_sector->sector, dp.view(),
-                                tobs.get<std::uint32_t>(), o0, std::min(nob, n_obs - o0), n_obs, n_pairs,
-                                bd.get<const qsa::dev::Zd* const>(), kd.get<const qsa::dev::Zd* const>(),
-                                partial.get<double>(), dout.get<double>(), st);
+        for (int p0 = 0; p0 < n_pairs; p0 += pc_max) {
+            const int pc = std::min(pc_max, n_pairs - p0);
+            const int nob = std::min(n_obs, qsa::dev::kObsCells / pc);
+            for (int o0 = 0; o0 < n_obs; o0 += nob)
+                qsa::dev::obs_sweep(B.view(), K.view(), bra_sector->sector == ket_sector->sector, dp.view(),
+                                    tobs.get<std::uint32_t>(), gb.get<std::uint32_t>(), go.get<std::uint32_t>(), o0,
+                                    std::min(nob, n_obs - o0), n_obs, pc,
+                                    bd.get<const qsa::dev::Zd* const>() + p0, kd.get<const qsa::dev::Zd* const>() + p0,
+                                    partial.get<double>(), dout.get<double>() + static_cast<std::size_t>(p0) * n_obs * 2,
+                                    st);
+        }
         QSA_HIP(hipGetLastError());
         qsa::copy_to_host(out, dout.get(), dout.bytes());
     });

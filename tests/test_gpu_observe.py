@@ -122,3 +122,25 @@ def test_levels_expect_through_the_sweep():
     for o, O in enumerate(ops):
         want = np.vdot(g, po.dense_matrix(O.products, N, states) @ g)
         assert abs(vals[0][o] - want) <= 1e-10, (o, vals[0][o], want)
+
+
+def test_sweep_is_reproducible_and_chunks_pairs():
+    """The sweep's sums have a fixed order (a xor tree per wavefront, rows in grid-stride order,
+    workgroups in order): two calls give the same bits. 1100 pairs take two chunks of pairs (at most
+    1024 pair-observable cells per walk), each pair against the per-pair result of a one-pair call."""
+    say("observable sweep: bits and pair chunks")
+    N = 12
+    g = sym.chain_group(N, 0)
+    sec = qs.Sector(g, N // 2)
+    ops = observables(N)[:3]
+    rng = np.random.default_rng(9)
+    vecs = [qs.DeviceArray.from_host(rng.normal(size=sec.dim) + 1j * rng.normal(size=sec.dim)) for _ in range(4)]
+    pairs = [(a % 4, (a * 7 + 1) % 4) for a in range(1100)]
+    bras, kets = [vecs[a] for a, _ in pairs], [vecs[b] for _, b in pairs]
+    one = qs.matrix_elements(ops, sec, sec, bras, kets)
+    two = qs.matrix_elements(ops, sec, sec, bras, kets)
+    assert np.array_equal(np.asarray(one).view(np.uint64), np.asarray(two).view(np.uint64))
+    for p in (0, 1, 513, 1023, 1024, 1099):
+        a, b = pairs[p]
+        single = qs.matrix_elements(ops, sec, sec, [vecs[a]], [vecs[b]])
+        assert np.array_equal(np.asarray(single[0]).view(np.uint64), np.asarray(one[p]).view(np.uint64)), p
