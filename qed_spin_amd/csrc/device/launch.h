@@ -141,6 +141,15 @@ void sell_scan(const u64* slice_words, i64 n_slices, u64* sell_ptr, void* temp, 
 void sell_fill(const u64* row_ptr, const u32* col, const void* vals, int value_bytes, i64 rows, const u64* sell_ptr,
                u32* sell_col, void* sell_vals, bool diag_first, bool packed, hipStream_t st);
 
+/// Workgroups of sell_lanczos on A (its partial buffer holds two doubles per workgroup).
+unsigned sell_lanczos_blocks(const CsrView& A);
+/// The first pass of a Lanczos step in the SELL SpMV's epilogue: w = A v - beta v_prev (v_prev null:
+/// w = A v), and <v, w> into alpha (device) through partial (2 sell_lanczos_blocks(A) doubles), summed
+/// in a fixed order. false (nothing launched) without the SELL layout, or real vectors on complex
+/// values.
+bool sell_lanczos(const CsrView& A, bool real, const void* v, const void* vprev, double beta, void* w,
+                  double* partial, Zd* alpha, hipStream_t st);
+
 // ---- blas.hip (vectors of T = double or Zd; n entries) -------------------------------------------
 /// Partial-sum scratch a reduction needs (device), sized by blas_scratch_bytes; never more than
 /// kBlasScratchMaxBytes (1024 blocks x 257 columns x one complex partial).

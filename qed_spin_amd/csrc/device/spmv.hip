@@ -288,16 +288,14 @@ template <int ACC> __device__ __forceinline__ double tree(const double* s) {
     else return s[0];
 }
 
+/// Row `row` of the SELL layout applied to the NV vectors: ACC partial sums per vector, entries in
+/// groups of four, U per round.
 template <int WIDTH, bool VREAL, bool REAL, int NV, int U, int ACC>
-__global__ __launch_bounds__(1024) void k_sell(CsrView A, Ptrs<NV> v) {
+__device__ __forceinline__ void sell_row_sums(const CsrView& A, const Dict& d, i64 row, const Ptrs<NV>& v,
+                                              double (&ar)[NV][ACC], double (&ai)[NV][ACC]) {
     static_assert(U % 4 == 0, "whole groups of four entries per round");
-    __shared__ double sre[(WIDTH == 1 || WIDTH == 3) ? 256 : 1], sim[(WIDTH == 1 || WIDTH == 3) && !VREAL ? 256 : 1];
-    const Dict d = stage_dict<WIDTH, VREAL>(A, sre, sim);
-    const i64 row = row_block(A) * blockDim.x + threadIdx.x;
-    if (row >= A.rows) return;
     const u32 n = A.nt ? __builtin_nontemporal_load(A.sell_len + row) : A.sell_len[row];
     const u64 first = A.sell_ptr[row >> 6] + static_cast<u64>(row & 63) * 4;   // group g at first + 256 g
-    double ar[NV][ACC], ai[NV][ACC];
 #pragma unroll
     for (int k = 0; k < NV; ++k)
 #pragma unroll
@@ -342,11 +340,95 @@ __global__ __launch_bounds__(1024) void k_sell(CsrView A, Ptrs<NV> v) {
         for (int u = 0; u < 4; ++u)
             if (k + static_cast<u32>(u) < n) accumulate(u % ACC, c[u], vr[u], vi[u]);
     }
+}
+
+template <int WIDTH, bool VREAL, bool REAL, int NV, int U, int ACC>
+__global__ __launch_bounds__(1024) void k_sell(CsrView A, Ptrs<NV> v) {
+    __shared__ double sre[(WIDTH == 1 || WIDTH == 3) ? 256 : 1], sim[(WIDTH == 1 || WIDTH == 3) && !VREAL ? 256 : 1];
+    const Dict d = stage_dict<WIDTH, VREAL>(A, sre, sim);
+    const i64 row = row_block(A) * blockDim.x + threadIdx.x;
+    if (row >= A.rows) return;
+    double ar[NV][ACC], ai[NV][ACC];
+    sell_row_sums<WIDTH, VREAL, REAL, NV, U, ACC>(A, d, row, v, ar, ai);
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
         if constexpr (REAL) static_cast<double*>(v.out[q])[row] = tree<ACC>(ar[q]);
         else static_cast<Zd*>(v.out[q])[row] = Zd{tree<ACC>(ar[q]), tree<ACC>(ai[q])};
     }
+}
+
+// The Lanczos step's first pass in the SpMV's epilogue: w = H v - beta v_prev (v_prev null at the
+// first step) written, and the workgroup's partial of <v, w> (the wavefronts' xor trees, then the
+// wavefronts in order) into partial[workgroup] -- the apply followed by axpy_dot, without writing
+// and reading w in between. k_lanczos_finish sums the partials in workgroup order.
+template <int WIDTH, bool VREAL, bool REAL, int U, int ACC>
+__global__ __launch_bounds__(1024) void k_sell_lanczos(CsrView A, Ptrs<1> v, const void* vprev, double beta,
+                                                       double* partial) {
+    __shared__ double sre[(WIDTH == 1 || WIDTH == 3) ? 256 : 1], sim[(WIDTH == 1 || WIDTH == 3) && !VREAL ? 256 : 1];
+    __shared__ double wr[16], wi[16];
+    const Dict d = stage_dict<WIDTH, VREAL>(A, sre, sim);
+    const i64 row = row_block(A) * blockDim.x + threadIdx.x;
+    double pr = 0.0, pi = 0.0;
+    if (row < A.rows) {
+        double ar[1][ACC], ai[1][ACC];
+        sell_row_sums<WIDTH, VREAL, REAL, 1, U, ACC>(A, d, row, v, ar, ai);
+        if constexpr (REAL) {
+            double w = tree<ACC>(ar[0]);
+            if (vprev) w = w - beta * static_cast<const double*>(vprev)[row];
+            static_cast<double*>(v.out[0])[row] = w;
+            pr = static_cast<const double*>(v.in[0])[row] * w;
+        } else {
+            Zd w{tree<ACC>(ar[0]), tree<ACC>(ai[0])};
+            if (vprev) {
+                const Zd p = static_cast<const Zd*>(vprev)[row];
+                w = Zd{w.re - beta * p.re, w.im - beta * p.im};
+            }
+            static_cast<Zd*>(v.out[0])[row] = w;
+            const Zd x = static_cast<const Zd*>(v.in[0])[row];   // conj(x) w
+            pr = x.re * w.re + x.im * w.im;
+            pi = x.re * w.im - x.im * w.re;
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        pr += __shfl_xor(pr, off, 64);
+        pi += __shfl_xor(pi, off, 64);
+    }
+    const int wave = static_cast<int>(threadIdx.x >> 6);
+    if ((threadIdx.x & 63) == 0) {
+        wr[wave] = pr;
+        wi[wave] = pi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0, b = 0.0;
+        for (int q = 0; q < static_cast<int>(blockDim.x >> 6); ++q) {
+            a += wr[q];
+            b += wi[q];
+        }
+        partial[2 * static_cast<u64>(blockIdx.x)] = a;
+        partial[2 * static_cast<u64>(blockIdx.x) + 1] = b;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_lanczos_finish(const double* partial, int nb, Zd* alpha) {
+    __shared__ double sr[256], si[256];
+    double a = 0.0, b = 0.0;
+    for (int i = static_cast<int>(threadIdx.x); i < nb; i += 256) {
+        a += partial[2 * i];
+        b += partial[2 * i + 1];
+    }
+    sr[threadIdx.x] = a;
+    si[threadIdx.x] = b;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (static_cast<int>(threadIdx.x) < h) {
+            sr[threadIdx.x] += sr[threadIdx.x + h];
+            si[threadIdx.x] += si[threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *alpha = Zd{sr[0], si[0]};
 }
 
 // Block-interleaved vectors X[row * NV + k] on the SELL layout. ACC lanes share a row, lane l
@@ -634,6 +716,44 @@ void csr_spmv(const CsrView& A, bool real, const VecSet& v, hipStream_t st) {
 void csr_spmv_block(const CsrView& A, bool real, int nv, const void* X, void* Y, hipStream_t st) {
     if (A.rows == 0) return;
     dispatch<true>(A, A.values_real, real, nullptr, nv, X, Y, st);
+}
+
+unsigned sell_lanczos_blocks(const CsrView& A) { return grid_blocks(A, static_cast<u64>(A.rows)); }
+
+bool sell_lanczos(const CsrView& A, bool real, const void* v, const void* vprev, double beta, void* w,
+                  double* partial, Zd* alpha, hipStream_t st) {
+    if (!A.sell_ptr || A.rows == 0 || (real && !A.values_real)) return false;
+    const unsigned nb = sell_lanczos_blocks(A);
+    const dim3 grid(nb), block(block_threads(A));
+    Ptrs<1> p;
+    p.in[0] = v;
+    p.out[0] = w;
+    auto go = [&](auto wd, auto vr, auto xr) {
+        constexpr int W = decltype(wd)::value;
+        constexpr bool VR = decltype(vr)::value, XR = decltype(xr)::value;
+        if (A.sell_acc == 4) {
+            if (A.unroll >= 16) hipLaunchKernelGGL((k_sell_lanczos<W, VR, XR, 16, 4>), grid, block, 0, st, A, p, vprev, beta, partial);
+            else if (A.unroll >= 8) hipLaunchKernelGGL((k_sell_lanczos<W, VR, XR, 8, 4>), grid, block, 0, st, A, p, vprev, beta, partial);
+            else hipLaunchKernelGGL((k_sell_lanczos<W, VR, XR, 4, 4>), grid, block, 0, st, A, p, vprev, beta, partial);
+        } else {
+            if (A.unroll >= 16) hipLaunchKernelGGL((k_sell_lanczos<W, VR, XR, 16, 1>), grid, block, 0, st, A, p, vprev, beta, partial);
+            else if (A.unroll >= 8) hipLaunchKernelGGL((k_sell_lanczos<W, VR, XR, 8, 1>), grid, block, 0, st, A, p, vprev, beta, partial);
+            else hipLaunchKernelGGL((k_sell_lanczos<W, VR, XR, 4, 1>), grid, block, 0, st, A, p, vprev, beta, partial);
+        }
+    };
+    using T = std::true_type;
+    using F = std::false_type;
+    auto by_width = [&](auto vr, auto xr) {
+        if (A.width == 1) go(std::integral_constant<int, 1>{}, vr, xr);
+        else if (A.width == 2) go(std::integral_constant<int, 2>{}, vr, xr);
+        else if (A.width == 3) go(std::integral_constant<int, 3>{}, vr, xr);
+        else go(std::integral_constant<int, 0>{}, vr, xr);
+    };
+    if (real) by_width(T{}, T{});
+    else if (A.values_real) by_width(T{}, F{});
+    else by_width(F{}, F{});
+    hipLaunchKernelGGL(k_lanczos_finish, dim3(1), dim3(256), 0, st, partial, static_cast<int>(nb), alpha);
+    return true;
 }
 
 }  // namespace qsa::dev

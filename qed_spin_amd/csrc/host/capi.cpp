@@ -529,7 +529,7 @@ int qedh_lanczos(qedh_matvec m, const void* v0_host, int complex_vectors, int ma
             o.keep_basis = o.reorth == qsa::Reorth::FullCGS2;
             if (breakdown_tol > 0.0) o.breakdown_tol = breakdown_tol;
             const bool cv = qsa::HipBackend<S>::kComplex;
-            auto r = qsa::lanczos_run(be, [&](const S* in, S* out) { op.apply(in, out, cv); }, n, v0.get(), o);
+            auto r = qsa::lanczos_run(be, qsa::OperatorMatvec<S>{op, cv}, n, v0.get(), o);
             for (std::size_t j = 0; j < r.alpha.size(); ++j) alpha[j] = r.alpha[j];
             for (std::size_t j = 0; j < r.beta.size(); ++j) beta[j] = r.beta[j];
             *iters = static_cast<int>(r.iters);
@@ -561,7 +561,7 @@ int qedh_block_lowest(qedh_matvec m, int k, int64_t max_iter, double* values, in
             using S = decltype(tag);
             qsa::HipBackend<S> be;
             const bool cv = qsa::HipBackend<S>::kComplex;
-            auto H = [&](const S* in, S* out) { op.apply(in, out, cv); };
+            const qsa::OperatorMatvec<S> H{op, cv};
             qsa::BlockSolution sol;
             const std::uint64_t budget = max_iter > 0 ? static_cast<std::uint64_t>(max_iter) : 0;
             if (kk > 1) sol = qsa::krylov_schur_lane(be, H, n, kk, op.norm_bound(), budget);
@@ -588,7 +588,7 @@ int qedh_block_estimate(qedh_matvec m, double* theta, double* residual, int64_t*
             using S = decltype(tag);
             qsa::HipBackend<S> be;
             const bool cv = qsa::HipBackend<S>::kComplex;
-            auto H = [&](const S* in, S* out) { op.apply(in, out, cv); };
+            const qsa::OperatorMatvec<S> H{op, cv};
             const qsa::BlockEstimate e = qsa::estimate_lowest(be, H, n);
             *theta = e.theta;
             *residual = e.residual;
@@ -611,7 +611,7 @@ int qedh_block_eigenpairs(qedh_matvec m, int k, int64_t max_iter, double* values
         if (k < 1) throw qsa::Error(qsa::kInvalid, "eigenpairs: k must be >= 1");
         const std::size_t kk = std::min<std::size_t>(static_cast<std::size_t>(k), n);
         qsa::HipBackend<qsa::cplx> be;
-        auto H = [&](const qsa::cplx* in, qsa::cplx* out) { op.apply(in, out, true); };
+        const qsa::OperatorMatvec<qsa::cplx> H{op, true};
         const qsa::BlockSolution sol =
             qsa::krylov_schur_lane(be, H, n, kk, op.norm_bound(), max_iter > 0 ? static_cast<std::uint64_t>(max_iter) : 0,
                                    static_cast<qsa::cplx*>(vectors));
@@ -636,7 +636,7 @@ int qedh_ftlm(qedh_matvec m, int samples, int krylov, uint64_t seed, int n_beta,
         o.seed = seed;
         o.betas.assign(betas, betas + n_beta);
         o.breakdown_tol = breakdown_tol;
-        const auto c = qsa::ftlm_block(be, [&](const qsa::cplx* in, qsa::cplx* out) { op.apply(in, out, true); }, n, o);
+        const auto c = qsa::ftlm_block(be, qsa::OperatorMatvec<qsa::cplx>{op, true}, n, o);
         for (int t = 0; t < n_beta; ++t) {
             lnZ[t] = c.lnZ[t];
             E[t] = c.E[t];
@@ -659,7 +659,7 @@ int qedh_cf(qedh_matvec m, const double* phi_host, int krylov, int n_omega, cons
         be.copy_from_host(reinterpret_cast<const qsa::cplx*>(phi_host), phi.get(), n);
         std::vector<double> w(omega, omega + n_omega);
         const auto r = qsa::cf_from_vector(
-            be, [&](const qsa::cplx* in, qsa::cplx* out) { op.apply(in, out, true); }, n, phi.get(), w, eta,
+            be, qsa::OperatorMatvec<qsa::cplx>{op, true}, n, phi.get(), w, eta,
             static_cast<std::size_t>(krylov), has_shift != 0, energy_shift, breakdown_tol > 0.0 ? breakdown_tol : 1e-12);
         for (int i = 0; i < n_omega; ++i) S[i] = r.S[i];
         if (out_shift) *out_shift = r.energy_shift;

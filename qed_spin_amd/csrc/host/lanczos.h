@@ -9,6 +9,8 @@
 #include <cmath>
 #include <functional>
 #include <stdexcept>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "backend.h"
@@ -51,6 +53,15 @@ template <class Scalar> struct LanczosResult {
     LanczosTimings timings;
 };
 
+/// A matvec that can also run a step's first pass fused (first_pass(v, v_prev, beta, w, alpha):
+/// w = H v - beta v_prev, alpha = <v, w>, false when it cannot this time).
+template <class M, class S, class = void> struct HasFirstPass : std::false_type {};
+template <class M, class S>
+struct HasFirstPass<M, S,
+                    std::void_t<decltype(std::declval<const M&>().first_pass(
+                        static_cast<const S*>(nullptr), static_cast<const S*>(nullptr), 0.0, static_cast<S*>(nullptr),
+                        std::declval<S&>()))>> : std::true_type {};
+
 template <class Scalar, class Matvec>
 LanczosResult<Scalar> lanczos_run(const HipBackend<Scalar>& be, Matvec&& matvec, std::size_t n, const Scalar* v0,
                                   const LanczosOptions<Scalar>& o) {
@@ -90,13 +101,22 @@ LanczosResult<Scalar> lanczos_run(const HipBackend<Scalar>& be, Matvec&& matvec,
     for (std::size_t j = 0; j < cap; ++j) {
         if (o.on_vector) o.on_vector(j, v_curr.get());
         const auto t0 = clock::now();
-        matvec(v_curr.get(), w.get());
-        be.synchronize();
+        // the apply, and alpha_j = <v_j, H v_j - beta_j v_{j-1}>: in the apply's epilogue when the
+        // matvec offers it, else apply, then axpy_dot
+        Scalar aj{};
+        bool fused = false;
+        if constexpr (HasFirstPass<std::decay_t<Matvec>, Scalar>::value)
+            fused = matvec.first_pass(v_curr.get(), j > 0 ? v_prev.get() : nullptr, R.beta[j], w.get(), aj);
+        if (!fused) {
+            matvec(v_curr.get(), w.get());
+            be.synchronize();
+        }
         const auto t1 = clock::now();
         R.timings.apply_s += secs(t0, t1);
         ++R.timings.applies;
-        const Scalar aj = (j > 0) ? be.axpy_dot(Scalar(-R.beta[j]), v_prev.get(), w.get(), v_curr.get(), n)
-                                  : be.dot(v_curr.get(), w.get(), n);
+        if (!fused)
+            aj = (j > 0) ? be.axpy_dot(Scalar(-R.beta[j]), v_prev.get(), w.get(), v_curr.get(), n)
+                         : be.dot(v_curr.get(), w.get(), n);
         R.alpha.push_back(std::real(aj));
         const bool local = o.reorth == Reorth::LocalDGKS3 && j > 0;
         Scalar overlap{};

@@ -2,6 +2,10 @@
 
 #include <chrono>
 #include <cstdlib>
+#include <mutex>
+#include <utility>
+#include <string>
+#include <cstdio>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -40,8 +44,81 @@ SectorOperator::SectorOperator(std::shared_ptr<const DeviceSector> rows, std::sh
         throw Error(kResourceLimit, "SectorOperator: the CSR lane was requested but does not fit its budget");
 }
 
+namespace {
+
+/// Stage buffers of finished CSR builds, kept for the next builds on the same device: the
+/// stream-ordered pool can take a second to map a new block of several GB once its free memory is
+/// split, while a kept buffer costs nothing. At most a quarter of the device's memory (and 48 GiB) is
+/// kept; the cache lives until the process ends and is never destroyed (no HIP call at exit). A
+/// buffer is given back only after the build's stream has finished with it.
+class StageCache {
+public:
+    DeviceBuffer take(std::size_t bytes) {
+        int dev = 0;
+        QSA_HIP(hipGetDevice(&dev));
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            std::size_t best = kept_.size();
+            for (std::size_t i = 0; i < kept_.size(); ++i)
+                if (kept_[i].dev == dev && kept_[i].buf.bytes() >= bytes &&
+                    (best == kept_.size() || kept_[i].buf.bytes() < kept_[best].buf.bytes()))
+                    best = i;
+            if (best < kept_.size()) {
+                DeviceBuffer b = std::move(kept_[best].buf);
+                held_ -= b.bytes();
+                kept_.erase(kept_.begin() + static_cast<std::ptrdiff_t>(best));
+                return b;
+            }
+        }
+        return DeviceBuffer(bytes);
+    }
+    void give(DeviceBuffer b) {
+        if (!b) return;
+        int dev = 0;
+        QSA_HIP(hipGetDevice(&dev));
+        std::vector<DeviceBuffer> drop;
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            if (cap_ == 0) {
+                std::size_t free_b = 0, total_b = 0;
+                QSA_HIP(hipMemGetInfo(&free_b, &total_b));
+                cap_ = std::min<std::size_t>(std::size_t{48} << 30, total_b / 4);
+            }
+            held_ += b.bytes();
+            kept_.push_back({dev, std::move(b)});
+            while (held_ > cap_ && !kept_.empty()) {   // the oldest go first
+                held_ -= kept_.front().buf.bytes();
+                drop.push_back(std::move(kept_.front().buf));
+                kept_.erase(kept_.begin());
+            }
+        }
+    }
+
+private:
+    struct Kept {
+        int dev;
+        DeviceBuffer buf;
+    };
+    std::mutex m_;
+    std::vector<Kept> kept_;
+    std::size_t held_ = 0, cap_ = 0;
+};
+
+StageCache& stage_cache() {
+    static StageCache* c = new StageCache;
+    return *c;
+}
+
+}  // namespace
+
 void SectorOperator::build_csr(std::uint64_t budget) {
     const auto t0 = std::chrono::steady_clock::now();
+    // QED_SPIN_AMD_BUILD_PROFILE set: the host seconds of each phase on stderr (a diagnostic)
+    static const bool phases = std::getenv("QED_SPIN_AMD_BUILD_PROFILE") != nullptr;
+    std::vector<std::pair<const char*, double>> marks;
+    auto mark = [&](const char* what) {
+        if (phases) marks.emplace_back(what, std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+    };
     const hipStream_t st = engine_stream();
     const std::int64_t rows = dim();
     const dev::SectorView R = rows_->view(), C = cols_->view();
@@ -74,26 +151,38 @@ void SectorOperator::build_csr(std::uint64_t budget) {
     const std::int64_t chunk = std::min<std::int64_t>(rows, static_cast<std::int64_t>(cap / slot_bytes));
     if (chunk < std::min<std::int64_t>(rows, 256)) return;   // no room to stage: the gather lane
     const std::int64_t n_chunks = (rows + chunk - 1) / chunk;
-    DeviceBuffer stage_col(static_cast<std::size_t>(chunk) * std::max<std::uint32_t>(P.n_groups, 1) * 4);
-    DeviceBuffer stage_val(static_cast<std::size_t>(chunk) * std::max<std::uint32_t>(P.n_groups, 1) *
-                           sizeof(dev::Zd));
+    mark("set-up");
+    // one buffer from the cache: the columns, then (256-byte aligned) the values
+    const std::size_t slots = static_cast<std::size_t>(chunk) * std::max<std::uint32_t>(P.n_groups, 1);
+    const std::size_t col_bytes = (slots * 4 + 255) / 256 * 256;
+    struct Stage {
+        DeviceBuffer buf;
+        hipStream_t st;
+        ~Stage() {
+            if (!buf) return;
+            (void)hipStreamSynchronize(st);   // the build's kernels are done with it (already, on every path)
+            stage_cache().give(std::move(buf));
+        }
+    } stage{stage_cache().take(col_bytes + slots * sizeof(dev::Zd)), st};
+    std::uint32_t* const stage_col = stage.buf.get<std::uint32_t>();
+    dev::Zd* const stage_val = reinterpret_cast<dev::Zd*>(stage.buf.get<char>() + col_bytes);
+    mark("stage alloc");
     auto walk = [&](std::int64_t c) {
         const std::int64_t r0 = c * chunk, n = std::min<std::int64_t>(chunk, rows - r0);
-        return dev::csr_walk(R, C, same(), P, r0, n, row_len.get<std::uint32_t>() + r0, stage_col.get<std::uint32_t>(),
-                             stage_val.get<dev::Zd>(), st);
+        return dev::csr_walk(R, C, same(), P, r0, n, row_len.get<std::uint32_t>() + r0, stage_col, stage_val, st);
     };
     for (std::int64_t c = 0; c < n_chunks; ++c) {
         if (!walk(c)) return;   // too many groups
         const std::int64_t r0 = c * chunk, n = std::min<std::int64_t>(chunk, rows - r0);
-        dev::csr_stage_dict(n, row_len.get<std::uint32_t>() + r0, stage_val.get<dev::Zd>(), D, st);
+        dev::csr_stage_dict(n, row_len.get<std::uint32_t>() + r0, stage_val, D, st);
     }
     QSA_HIP(hipGetLastError());
     auto fill = [&](int w) {
         for (std::int64_t c = 0; c < n_chunks; ++c) {
             const std::int64_t r0 = c * chunk, n = std::min<std::int64_t>(chunk, rows - r0);
             if (n_chunks > 1) walk(c);
-            dev::csr_compact(n, row_ptr_.get<std::uint64_t>() + r0, stage_col.get<std::uint32_t>(),
-                             stage_val.get<dev::Zd>(), col_.get<std::uint32_t>(), vals_.get(), w, real_, D, st);
+            dev::csr_compact(n, row_ptr_.get<std::uint64_t>() + r0, stage_col, stage_val, col_.get<std::uint32_t>(),
+                             vals_.get(), w, real_, D, st);
         }
     };
     row_ptr_.reset(static_cast<std::size_t>(rows + 1) * 8);
@@ -124,6 +213,7 @@ void SectorOperator::build_csr(std::uint64_t budget) {
         row_ptr_.release();
         return;
     }
+    mark("walk + scan");
     col_.reset(nnz * 4);
     vals_.reset(nnz * vbytes);
     if (width > 0) {
@@ -147,6 +237,7 @@ void SectorOperator::build_csr(std::uint64_t budget) {
         dict_im_ = upload(di.data(), di.size());
         csr_.dict = static_cast<std::int64_t>(dr.size());
     }
+    mark("csr alloc + dictionary");
     fill(width);
     QSA_HIP(hipGetLastError());
     QSA_HIP(hipMemcpyAsync(cnt, dcnt.get(), 8, hipMemcpyDeviceToHost, st));
@@ -182,10 +273,23 @@ void SectorOperator::build_csr(std::uint64_t budget) {
     lanes_block_ = L;
     lane_ = kLaneCsr;
     if (tune_ && rows >= (std::int64_t{1} << 18)) {
+        mark("compact + pack");
         build_sell();
+        mark("sell");
         autotune();
+        mark("autotune");
     }
     csr_.build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (phases) {
+        std::string line = "[csr build] rows " + std::to_string(rows) + " chunks " + std::to_string(n_chunks);
+        double prev = 0.0;
+        for (const auto& [what, at] : marks) {
+            line += std::string(" | ") + what + " " + std::to_string(at - prev);
+            prev = at;
+        }
+        line += " | total " + std::to_string(csr_.build_seconds) + "\n";
+        std::fputs(line.c_str(), stderr);
+    }
 }
 
 void SectorOperator::build_csr_irrep(std::uint64_t budget) {
@@ -389,6 +493,22 @@ void SectorOperator::apply(const void* in, void* out, bool complex_vectors) cons
     const void* ins[1] = {in};
     void* outs[1] = {out};
     apply_multi(ins, outs, 1, complex_vectors);
+}
+
+bool SectorOperator::lanczos_first_pass(const void* v, const void* vprev, double beta, void* w, bool complex_vectors,
+                                        cplx* alpha) const {
+    if (lane_ != kLaneCsr || !sell() || !same()) return false;
+    const dev::CsrView A = csr_view(false, true);
+    const hipStream_t st = engine_stream();
+    DeviceBuffer partial(static_cast<std::size_t>(dev::sell_lanczos_blocks(A)) * 2 * sizeof(double) + sizeof(dev::Zd));
+    dev::Zd* a = reinterpret_cast<dev::Zd*>(partial.get<char>() + partial.bytes() - sizeof(dev::Zd));
+    if (!dev::sell_lanczos(A, real_ && !complex_vectors, v, vprev, beta, w, partial.get<double>(), a, st)) return false;
+    QSA_HIP(hipGetLastError());
+    dev::Zd h{};
+    QSA_HIP(hipMemcpyAsync(&h, a, sizeof(h), hipMemcpyDeviceToHost, st));
+    QSA_HIP(hipStreamSynchronize(st));
+    *alpha = cplx(h.re, h.im);
+    return true;
 }
 
 void SectorOperator::apply_multi(const void* const* ins, void* const* outs, int k, bool complex_vectors) const {

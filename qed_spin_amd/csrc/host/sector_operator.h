@@ -7,6 +7,7 @@
 #pragma once
 
 #include <memory>
+#include <type_traits>
 #include <vector>
 
 #include "../device/launch.h"
@@ -53,6 +54,11 @@ public:
     /// out = O in on device vectors (double when is_real() and not complex_vectors, else complex),
     /// on the engine stream.
     void apply(const void* in, void* out, bool complex_vectors = false) const;
+    /// The first pass of a Lanczos step fused into the apply (the SELL layout only): w = H v -
+    /// beta v_prev (v_prev null: w = H v) and <v, w> into *alpha. false (nothing done) when the
+    /// operator cannot (no SELL layout, not square): the caller applies and forms alpha itself.
+    bool lanczos_first_pass(const void* v, const void* vprev, double beta, void* w, bool complex_vectors,
+                            cplx* alpha) const;
     /// outs[i] = O ins[i], i < k (k-vector applies equal k single applies bit for bit).
     void apply_multi(const void* const* ins, void* const* outs, int k, bool complex_vectors = false) const;
     /// Block-interleaved vectors: X[row * nv + i] (nv in 1, 2, 4, 8); CSR lane only.
@@ -127,6 +133,21 @@ private:
     bool sell_ = false, use_sell_ = true, sell_diag_first_ = false;
     int sell_acc_ = 4;   // four partial sums per row: as fast single, and the block kernel's 4 lanes per row
     DeviceBuffer sell_ptr_, sell_len_, sell_col_, sell_vals_;
+};
+
+/// H v for the Krylov drivers: a sector operator's apply, with its fused first Lanczos pass
+/// (lanczos_run takes it when the operator can).
+template <class Scalar> struct OperatorMatvec {
+    const SectorOperator& op;
+    bool complex_vectors;
+    void operator()(const Scalar* in, Scalar* out) const { op.apply(in, out, complex_vectors); }
+    bool first_pass(const Scalar* v, const Scalar* vprev, double beta, Scalar* w, Scalar& alpha) const {
+        cplx a;
+        if (!op.lanczos_first_pass(v, vprev, beta, w, complex_vectors, &a)) return false;
+        if constexpr (std::is_same_v<Scalar, double>) alpha = a.real();
+        else alpha = a;
+        return true;
+    }
 };
 
 }  // namespace qsa

@@ -187,7 +187,7 @@ void solve(const Request& r, const ServeOptions& o, SectorCache& cache, RequestR
     build_done();
     const std::size_t n = static_cast<std::size_t>(src->dim());
     HipBackend<cplx> be;
-    auto applyH = [&](const cplx* in, cplx* out) { Hop->apply(in, out, true); };
+    const OperatorMatvec<cplx> applyH{*Hop, true};
     if (r.kind == kReqEigs) {
         const std::size_t k = std::min<std::size_t>(static_cast<std::size_t>(std::max(1, r.k)), n);
         const BlockSolution sol = krylov_schur_lane(be, applyH, n, k, sH, 0, static_cast<cplx*>(nullptr));
@@ -244,7 +244,7 @@ void solve(const Request& r, const ServeOptions& o, SectorCache& cache, RequestR
         X.apply(gs.get(), phi.get(), true);
         SectorOperator Ht(tgt, tgt, hprog, sH, kLaneCsr, ~0ull, false);
         build_done();
-        auto applyT = [&](const cplx* in, cplx* out) { Ht.apply(in, out, true); };
+        const OperatorMatvec<cplx> applyT{Ht, true};
         const CfResult cf = cf_from_vector(be, applyT, nt, phi.get(), o.omega, o.eta,
                                            static_cast<std::size_t>(std::max(2, r.krylov)), true, e0, kbreak(sH));
         res.values = cf.S;
