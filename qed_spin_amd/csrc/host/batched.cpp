@@ -206,7 +206,9 @@ int BatchedLanczos::tune_schedule() {
         set_schedule(std::atoi(e) == 1 ? 1 : 0);
         return sched_kind_;
     }
-    if (tiles_.size() < 4096) {
+    // small engines finish a step in microseconds either way; above 65536 tiles the two schedules
+    // measured alike and a timed apply costs tens of milliseconds
+    if (tiles_.size() < 4096 || tiles_.size() > 65536) {
         set_schedule(0);
         return 0;
     }

@@ -76,7 +76,8 @@ public:
     void set_schedule(int kind);
     int schedule() const { return sched_kind_; }
     /// The schedule of the faster packed matvec, timed over a few applies of each (the engine's
-    /// autotuner); returns the chosen kind. Engines of fewer than 4096 tiles keep schedule 0.
+    /// autotuner); returns the chosen kind. Engines of fewer than 4096 or more than 65536 tiles keep
+    /// schedule 0.
     int tune_schedule();
     /// The packed layout as the kernels read it (tiles, R); every pair live.
     dev::BatchView packed_view() const {
