@@ -55,7 +55,7 @@ template <bool SLC> __device__ bool is_rep(const SectorView& S, u64 s) {
         const u64 k = key_of(S, s);
         u32 least = 0xFFFFFFFFu;
         for (int j = 0; j < S.slc_m; ++j) {
-            const u32 l = S.lead[(static_cast<u32>(j) << S.slc_L) + pattern_of(S, k, j)];
+            const u32 l = static_cast<u32>(cand_record(S, k, j) & 0xFFFFu);
             least = l < least ? l : least;
         }
         if (least < pattern_of(S, k, 0)) return false;

@@ -40,9 +40,8 @@ struct SectorView {
     const double* chi_im = nullptr;  // G
     int slc_L = 0, slc_m = 0;
     const u64* to_key = nullptr;     // bpw x 256
-    const u16* lead = nullptr;       // m x 2^L
-    const u32* cand_off = nullptr;   // m x 2^L + 1
-    const u16* cand = nullptr;
+    const u64* cand_rec = nullptr;   // m x 2^L: lead | count << 16 | (count == 1 ? element : offset) << 32
+    const u16* cand = nullptr;       // the candidate lists, entry after entry
     const u64* reps = nullptr;       // dim, ascending
     const double* inv_norms = nullptr;
     i64 dim = 0;
@@ -109,25 +108,40 @@ __device__ __forceinline__ u32 pattern_of(const SectorView& S, u64 k, int j) {
     return static_cast<u32>(k >> (S.n_sites - (j + 1) * S.slc_L)) & ((1u << S.slc_L) - 1u);
 }
 
+/// The candidate record of block j of key k: the least leading block reachable from that block's
+/// pattern (bits 0-15), how many elements reach it (bits 16-31) and the element itself when it is
+/// the only one, else its list's offset in cand (bits 32-63). One load per block.
+__device__ __forceinline__ u64 cand_record(const SectorView& S, u64 k, int j) {
+    return S.cand_rec[(static_cast<u32>(j) << S.slc_L) + pattern_of(S, k, j)];
+}
+
 /// fn(g) for the elements that can map the state (key k) to its representative, ascending: those
 /// whose image has the least leading block (sublattice_code.h:115-147).
 template <class Fn> __device__ __forceinline__ void for_each_candidate(const SectorView& S, u64 k, Fn&& fn) {
-    u32 entry[8];
+    u64 rec[8];
     u32 best = 0xFFFFFFFFu;
     for (int j = 0; j < S.slc_m; ++j) {
-        entry[j] = (static_cast<u32>(j) << S.slc_L) + pattern_of(S, k, j);
-        const u32 l = S.lead[entry[j]];
+        rec[j] = cand_record(S, k, j);
+        const u32 l = static_cast<u32>(rec[j] & 0xFFFFu);
         best = l < best ? l : best;
     }
-    u32 pos[8], end[8];
+    // the blocks reaching the least leading block: a single element, or a list cand[pos, end)
+    u32 pos[8], end[8], one[8];
     int n = 0;
     for (int j = 0; j < S.slc_m; ++j)
-        if (S.lead[entry[j]] == best) {
-            pos[n] = S.cand_off[entry[j]];
-            end[n] = S.cand_off[entry[j] + 1];
+        if (static_cast<u32>(rec[j] & 0xFFFFu) == best) {
+            const u32 cnt = static_cast<u32>((rec[j] >> 16) & 0xFFFFu), pay = static_cast<u32>(rec[j] >> 32);
+            one[n] = cnt == 1 ? pay : 0xFFFFFFFFu;
+            pos[n] = cnt == 1 ? 0 : pay;
+            end[n] = cnt == 1 ? 1 : pay + cnt;
             ++n;
         }
+    auto at = [&](int i) { return one[i] != 0xFFFFFFFFu ? one[i] : static_cast<u32>(S.cand[pos[i]]); };
     if (n == 1) {
+        if (one[0] != 0xFFFFFFFFu) {
+            fn(static_cast<int>(one[0]));
+            return;
+        }
         for (u32 p = pos[0]; p < end[0]; ++p) fn(static_cast<int>(S.cand[p]));
         return;
     }
@@ -135,8 +149,8 @@ template <class Fn> __device__ __forceinline__ void for_each_candidate(const Sec
         int pick = -1;
         u32 g = 0xFFFFFFFFu;
         for (int i = 0; i < n; ++i)
-            if (pos[i] < end[i] && S.cand[pos[i]] < g) {
-                g = S.cand[pos[i]];
+            if (pos[i] < end[i] && at(i) < g) {
+                g = at(i);
                 pick = i;
             }
         if (pick < 0) return;

@@ -104,8 +104,18 @@ void DeviceSector::setup_tables() {
     }
     if (spec_.slc.engaged()) {
         to_key_ = upload(spec_.slc.to_key.data(), spec_.slc.to_key.size());
-        lead_ = upload(spec_.slc.lead.data(), spec_.slc.lead.size());
-        cand_off_ = upload(spec_.slc.cand_off.data(), spec_.slc.cand_off.size());
+        // one record per (block, pattern): lead | count << 16 | (the element, or the list's offset) << 32
+        const auto& t = spec_.slc;
+        std::vector<std::uint64_t> rec(t.lead.size());
+        for (std::size_t e = 0; e < rec.size(); ++e) {
+            const std::uint32_t off = t.cand_off[e], cnt = t.cand_off[e + 1] - off;
+            // (a block no element maps to the leading one has no candidates: its lead is the 0xFFFF
+            // sentinel and its empty list adds nothing)
+            if (cnt > 0xFFFFu) throw Error(kDeviceUnsupported, "sector: more than 65535 candidate elements");
+            const std::uint64_t pay = cnt == 1 ? t.cand[off] : off;
+            rec[e] = static_cast<std::uint64_t>(t.lead[e]) | (static_cast<std::uint64_t>(cnt) << 16) | (pay << 32);
+        }
+        cand_rec_ = upload(rec.data(), rec.size());
         cand_ = upload(spec_.slc.cand.data(), spec_.slc.cand.size());
     }
 }
@@ -139,8 +149,8 @@ void DeviceSector::build_rank_dir(std::uint64_t total, std::uint64_t rank_budget
         stats_.rank_table = true;
     }
     QSA_HIP(hipStreamSynchronize(st));
-    stats_.device_bytes = lut_.bytes() + nib_.bytes() + reps_.bytes() + inv_norms_.bytes() + rank_dir_.bytes() + lead_.bytes() +
-                          cand_off_.bytes() + cand_.bytes();
+    stats_.device_bytes = lut_.bytes() + nib_.bytes() + reps_.bytes() + inv_norms_.bytes() + rank_dir_.bytes() + cand_rec_.bytes() +
+                          cand_.bytes();
 }
 
 DeviceSector::DeviceSector(GroupSpec spec, std::uint64_t rank_budget_bytes) : spec_(std::move(spec)) {
@@ -249,8 +259,7 @@ dev::SectorView DeviceSector::view() const {
         v.slc_L = spec_.slc.L;
         v.slc_m = spec_.slc.m;
         v.to_key = to_key_.get<std::uint64_t>();
-        v.lead = lead_.get<std::uint16_t>();
-        v.cand_off = cand_off_.get<std::uint32_t>();
+        v.cand_rec = cand_rec_.get<std::uint64_t>();
         v.cand = cand_.get<std::uint16_t>();
     }
     v.reps = reps_.get<std::uint64_t>();

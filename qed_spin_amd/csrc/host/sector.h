@@ -83,7 +83,7 @@ private:
     GroupSpec spec_;
     std::int64_t dim_ = 0;
     SectorBuildStats stats_;
-    DeviceBuffer lut_, flips_, chi_re_, chi_im_, to_key_, lead_, cand_off_, cand_, binom_;
+    DeviceBuffer lut_, flips_, chi_re_, chi_im_, to_key_, cand_rec_, cand_, binom_;
     DeviceBuffer reps_, inv_norms_, rank_dir_;
     DeviceBuffer nib_;                          ///< GroupSpec::nibble_tables
     std::array<std::size_t, 3> nib_sizes_{};    ///< entries of its images, flips and key tables
