@@ -176,7 +176,8 @@ int qedh_sublattice_tables(int n_sites, int group_size, const int* perms, const 
  * the range of sum_{s in S} conj(D(s)) (rank(C) states per representative, C^dag M C = I), and the
  * representatives of rank 0 are dropped. The sector's basis states are the partner-0 states
  * |r; a> = sqrt(d/|G|) sum_j C_r[j][a] sum_g conj(D(g)[0][j]) |g r>, a < rank(r); its dim counts
- * them. irrep_dim 1: D holds the characters (qedh_sector_create). d above 4: DeviceUnsupported. */
+ * them. irrep_dim 1: D holds the characters (qedh_sector_create). d above 8 (the reference's
+ * kMaxIrrepDim, include/ed/matvec/sector_rows.h:66): DeviceUnsupported. */
 int qedh_sector_create_irrep(int n_sites, int n_up, int group_size, const int* perms, const uint64_t* flips,
                              int irrep_dim, const double* D, int sublattice_mode, int relaxed,
                              uint64_t rank_budget_bytes, qedh_sector* out);
@@ -242,7 +243,10 @@ typedef struct {
     double norm_bound;
 } qedh_matvec_info_t;
 /* cols == NULL or == rows: one sector (the diagonal shortcut applies). prog: the row program
- * (the walk of O^dagger; for two sectors the projected program of O^dagger with rows as ket). */
+ * (the walk of O^dagger; for two sectors the projected program of O^dagger with rows as ket).
+ * Sectors of an irrep of dimension d > 1 (CSR lane only): two of them must carry the same D, and
+ * prog is the plain walk of O^dagger for an O that commutes with the group (total S+-, S^2; the
+ * reference's CrossSectorMatVec, src/engine/internal.h:557-563); else InvalidRequest. */
 int qedh_matvec_create(qedh_sector rows, qedh_sector cols, qedh_program prog, double norm_bound, int lane,
                        uint64_t csr_budget_bytes, qedh_matvec* out);
 int qedh_matvec_info(qedh_matvec m, qedh_matvec_info_t* info);

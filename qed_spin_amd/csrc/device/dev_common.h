@@ -72,8 +72,9 @@ struct IrrepView {
     const u64* offset = nullptr;    // per representative, and one past the last
 };
 
-/// The largest irrep dimension the device kernels hold in registers.
-constexpr int kMaxIrrepDim = 4;
+/// The largest irrep dimension of a sector (the reference's kMaxIrrepDim, include/ed/matvec/
+/// sector_rows.h:66). The row kernel holds a d x d block per thread.
+constexpr int kMaxIrrepDim = 8;
 
 /// A compiled program (program.h layout) in device memory.
 struct ProgramView {

@@ -107,25 +107,24 @@ __global__ __launch_bounds__(256) void k_irrep_rows(SectorView R, IrrepView IR, 
         if (val) {
             const Zd* Cc = IC.C + static_cast<size_t>(cc) * D * D;
             const u64 o_c = IC.offset[c];
-            Zd T[D][D];   // A C_c, columns below rc
+            // column b of T = A C_c, then entry (a, b) = conj(h) (C_r^dag T)[a][b] into slot b of
+            // rank row a: one column of T in registers at a time besides A
+            for (int b = 0; b < rc; ++b) {
+                Zd t[D];
 #pragma unroll
-            for (int i = 0; i < D; ++i)
-#pragma unroll
-                for (int b = 0; b < D; ++b) {
+                for (int i = 0; i < D; ++i) {
                     Zd acc{0.0, 0.0};
-                    if (b < rc)
 #pragma unroll
-                        for (int j = 0; j < D; ++j) acc = zadd(acc, zmul(A[i][j], Cc[j * D + b]));
-                    T[i][b] = acc;
+                    for (int j = 0; j < D; ++j) acc = zadd(acc, zmul(A[i][j], Cc[j * D + b]));
+                    t[i] = acc;
                 }
-            for (int a = 0; a < rr; ++a) {
-                const u64 base = row_ptr[o_r + a] + count;
-                for (int b = 0; b < rc; ++b) {
+                for (int a = 0; a < rr; ++a) {
                     Zd acc{0.0, 0.0};
 #pragma unroll
-                    for (int i = 0; i < D; ++i) acc = zadd(acc, zmul(zconj(Cr[i * D + a]), T[i][b]));
-                    col[base + b] = static_cast<u32>(o_c + b);
-                    val[base + b] = zmul(ch, acc);
+                    for (int i = 0; i < D; ++i) acc = zadd(acc, zmul(zconj(Cr[i * D + a]), t[i]));
+                    const u64 pos = row_ptr[o_r + a] + count + static_cast<u64>(b);
+                    col[pos] = static_cast<u32>(o_c + b);
+                    val[pos] = zmul(ch, acc);
                 }
             }
         }
@@ -225,6 +224,10 @@ bool irrep_rows(const SectorView& R, const IrrepView& IR, const SectorView& C, c
     case 2: launch_rows<2>(R, IR, C, IC, same, P, row_len, row_ptr, col, val, st); return true;
     case 3: launch_rows<3>(R, IR, C, IC, same, P, row_len, row_ptr, col, val, st); return true;
     case 4: launch_rows<4>(R, IR, C, IC, same, P, row_len, row_ptr, col, val, st); return true;
+    case 5: launch_rows<5>(R, IR, C, IC, same, P, row_len, row_ptr, col, val, st); return true;
+    case 6: launch_rows<6>(R, IR, C, IC, same, P, row_len, row_ptr, col, val, st); return true;
+    case 7: launch_rows<7>(R, IR, C, IC, same, P, row_len, row_ptr, col, val, st); return true;
+    case 8: launch_rows<8>(R, IR, C, IC, same, P, row_len, row_ptr, col, val, st); return true;
     default: return false;
     }
 }

@@ -318,7 +318,7 @@ std::shared_ptr<DeviceSector> DeviceSector::make_irrep(GroupSpec spec, int d, co
     if (d < 2) throw Error(kInvalid, "irrep sector: an irrep of dimension at least 2 (use the characters for d = 1)");
     if (d > dev::kMaxIrrepDim)
         throw Error(kDeviceUnsupported, "irrep sector: irreps up to dimension " + std::to_string(dev::kMaxIrrepDim) +
-                                            " (the device kernels hold a block in registers)");
+                                            " (the reference's limit)");
     if (G < 1 || G > 256) throw Error(kDeviceUnsupported, "irrep sector: groups of 1 to 256 elements");
     const std::size_t dd = static_cast<std::size_t>(d) * static_cast<std::size_t>(d);
     if (D.size() != static_cast<std::size_t>(G) * dd) throw Error(kInvalid, "irrep sector: D needs G x d x d entries");

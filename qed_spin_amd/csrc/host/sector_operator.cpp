@@ -27,6 +27,11 @@ SectorOperator::SectorOperator(std::shared_ptr<const DeviceSector> rows, std::sh
         throw Error(kInvalid, "SectorOperator: the two sectors use different representative orders");
     if (rows_->irrep_dim() != cols_->irrep_dim())
         throw Error(kInvalid, "SectorOperator: the two sectors carry irreps of different dimensions");
+    // between two sectors of an irrep of dimension d > 1 the rows are blocks written for one irrep
+    // and an operator that commutes with the group (total S+-, S^2): the reference's
+    // CrossSectorMatVec refuses any other pair (src/engine/internal.h:557-563)
+    if (rows_->irrep_dim() > 1 && !same() && rows_->irrep().D != cols_->irrep().D)
+        throw Error(kInvalid, "SectorOperator: two sectors of different irreps of dimension > 1");
     real_ = prog_->real && rows_->real_characters() && cols_->real_characters();
     lane_ = kLaneGather;
     if (rows_->irrep_dim() > 1) {

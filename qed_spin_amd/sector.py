@@ -235,13 +235,17 @@ class SectorMatvec:
         self.op = op
         self.rows = rows
         self.cols = cols if cols is not None else rows
-        if getattr(rows, "irrep_dim", 1) > 1 and self.cols is not rows:
-            # the one-representative row walk needs an operator that commutes with the group; a
-            # probe between two irrep sectors is not of that kind
-            raise InvalidRequest("matvec: sectors of irreps of dimension > 1 take one sector and a symmetric operator")
+        blocks = getattr(rows, "irrep_dim", 1) > 1 or getattr(self.cols, "irrep_dim", 1) > 1
+        if blocks and self.cols is not rows:
+            # the rows of an irrep sector are blocks written for an operator that commutes with the
+            # group, between two sectors of one irrep (total S+-, S^2; the reference's
+            # CrossSectorMatVec, src/engine/internal.h:557-563): its own program, not a projection
+            if (rows.irrep_dim != self.cols.irrep_dim or rows.irrep.shape != self.cols.irrep.shape
+                    or not np.array_equal(rows.irrep, self.cols.irrep)):
+                raise InvalidRequest("matvec: two sectors of different irreps of dimension > 1")
         if program is not None:
             self.program = program
-        elif self.cols is rows:
+        elif self.cols is rows or blocks:
             self.program = Program(op, adjoint=True)
         else:
             self.program = projected_program(op, rows, self.cols)

@@ -5,8 +5,10 @@ representative by representative), against the oracle and what the mathematics f
   E^dag H E over the Sz sector entry by entry (1e-12), its applies the same within 1e-13;
 - the spectrum equals the momentum-m block's (the E_m copies and the momentum-m states correspond
   one to one), at small sizes by dense eigensolves and at N = 24 through the device Krylov lanes;
-- refusals: the on-the-fly lane, d above the kernels' limit, matrices that are not a
-  representation, the per-state lookup.
+- d = 4..8 from 2-transitive groups, against the same checks;
+- the operator between two sectors of one irrep (total S-) against E^dag S- E;
+- refusals: the on-the-fly lane, d above the reference's limit of 8, matrices that are not a
+  representation, the per-state lookup, two sectors of different irreps.
 Every case prints its name before it touches the device."""
 
 import sys
@@ -159,13 +161,124 @@ def test_irrep_sector_refusals():
         qs.SectorMatvec(H, s, lane=qs.LANE_GATHER)
     with pytest.raises(DeviceUnsupported):
         s.index_and_projection(np.array([0b00001111], np.uint64))
-    with pytest.raises(InvalidRequest):
-        qs.SectorMatvec(H, s, qs.Sector(gD, 4, irrep=D))
-    big = np.zeros((gD.G, 5, 5), complex)
-    big[:, :, :] = np.eye(5)
+    # two handles of one irrep sector: the cross operator (no diagonal shortcut) is the one-sector
+    # operator within rounding
+    a = dense(qs.SectorMatvec(H, s, lane=qs.LANE_CSR, tune=False))
+    b = dense(qs.SectorMatvec(H, s, qs.Sector(gD, 4, irrep=D), lane=qs.LANE_CSR, tune=False))
+    assert np.abs(a - b).max() <= 1e-13
+    big = np.zeros((gD.G, 9, 9), complex)
+    big[:, :, :] = np.eye(9)
     with pytest.raises(DeviceUnsupported):
         qs.Sector(gD, 4, irrep=big)
     rng = np.random.default_rng(3)
     Q = np.linalg.qr(rng.normal(size=(gD.G, 2, 2)) + 1j * rng.normal(size=(gD.G, 2, 2)))[0]
     with pytest.raises(InvalidRequest):
         qs.Sector(gD, 4, irrep=Q)
+
+
+@pytest.mark.parametrize("name,layers,n_up", [("agl5", 2, 4), ("psl2_5", 2, 5), ("agl7", 2, 4), ("agl8", 2, 4),
+                                              ("agl9", 2, 4)])
+def test_standard_irreps_of_two_transitive_groups_d4_to_d8(name, layers, n_up):
+    """d = 4..8 (up to the reference's kMaxIrrepDim): representatives and ranks equal the oracle's,
+    the device's C matrices give an orthonormal basis, the device CSR equals E^dag H E entry by
+    entry, and its spectrum is the isotypic spectrum of the Sz sector."""
+    from irrep_groups import layered_heisenberg, standard_irrep
+    from test_irrep_oracle import isotypic_spectrum, two_transitive
+
+    say("two-transitive irrep sector", name, layers, n_up)
+    perms, els, n = two_transitive(name, layers)
+    N = n * layers
+    D = standard_irrep(els, n)
+    d = D.shape[1]
+    g = sym.GroupSpec(perms=perms, chars=np.trace(D, axis1=1, axis2=2))
+    s = qs.Sector(g, n_up, irrep=D)
+    assert s.irrep_dim == d
+    reps_o, Cs_o = po.irrep_fields(N, n_up, perms, D)
+    reps, _ = s.export()
+    assert np.array_equal(reps, reps_o)
+    rep_class, offset, rank, Cm = s.export_irrep()
+    ranks = np.diff(offset.astype(np.int64))
+    assert list(ranks) == [C.shape[1] for C in Cs_o]
+    Cs = [Cm[rep_class[i]][:, : ranks[i]] for i in range(len(reps))]
+    H = layered_heisenberg(qs, n, layers)
+    Hs, states = sz_hamiltonian(H, N, n_up)
+    E = po.irrep_basis_matrix(N, perms, D, reps, Cs, states)
+    assert np.abs(E.conj().T @ E - np.eye(s.dim)).max() <= 1e-12
+    mv = qs.SectorMatvec(H, s, lane=qs.LANE_CSR, tune=False)
+    assert merged(mv)
+    A = dense(mv)
+    assert np.abs(A - E.conj().T @ Hs @ E).max() <= 1e-12 * max(1.0, np.abs(Hs).max())
+    iso, _ = isotypic_spectrum(Hs, states, N, perms, D)
+    assert np.abs(np.sort(np.repeat(np.linalg.eigvalsh(A), d)) - np.sort(iso)).max() <= 1e-11
+
+
+def sz_states(N, n_up):
+    return np.array([s for s in range(1 << N) if bin(s).count("1") == n_up], np.int64)
+
+
+@pytest.mark.parametrize("name,n_up", [("s4", 4), ("agl7", 4), ("agl9", 4)])
+def test_cross_operator_between_sectors_of_one_irrep(name, n_up):
+    """Total S- from the irrep sector at n_up to the one at n_up - 1 (the reference's
+    CrossSectorMatVec between two sectors of one irrep): the device CSR equals E_rows^dag S- E_cols
+    entry by entry, its applies agree with it, and S- carries an eigenvector of the SU(2)-invariant
+    H at n_up to one at n_up - 1 with the same energy. Sectors of different irreps refuse."""
+    from irrep_groups import layered_heisenberg, layered_symmetric_group, standard_irrep
+    from test_irrep_oracle import two_transitive
+
+    say("cross operator between irrep sectors", name, n_up)
+    if name == "s4":
+        n = 4
+        perms, els = layered_symmetric_group(n, 2)
+    else:
+        perms, els, n = two_transitive(name, 2)
+    N = 2 * n
+    D = standard_irrep(els, n)
+    g = sym.GroupSpec(perms=perms, chars=np.trace(D, axis1=1, axis2=2))
+    src, tgt = qs.Sector(g, n_up, irrep=D), qs.Sector(g, n_up - 1, irrep=D)
+    Sm = qs.Operator(N)
+    for i in range(N):
+        Sm.add_product("-", (i,), 1.0)
+    mv = qs.SectorMatvec(Sm, tgt, src, lane=qs.LANE_CSR)
+    assert (mv.dim, src.dim) == (tgt.dim, src.dim)
+    row_ptr, col, val = mv.export_csr()
+    A = np.zeros((tgt.dim, src.dim), complex)
+    for r in range(tgt.dim):
+        np.add.at(A[r], col[row_ptr[r]:row_ptr[r + 1]].astype(np.int64), val[row_ptr[r]:row_ptr[r + 1]])
+
+    def basis(sec, nu):
+        st = sz_states(N, nu)
+        rc, off, _, Cm = sec.export_irrep()
+        rk = np.diff(off.astype(np.int64))
+        reps, _ = sec.export()
+        Cs = [Cm[rc[i]][:, : rk[i]] for i in range(len(reps))]
+        return po.irrep_basis_matrix(N, perms, D, reps, Cs, st), st
+
+    Es, ss = basis(src, n_up)
+    Et, st = basis(tgt, n_up - 1)
+    allst = np.concatenate([st, ss])
+    M = po.dense_matrix(Sm.products, N, allst)[: len(st), len(st):]
+    ref = Et.conj().T @ M @ Es
+    assert np.abs(A - ref).max() <= 1e-12
+    rng = np.random.default_rng(7)
+    x = rng.normal(size=src.dim) + 1j * rng.normal(size=src.dim)
+    y = mv.apply_host(x, complex_vectors=True)
+    assert np.abs(y - ref @ x).max() <= 1e-12 * max(1.0, np.abs(ref @ x).max())
+    H = layered_heisenberg(qs, n, 2)
+    Hs = dense(qs.SectorMatvec(H, src, lane=qs.LANE_CSR, tune=False))
+    Ht = dense(qs.SectorMatvec(H, tgt, lane=qs.LANE_CSR, tune=False))
+    w, V = np.linalg.eigh(Hs)
+    moved = 0
+    for k in range(len(w)):
+        u = ref @ V[:, k]
+        if np.linalg.norm(u) > 1e-6:
+            assert np.abs(Ht @ u - w[k] * u).max() <= 1e-10 * max(1.0, abs(w[k])) * np.linalg.norm(u)
+            moved += 1
+    assert moved > 0
+    # an equivalent irrep in another basis is another D: refused, as is a sector of characters
+    Q = np.linalg.qr(rng.normal(size=(D.shape[1], D.shape[1])))[0]
+    other = qs.Sector(g, n_up - 1, irrep=Q.T @ D @ Q)
+    with pytest.raises(InvalidRequest):
+        qs.SectorMatvec(Sm, other, src, lane=qs.LANE_CSR)
+    trivial = sym.GroupSpec(perms=perms, chars=np.ones(len(els)))
+    with pytest.raises(InvalidRequest):
+        qs.SectorMatvec(Sm, qs.Sector(trivial, n_up - 1), src, lane=qs.LANE_CSR)

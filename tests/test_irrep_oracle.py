@@ -121,3 +121,46 @@ def test_standard_irreps_of_symmetric_groups(n, layers, n_up):
     assert rank == d * E.shape[1]
     ev = np.linalg.eigvalsh(E.conj().T @ Hs @ E)
     assert np.abs(np.sort(np.repeat(ev, d)) - np.sort(iso)).max() <= 1e-11
+
+
+TWO_TRANSITIVE = {"agl5": (5, 4), "psl2_5": (6, 5), "agl7": (7, 6), "agl8": (8, 7), "agl9": (9, 8)}
+
+
+def two_transitive(name, layers):
+    """(site permutations, point permutations, n) of the 2-transitive groups of irrep_groups."""
+    from irrep_groups import affine_group, psl2_5
+
+    n = TWO_TRANSITIVE[name][0]
+    perms, els = psl2_5(layers) if name == "psl2_5" else affine_group(n, layers)
+    return perms, els, n
+
+
+@pytest.mark.parametrize("name,layers,n_up", [("agl5", 2, 4), ("psl2_5", 2, 5), ("agl7", 2, 4), ("agl8", 2, 3),
+                                              ("agl9", 2, 3)])
+def test_standard_irreps_of_two_transitive_groups_d4_to_d8(name, layers, n_up):
+    """d = n - 1 = 4..8: D is a unitary representation (and the reversed composition is not), its
+    character has norm 1 (irreducible), and the oracle's sector has an orthonormal basis on which H
+    has the isotypic spectrum (each level d times)."""
+    from irrep_groups import layered_heisenberg, standard_irrep
+
+    perms, els, n = two_transitive(name, layers)
+    N = n * layers
+    D = standard_irrep(els, n)
+    d = D.shape[1]
+    assert d == TWO_TRANSITIVE[name][1]
+    G = len(els)
+    assert sym.is_representation(perms, D)
+    # g -> D(g)^dag reverses products: a representation only of an abelian group
+    assert not sym.is_representation(perms, np.conj(np.transpose(D, (0, 2, 1))))
+    chi = np.trace(D, axis1=1, axis2=2)
+    assert abs(np.vdot(chi, chi).real / G - 1.0) <= 1e-12
+    H = layered_heisenberg(qs, n, layers)
+    Hs, states = sz_hamiltonian(H, N, n_up)
+    reps, Cs = po.irrep_fields(N, n_up, perms, D)
+    E = po.irrep_basis_matrix(N, perms, D, reps, Cs, states)
+    assert E.shape[1] > 0
+    assert np.abs(E.conj().T @ E - np.eye(E.shape[1])).max() <= 1e-12
+    iso, rank = isotypic_spectrum(Hs, states, N, perms, D)
+    assert rank == d * E.shape[1]
+    ev = np.linalg.eigvalsh(E.conj().T @ Hs @ E)
+    assert np.abs(np.sort(np.repeat(ev, d)) - np.sort(iso)).max() <= 1e-11
