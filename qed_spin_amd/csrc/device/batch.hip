@@ -174,6 +174,84 @@ __global__ __launch_bounds__(kThreads) void k_batch_spmv_wide_fused(BatchView B,
     tile_partial(B, pr, pi, partial);
 }
 
+// The same for R = 1 (the continued fractions: one vector per block), THREADS threads per tile.
+// Each row's entries go to L lanes (lane l: entries e0 + l, e0 + l + L, ... in order) and the
+// lanes' sums meet in an xor butterfly; with L = 8 that is the arithmetic of the sector SpMV that
+// batch_spmv uses at R = 1, so W equals its product followed by batch_vec mode 0 bit for bit. The
+// partial of <V_cur, W> sums the tile's rows in row order, as mode 0 does with one row per
+// thread. Workgroups take the tiles in XCD-contiguous runs (workgroup w: tile (w mod 8) * per +
+// w / 8) unless a schedule is given, so each XCD's L2 sees neighbouring rows, as the sector SpMV's
+// row ranges do.
+template <int L, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_batch_spmv_one_fused(BatchView B, const Zd* Vprev, const Zd* Vcur,
+                                                                  Zd* W, const double* beta, double* partial,
+                                                                  const double* sprev, const double* scur) {
+    static_assert(THREADS % L == 0 && THREADS >= kBatchTileRows, "whole row groups; one LDS cell per row");
+    __shared__ double sr[kBatchTileRows], si[kBatchTileRows];
+    u32 tile;
+    if (B.sched) {
+        tile = B.sched[blockIdx.x];
+    } else {
+        const u32 per = gridDim.x >> 3;
+        tile = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+        if (tile >= B.n_tiles) return;   // the whole workgroup: the grid is rounded up to 8
+    }
+    const Tile t = B.tiles[tile];
+    constexpr int rows_per_pass = THREADS / L;
+    const int g = threadIdx.x / L, lane = threadIdx.x % L;
+    if (threadIdx.x < kBatchTileRows) {
+        sr[threadIdx.x] = 0.0;
+        si[threadIdx.x] = 0.0;
+    }
+    __syncthreads();
+    if (pair_live(B, t.block, 0)) {
+        const int step = *B.step;
+        const double b = beta[t.block];
+        const double fp = sprev ? sprev[t.block] : 1.0, fc = scur ? scur[t.block] : 1.0;
+        // every thread of a row group runs the same rounds (the butterfly needs all of them)
+        for (u64 base = t.row_begin; base < t.row_end; base += rows_per_pass) {
+            const u64 r = base + g;
+            double ar = 0.0, ai = 0.0;
+            if (r < t.row_end) {
+                const u64 e0 = B.row_ptr[r], e1 = B.row_ptr[r + 1];
+                for (u64 e = e0 + lane; e < e1; e += L) {
+                    const u64 c = B.col[e];
+                    const Zd v = entry_value(B, e);
+                    const Zd x = scaled(Vcur[c], fc);
+                    ar += v.re * x.re - v.im * x.im;
+                    ai += v.re * x.im + v.im * x.re;
+                }
+            }
+#pragma unroll
+            for (int off = L / 2; off > 0; off >>= 1) {
+                ar += __shfl_xor(ar, off, L);
+                ai += __shfl_xor(ai, off, L);
+            }
+            if (r < t.row_end && lane == 0) {
+                Zd w{ar, ai};
+                if (step > 0) {
+                    const Zd p = scaled(Vprev[r], fp);
+                    w = Zd{w.re - b * p.re, w.im - b * p.im};
+                }
+                W[r] = w;
+                const Zd d = cdot(scaled(Vcur[r], fc), w);
+                sr[r - t.row_begin] = d.re;
+                si[r - t.row_begin] = d.im;
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0, c = 0.0;
+        for (int q = 0; q < kBatchTileRows; ++q) {
+            a += sr[q];
+            c += si[q];
+        }
+        partial[static_cast<u64>(tile) * 64 * 2] = a;
+        partial[static_cast<u64>(tile) * 64 * 2 + 1] = c;
+    }
+}
+
 // mode 0: W -= beta V_prev (skipped at step 0), then partial of <V_cur, W> (alpha);
 // mode 1: W -= alpha V_cur, then partial of <V_cur, W> (the local overlap);
 // mode 5: mode 1, and the partial of ||W||^2 into partial2 as well (what mode 2 sums when it does
@@ -683,8 +761,17 @@ void batch_spmv(const BatchView& B, const Zd* X, Zd* Y, hipStream_t st) {
 
 bool batch_spmv_fused(const BatchView& B, const Zd* Vprev, const Zd* Vcur, Zd* W, const double* beta, double* partial,
                       hipStream_t st, const double* sprev, const double* scur) {
-    if (B.R < 8) return false;
+    if (B.R != 1 && B.R < 8) return false;
     if (!B.n_tiles) return true;
+    if (B.R == 1) {
+        // 8 lanes per row (the lane count of the sector SpMV that batch_spmv uses at R = 1), 256
+        // threads: on the N = 28 continued fractions 1024 threads (two passes per tile) took 0.99 s
+        // for the 200 steps against 0.87 s, and 4 lanes per row 0.89 s
+        const u32 grid = B.sched ? B.n_tiles : (B.n_tiles + 7u) / 8u * 8u;
+        hipLaunchKernelGGL((k_batch_spmv_one_fused<8, kThreads>), dim3(grid), dim3(kThreads), 0, st, B, Vprev, Vcur, W,
+                           beta, partial, sprev, scur);
+        return true;
+    }
     hipLaunchKernelGGL(k_batch_spmv_wide_fused, dim3(B.n_tiles), dim3(kThreads), 0, st, B, Vprev, Vcur, W, beta,
                        partial, sprev, scur);
     return true;

@@ -207,8 +207,10 @@ int BatchedLanczos::tune_schedule() {
         return sched_kind_;
     }
     // small engines finish a step in microseconds either way; above 65536 tiles the two schedules
-    // measured alike and a timed apply costs tens of milliseconds
-    if (tiles_.size() < 4096 || tiles_.size() > 65536) {
+    // measured alike and a timed apply costs tens of milliseconds. R = 1 keeps the in-order schedule,
+    // which its fused step kernel runs in XCD-contiguous runs of tiles (the plain apply timed here
+    // is the sector SpMV, which takes no schedule)
+    if (R_ == 1 || tiles_.size() < 4096 || tiles_.size() > 65536) {
         set_schedule(0);
         return 0;
     }
@@ -373,11 +375,11 @@ void BatchedLanczos::run(int M, const std::vector<double>& breakdown_tol, bool l
     double* ib = inv_b_.get<double>();
     const double* tol = tol_d_.get<double>();
     const double thr = 1.49011611938476562e-08;   // sqrt(eps): LocalDGKS3's overlap threshold
-    // With the fused SpMV (R >= 8) the next vector is never scaled in place: each of the three
+    // With the fused SpMV (R = 1 or R >= 8) the next vector is never scaled in place: each of the three
     // buffers keeps a per-pair factor (1 / beta of the step that wrote it; 1 for the normalised
     // start) that every read applies, which rounds as the scaling pass would (same results, one
     // pass over the vectors fewer per step).
-    const bool defer = R_ >= 8;
+    const bool defer = R_ == 1 || R_ >= 8;
     const std::size_t np = pairs;
     double* S[3] = {scale_.get<double>(), scale_.get<double>() + np, scale_.get<double>() + 2 * np};
     auto step = [&](int phase) {
