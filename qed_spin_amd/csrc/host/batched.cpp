@@ -519,8 +519,12 @@ void BatchedLanczos::run_kept(int M, const std::vector<double>& breakdown_tol, c
         const dev::Zd* Vp = basis + static_cast<std::uint64_t>(j > 0 ? j - 1 : 0) * layer;
         const dev::Zd* Vc = basis + static_cast<std::uint64_t>(j) * layer;
         dev::Zd* W = basis + static_cast<std::uint64_t>(j + 1) * layer;
-        dev::batch_spmv(B, Vc, W, st);
-        dev::batch_vec(B, 0, Vp, Vc, W, cb, nullptr, 0.0, partial, st);
+        // the kept layers are stored normalised (the overlaps read them later), so only the first
+        // recurrence pass is fused here, not the deferred scaling of run()
+        if (!dev::batch_spmv_fused(B, Vp, Vc, W, cb, partial, st)) {
+            dev::batch_spmv(B, Vc, W, st);
+            dev::batch_vec(B, 0, Vp, Vc, W, cb, nullptr, 0.0, partial, st);
+        }
         dev::batch_finish(B, 0, partial, ah, bh, ov, ca, cb, ib, tol, st);
         dev::batch_vec(B, 5, nullptr, Vc, W, ca, nullptr, 0.0, partial, st, partial2);
         dev::batch_finish(B, 1, partial, ah, bh, ov, ca, cb, ib, tol, st, thr);
