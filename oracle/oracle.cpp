@@ -18,6 +18,11 @@
 //        reference's golden sampled-FTLM record (one 4096-state block, seed 11).
 //   continued fraction                                -> closed form of a 1x1 / 2x2 tridiagonal,
 //        the T=0 Lehmann sum of a dense ground state.
+//
+// Parity unpinned: orc_block_seed is pinned on the first sampled block of a recorded walk, but the
+// ORDER in which a multi-block thermal walk hands seeds to its blocks, and the keys of the sources
+// of a multi-source T > 0 walk, are restated from the reference's source text only: no recorded
+// value of the reference pins the seed of a sampled block beyond the first.
 // =============================================================================
 #include <algorithm>
 #include <cmath>

@@ -2,6 +2,11 @@
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
 leg as the checker. Nothing in qed_spin_amd/ imports it.
+
+Parity unpinned: seed32 / gaussian_vector are pinned on the first sampled block of the reference's
+recorded walks; the order in which a multi-block walk hands seeds to its blocks (and the source keys
+of a multi-source T > 0 walk) is restated from the reference's source text, with no recorded value
+beyond the first sampled block to pin it.
 """
 
 from __future__ import annotations
