@@ -282,3 +282,47 @@ def test_cross_operator_between_sectors_of_one_irrep(name, n_up):
     trivial = sym.GroupSpec(perms=perms, chars=np.ones(len(els)))
     with pytest.raises(InvalidRequest):
         qs.SectorMatvec(Sm, qs.Sector(trivial, n_up - 1), src, lane=qs.LANE_CSR)
+
+
+@pytest.mark.parametrize("name", ["agl8", "agl9"])
+def test_irrep_sector_krylov_and_multi_vector_applies(name):
+    """On d = 7 and 8 sectors: the device Krylov lane's lowest level equals the dense spectrum's of
+    the device CSR (1e-10; the all-to-all layers' larger symmetry can make higher levels
+    degenerate, which one Krylov run need not resolve), multi-vector and block-interleaved applies equal single applies bit for
+    bit, on one sector and on the cross operator total S- between two sectors of the irrep."""
+    from irrep_groups import layered_heisenberg, standard_irrep
+    from test_irrep_oracle import two_transitive
+
+    say("irrep sector krylov and multi-vector applies", name)
+    perms, els, n = two_transitive(name, 2)
+    N = 2 * n
+    D = standard_irrep(els, n)
+    g = sym.GroupSpec(perms=perms, chars=np.trace(D, axis1=1, axis2=2))
+    n_up = 5
+    s = qs.Sector(g, n_up, irrep=D)
+    H = layered_heisenberg(qs, n, 2, rung=0.45)
+    mv = qs.SectorMatvec(H, s, lane=qs.LANE_CSR)
+    ev = np.linalg.eigvalsh(dense(mv))
+    vals, conv, _ = mv.lowest(1)
+    assert conv
+    assert abs(vals[0] - ev[0]) <= 1e-10 * max(1.0, abs(ev[0]))
+    Sm = qs.Operator(N)
+    for i in range(N):
+        Sm.add_product("-", (i,), 1.0)
+    down = qs.Sector(g, n_up - 1, irrep=D)
+    cross = qs.SectorMatvec(Sm, down, s, lane=qs.LANE_CSR)
+    rng = np.random.default_rng(11)
+    for m, cols in ((mv, s.dim), (cross, s.dim)):
+        xs = [rng.normal(size=cols) + 1j * rng.normal(size=cols) for _ in range(4)]
+        singles = [m.apply_host(x, complex_vectors=True) for x in xs]
+        X = [qs.DeviceArray.from_host(x) for x in xs]
+        Y = [qs.DeviceArray(m.dim) for _ in xs]
+        m.apply_multi(X, Y, complex_vectors=True)
+        for y, ref in zip(Y, singles):
+            assert np.array_equal(y.to_host().view(np.uint8), ref.view(np.uint8))
+        XB = qs.DeviceArray.from_host(np.ascontiguousarray(np.stack(xs, axis=1)).reshape(-1))
+        YB = qs.DeviceArray(m.dim * 4)
+        m.apply_block(4, XB, YB, complex_vectors=True)
+        got = YB.to_host().reshape(m.dim, 4)
+        for i, ref in enumerate(singles):
+            assert np.array_equal(np.ascontiguousarray(got[:, i]).view(np.uint8), ref.view(np.uint8))
