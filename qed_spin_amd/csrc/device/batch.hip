@@ -761,7 +761,9 @@ void batch_spmv(const BatchView& B, const Zd* X, Zd* Y, hipStream_t st) {
 
 bool batch_spmv_fused(const BatchView& B, const Zd* Vprev, const Zd* Vcur, Zd* W, const double* beta, double* partial,
                       hipStream_t st, const double* sprev, const double* scur) {
-    if (B.R != 1 && B.R < 8) return false;
+    // 1 < R < 8: a fused form of the narrow kernel (the terms of alpha through LDS to keep mode 0's
+    // summation order) made the chain24 T > 0 Lanczos runs slower (3.0 -> 3.7 s); the two kernels stay
+    if (B.R > 1 && B.R < 8) return false;
     if (!B.n_tiles) return true;
     if (B.R == 1) {
         // 8 lanes per row (the lane count of the sector SpMV that batch_spmv uses at R = 1), 256

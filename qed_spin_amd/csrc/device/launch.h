@@ -232,7 +232,8 @@ constexpr int kBatchTileRows = 256;
 constexpr int kBatchMaxR = 64;
 void batch_spmv(const BatchView& B, const Zd* X, Zd* Y, hipStream_t st);
 /// W = H V_cur - beta V_prev with the partials of <V_cur, W> (batch_spmv + batch_vec mode 0 in one
-/// kernel, the same arithmetic); false (nothing launched) when 1 < R < 8. sprev / scur: per-pair factors
+/// kernel, the same arithmetic, bit for bit); false (nothing launched) when 1 < R < 8. sprev / scur:
+/// per-pair factors
 /// of the stored vectors (V = stored times factor), or null (1).
 bool batch_spmv_fused(const BatchView& B, const Zd* Vprev, const Zd* Vcur, Zd* W, const double* beta, double* partial,
                       hipStream_t st, const double* sprev = nullptr, const double* scur = nullptr);
